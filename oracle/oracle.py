@@ -1,0 +1,214 @@
+"""ctypes front end for the CPU checkers.  TEST INFRASTRUCTURE ONLY.
+
+``OracleBrain(kind="oracle")`` drives the restatement in lif_oracle.c;
+``kind="ref"`` drives the reference's own lif.c compiled into oracle/_ref/
+(present only where /root/reference was available at build time).  Both
+expose the same calls, so a test can run one against the other.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liblif_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_lif.so")
+
+FIELDS = ("dt", "vRest", "vThresh", "vReset", "tauM", "tauSyn", "tRef", "delay",
+          "traceTau", "adaptInc", "adaptTau", "depU", "depTau", "coba", "eExc",
+          "eInh", "bgRate", "bgAmp", "preInh")
+ARRAYS = ("v", "gE", "gI", "refr", "trace", "adapt", "res", "bias", "thr", "drive")
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in FIELDS]
+
+
+def params_struct(p: dict) -> _Params:
+    s = _Params()
+    for n in FIELDS:
+        v = p.get(n, 0.0) or 0.0
+        setattr(s, n, float(bool(v)) if n == "coba" else float(v))
+    return s
+
+
+_libs: dict = {}
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(ORACLE_SO if kind == "oracle" else REF_SO)
+
+
+def _lib(kind: str):
+    if kind in _libs:
+        return _libs[kind]
+    path = ORACLE_SO if kind == "oracle" else REF_SO
+    lib = ctypes.CDLL(path)
+    pre = "fbo_" if kind == "oracle" else "ref_"
+    P, I32, I64, U32, F = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_float
+    sig = {
+        "create": (P, [I32, I64, P, P, P, P, ctypes.POINTER(_Params), ctypes.c_double]),
+        "destroy": (None, [P]), "reset": (None, [P]),
+        "set_driven": (None, [P, P, I32]),
+        "array": (ctypes.POINTER(F), [P, I32]),
+        "spike_count": (ctypes.POINTER(U32), [P]),
+        "rng_state": (U32, [P]), "head": (I32, [P]), "nslots_of": (I32, [P]),
+        "bg_acc": (F, [P]),
+        "last_fired": (ctypes.POINTER(I32), [P, ctypes.POINTER(I32)]),
+        "step": (I32, [P]), "run": (I64, [P, I32, P, P, I64]),
+        "add": (None, [P, I32, I32, ctypes.c_double]),
+        "store": (None, [P, I32, I32, ctypes.c_double]),
+        "set_background": (None, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, pre + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    if kind == "oracle":
+        lib.fbo_seed_hash.restype = U32
+        lib.fbo_seed_hash.argtypes = [ctypes.c_double]
+        lib.fbo_xorshift32.restype = U32
+        lib.fbo_xorshift32.argtypes = [ctypes.POINTER(U32)]
+        lib.fbo_nslots.restype = I32
+        lib.fbo_nslots.argtypes = [ctypes.c_double, ctypes.c_double]
+        fns["_lib"] = lib
+    _libs[kind] = fns
+    return fns
+
+
+def seed_hash(seed) -> int:
+    return int(_lib("oracle")["_lib"].fbo_seed_hash(float(seed)))
+
+
+def xorshift_stream(state: int, n: int) -> np.ndarray:
+    st = ctypes.c_uint32(state)
+    f = _lib("oracle")["_lib"].fbo_xorshift32
+    return np.array([f(ctypes.byref(st)) for _ in range(n)], dtype=np.uint32)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleBrain:
+    """One brain on the CPU checker: same state arrays and setters as the
+    reference's LIFWasm (src/lifwasm.js:36-87)."""
+
+    def __init__(self, graph, params: dict, seed=1, kind: str = "oracle", W=None, sign=None):
+        self.kind = kind
+        self.f = _lib(kind)
+        self.p = dict(params)
+        self.N = int(graph.N)
+        self._indptr = np.ascontiguousarray(graph.indptr, dtype=np.uint32)
+        self._indices = np.ascontiguousarray(graph.indices, dtype=np.uint32)
+        self._W = np.ascontiguousarray(graph.weights if W is None else W, dtype=np.float32)
+        self._sign = np.ascontiguousarray(graph.sign if sign is None else sign, dtype=np.float32)
+        ps = params_struct(self.p)
+        self.h = self.f["create"](self.N, int(self._indices.size), _ptr(self._indptr), _ptr(self._indices),
+                                  _ptr(self._W), _ptr(self._sign), ctypes.byref(ps), float(seed))
+        self.drivenSet: list = []
+        self._driven_dirty = True
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.f["destroy"](h)
+            self.h = None
+
+    # ---------------------------------------------------------------- state
+    def array(self, name: str) -> np.ndarray:
+        ptr = self.f["array"](self.h, ARRAYS.index(name))
+        return np.ctypeslib.as_array(ptr, shape=(self.N,))
+
+    def __getattr__(self, name):
+        if name in ARRAYS:
+            return self.array(name)
+        raise AttributeError(name)
+
+    @property
+    def spikeCount(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.f["spike_count"](self.h), shape=(self.N,))
+
+    @property
+    def rng(self) -> int:
+        return int(self.f["rng_state"](self.h))
+
+    @property
+    def head(self) -> int:
+        return int(self.f["head"](self.h))
+
+    @property
+    def nslots(self) -> int:
+        return int(self.f["nslots_of"](self.h))
+
+    def state_digest(self) -> dict:
+        return {n: self.array(n).copy() for n in ARRAYS[:7]} | {"spikeCount": self.spikeCount.copy()}
+
+    # ---------------------------------------------------------------- setters (lifwasm.js:70-78)
+    def setDriveOne(self, i: int, rate: float):
+        self.f["store"](self.h, ARRAYS.index("drive"), int(i), float(rate))
+        if rate > 0:
+            if i not in self.drivenSet:
+                self.drivenSet.append(int(i))
+                self._driven_dirty = True
+        elif i in self.drivenSet:
+            self.drivenSet.remove(int(i))
+            self._driven_dirty = True
+
+    def setDrive(self, ix, rate: float):
+        for i in ix:
+            self.setDriveOne(int(i), rate)
+
+    def setBias(self, ix, mv: float):
+        for i in ix:
+            self.f["store"](self.h, ARRAYS.index("bias"), int(i), float(mv))
+
+    def setThr(self, i: int, mv: float):
+        self.f["store"](self.h, ARRAYS.index("thr"), int(i), float(mv))
+
+    def addG(self, i: int, e: float, ii: float = 0.0):
+        self.f["add"](self.h, ARRAYS.index("gE"), int(i), float(e))
+        self.f["add"](self.h, ARRAYS.index("gI"), int(i), float(ii or 0.0))
+
+    def pulse(self, ix, mv: float):
+        for i in ix:
+            self.f["add"](self.h, ARRAYS.index("gE"), int(i), float(mv))
+
+    def setBackground(self, rate_hz: float, amp_mv: float):
+        self.f["set_background"](self.h, float(rate_hz), float(amp_mv), float(self.p["dt"]))
+
+    def reset(self):
+        self.f["reset"](self.h)
+
+    # ---------------------------------------------------------------- stepping
+    def _sync_driven(self):
+        if self._driven_dirty:
+            lst = np.array(self.drivenSet, dtype=np.int32)
+            self.f["set_driven"](self.h, _ptr(lst), int(lst.size))
+            self._driven_dirty = False
+
+    def step(self) -> np.ndarray:
+        self._sync_driven()
+        self.f["step"](self.h)
+        n = ctypes.c_int32()
+        ptr = self.f["last_fired"](self.h, ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros(0, dtype=np.int32)
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+
+    def run(self, steps: int, raster_cap: int | None = None):
+        """Advance ``steps`` steps; returns (nfired per step, concatenated spike lists)."""
+        self._sync_driven()
+        nf = np.zeros(steps, dtype=np.int32)
+        cap = raster_cap if raster_cap is not None else steps * 64
+        ras = np.zeros(max(1, cap), dtype=np.int32)
+        tot = self.f["run"](self.h, steps, _ptr(nf), _ptr(ras), cap)
+        if tot > cap:
+            ras = None
+        else:
+            ras = ras[:tot].copy()
+        return nf, ras, int(tot)
