@@ -1,0 +1,770 @@
+// fb_api.hip -- the C ABI (include/flybrain.h): graphs, brains, setters,
+// stepping engines and state transfer.
+//
+// Host objects mirror the reference's JS wrappers: fb_brain holds what LIFWasm
+// holds on the JS side (params, the insertion-ordered drivenSet, t, ring head;
+// src/lifwasm.js:36-87) while the state arrays live on the device.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <numeric>
+
+#include "fb_internal.h"
+
+namespace fb {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
+                      hipStream_t st);
+fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
+fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
+fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st);
+
+// ------------------------------------------------------------------ jump tables
+
+// Byte tables of M^(2^b), b = 0..31, where M is one xorshift32 step viewed as a
+// 32x32 matrix over GF(2) (the map is linear: shifts and xors only).
+static void build_jump_tables(std::vector<uint32_t> &tab) {
+    auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
+    uint32_t col[32], tmp[32];
+    for (int b = 0; b < 32; b++) col[b] = step(1u << b);
+    auto apply = [](const uint32_t *c, uint32_t x) {
+        uint32_t r = 0;
+        for (int b = 0; b < 32; b++) if ((x >> b) & 1u) r ^= c[b];
+        return r;
+    };
+    tab.assign(32 * 1024, 0);
+    for (int p = 0; p < 32; p++) {
+        for (int byte = 0; byte < 4; byte++)
+            for (int v = 0; v < 256; v++)
+                tab[(size_t)p * 1024 + byte * 256 + v] = apply(col, (uint32_t)v << (8 * byte));
+        for (int b = 0; b < 32; b++) tmp[b] = apply(col, col[b]);   // square
+        memcpy(col, tmp, sizeof col);
+    }
+}
+
+static std::mutex g_jump_mu;
+static std::map<int, uint32_t *> g_jump_dev;
+
+static fb_status jump_table(uint32_t **out) {
+    int dev = 0;
+    FB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_jump_mu);
+    auto it = g_jump_dev.find(dev);
+    if (it != g_jump_dev.end()) { *out = it->second; return FB_OK; }
+    std::vector<uint32_t> tab;
+    build_jump_tables(tab);
+    uint32_t *p = nullptr;
+    FB_HIP(hipMalloc(&p, tab.size() * sizeof(uint32_t)));
+    FB_HIP(hipMemcpy(p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    g_jump_dev[dev] = p;
+    *out = p;
+    return FB_OK;
+}
+
+// ------------------------------------------------------------------ constants
+
+static double js_round(double x) {
+    double f = std::floor(x);
+    return (x - f >= 0.5) ? f + 1.0 : f;
+}
+
+int32_t nslots_of(const fb_params &p) {
+    double r = js_round(p.delay / p.dt);
+    if (r < 1.0) r = 1.0;
+    return (int32_t)r + 1;
+}
+
+void derive_consts(const fb_params &p, int32_t N, Consts &c) {
+    c.dt = (float)p.dt; c.vRest = (float)p.vRest; c.vThresh = (float)p.vThresh; c.vReset = (float)p.vReset;
+    c.tRef = (float)p.tRef; c.adaptInc = (float)p.adaptInc; c.depU = (float)p.depU;
+    c.dE = (float)std::exp(-p.dt / p.tauSyn);
+    c.dTr = (float)std::exp(-p.dt / p.traceTau);
+    c.dA = (float)std::exp(-p.dt / p.adaptTau);
+    c.kRec = (float)(p.dt / p.depTau);
+    c.kM = (float)(p.dt / p.tauM);
+    c.dtS = (float)(p.dt / 1000.0);
+    c.eExc = (float)p.eExc; c.eInh = (float)p.eInh;
+    c.cE = (float)(1.0 / (p.eExc - p.vRest));
+    c.cI = (float)(1.0 / (p.vRest - p.eInh));
+    c.bgEvents = (float)((double)N * p.bgRate * p.dt / 1000.0);
+    c.bgAmp = (float)p.bgAmp;
+    c.preInh = (float)p.preInh;
+    c.coba = p.coba != 0.0 ? 1 : 0;
+}
+
+uint32_t seed_hash(double seed) {
+    double prod = seed * 2654435761.0;
+    double t = prod < 0 ? -std::floor(-prod) : std::floor(prod);
+    double m = std::fmod(t, 4294967296.0);
+    if (m < 0) m += 4294967296.0;
+    uint32_t s = (uint32_t)m;
+    return s ? s : 1u;
+}
+
+}  // namespace fb
+
+using namespace fb;
+
+// ------------------------------------------------------------------ objects
+
+struct fb_graph {
+    int refs = 1;
+    int device = 0;
+    DevGraph d{};
+    bool rowsUnique = true;
+    std::vector<void *> allocs;
+    std::vector<uint32_t> indptr;   // host copy for validation
+};
+
+struct GraphCache {
+    int32_t head = -1;
+    int32_t steps = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+struct fb_brain {
+    fb_graph *g = nullptr;
+    fb_params p{};
+    DevBrain d{};
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> drivenList;   // insertion order of the driven set
+    std::vector<uint8_t> inDriven;
+    bool drivenDirty = true;
+    int32_t head = 0, lastSlot = 0;
+    double t = 0.0;
+    int64_t steps = 0;
+    std::vector<void *> allocs;
+    int32_t *ringAlloc = nullptr;
+    int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
+    int32_t *gatherCnt = nullptr, *gatherBuf = nullptr;
+    std::map<int32_t, GraphCache> graphs;   // keyed by start head
+    uint64_t version = 0;                   // bumps when DevBrain arguments change
+    uint64_t graphVersion = ~0ull;
+};
+
+extern "C" const char *fb_last_error(void) { return g_err.c_str(); }
+
+#define FB_REQUIRE(cond, code, ...)      \
+    do {                                 \
+        if (!(cond)) {                   \
+            set_error(__VA_ARGS__);      \
+            return code;                 \
+        }                                \
+    } while (0)
+
+template <class T>
+static fb_status dalloc(std::vector<void *> &list, T **p, size_t count) {
+    void *q = nullptr;
+    FB_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    list.push_back(q);
+    *p = (T *)q;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_device_info(char *name, int32_t cap, int32_t *n_cu) {
+    int dev = 0;
+    FB_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    FB_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name && cap > 0) snprintf(name, (size_t)cap, "%s (%s)", prop.name, prop.gcnArchName);
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return FB_OK;
+}
+
+// ------------------------------------------------------------------ graph
+
+extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t *indices,
+                                     const float *weights, const float *sign, const int32_t *counts,
+                                     fb_graph **out) {
+    FB_REQUIRE(out && indptr && sign && N > 0 && E >= 0, FB_ERR_ARG, "fb_graph_create: bad arguments");
+    FB_REQUIRE(E == 0 || (indices && weights), FB_ERR_ARG, "fb_graph_create: indices/weights missing");
+    FB_REQUIRE(E < (int64_t)UINT32_MAX, FB_ERR_ARG, "fb_graph_create: E exceeds the u32 CSR layout");
+    FB_REQUIRE(indptr[0] == 0 && (int64_t)indptr[N] == E, FB_ERR_ARG,
+               "fb_graph_create: indptr must start at 0 and end at E");
+    for (int32_t i = 0; i < N; i++)
+        FB_REQUIRE(indptr[i] <= indptr[i + 1], FB_ERR_ARG, "fb_graph_create: indptr not monotone at %d", i);
+    for (int64_t j = 0; j < E; j++)
+        FB_REQUIRE(indices[j] < (uint32_t)N, FB_ERR_ARG, "fb_graph_create: target %u out of range at edge %lld",
+                   indices[j], (long long)j);
+    fb_graph *g = new fb_graph();
+    FB_HIP(hipGetDevice(&g->device));
+    g->indptr.assign(indptr, indptr + N + 1);
+    // Rows are stored sorted by target (stable, so repeated targets keep their
+    // relative order); the order key of an arrival is its position in this row.
+    std::vector<uint2> edges((size_t)std::max<int64_t>(E, 1));
+    std::vector<int32_t> cnts;
+    if (counts) cnts.resize((size_t)E);
+    std::vector<uint32_t> perm;
+    for (int32_t i = 0; i < N; i++) {
+        const uint32_t a = indptr[i], b = indptr[i + 1];
+        bool sorted = true;
+        for (uint32_t j = a + 1; j < b; j++) {
+            if (indices[j] < indices[j - 1]) sorted = false;
+            if (indices[j] == indices[j - 1]) g->rowsUnique = false;
+        }
+        if (sorted) {
+            for (uint32_t j = a; j < b; j++) {
+                uint32_t wb;
+                memcpy(&wb, &weights[j], 4);
+                edges[j] = make_uint2(indices[j], wb);
+                if (counts) cnts[j] = counts[j];
+            }
+        } else {
+            perm.resize(b - a);
+            std::iota(perm.begin(), perm.end(), a);
+            std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) { return indices[x] < indices[y]; });
+            for (uint32_t j = a; j < b; j++) {
+                const uint32_t src = perm[j - a];
+                uint32_t wb;
+                memcpy(&wb, &weights[src], 4);
+                edges[j] = make_uint2(indices[src], wb);
+                if (counts) cnts[j] = counts[src];
+            }
+            for (uint32_t j = a + 1; j < b; j++)
+                if (edges[j].x == edges[j - 1].x) g->rowsUnique = false;
+        }
+    }
+    uint32_t *dIndptr; uint2 *dEdges; float *dSign; int32_t *dCounts = nullptr;
+    fb_status st;
+    if ((st = dalloc(g->allocs, &dIndptr, (size_t)N + 1)) ||
+        (st = dalloc(g->allocs, &dEdges, (size_t)std::max<int64_t>(E, 1))) ||
+        (st = dalloc(g->allocs, &dSign, (size_t)N)))
+        return st;
+    FB_HIP(hipMemcpy(dIndptr, indptr, sizeof(uint32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
+    if (E) FB_HIP(hipMemcpy(dEdges, edges.data(), sizeof(uint2) * (size_t)E, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(dSign, sign, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+    if (counts) {
+        if ((st = dalloc(g->allocs, &dCounts, (size_t)std::max<int64_t>(E, 1)))) return st;
+        if (E) FB_HIP(hipMemcpy(dCounts, cnts.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+    }
+    g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts;
+    *out = g;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_graph_retain(fb_graph *g) {
+    FB_REQUIRE(g, FB_ERR_ARG, "fb_graph_retain: null graph");
+    g->refs++;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_graph_release(fb_graph *g) {
+    FB_REQUIRE(g, FB_ERR_ARG, "fb_graph_release: null graph");
+    if (--g->refs > 0) return FB_OK;
+    for (void *p : g->allocs) hipFree(p);
+    delete g;
+    return FB_OK;
+}
+
+// ------------------------------------------------------------------ brain helpers
+
+__global__ void k_reset_state(DevBrain d) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.N) {
+        d.v[i] = d.k.vRest; d.gE[i] = 0.f; d.gI[i] = 0.f; d.refr[i] = 0.f;
+        d.trace[i] = 0.f; d.adapt[i] = 0.f; d.res[i] = 1.f; d.spikeCount[i] = 0u;
+    }
+    if (i < d.nslots) d.ringCount[i] = 0;
+}
+
+__global__ void k_init_scratch(DevBrain d) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * d.N) { d.cnt[i] = 0; d.chain[i] = -1; }
+    if (i < d.N) d.bgCnt[i] = 0;
+}
+
+// setters: mode 0 stores f32(value); mode 1 stores f32(f64(old) + value)
+__global__ void k_set(float *arr, const int32_t *idx, int32_t n, int32_t one, double value, int32_t mode,
+                      int32_t serial) {
+    if (serial) {
+        if (blockIdx.x || threadIdx.x) return;
+        for (int32_t q = 0; q < n; q++) {
+            const int32_t i = idx ? idx[q] : one;
+            arr[i] = mode ? (float)((double)arr[i] + value) : (float)value;
+        }
+        return;
+    }
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int32_t i = idx ? idx[q] : one;
+    arr[i] = mode ? (float)((double)arr[i] + value) : (float)value;
+}
+
+static fb_status sync_driven(fb_brain *b) {
+    if (!b->drivenDirty) return FB_OK;
+    const size_t n = b->drivenList.size();
+    if (n) FB_HIP(hipMemcpyAsync(b->d.driven, b->drivenList.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    if (b->d.nDriven != (int32_t)n) b->version++;
+    b->d.nDriven = (int32_t)n;
+    b->drivenDirty = false;
+    return FB_OK;
+}
+
+static float *array_ptr(fb_brain *b, int32_t which) {
+    switch (which) {
+        case FB_V: return b->d.v;
+        case FB_GE: return b->d.gE;
+        case FB_GI: return b->d.gI;
+        case FB_REFR: return b->d.refr;
+        case FB_TRACE: return b->d.trace;
+        case FB_ADAPT: return b->d.adapt;
+        case FB_RES: return b->d.res;
+        case FB_BIAS: return b->d.bias;
+        case FB_THR: return b->d.thr;
+        case FB_DRIVE: return b->d.drive;
+        case FB_SPIKECOUNT: return (float *)b->d.spikeCount;
+        default: return nullptr;
+    }
+}
+
+static fb_status alloc_ring(fb_brain *b) {
+    const int32_t N = b->d.N;
+    if (b->ringAlloc) {
+        hipFree(b->ringAlloc);
+        b->allocs.erase(std::find(b->allocs.begin(), b->allocs.end(), (void *)b->ringAlloc));
+        b->ringAlloc = nullptr;
+    }
+    int32_t *ring;
+    b->d.ringStride = 2 * (int64_t)N;
+    fb_status st = dalloc(b->allocs, &ring, (size_t)b->d.nslots * (size_t)b->d.ringStride + (size_t)b->d.nslots);
+    if (st) return st;
+    b->ringAlloc = ring;
+    b->d.ring = ring;
+    b->d.ringCount = ring + (size_t)b->d.nslots * (size_t)b->d.ringStride;
+    FB_HIP(hipMemsetAsync(b->d.ringCount, 0, sizeof(int32_t) * (size_t)b->d.nslots, b->stream));
+    return FB_OK;
+}
+
+static void drop_graphs(fb_brain *b) {
+    for (auto &kv : b->graphs) {
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+    }
+    b->graphs.clear();
+}
+
+// ------------------------------------------------------------------ brain API
+
+extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double seed, fb_brain **out) {
+    FB_REQUIRE(g && p && out, FB_ERR_ARG, "fb_brain_create: null argument");
+    FB_REQUIRE(p->dt > 0 && p->tauM > 0 && p->tauSyn > 0, FB_ERR_ARG, "fb_brain_create: dt, tauM, tauSyn must be > 0");
+    fb_brain *b = new fb_brain();
+    b->g = g;
+    g->refs++;
+    b->p = *p;
+    const int32_t N = g->d.N;
+    const int64_t E = g->d.E;
+    DevBrain &d = b->d;
+    d.N = N;
+    d.nslots = nslots_of(*p);
+    d.nDriven = 0;
+    d.nblocks = (N + 255) / 256;
+    derive_consts(*p, N, d.k);
+    d.g = g->d;
+    FB_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    fb_status st;
+    float *f;
+    // ten f32 state arrays back to back, then spike counts
+    if ((st = dalloc(b->allocs, &f, (size_t)N * 10))) return st;
+    d.v = f; d.gE = f + N; d.gI = f + 2 * (size_t)N; d.refr = f + 3 * (size_t)N; d.trace = f + 4 * (size_t)N;
+    d.adapt = f + 5 * (size_t)N; d.res = f + 6 * (size_t)N; d.bias = f + 7 * (size_t)N; d.thr = f + 8 * (size_t)N;
+    d.drive = f + 9 * (size_t)N;
+    if ((st = dalloc(b->allocs, &d.spikeCount, (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.driven, (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.cnt, 2 * (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.chain, 2 * (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.add, 2 * (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.key, 2 * (size_t)N))) return st;
+    d.collCap = std::max<int64_t>(E, 1);
+    if ((st = dalloc(b->allocs, &d.coll, (size_t)d.collCap))) return st;
+    if ((st = dalloc(b->allocs, &d.bgCnt, (size_t)N))) return st;
+    if ((st = dalloc(b->allocs, &d.stage, (size_t)d.nblocks * 256))) return st;
+    if ((st = dalloc(b->allocs, &d.blkCnt, (size_t)d.nblocks))) return st;
+    if ((st = dalloc(b->allocs, &d.sc, 1))) return st;
+    if ((st = dalloc(b->allocs, &b->idxScratch, (size_t)N))) return st;
+    if ((st = jump_table((uint32_t **)&d.jumpTab))) return st;
+    if ((st = alloc_ring(b))) return st;
+    Scalars sc{};
+    sc.rng = seed_hash(seed);
+    FB_HIP(hipMemcpyAsync(d.sc, &sc, sizeof sc, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemsetAsync(d.bias, 0, sizeof(float) * 3 * (size_t)N, b->stream));   // bias, thr, drive
+    const int nb = (2 * N + 255) / 256;
+    hipLaunchKernelGGL(k_init_scratch, dim3(nb), dim3(256), 0, b->stream, d);
+    hipLaunchKernelGGL(k_reset_state, dim3((std::max(N, d.nslots) + 255) / 256), dim3(256), 0, b->stream, d);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(b->stream));
+    b->inDriven.assign((size_t)N, 0);
+    *out = b;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_brain_destroy(fb_brain *b) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_destroy: null brain");
+    hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (void *p : b->allocs) hipFree(p);
+    if (b->gatherBuf) hipFree(b->gatherBuf);
+    if (b->gatherCnt) hipFree(b->gatherCnt);
+    hipStreamDestroy(b->stream);
+    fb_graph_release(b->g);
+    delete b;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_reset(fb_brain *b) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_reset: null brain");
+    hipLaunchKernelGGL(k_reset_state, dim3((std::max(b->d.N, b->d.nslots) + 255) / 256), dim3(256), 0, b->stream,
+                       b->d);
+    FB_HIP(hipGetLastError());
+    b->t = 0.0;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_params(fb_brain *b, const fb_params *p) {
+    FB_REQUIRE(b && p, FB_ERR_ARG, "fb_set_params: null argument");
+    FB_REQUIRE(p->dt > 0 && p->tauM > 0 && p->tauSyn > 0, FB_ERR_ARG, "fb_set_params: dt, tauM, tauSyn must be > 0");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    const int32_t ns = nslots_of(*p);
+    b->p = *p;
+    derive_consts(*p, b->d.N, b->d.k);
+    if (ns != b->d.nslots) {
+        // as src/lif.js:47 re-creates its delay line when delay or dt change
+        b->d.nslots = ns;
+        fb_status st = alloc_ring(b);
+        if (st) return st;
+        b->head = 0;
+        b->lastSlot = 0;
+    }
+    b->version++;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_get_params(const fb_brain *b, fb_params *p) {
+    FB_REQUIRE(b && p, FB_ERR_ARG, "fb_get_params: null argument");
+    *p = b->p;
+    return FB_OK;
+}
+
+static fb_status set_values(fb_brain *b, float *arr, const int32_t *idx, int32_t n, double value, int mode) {
+    FB_REQUIRE(n >= 0 && (n == 0 || idx), FB_ERR_ARG, "setter: bad index list");
+    for (int32_t q = 0; q < n; q++)
+        FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "setter: index %d out of range", idx[q]);
+    if (n == 0) return FB_OK;
+    if (n == 1) {
+        hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, b->stream, arr, (const int32_t *)nullptr, 1, idx[0], value,
+                           mode, 0);
+    } else {
+        int serial = 0;
+        if (mode == 1) {   // repeated indices must add in list order
+            std::vector<int32_t> s(idx, idx + n);
+            std::sort(s.begin(), s.end());
+            serial = std::adjacent_find(s.begin(), s.end()) != s.end();
+        }
+        FB_HIP(hipMemcpyAsync(b->idxScratch, idx, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+        hipLaunchKernelGGL(k_set, dim3((n + 255) / 256), dim3(256), 0, b->stream, arr, (const int32_t *)b->idxScratch,
+                           n, 0, value, mode, serial);
+        FB_HIP(hipStreamSynchronize(b->stream));   // idxScratch is reused by the next setter
+    }
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_drive_one(fb_brain *b, int32_t i, double hz) {
+    FB_REQUIRE(b && i >= 0 && i < (b ? b->d.N : 0), FB_ERR_ARG, "fb_set_drive_one: bad neuron");
+    fb_status st = set_values(b, b->d.drive, &i, 1, hz, 0);
+    if (st) return st;
+    // drivenSet semantics (src/lifwasm.js:71): add keeps first-insertion order,
+    // removal drops the entry so a later re-add goes to the end
+    if (hz > 0) {
+        if (!b->inDriven[i]) {
+            b->inDriven[i] = 1;
+            b->drivenList.push_back(i);
+            b->drivenDirty = true;
+        }
+    } else if (b->inDriven[i]) {
+        b->inDriven[i] = 0;
+        b->drivenList.erase(std::find(b->drivenList.begin(), b->drivenList.end(), i));
+        b->drivenDirty = true;
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_drive(fb_brain *b, const int32_t *idx, int32_t n, double hz) {
+    FB_REQUIRE(b && n >= 0 && (n == 0 || idx), FB_ERR_ARG, "fb_set_drive: bad arguments");
+    for (int32_t q = 0; q < n; q++)
+        FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "fb_set_drive: index %d out of range", idx[q]);
+    fb_status st = set_values(b, b->d.drive, idx, n, hz, 0);
+    if (st) return st;
+    for (int32_t q = 0; q < n; q++) {
+        const int32_t i = idx[q];
+        if (hz > 0) {
+            if (!b->inDriven[i]) { b->inDriven[i] = 1; b->drivenList.push_back(i); b->drivenDirty = true; }
+        } else if (b->inDriven[i]) {
+            b->inDriven[i] = 0;
+            b->drivenList.erase(std::find(b->drivenList.begin(), b->drivenList.end(), i));
+            b->drivenDirty = true;
+        }
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_bias(fb_brain *b, const int32_t *idx, int32_t n, double mv) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_set_bias: null brain");
+    return set_values(b, b->d.bias, idx, n, mv, 0);
+}
+
+extern "C" fb_status fb_set_thr(fb_brain *b, int32_t i, double mv) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_set_thr: null brain");
+    return set_values(b, b->d.thr, &i, 1, mv, 0);
+}
+
+extern "C" fb_status fb_add_g(fb_brain *b, int32_t i, double e, double ii) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_add_g: null brain");
+    fb_status st = set_values(b, b->d.gE, &i, 1, e, 1);
+    if (st) return st;
+    return set_values(b, b->d.gI, &i, 1, ii, 1);
+}
+
+extern "C" fb_status fb_pulse(fb_brain *b, const int32_t *idx, int32_t n, double mv) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_pulse: null brain");
+    return set_values(b, b->d.gE, idx, n, mv, 1);
+}
+
+extern "C" fb_status fb_set_background(fb_brain *b, double hz, double mv) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_set_background: null brain");
+    b->p.bgRate = hz;
+    b->p.bgAmp = mv;
+    b->d.k.bgEvents = (float)((double)b->d.N * hz * b->p.dt / 1000.0);
+    b->d.k.bgAmp = (float)mv;
+    b->version++;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_driven(const fb_brain *b, int32_t *out, int32_t cap, int32_t *n) {
+    FB_REQUIRE(b && n, FB_ERR_ARG, "fb_driven: null argument");
+    *n = (int32_t)b->drivenList.size();
+    if (out) for (int32_t q = 0; q < std::min(cap, *n); q++) out[q] = b->drivenList[q];
+    return FB_OK;
+}
+
+// ------------------------------------------------------------------ stepping
+
+// lif.c's deliver loop is order-dependent beyond what the parallel scatter
+// reproduces in two cases: presynaptic inhibition reads gI of the arriving
+// neuron mid-delivery, and with tRef + dt - dt <= 0 a forced neuron can also
+// cross threshold in its own step, so one neuron arrives twice in one list
+// and its depression is applied twice in order.  Both take the serial path.
+static bool serial_deliver(const fb_brain *b) {
+    const Consts &k = b->d.k;
+    const float forced = k.tRef + k.dt;
+    return k.preInh > 0.f || !(forced - k.dt > 0.f) || !(k.tRef > 0.f);
+}
+
+static fb_status check_device_errors(fb_brain *b) {
+    Scalars sc;
+    FB_HIP(hipMemcpyAsync(&sc, b->d.sc, sizeof sc, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    FB_REQUIRE(!(sc.err & FB_DEVERR_COLL_OVERFLOW), FB_ERR_OVERFLOW, "arrival chain pool overflowed");
+    return FB_OK;
+}
+
+static fb_status run_launch(fb_brain *b, int32_t nsteps) {
+    for (int32_t s = 0; s < nsteps; s++) {
+        const int32_t slot = (b->head + b->d.nslots - 1) % b->d.nslots;
+        fb_status st = launch_step(b->d, b->head, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
+        if (st) return st;
+        b->lastSlot = slot;
+        b->head = (b->head + 1) % b->d.nslots;
+    }
+    return FB_OK;
+}
+
+// One hipGraph holds nslots steps starting at a given head; after a replay the
+// head is back where it started, so the same graph replays again.
+static fb_status run_graph(fb_brain *b, int32_t nsteps) {
+    const int32_t ns = b->d.nslots;
+    if (b->graphVersion != b->version) {
+        drop_graphs(b);
+        b->graphVersion = b->version;
+    }
+    while (nsteps >= ns) {
+        GraphCache &gc = b->graphs[b->head];
+        if (!gc.exec) {
+            FB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int32_t h = b->head;
+            for (int32_t s = 0; s < ns; s++) {
+                const int32_t slot = (h + ns - 1) % ns;
+                fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
+                if (st) { hipGraph_t tmp; hipStreamEndCapture(b->stream, &tmp); if (tmp) hipGraphDestroy(tmp); return st; }
+                h = (h + 1) % ns;
+            }
+            FB_HIP(hipStreamEndCapture(b->stream, &gc.graph));
+            FB_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+            gc.head = b->head;
+            gc.steps = ns;
+        }
+        FB_HIP(hipGraphLaunch(gc.exec, b->stream));
+        b->lastSlot = (b->head + ns - 2 + ns) % ns;   // out slot of the last step in the replay
+        nsteps -= ns;
+    }
+    return run_launch(b, nsteps);
+}
+
+static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
+    FB_REQUIRE(nsteps >= 0, FB_ERR_ARG, "fb_step: nsteps must be >= 0");
+    fb_status st = sync_driven(b);
+    if (st) return st;
+    if (nsteps == 0) return FB_OK;
+    if (engine == FB_ENGINE_AUTO) {
+        std::string why;
+        engine = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
+                                                                                : FB_ENGINE_LAUNCH;
+    }
+    switch (engine) {
+        case FB_ENGINE_LAUNCH: st = run_launch(b, nsteps); break;
+        case FB_ENGINE_GRAPH: st = run_graph(b, nsteps); break;
+        case FB_ENGINE_PERSISTENT: {
+            std::string why;
+            st = persistent_supported(b->d, serial_deliver(b), &why);
+            if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
+            st = launch_persistent(b->d, b->head, nsteps, b->stream);
+            if (st) return st;
+            b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
+            b->head = (b->head + nsteps) % b->d.nslots;
+            break;
+        }
+        default: set_error("fb_step: unknown engine %d", engine); return FB_ERR_ARG;
+    }
+    if (st) return st;
+    b->t += (double)nsteps * b->p.dt;
+    b->steps += nsteps;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_step(fb_brain *b, int32_t nsteps, int32_t engine, int32_t *nfired_last) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_step: null brain");
+    fb_status st = run_steps(b, nsteps, engine);
+    if (st) return st;
+    if (nfired_last) {
+        FB_HIP(hipMemcpyAsync(nfired_last, b->d.ringCount + b->lastSlot, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              b->stream));
+        FB_HIP(hipStreamSynchronize(b->stream));
+        return check_device_errors(b);
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_synchronize(fb_brain *b) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_synchronize: null brain");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    return check_device_errors(b);
+}
+
+extern "C" fb_status fb_fired(fb_brain *b, int32_t *out, int32_t cap, int32_t *n) {
+    FB_REQUIRE(b && n, FB_ERR_ARG, "fb_fired: null argument");
+    int32_t cnt = 0;
+    FB_HIP(hipMemcpyAsync(&cnt, b->d.ringCount + b->lastSlot, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    *n = cnt;
+    if (out && cap > 0 && cnt > 0) {
+        FB_HIP(hipMemcpyAsync(out, b->d.ring + (int64_t)b->lastSlot * b->d.ringStride, sizeof(int32_t) * (size_t)std::min(cap, cnt),
+                              hipMemcpyDeviceToHost, b->stream));
+        FB_HIP(hipStreamSynchronize(b->stream));
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_run_record(fb_brain *b, int32_t nsteps, int32_t engine, int32_t *nfired, int32_t *raster,
+                                   int64_t cap, int64_t *total) {
+    FB_REQUIRE(b && nsteps >= 0 && nfired && total, FB_ERR_ARG, "fb_run_record: bad arguments");
+    const int32_t ns = b->d.nslots, N = b->d.N;
+    const int32_t chunk = std::max(1, ns - 1);   // a list is delivered (and its count cleared) ns-1 steps later
+    if (!b->gatherBuf) {
+        FB_HIP(hipMalloc(&b->gatherBuf, sizeof(int32_t) * (size_t)chunk * 2 * (size_t)N));
+        FB_HIP(hipMalloc(&b->gatherCnt, sizeof(int32_t) * (size_t)ns));
+    }
+    int64_t tot = 0;
+    std::vector<int32_t> cnts((size_t)chunk);
+    for (int32_t done = 0; done < nsteps;) {
+        const int32_t m = std::min(chunk, nsteps - done);
+        const int32_t firstSlot = (b->head + ns - 1) % ns;
+        fb_status st = run_steps(b, m, engine);
+        if (st) return st;
+        st = launch_gather(b->d, firstSlot, m, b->gatherCnt, b->gatherBuf, b->stream);
+        if (st) return st;
+        FB_HIP(hipMemcpyAsync(cnts.data(), b->gatherCnt, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost, b->stream));
+        FB_HIP(hipStreamSynchronize(b->stream));
+        int64_t sum = 0;
+        for (int32_t q = 0; q < m; q++) { nfired[done + q] = cnts[q]; sum += cnts[q]; }
+        if (raster && tot < cap && sum > 0) {
+            const int64_t take = std::min<int64_t>(sum, cap - tot);
+            FB_HIP(hipMemcpyAsync(raster + tot, b->gatherBuf, sizeof(int32_t) * (size_t)take, hipMemcpyDeviceToHost,
+                                  b->stream));
+            FB_HIP(hipStreamSynchronize(b->stream));
+        }
+        tot += sum;
+        done += m;
+    }
+    *total = tot;
+    return check_device_errors(b);
+}
+
+extern "C" fb_status fb_get_array(fb_brain *b, int32_t which, void *dst) {
+    FB_REQUIRE(b && dst, FB_ERR_ARG, "fb_get_array: null argument");
+    float *src = array_ptr(b, which);
+    FB_REQUIRE(src, FB_ERR_ARG, "fb_get_array: unknown array %d", which);
+    FB_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)b->d.N, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_array(fb_brain *b, int32_t which, const void *src) {
+    FB_REQUIRE(b && src, FB_ERR_ARG, "fb_set_array: null argument");
+    float *dst = array_ptr(b, which);
+    FB_REQUIRE(dst, FB_ERR_ARG, "fb_set_array: unknown array %d", which);
+    FB_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)b->d.N, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head, float *bg_acc, double *t_ms) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_scalars: null brain");
+    Scalars sc;
+    FB_HIP(hipMemcpyAsync(&sc, b->d.sc, sizeof sc, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    if (rng) *rng = sc.rng;
+    if (head) *head = b->head;
+    if (bg_acc) *bg_acc = sc.bgAcc;
+    if (t_ms) *t_ms = b->t;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *spikes, int64_t *steps) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_counters: null brain");
+    Scalars sc;
+    FB_HIP(hipMemcpyAsync(&sc, b->d.sc, sizeof sc, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    if (edge_events) *edge_events = (int64_t)sc.edgeEvents;
+    if (spikes) *spikes = (int64_t)sc.spikes;
+    if (steps) *steps = b->steps;
+    return FB_OK;
+}

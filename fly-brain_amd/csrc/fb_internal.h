@@ -1,0 +1,196 @@
+// fb_internal.h -- shared definitions of the HIP engine (not part of the ABI).
+//
+// Layout in HBM, per brain (all SoA, f32 unless noted; N neurons):
+//   state      v gE gI refr trace adapt res bias thr drive  (10 x N), spikeCount u32[N]
+//   delay line ring i32[nslots x 2N] (one spike list per slot; 2N because with
+//              tRef <= 0 a forced neuron may also cross threshold in its step),
+//              ringCount i32[nslots]
+//   delivery   cnt i32[2N]   same-step arrivals per (sign class, target)
+//              add f32[2N]   the first arrival's addend f32(W*s)
+//              key u64[2N]   its order key (arrival position << 32 | position in row)
+//              chain i32[2N] head of the per-target list of further arrivals
+//              coll          further arrivals {key, addend, next}, one per edge event at most
+//   background bgCnt i32[N]  background hits this step (all add the same bgAmp)
+//   compaction stage i32[N], blkCnt i32[nblocks]
+// Per graph: indptr u32[N+1], edges {u32 target, f32 W}[E] interleaved so one
+// 8-byte load feeds one edge, sign f32[N], counts i32[E] (optional).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/flybrain.h"
+
+namespace fb {
+
+void set_error(const char *fmt, ...);
+
+#define FB_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t _e = (call);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            ::fb::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e),   \
+                            __FILE__, __LINE__);                                     \
+            return FB_ERR_DEVICE;                                                    \
+        }                                                                            \
+    } while (0)
+
+// f32 header constants, each formed in f64 and rounded once (src/lifwasm.js:58-67)
+struct Consts {
+    float dt, vRest, vThresh, vReset, tRef, adaptInc, depU;
+    float dE, dTr, dA, kRec, kM, dtS, eExc, eInh, cE, cI;
+    float bgEvents, bgAmp, preInh;
+    int32_t coba;
+};
+
+struct CollEntry {
+    uint64_t key;
+    float addend;
+    int32_t next;
+};
+
+// device-side scalars of one brain
+struct Scalars {
+    uint32_t rng;
+    float bgAcc;
+    int32_t nDrivenFired;   // forced spikes emitted this step
+    int32_t ncoll;          // entries used in coll this step
+    int32_t err;            // sticky error bits (FB_DEVERR_*)
+    int32_t pad;
+    unsigned long long edgeEvents;
+    unsigned long long spikes;
+};
+enum { FB_DEVERR_COLL_OVERFLOW = 1 };
+
+struct DevGraph {
+    int32_t N;
+    int64_t E;
+    const uint32_t *indptr;
+    const uint2 *edges;      // {target, bits of W}
+    const float *sign;
+    const int32_t *counts;
+};
+
+struct DevBrain {
+    int32_t N, nslots, nDriven, nblocks;
+    int64_t ringStride;        // entries per ring slot (2N)
+    Consts k;
+    DevGraph g;
+    float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr, *drive;
+    uint32_t *spikeCount;
+    int32_t *ring, *ringCount, *driven;
+    int32_t *cnt, *chain;
+    float *add;
+    uint64_t *key;
+    CollEntry *coll;
+    int64_t collCap;
+    int32_t *bgCnt;
+    int32_t *stage, *blkCnt;
+    Scalars *sc;
+    const uint32_t *jumpTab;   // 32 powers x 4 bytes x 256 entries of xorshift32 jumps
+};
+
+// ------------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ uint32_t xs32(uint32_t x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+// Applies M^(2^b) (M = one xorshift32 step as a GF(2) matrix) by 4 byte-table lookups.
+__device__ __forceinline__ uint32_t jump_pow2(const uint32_t *tab, int b, uint32_t x) {
+    const uint32_t *t = tab + (size_t)b * 1024;
+    return t[x & 255u] ^ t[256 + ((x >> 8) & 255u)] ^ t[512 + ((x >> 16) & 255u)] ^ t[768 + (x >> 24)];
+}
+
+// State after n further xorshift32 steps.
+__device__ __forceinline__ uint32_t jump(const uint32_t *tab, uint32_t x, uint32_t n) {
+    while (n) {
+        int b = __ffs(n) - 1;
+        x = jump_pow2(tab, b, x);
+        n &= n - 1;
+    }
+    return x;
+}
+
+// Membrane update of one neuron (src/wasm/lif.c:57-71): the reference's C
+// association, one f32 rounding per operation (the library is built with
+// -ffp-contract=off so no multiply-add is fused).
+struct NeuronState {
+    float v, gE, gI, refr, trace, adapt, res;
+};
+
+__device__ __forceinline__ void membrane(const Consts &k, NeuronState &s, float bias) {
+    const float vi = s.v, r = s.refr;
+    float drv;
+    if (k.coba) {
+        const float exc = (s.gE * (k.eExc - vi)) * k.cE;
+        const float inh = (s.gI * (vi - k.eInh)) * k.cI;
+        drv = (((k.vRest - vi) + exc) + inh) + bias;
+    } else {
+        drv = (((k.vRest - vi) + s.gE) + s.gI) + bias;
+    }
+    const float dv = drv * k.kM;
+    s.v = r > 0 ? k.vReset : vi + dv;
+    s.refr = r > 0 ? r - k.dt : r;
+    s.gE = s.gE * k.dE;
+    s.gI = s.gI * k.dE;
+    s.trace = s.trace * k.dTr;
+    s.adapt = s.adapt * k.dA;
+    s.res = s.res + (1.f - s.res) * k.kRec;
+}
+
+// lif.c:75 -- threshold test against the decayed adapt and the per-neuron offset
+__device__ __forceinline__ bool crosses(const Consts &k, const NeuronState &s, float thr) {
+    return s.v >= (k.vThresh + s.adapt) + thr && s.refr <= 0;
+}
+
+__device__ __forceinline__ void spike_effects(const Consts &k, NeuronState &s, float refrValue) {
+    s.v = k.vReset;
+    s.refr = refrValue;
+    s.trace = 1.f;
+    s.adapt = s.adapt + k.adaptInc;
+}
+
+// Sum of the same-step arrivals on one (sign class, target) in arrival order.
+// The first arrival sits in (key, add); the others hang off chain[] in the
+// coll pool in arbitrary order.  Insertion-sorts up to 32 of them in
+// registers; beyond that falls back to repeated minimum search (exact, slow).
+__device__ inline float ordered_sum(float g, int32_t count, uint64_t key0, float add0, int32_t head,
+                                    const CollEntry *coll) {
+    if (count <= 32) {
+        uint64_t ks[32];
+        float as[32];
+        int n = 0;
+        ks[n] = key0; as[n] = add0; n++;
+        for (int32_t p = head; p >= 0 && n < 32; p = coll[p].next) {
+            uint64_t kk = coll[p].key;
+            float aa = coll[p].addend;
+            int m = n++;
+            while (m > 0 && ks[m - 1] > kk) { ks[m] = ks[m - 1]; as[m] = as[m - 1]; m--; }
+            ks[m] = kk; as[m] = aa;
+        }
+        for (int m = 0; m < n; m++) g = g + as[m];
+        return g;
+    }
+    uint64_t last = 0;
+    bool first = true;
+    for (int32_t c = 0; c < count; c++) {
+        uint64_t best = ~0ull;
+        float ba = 0.f;
+        if ((first || key0 > last) && key0 < best) { best = key0; ba = add0; }
+        for (int32_t p = head; p >= 0; p = coll[p].next) {
+            uint64_t kk = coll[p].key;
+            if ((first || kk > last) && kk < best) { best = kk; ba = coll[p].addend; }
+        }
+        g = g + ba;
+        last = best;
+        first = false;
+    }
+    return g;
+}
+
+}  // namespace fb

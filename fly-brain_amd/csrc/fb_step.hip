@@ -1,0 +1,312 @@
+// fb_step.hip -- the per-step kernel sequence of the forward LIF engine.
+//
+// One step of the reference's lif_step (src/wasm/lif.c:29-80) runs as
+//   k_deliver   arrivals of ring[head] scatter into per-(sign, target)
+//               accumulators; the first arrival on a target keeps its exact
+//               addend f32(W*s) and order key, later ones go on a per-target
+//               chain.  No floating-point atomics: the sum is formed later,
+//               in arrival order, so it is bit-identical to lif.c's serial loop.
+//   k_driven    one workgroup: background draws and forced spikes.  lif.c
+//               draws from one serial xorshift32 stream; each thread counts the
+//               non-refractory driven neurons of its slice, a workgroup scan
+//               gives each slice its position in the stream, and the state is
+//               jumped there with precomputed GF(2) matrix powers.
+//   k_membrane  dense pass: apply the step's arrivals (single arrival: one
+//               add; several: the ordered sum), background hits, the membrane
+//               and decay update, the threshold test, and wave-ballot
+//               compaction of the spikes into a per-block staging list.
+//   k_compact   block prefix over the staged counts: the ring slot gets the
+//               forced spikes first, then threshold spikes in ascending index,
+//               which is lif.c's output order.
+#include "fb_internal.h"
+
+namespace fb {
+
+// ------------------------------------------------------------------ deliver
+
+__global__ void __launch_bounds__(256) k_deliver(DevBrain d, int32_t head) {
+    const int32_t A = d.ringCount[head];
+    const int32_t *list = d.ring + (int64_t)head * d.ringStride;
+    for (int32_t k = blockIdx.x; k < A; k += gridDim.x) {
+        const int32_t pre = list[k];
+        const float s = d.g.sign[pre] * d.res[pre];
+        if (s == 0.f) continue;                       // uniform across the block
+        const uint32_t a = d.g.indptr[pre], e = d.g.indptr[pre + 1];
+        __syncthreads();                              // every thread has read res[pre]
+        if (threadIdx.x == 0) {
+            d.res[pre] = d.res[pre] - d.k.depU * d.res[pre];
+            atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
+        }
+        const int32_t cls = s > 0 ? 0 : d.N;
+        for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x) {
+            const uint2 ed = d.g.edges[j];
+            const float addend = __uint_as_float(ed.y) * s;
+            const int32_t ci = cls + (int32_t)ed.x;
+            const uint64_t key = ((uint64_t)(uint32_t)k << 32) | (uint64_t)(j - a);
+            const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+            if (slot == 0) {
+                d.add[ci] = addend;
+                d.key[ci] = key;
+            } else {
+                const int32_t p = atomicAdd(&d.sc->ncoll, 1);
+                if (p < d.collCap) {
+                    d.coll[p].key = key;
+                    d.coll[p].addend = addend;
+                    d.coll[p].next = atomicExch(&d.chain[ci], p);
+                } else {
+                    atomicOr(&d.sc->err, FB_DEVERR_COLL_OVERFLOW);
+                }
+            }
+        }
+    }
+}
+
+// Presynaptic inhibition (lif.c:38) reads gI of the arriving neuron while the
+// step's own deliveries are still landing, so its result depends on the order
+// of every earlier arrival.  This variant keeps lif.c's serial order: one
+// workgroup walks the arrivals in list order and adds into gE/gI directly.
+__global__ void __launch_bounds__(1024) k_deliver_serial(DevBrain d, int32_t head, int32_t rowsUnique) {
+    const int32_t A = d.ringCount[head];
+    const int32_t *list = d.ring + (int64_t)head * d.ringStride;
+    for (int32_t k = 0; k < A; k++) {
+        const int32_t pre = list[k];
+        float s = d.g.sign[pre] * d.res[pre];
+        if (s == 0.f) continue;
+        if (d.k.preInh > 0.f && d.drive[pre] > 0.f) {
+            const float gi = d.gI[pre];
+            const float inh = gi < 0.f ? -gi : 0.f;
+            s = s / (1.f + d.k.preInh * inh);
+        }
+        const uint32_t a = d.g.indptr[pre], e = d.g.indptr[pre + 1];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            d.res[pre] = d.res[pre] - d.k.depU * d.res[pre];
+            atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
+        }
+        float *acc = s > 0 ? d.gE : d.gI;
+        if (rowsUnique) {
+            for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x) {
+                const uint2 ed = d.g.edges[j];
+                acc[ed.x] = acc[ed.x] + __uint_as_float(ed.y) * s;
+            }
+        } else if (threadIdx.x == 0) {
+            for (uint32_t j = a; j < e; j++) {
+                const uint2 ed = d.g.edges[j];
+                acc[ed.x] = acc[ed.x] + __uint_as_float(ed.y) * s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ driven
+
+template <int T>
+__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t *sh, int32_t *total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        int32_t add = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    const int32_t incl = sh[tid];
+    *total = sh[T - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(1024) k_driven(DevBrain d, int32_t slot) {
+    constexpr int T = 1024;
+    __shared__ int32_t scan[T];
+    __shared__ uint32_t s_rng;
+    __shared__ int32_t s_nbg;
+    const int tid = threadIdx.x;
+    const int32_t N = d.N;
+    if (tid == 0) {
+        // lif.c:47 -- the fractional background accumulator
+        float acc = d.sc->bgAcc + d.k.bgEvents;
+        const int32_t nbg = (int32_t)acc;
+        acc = acc - (float)nbg;
+        d.sc->bgAcc = acc;
+        s_nbg = nbg;
+        s_rng = d.sc->rng;
+    }
+    __syncthreads();
+    const uint32_t rng0 = s_rng;
+    const int32_t nbg = s_nbg;
+    // lif.c:48 -- draw t of the background burst is the stream's (t+1)-th state
+    for (int32_t t = tid; t < nbg; t += T) {
+        const uint32_t x = jump(d.jumpTab, rng0, (uint32_t)t + 1u);
+        atomicAdd(&d.bgCnt[x % (uint32_t)N], 1);
+    }
+    const uint32_t rng1 = nbg ? jump(d.jumpTab, rng0, (uint32_t)nbg) : rng0;
+
+    // lif.c:51-52 -- one draw per non-refractory driven neuron, in list order
+    const int32_t nD = d.nDriven;
+    const int32_t per = (nD + T - 1) / T;
+    const int32_t lo = min(nD, tid * per), hi = min(nD, lo + per);
+    int32_t draws = 0;
+    for (int32_t q = lo; q < hi; q++)
+        if (!(d.refr[d.driven[q]] > 0)) draws++;
+    int32_t totalDraws;
+    const int32_t off = block_exclusive_scan<T>(draws, scan, &totalDraws);
+    uint32_t x = jump(d.jumpTab, rng1, (uint32_t)off);
+    const float forcedRefr = d.k.tRef + d.k.dt;
+    int32_t fired = 0;
+    int32_t *flag = d.stage;   // stage[] is free until k_membrane; one flag per list slot
+    for (int32_t q = lo; q < hi; q++) {
+        const int32_t i = d.driven[q];
+        int32_t f = 0;
+        if (!(d.refr[i] > 0)) {
+            const float pFire = d.drive[i] * d.k.dtS;
+            x = xs32(x);
+            const float u = (float)(x >> 8) * (1.0f / 16777216.0f);
+            if (u < pFire) {
+                d.v[i] = d.k.vReset;
+                d.refr[i] = forcedRefr;
+                d.spikeCount[i] += 1u;
+                d.trace[i] = 1.f;
+                d.adapt[i] = d.adapt[i] + d.k.adaptInc;
+                f = 1;
+                fired++;
+            }
+        }
+        flag[q] = f;
+    }
+    int32_t totalFired;
+    int32_t pos = block_exclusive_scan<T>(fired, scan, &totalFired);
+    int32_t *out = d.ring + (int64_t)slot * d.ringStride;
+    for (int32_t q = lo; q < hi; q++)
+        if (flag[q]) out[pos++] = d.driven[q];
+    if (tid == 0) {
+        d.sc->rng = jump(d.jumpTab, rng1, (uint32_t)totalDraws);
+        d.sc->nDrivenFired = totalFired;
+    }
+}
+
+// ------------------------------------------------------------------ membrane + threshold
+
+__global__ void __launch_bounds__(256) k_membrane(DevBrain d) {
+    __shared__ int32_t wcount[4];
+    const int32_t N = d.N;
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    bool spk = false;
+    if (i < N) {
+        NeuronState s;
+        s.v = d.v[i]; s.gE = d.gE[i]; s.gI = d.gI[i]; s.refr = d.refr[i];
+        s.trace = d.trace[i]; s.adapt = d.adapt[i]; s.res = d.res[i];
+        // this step's arrivals, excitatory then inhibitory accumulator
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int32_t ci = c * N + i;
+            const int32_t n = d.cnt[ci];
+            if (n) {
+                float g = c ? s.gI : s.gE;
+                if (n == 1) {
+                    g = g + d.add[ci];
+                } else {
+                    g = ordered_sum(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
+                    d.chain[ci] = -1;
+                }
+                d.cnt[ci] = 0;
+                if (c) s.gI = g; else s.gE = g;
+            }
+        }
+        // background hits land after every delivery (lif.c:48)
+        const int32_t nb = d.bgCnt[i];
+        if (nb) {
+            for (int32_t t = 0; t < nb; t++) s.gE = s.gE + d.k.bgAmp;
+            d.bgCnt[i] = 0;
+        }
+        membrane(d.k, s, d.bias[i]);
+        if (crosses(d.k, s, d.thr[i])) {
+            spike_effects(d.k, s, d.k.tRef);
+            d.spikeCount[i] += 1u;
+            spk = true;
+        }
+        d.v[i] = s.v; d.gE[i] = s.gE; d.gI[i] = s.gI; d.refr[i] = s.refr;
+        d.trace[i] = s.trace; d.adapt[i] = s.adapt; d.res[i] = s.res;
+    }
+    // ascending compaction inside the block: wave ballot, then wave prefix
+    const uint64_t mask = __ballot(spk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wcount[wave] = __popcll(mask);
+    __syncthreads();
+    int32_t base = 0;
+    for (int w = 0; w < wave; w++) base += wcount[w];
+    if (spk) {
+        const int32_t pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+        d.stage[blockIdx.x * 256 + pos] = i;
+    }
+    if (threadIdx.x == 0) d.blkCnt[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+}
+
+__global__ void __launch_bounds__(256) k_compact(DevBrain d, int32_t slot, int32_t head) {
+    __shared__ int32_t red[256];
+    const int32_t b = blockIdx.x;
+    int32_t sum = 0;
+    for (int32_t j = threadIdx.x; j < b; j += 256) sum += d.blkCnt[j];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const int32_t before = red[0];
+    const int32_t nd = d.sc->nDrivenFired;
+    const int32_t nb = d.blkCnt[b];
+    int32_t *out = d.ring + (int64_t)slot * d.ringStride;
+    for (int32_t t = threadIdx.x; t < nb; t += 256) out[nd + before + t] = d.stage[b * 256 + t];
+    if (b == (int32_t)gridDim.x - 1 && threadIdx.x == 0) {
+        const int32_t total = nd + before + nb;
+        d.ringCount[slot] = total;
+        d.ringCount[head] = 0;               // lif.c:44, after k_deliver consumed it
+        d.sc->spikes += (unsigned long long)total;
+        d.sc->ncoll = 0;
+    }
+}
+
+// Copies the spike lists of `count` consecutive steps (slots s0, s0+1, ... mod
+// nslots) back to back into dst, counts into cnt.
+__global__ void __launch_bounds__(256) k_gather(DevBrain d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst) {
+    int64_t off = 0;
+    for (int32_t q = 0; q < count; q++) {
+        const int32_t slot = (s0 + q) % d.nslots;
+        const int32_t n = d.ringCount[slot];
+        if (threadIdx.x == 0 && blockIdx.x == 0) cnt[q] = n;
+        const int32_t *src = d.ring + (int64_t)slot * d.ringStride;
+        for (int32_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) dst[off + t] = src[t];
+        off += n;
+    }
+}
+
+// ------------------------------------------------------------------ host side
+
+int deliver_grid(const DevBrain &d) {
+    (void)d;
+    return 1024;
+}
+
+fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
+                      hipStream_t st) {
+    if (serialDeliver)
+        hipLaunchKernelGGL(k_deliver_serial, dim3(1), dim3(1024), 0, st, d, head, (int32_t)rowsUnique);
+    else
+        hipLaunchKernelGGL(k_deliver, dim3(deliver_grid(d)), dim3(256), 0, st, d, head);
+    hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, st, d, slot);
+    hipLaunchKernelGGL(k_membrane, dim3(d.nblocks), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_compact, dim3(d.nblocks), dim3(256), 0, st, d, slot, head);
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather, dim3(64), dim3(256), 0, st, d, s0, count, cnt, dst);
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+}  // namespace fb

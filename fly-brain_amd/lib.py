@@ -1,0 +1,101 @@
+"""ctypes binding of libflybrain.so (the C ABI in include/flybrain.h).
+
+The library is loaded from fly-brain_amd/lib/ only.  There is no fallback:
+if it is missing, loading raises, so nothing silently runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libflybrain.so")
+HEADER = os.path.join(ROOT, "include", "flybrain.h")
+
+FB_OK = 0
+ENGINES = {"auto": 0, "launch": 1, "graph": 2, "persistent": 3}
+ARRAYS = ("v", "gE", "gI", "refr", "trace", "adapt", "res", "bias", "thr", "drive", "spikeCount")
+
+_lib = None
+
+
+class FlyBrainError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"flybrain error {code}: {msg}")
+        self.code = code
+
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+U32 = ctypes.c_uint32
+D = ctypes.c_double
+F = ctypes.c_float
+PP = ctypes.POINTER(ctypes.c_void_p)
+PI32 = ctypes.POINTER(ctypes.c_int32)
+PI64 = ctypes.POINTER(ctypes.c_int64)
+
+_SIGS = {
+    "fb_last_error": (ctypes.c_char_p, []),
+    "fb_device_info": (I32, [ctypes.c_char_p, I32, PI32]),
+    "fb_graph_create": (I32, [I32, I64, P, P, P, P, P, PP]),
+    "fb_graph_retain": (I32, [P]),
+    "fb_graph_release": (I32, [P]),
+    "fb_brain_create": (I32, [P, P, D, PP]),
+    "fb_brain_destroy": (I32, [P]),
+    "fb_reset": (I32, [P]),
+    "fb_set_params": (I32, [P, P]),
+    "fb_get_params": (I32, [P, P]),
+    "fb_set_drive": (I32, [P, P, I32, D]),
+    "fb_set_drive_one": (I32, [P, I32, D]),
+    "fb_set_bias": (I32, [P, P, I32, D]),
+    "fb_set_thr": (I32, [P, I32, D]),
+    "fb_add_g": (I32, [P, I32, D, D]),
+    "fb_pulse": (I32, [P, P, I32, D]),
+    "fb_set_background": (I32, [P, D, D]),
+    "fb_driven": (I32, [P, P, I32, PI32]),
+    "fb_step": (I32, [P, I32, I32, PI32]),
+    "fb_fired": (I32, [P, P, I32, PI32]),
+    "fb_run_record": (I32, [P, I32, I32, P, P, I64, PI64]),
+    "fb_get_array": (I32, [P, I32, P]),
+    "fb_set_array": (I32, [P, I32, P]),
+    "fb_brain_scalars": (I32, [P, ctypes.POINTER(U32), PI32, ctypes.POINTER(F), ctypes.POINTER(D)]),
+    "fb_brain_counters": (I32, [P, PI64, PI64, PI64]),
+    "fb_synchronize": (I32, [P]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    txt = open(HEADER).read()
+    pat = r"^\s*(?:const\s+)?[A-Za-z_0-9]+\s*\*?\s*(fb_[a-z0-9_]+)\s*\("
+    return sorted(set(re.findall(pat, txt, flags=re.MULTILINE)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlyBrainError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int):
+    if code != FB_OK:
+        msg = load().fb_last_error().decode(errors="replace")
+        raise FlyBrainError(code, msg)
+
+
+def ptr(a) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)

@@ -1,0 +1,135 @@
+/*
+ * flybrain.h -- C ABI of the MI355X engine for fly-brain's connectome LIF step
+ * and its surrogate-gradient adjoint.
+ *
+ * Every entry point returns an fb_status (0 = FB_OK) and never throws; the text
+ * of the last failure on the calling thread is fb_last_error().  Buffers are
+ * plain host pointers; device memory is owned by the library.
+ *
+ * Reference interfaces replaced (paths into Lulzx/fly-brain):
+ *   fb_graph_*   the shared read-only connectome of src/brainsetup.js:10-19 and
+ *                the per-device graph cache of src/lifgpu.js:154-181
+ *   fb_brain_*,
+ *   fb_step, ... LIFWasm (src/lifwasm.js:36-87) over lif_step (src/wasm/lif.c:29-80),
+ *                and its WebGPU sibling LIFGpu (src/lifgpu.js:183-370)
+ *   fb_batch_*   the one-brain-per-worker ensembles (src/brainsetup.js:8,
+ *                scripts/lif_ensemble.mjs:16-35) and the Bend lane ensemble
+ *                (kernel.bend:293-378) as one launch over B instances
+ *   fb_diff_*    LIFDiff.forward / backward (src/lifdiff.js:219-686) and the
+ *                WebGPU adjoint LIFAdjointGPU (src/lifdiff_gpu.js:246-503)
+ *
+ * Threading: one brain per host thread; a graph may be shared by any number of
+ * brains and threads (it is reference counted and read-only on the device).
+ */
+#ifndef FLYBRAIN_H
+#define FLYBRAIN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t fb_status;
+enum {
+    FB_OK = 0,
+    FB_ERR_ARG = 1,          /* bad argument (null, out of range, size mismatch) */
+    FB_ERR_DEVICE = 2,       /* HIP runtime failure or no usable GPU */
+    FB_ERR_UNSUPPORTED = 3,  /* valid request this build does not run on the GPU */
+    FB_ERR_OVERFLOW = 4,     /* a bounded device buffer would overflow */
+    FB_ERR_STATE = 5         /* call not valid in the object's current state */
+};
+
+/* Model parameters, as the reference's JS wrapper holds them (f64), restricted
+ * to what the step reads: DEFAULTS of src/lif.js:6-15 plus the background
+ * fields of src/lifwasm.js:67,69.  Derived constants (exp(-dt/tauSyn), dt/tauM,
+ * 1/(eExc-vRest), ...) are formed in f64 and rounded once to f32, exactly as
+ * src/lifwasm.js:58-67 writes the brain header.  coba is 0 or 1. */
+typedef struct {
+    double dt, vRest, vThresh, vReset, tauM, tauSyn, tRef, delay;
+    double traceTau, adaptInc, adaptTau, depU, depTau;
+    double coba, eExc, eInh;
+    double bgRate, bgAmp, preInh;
+} fb_params;
+
+/* State arrays of one brain (the typed-array fields of LIFWasm,
+ * src/lifwasm.js:43-45).  All are f32[N] except FB_SPIKECOUNT (u32[N]). */
+enum {
+    FB_V = 0, FB_GE, FB_GI, FB_REFR, FB_TRACE, FB_ADAPT, FB_RES,
+    FB_BIAS, FB_THR, FB_DRIVE, FB_SPIKECOUNT, FB_NARRAYS
+};
+
+/* Stepping engines.  All produce identical rasters and state. */
+enum {
+    FB_ENGINE_AUTO = 0,        /* the fastest engine valid for the brain's parameters */
+    FB_ENGINE_LAUNCH = 1,      /* one kernel sequence per step */
+    FB_ENGINE_GRAPH = 2,       /* the same sequence captured in a hipGraph, replayed */
+    FB_ENGINE_PERSISTENT = 3   /* one cooperative launch advances many steps; the
+                                  synaptic delay is used as a lookahead window */
+};
+
+typedef struct fb_graph fb_graph;
+typedef struct fb_brain fb_brain;
+
+const char *fb_last_error(void);
+fb_status fb_device_info(char *name, int32_t cap, int32_t *n_cu);
+
+/* ---------------------------------------------------------------- graph
+ * CSR by presynaptic neuron: indptr u32[N+1], indices u32[E], the baked
+ * weight W f32[E] and sign f32[N] that writeGraph produces (src/lifwasm.js:14-34;
+ * lif.c reads them as W[j] * sign[pre] * res[pre]).  counts (i32[E], may be
+ * NULL) are the raw synapse counts; they are required by the ensemble and
+ * gradient engines, which rebuild W = count x gain on the device. */
+fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t *indices,
+                          const float *weights, const float *sign, const int32_t *counts,
+                          fb_graph **out);
+fb_status fb_graph_retain(fb_graph *g);
+fb_status fb_graph_release(fb_graph *g);
+
+/* ---------------------------------------------------------------- brain
+ * new LIFWasm({graph, N, params, seed}) (src/lifwasm.js:37-54): state reset,
+ * bias/thr/drive zero, RNG = (seed * 2654435761) >>> 0 || 1, ring head 0. */
+fb_status fb_brain_create(fb_graph *g, const fb_params *p, double seed, fb_brain **out);
+fb_status fb_brain_destroy(fb_brain *b);
+/* reset(): v = vRest, gE = gI = refr = trace = adapt = 0, res = 1, spike counts
+ * and ring counts zero; RNG, ring head, bias, thr, drive and the driven set are
+ * kept (src/lifwasm.js:78). */
+fb_status fb_reset(fb_brain *b);
+/* setParams(q) (src/lif.js:47, src/lifgpu.js:349): rewrites the constants; a
+ * change of delay/dt that changes nslots re-creates the delay line empty. */
+fb_status fb_set_params(fb_brain *b, const fb_params *p);
+fb_status fb_get_params(const fb_brain *b, fb_params *p);
+
+/* Setters of src/lifwasm.js:69-76.  Values are f64 as JS numbers are; a
+ * setter stores f32(value) and an adder stores f32(f64(old) + value). */
+fb_status fb_set_drive(fb_brain *b, const int32_t *idx, int32_t n, double hz);
+fb_status fb_set_drive_one(fb_brain *b, int32_t i, double hz);
+fb_status fb_set_bias(fb_brain *b, const int32_t *idx, int32_t n, double mv);
+fb_status fb_set_thr(fb_brain *b, int32_t i, double mv);
+fb_status fb_add_g(fb_brain *b, int32_t i, double e, double ii);
+fb_status fb_pulse(fb_brain *b, const int32_t *idx, int32_t n, double mv);
+fb_status fb_set_background(fb_brain *b, double hz, double mv);
+/* the driven set in insertion order (drivenSet of src/lifwasm.js:51,70-71) */
+fb_status fb_driven(const fb_brain *b, int32_t *out, int32_t cap, int32_t *n);
+
+/* step() x nsteps (src/lifwasm.js:79-85 over lif_step).  nfired_last may be NULL. */
+fb_status fb_step(fb_brain *b, int32_t nsteps, int32_t engine, int32_t *nfired_last);
+/* The spikes of the last step, in lif.c's order: forced (driven) spikes in
+ * driven-set order, then threshold spikes in ascending index (lif.c:49-76). */
+fb_status fb_fired(fb_brain *b, int32_t *out, int32_t cap, int32_t *n);
+/* Runs nsteps and records every step's spike count and spike list.
+ * raster receives the lists back to back (at most cap entries); total is the
+ * number of spikes in the run even if it exceeds cap. */
+fb_status fb_run_record(fb_brain *b, int32_t nsteps, int32_t engine, int32_t *nfired,
+                        int32_t *raster, int64_t cap, int64_t *total);
+fb_status fb_get_array(fb_brain *b, int32_t which, void *dst);
+fb_status fb_set_array(fb_brain *b, int32_t which, const void *src);
+/* RNG state, ring head, background accumulator, simulated time in ms */
+fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head, float *bg_acc, double *t_ms);
+/* edge events (sum of out-degrees of the arriving spikes) delivered since create */
+fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *spikes, int64_t *steps);
+fb_status fb_synchronize(fb_brain *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -133,7 +133,7 @@ def play(brain, ops, N: int):
 
 def brain_run(brain, steps: int):
     if hasattr(brain, "run_record"):
-        return brain.run_record(steps)
+        return brain.run_record(steps, cap=steps * min(brain.N, 512))
     nf, ras, tot = brain.run(steps, raster_cap=steps * brain.N)
     return nf, ras
 
