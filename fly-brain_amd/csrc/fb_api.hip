@@ -143,7 +143,8 @@ struct fb_brain {
     fb_graph *g = nullptr;
     fb_params p{};
     DevBrain d{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream all work is queued on
+    hipStream_t ownStream = nullptr;   // the brain's own stream
     std::vector<int32_t> drivenList;   // insertion order of the driven set
     std::vector<uint8_t> inDriven;
     bool drivenDirty = true;
@@ -241,12 +242,22 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
                 if (edges[j].x == edges[j - 1].x) g->rowsUnique = false;
         }
     }
-    uint32_t *dIndptr; uint2 *dEdges; float *dSign; int32_t *dCounts = nullptr;
+    uint32_t *dIndptr; uint2 *dEdges; float *dSign; int32_t *dCounts = nullptr; uint32_t *dRowSyn;
     fb_status st;
     if ((st = dalloc(g->allocs, &dIndptr, (size_t)N + 1)) ||
         (st = dalloc(g->allocs, &dEdges, (size_t)std::max<int64_t>(E, 1))) ||
-        (st = dalloc(g->allocs, &dSign, (size_t)N)))
+        (st = dalloc(g->allocs, &dSign, (size_t)N)) || (st = dalloc(g->allocs, &dRowSyn, (size_t)N)))
         return st;
+    {
+        std::vector<uint32_t> rs((size_t)N, 0);
+        if (counts)
+            for (int32_t i = 0; i < N; i++) {
+                uint64_t s = 0;
+                for (uint32_t j = indptr[i]; j < indptr[i + 1]; j++) s += (uint64_t)std::max(counts[j], 0);
+                rs[i] = (uint32_t)std::min<uint64_t>(s, UINT32_MAX);
+            }
+        FB_HIP(hipMemcpy(dRowSyn, rs.data(), sizeof(uint32_t) * (size_t)N, hipMemcpyHostToDevice));
+    }
     FB_HIP(hipMemcpy(dIndptr, indptr, sizeof(uint32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
     if (E) FB_HIP(hipMemcpy(dEdges, edges.data(), sizeof(uint2) * (size_t)E, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(dSign, sign, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
@@ -254,7 +265,7 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
         if ((st = dalloc(g->allocs, &dCounts, (size_t)std::max<int64_t>(E, 1)))) return st;
         if (E) FB_HIP(hipMemcpy(dCounts, cnts.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
     }
-    g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts;
+    g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts; g->d.rowSyn = dRowSyn;
     *out = g;
     return FB_OK;
 }
@@ -380,7 +391,8 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     d.nblocks = (N + 255) / 256;
     derive_consts(*p, N, d.k);
     d.g = g->d;
-    FB_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    FB_HIP(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking));
+    b->stream = b->ownStream;
     fb_status st;
     float *f;
     // ten f32 state arrays back to back, then spike counts
@@ -424,7 +436,7 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
     for (void *p : b->allocs) hipFree(p);
     if (b->gatherBuf) hipFree(b->gatherBuf);
     if (b->gatherCnt) hipFree(b->gatherCnt);
-    hipStreamDestroy(b->stream);
+    hipStreamDestroy(b->ownStream);
     fb_graph_release(b->g);
     delete b;
     return FB_OK;
@@ -673,6 +685,14 @@ extern "C" fb_status fb_step(fb_brain *b, int32_t nsteps, int32_t engine, int32_
     return FB_OK;
 }
 
+extern "C" fb_status fb_brain_set_stream(fb_brain *b, void *stream) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_set_stream: null brain");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    b->stream = stream ? (hipStream_t)stream : b->ownStream;
+    drop_graphs(b);   // graphs were captured on the previous stream's work queue
+    return FB_OK;
+}
+
 extern "C" fb_status fb_synchronize(fb_brain *b) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_synchronize: null brain");
     FB_HIP(hipStreamSynchronize(b->stream));
@@ -758,12 +778,14 @@ extern "C" fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head,
     return FB_OK;
 }
 
-extern "C" fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *spikes, int64_t *steps) {
+extern "C" fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
+                                       int64_t *steps) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_counters: null brain");
     Scalars sc;
     FB_HIP(hipMemcpyAsync(&sc, b->d.sc, sizeof sc, hipMemcpyDeviceToHost, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
     if (edge_events) *edge_events = (int64_t)sc.edgeEvents;
+    if (synapse_events) *synapse_events = (int64_t)sc.synEvents;
     if (spikes) *spikes = (int64_t)sc.spikes;
     if (steps) *steps = b->steps;
     return FB_OK;

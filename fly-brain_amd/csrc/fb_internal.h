@@ -59,6 +59,7 @@ struct Scalars {
     int32_t err;            // sticky error bits (FB_DEVERR_*)
     int32_t pad;
     unsigned long long edgeEvents;
+    unsigned long long synEvents;
     unsigned long long spikes;
 };
 enum { FB_DEVERR_COLL_OVERFLOW = 1 };
@@ -70,6 +71,7 @@ struct DevGraph {
     const uint2 *edges;      // {target, bits of W}
     const float *sign;
     const int32_t *counts;
+    const uint32_t *rowSyn;  // synapse count summed over each row (0 without counts)
 };
 
 struct DevBrain {

@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(256) k_deliver(DevBrain d, int32_t head) {
         if (threadIdx.x == 0) {
             d.res[pre] = d.res[pre] - d.k.depU * d.res[pre];
             atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
+            atomicAdd(&d.sc->synEvents, (unsigned long long)d.g.rowSyn[pre]);
         }
         const int32_t cls = s > 0 ? 0 : d.N;
         for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x) {
@@ -82,6 +83,7 @@ __global__ void __launch_bounds__(1024) k_deliver_serial(DevBrain d, int32_t hea
         if (threadIdx.x == 0) {
             d.res[pre] = d.res[pre] - d.k.depU * d.res[pre];
             atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
+            atomicAdd(&d.sc->synEvents, (unsigned long long)d.g.rowSyn[pre]);
         }
         float *acc = s > 0 ? d.gE : d.gI;
         if (rowsUnique) {

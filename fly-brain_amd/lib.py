@@ -62,7 +62,8 @@ _SIGS = {
     "fb_get_array": (I32, [P, I32, P]),
     "fb_set_array": (I32, [P, I32, P]),
     "fb_brain_scalars": (I32, [P, ctypes.POINTER(U32), PI32, ctypes.POINTER(F), ctypes.POINTER(D)]),
-    "fb_brain_counters": (I32, [P, PI64, PI64, PI64]),
+    "fb_brain_counters": (I32, [P, PI64, PI64, PI64, PI64]),
+    "fb_brain_set_stream": (I32, [P, P]),
     "fb_synchronize": (I32, [P]),
 }
 

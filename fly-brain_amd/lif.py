@@ -140,9 +140,14 @@ class LIFHip:
         return int(h.value)
 
     def counters(self) -> dict:
-        e, s, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        _l.check(self.lib.fb_brain_counters(self.h, ctypes.byref(e), ctypes.byref(s), ctypes.byref(n)))
-        return {"edge_events": e.value, "spikes": s.value, "steps": n.value}
+        e, y, s, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _l.check(self.lib.fb_brain_counters(self.h, ctypes.byref(e), ctypes.byref(y), ctypes.byref(s),
+                                            ctypes.byref(n)))
+        return {"edge_events": e.value, "synapse_events": y.value, "spikes": s.value, "steps": n.value}
+
+    def set_stream(self, stream_handle: int | None):
+        """Queue this brain's work on an external HIP stream (e.g. torch's)."""
+        _l.check(self.lib.fb_brain_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)))
 
     # ---------------------------------------------------------------- setters (src/lifwasm.js:69-76)
     def setDriveOne(self, i: int, rate: float):

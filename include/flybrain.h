@@ -125,9 +125,18 @@ fb_status fb_get_array(fb_brain *b, int32_t which, void *dst);
 fb_status fb_set_array(fb_brain *b, int32_t which, const void *src);
 /* RNG state, ring head, background accumulator, simulated time in ms */
 fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head, float *bg_acc, double *t_ms);
-/* edge events (sum of out-degrees of the arriving spikes) delivered since create */
-fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *spikes, int64_t *steps);
+/* Work done since create: edge events (the sum of out-degrees of the arriving
+ * spikes, the reference's "edges/ms" of scripts/profile_fly.mjs:19,25),
+ * synapse events (the same sum weighted by synapse count; needs counts at
+ * graph creation), spikes emitted and steps run. */
+fb_status fb_brain_counters(fb_brain *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
+                            int64_t *steps);
 fb_status fb_synchronize(fb_brain *b);
+/* Runs the brain's work on the caller's HIP stream (a hipStream_t passed as a
+ * pointer; NULL restores the brain's own stream), so a caller can order it with
+ * its own work and time it with its own events.  No counterpart in the
+ * reference, which has no streams. */
+fb_status fb_brain_set_stream(fb_brain *b, void *stream);
 
 #ifdef __cplusplus
 }
