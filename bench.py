@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Synaptic events/s of the connectome LIF step on one MI355X.
+
+Workload: the full-scale synthetic connectome -- 165,122 neurons, ~8.0M
+directed edges (heavy-tailed counts, ~104M synapses, hub rows of 20,000),
+CUBA, dt 0.1 ms, 1% of neurons Poisson-driven at U(100, 200) Hz.  A "step" is
+one 0.1 ms LIF step of the whole network (deliver, forced spikes, membrane,
+threshold); K steps are timed with HIP events on the engine's stream, after W
+warm-up steps, with the state already resident in HBM.
+
+value = edge events delivered in the timed region / seconds, where an edge
+event is one (arriving spike, outgoing edge) pair -- the reference's
+"edges/ms" (scripts/profile_fly.mjs:19,25 of Lulzx/fly-brain).
+
+cpu_baseline: the reference's own lif.c compiled natively (oracle/_ref) when
+present, else the oracle restatement, on one host core, over a bounded sample
+of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pkg(name):
+    return importlib.import_module("fly-brain_amd." + name)
+
+
+def algorithmic_bytes(N: int, edge_events: int, spikes: int, steps: int, live_adapt_res: bool = False) -> int:
+    """Per-step forward bytes, summed: 32 N (v, gE, gI, refr read and written)
+    + 8 per edge event (u32 target + f32 weight) + 4 per emitted spike (+16 N
+    when adaptation or depression are live)."""
+    per_neuron = 32 + (16 if live_adapt_res else 0)
+    return per_neuron * N * steps + 8 * edge_events + 4 * spikes
+
+
+def cpu_baseline(g, p, drv, rates, steps: int, warm: int):
+    """lif.c (or its restatement) on one core over `steps` steps after `warm`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    kind = "ref" if orc.available("ref") else "oracle"
+    if not orc.available(kind):
+        return None
+    b = orc.OracleBrain(g, p, seed=1, kind=kind)
+    for i, r in zip(drv.tolist(), rates.tolist()):
+        b.setDriveOne(i, r)
+    outdeg = g.outdeg
+    nf_w, ras_w, _ = b.run(warm, raster_cap=warm * 4096)
+    t0 = time.perf_counter()
+    nf, ras, _ = b.run(steps, raster_cap=steps * 4096)
+    dt = time.perf_counter() - t0
+    # arrivals in the timed window: spikes emitted D steps earlier
+    D = b.nslots - 1
+    nf_all = np.concatenate([nf_w, nf])
+    ras_all = np.concatenate([ras_w, ras])
+    start = np.concatenate([[0], np.cumsum(nf_all)])
+    lo, hi = max(0, warm - D), max(0, warm + steps - D)
+    events = int(outdeg[ras_all[start[lo]:start[hi]]].sum())
+    return {
+        "value": events / dt, "unit": "synaptic events/s", "cores": 1,
+        "kind": "reference" if kind == "ref" else "port",
+        "sample": f"{steps} steps (after {warm} warm-up) of the same 165,122-neuron workload, "
+                  f"{'lif.c compiled natively (gcc -O2 -ffp-contract=off)' if kind == 'ref' else 'oracle/lif_oracle.c'}, 1 thread",
+        "sim_s_per_wall_s": steps * p["dt"] / 1000.0 / dt,
+        "ms_per_step": dt * 1000.0 / steps,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--engine", default="auto", choices=["auto", "launch", "graph", "persistent"])
+    ap.add_argument("--coba", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=10000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
+              "(multi-GPU sharding is design only).", file=sys.stderr)
+        sys.exit(2)
+
+    synth, P, lif, hiprt = pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt")
+    t_build = time.perf_counter()
+    g = synth.config2_graph()
+    drv = synth.driven_set(g.N, 0.01, 4)
+    rates = synth.drive_rates(drv.size, 100.0, 200.0, 5)
+    p = P.config_params(coba=args.coba)
+    hg = lif.HipGraph.from_synth(g)
+    brain = lif.LIFHip(hg, p, seed=1, engine=args.engine)
+    for i, r in zip(drv.tolist(), rates.tolist()):
+        brain.setDriveOne(i, r)
+    print(f"# graph N={g.N} E={g.E} synapses={int(g.counts.sum())} built in {time.perf_counter() - t_build:.1f}s",
+          file=sys.stderr, flush=True)
+
+    stream = hiprt.Stream()
+    brain.set_stream(stream.handle)
+    brain.advance(args.warmup)
+    c0 = brain.counters()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    hiprt.device_synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    brain.run_async(args.steps)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_ms(e1)
+    brain.flush()
+    c1 = brain.counters()
+    events = c1["edge_events"] - c0["edge_events"]
+    syn = c1["synapse_events"] - c0["synapse_events"]
+    spikes = c1["spikes"] - c0["spikes"]
+    secs = ms / 1000.0
+    value = events / secs
+    bytes_ = algorithmic_bytes(g.N, events, spikes, args.steps)
+    achieved = bytes_ / secs / 1e9
+    engine = args.engine
+    out = {
+        "metric": "synaptic events/s, full 165k LIF: plain vs batch-64 vs fwd+adjoint",
+        "value": value,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "full-scale forward: 165,122 neurons, ~8.0M edges, ~104M synapses, "
+                        f"{'COBA' if args.coba else 'CUBA'}, dt 0.1 ms, 1% driven at U(100,200) Hz",
+            "neurons": g.N, "edges": g.E, "synapses": int(g.counts.sum()), "engine": engine,
+        },
+        "sim_s_per_wall_s": args.steps * p["dt"] / 1000.0 / secs,
+        "synapse_events_per_s": syn / secs,
+        "spikes_per_step": spikes / args.steps,
+        "mean_rate_hz": spikes / args.steps / g.N / (p["dt"] / 1000.0),
+        "wall_s_host_clock": wall,
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "whole step sequence (per-step algorithmic bytes 32N + 8 A_e + 4 S)",
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)
+        out["cpu_baseline"] = cb
+        if cb:
+            out["gpu_over_cpu_sim_rate"] = out["sim_s_per_wall_s"] / cb["sim_s_per_wall_s"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -649,7 +649,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     if (engine == FB_ENGINE_AUTO) {
         std::string why;
         engine = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
-                                                                                : FB_ENGINE_LAUNCH;
+                                                                                : FB_ENGINE_GRAPH;
     }
     switch (engine) {
         case FB_ENGINE_LAUNCH: st = run_launch(b, nsteps); break;
