@@ -33,7 +33,9 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
                       hipStream_t st);
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
-fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st);
+fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
+                            int64_t *launches);
+void persistent_release(const int32_t *key);
 
 // ------------------------------------------------------------------ jump tables
 
@@ -151,6 +153,7 @@ struct fb_brain {
     int32_t head = 0, lastSlot = 0;
     double t = 0.0;
     int64_t steps = 0;
+    int64_t fallbackSteps = 0, persistentLaunches = 0;
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
@@ -265,6 +268,7 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
         if ((st = dalloc(g->allocs, &dCounts, (size_t)std::max<int64_t>(E, 1)))) return st;
         if (E) FB_HIP(hipMemcpy(dCounts, cnts.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
     }
+    for (int32_t i = 0; i < N; i++) g->d.maxRow = std::max<int64_t>(g->d.maxRow, (int64_t)indptr[i + 1] - indptr[i]);
     g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts; g->d.rowSyn = dRowSyn;
     *out = g;
     return FB_OK;
@@ -433,6 +437,7 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_brain_destroy: null brain");
     hipStreamSynchronize(b->stream);
     drop_graphs(b);
+    persistent_release(b->d.driven);
     for (void *p : b->allocs) hipFree(p);
     if (b->gatherBuf) hipFree(b->gatherBuf);
     if (b->gatherCnt) hipFree(b->gatherCnt);
@@ -658,7 +663,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             std::string why;
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
-            st = launch_persistent(b->d, b->head, nsteps, b->stream);
+            st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches);
             if (st) return st;
             b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
             b->head = (b->head + nsteps) % b->d.nslots;
@@ -682,6 +687,20 @@ extern "C" fb_status fb_step(fb_brain *b, int32_t nsteps, int32_t engine, int32_
         FB_HIP(hipStreamSynchronize(b->stream));
         return check_device_errors(b);
     }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value) {
+    FB_REQUIRE(b && name && value, FB_ERR_ARG, "fb_brain_stat: null argument");
+    std::string n(name);
+    std::string why;
+    if (n == "fallback_steps") *value = (double)b->fallbackSteps;
+    else if (n == "persistent_launches") *value = (double)b->persistentLaunches;
+    else if (n == "persistent_supported")
+        *value = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? 1.0 : 0.0;
+    else if (n == "nslots") *value = (double)b->d.nslots;
+    else if (n == "serial_deliver") *value = serial_deliver(b) ? 1.0 : 0.0;
+    else { set_error("fb_brain_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
     return FB_OK;
 }
 

@@ -72,6 +72,7 @@ struct DevGraph {
     const float *sign;
     const int32_t *counts;
     const uint32_t *rowSyn;  // synapse count summed over each row (0 without counts)
+    int64_t maxRow;          // longest row
 };
 
 struct DevBrain {

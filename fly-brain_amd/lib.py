@@ -64,6 +64,7 @@ _SIGS = {
     "fb_brain_scalars": (I32, [P, ctypes.POINTER(U32), PI32, ctypes.POINTER(F), ctypes.POINTER(D)]),
     "fb_brain_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_brain_set_stream": (I32, [P, P]),
+    "fb_brain_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
     "fb_synchronize": (I32, [P]),
 }
 

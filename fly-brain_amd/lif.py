@@ -145,6 +145,11 @@ class LIFHip:
                                             ctypes.byref(n)))
         return {"edge_events": e.value, "synapse_events": y.value, "spikes": s.value, "steps": n.value}
 
+    def stat(self, name: str) -> float:
+        v = ctypes.c_double()
+        _l.check(self.lib.fb_brain_stat(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
+
     def set_stream(self, stream_handle: int | None):
         """Queue this brain's work on an external HIP stream (e.g. torch's)."""
         _l.check(self.lib.fb_brain_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)))
