@@ -34,7 +34,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches);
+                            int64_t *launches, double *kernelMs);
 void persistent_release(const int32_t *key);
 
 // ------------------------------------------------------------------ jump tables
@@ -154,6 +154,7 @@ struct fb_brain {
     double t = 0.0;
     int64_t steps = 0;
     int64_t fallbackSteps = 0, persistentLaunches = 0;
+    double persistentMs = 0.0;   // summed HIP-event duration of the lookahead kernel launches
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
@@ -663,7 +664,8 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             std::string why;
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
-            st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches);
+            st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
+                                   &b->persistentMs);
             if (st) return st;
             b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
             b->head = (b->head + nsteps) % b->d.nslots;
@@ -696,6 +698,7 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
     std::string why;
     if (n == "fallback_steps") *value = (double)b->fallbackSteps;
     else if (n == "persistent_launches") *value = (double)b->persistentLaunches;
+    else if (n == "persistent_kernel_ms") *value = b->persistentMs;
     else if (n == "persistent_supported")
         *value = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? 1.0 : 0.0;
     else if (n == "nslots") *value = (double)b->d.nslots;

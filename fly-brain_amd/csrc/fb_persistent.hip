@@ -628,6 +628,7 @@ struct PersistBuffers {
     int32_t *ctl = nullptr;
     int32_t N = 0, nDrivenCap = 0, fwdCap = 0;
     int32_t drivenVersion = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain's driven buffer
@@ -640,6 +641,7 @@ void persistent_release(const int32_t *key) {
     if (it == g_pbuf.end()) return;
     PersistBuffers &pb = it->second;
     if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); }
+    if (pb.ev0) { hipEventDestroy(pb.ev0); hipEventDestroy(pb.ev1); }
     g_pbuf.erase(it);
 }
 
@@ -653,7 +655,7 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
 }
 
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches) {
+                            int64_t *launches, double *kernelMs) {
     std::string why;
     fb_status rc = persistent_supported(d, false, &why);
     if (rc) { set_error("persistent engine: %s", why.c_str()); return rc; }
@@ -670,6 +672,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.fwdKey, sizeof(uint64_t) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.fwdAdd, sizeof(float) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
+        if (!pb.ev0) {
+            FB_HIP(hipEventCreate(&pb.ev0));
+            FB_HIP(hipEventCreate(&pb.ev1));
+        }
     }
     // driven positions (the list may have changed since the last launch)
     hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven, d.N, d.nDriven);
@@ -709,11 +715,16 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
         void *args[] = {&a};
+        FB_HIP(hipEventRecord(pb.ev0, st));
         FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(grid), dim3(PT), args, 0, st));
+        FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         int32_t ctl[4];
         FB_HIP(hipMemcpyAsync(ctl, pb.ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
         FB_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
+        if (kernelMs) *kernelMs += ms;
         done += ctl[0];
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {

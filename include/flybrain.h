@@ -139,6 +139,7 @@ fb_status fb_synchronize(fb_brain *b);
 fb_status fb_brain_set_stream(fb_brain *b, void *stream);
 /* Engine statistics by name: "fallback_steps" (steps the lookahead engine
  * handed to the per-step engine after a buffer overflow), "persistent_launches",
+ * "persistent_kernel_ms" (summed HIP-event time of those launches),
  * "persistent_supported" (1 if the brain's parameters allow the lookahead
  * engine), "nslots", "serial_deliver".  No counterpart in the reference. */
 fb_status fb_brain_stat(fb_brain *b, const char *name, double *value);
