@@ -123,6 +123,8 @@ def main():
     brain.flush()
     c1 = brain.counters()
     kms = brain.stat("persistent_kernel_ms") - k0
+    phases = {f"{blk}_{ph}": brain.stat(f"phase_us_{blk}_{ph}")
+              for blk in ("drv", "reg") for ph in ("e1", "w1", "e2", "w2", "e3", "w3")}
     launches = int(brain.stat("persistent_launches") - l0)
     fallback = int(brain.stat("fallback_steps") - f0)
     events = c1["edge_events"] - c0["edge_events"]
@@ -193,6 +195,7 @@ def main():
         a1.synchronize()
         cmp[eng] = {"ms_per_step": a0.elapsed_ms(a1) / n2, "steps": n2}
         b2.destroy()
+    out["lookahead_phases_us_total"] = phases
     out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
     if not args.no_cpu:
         cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)

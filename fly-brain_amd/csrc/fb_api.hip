@@ -34,7 +34,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches, double *kernelMs);
+                            int64_t *launches, double *kernelMs, double *phaseUs);
 void persistent_release(const int32_t *key);
 
 // ------------------------------------------------------------------ jump tables
@@ -60,22 +60,43 @@ static void build_jump_tables(std::vector<uint32_t> &tab) {
     }
 }
 
-static std::mutex g_jump_mu;
-static std::map<int, uint32_t *> g_jump_dev;
+// Leapfrog matrices M^1..M^32, then M^(32h) for h = 0..63 (M^0 = identity),
+// each as its 32 columns.
+static void build_leap_matrices(std::vector<uint32_t> &tab) {
+    auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
+    tab.assign((size_t)LEAP_MATS * 32, 0);
+    uint32_t col[32];
+    for (int b = 0; b < 32; b++) col[b] = 1u << b;   // M^0
+    for (int b = 0; b < 32; b++) tab[(size_t)32 * 32 + b] = col[b];
+    for (int p = 1; p <= 32 * 63; p++) {
+        for (int b = 0; b < 32; b++) col[b] = step(col[b]);   // M^p
+        if (p <= 32)
+            for (int b = 0; b < 32; b++) tab[(size_t)(p - 1) * 32 + b] = col[b];
+        if (p % 32 == 0)
+            for (int b = 0; b < 32; b++) tab[(size_t)(32 + p / 32) * 32 + b] = col[b];
+    }
+}
 
-static fb_status jump_table(uint32_t **out) {
+static std::mutex g_jump_mu;
+static std::map<int, std::pair<uint32_t *, uint32_t *>> g_jump_dev;
+
+static fb_status jump_table(uint32_t **out, uint32_t **leap) {
     int dev = 0;
     FB_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_jump_mu);
     auto it = g_jump_dev.find(dev);
-    if (it != g_jump_dev.end()) { *out = it->second; return FB_OK; }
-    std::vector<uint32_t> tab;
+    if (it != g_jump_dev.end()) { *out = it->second.first; *leap = it->second.second; return FB_OK; }
+    std::vector<uint32_t> tab, lp;
     build_jump_tables(tab);
-    uint32_t *p = nullptr;
+    build_leap_matrices(lp);
+    uint32_t *p = nullptr, *q = nullptr;
     FB_HIP(hipMalloc(&p, tab.size() * sizeof(uint32_t)));
     FB_HIP(hipMemcpy(p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    g_jump_dev[dev] = p;
+    FB_HIP(hipMalloc(&q, lp.size() * sizeof(uint32_t)));
+    FB_HIP(hipMemcpy(q, lp.data(), lp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    g_jump_dev[dev] = {p, q};
     *out = p;
+    *leap = q;
     return FB_OK;
 }
 
@@ -155,6 +176,7 @@ struct fb_brain {
     int64_t steps = 0;
     int64_t fallbackSteps = 0, persistentLaunches = 0;
     double persistentMs = 0.0;   // summed HIP-event duration of the lookahead kernel launches
+    double phaseUs[12] = {0};    // lookahead kernel phases (driven block, then a regular block)
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
@@ -418,7 +440,7 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     if ((st = dalloc(b->allocs, &d.blkCnt, (size_t)d.nblocks))) return st;
     if ((st = dalloc(b->allocs, &d.sc, 1))) return st;
     if ((st = dalloc(b->allocs, &b->idxScratch, (size_t)N))) return st;
-    if ((st = jump_table((uint32_t **)&d.jumpTab))) return st;
+    if ((st = jump_table((uint32_t **)&d.jumpTab, (uint32_t **)&d.leap))) return st;
     if ((st = alloc_ring(b))) return st;
     Scalars sc{};
     sc.rng = seed_hash(seed);
@@ -665,7 +687,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
-                                   &b->persistentMs);
+                                   &b->persistentMs, b->phaseUs);
             if (st) return st;
             b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
             b->head = (b->head + nsteps) % b->d.nslots;
@@ -699,6 +721,17 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
     if (n == "fallback_steps") *value = (double)b->fallbackSteps;
     else if (n == "persistent_launches") *value = (double)b->persistentLaunches;
     else if (n == "persistent_kernel_ms") *value = b->persistentMs;
+    else if (n.rfind("phase_us_", 0) == 0) {
+        // phase_us_<drv|reg>_<e1|w1|e2|w2|e3|w3>: time in each phase of the
+        // lookahead kernel (e = work, w = waiting at the grid barrier after it)
+        static const char *ph[6] = {"e1", "w1", "e2", "w2", "e3", "w3"};
+        int idx = -1;
+        for (int blk = 0; blk < 2; blk++)
+            for (int k = 0; k < 6; k++)
+                if (n == std::string("phase_us_") + (blk ? "reg_" : "drv_") + ph[k]) idx = blk * 6 + k;
+        if (idx < 0) { set_error("fb_brain_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
+        *value = b->phaseUs[idx];
+    }
     else if (n == "persistent_supported")
         *value = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? 1.0 : 0.0;
     else if (n == "nslots") *value = (double)b->d.nslots;

@@ -92,6 +92,7 @@ struct DevBrain {
     int32_t *stage, *blkCnt;
     Scalars *sc;
     const uint32_t *jumpTab;   // 32 powers x 4 bytes x 256 entries of xorshift32 jumps
+    const uint32_t *leap;      // LEAP_MATS leapfrog matrices (see leap_fill)
 };
 
 // ------------------------------------------------------------------ device helpers
@@ -117,6 +118,70 @@ __device__ __forceinline__ uint32_t jump(const uint32_t *tab, uint32_t x, uint32
         n &= n - 1;
     }
     return x;
+}
+
+// ------------------------------------------------------------------ parallel stream generation
+//
+// lif.c draws from one serial xorshift32 stream.  xorshift32 is linear over
+// GF(2): one step is a 32x32 bit matrix M, so the state p steps ahead is
+// M^p x.  A workgroup of 1024 threads produces 2048 consecutive states per
+// round: thread t computes positions t+1 and t+1025 as
+//   M^(t%32 + 1) (M^(32 h) x),   h = t/32 and t/32 + 32,
+// from 96 matrices held in LDS: M^1..M^32 (fine) then M^0, M^32, ..., M^2016
+// (coarse).  Each matrix is stored as its 32 columns (images of the unit
+// vectors); a product is 32 unconditional LDS reads masked by the bits of x.
+constexpr int LEAP_MATS = 96;
+constexpr int LEAP_ROUND = 2048;
+
+__device__ __forceinline__ uint32_t matvec(const uint32_t *col, uint32_t x) {
+    uint32_t r = 0;
+#pragma unroll 8
+    for (int b = 0; b < 32; b++) r ^= col[b] & (0u - ((x >> b) & 1u));
+    return r;
+}
+
+// Exclusive scan of one value per thread over a T-thread workgroup (T a
+// multiple of 64, at most 4096): wave shuffles, then one wave over the wave
+// totals.  sh needs 64 + T/64 entries.  Every thread must call it.
+template <int T>
+__device__ inline int32_t wg_exclusive_scan(int32_t v, int32_t *sh, int32_t *total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    if (tid < 64) {
+        int32_t w = tid < T / 64 ? sh[tid] : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(w, off, 64);
+            if (lane >= off) w += y;
+        }
+        if (tid < T / 64) sh[64 + tid] = w;
+    }
+    __syncthreads();
+    const int32_t waveBase = wave ? sh[64 + wave - 1] : 0;
+    *total = sh[64 + T / 64 - 1];
+    __syncthreads();
+    return waveBase + x - v;
+}
+
+// Fills out[0, min(n - r0, 2048)) with the stream states at positions
+// r0+1 .. r0+2048 after `base`, where base is the state at position r0.
+// leapS: the 96 matrices in LDS; ybase: 64 words of LDS.  1024 threads.
+__device__ inline void leap_round(const uint32_t *leapS, uint32_t *ybase, uint32_t *out, int32_t count,
+                                  uint32_t base) {
+    const int tid = threadIdx.x;
+    if (tid < 64) ybase[tid] = matvec(leapS + (32 + tid) * 32, base);
+    __syncthreads();
+    const uint32_t *fine = leapS + (tid & 31) * 32;
+    if (tid < count) out[tid] = matvec(fine, ybase[tid >> 5]);
+    if (tid + 1024 < count) out[tid + 1024] = matvec(fine, ybase[(tid >> 5) + 32]);
+    __syncthreads();
 }
 
 // Membrane update of one neuron (src/wasm/lif.c:57-71): the reference's C

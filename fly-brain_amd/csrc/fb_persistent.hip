@@ -80,7 +80,7 @@ struct PersistArgs {
     float *fwdAdd;
     int32_t fwdCap;
     int32_t *ctl;        // [0] steps done, [1] abort flag, [2] forwarded count
-    const uint32_t *leap;    // [64][32]: columns of M^(l+1), l < 32, then of M^(32h), h < 32
+    uint64_t *prof;      // per-phase clock ticks of blocks 0 and 1 (12 counters)
 };
 
 struct Shared {
@@ -89,9 +89,10 @@ struct Shared {
     uint32_t hasEv[D_MAX * WORDS_PER_STEP];
     uint32_t arr[D_MAX * WORDS_PER_STEP];
     uint32_t draws[DRV_CAP];
-    uint32_t leap[64 * 32];
-    uint32_t ybase[32];
-    int32_t scan[PT];
+    uint32_t leap[LEAP_MATS * 32];
+    uint32_t ybase[64];
+    int32_t scan[64 + PT / 64];
+    uint64_t ph[7];      // phase clocks of thread 0: six accumulators and the last stamp
     int32_t nev;
     int32_t pref[D_MAX + 1];
     uint32_t rngBase;
@@ -105,33 +106,6 @@ __device__ __forceinline__ uint32_t lower_bound_target(const uint2 *edges, uint3
         if (edges[mid].x < v) a = mid + 1; else e = mid;
     }
     return a;
-}
-
-// exclusive workgroup scan of one value per thread (PT threads)
-__device__ inline int32_t wg_scan(int32_t v, int32_t *sh, int32_t *total) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) sh[wave] = x;
-    __syncthreads();
-    if (tid < 64) {
-        int32_t w = tid < PT / 64 ? sh[tid] : 0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int32_t y = __shfl_up(w, off, 64);
-            if (lane >= off) w += y;
-        }
-        if (tid < PT / 64) sh[64 + tid] = w;
-    }
-    __syncthreads();
-    const int32_t waveBase = wave ? sh[64 + wave - 1] : 0;
-    *total = sh[64 + PT / 64 - 1];
-    __syncthreads();
-    return waveBase + x - v;
 }
 
 // LDS bitonic sort of evKey/evAdd[0, n), padded to a power of two with ~0 keys
@@ -345,14 +319,6 @@ __device__ void e2_regular(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], 
     (void)done;
 }
 
-__device__ __forceinline__ uint32_t matvec(const uint32_t *col, uint32_t x) {
-    uint32_t r = 0;
-#pragma unroll
-    for (int b = 0; b < 32; b++)
-        if ((x >> b) & 1u) r ^= col[b];
-    return r;
-}
-
 __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], float (&pFire)[DPT], int32_t L,
                           uint32_t &rng) {
     const DevBrain &d = a.d;
@@ -390,19 +356,12 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
         }
         // lif.c phase 2: one draw per non-refractory driven neuron, in list order
         int32_t n;
-        const int32_t off = wg_scan(draws, s.scan, &n);
-        uint32_t base = rng;
-        for (int32_t r0 = 0; r0 < n; r0 += PT) {
-            if (tid < 32) s.ybase[tid] = matvec(s.leap + (32 + tid) * 32, base);   // M^(32 tid) base
-            __syncthreads();
-            // stream position r0 + tid + 1
-            const uint32_t x = matvec(s.leap + (tid & 31) * 32, s.ybase[tid >> 5]);
-            if (r0 + tid < n) s.draws[r0 + tid] = x;
-            __syncthreads();
-            base = __builtin_amdgcn_readfirstlane(s.draws[min(n, r0 + PT) - 1]);
-            __syncthreads();
+        const int32_t off = wg_exclusive_scan<PT>(draws, s.scan, &n);
+        // n <= 2048 driven neurons: one leapfrog round covers the step
+        if (n > 0) {
+            leap_round(s.leap, s.ybase, s.draws, n, rng);
+            rng = s.draws[n - 1];
         }
-        rng = base;
         int32_t pos = off;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
@@ -455,7 +414,7 @@ __device__ void e3_list(const PersistArgs &a, Shared &s, int32_t done, int32_t m
     int32_t nf = 0;
     for (int32_t q = dlo; q < dhi; q++) nf += fl[q];
     int32_t ndTot;
-    int32_t pos = wg_scan(nf, s.scan, &ndTot);
+    int32_t pos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
     for (int32_t q = dlo; q < dhi; q++)
         if (fl[q]) out[pos++] = d.driven[q];
     // threshold spikes in ascending index
@@ -465,7 +424,7 @@ __device__ void e3_list(const PersistArgs &a, Shared &s, int32_t done, int32_t m
     int32_t cnt = 0;
     for (int32_t w = wlo; w < whi; w++) cnt += __popcll(bits[w]);
     int32_t thTot;
-    pos = ndTot + wg_scan(cnt, s.scan, &thTot);
+    pos = ndTot + wg_exclusive_scan<PT>(cnt, s.scan, &thTot);
     for (int32_t w = wlo; w < whi; w++) {
         uint64_t b = bits[w];
         if (!b) continue;
@@ -505,7 +464,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             load_cell(d, cells[u], has ? d.driven[q] : -1);
             pFire[u] = has ? d.drive[cells[u].i] * d.k.dtS : 0.f;
         }
-        for (int32_t q = tid; q < 64 * 32; q += PT) s.leap[q] = a.leap[q];
+        for (int32_t q = tid; q < LEAP_MATS * 32; q += PT) s.leap[q] = d.leap[q];
         rng = d.sc->rng;
     } else {
 #pragma unroll
@@ -515,13 +474,27 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         }
     }
 
+    // per-phase clock ticks, kept by thread 0 of blocks 0 and 1
+    const bool timed = tid == 0 && blockIdx.x < 2;
+    if (timed) {
+        for (int k = 0; k < 6; k++) s.ph[k] = 0;
+        s.ph[6] = wall_clock64();
+    }
+#define FB_PHASE(k)                                    \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.ph[k] += now - s.ph[6];                      \
+        s.ph[6] = now;                                 \
+    }
     int32_t done = 0;
     while (done < a.nsteps) {
         const int32_t L = min(a.L, a.nsteps - done);
         unsigned long long ev = 0, syn = 0;
         if (drivenBlock) e1_driven(a, s, done, L, ev, syn);
         else e1_regular(a, s, done, L, lo, hi);
+        FB_PHASE(0)
         grid.sync();
+        FB_PHASE(1)
         // every block reads the same flags after the barrier
         if (a.ctl[1] || a.ctl[2] > EV_CAP) break;
         if (ev) atomicAdd(&d.sc->edgeEvents, ev);
@@ -536,16 +509,23 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
 #pragma unroll
             for (int u = 0; u < NPT; u++) store_cell(d, cells[u]);
         }
+        FB_PHASE(2)
         grid.sync();
+        FB_PHASE(3)
         if ((int32_t)blockIdx.x < L) e3_list(a, s, done, (int32_t)blockIdx.x);
         if (blockIdx.x == gridDim.x - 1 && tid == 0) {
             // the slot delivered at the last step of the epoch is empty now
             d.ringCount[ring_slot(a, done, L - 1)] = 0;
             a.ctl[2] = 0;
         }
+        FB_PHASE(4)
         grid.sync();
+        FB_PHASE(5)
         done += L;
     }
+#undef FB_PHASE
+    if (timed)
+        for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[blockIdx.x * 6 + k], (unsigned long long)s.ph[k]);
     if (blockIdx.x == 0 && tid == 0) a.ctl[0] = done;
 }
 
@@ -554,7 +534,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
 struct PersistState {
     int device = -1;
     int maxBlocks = 0;
-    uint32_t *leap = nullptr;
+    double tickHz = 100e6;
 };
 static PersistState g_ps;
 
@@ -571,22 +551,9 @@ static fb_status persist_init() {
         return FB_ERR_UNSUPPORTED;
     }
     g_ps.maxBlocks = perCU * prop.multiProcessorCount;
-    // leapfrog matrices: M^(l+1) for l < 32, then M^(32h) for h < 32
-    std::vector<uint32_t> leapCols((size_t)64 * 32);
-    auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
-    uint32_t col[32];
-    for (int b = 0; b < 32; b++) col[b] = 1u << b;          // M^0
-    for (int t = 1; t <= 32 * 31; t++) {
-        for (int b = 0; b < 32; b++) col[b] = step(col[b]);  // M^t
-        if (t <= 32)
-            for (int b = 0; b < 32; b++) leapCols[(size_t)(t - 1) * 32 + b] = col[b];
-        if (t % 32 == 0)
-            for (int b = 0; b < 32; b++) leapCols[(size_t)(32 + t / 32) * 32 + b] = col[b];
-    }
-    for (int b = 0; b < 32; b++) leapCols[(size_t)32 * 32 + b] = 1u << b;   // M^0 for h = 0
-    if (g_ps.leap) hipFree(g_ps.leap);
-    FB_HIP(hipMalloc(&g_ps.leap, leapCols.size() * sizeof(uint32_t)));
-    FB_HIP(hipMemcpy(g_ps.leap, leapCols.data(), leapCols.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+        g_ps.tickHz = (double)khz * 1000.0;
     g_ps.device = dev;
     return FB_OK;
 }
@@ -626,6 +593,7 @@ struct PersistBuffers {
     uint64_t *fwdKey = nullptr;
     float *fwdAdd = nullptr;
     int32_t *ctl = nullptr;
+    uint64_t *prof = nullptr;
     int32_t N = 0, nDrivenCap = 0, fwdCap = 0;
     int32_t drivenVersion = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -640,7 +608,7 @@ void persistent_release(const int32_t *key) {
     auto it = g_pbuf.find(key);
     if (it == g_pbuf.end()) return;
     PersistBuffers &pb = it->second;
-    if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); }
+    if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); hipFree(pb.prof); }
     if (pb.ev0) { hipEventDestroy(pb.ev0); hipEventDestroy(pb.ev1); }
     g_pbuf.erase(it);
 }
@@ -655,14 +623,14 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
 }
 
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches, double *kernelMs) {
+                            int64_t *launches, double *kernelMs, double *phaseUs) {
     std::string why;
     fb_status rc = persistent_supported(d, false, &why);
     if (rc) { set_error("persistent engine: %s", why.c_str()); return rc; }
     PersistBuffers &pb = g_pbuf[d.driven];
     const int32_t nwords = (d.N + 63) / 64;
     if (pb.N != d.N) {
-        if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); }
+        if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); hipFree(pb.prof); }
         pb.N = d.N;
         pb.fwdCap = 1 << 16;
         FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * (size_t)D_MAX * nwords));
@@ -672,6 +640,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.fwdKey, sizeof(uint64_t) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.fwdAdd, sizeof(float) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
+        FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 12));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
@@ -707,13 +676,14 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.fwdAdd = pb.fwdAdd;
     a.fwdCap = pb.fwdCap;
     a.ctl = pb.ctl;
-    a.leap = g_ps.leap;
+    a.prof = pb.prof;
 
     int32_t done = 0, cur = head;
     while (done < nsteps) {
         a.head0 = cur;
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
+        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 12, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(grid), dim3(PT), args, 0, st));
@@ -725,6 +695,11 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         float ms = 0.f;
         FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
         if (kernelMs) *kernelMs += ms;
+        if (phaseUs) {
+            uint64_t prof[12];
+            FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
+            for (int k = 0; k < 12; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
+        }
         done += ctl[0];
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {

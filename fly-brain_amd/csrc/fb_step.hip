@@ -9,8 +9,8 @@
 //   k_driven    one workgroup: background draws and forced spikes.  lif.c
 //               draws from one serial xorshift32 stream; each thread counts the
 //               non-refractory driven neurons of its slice, a workgroup scan
-//               gives each slice its position in the stream, and the state is
-//               jumped there with precomputed GF(2) matrix powers.
+//               gives each slice its position in the stream, and the stream is
+//               produced 2048 states per round by GF(2) leapfrog matrices.
 //   k_membrane  dense pass: apply the step's arrivals (single arrival: one
 //               add; several: the ordered sum), background hits, the membrane
 //               and decay update, the threshold test, and wave-ballot
@@ -103,30 +103,17 @@ __global__ void __launch_bounds__(1024) k_deliver_serial(DevBrain d, int32_t hea
 
 // ------------------------------------------------------------------ driven
 
-template <int T>
-__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t *sh, int32_t *total) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        int32_t add = tid >= off ? sh[tid - off] : 0;
-        __syncthreads();
-        sh[tid] += add;
-        __syncthreads();
-    }
-    const int32_t incl = sh[tid];
-    *total = sh[T - 1];
-    __syncthreads();
-    return incl - v;
-}
-
 __global__ void __launch_bounds__(1024) k_driven(DevBrain d, int32_t slot) {
     constexpr int T = 1024;
-    __shared__ int32_t scan[T];
+    __shared__ uint32_t leapS[LEAP_MATS * 32];
+    __shared__ uint32_t ybase[64];
+    __shared__ uint32_t draws[LEAP_ROUND];
+    __shared__ int32_t scan[64 + T / 64];
     __shared__ uint32_t s_rng;
     __shared__ int32_t s_nbg;
     const int tid = threadIdx.x;
     const int32_t N = d.N;
+    for (int q = tid; q < LEAP_MATS * 32; q += T) leapS[q] = d.leap[q];
     if (tid == 0) {
         // lif.c:47 -- the fractional background accumulator
         float acc = d.sc->bgAcc + d.k.bgEvents;
@@ -137,54 +124,61 @@ __global__ void __launch_bounds__(1024) k_driven(DevBrain d, int32_t slot) {
         s_rng = d.sc->rng;
     }
     __syncthreads();
-    const uint32_t rng0 = s_rng;
+    uint32_t base = s_rng;
     const int32_t nbg = s_nbg;
-    // lif.c:48 -- draw t of the background burst is the stream's (t+1)-th state
-    for (int32_t t = tid; t < nbg; t += T) {
-        const uint32_t x = jump(d.jumpTab, rng0, (uint32_t)t + 1u);
-        atomicAdd(&d.bgCnt[x % (uint32_t)N], 1);
+    // lif.c:48 -- background draw t is the stream's state at position t+1
+    for (int32_t r0 = 0; r0 < nbg; r0 += LEAP_ROUND) {
+        const int32_t cnt = min(LEAP_ROUND, nbg - r0);
+        leap_round(leapS, ybase, draws, cnt, base);
+        for (int32_t t = tid; t < cnt; t += T) atomicAdd(&d.bgCnt[draws[t] % (uint32_t)N], 1);
+        base = draws[cnt - 1];
+        __syncthreads();
     }
-    const uint32_t rng1 = nbg ? jump(d.jumpTab, rng0, (uint32_t)nbg) : rng0;
 
     // lif.c:51-52 -- one draw per non-refractory driven neuron, in list order
     const int32_t nD = d.nDriven;
     const int32_t per = (nD + T - 1) / T;
     const int32_t lo = min(nD, tid * per), hi = min(nD, lo + per);
-    int32_t draws = 0;
+    int32_t mine = 0;
     for (int32_t q = lo; q < hi; q++)
-        if (!(d.refr[d.driven[q]] > 0)) draws++;
+        if (!(d.refr[d.driven[q]] > 0)) mine++;
     int32_t totalDraws;
-    const int32_t off = block_exclusive_scan<T>(draws, scan, &totalDraws);
-    uint32_t x = jump(d.jumpTab, rng1, (uint32_t)off);
+    const int32_t off = wg_exclusive_scan<T>(mine, scan, &totalDraws);
     const float forcedRefr = d.k.tRef + d.k.dt;
-    int32_t fired = 0;
     int32_t *flag = d.stage;   // stage[] is free until k_membrane; one flag per list slot
-    for (int32_t q = lo; q < hi; q++) {
-        const int32_t i = d.driven[q];
-        int32_t f = 0;
-        if (!(d.refr[i] > 0)) {
-            const float pFire = d.drive[i] * d.k.dtS;
-            x = xs32(x);
-            const float u = (float)(x >> 8) * (1.0f / 16777216.0f);
-            if (u < pFire) {
-                d.v[i] = d.k.vReset;
-                d.refr[i] = forcedRefr;
-                d.spikeCount[i] += 1u;
-                d.trace[i] = 1.f;
-                d.adapt[i] = d.adapt[i] + d.k.adaptInc;
-                f = 1;
-                fired++;
+    int32_t q = lo, next = off, fired = 0;
+    for (int32_t q2 = lo; q2 < hi; q2++) flag[q2] = 0;
+    for (int32_t r0 = 0; r0 < totalDraws; r0 += LEAP_ROUND) {
+        const int32_t cnt = min(LEAP_ROUND, totalDraws - r0);
+        leap_round(leapS, ybase, draws, cnt, base);
+        while (next < off + mine && next < r0 + cnt) {
+            const int32_t i = d.driven[q];
+            if (!(d.refr[i] > 0)) {
+                const float pFire = d.drive[i] * d.k.dtS;
+                const float u = (float)(draws[next - r0] >> 8) * (1.0f / 16777216.0f);
+                if (u < pFire) {
+                    d.v[i] = d.k.vReset;
+                    d.refr[i] = forcedRefr;
+                    d.spikeCount[i] += 1u;
+                    d.trace[i] = 1.f;
+                    d.adapt[i] = d.adapt[i] + d.k.adaptInc;
+                    flag[q] = 1;
+                    fired++;
+                }
+                next++;
             }
+            q++;
         }
-        flag[q] = f;
+        base = draws[cnt - 1];
+        __syncthreads();
     }
     int32_t totalFired;
-    int32_t pos = block_exclusive_scan<T>(fired, scan, &totalFired);
+    int32_t pos = wg_exclusive_scan<T>(fired, scan, &totalFired);
     int32_t *out = d.ring + (int64_t)slot * d.ringStride;
-    for (int32_t q = lo; q < hi; q++)
-        if (flag[q]) out[pos++] = d.driven[q];
+    for (int32_t q2 = lo; q2 < hi; q2++)
+        if (flag[q2]) out[pos++] = d.driven[q2];
     if (tid == 0) {
-        d.sc->rng = jump(d.jumpTab, rng1, (uint32_t)totalDraws);
+        d.sc->rng = base;
         d.sc->nDrivenFired = totalFired;
     }
 }
