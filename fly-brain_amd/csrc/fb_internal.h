@@ -130,8 +130,18 @@ __device__ __forceinline__ uint32_t jump(const uint32_t *tab, uint32_t x, uint32
 // from 96 matrices held in LDS: M^1..M^32 (fine) then M^0, M^32, ..., M^2016
 // (coarse).  Each matrix is stored as its 32 columns (images of the unit
 // vectors); a product is 32 unconditional LDS reads masked by the bits of x.
+// In LDS the matrices sit LEAP_STRIDE = 33 words apart, so the 32 lanes that
+// read column b of 32 different matrices hit 32 different banks.
 constexpr int LEAP_MATS = 96;
 constexpr int LEAP_ROUND = 2048;
+constexpr int LEAP_STRIDE = 33;
+constexpr int LEAP_LDS_WORDS = LEAP_MATS * LEAP_STRIDE;
+
+// global layout (stride 32) -> LDS layout (stride 33); every thread of the block calls it
+template <int T>
+__device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap) {
+    for (int q = threadIdx.x; q < LEAP_MATS * 32; q += T) leapS[(q >> 5) * LEAP_STRIDE + (q & 31)] = leap[q];
+}
 
 __device__ __forceinline__ uint32_t matvec(const uint32_t *col, uint32_t x) {
     uint32_t r = 0;
@@ -176,9 +186,9 @@ __device__ inline int32_t wg_exclusive_scan(int32_t v, int32_t *sh, int32_t *tot
 __device__ inline void leap_round(const uint32_t *leapS, uint32_t *ybase, uint32_t *out, int32_t count,
                                   uint32_t base) {
     const int tid = threadIdx.x;
-    if (tid < 64) ybase[tid] = matvec(leapS + (32 + tid) * 32, base);
+    if (tid < 64) ybase[tid] = matvec(leapS + (32 + tid) * LEAP_STRIDE, base);
     __syncthreads();
-    const uint32_t *fine = leapS + (tid & 31) * 32;
+    const uint32_t *fine = leapS + (tid & 31) * LEAP_STRIDE;
     if (tid < count) out[tid] = matvec(fine, ybase[tid >> 5]);
     if (tid + 1024 < count) out[tid + 1024] = matvec(fine, ybase[(tid >> 5) + 32]);
     __syncthreads();

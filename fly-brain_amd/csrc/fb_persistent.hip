@@ -89,7 +89,7 @@ struct Shared {
     uint32_t hasEv[D_MAX * WORDS_PER_STEP];
     uint32_t arr[D_MAX * WORDS_PER_STEP];
     uint32_t draws[DRV_CAP];
-    uint32_t leap[LEAP_MATS * 32];
+    uint32_t leap[LEAP_LDS_WORDS];
     uint32_t ybase[64];
     int32_t scan[64 + PT / 64];
     uint64_t ph[7];      // phase clocks of thread 0: six accumulators and the last stamp
@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             load_cell(d, cells[u], has ? d.driven[q] : -1);
             pFire[u] = has ? d.drive[cells[u].i] * d.k.dtS : 0.f;
         }
-        for (int32_t q = tid; q < LEAP_MATS * 32; q += PT) s.leap[q] = d.leap[q];
+        leap_load<PT>(s.leap, d.leap);
         rng = d.sc->rng;
     } else {
 #pragma unroll

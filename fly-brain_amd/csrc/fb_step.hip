@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(1024) k_deliver_serial(DevBrain d, int32_t hea
 
 __global__ void __launch_bounds__(1024) k_driven(DevBrain d, int32_t slot) {
     constexpr int T = 1024;
-    __shared__ uint32_t leapS[LEAP_MATS * 32];
+    __shared__ uint32_t leapS[LEAP_LDS_WORDS];
     __shared__ uint32_t ybase[64];
     __shared__ uint32_t draws[LEAP_ROUND];
     __shared__ int32_t scan[64 + T / 64];
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(1024) k_driven(DevBrain d, int32_t slot) {
     __shared__ int32_t s_nbg;
     const int tid = threadIdx.x;
     const int32_t N = d.N;
-    for (int q = tid; q < LEAP_MATS * 32; q += T) leapS[q] = d.leap[q];
+    leap_load<T>(leapS, d.leap);
     if (tid == 0) {
         // lif.c:47 -- the fractional background accumulator
         float acc = d.sc->bgAcc + d.k.bgEvents;
