@@ -125,6 +125,8 @@ def main():
     kms = brain.stat("persistent_kernel_ms") - k0
     phases = {f"{blk}_{ph}": brain.stat(f"phase_us_{blk}_{ph}")
               for blk in ("drv", "reg") for ph in ("e1", "w1", "e2", "w2", "e3", "w3")}
+    phases.update({f"drvstep_{ph}": brain.stat(f"phase_us_drvstep_{ph}")
+                   for ph in ("inputs", "scan", "draws", "decide", "membrane", "sync")})
     launches = int(brain.stat("persistent_launches") - l0)
     fallback = int(brain.stat("fallback_steps") - f0)
     events = c1["edge_events"] - c0["edge_events"]

@@ -176,7 +176,7 @@ struct fb_brain {
     int64_t steps = 0;
     int64_t fallbackSteps = 0, persistentLaunches = 0;
     double persistentMs = 0.0;   // summed HIP-event duration of the lookahead kernel launches
-    double phaseUs[12] = {0};    // lookahead kernel phases (driven block, then a regular block)
+    double phaseUs[18] = {0};    // lookahead kernel phases (driven block, a regular block, driven-step parts)
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
@@ -729,6 +729,9 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
         for (int blk = 0; blk < 2; blk++)
             for (int k = 0; k < 6; k++)
                 if (n == std::string("phase_us_") + (blk ? "reg_" : "drv_") + ph[k]) idx = blk * 6 + k;
+        static const char *sub[6] = {"inputs", "scan", "draws", "decide", "membrane", "sync"};
+        for (int k = 0; k < 6; k++)
+            if (n == std::string("phase_us_drvstep_") + sub[k]) idx = 12 + k;
         if (idx < 0) { set_error("fb_brain_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
         *value = b->phaseUs[idx];
     }

@@ -80,7 +80,7 @@ struct PersistArgs {
     float *fwdAdd;
     int32_t fwdCap;
     int32_t *ctl;        // [0] steps done, [1] abort flag, [2] forwarded count
-    uint64_t *prof;      // per-phase clock ticks of blocks 0 and 1 (12 counters)
+    uint64_t *prof;      // per-phase clock ticks of blocks 0 and 1 (12), driven-block step parts (6)
 };
 
 struct Shared {
@@ -93,6 +93,7 @@ struct Shared {
     uint32_t ybase[64];
     int32_t scan[64 + PT / 64];
     uint64_t ph[7];      // phase clocks of thread 0: six accumulators and the last stamp
+    uint64_t sub[7];     // driven-block step clocks: five accumulators and the last stamp
     int32_t nev;
     int32_t pref[D_MAX + 1];
     uint32_t rngBase;
@@ -342,6 +343,14 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
     if (nev > 1) lds_sort(s, nev);
     __syncthreads();
     const float forcedRefr = d.k.tRef + d.k.dt;
+    const bool timed = tid == 0;
+#define FB_SUB(k)                                      \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.sub[k] += now - s.sub[6];                    \
+        s.sub[6] = now;                                \
+    }
+    if (timed) s.sub[6] = wall_clock64();
     for (int32_t m = 0; m < L; m++) {
         // lif.c phase 1: arrivals for the driven neurons
         int32_t draws = 0;
@@ -354,14 +363,17 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
                 if (!(c.st.refr > 0)) { nonRefr |= 1u << u; draws++; }
             }
         }
+        FB_SUB(0)
         // lif.c phase 2: one draw per non-refractory driven neuron, in list order
         int32_t n;
         const int32_t off = wg_exclusive_scan<PT>(draws, s.scan, &n);
+        FB_SUB(1)
         // n <= 2048 driven neurons: one leapfrog round covers the step
         if (n > 0) {
             leap_round(s.leap, s.ybase, s.draws, n, rng);
             rng = s.draws[n - 1];
         }
+        FB_SUB(2)
         int32_t pos = off;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
@@ -381,6 +393,7 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
             }
             if (u < P && tid * P + u < nD) a.dflag[(int64_t)m * nD + tid * P + u] = f;
         }
+        FB_SUB(3)
         // lif.c phases 3-4 for the driven neurons
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
@@ -395,8 +408,11 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
                 }
             }
         }
+        FB_SUB(4)
         __syncthreads();   // s.draws is rewritten next step
+        FB_SUB(5)
     }
+#undef FB_SUB
 }
 
 // ------------------------------------------------------------------ E3: spike lists in lif.c order
@@ -477,7 +493,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     // per-phase clock ticks, kept by thread 0 of blocks 0 and 1
     const bool timed = tid == 0 && blockIdx.x < 2;
     if (timed) {
-        for (int k = 0; k < 6; k++) s.ph[k] = 0;
+        for (int k = 0; k < 6; k++) { s.ph[k] = 0; s.sub[k] = 0; }
         s.ph[6] = wall_clock64();
     }
 #define FB_PHASE(k)                                    \
@@ -524,8 +540,11 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         done += L;
     }
 #undef FB_PHASE
-    if (timed)
+    if (timed) {
         for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[blockIdx.x * 6 + k], (unsigned long long)s.ph[k]);
+        if (blockIdx.x == 0)
+            for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[12 + k], (unsigned long long)s.sub[k]);
+    }
     if (blockIdx.x == 0 && tid == 0) a.ctl[0] = done;
 }
 
@@ -640,7 +659,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.fwdKey, sizeof(uint64_t) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.fwdAdd, sizeof(float) * (size_t)pb.fwdCap));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
-        FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 12));
+        FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 18));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
@@ -683,7 +702,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.head0 = cur;
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
-        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 12, st));
+        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 18, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(grid), dim3(PT), args, 0, st));
@@ -696,9 +715,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
         if (kernelMs) *kernelMs += ms;
         if (phaseUs) {
-            uint64_t prof[12];
+            uint64_t prof[18];
             FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
-            for (int k = 0; k < 12; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
+            for (int k = 0; k < 18; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
         }
         done += ctl[0];
         cur = (cur + ctl[0]) % d.nslots;
