@@ -61,19 +61,28 @@ static void build_jump_tables(std::vector<uint32_t> &tab) {
 }
 
 // Leapfrog matrices M^1..M^32, then M^(32h) for h = 0..63 (M^0 = identity),
-// each as its 32 columns.
+// each as 8 nibble tables: entry [k][v] is the XOR of the columns 4k..4k+3
+// selected by the bits of v.
+static void put_nibbles(uint32_t *dst, const uint32_t *col) {
+    for (int k = 0; k < 8; k++)
+        for (int v = 0; v < 16; v++) {
+            uint32_t r = 0;
+            for (int i = 0; i < 4; i++)
+                if ((v >> i) & 1) r ^= col[4 * k + i];
+            dst[k * 16 + v] = r;
+        }
+}
+
 static void build_leap_matrices(std::vector<uint32_t> &tab) {
     auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
-    tab.assign((size_t)LEAP_MATS * 32, 0);
+    tab.assign((size_t)LEAP_MATS * LEAP_WORDS, 0);
     uint32_t col[32];
     for (int b = 0; b < 32; b++) col[b] = 1u << b;   // M^0
-    for (int b = 0; b < 32; b++) tab[(size_t)32 * 32 + b] = col[b];
+    put_nibbles(&tab[(size_t)32 * LEAP_WORDS], col);
     for (int p = 1; p <= 32 * 63; p++) {
         for (int b = 0; b < 32; b++) col[b] = step(col[b]);   // M^p
-        if (p <= 32)
-            for (int b = 0; b < 32; b++) tab[(size_t)(p - 1) * 32 + b] = col[b];
-        if (p % 32 == 0)
-            for (int b = 0; b < 32; b++) tab[(size_t)(32 + p / 32) * 32 + b] = col[b];
+        if (p <= 32) put_nibbles(&tab[(size_t)(p - 1) * LEAP_WORDS], col);
+        if (p % 32 == 0) put_nibbles(&tab[(size_t)(32 + p / 32) * LEAP_WORDS], col);
     }
 }
 

@@ -127,27 +127,51 @@ __device__ __forceinline__ uint32_t jump(const uint32_t *tab, uint32_t x, uint32
 // M^p x.  A workgroup of 1024 threads produces 2048 consecutive states per
 // round: thread t computes positions t+1 and t+1025 as
 //   M^(t%32 + 1) (M^(32 h) x),   h = t/32 and t/32 + 32,
-// from 96 matrices held in LDS: M^1..M^32 (fine) then M^0, M^32, ..., M^2016
-// (coarse).  Each matrix is stored as its 32 columns (images of the unit
-// vectors); a product is 32 unconditional LDS reads masked by the bits of x.
-// In LDS the matrices sit LEAP_STRIDE = 33 words apart, so the 32 lanes that
-// read column b of 32 different matrices hit 32 different banks.
+// from 96 matrices: M^1..M^32 (fine) then M^0, M^32, ..., M^2016 (coarse).
+// A matrix is kept as 8 nibble tables (16 words each: the XOR of the columns
+// selected by a 4-bit slice of x), so a product is 8 LDS reads.  In LDS the
+// matrices sit LEAP_STRIDE = 129 words apart, so lanes that read the same
+// (nibble, value) entry of 32 different matrices hit 32 different banks.
 constexpr int LEAP_MATS = 96;
+constexpr int LEAP_WORDS = 128;           // 8 nibbles x 16 values
 constexpr int LEAP_ROUND = 2048;
-constexpr int LEAP_STRIDE = 33;
+constexpr int LEAP_STRIDE = 129;
 constexpr int LEAP_LDS_WORDS = LEAP_MATS * LEAP_STRIDE;
 
-// global layout (stride 32) -> LDS layout (stride 33); every thread of the block calls it
-template <int T>
-__device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap) {
-    for (int q = threadIdx.x; q < LEAP_MATS * 32; q += T) leapS[(q >> 5) * LEAP_STRIDE + (q & 31)] = leap[q];
+__device__ __forceinline__ uint32_t matvec(const uint32_t *tab, uint32_t x) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) r ^= tab[k * 16 + ((x >> (4 * k)) & 15u)];
+    return r;
 }
 
-__device__ __forceinline__ uint32_t matvec(const uint32_t *col, uint32_t x) {
-    uint32_t r = 0;
-#pragma unroll 8
-    for (int b = 0; b < 32; b++) r ^= col[b] & (0u - ((x >> b) & 1u));
-    return r;
+// global layout (stride 128) -> LDS layout (stride 129); every thread of the block calls it
+template <int T>
+__device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap) {
+    for (int q = threadIdx.x; q < LEAP_MATS * LEAP_WORDS; q += T) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = leap[q];
+}
+
+// Exclusive scan over a T-thread workgroup of per-thread counts given as two
+// flags (each 0 or 1): ballots and popcounts inside a wave, wave totals
+// through LDS.  sh needs T/64 entries; one barrier.  Every thread must call it,
+// and sh must not be rewritten before every thread has returned.
+template <int T>
+__device__ __forceinline__ int32_t wg_scan_flags2(bool f0, bool f1, int32_t *sh, int32_t *total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t m0 = __ballot(f0), m1 = __ballot(f1);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int32_t inWave = __popcll(m0 & lt) + __popcll(m1 & lt);
+    if (lane == 0) sh[wave] = __popcll(m0) + __popcll(m1);
+    __syncthreads();
+    int32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < T / 64; w++) {
+        const int32_t c = sh[w];
+        before += w < wave ? c : 0;
+        all += c;
+    }
+    *total = all;
+    return before + inWave;
 }
 
 // Exclusive scan of one value per thread over a T-thread workgroup (T a

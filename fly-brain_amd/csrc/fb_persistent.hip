@@ -55,7 +55,7 @@ constexpr int NPT = 2;              // neurons per thread in a regular block
 constexpr int NB_MAX = PT * NPT;    // neurons per regular block
 constexpr int D_MAX = 32;           // epoch length bound
 constexpr int EV_CAP = 4096;        // LDS events per block per epoch
-constexpr int DPT = 2;              // driven neurons per thread in block 0
+constexpr int DPT = 2;              // driven neurons per thread in block 0 (wg_scan_flags2 takes two)
 constexpr int DRV_CAP = PT * DPT;
 constexpr int WORDS_PER_STEP = NB_MAX / 32;
 
@@ -363,10 +363,11 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
                 if (!(c.st.refr > 0)) { nonRefr |= 1u << u; draws++; }
             }
         }
+        (void)draws;
         FB_SUB(0)
         // lif.c phase 2: one draw per non-refractory driven neuron, in list order
         int32_t n;
-        const int32_t off = wg_exclusive_scan<PT>(draws, s.scan, &n);
+        const int32_t off = wg_scan_flags2<PT>(nonRefr & 1u, (nonRefr >> 1) & 1u, s.scan + (m & 1) * 16, &n);
         FB_SUB(1)
         // n <= 2048 driven neurons: one leapfrog round covers the step
         if (n > 0) {
