@@ -2,43 +2,45 @@
 // many LIF steps, with the delay line as a lookahead window.
 //
 // A spike emitted at step t is delivered at step t + D (D = nslots - 1, 18 at
-// dt 0.1 ms).  So at the start of an epoch [t0, t0+L) with L <= D, every
-// arrival of the epoch is already in the ring, and each neuron evolves
-// independently of every other neuron until the epoch ends -- except for the
-// forced spikes, which draw from one serial xorshift32 stream in driven-list
-// order.  The launch therefore splits the grid into:
+// dt 0.1 ms).  The launch advances in epochs of D steps.  Every arrival of
+// epoch e+1 is a spike of epoch e, so while the grid simulates epoch e no
+// neuron needs anything from another workgroup -- except the forced spikes,
+// which draw from one serial xorshift32 stream in driven-list order.  Roles:
 //
 //   block 0         the driven block: owns every driven neuron (state in
-//                   registers, list order across threads), draws the stream
-//                   per step with a workgroup scan and a two-level leapfrog:
-//                   draw t+1 of a step is M^(t%32+1) M^(32*(t/32)) x, where M is
-//                   one xorshift32 step as a 32x32 matrix over GF(2) and the 64
-//                   matrices sit in LDS.
-//   blocks 1..G-1   regular blocks: block b owns a contiguous range of up to
-//                   2048 neurons (2 per thread, state in registers).
+//                   registers, list order across threads).  Per step: a ballot
+//                   scan gives each thread its position in the stream; 64
+//                   threads compute M^(32h) x for the step's coarse offsets
+//                   and every thread finishes its own draws with one
+//                   nibble-table product and xorshift steps (M = one
+//                   xorshift32 step as a matrix over GF(2)).
+//   blocks 1..R     regular blocks: block b owns a contiguous, 64-aligned
+//                   range of up to 2048 neurons (2 per thread, in registers).
+//   blocks R+1..    spare blocks (when the grid has them): they turn the
+//                   previous epoch's spikes into ring lists in lif.c order.
 //
-// Per epoch, three grid barriers:
-//   E1  each regular block binary-searches every arriving row for the slice
-//       of targets in its range (rows are sorted by target) and keeps those
-//       edge events in LDS, sorted by (step, sign class, target, arrival
-//       position, position in row) -- lif.c's addition order per target.
-//       Events for driven targets are forwarded to the driven block.
-//   E2  every block advances its neurons L steps from registers: arrivals
-//       (one add, or the sorted run for several), forced spikes, membrane,
-//       threshold.  Threshold spikes go to a per-step bitmask (wave ballots),
-//       forced spikes to per-step flags in list order.
-//   E3  block m materialises step m's spike list into its ring slot in
-//       lif.c's order (forced, then ascending) by a workgroup scan.
+// One grid barrier per epoch.  Within epoch e every owning block:
+//   gather  loads the edge events addressed to it (pushed during epoch e-1)
+//           into LDS and sorts them by (step, sign class, target, arrival
+//           key, position in row) -- lif.c's addition order per target;
+//   simulate  D steps from registers: arrivals (one add, or the sorted run
+//           for several), forced spikes, membrane, threshold.  Threshold
+//           spikes go to a per-step bitmask, forced spikes to per-step flags;
+//   push    walks the rows of its own spikes and writes their events into
+//           per-(source, destination) sub-buckets for epoch e+1, allocating
+//           slots with LDS counters (no global atomics per event).
+// The arrival key of a spike is its rank in lif.c's list order: a forced
+// spike's driven-list position, or 2048 + index for a threshold spike.
 //
-// Preconditions (checked on the host; otherwise the caller uses the per-step
-// engines, which accept every parameter set): no background events, no
-// presynaptic inhibition, at most 2048 driven neurons, at most 2048 neurons
-// per block.  With synaptic depression live the epoch is also capped at the
-// refractory period, so a neuron arrives at most once per epoch and its
-// depression bookkeeping stays local.  If an
-// epoch's events overflow the LDS, the launch stops at the epoch boundary and
-// reports how many steps it completed; the host runs that epoch with the
-// per-step engine and relaunches.
+// A launch starts with a prologue that scatters the ring lists already in
+// the delay line (from the per-step engines or the previous launch) into the
+// sub-buckets, keyed by list position.  Preconditions (checked on the host;
+// otherwise the per-step engines run): no background events, no presynaptic
+// inhibition, D <= 32, at most 2048 driven neurons and 2048 neurons per
+// block, and with synaptic depression live a refractory period of at least D
+// steps (a neuron then arrives at most once per epoch).  A sub-bucket or LDS
+// overflow stops the launch at an epoch boundary; the host runs one epoch on
+// the per-step engine and relaunches.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -54,12 +56,17 @@ constexpr int PT = 1024;            // threads per block
 constexpr int NPT = 2;              // neurons per thread in a regular block
 constexpr int NB_MAX = PT * NPT;    // neurons per regular block
 constexpr int D_MAX = 32;           // epoch length bound
-constexpr int EV_CAP = 4096;        // LDS events per block per epoch
+constexpr int EV_CAP = 4096;        // events per destination block per epoch (LDS)
+constexpr int SPK_CAP = 2048;       // spikes per block per epoch (LDS push list)
 constexpr int DPT = 2;              // driven neurons per thread in block 0 (wg_scan_flags2 takes two)
 constexpr int DRV_CAP = PT * DPT;
-constexpr int WORDS_PER_STEP = NB_MAX / 32;
+constexpr int S_MAX = 1024;         // owning blocks (driven + regular)
+constexpr int CAP_DRV = 1024;       // sub-bucket slots from the driven block to one destination
+constexpr int CAP_REG = 128;        // sub-bucket slots from a regular block to one destination
+constexpr int WPS = NB_MAX / 32;    // words of one step in the hasEv bitmap
+constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
 
-// event key: step(5) | class(1) | local target(12) | arrival position(20) | position in row(26)
+// event key: step(5) | class(1) | local target(12) | arrival key(20) | position in row(26)
 __device__ __forceinline__ uint64_t ev_key(uint32_t m, uint32_t c, uint32_t ql, uint32_t k, uint32_t jr) {
     return ((uint64_t)m << 59) | ((uint64_t)c << 58) | ((uint64_t)ql << 46) | ((uint64_t)k << 26) | (uint64_t)jr;
 }
@@ -70,46 +77,46 @@ struct PersistArgs {
     DevBrain d;
     int32_t head0;       // ring head at launch
     int32_t nsteps;
-    int32_t L;           // epoch length (<= D, <= refractory length)
-    int32_t nb;          // neurons per regular block
+    int32_t D;           // epoch length (the delay in steps)
+    int32_t nb;          // neurons per regular block (multiple of 64)
+    int32_t S;           // owning blocks: the driven block + regular blocks
+    int32_t nSpare;      // spare blocks after them
     int32_t nwords;      // 64-bit words of one step's spike bitmask
-    uint64_t *spk;       // [D_MAX][nwords]
-    uint8_t *dflag;      // [D_MAX][nDriven]
+    uint64_t *spk;       // [2][D_MAX][nwords] threshold spikes of non-driven neurons
+    uint8_t *dflag;      // [2][D_MAX][DRV_CAP] forced spikes, by driven-list position
+    uint8_t *tflag;      // [2][D_MAX][DRV_CAP] threshold spikes of driven neurons
     const int32_t *drvPos;   // [N] position in the driven list, -1 if not driven
-    uint64_t *fwdKey;    // events forwarded to the driven block
-    float *fwdAdd;
-    int32_t fwdCap;
-    int32_t *ctl;        // [0] steps done, [1] abort flag, [2] forwarded count
-    uint64_t *prof;      // per-phase clock ticks of blocks 0 and 1 (12), driven-block step parts (6)
+    uint32_t *arrMask;   // [N] arrival steps of the first epoch (depression bookkeeping)
+    uint64_t *sbKey;     // [2][perPar] sub-buckets
+    float *sbAdd;
+    int64_t perPar;
+    int32_t *sbCnt;      // [2][S][S]  count from source s to destination r at [r*S + s]
+    int32_t *dstTot;     // [2][S]     events addressed to each destination
+    int32_t *ctl;        // [0] steps done, [1] overflow flags
+    uint64_t *prof;      // phase clocks (see k_persist)
 };
 
 struct Shared {
     uint64_t evKey[EV_CAP];
     float evAdd[EV_CAP];
-    uint32_t hasEv[D_MAX * WORDS_PER_STEP];
-    uint32_t arr[D_MAX * WORDS_PER_STEP];
-    uint32_t draws[DRV_CAP];
+    uint32_t hasEv[D_MAX * WPS];
     uint32_t leap[LEAP_LDS_WORDS];
     uint32_t ybase[64];
+    int32_t pref[S_MAX + 1];
+    int32_t cnt[S_MAX];          // push slots per destination
+    int32_t spkI[SPK_CAP];       // push list: neuron, step | key, res at arrival start
+    uint32_t spkK[SPK_CAP];
+    float spkR[SPK_CAP];
     int32_t scan[64 + PT / 64];
-    uint64_t ph[7];      // phase clocks of thread 0: six accumulators and the last stamp
-    uint64_t sub[7];     // driven-block step clocks: five accumulators and the last stamp
-    int32_t nev;
-    int32_t pref[D_MAX + 1];
-    uint32_t rngBase;
+    uint32_t rngNext[2];
+    int32_t nev, nspk;
+    unsigned long long ev, syn;
+    uint64_t ph[7];              // phase clocks of thread 0
+    uint64_t sub[7];             // driven-step clocks of thread 0
 };
 
-// ------------------------------------------------------------------ small helpers
+// ------------------------------------------------------------------ helpers
 
-__device__ __forceinline__ uint32_t lower_bound_target(const uint2 *edges, uint32_t a, uint32_t e, uint32_t v) {
-    while (a < e) {
-        const uint32_t mid = a + ((e - a) >> 1);
-        if (edges[mid].x < v) a = mid + 1; else e = mid;
-    }
-    return a;
-}
-
-// LDS bitonic sort of evKey/evAdd[0, n), padded to a power of two with ~0 keys
 __device__ inline void lds_sort(Shared &s, int32_t n) {
     int32_t P = 1;
     while (P < n) P <<= 1;
@@ -133,7 +140,6 @@ __device__ inline void lds_sort(Shared &s, int32_t n) {
     }
 }
 
-// first index in evKey[0, n) whose group is >= g
 __device__ __forceinline__ int32_t find_group(const Shared &s, int32_t n, uint32_t g) {
     int32_t a = 0, e = n;
     while (a < e) {
@@ -154,113 +160,147 @@ __device__ __forceinline__ float res_after(const Consts &k, float r, int32_t m) 
     return r;
 }
 
-__device__ __forceinline__ int32_t ring_slot(const PersistArgs &a, int32_t done, int32_t m) {
-    return (a.head0 + done + m) % a.d.nslots;      // head of epoch step m
+// ring head at step `t` of the launch
+__device__ __forceinline__ int32_t head_at(const PersistArgs &a, int32_t t) { return (a.head0 + t) % a.d.nslots; }
+
+__device__ __forceinline__ int64_t sb_base(const PersistArgs &a, int32_t par, int32_t src, int32_t dst) {
+    const int64_t b = src == 0 ? (int64_t)dst * CAP_DRV
+                               : (int64_t)CAP_DRV * a.S + ((int64_t)(src - 1) * a.S + dst) * CAP_REG;
+    return (int64_t)par * a.perPar + b;
+}
+__device__ __forceinline__ int32_t sb_cap(int32_t src) { return src == 0 ? CAP_DRV : CAP_REG; }
+
+// destination block and local index of a target neuron
+__device__ __forceinline__ void owner_of(const PersistArgs &a, int32_t q, int32_t &dst, uint32_t &ql) {
+    const int32_t dp = a.drvPos[q];
+    if (dp >= 0) { dst = 0; ql = (uint32_t)dp; }
+    else { dst = 1 + q / a.nb; ql = (uint32_t)(q - (dst - 1) * a.nb); }
 }
 
-// ------------------------------------------------------------------ E1: arrivals -> events
-
-__device__ void e1_regular(const PersistArgs &a, Shared &s, int32_t done, int32_t L, int32_t lo, int32_t hi) {
+// One wave writes the events of one arriving row into this block's
+// sub-buckets for parity `par`.  Slots come from LDS counters (s.cnt).
+__device__ inline void push_row(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t pre, uint32_t m,
+                                uint32_t key, float sv, int lane) {
     const DevBrain &d = a.d;
-    const int tid = threadIdx.x;
-    const int32_t words = (a.nb + 31) >> 5;
-    for (int32_t w = tid; w < L * WORDS_PER_STEP; w += PT) { s.hasEv[w] = 0u; s.arr[w] = 0u; }
-    if (tid == 0) {
-        s.nev = 0;
-        int32_t acc = 0;
-        for (int32_t m = 0; m < L; m++) { s.pref[m] = acc; acc += d.ringCount[ring_slot(a, done, m)]; }
-        s.pref[L] = acc;
-    }
-    __syncthreads();
-    (void)words;
-    const int32_t Atot = s.pref[L];
-    for (int32_t g = tid; g < Atot; g += PT) {
-        int32_t m = 0;
-        while (s.pref[m + 1] <= g) m++;
-        const int32_t k = g - s.pref[m];
-        const int32_t pre = d.ring[(int64_t)ring_slot(a, done, m) * d.ringStride + k];
-        const float r = res_after(d.k, d.res[pre], m);
-        const float sv = d.g.sign[pre] * r;
-        if (pre >= lo && pre < hi && a.drvPos[pre] < 0 && sv != 0.f) {
-            const int32_t ql = pre - lo;
-            atomicOr(&s.arr[m * WORDS_PER_STEP + (ql >> 5)], 1u << (ql & 31));
-        }
-        if (sv == 0.f) continue;
-        const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
-        const uint32_t j0 = lower_bound_target(d.g.edges, ra, re, (uint32_t)lo);
-        const uint32_t j1 = lower_bound_target(d.g.edges, j0, re, (uint32_t)hi);
-        if (j1 == j0) continue;
-        const uint32_t c = sv > 0 ? 0u : 1u;
-        const int32_t base = atomicAdd(&s.nev, (int32_t)(j1 - j0));
-        for (uint32_t j = j0; j < j1; j++) {
-            const uint2 ed = d.g.edges[j];
-            const float addend = __uint_as_float(ed.y) * sv;
-            const int32_t q = (int32_t)ed.x;
-            const int32_t idx = base + (int32_t)(j - j0);
-            const int32_t dp = a.drvPos[q];
-            if (dp >= 0) {
-                const int32_t p = atomicAdd(&a.ctl[2], 1);
-                if (p < a.fwdCap) {
-                    a.fwdKey[p] = ev_key((uint32_t)m, c, (uint32_t)dp, (uint32_t)k, j - ra);
-                    a.fwdAdd[p] = addend;
-                } else {
-                    atomicOr(&a.ctl[1], 1);
-                }
-                if (idx < EV_CAP) { s.evKey[idx] = ~0ull; s.evAdd[idx] = 0.f; }
-            } else if (idx < EV_CAP) {
-                const uint32_t ql = (uint32_t)(q - lo);
-                s.evKey[idx] = ev_key((uint32_t)m, c, ql, (uint32_t)k, j - ra);
-                s.evAdd[idx] = addend;
-                atomicOr(&s.hasEv[m * WORDS_PER_STEP + (ql >> 5)], 1u << (ql & 31));
-            }
+    const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
+    const uint32_t c = sv > 0 ? 0u : 1u;
+    const int32_t cap = sb_cap(src);
+    for (uint32_t j = ra + lane; j < re; j += 64) {
+        const uint2 ed = d.g.edges[j];
+        int32_t dst;
+        uint32_t ql;
+        owner_of(a, (int32_t)ed.x, dst, ql);
+        const int32_t pos = atomicAdd(&s.cnt[dst], 1);
+        if (pos < cap) {
+            const int64_t at = sb_base(a, par, src, dst) + pos;
+            a.sbKey[at] = ev_key(m, c, ql, key, j - ra);
+            a.sbAdd[at] = __uint_as_float(ed.y) * sv;
         }
     }
-    __syncthreads();
-    if (tid == 0 && s.nev > EV_CAP) atomicOr(&a.ctl[1], 2);
+    if (lane == 0) {
+        atomicAdd(&s.ev, (unsigned long long)(re - ra));
+        atomicAdd(&s.syn, (unsigned long long)d.g.rowSyn[pre]);
+    }
 }
 
-// the driven block: arrival marks for its own neurons and the work counters
-__device__ void e1_driven(const PersistArgs &a, Shared &s, int32_t done, int32_t L, unsigned long long &ev,
-                          unsigned long long &syn) {
-    const DevBrain &d = a.d;
+// After a push: publish this block's counts to every destination, check the
+// capacities, fold the work counters into the brain's.
+__device__ inline void publish_counts(const PersistArgs &a, Shared &s, int32_t par, int32_t src) {
     const int tid = threadIdx.x;
-    for (int32_t w = tid; w < L * WORDS_PER_STEP; w += PT) s.arr[w] = 0u;
-    if (tid == 0) {
-        int32_t acc = 0;
-        for (int32_t m = 0; m < L; m++) { s.pref[m] = acc; acc += d.ringCount[ring_slot(a, done, m)]; }
-        s.pref[L] = acc;
-    }
     __syncthreads();
-    const int32_t Atot = s.pref[L];
-    ev = 0;
-    syn = 0;
-    for (int32_t g = tid; g < Atot; g += PT) {
+    const int32_t cap = sb_cap(src);
+    for (int32_t r = tid; r < a.S; r += PT) {
+        const int32_t c = s.cnt[r];
+        a.sbCnt[(int64_t)par * a.S * a.S + (int64_t)r * a.S + src] = min(c, cap);
+        if (c > cap) atomicOr(&a.ctl[1], 1);
+        if (c) {
+            const int32_t tot = atomicAdd(&a.dstTot[par * a.S + r], c) + c;
+            if (tot > EV_CAP) atomicOr(&a.ctl[1], 2);
+        }
+        s.cnt[r] = 0;
+    }
+    if (tid == 0) {
+        if (s.ev) atomicAdd(&a.d.sc->edgeEvents, s.ev);
+        if (s.syn) atomicAdd(&a.d.sc->synEvents, s.syn);
+        s.ev = 0;
+        s.syn = 0;
+    }
+}
+
+// ------------------------------------------------------------------ prologue: ring lists -> sub-buckets
+
+__device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
+    const DevBrain &d = a.d;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t me = blockIdx.x;
+    int32_t pref[D_MAX + 1];
+    int32_t acc = 0;
+    for (int32_t m = 0; m < L0; m++) { pref[m] = acc; acc += d.ringCount[head_at(a, m)]; }
+    pref[L0] = acc;
+    const int32_t Atot = acc;
+    const int32_t waves = a.S * (PT / 64);
+    for (int32_t g = me * (PT / 64) + wave; g < Atot; g += waves) {
         int32_t m = 0;
-        while (s.pref[m + 1] <= g) m++;
-        const int32_t k = g - s.pref[m];
-        const int32_t pre = d.ring[(int64_t)ring_slot(a, done, m) * d.ringStride + k];
+        while (pref[m + 1] <= g) m++;
+        const int32_t k = g - pref[m];
+        const int32_t pre = d.ring[(int64_t)head_at(a, m) * d.ringStride + k];
         const float sv = d.g.sign[pre] * res_after(d.k, d.res[pre], m);
         if (sv == 0.f) continue;
-        ev += d.g.indptr[pre + 1] - d.g.indptr[pre];
-        syn += d.g.rowSyn[pre];
-        const int32_t dp = a.drvPos[pre];
-        if (dp >= 0) atomicOr(&s.arr[m * WORDS_PER_STEP + (dp >> 5)], 1u << (dp & 31));
+        if (d.k.depU != 0.f && lane == 0) atomicOr(&a.arrMask[pre], 1u << m);
+        push_row(a, s, 0, me, pre, (uint32_t)m, (uint32_t)k, sv, lane);
     }
-    __syncthreads();
+    publish_counts(a, s, 0, me);
 }
 
-// ------------------------------------------------------------------ E2: L steps from registers
+// ------------------------------------------------------------------ gather: sub-buckets -> sorted LDS events
+
+__device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
+    const int tid = threadIdx.x;
+    const int32_t *cnt = a.sbCnt + (int64_t)par * a.S * a.S + (int64_t)me * a.S;
+    int32_t total = 0;
+    // S <= 1024: one count per thread
+    const int32_t c = tid < a.S ? cnt[tid] : 0;
+    const int32_t off = wg_exclusive_scan<PT>(c, s.scan, &total);
+    if (tid < a.S) s.pref[tid] = off;
+    if (tid == 0) { s.pref[a.S] = total; a.dstTot[par * a.S + me] = 0; }
+    for (int32_t w = tid; w < D_MAX * WPS; w += PT) s.hasEv[w] = 0u;
+    __syncthreads();
+    const int32_t n = min(total, EV_CAP);
+    for (int32_t g = tid; g < n; g += PT) {
+        int32_t lo = 0, hi = a.S;   // last src with pref <= g
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (s.pref[mid] <= g) lo = mid; else hi = mid;
+        }
+        const int64_t at = sb_base(a, par, lo, me) + (g - s.pref[lo]);
+        const uint64_t key = a.sbKey[at];
+        s.evKey[g] = key;
+        s.evAdd[g] = a.sbAdd[at];
+        const uint32_t grp = ev_group(key);
+        const uint32_t m = grp >> 13, ql = grp & 4095u;
+        atomicOr(&s.hasEv[m * WPS + (ql >> 5)], 1u << (ql & 31));
+    }
+    __syncthreads();
+    if (n > 1) lds_sort(s, n);
+    __syncthreads();
+    return n;
+}
+
+// ------------------------------------------------------------------ cells
 
 struct Cell {
     NeuronState st;
     float bias, thr, sign;
     uint32_t count;   // spike count increments this launch
+    uint32_t mask;    // steps of this epoch at which the neuron spiked
+    uint32_t prev;    // the same for the previous epoch: its arrivals now
+    uint32_t forced;  // driven cells: steps whose spike was forced
     int32_t i;        // neuron index, -1 if the slot is empty
 };
 
 __device__ __forceinline__ void load_cell(const DevBrain &d, Cell &c, int32_t i) {
     c.i = i;
-    c.count = 0;
+    c.count = 0; c.mask = 0; c.prev = 0; c.forced = 0;
     if (i < 0) return;
     c.st.v = d.v[i]; c.st.gE = d.gE[i]; c.st.gI = d.gI[i]; c.st.refr = d.refr[i];
     c.st.trace = d.trace[i]; c.st.adapt = d.adapt[i]; c.st.res = d.res[i];
@@ -279,70 +319,57 @@ __device__ __forceinline__ void store_cell(const DevBrain &d, Cell &c) {
 // lif.c phase 1 for one neuron: depression if it arrives now, then this step's inputs
 __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shared &s, int32_t nev, int32_t m,
                                             uint32_t ql) {
-    if ((s.arr[m * WORDS_PER_STEP + (ql >> 5)] >> (ql & 31)) & 1u) {
+    if ((c.prev >> m) & 1u) {
         if (c.sign * c.st.res != 0.f) c.st.res = c.st.res - k.depU * c.st.res;
     }
-    if ((s.hasEv[m * WORDS_PER_STEP + (ql >> 5)] >> (ql & 31)) & 1u) {
+    if ((s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u) {
         c.st.gE = apply_group(s, nev, group_of((uint32_t)m, 0u, ql), c.st.gE);
         c.st.gI = apply_group(s, nev, group_of((uint32_t)m, 1u, ql), c.st.gI);
     }
 }
 
-__device__ void e2_regular(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t done, int32_t L,
-                           int32_t lo) {
+// ------------------------------------------------------------------ simulate
+
+__device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t nev, int32_t par,
+                            int32_t L, int32_t lo) {
     const DevBrain &d = a.d;
-    const int32_t nev = min(s.nev, EV_CAP);
-    if (nev > 1) lds_sort(s, nev);
-    __syncthreads();
     const int lane = threadIdx.x & 63;
+    uint64_t *spk = a.spk + (int64_t)par * D_MAX * a.nwords;
     for (int32_t m = 0; m < L; m++) {
 #pragma unroll
         for (int u = 0; u < NPT; u++) {
             Cell &c = cells[u];
-            bool spk = false;
+            bool fire = false;
             if (c.i >= 0) {
                 cell_inputs(d.k, c, s, nev, m, (uint32_t)(c.i - lo));
                 membrane(d.k, c.st, c.bias);
                 if (crosses(d.k, c.st, c.thr)) {
                     spike_effects(d.k, c.st, d.k.tRef);
                     c.count++;
-                    spk = true;
+                    c.mask |= 1u << m;
+                    fire = true;
                 }
             }
-            const uint64_t mask = __ballot(spk);
-            // the wave's 64 neurons are consecutive and 64-aligned
-            if (lane == 0 && mask) {
+            // the wave's 64 neurons are consecutive and 64-aligned, and no
+            // other block writes this word: a plain store
+            const uint64_t bits = __ballot(fire);
+            if (lane == 0 && bits) {
                 const int32_t first = lo + (int32_t)(threadIdx.x & ~63) + u * PT;
-                atomicOr((unsigned long long *)&a.spk[(int64_t)m * a.nwords + (first >> 6)], (unsigned long long)mask);
+                spk[(int64_t)m * a.nwords + (first >> 6)] = bits;
             }
         }
     }
-    (void)done;
 }
 
-__device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], float (&pFire)[DPT], int32_t L,
-                          uint32_t &rng) {
+__device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], const float (&pFire)[DPT],
+                           int32_t nev, int32_t par, int32_t L, uint32_t &rng) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t nD = d.nDriven;
     const int32_t P = (nD + PT - 1) / PT;
-    // forwarded events
-    const int32_t nfw = min(a.ctl[2], a.fwdCap);
-    for (int32_t w = tid; w < L * WORDS_PER_STEP; w += PT) s.hasEv[w] = 0u;
-    __syncthreads();
-    const int32_t nev = min(nfw, EV_CAP);
-    for (int32_t q = tid; q < nev; q += PT) {
-        const uint64_t key = a.fwdKey[q];
-        s.evKey[q] = key;
-        s.evAdd[q] = a.fwdAdd[q];
-        const uint32_t g = ev_group(key);
-        const uint32_t m = g >> 13, ql = g & 4095u;
-        atomicOr(&s.hasEv[m * WORDS_PER_STEP + (ql >> 5)], 1u << (ql & 31));
-    }
-    __syncthreads();
-    if (nev > 1) lds_sort(s, nev);
-    __syncthreads();
     const float forcedRefr = d.k.tRef + d.k.dt;
+    uint8_t *dflag = a.dflag + (int64_t)par * D_MAX * DRV_CAP;
+    uint8_t *tflag = a.tflag + (int64_t)par * D_MAX * DRV_CAP;
     const bool timed = tid == 0;
 #define FB_SUB(k)                                      \
     if (timed) {                                       \
@@ -351,37 +378,41 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
         s.sub[6] = now;                                \
     }
     if (timed) s.sub[6] = wall_clock64();
+    int32_t nPrev = 0;
     for (int32_t m = 0; m < L; m++) {
         // lif.c phase 1: arrivals for the driven neurons
-        int32_t draws = 0;
-        uint32_t nonRefr = 0;
+        bool nr0 = false, nr1 = false;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
-            if (u < P && c.i >= 0) {
+            if (c.i >= 0) {
                 cell_inputs(d.k, c, s, nev, m, (uint32_t)(tid * P + u));
-                if (!(c.st.refr > 0)) { nonRefr |= 1u << u; draws++; }
+                const bool nr = !(c.st.refr > 0);
+                if (u == 0) nr0 = nr; else nr1 = nr;
             }
         }
-        (void)draws;
         FB_SUB(0)
         // lif.c phase 2: one draw per non-refractory driven neuron, in list order
         int32_t n;
-        const int32_t off = wg_scan_flags2<PT>(nonRefr & 1u, (nonRefr >> 1) & 1u, s.scan + (m & 1) * 16, &n);
+        const int32_t off = wg_scan_flags2<PT>(nr0, nr1, s.scan + (m & 1) * 16, &n);
+        if (nPrev > 0) rng = s.rngNext[(m - 1) & 1];   // the last draw of the previous step
+        nPrev = n;
         FB_SUB(1)
-        // n <= 2048 driven neurons: one leapfrog round covers the step
-        if (n > 0) {
-            leap_round(s.leap, s.ybase, s.draws, n, rng);
-            rng = s.draws[n - 1];
-        }
+        if (tid < 64 && tid * 32 < n) s.ybase[tid] = matvec(s.leap + (32 + tid) * LEAP_STRIDE, rng);
+        __syncthreads();
         FB_SUB(2)
-        int32_t pos = off;
+        int32_t j = off;
+        uint32_t x = 0;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
+            const bool nr = u == 0 ? nr0 : nr1;
             uint8_t f = 0;
-            if ((nonRefr >> u) & 1u) {
-                const uint32_t x = s.draws[pos++];
+            if (nr) {
+                // stream position j+1 after rng: M^(j%32+1) (M^(32 (j/32)) rng), or one step past the previous draw
+                x = j == off ? matvec(s.leap + (j & 31) * LEAP_STRIDE, s.ybase[j >> 5]) : xs32(x);
+                if (j == n - 1) s.rngNext[m & 1] = x;
+                j++;
                 const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
                 if (uu < pFire[u]) {
                     c.st.v = d.k.vReset;
@@ -389,61 +420,114 @@ __device__ void e2_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], f
                     c.st.trace = 1.f;
                     c.st.adapt = c.st.adapt + d.k.adaptInc;
                     c.count++;
+                    c.mask |= 1u << m;
+                    c.forced |= 1u << m;
                     f = 1;
                 }
             }
-            if (u < P && tid * P + u < nD) a.dflag[(int64_t)m * nD + tid * P + u] = f;
+            if (c.i >= 0) dflag[(int64_t)m * DRV_CAP + tid * P + u] = f;
         }
         FB_SUB(3)
         // lif.c phases 3-4 for the driven neurons
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
-            if (u < P && c.i >= 0) {
+            if (c.i >= 0) {
                 membrane(d.k, c.st, c.bias);
+                uint8_t t = 0;
                 if (crosses(d.k, c.st, c.thr)) {
                     spike_effects(d.k, c.st, d.k.tRef);
                     c.count++;
-                    atomicOr((unsigned long long *)&a.spk[(int64_t)m * a.nwords + (c.i >> 6)],
-                             1ull << (c.i & 63));
+                    c.mask |= 1u << m;
+                    t = 1;
                 }
+                tflag[(int64_t)m * DRV_CAP + tid * P + u] = t;
             }
         }
         FB_SUB(4)
-        __syncthreads();   // s.draws is rewritten next step
-        FB_SUB(5)
     }
+    __syncthreads();
+    if (nPrev > 0) rng = s.rngNext[(L - 1) & 1];
+    FB_SUB(5)
 #undef FB_SUB
 }
 
-// ------------------------------------------------------------------ E3: spike lists in lif.c order
+// ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
 
-__device__ void e3_list(const PersistArgs &a, Shared &s, int32_t done, int32_t m) {
+__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t par, int32_t me,
+                            bool drivenBlock) {
+    const DevBrain &d = a.d;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t P = drivenBlock ? (d.nDriven + PT - 1) / PT : 0;
+    if (tid == 0) s.nspk = 0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NPT; u++) {
+        const Cell &c = cells[u];
+        uint32_t bits = c.mask;
+        while (bits) {
+            const int m = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int32_t e = atomicAdd(&s.nspk, 1);
+            if (e < SPK_CAP) {
+                const bool forced = (c.forced >> m) & 1u;
+                s.spkI[e] = c.i;
+                s.spkK[e] = ((uint32_t)m << 24) |
+                            (forced ? (uint32_t)(tid * P + u) : KEY_THRESHOLD + (uint32_t)c.i);
+                s.spkR[e] = res_after(d.k, c.st.res, m);   // res at its arrival step of the next epoch
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t ns = s.nspk;
+    if (tid == 0 && ns > SPK_CAP) atomicOr(&a.ctl[1], 4);
+    for (int32_t e = wave; e < min(ns, SPK_CAP); e += PT / 64) {
+        const int32_t i = s.spkI[e];
+        const uint32_t kk = s.spkK[e];
+        const float sv = d.g.sign[i] * s.spkR[e];
+        if (sv == 0.f) continue;
+        push_row(a, s, par, me, i, kk >> 24, kk & 0xFFFFFFu, sv, lane);
+    }
+    publish_counts(a, s, par, me);
+}
+
+// ------------------------------------------------------------------ lists: one step's spikes in lif.c order
+
+__device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t t, int32_t m) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t nD = d.nDriven;
-    const int32_t slot = (ring_slot(a, done, m) + d.nslots - 1) % d.nslots;
+    const int32_t slot = (head_at(a, t) + d.nslots - 1) % d.nslots;
     int32_t *out = d.ring + (int64_t)slot * d.ringStride;
-    // forced spikes in list order
+    const uint8_t *fl = a.dflag + ((int64_t)par * D_MAX + m) * DRV_CAP;
+    const uint8_t *tf = a.tflag + ((int64_t)par * D_MAX + m) * DRV_CAP;
+    uint64_t *bits = a.spk + ((int64_t)par * D_MAX + m) * a.nwords;
+    // driven list position q sits at fl[(q / P) * P + q % P] = fl[q] with the
+    // chunked layout of the driven block (thread t holds positions tP..tP+P-1)
     const int32_t P = (nD + PT - 1) / PT;
     const int32_t dlo = min(nD, tid * P), dhi = min(nD, dlo + P);
-    const uint8_t *fl = a.dflag + (int64_t)m * nD;
     int32_t nf = 0;
-    for (int32_t q = dlo; q < dhi; q++) nf += fl[q];
+    for (int32_t q = dlo; q < dhi; q++) {
+        nf += fl[q];
+        if (tf[q]) {   // threshold spikes of driven neurons join the bitmask
+            const int32_t i = d.driven[q];
+            atomicOr((unsigned long long *)&bits[i >> 6], 1ull << (i & 63));
+        }
+    }
     int32_t ndTot;
     int32_t pos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
     for (int32_t q = dlo; q < dhi; q++)
         if (fl[q]) out[pos++] = d.driven[q];
-    // threshold spikes in ascending index
-    uint64_t *bits = a.spk + (int64_t)m * a.nwords;
+    __threadfence_block();
+    __syncthreads();
     const int32_t W = (a.nwords + PT - 1) / PT;
     const int32_t wlo = min(a.nwords, tid * W), whi = min(a.nwords, wlo + W);
     int32_t cnt = 0;
-    for (int32_t w = wlo; w < whi; w++) cnt += __popcll(bits[w]);
+    for (int32_t w = wlo; w < whi; w++) cnt += __popcll(__hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     int32_t thTot;
     pos = ndTot + wg_exclusive_scan<PT>(cnt, s.scan, &thTot);
     for (int32_t w = wlo; w < whi; w++) {
-        uint64_t b = bits[w];
+        uint64_t b = __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!b) continue;
         bits[w] = 0ull;
         while (b) {
@@ -456,6 +540,21 @@ __device__ void e3_list(const PersistArgs &a, Shared &s, int32_t done, int32_t m
         d.ringCount[slot] = ndTot + thTot;
         atomicAdd(&d.sc->spikes, (unsigned long long)(ndTot + thTot));
     }
+    __syncthreads();
+}
+
+// lists of epoch steps [t0, t0+L), spread over the spare blocks (or the
+// regular blocks when the grid has no spare ones)
+__device__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_t t0, int32_t L) {
+    const int32_t first = a.nSpare > 0 ? a.S : 1;
+    const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - 1);
+    const int32_t b = (int32_t)blockIdx.x - first;
+    if (b < 0 || b >= count) return;
+    for (int32_t m = b; m < L; m += count) build_list(a, s, par, t0 + m, m);
+    if (b == 0 && threadIdx.x == 0) {
+        // the slot delivered at the epoch's last step is empty until a later list fills it
+        a.d.ringCount[head_at(a, t0 + L - 1)] = 0;
+    }
 }
 
 // ------------------------------------------------------------------ the kernel
@@ -465,10 +564,26 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     cg::grid_group grid = cg::this_grid();
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    const bool drivenBlock = blockIdx.x == 0;
-    const int32_t lo = drivenBlock ? 0 : ((int32_t)blockIdx.x - 1) * a.nb;
-    const int32_t hi = drivenBlock ? 0 : min(d.N, lo + a.nb);
+    const int32_t me = blockIdx.x;
+    const bool drivenBlock = me == 0;
+    const bool owning = me < a.S;
+    const int32_t lo = drivenBlock || !owning ? 0 : (me - 1) * a.nb;
+    const int32_t hi = drivenBlock || !owning ? 0 : min(d.N, lo + a.nb);
 
+    const bool timed = tid == 0 && me < 2;
+    if (timed) {
+        for (int k = 0; k < 6; k++) { s.ph[k] = 0; s.sub[k] = 0; }
+        s.ph[6] = wall_clock64();
+    }
+#define FB_PHASE(k)                                    \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.ph[k] += now - s.ph[6];                      \
+        s.ph[6] = now;                                 \
+    }
+
+    if (tid < S_MAX) s.cnt[tid] = 0;
+    if (tid == 0) { s.ev = 0; s.syn = 0; s.rngNext[0] = s.rngNext[1] = 0; }
     Cell cells[NPT];
     float pFire[DPT];
     uint32_t rng = 0;
@@ -487,66 +602,67 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
 #pragma unroll
         for (int u = 0; u < NPT; u++) {
             const int32_t i = lo + tid + u * PT;
-            load_cell(d, cells[u], (i < hi && a.drvPos[i] < 0) ? i : -1);
+            load_cell(d, cells[u], (owning && i < hi && a.drvPos[i] < 0) ? i : -1);
         }
+    }
+    __syncthreads();
+
+    const int32_t L0 = min(a.D, a.nsteps);
+    if (owning) prologue_scatter(a, s, L0);
+    FB_PHASE(5)
+    grid.sync();
+    FB_PHASE(4)
+    if (d.k.depU != 0.f && owning) {
+#pragma unroll
+        for (int u = 0; u < NPT; u++)
+            if (cells[u].i >= 0) { cells[u].prev = a.arrMask[cells[u].i]; a.arrMask[cells[u].i] = 0u; }
     }
 
-    // per-phase clock ticks, kept by thread 0 of blocks 0 and 1
-    const bool timed = tid == 0 && blockIdx.x < 2;
-    if (timed) {
-        for (int k = 0; k < 6; k++) { s.ph[k] = 0; s.sub[k] = 0; }
-        s.ph[6] = wall_clock64();
-    }
-#define FB_PHASE(k)                                    \
-    if (timed) {                                       \
-        const uint64_t now = wall_clock64();           \
-        s.ph[k] += now - s.ph[6];                      \
-        s.ph[6] = now;                                 \
-    }
-    int32_t done = 0;
-    while (done < a.nsteps) {
-        const int32_t L = min(a.L, a.nsteps - done);
-        unsigned long long ev = 0, syn = 0;
-        if (drivenBlock) e1_driven(a, s, done, L, ev, syn);
-        else e1_regular(a, s, done, L, lo, hi);
-        FB_PHASE(0)
-        grid.sync();
-        FB_PHASE(1)
-        // every block reads the same flags after the barrier
-        if (a.ctl[1] || a.ctl[2] > EV_CAP) break;
-        if (ev) atomicAdd(&d.sc->edgeEvents, ev);
-        if (syn) atomicAdd(&d.sc->synEvents, syn);
-        if (drivenBlock) {
-            e2_driven(a, s, cells, pFire, L, rng);
+    int32_t done = 0, e = 0, prevL = 0;
+    if (!a.ctl[1]) {
+        while (done < a.nsteps) {
+            const int32_t L = min(a.D, a.nsteps - done);
+            const int32_t par = e & 1;
+            if (e > 0) build_lists(a, s, par ^ 1, done - prevL, prevL);
+            FB_PHASE(3)
+            if (owning) {
+                const int32_t nev = gather(a, s, par, me);
+                FB_PHASE(0)
+                if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
+                else sim_regular(a, s, cells, nev, par, L, lo);
+                FB_PHASE(1)
+                if (done + L < a.nsteps) push_spikes(a, s, cells, par ^ 1, me, drivenBlock);
 #pragma unroll
-            for (int u = 0; u < DPT; u++) store_cell(d, cells[u]);
-            if (tid == 0) d.sc->rng = rng;
-        } else {
-            e2_regular(a, s, cells, done, L, lo);
+                for (int u = 0; u < NPT; u++) {
+                    cells[u].prev = cells[u].mask;
+                    cells[u].mask = 0;
+                    cells[u].forced = 0;
+                }
+                FB_PHASE(2)
+            }
+            grid.sync();
+            FB_PHASE(4)
+            done += L;
+            prevL = L;
+            e++;
+            if (a.ctl[1]) break;   // every block reads the same flags after the barrier
+        }
+    }
+    // the last epoch's lists, and the state back to memory
+    if (e > 0) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
+    FB_PHASE(3)
+    if (owning) {
 #pragma unroll
-            for (int u = 0; u < NPT; u++) store_cell(d, cells[u]);
-        }
-        FB_PHASE(2)
-        grid.sync();
-        FB_PHASE(3)
-        if ((int32_t)blockIdx.x < L) e3_list(a, s, done, (int32_t)blockIdx.x);
-        if (blockIdx.x == gridDim.x - 1 && tid == 0) {
-            // the slot delivered at the last step of the epoch is empty now
-            d.ringCount[ring_slot(a, done, L - 1)] = 0;
-            a.ctl[2] = 0;
-        }
-        FB_PHASE(4)
-        grid.sync();
-        FB_PHASE(5)
-        done += L;
+        for (int u = 0; u < NPT; u++) store_cell(d, cells[u]);
+        if (drivenBlock && tid == 0) d.sc->rng = rng;
     }
 #undef FB_PHASE
     if (timed) {
-        for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[blockIdx.x * 6 + k], (unsigned long long)s.ph[k]);
-        if (blockIdx.x == 0)
+        for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[me * 6 + k], (unsigned long long)s.ph[k]);
+        if (me == 0)
             for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[12 + k], (unsigned long long)s.sub[k]);
     }
-    if (blockIdx.x == 0 && tid == 0) a.ctl[0] = done;
+    if (me == 0 && tid == 0) a.ctl[0] = done;
 }
 
 // ------------------------------------------------------------------ host side
@@ -586,40 +702,74 @@ static int32_t refractory_steps(const Consts &k) {
     return n;
 }
 
+struct Layout {
+    int32_t nb, S, nSpare, grid;
+};
+
+// Regular blocks own 64-aligned ranges; a few blocks of the grid are left
+// spare for the list building when the network leaves room.
+static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
+    const int32_t maxB = g_ps.maxBlocks;
+    if (maxB < 2) return false;
+    const int32_t want = std::max(1, maxB - 1 - D);   // regular blocks if D spare blocks fit
+    int64_t nb = ((int64_t)(d.N + want - 1) / want + 63) / 64 * 64;
+    if (nb > NB_MAX) {
+        nb = ((int64_t)(d.N + (maxB - 1) - 1) / (maxB - 1) + 63) / 64 * 64;   // no spares
+        if (nb > NB_MAX) return false;
+    }
+    l.nb = (int32_t)nb;
+    const int32_t need = (int32_t)((d.N + nb - 1) / nb);
+    l.S = need + 1;
+    if (l.S > S_MAX) return false;
+    l.nSpare = std::min(D, maxB - l.S);
+    l.grid = l.S + l.nSpare;
+    return true;
+}
+
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why) {
     if (serialDeliver) { *why = "presynaptic inhibition or tRef <= 0 needs the serial deliver path"; return FB_ERR_UNSUPPORTED; }
     if (d.k.bgEvents != 0.f) { *why = "background events draw from the shared stream every step"; return FB_ERR_UNSUPPORTED; }
     if (d.nDriven > DRV_CAP) { *why = "more than 2048 driven neurons"; return FB_ERR_UNSUPPORTED; }
     const int32_t D = d.nslots - 1;
-    if (D < 1) { *why = "no delay"; return FB_ERR_UNSUPPORTED; }
-    if (d.k.depU != 0.f && refractory_steps(d.k) < 1) {
-        *why = "synaptic depression with a refractory period shorter than a step";
+    if (D < 1 || D > D_MAX) { *why = "delay outside 1..32 steps"; return FB_ERR_UNSUPPORTED; }
+    if (d.k.depU != 0.f && refractory_steps(d.k) < D) {
+        *why = "synaptic depression with a refractory period shorter than the delay";
         return FB_ERR_UNSUPPORTED;
     }
-    if (2LL * d.N >= (1LL << 20)) { *why = "more than 524,287 neurons (arrival index field)"; return FB_ERR_UNSUPPORTED; }
+    if (KEY_THRESHOLD + (int64_t)d.N >= (1LL << 20)) { *why = "more than 1,046,527 neurons (arrival key field)"; return FB_ERR_UNSUPPORTED; }
     if (d.g.maxRow >= (1LL << 26)) { *why = "a row longer than 2^26 edges (row position field)"; return FB_ERR_UNSUPPORTED; }
     fb_status st = persist_init();
     if (st) { *why = "cooperative launch unavailable"; return st; }
-    const int32_t regular = std::max(1, g_ps.maxBlocks - 1);
-    const int64_t nb = ((int64_t)(d.N + regular - 1) / regular + 63) / 64 * 64;
-    if (nb > NB_MAX) { *why = "more than 2048 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
+    Layout l;
+    if (!layout_for(d, D, l)) { *why = "more than 2048 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
     return FB_OK;
 }
 
 struct PersistBuffers {
     uint64_t *spk = nullptr;
-    uint8_t *dflag = nullptr;
+    uint8_t *dflag = nullptr, *tflag = nullptr;
     int32_t *drvPos = nullptr;
-    uint64_t *fwdKey = nullptr;
-    float *fwdAdd = nullptr;
+    uint32_t *arrMask = nullptr;
+    uint64_t *sbKey = nullptr;
+    float *sbAdd = nullptr;
+    int32_t *sbCnt = nullptr, *dstTot = nullptr;
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
-    int32_t N = 0, nDrivenCap = 0, fwdCap = 0;
-    int32_t drivenVersion = -1;
+    int32_t N = 0, S = 0;
+    int64_t perPar = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain's driven buffer
+
+static void free_buffers(PersistBuffers &pb) {
+    void *ps[] = {pb.spk, pb.dflag, pb.tflag, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
+                  pb.ctl, pb.prof};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    pb.spk = nullptr; pb.dflag = pb.tflag = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
+    pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
+}
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
                       hipStream_t st);
@@ -627,13 +777,12 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
 void persistent_release(const int32_t *key) {
     auto it = g_pbuf.find(key);
     if (it == g_pbuf.end()) return;
-    PersistBuffers &pb = it->second;
-    if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); hipFree(pb.prof); }
-    if (pb.ev0) { hipEventDestroy(pb.ev0); hipEventDestroy(pb.ev1); }
+    free_buffers(it->second);
+    if (it->second.ev0) { (void)hipEventDestroy(it->second.ev0); (void)hipEventDestroy(it->second.ev1); }
     g_pbuf.erase(it);
 }
 
-__global__ void k_drvpos(int32_t *drvPos, const int32_t *driven, int32_t N, int32_t nD) {
+__global__ void k_drvpos(int32_t *drvPos, int32_t N) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) drvPos[i] = -1;
 }
@@ -647,18 +796,28 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     std::string why;
     fb_status rc = persistent_supported(d, false, &why);
     if (rc) { set_error("persistent engine: %s", why.c_str()); return rc; }
+    const int32_t D = d.nslots - 1;
+    Layout lay;
+    layout_for(d, D, lay);
     PersistBuffers &pb = g_pbuf[d.driven];
     const int32_t nwords = (d.N + 63) / 64;
-    if (pb.N != d.N) {
-        if (pb.spk) { hipFree(pb.spk); hipFree(pb.dflag); hipFree(pb.drvPos); hipFree(pb.fwdKey); hipFree(pb.fwdAdd); hipFree(pb.ctl); hipFree(pb.prof); }
+    const int64_t perPar = (int64_t)CAP_DRV * lay.S + (int64_t)CAP_REG * (lay.S - 1) * lay.S;
+    if (pb.N != d.N || pb.S != lay.S) {
+        free_buffers(pb);
         pb.N = d.N;
-        pb.fwdCap = 1 << 16;
-        FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * (size_t)D_MAX * nwords));
-        FB_HIP(hipMemsetAsync(pb.spk, 0, sizeof(uint64_t) * (size_t)D_MAX * nwords, st));
-        FB_HIP(hipMalloc(&pb.dflag, (size_t)D_MAX * DRV_CAP));
+        pb.S = lay.S;
+        pb.perPar = perPar;
+        FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords));
+        FB_HIP(hipMemsetAsync(pb.spk, 0, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords, st));
+        FB_HIP(hipMalloc(&pb.dflag, (size_t)2 * D_MAX * DRV_CAP));
+        FB_HIP(hipMalloc(&pb.tflag, (size_t)2 * D_MAX * DRV_CAP));
         FB_HIP(hipMalloc(&pb.drvPos, sizeof(int32_t) * (size_t)d.N));
-        FB_HIP(hipMalloc(&pb.fwdKey, sizeof(uint64_t) * (size_t)pb.fwdCap));
-        FB_HIP(hipMalloc(&pb.fwdAdd, sizeof(float) * (size_t)pb.fwdCap));
+        FB_HIP(hipMalloc(&pb.arrMask, sizeof(uint32_t) * (size_t)d.N));
+        FB_HIP(hipMemsetAsync(pb.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
+        FB_HIP(hipMalloc(&pb.sbKey, sizeof(uint64_t) * 2 * (size_t)perPar));
+        FB_HIP(hipMalloc(&pb.sbAdd, sizeof(float) * 2 * (size_t)perPar));
+        FB_HIP(hipMalloc(&pb.sbCnt, sizeof(int32_t) * 2 * (size_t)lay.S * lay.S));
+        FB_HIP(hipMalloc(&pb.dstTot, sizeof(int32_t) * 2 * (size_t)lay.S));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 18));
         if (!pb.ev0) {
@@ -667,34 +826,29 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
     }
     // driven positions (the list may have changed since the last launch)
-    hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven, d.N, d.nDriven);
+    hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
     if (d.nDriven)
         hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
                            d.nDriven);
     FB_HIP(hipGetLastError());
 
-    const int32_t D = d.nslots - 1;
-    // With depression live, an epoch may not be longer than the refractory
-    // period, so a neuron arrives at most once per epoch (E1 then derives its
-    // res at arrival from the epoch-start value by recoveries alone).
-    const int32_t L0 = std::min({D, d.k.depU != 0.f ? refractory_steps(d.k) : D, D_MAX});
-    const int32_t regular = std::max(1, g_ps.maxBlocks - 1);
-    const int32_t nb = (int32_t)(((int64_t)(d.N + regular - 1) / regular + 63) / 64 * 64);
-    const int32_t need = (d.N + nb - 1) / nb;
-    const int32_t grid = std::max(need + 1, std::min(g_ps.maxBlocks, std::max(L0, 2)));
-    const int32_t L = std::min(L0, grid);   // block m materialises step m of an epoch
-
     PersistArgs a{};
     a.d = d;
-    a.L = L;
-    a.nb = nb;
+    a.D = D;
+    a.nb = lay.nb;
+    a.S = lay.S;
+    a.nSpare = lay.nSpare;
     a.nwords = nwords;
     a.spk = pb.spk;
     a.dflag = pb.dflag;
+    a.tflag = pb.tflag;
     a.drvPos = pb.drvPos;
-    a.fwdKey = pb.fwdKey;
-    a.fwdAdd = pb.fwdAdd;
-    a.fwdCap = pb.fwdCap;
+    a.arrMask = pb.arrMask;
+    a.sbKey = pb.sbKey;
+    a.sbAdd = pb.sbAdd;
+    a.perPar = perPar;
+    a.sbCnt = pb.sbCnt;
+    a.dstTot = pb.dstTot;
     a.ctl = pb.ctl;
     a.prof = pb.prof;
 
@@ -704,9 +858,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
         FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 18, st));
+        FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
-        FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(grid), dim3(PT), args, 0, st));
+        FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st));
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         int32_t ctl[4];
@@ -723,8 +878,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         done += ctl[0];
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {
-            // an epoch overflowed a bounded buffer: run it with the per-step engine
-            const int32_t m = std::min(L, nsteps - done);
+            // a bounded buffer overflowed: the next epoch runs on the per-step engine
+            if (d.k.depU != 0.f)
+                FB_HIP(hipMemsetAsync(pb.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
+            const int32_t m = std::min(D, nsteps - done);
             for (int32_t s2 = 0; s2 < m; s2++) {
                 const int32_t slot = (cur + d.nslots - 1) % d.nslots;
                 fb_status r2 = launch_step(d, cur, slot, false, true, st);
