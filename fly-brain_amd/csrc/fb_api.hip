@@ -34,7 +34,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches, double *kernelMs, double *phaseUs);
+                            int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion);
 void persistent_release(const int32_t *key);
 
 // ------------------------------------------------------------------ jump tables
@@ -192,6 +192,7 @@ struct fb_brain {
     int32_t *gatherCnt = nullptr, *gatherBuf = nullptr;
     std::map<int32_t, GraphCache> graphs;   // keyed by start head
     uint64_t version = 0;                   // bumps when DevBrain arguments change
+    uint64_t drivenVersion = 0;             // bumps when the device driven list is rewritten
     uint64_t graphVersion = ~0ull;
 };
 
@@ -363,6 +364,7 @@ static fb_status sync_driven(fb_brain *b) {
     if (b->d.nDriven != (int32_t)n) b->version++;
     b->d.nDriven = (int32_t)n;
     b->drivenDirty = false;
+    b->drivenVersion++;
     return FB_OK;
 }
 
@@ -684,9 +686,11 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     if (st) return st;
     if (nsteps == 0) return FB_OK;
     if (engine == FB_ENGINE_AUTO) {
+        // the lookahead engine pays a prologue per call: worth it from two epochs on
         std::string why;
-        engine = persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
-                                                                                : FB_ENGINE_GRAPH;
+        const bool longRun = nsteps >= 2 * (b->d.nslots - 1);
+        engine = longRun && persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
+                                                                                          : FB_ENGINE_GRAPH;
     }
     switch (engine) {
         case FB_ENGINE_LAUNCH: st = run_launch(b, nsteps); break;
@@ -696,7 +700,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
-                                   &b->persistentMs, b->phaseUs);
+                                   &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st) return st;
             b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
             b->head = (b->head + nsteps) % b->d.nslots;

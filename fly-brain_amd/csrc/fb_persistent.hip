@@ -44,6 +44,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 
 #include "fb_internal.h"
@@ -94,6 +95,8 @@ struct PersistArgs {
     int32_t *dstTot;     // [2][S]     events addressed to each destination
     int32_t *ctl;        // [0] steps done, [1] overflow flags
     uint64_t *prof;      // phase clocks (see k_persist)
+    const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
+    int32_t flags;       // bit 0: time the driven block's step parts
 };
 
 struct Shared {
@@ -104,9 +107,11 @@ struct Shared {
     uint32_t ybase[64];
     int32_t pref[S_MAX + 1];
     int32_t cnt[S_MAX];          // push slots per destination
-    int32_t spkI[SPK_CAP];       // push list: neuron, step | key, res at arrival start
+    int32_t spkI[SPK_CAP];       // push list: neuron, step << 24 | arrival key, sign x res at arrival
     uint32_t spkK[SPK_CAP];
     float spkR[SPK_CAP];
+    int32_t rowOff[SPK_CAP + 1]; // first edge of each pushed row in the block's flattened edge list
+    uint32_t rowA[SPK_CAP];      // row start in the CSR
     int32_t scan[64 + PT / 64];
     uint32_t rngNext[2];
     int32_t nev, nspk;
@@ -170,36 +175,74 @@ __device__ __forceinline__ int64_t sb_base(const PersistArgs &a, int32_t par, in
 }
 __device__ __forceinline__ int32_t sb_cap(int32_t src) { return src == 0 ? CAP_DRV : CAP_REG; }
 
-// destination block and local index of a target neuron
-__device__ __forceinline__ void owner_of(const PersistArgs &a, int32_t q, int32_t &dst, uint32_t &ql) {
-    const int32_t dp = a.drvPos[q];
-    if (dp >= 0) { dst = 0; ql = (uint32_t)dp; }
-    else { dst = 1 + q / a.nb; ql = (uint32_t)(q - (dst - 1) * a.nb); }
-}
-
-// One wave writes the events of one arriving row into this block's
-// sub-buckets for parity `par`.  Slots come from LDS counters (s.cnt).
-__device__ inline void push_row(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t pre, uint32_t m,
-                                uint32_t key, float sv, int lane) {
+// Writes the edge events of `ns` arriving rows into this block's sub-buckets
+// for parity `par`.  Row r is (s.spkI[r], step and key in s.spkK[r], sign x res
+// in s.spkR[r]).  Edges of all rows are spread over the block's threads (a
+// scan of the row lengths maps an edge to its row); slots come from LDS
+// counters, and four edges per thread are in flight at once.
+__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns) {
     const DevBrain &d = a.d;
-    const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
-    const uint32_t c = sv > 0 ? 0u : 1u;
-    const int32_t cap = sb_cap(src);
-    for (uint32_t j = ra + lane; j < re; j += 64) {
-        const uint2 ed = d.g.edges[j];
-        int32_t dst;
-        uint32_t ql;
-        owner_of(a, (int32_t)ed.x, dst, ql);
-        const int32_t pos = atomicAdd(&s.cnt[dst], 1);
-        if (pos < cap) {
-            const int64_t at = sb_base(a, par, src, dst) + pos;
-            a.sbKey[at] = ev_key(m, c, ql, key, j - ra);
-            a.sbAdd[at] = __uint_as_float(ed.y) * sv;
-        }
+    const int tid = threadIdx.x;
+    // row lengths -> prefix (two rows per thread)
+    int32_t len0 = 0, len1 = 0;
+    uint32_t a0 = 0, a1 = 0;
+    const int32_t r0 = 2 * tid, r1 = 2 * tid + 1;
+    if (r0 < ns && s.spkR[r0] != 0.f) {
+        a0 = d.g.indptr[s.spkI[r0]];
+        len0 = (int32_t)(d.g.indptr[s.spkI[r0] + 1] - a0);
     }
-    if (lane == 0) {
-        atomicAdd(&s.ev, (unsigned long long)(re - ra));
-        atomicAdd(&s.syn, (unsigned long long)d.g.rowSyn[pre]);
+    if (r1 < ns && s.spkR[r1] != 0.f) {
+        a1 = d.g.indptr[s.spkI[r1]];
+        len1 = (int32_t)(d.g.indptr[s.spkI[r1] + 1] - a1);
+    }
+    int32_t total;
+    const int32_t off = wg_exclusive_scan<PT>(len0 + len1, s.scan, &total);
+    if (r0 < ns) { s.rowOff[r0] = off; s.rowA[r0] = a0; }
+    if (r1 < ns) { s.rowOff[r1] = off + len0; s.rowA[r1] = a1; }
+    if (tid == 0) s.rowOff[ns] = total;
+    unsigned long long ev = (unsigned long long)(len0 + len1), syn = 0;
+    if (len0) syn += d.g.rowSyn[s.spkI[r0]];
+    if (len1) syn += d.g.rowSyn[s.spkI[r1]];
+    if (ev) { atomicAdd(&s.ev, ev); atomicAdd(&s.syn, syn); }
+    __syncthreads();
+    const int32_t cap = sb_cap(src);
+    constexpr int U = 4;
+    for (int32_t g0 = tid; g0 < total; g0 += PT * U) {
+        uint2 ed[U];
+        uint32_t own[U];
+        int32_t row[U];
+        uint32_t j[U];
+#pragma unroll
+        for (int v = 0; v < U; v++) {
+            const int32_t g = g0 + v * PT;
+            row[v] = -1;
+            if (g < total) {
+                int32_t lo = 0, hi = ns;   // last row with rowOff <= g
+                while (hi - lo > 1) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (s.rowOff[mid] <= g) lo = mid; else hi = mid;
+                }
+                row[v] = lo;
+                j[v] = s.rowA[lo] + (uint32_t)(g - s.rowOff[lo]);
+                ed[v] = d.g.edges[j[v]];
+                own[v] = a.eown[j[v]];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < U; v++) {
+            if (row[v] < 0) continue;
+            const int32_t r = row[v];
+            const uint32_t kk = s.spkK[r];
+            const float sv = s.spkR[r];
+            const int32_t dst = (int32_t)(own[v] >> 12);
+            const uint32_t ql = own[v] & 4095u;
+            const int32_t pos = atomicAdd(&s.cnt[dst], 1);
+            if (pos < cap) {
+                const int64_t at = sb_base(a, par, src, dst) + pos;
+                a.sbKey[at] = ev_key(kk >> 24, sv > 0 ? 0u : 1u, ql, kk & 0xFFFFFFu, j[v] - s.rowA[r]);
+                a.sbAdd[at] = __uint_as_float(ed[v].y) * sv;
+            }
+        }
     }
 }
 
@@ -231,23 +274,34 @@ __device__ inline void publish_counts(const PersistArgs &a, Shared &s, int32_t p
 
 __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
     const DevBrain &d = a.d;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int32_t me = blockIdx.x;
-    int32_t pref[D_MAX + 1];
     int32_t acc = 0;
-    for (int32_t m = 0; m < L0; m++) { pref[m] = acc; acc += d.ringCount[head_at(a, m)]; }
-    pref[L0] = acc;
-    const int32_t Atot = acc;
-    const int32_t waves = a.S * (PT / 64);
-    for (int32_t g = me * (PT / 64) + wave; g < Atot; g += waves) {
-        int32_t m = 0;
-        while (pref[m + 1] <= g) m++;
-        const int32_t k = g - pref[m];
-        const int32_t pre = d.ring[(int64_t)head_at(a, m) * d.ringStride + k];
-        const float sv = d.g.sign[pre] * res_after(d.k, d.res[pre], m);
-        if (sv == 0.f) continue;
-        if (d.k.depU != 0.f && lane == 0) atomicOr(&a.arrMask[pre], 1u << m);
-        push_row(a, s, 0, me, pre, (uint32_t)m, (uint32_t)k, sv, lane);
+    if (tid == 0) {
+        for (int32_t m = 0; m < L0; m++) { s.pref[m] = acc; acc += d.ringCount[head_at(a, m)]; }
+        s.pref[L0] = acc;
+    }
+    __syncthreads();
+    const int32_t Atot = s.pref[L0];
+    // this block scatters arrivals g = me, me + S, me + 2S, ... in chunks of SPK_CAP rows
+    const int32_t mine = Atot > me ? (Atot - me + a.S - 1) / a.S : 0;
+    for (int32_t c0 = 0; c0 < mine; c0 += SPK_CAP) {
+        const int32_t ns = min(SPK_CAP, mine - c0);
+        for (int32_t r = tid; r < ns; r += PT) {
+            const int32_t g = me + (c0 + r) * a.S;
+            int32_t m = 0;
+            while (s.pref[m + 1] <= g) m++;
+            const int32_t k = g - s.pref[m];
+            const int32_t pre = d.ring[(int64_t)head_at(a, m) * d.ringStride + k];
+            const float sv = d.g.sign[pre] * res_after(d.k, d.res[pre], m);
+            if (sv != 0.f && d.k.depU != 0.f) atomicOr(&a.arrMask[pre], 1u << m);
+            s.spkI[r] = pre;
+            s.spkK[r] = ((uint32_t)m << 24) | (uint32_t)k;
+            s.spkR[r] = sv;
+        }
+        __syncthreads();
+        push_rows(a, s, 0, me, ns);
+        __syncthreads();
     }
     publish_counts(a, s, 0, me);
 }
@@ -370,7 +424,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
     const float forcedRefr = d.k.tRef + d.k.dt;
     uint8_t *dflag = a.dflag + (int64_t)par * D_MAX * DRV_CAP;
     uint8_t *tflag = a.tflag + (int64_t)par * D_MAX * DRV_CAP;
-    const bool timed = tid == 0;
+    const bool timed = tid == 0 && (a.flags & 1);
 #define FB_SUB(k)                                      \
     if (timed) {                                       \
         const uint64_t now = wall_clock64();           \
@@ -457,7 +511,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
 __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t par, int32_t me,
                             bool drivenBlock) {
     const DevBrain &d = a.d;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int32_t P = drivenBlock ? (d.nDriven + PT - 1) / PT : 0;
     if (tid == 0) s.nspk = 0;
     __syncthreads();
@@ -474,20 +528,15 @@ __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[NPT],
                 s.spkI[e] = c.i;
                 s.spkK[e] = ((uint32_t)m << 24) |
                             (forced ? (uint32_t)(tid * P + u) : KEY_THRESHOLD + (uint32_t)c.i);
-                s.spkR[e] = res_after(d.k, c.st.res, m);   // res at its arrival step of the next epoch
+                // sign x res at its arrival step of the next epoch
+                s.spkR[e] = c.sign * res_after(d.k, c.st.res, m);
             }
         }
     }
     __syncthreads();
     const int32_t ns = s.nspk;
     if (tid == 0 && ns > SPK_CAP) atomicOr(&a.ctl[1], 4);
-    for (int32_t e = wave; e < min(ns, SPK_CAP); e += PT / 64) {
-        const int32_t i = s.spkI[e];
-        const uint32_t kk = s.spkK[e];
-        const float sv = d.g.sign[i] * s.spkR[e];
-        if (sv == 0.f) continue;
-        push_row(a, s, par, me, i, kk >> 24, kk & 0xFFFFFFu, sv, lane);
-    }
+    push_rows(a, s, par, me, min(ns, SPK_CAP));
     publish_counts(a, s, par, me);
 }
 
@@ -755,8 +804,11 @@ struct PersistBuffers {
     int32_t *sbCnt = nullptr, *dstTot = nullptr;
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
+    uint32_t *eown = nullptr;
     int32_t N = 0, S = 0;
-    int64_t perPar = 0;
+    int64_t perPar = 0, E = 0;
+    int32_t eownNb = -1;
+    uint64_t eownDriven = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -764,11 +816,14 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dflag, pb.tflag, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
-                  pb.ctl, pb.prof};
+                  pb.ctl, pb.prof, pb.eown};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dflag = pb.tflag = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
+    pb.eown = nullptr;
+    pb.eownNb = -1;
+    pb.eownDriven = ~0ull;
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
@@ -791,8 +846,19 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
     if (q < nD) drvPos[driven[q]] = q;
 }
 
+// destination block << 12 | local index, per edge: driven targets belong to
+// block 0 (index = driven-list position), others to block 1 + q / nb
+__global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos, int32_t nb) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < E; j += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = (int32_t)edges[j].x;
+        const int32_t dp = drvPos[q];
+        const int32_t blk = q / nb;
+        eown[j] = dp >= 0 ? (uint32_t)dp : ((uint32_t)(1 + blk) << 12) | (uint32_t)(q - blk * nb);
+    }
+}
+
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
-                            int64_t *launches, double *kernelMs, double *phaseUs) {
+                            int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion) {
     std::string why;
     fb_status rc = persistent_supported(d, false, &why);
     if (rc) { set_error("persistent engine: %s", why.c_str()); return rc; }
@@ -820,17 +886,25 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.dstTot, sizeof(int32_t) * 2 * (size_t)lay.S));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 18));
+        FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
         }
     }
-    // driven positions (the list may have changed since the last launch)
-    hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
-    if (d.nDriven)
-        hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
-                           d.nDriven);
-    FB_HIP(hipGetLastError());
+    // driven positions and edge owners (again when the driven set or layout changed)
+    if (pb.eownDriven != drivenVersion || pb.eownNb != lay.nb) {
+        hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
+        if (d.nDriven)
+            hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
+                               d.nDriven);
+        if (d.g.E)
+            hipLaunchKernelGGL(k_edge_owner, dim3(1024), dim3(256), 0, st, pb.eown, d.g.edges, d.g.E, pb.drvPos,
+                               lay.nb);
+        FB_HIP(hipGetLastError());
+        pb.eownDriven = drivenVersion;
+        pb.eownNb = lay.nb;
+    }
 
     PersistArgs a{};
     a.d = d;
@@ -851,6 +925,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.dstTot = pb.dstTot;
     a.ctl = pb.ctl;
     a.prof = pb.prof;
+    a.eown = pb.eown;
+    static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
+    a.flags = profileSteps ? 1 : 0;
 
     int32_t done = 0, cur = head;
     while (done < nsteps) {
