@@ -25,7 +25,10 @@
 //           key, position in row) -- lif.c's addition order per target;
 //   simulate  D steps from registers: arrivals (one add, or the sorted run
 //           for several), forced spikes, membrane, threshold.  Threshold
-//           spikes go to a per-step bitmask, forced spikes to per-step flags;
+//           spikes go to a per-step bitmask; the driven block keeps its
+//           neurons' spike steps as bit masks and writes them once per epoch
+//           (a global store inside its step loop would make every workgroup
+//           barrier wait for the store to complete);
 //   push    walks the rows of its own spikes and writes their events into
 //           per-(source, destination) sub-buckets for epoch e+1, allocating
 //           slots with LDS counters (no global atomics per event).
@@ -84,8 +87,8 @@ struct PersistArgs {
     int32_t nSpare;      // spare blocks after them
     int32_t nwords;      // 64-bit words of one step's spike bitmask
     uint64_t *spk;       // [2][D_MAX][nwords] threshold spikes of non-driven neurons
-    uint8_t *dflag;      // [2][D_MAX][DRV_CAP] forced spikes, by driven-list position
-    uint8_t *tflag;      // [2][D_MAX][DRV_CAP] threshold spikes of driven neurons
+    uint32_t *dmask;     // [2][DRV_CAP] steps with a forced spike, by driven-list position
+    uint32_t *tmask;     // [2][DRV_CAP] steps with a threshold spike of a driven neuron
     const int32_t *drvPos;   // [N] position in the driven list, -1 if not driven
     uint32_t *arrMask;   // [N] arrival steps of the first epoch (depression bookkeeping)
     uint64_t *sbKey;     // [2][perPar] sub-buckets
@@ -422,8 +425,6 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
     const int32_t nD = d.nDriven;
     const int32_t P = (nD + PT - 1) / PT;
     const float forcedRefr = d.k.tRef + d.k.dt;
-    uint8_t *dflag = a.dflag + (int64_t)par * D_MAX * DRV_CAP;
-    uint8_t *tflag = a.tflag + (int64_t)par * D_MAX * DRV_CAP;
     const bool timed = tid == 0 && (a.flags & 1);
 #define FB_SUB(k)                                      \
     if (timed) {                                       \
@@ -461,7 +462,6 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
             const bool nr = u == 0 ? nr0 : nr1;
-            uint8_t f = 0;
             if (nr) {
                 // stream position j+1 after rng: M^(j%32+1) (M^(32 (j/32)) rng), or one step past the previous draw
                 x = j == off ? matvec(s.leap + (j & 31) * LEAP_STRIDE, s.ybase[j >> 5]) : xs32(x);
@@ -476,10 +476,8 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
                     c.count++;
                     c.mask |= 1u << m;
                     c.forced |= 1u << m;
-                    f = 1;
                 }
             }
-            if (c.i >= 0) dflag[(int64_t)m * DRV_CAP + tid * P + u] = f;
         }
         FB_SUB(3)
         // lif.c phases 3-4 for the driven neurons
@@ -488,20 +486,26 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
             Cell &c = cells[u];
             if (c.i >= 0) {
                 membrane(d.k, c.st, c.bias);
-                uint8_t t = 0;
                 if (crosses(d.k, c.st, c.thr)) {
                     spike_effects(d.k, c.st, d.k.tRef);
                     c.count++;
                     c.mask |= 1u << m;
-                    t = 1;
                 }
-                tflag[(int64_t)m * DRV_CAP + tid * P + u] = t;
             }
         }
         FB_SUB(4)
     }
     __syncthreads();
     if (nPrev > 0) rng = s.rngNext[(L - 1) & 1];
+    // the epoch's spike steps of every driven neuron, for the list builders
+#pragma unroll
+    for (int u = 0; u < DPT; u++) {
+        const Cell &c = cells[u];
+        if (c.i >= 0) {
+            a.dmask[par * DRV_CAP + tid * P + u] = c.forced;
+            a.tmask[par * DRV_CAP + tid * P + u] = c.mask & ~c.forced;
+        }
+    }
     FB_SUB(5)
 #undef FB_SUB
 }
@@ -548,17 +552,15 @@ __device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t
     const int32_t nD = d.nDriven;
     const int32_t slot = (head_at(a, t) + d.nslots - 1) % d.nslots;
     int32_t *out = d.ring + (int64_t)slot * d.ringStride;
-    const uint8_t *fl = a.dflag + ((int64_t)par * D_MAX + m) * DRV_CAP;
-    const uint8_t *tf = a.tflag + ((int64_t)par * D_MAX + m) * DRV_CAP;
+    const uint32_t *fl = a.dmask + par * DRV_CAP;
+    const uint32_t *tf = a.tmask + par * DRV_CAP;
     uint64_t *bits = a.spk + ((int64_t)par * D_MAX + m) * a.nwords;
-    // driven list position q sits at fl[(q / P) * P + q % P] = fl[q] with the
-    // chunked layout of the driven block (thread t holds positions tP..tP+P-1)
     const int32_t P = (nD + PT - 1) / PT;
     const int32_t dlo = min(nD, tid * P), dhi = min(nD, dlo + P);
     int32_t nf = 0;
     for (int32_t q = dlo; q < dhi; q++) {
-        nf += fl[q];
-        if (tf[q]) {   // threshold spikes of driven neurons join the bitmask
+        nf += (fl[q] >> m) & 1u;
+        if ((tf[q] >> m) & 1u) {   // threshold spikes of driven neurons join the bitmask
             const int32_t i = d.driven[q];
             atomicOr((unsigned long long *)&bits[i >> 6], 1ull << (i & 63));
         }
@@ -566,7 +568,7 @@ __device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t
     int32_t ndTot;
     int32_t pos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
     for (int32_t q = dlo; q < dhi; q++)
-        if (fl[q]) out[pos++] = d.driven[q];
+        if ((fl[q] >> m) & 1u) out[pos++] = d.driven[q];
     __threadfence_block();
     __syncthreads();
     const int32_t W = (a.nwords + PT - 1) / PT;
@@ -796,7 +798,7 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
 
 struct PersistBuffers {
     uint64_t *spk = nullptr;
-    uint8_t *dflag = nullptr, *tflag = nullptr;
+    uint32_t *dmask = nullptr, *tmask = nullptr;
     int32_t *drvPos = nullptr;
     uint32_t *arrMask = nullptr;
     uint64_t *sbKey = nullptr;
@@ -815,11 +817,11 @@ struct PersistBuffers {
 static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain's driven buffer
 
 static void free_buffers(PersistBuffers &pb) {
-    void *ps[] = {pb.spk, pb.dflag, pb.tflag, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
+    void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
                   pb.ctl, pb.prof, pb.eown};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    pb.spk = nullptr; pb.dflag = pb.tflag = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
+    pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
     pb.eownNb = -1;
@@ -875,8 +877,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         pb.perPar = perPar;
         FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords));
         FB_HIP(hipMemsetAsync(pb.spk, 0, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords, st));
-        FB_HIP(hipMalloc(&pb.dflag, (size_t)2 * D_MAX * DRV_CAP));
-        FB_HIP(hipMalloc(&pb.tflag, (size_t)2 * D_MAX * DRV_CAP));
+        FB_HIP(hipMalloc(&pb.dmask, sizeof(uint32_t) * 2 * DRV_CAP));
+        FB_HIP(hipMalloc(&pb.tmask, sizeof(uint32_t) * 2 * DRV_CAP));
         FB_HIP(hipMalloc(&pb.drvPos, sizeof(int32_t) * (size_t)d.N));
         FB_HIP(hipMalloc(&pb.arrMask, sizeof(uint32_t) * (size_t)d.N));
         FB_HIP(hipMemsetAsync(pb.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
@@ -914,8 +916,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.nSpare = lay.nSpare;
     a.nwords = nwords;
     a.spk = pb.spk;
-    a.dflag = pb.dflag;
-    a.tflag = pb.tflag;
+    a.dmask = pb.dmask;
+    a.tmask = pb.tmask;
     a.drvPos = pb.drvPos;
     a.arrMask = pb.arrMask;
     a.sbKey = pb.sbKey;
