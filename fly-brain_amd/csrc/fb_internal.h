@@ -151,17 +151,22 @@ __device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap)
     for (int q = threadIdx.x; q < LEAP_MATS * LEAP_WORDS; q += T) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = leap[q];
 }
 
-// Exclusive scan over a T-thread workgroup of per-thread counts given as two
-// flags (each 0 or 1): ballots and popcounts inside a wave, wave totals
-// through LDS.  sh needs T/64 entries; one barrier.  Every thread must call it,
-// and sh must not be rewritten before every thread has returned.
-template <int T>
-__device__ __forceinline__ int32_t wg_scan_flags2(bool f0, bool f1, int32_t *sh, int32_t *total) {
+// Exclusive scan over a T-thread workgroup of per-thread counts given as K
+// flag bits (bit k of `bits`): ballots and popcounts inside a wave, wave
+// totals through LDS.  sh needs T/64 entries; one barrier.  Every thread must
+// call it, and sh must not be rewritten before every thread has returned.
+template <int T, int K>
+__device__ __forceinline__ int32_t wg_scan_flags(uint32_t bits, int32_t *sh, int32_t *total) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t m0 = __ballot(f0), m1 = __ballot(f1);
     const uint64_t lt = (1ull << lane) - 1ull;
-    const int32_t inWave = __popcll(m0 & lt) + __popcll(m1 & lt);
-    if (lane == 0) sh[wave] = __popcll(m0) + __popcll(m1);
+    int32_t inWave = 0, waveTotal = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint64_t mk = __ballot((bits >> k) & 1u);
+        inWave += __popcll(mk & lt);
+        waveTotal += __popcll(mk);
+    }
+    if (lane == 0) sh[wave] = waveTotal;
     __syncthreads();
     int32_t before = 0, all = 0;
 #pragma unroll

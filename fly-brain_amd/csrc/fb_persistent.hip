@@ -15,7 +15,7 @@
 //                   nibble-table product and xorshift steps (M = one
 //                   xorshift32 step as a matrix over GF(2)).
 //   blocks 1..R     regular blocks: block b owns a contiguous, 64-aligned
-//                   range of up to 2048 neurons (2 per thread, in registers).
+//                   range of up to 1024 neurons (2 per thread, in registers).
 //   blocks R+1..    spare blocks (when the grid has them): they turn the
 //                   previous epoch's spikes into ring lists in lif.c order.
 //
@@ -39,7 +39,7 @@
 // the delay line (from the per-step engines or the previous launch) into the
 // sub-buckets, keyed by list position.  Preconditions (checked on the host;
 // otherwise the per-step engines run): no background events, no presynaptic
-// inhibition, D <= 32, at most 2048 driven neurons and 2048 neurons per
+// inhibition, D <= 32, at most 2048 driven neurons and 1024 neurons per
 // block, and with synaptic depression live a refractory period of at least D
 // steps (a neuron then arrives at most once per epoch).  A sub-bucket or LDS
 // overflow stops the launch at an epoch boundary; the host runs one epoch on
@@ -56,18 +56,22 @@ namespace cg = cooperative_groups;
 
 namespace fb {
 
-constexpr int PT = 1024;            // threads per block
+// 512 threads per block: the kernel may then use 256 VGPRs per lane, which
+// keeps the driven block's per-step state out of scratch memory.
+constexpr int PT = 512;             // threads per block
 constexpr int NPT = 2;              // neurons per thread in a regular block
 constexpr int NB_MAX = PT * NPT;    // neurons per regular block
 constexpr int D_MAX = 32;           // epoch length bound
 constexpr int EV_CAP = 4096;        // events per destination block per epoch (LDS)
 constexpr int SPK_CAP = 2048;       // spikes per block per epoch (LDS push list)
-constexpr int DPT = 2;              // driven neurons per thread in block 0 (wg_scan_flags2 takes two)
+constexpr int RPT = SPK_CAP / PT;   // push rows per thread in the row scan
+constexpr int DPT = 4;              // driven neurons per thread in block 0
 constexpr int DRV_CAP = PT * DPT;
-constexpr int S_MAX = 1024;         // owning blocks (driven + regular)
+constexpr int CELLS = DPT > NPT ? DPT : NPT;
+constexpr int S_MAX = 512;          // owning blocks (driven + regular)
 constexpr int CAP_DRV = 1024;       // sub-bucket slots from the driven block to one destination
 constexpr int CAP_REG = 128;        // sub-bucket slots from a regular block to one destination
-constexpr int WPS = NB_MAX / 32;    // words of one step in the hasEv bitmap
+constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
 constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
 
 // event key: step(5) | class(1) | local target(12) | arrival key(20) | position in row(26)
@@ -186,27 +190,33 @@ __device__ __forceinline__ int32_t sb_cap(int32_t src) { return src == 0 ? CAP_D
 __device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    // row lengths -> prefix (two rows per thread)
-    int32_t len0 = 0, len1 = 0;
-    uint32_t a0 = 0, a1 = 0;
-    const int32_t r0 = 2 * tid, r1 = 2 * tid + 1;
-    if (r0 < ns && s.spkR[r0] != 0.f) {
-        a0 = d.g.indptr[s.spkI[r0]];
-        len0 = (int32_t)(d.g.indptr[s.spkI[r0] + 1] - a0);
-    }
-    if (r1 < ns && s.spkR[r1] != 0.f) {
-        a1 = d.g.indptr[s.spkI[r1]];
-        len1 = (int32_t)(d.g.indptr[s.spkI[r1] + 1] - a1);
+    // row lengths -> prefix (RPT consecutive rows per thread)
+    int32_t len[RPT];
+    uint32_t ra[RPT];
+    int32_t mySum = 0;
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        len[v] = 0;
+        ra[v] = 0;
+        if (r < ns && s.spkR[r] != 0.f) {
+            ra[v] = d.g.indptr[s.spkI[r]];
+            len[v] = (int32_t)(d.g.indptr[s.spkI[r] + 1] - ra[v]);
+        }
+        mySum += len[v];
     }
     int32_t total;
-    const int32_t off = wg_exclusive_scan<PT>(len0 + len1, s.scan, &total);
-    if (r0 < ns) { s.rowOff[r0] = off; s.rowA[r0] = a0; }
-    if (r1 < ns) { s.rowOff[r1] = off + len0; s.rowA[r1] = a1; }
+    int32_t off = wg_exclusive_scan<PT>(mySum, s.scan, &total);
+    unsigned long long syn = 0;
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        if (r < ns) { s.rowOff[r] = off; s.rowA[r] = ra[v]; }
+        if (len[v]) syn += d.g.rowSyn[s.spkI[r]];
+        off += len[v];
+    }
     if (tid == 0) s.rowOff[ns] = total;
-    unsigned long long ev = (unsigned long long)(len0 + len1), syn = 0;
-    if (len0) syn += d.g.rowSyn[s.spkI[r0]];
-    if (len1) syn += d.g.rowSyn[s.spkI[r1]];
-    if (ev) { atomicAdd(&s.ev, ev); atomicAdd(&s.syn, syn); }
+    if (mySum) { atomicAdd(&s.ev, (unsigned long long)mySum); atomicAdd(&s.syn, syn); }
     __syncthreads();
     const int32_t cap = sb_cap(src);
     constexpr int U = 4;
@@ -315,7 +325,7 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
     const int tid = threadIdx.x;
     const int32_t *cnt = a.sbCnt + (int64_t)par * a.S * a.S + (int64_t)me * a.S;
     int32_t total = 0;
-    // S <= 1024: one count per thread
+    // S <= S_MAX = PT: one count per thread
     const int32_t c = tid < a.S ? cnt[tid] : 0;
     const int32_t off = wg_exclusive_scan<PT>(c, s.scan, &total);
     if (tid < a.S) s.pref[tid] = off;
@@ -387,7 +397,7 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
 
 // ------------------------------------------------------------------ simulate
 
-__device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t nev, int32_t par,
+__device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t nev, int32_t par,
                             int32_t L, int32_t lo) {
     const DevBrain &d = a.d;
     const int lane = threadIdx.x & 63;
@@ -418,7 +428,7 @@ __device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[NPT],
     }
 }
 
-__device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], const float (&pFire)[DPT],
+__device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
                            int32_t nev, int32_t par, int32_t L, uint32_t &rng) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
@@ -436,20 +446,19 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
     int32_t nPrev = 0;
     for (int32_t m = 0; m < L; m++) {
         // lif.c phase 1: arrivals for the driven neurons
-        bool nr0 = false, nr1 = false;
+        uint32_t nr = 0;   // bit u: cell u is not refractory and draws
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
             if (c.i >= 0) {
                 cell_inputs(d.k, c, s, nev, m, (uint32_t)(tid * P + u));
-                const bool nr = !(c.st.refr > 0);
-                if (u == 0) nr0 = nr; else nr1 = nr;
+                if (!(c.st.refr > 0)) nr |= 1u << u;
             }
         }
         FB_SUB(0)
         // lif.c phase 2: one draw per non-refractory driven neuron, in list order
         int32_t n;
-        const int32_t off = wg_scan_flags2<PT>(nr0, nr1, s.scan + (m & 1) * 16, &n);
+        const int32_t off = wg_scan_flags<PT, DPT>(nr, s.scan + (m & 1) * 16, &n);
         if (nPrev > 0) rng = s.rngNext[(m - 1) & 1];   // the last draw of the previous step
         nPrev = n;
         FB_SUB(1)
@@ -461,8 +470,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
-            const bool nr = u == 0 ? nr0 : nr1;
-            if (nr) {
+            if ((nr >> u) & 1u) {
                 // stream position j+1 after rng: M^(j%32+1) (M^(32 (j/32)) rng), or one step past the previous draw
                 x = j == off ? matvec(s.leap + (j & 31) * LEAP_STRIDE, s.ybase[j >> 5]) : xs32(x);
                 if (j == n - 1) s.rngNext[m & 1] = x;
@@ -512,7 +520,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[DPT], 
 
 // ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
 
-__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[NPT], int32_t par, int32_t me,
+__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me,
                             bool drivenBlock) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
@@ -520,7 +528,7 @@ __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[NPT],
     if (tid == 0) s.nspk = 0;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < NPT; u++) {
+    for (int u = 0; u < CELLS; u++) {
         const Cell &c = cells[u];
         uint32_t bits = c.mask;
         while (bits) {
@@ -633,9 +641,9 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         s.ph[6] = now;                                 \
     }
 
-    if (tid < S_MAX) s.cnt[tid] = 0;
+    for (int r = tid; r < S_MAX; r += PT) s.cnt[r] = 0;
     if (tid == 0) { s.ev = 0; s.syn = 0; s.rngNext[0] = s.rngNext[1] = 0; }
-    Cell cells[NPT];
+    Cell cells[CELLS];
     float pFire[DPT];
     uint32_t rng = 0;
     if (drivenBlock) {
@@ -651,9 +659,9 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         rng = d.sc->rng;
     } else {
 #pragma unroll
-        for (int u = 0; u < NPT; u++) {
+        for (int u = 0; u < CELLS; u++) {
             const int32_t i = lo + tid + u * PT;
-            load_cell(d, cells[u], (owning && i < hi && a.drvPos[i] < 0) ? i : -1);
+            load_cell(d, cells[u], (u < NPT && owning && i < hi && a.drvPos[i] < 0) ? i : -1);
         }
     }
     __syncthreads();
@@ -665,7 +673,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     FB_PHASE(4)
     if (d.k.depU != 0.f && owning) {
 #pragma unroll
-        for (int u = 0; u < NPT; u++)
+        for (int u = 0; u < CELLS; u++)
             if (cells[u].i >= 0) { cells[u].prev = a.arrMask[cells[u].i]; a.arrMask[cells[u].i] = 0u; }
     }
 
@@ -684,7 +692,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 FB_PHASE(1)
                 if (done + L < a.nsteps) push_spikes(a, s, cells, par ^ 1, me, drivenBlock);
 #pragma unroll
-                for (int u = 0; u < NPT; u++) {
+                for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
                     cells[u].mask = 0;
                     cells[u].forced = 0;
@@ -704,7 +712,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     FB_PHASE(3)
     if (owning) {
 #pragma unroll
-        for (int u = 0; u < NPT; u++) store_cell(d, cells[u]);
+        for (int u = 0; u < CELLS; u++) store_cell(d, cells[u]);
         if (drivenBlock && tid == 0) d.sc->rng = rng;
     }
 #undef FB_PHASE
@@ -792,7 +800,7 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     fb_status st = persist_init();
     if (st) { *why = "cooperative launch unavailable"; return st; }
     Layout l;
-    if (!layout_for(d, D, l)) { *why = "more than 2048 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
+    if (!layout_for(d, D, l)) { *why = "more than 1024 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
     return FB_OK;
 }
 
