@@ -62,7 +62,7 @@ constexpr int PT = 512;             // threads per block
 constexpr int NPT = 2;              // neurons per thread in a regular block
 constexpr int NB_MAX = PT * NPT;    // neurons per regular block
 constexpr int D_MAX = 32;           // epoch length bound
-constexpr int EV_CAP = 4096;        // events per destination block per epoch (LDS)
+constexpr int EV_CAP = 2048;        // events per destination block per epoch (LDS)
 constexpr int SPK_CAP = 2048;       // spikes per block per epoch (LDS push list)
 constexpr int RPT = SPK_CAP / PT;   // push rows per thread in the row scan
 constexpr int DPT = 4;              // driven neurons per thread in block 0
@@ -74,12 +74,14 @@ constexpr int CAP_REG = 128;        // sub-bucket slots from a regular block to 
 constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
 constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
 
-// event key: step(5) | class(1) | local target(12) | arrival key(20) | position in row(26)
+// event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
+// Sorted, the events of one (step, target) are contiguous, excitatory first,
+// each class in lif.c's addition order.
 __device__ __forceinline__ uint64_t ev_key(uint32_t m, uint32_t c, uint32_t ql, uint32_t k, uint32_t jr) {
-    return ((uint64_t)m << 59) | ((uint64_t)c << 58) | ((uint64_t)ql << 46) | ((uint64_t)k << 26) | (uint64_t)jr;
+    return ((uint64_t)m << 59) | ((uint64_t)ql << 47) | ((uint64_t)c << 46) | ((uint64_t)k << 26) | (uint64_t)jr;
 }
-__device__ __forceinline__ uint32_t ev_group(uint64_t key) { return (uint32_t)(key >> 46); }
-__device__ __forceinline__ uint32_t group_of(uint32_t m, uint32_t c, uint32_t ql) { return (m << 13) | (c << 12) | ql; }
+__device__ __forceinline__ uint32_t ev_slot(uint64_t key) { return (uint32_t)(key >> 47); }   // step << 12 | target
+__device__ __forceinline__ uint32_t ev_class(uint64_t key) { return (uint32_t)(key >> 46) & 1u; }
 
 struct PersistArgs {
     DevBrain d;
@@ -109,7 +111,9 @@ struct PersistArgs {
 struct Shared {
     uint64_t evKey[EV_CAP];
     float evAdd[EV_CAP];
-    uint32_t hasEv[D_MAX * WPS];
+    uint32_t hasEv[D_MAX * WPS];     // bit (step, target): the target has arrivals at that step
+    int32_t evRank[D_MAX * WPS];     // set bits of hasEv before each word
+    uint16_t evStart[EV_CAP];        // first sorted event of the r-th (step, target) with arrivals
     uint32_t leap[LEAP_LDS_WORDS];
     uint32_t ybase[64];
     int32_t pref[S_MAX + 1];
@@ -152,18 +156,60 @@ __device__ inline void lds_sort(Shared &s, int32_t n) {
     }
 }
 
-__device__ __forceinline__ int32_t find_group(const Shared &s, int32_t n, uint32_t g) {
-    int32_t a = 0, e = n;
-    while (a < e) {
-        const int32_t mid = (a + e) >> 1;
-        if (ev_group(s.evKey[mid]) < g) a = mid + 1; else e = mid;
+// Sorts evKey/evAdd[0, n) (keys are distinct).  Up to PT events: each thread
+// counts the smaller keys of its event and scatters it to that rank, through
+// the staging halves of the push list; beyond that, a bitonic sort.
+__device__ inline void sort_events(Shared &s, int32_t n) {
+    if (n > PT) { lds_sort(s, n); return; }
+    const int tid = threadIdx.x;
+    uint64_t mine = 0;
+    float add = 0.f;
+    int32_t rank = 0;
+    if (tid < n) {
+        mine = s.evKey[tid];
+        add = s.evAdd[tid];
+        for (int32_t q = 0; q < n; q++) rank += s.evKey[q] < mine;
     }
-    return a;
+    __syncthreads();
+    if (tid < n) { s.evKey[rank] = mine; s.evAdd[rank] = add; }
+    __syncthreads();
 }
 
-__device__ __forceinline__ float apply_group(const Shared &s, int32_t n, uint32_t g, float acc) {
-    for (int32_t q = find_group(s, n, g); q < n && ev_group(s.evKey[q]) == g; q++) acc = acc + s.evAdd[q];
-    return acc;
+// After the sort: evStart[r] = first event of the r-th (step, target) with
+// arrivals, r counted in hasEv bit order, so a neuron finds its run in O(1).
+__device__ inline void index_events(Shared &s, int32_t n) {
+    const int tid = threadIdx.x;
+    constexpr int W = D_MAX * WPS, PER = W / PT;
+    int32_t cnt = 0;
+#pragma unroll
+    for (int v = 0; v < PER; v++) cnt += __popc(s.hasEv[tid * PER + v]);
+    int32_t total;
+    int32_t off = wg_exclusive_scan<PT>(cnt, s.scan, &total);
+#pragma unroll
+    for (int v = 0; v < PER; v++) { s.evRank[tid * PER + v] = off; off += __popc(s.hasEv[tid * PER + v]); }
+    __syncthreads();
+    for (int32_t g = tid; g < n; g += PT) {
+        const uint32_t sl = ev_slot(s.evKey[g]);
+        if (g == 0 || ev_slot(s.evKey[g - 1]) != sl) {
+            const uint32_t m = sl >> 12, ql = sl & 4095u;
+            const int32_t w = (int32_t)(m * WPS + (ql >> 5));
+            const int32_t r = s.evRank[w] + __popc(s.hasEv[w] & ((1u << (ql & 31)) - 1u));
+            s.evStart[r] = (uint16_t)g;
+        }
+    }
+    __syncthreads();
+}
+
+// This step's arrivals on one neuron: the contiguous run of its (step, target)
+__device__ __forceinline__ void apply_run(const Shared &s, int32_t n, uint32_t m, uint32_t ql, float &gE, float &gI) {
+    const int32_t w = (int32_t)(m * WPS + (ql >> 5));
+    const int32_t r = s.evRank[w] + __popc(s.hasEv[w] & ((1u << (ql & 31)) - 1u));
+    const uint32_t sl = (m << 12) | ql;
+    for (int32_t q = s.evStart[r]; q < n; q++) {
+        const uint64_t key = s.evKey[q];
+        if (ev_slot(key) != sl) break;
+        if (ev_class(key)) gI = gI + s.evAdd[q]; else gE = gE + s.evAdd[q];
+    }
 }
 
 // res at the start of epoch step m from its value at the epoch start: m recoveries
@@ -343,13 +389,13 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
         const uint64_t key = a.sbKey[at];
         s.evKey[g] = key;
         s.evAdd[g] = a.sbAdd[at];
-        const uint32_t grp = ev_group(key);
-        const uint32_t m = grp >> 13, ql = grp & 4095u;
+        const uint32_t sl = ev_slot(key);
+        const uint32_t m = sl >> 12, ql = sl & 4095u;
         atomicOr(&s.hasEv[m * WPS + (ql >> 5)], 1u << (ql & 31));
     }
     __syncthreads();
-    if (n > 1) lds_sort(s, n);
-    __syncthreads();
+    if (n > 1) sort_events(s, n);
+    index_events(s, n);
     return n;
 }
 
@@ -389,10 +435,7 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
     if ((c.prev >> m) & 1u) {
         if (c.sign * c.st.res != 0.f) c.st.res = c.st.res - k.depU * c.st.res;
     }
-    if ((s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u) {
-        c.st.gE = apply_group(s, nev, group_of((uint32_t)m, 0u, ql), c.st.gE);
-        c.st.gI = apply_group(s, nev, group_of((uint32_t)m, 1u, ql), c.st.gI);
-    }
+    if ((s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u) apply_run(s, nev, (uint32_t)m, ql, c.st.gE, c.st.gI);
 }
 
 // ------------------------------------------------------------------ simulate
