@@ -106,10 +106,6 @@ struct PersistArgs {
     uint64_t *prof;      // phase clocks (see k_persist)
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
     int32_t flags;       // bit 0: time the driven block's step parts
-    int32_t *dspkI;      // [SPK_CAP] the driven block's spikes of the epoch, handed to the
-    uint32_t *dspkK;     //           regular blocks, which push them (see push_spikes)
-    float *dspkR;
-    int32_t *dspkN;      // [0] count, [1] epoch stamp (epoch + 1 when the list is complete)
 };
 
 struct Shared {
@@ -568,7 +564,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
 // ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
 
 __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me,
-                            bool drivenBlock, int32_t epoch) {
+                            bool drivenBlock) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t P = drivenBlock ? (d.nDriven + PT - 1) / PT : 0;
@@ -593,47 +589,9 @@ __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS
         }
     }
     __syncthreads();
-    int32_t ns = s.nspk;
+    const int32_t ns = s.nspk;
     if (tid == 0 && ns > SPK_CAP) atomicOr(&a.ctl[1], 4);
-    ns = min(ns, SPK_CAP);
-    if (drivenBlock) {
-        // Hand the list to the regular blocks, which are idle until the
-        // barrier; the driven block itself pushes nothing.
-        for (int32_t r = tid; r < ns; r += PT) { a.dspkI[r] = s.spkI[r]; a.dspkK[r] = s.spkK[r]; a.dspkR[r] = s.spkR[r]; }
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) {
-            __hip_atomic_store(&a.dspkN[0], ns, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            __hip_atomic_store(&a.dspkN[1], epoch + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        ns = 0;
-    } else {
-        // take rows (me-1), (me-1) + R, ... of the driven block's list, R = regular blocks
-        if (tid == 0) {
-            int32_t spins = 0;
-            while (__hip_atomic_load(&a.dspkN[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch + 1) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > (1 << 22)) { atomicOr(&a.ctl[1], 8); break; }   // never wait forever
-            }
-        }
-        __syncthreads();
-        __threadfence();
-        const int32_t nd = __hip_atomic_load(&a.dspkN[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int32_t R = a.S - 1;
-        const int32_t mineD = nd > me - 1 ? (nd - (me - 1) + R - 1) / R : 0;
-        const int32_t take = min(mineD, SPK_CAP - ns);
-        if (tid == 0 && take < mineD) atomicOr(&a.ctl[1], 4);
-        for (int32_t t = tid; t < take; t += PT) {
-            const int32_t r = (me - 1) + t * R;
-            s.spkI[ns + t] = __hip_atomic_load(&a.dspkI[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s.spkK[ns + t] = __hip_atomic_load(&a.dspkK[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s.spkR[ns + t] = __hip_atomic_load(&a.dspkR[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        ns += take;
-        __syncthreads();
-    }
-    push_rows(a, s, par, me, ns);
+    push_rows(a, s, par, me, min(ns, SPK_CAP));
     publish_counts(a, s, par, me);
 }
 
@@ -775,7 +733,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
-                if (done + L < a.nsteps) push_spikes(a, s, cells, par ^ 1, me, drivenBlock, e);
+                if (done + L < a.nsteps) push_spikes(a, s, cells, par ^ 1, me, drivenBlock);
 #pragma unroll
                 for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
@@ -900,9 +858,6 @@ struct PersistBuffers {
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
     uint32_t *eown = nullptr;
-    int32_t *dspkI = nullptr, *dspkN = nullptr;
-    uint32_t *dspkK = nullptr;
-    float *dspkR = nullptr;
     int32_t N = 0, S = 0;
     int64_t perPar = 0, E = 0;
     int32_t eownNb = -1;
@@ -914,13 +869,12 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
-                  pb.ctl, pb.prof, pb.eown, pb.dspkI, pb.dspkK, pb.dspkR, pb.dspkN};
+                  pb.ctl, pb.prof, pb.eown};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
-    pb.dspkI = nullptr; pb.dspkK = nullptr; pb.dspkR = nullptr; pb.dspkN = nullptr;
     pb.eownNb = -1;
     pb.eownDriven = ~0ull;
 }
@@ -986,10 +940,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 18));
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
-        FB_HIP(hipMalloc(&pb.dspkI, sizeof(int32_t) * SPK_CAP));
-        FB_HIP(hipMalloc(&pb.dspkK, sizeof(uint32_t) * SPK_CAP));
-        FB_HIP(hipMalloc(&pb.dspkR, sizeof(float) * SPK_CAP));
-        FB_HIP(hipMalloc(&pb.dspkN, sizeof(int32_t) * 2));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
@@ -1029,10 +979,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.ctl = pb.ctl;
     a.prof = pb.prof;
     a.eown = pb.eown;
-    a.dspkI = pb.dspkI;
-    a.dspkK = pb.dspkK;
-    a.dspkR = pb.dspkR;
-    a.dspkN = pb.dspkN;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
@@ -1043,7 +989,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
         FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 18, st));
         FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
-        FB_HIP(hipMemsetAsync(pb.dspkN, 0, sizeof(int32_t) * 2, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st));
@@ -1059,10 +1004,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             uint64_t prof[18];
             FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
             for (int k = 0; k < 18; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
-        }
-        if (ctl[1] & 8) {
-            set_error("lookahead engine: the driven block's spike hand-off timed out");
-            return FB_ERR_DEVICE;
         }
         done += ctl[0];
         cur = (cur + ctl[0]) % d.nslots;
