@@ -198,6 +198,11 @@ struct fb_brain {
 
 extern "C" const char *fb_last_error(void) { return g_err.c_str(); }
 
+namespace fb {
+const DevGraph *graph_dev(const fb_graph *g) { return &g->d; }
+fb_status leap_tables(uint32_t **jump, uint32_t **leap) { return jump_table(jump, leap); }
+}  // namespace fb
+
 #define FB_REQUIRE(cond, code, ...)      \
     do {                                 \
         if (!(cond)) {                   \

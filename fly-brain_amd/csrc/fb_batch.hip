@@ -1,0 +1,618 @@
+// fb_batch.hip -- B ensemble members of one connectome in one engine.
+//
+// The reference runs one brain per worker over a shared read-only graph
+// (src/brainsetup.js:8-27) and its Bend port runs one member per lane
+// (kernel.bend:1-17, 293-378).  Here member b of up to 64 is lane b of a
+// wavefront: per-member state is laid out [N][B], so one wave advances one
+// neuron in every member with coalesced loads, and a step's spikes are one
+// 64-bit ballot per neuron.  Each member is exactly a single brain with
+//   seed            its own ((seed * 2654435761) >>> 0 || 1 xorshift32 stream)
+//   weights         W[j] = f32(count[j] * gain_b[type(pre)])  (writeGraph's
+//                   outScale rule, src/lifwasm.js:23,31, with a per-type gain)
+//   thr             its own per-neuron threshold offsets (silencing is
+//                   thr = 1e6, scripts/lif_ensemble.mjs:35)
+//   run length      it stops (state frozen, no draws) after its own step count
+// and all members share the driven list and drive rates.
+//
+// Per step:
+//   kb_deliver   every presynaptic neuron with a nonzero arrival mask walks
+//                its row once (lanes = edges); for each set bit b the edge
+//                events go into member b's per-(sign, target) accumulators with
+//                the ordered-sum scheme of the single brain (bitmask x CSR).
+//   kb_driven    member b's background and forced draws: per-wave counts of
+//                non-refractory driven neurons, a prefix across waves per lane,
+//                and a leapfrog jump of member b's stream to its offset.
+//   kb_membrane  wave = neuron, lane = member: arrivals, membrane, threshold,
+//                ballot; neurons with any spike are appended to the slot's
+//                active list.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fb_internal.h"
+
+namespace fb {
+
+uint32_t seed_hash(double seed);
+void derive_consts(const fb_params &p, int32_t N, Consts &c);
+int32_t nslots_of(const fb_params &p);
+
+constexpr int BMAX = 64;
+
+struct BatchDev {
+    int32_t N, B, nslots, nDriven, nTypes;
+    Consts k;
+    DevGraph g;
+    // [N][B]
+    float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr;
+    uint32_t *spikeCount;
+    float *drive;                  // [N], shared by every member
+    const int32_t *driven;         // [nDriven]
+    const int32_t *drvPos;         // [N]
+    const int32_t *typeOf;         // [N]
+    const float *gain;             // [B][nTypes]
+    uint32_t *rng;                 // [B]
+    float *bgAcc;                  // [1] (the same for every member)
+    int32_t *steps;                // [B] run length of each member
+    // delay line: per slot, spike and forced masks per neuron and a list of
+    // neurons with a nonzero mask
+    uint64_t *mask, *fmask;        // [nslots][N]
+    int32_t *active, *nActive;     // [nslots][N], [nslots]
+    // accumulators [2N][B] and the chain pool
+    int32_t *cnt, *chain;
+    float *add;
+    uint64_t *key;
+    CollEntry *coll;
+    int64_t collCap;
+    int32_t *ncoll;
+    int32_t *bgCnt;                // [N][B]
+    const uint32_t *leap;
+    int32_t *err;
+    unsigned long long *counters;  // [0] edge events, [1] synapse events, [2] spikes
+};
+
+__device__ __forceinline__ uint64_t lanes_active(const BatchDev &d, int32_t t) {
+    // members still running at launch step t
+    const int lane = threadIdx.x & 63;
+    const bool on = lane < d.B && t < d.steps[lane];
+    return __ballot(on);
+}
+
+// ------------------------------------------------------------------ deliver
+
+__global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int32_t t) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint64_t on = lanes_active(d, t);
+    const int32_t A = d.nActive[head];
+    const int32_t *list = d.active + (int64_t)head * d.N;
+    const int32_t N = d.N, B = d.B;
+    for (int32_t k = wave; k < A; k += nwaves) {
+        const int32_t pre = list[k];
+        uint64_t bits = d.mask[(int64_t)head * N + pre] & on;
+        const uint64_t forced = d.fmask[(int64_t)head * N + pre];
+        const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
+        const float sgn = d.g.sign[pre];
+        const int32_t typ = d.typeOf[pre];
+        const int32_t dp = d.drvPos[pre];
+        while (bits) {
+            const int b = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int64_t ib = (int64_t)pre * B + b;
+            const float s = sgn * d.res[ib];
+            if (s == 0.f) continue;
+            // arrival rank in member b's lif.c list: forced spikes (driven
+            // order) before threshold spikes (ascending index)
+            const uint32_t rank = ((forced >> b) & 1ull) ? (uint32_t)dp : (uint32_t)N + (uint32_t)pre;
+            if (lane == 0) {
+                d.res[ib] = d.res[ib] - d.k.depU * d.res[ib];
+                atomicAdd(&d.counters[0], (unsigned long long)(re - ra));
+                atomicAdd(&d.counters[1], (unsigned long long)d.g.rowSyn[pre]);
+            }
+            const float g = d.gain[(int64_t)b * d.nTypes + typ];
+            const int32_t cls = s > 0 ? 0 : N;
+            for (uint32_t j = ra + lane; j < re; j += 64) {
+                const uint32_t q = d.g.edges[j].x;
+                const float w = (float)d.g.counts[j] * g;
+                const float addend = w * s;
+                const int64_t ci = ((int64_t)cls + q) * B + b;
+                const uint64_t kk = ((uint64_t)rank << 32) | (uint64_t)(j - ra);
+                const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+                if (slot == 0) {
+                    d.add[ci] = addend;
+                    d.key[ci] = kk;
+                } else {
+                    const int32_t p = atomicAdd(d.ncoll, 1);
+                    if (p < d.collCap) {
+                        d.coll[p].key = kk;
+                        d.coll[p].addend = addend;
+                        d.coll[p].next = atomicExch(&d.chain[ci], p);
+                    } else {
+                        atomicOr(d.err, FB_DEVERR_COLL_OVERFLOW);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ driven
+
+__global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int32_t t) {
+    constexpr int T = 1024, W = T / 64;
+    __shared__ uint32_t leapS[LEAP_LDS_WORDS];
+    __shared__ int32_t cnt[W][64];
+    __shared__ int32_t s_nbg;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    leap_load<T>(leapS, d.leap);
+    if (tid == 0) {
+        float acc = d.bgAcc[0] + d.k.bgEvents;
+        int32_t nbg = (int32_t)acc;
+        acc = acc - (float)nbg;
+        d.bgAcc[0] = acc;
+        // the leapfrog tables reach 2047 draws per step, background and forced together
+        if (nbg + d.nDriven > 2047) { atomicOr(d.err, 2); nbg = 0; }
+        s_nbg = nbg;
+    }
+    __syncthreads();
+    const bool on = lane < d.B && t < d.steps[lane];
+    uint32_t x0 = on ? d.rng[lane] : 0u;
+    const int32_t nbg = s_nbg;
+    // background: draws 1..nbg of each running member's stream (lif.c:48);
+    // wave w takes draws w+1, w+1+W, ... via x -> M^(p) x
+    if (on && nbg > 0) {
+        for (int32_t p = wave + 1; p <= nbg; p += W) {
+            const uint32_t y = matvec(leapS + (32 + (p >> 5)) * LEAP_STRIDE, x0);   // M^(32 (p/32)) x
+            const uint32_t z = (p & 31) ? matvec(leapS + ((p & 31) - 1) * LEAP_STRIDE, y) : y;
+            atomicAdd(&d.bgCnt[(int64_t)(z % (uint32_t)d.N) * d.B + lane], 1);
+        }
+        // state after nbg draws
+        const uint32_t y = matvec(leapS + (32 + (nbg >> 5)) * LEAP_STRIDE, x0);
+        x0 = (nbg & 31) ? matvec(leapS + ((nbg & 31) - 1) * LEAP_STRIDE, y) : y;
+    }
+    // forced spikes: wave w owns driven positions [lo, hi) for every member
+    const int32_t nD = d.nDriven;
+    const int32_t per = (nD + W - 1) / W;
+    const int32_t lo = min(nD, wave * per), hi = min(nD, lo + per);
+    int32_t mine = 0;
+    if (on)
+        for (int32_t q = lo; q < hi; q++)
+            if (!(d.refr[(int64_t)d.driven[q] * d.B + lane] > 0)) mine++;
+    cnt[wave][lane] = mine;
+    __syncthreads();
+    int32_t off = 0, total = 0;
+    for (int w = 0; w < W; w++) {
+        const int32_t c = cnt[w][lane];
+        off += w < wave ? c : 0;
+        total += c;
+    }
+    // the state before this wave's first draw: M^off x0 (off < 2048)
+    uint32_t x = x0;
+    if (on && mine > 0) {
+        const uint32_t y = matvec(leapS + (32 + (off >> 5)) * LEAP_STRIDE, x0);
+        x = (off & 31) ? matvec(leapS + ((off & 31) - 1) * LEAP_STRIDE, y) : y;
+    }
+    const float forcedRefr = d.k.tRef + d.k.dt;
+    for (int32_t q = lo; q < hi; q++) {
+        const int32_t i = d.driven[q];
+        const int64_t ib = (int64_t)i * d.B + lane;
+        bool f = false;
+        if (on && !(d.refr[ib] > 0)) {
+            x = xs32(x);
+            const float u = (float)(x >> 8) * (1.0f / 16777216.0f);
+            if (u < d.drive[i] * d.k.dtS) {
+                d.v[ib] = d.k.vReset;
+                d.refr[ib] = forcedRefr;
+                d.spikeCount[ib] += 1u;
+                d.trace[ib] = 1.f;
+                d.adapt[ib] = d.adapt[ib] + d.k.adaptInc;
+                f = true;
+            }
+        }
+        const uint64_t fb = __ballot(f);
+        if (lane == 0) d.fmask[(int64_t)slot * d.N + i] = fb;
+    }
+    if (wave == 0 && on) {
+        // the stream after all of this step's draws
+        const uint32_t y = matvec(leapS + (32 + (total >> 5)) * LEAP_STRIDE, x0);
+        d.rng[lane] = (total & 31) ? matvec(leapS + ((total & 31) - 1) * LEAP_STRIDE, y) : y;
+    }
+}
+
+// ------------------------------------------------------------------ membrane + threshold
+
+__global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int32_t t) {
+    const int lane = threadIdx.x & 63;
+    const int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (i >= d.N) return;   // whole wave
+    const int32_t B = d.B, N = d.N;
+    const bool on = lane < B && t < d.steps[lane];
+    bool spk = false;
+    if (on) {
+        const int64_t ib = (int64_t)i * B + lane;
+        NeuronState s;
+        s.v = d.v[ib]; s.gE = d.gE[ib]; s.gI = d.gI[ib]; s.refr = d.refr[ib];
+        s.trace = d.trace[ib]; s.adapt = d.adapt[ib]; s.res = d.res[ib];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int64_t ci = ((int64_t)c * N + i) * B + lane;
+            const int32_t n = d.cnt[ci];
+            if (n) {
+                float g = c ? s.gI : s.gE;
+                if (n == 1) g = g + d.add[ci];
+                else {
+                    g = ordered_sum(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
+                    d.chain[ci] = -1;
+                }
+                d.cnt[ci] = 0;
+                if (c) s.gI = g; else s.gE = g;
+            }
+        }
+        const int32_t nb = d.bgCnt[ib];
+        if (nb) {
+            for (int32_t q = 0; q < nb; q++) s.gE = s.gE + d.k.bgAmp;
+            d.bgCnt[ib] = 0;
+        }
+        membrane(d.k, s, d.bias[ib]);
+        if (crosses(d.k, s, d.thr[ib])) {
+            spike_effects(d.k, s, d.k.tRef);
+            d.spikeCount[ib] += 1u;
+            spk = true;
+        }
+        d.v[ib] = s.v; d.gE[ib] = s.gE; d.gI[ib] = s.gI; d.refr[ib] = s.refr;
+        d.trace[ib] = s.trace; d.adapt[ib] = s.adapt; d.res[ib] = s.res;
+    }
+    const uint64_t th = __ballot(spk);
+    if (lane == 0) {
+        const bool drv = d.drvPos[i] >= 0;
+        const uint64_t fm = drv ? d.fmask[(int64_t)slot * N + i] : 0ull;
+        if (!drv) d.fmask[(int64_t)slot * N + i] = 0ull;
+        const uint64_t m = th | fm;
+        d.mask[(int64_t)slot * N + i] = m;
+        if (m) {
+            const int32_t p = atomicAdd(&d.nActive[slot], 1);
+            d.active[(int64_t)slot * N + p] = i;
+            atomicAdd(&d.counters[2], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+// after a step: the slot delivered this step is empty, the pool is free
+__global__ void kb_close(BatchDev d, int32_t head, int32_t nextOut) {
+    if (threadIdx.x == 0) {
+        d.nActive[head] = 0;
+        d.ncoll[0] = 0;
+    }
+    (void)nextOut;
+}
+
+// before a step: the out slot's active count starts at zero
+__global__ void kb_open(BatchDev d, int32_t slot) {
+    if (threadIdx.x == 0) d.nActive[slot] = 0;
+}
+
+// records the active list, spike masks and forced masks of one slot
+__global__ void kb_record(BatchDev d, int32_t slot, int32_t *cntOut, int32_t *ids, uint64_t *masks, uint64_t *fmasks,
+                          int32_t cap) {
+    const int32_t n = d.nActive[slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cntOut = n;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < min(n, cap); k += gridDim.x * blockDim.x) {
+        const int32_t i = d.active[(int64_t)slot * d.N + k];
+        ids[k] = i;
+        masks[k] = d.mask[(int64_t)slot * d.N + i];
+        fmasks[k] = d.fmask[(int64_t)slot * d.N + i];
+    }
+}
+
+__global__ void kb_init(BatchDev d) {
+    const int64_t n = (int64_t)d.N * d.B;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        d.v[q] = d.k.vRest; d.gE[q] = 0.f; d.gI[q] = 0.f; d.refr[q] = 0.f; d.trace[q] = 0.f;
+        d.adapt[q] = 0.f; d.res[q] = 1.f; d.spikeCount[q] = 0u; d.bias[q] = 0.f; d.thr[q] = 0.f;
+        d.bgCnt[q] = 0;
+        d.cnt[q] = 0; d.cnt[q + n] = 0; d.chain[q] = -1; d.chain[q + n] = -1;
+    }
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < (int64_t)d.nslots * d.N;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        d.mask[q] = 0ull;
+        d.fmask[q] = 0ull;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < d.nslots) d.nActive[threadIdx.x] = 0;
+}
+
+__global__ void kb_set_member(float *arr, int32_t B, int32_t b, const int32_t *idx, int32_t n, float value) {
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) arr[(int64_t)idx[q] * B + b] = value;
+}
+
+}  // namespace fb
+
+using namespace fb;
+
+struct fb_batch {
+    fb_graph *g = nullptr;
+    fb_params p{};
+    BatchDev d{};
+    hipStream_t stream = nullptr, ownStream = nullptr;
+    std::vector<void *> allocs;
+    std::vector<int32_t> drivenList;
+    std::vector<int32_t> steps;
+    std::vector<int32_t> typeOf;
+    int32_t head = 0, lastSlot = 0, t = 0;
+    int32_t *idxScratch = nullptr;
+    int32_t *recCnt = nullptr, *recIds = nullptr;
+    uint64_t *recMasks = nullptr, *recF = nullptr;
+};
+
+// graph internals needed here (defined in fb_api.hip)
+struct fb_graph;
+extern "C" fb_status fb_graph_retain(fb_graph *g);
+extern "C" fb_status fb_graph_release(fb_graph *g);
+namespace fb {
+const DevGraph *graph_dev(const fb_graph *g);
+fb_status leap_tables(uint32_t **jump, uint32_t **leap);
+}
+
+template <class T>
+static fb_status balloc(fb_batch *b, T **p, size_t count) {
+    void *q = nullptr;
+    FB_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    b->allocs.push_back(q);
+    *p = (T *)q;
+    return FB_OK;
+}
+
+#define FB_REQUIRE_B(cond, code, ...)    \
+    do {                                 \
+        if (!(cond)) {                   \
+            set_error(__VA_ARGS__);      \
+            return code;                 \
+        }                                \
+    } while (0)
+
+extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B, const double *seeds,
+                                     const int32_t *typeOf, int32_t nTypes, const float *gains, fb_batch **out) {
+    FB_REQUIRE_B(g && p && seeds && typeOf && gains && out, FB_ERR_ARG, "fb_batch_create: null argument");
+    FB_REQUIRE_B(B >= 1 && B <= BMAX, FB_ERR_ARG, "fb_batch_create: 1 <= B <= 64");
+    FB_REQUIRE_B(nTypes >= 1, FB_ERR_ARG, "fb_batch_create: nTypes >= 1");
+    const DevGraph *dg = graph_dev(g);
+    FB_REQUIRE_B(dg->counts, FB_ERR_ARG, "fb_batch_create: the graph needs synapse counts");
+    FB_REQUIRE_B(p->preInh == 0.0, FB_ERR_UNSUPPORTED, "fb_batch_create: presynaptic inhibition is not batched");
+    FB_REQUIRE_B(p->tRef > 0.0, FB_ERR_UNSUPPORTED, "fb_batch_create: tRef must be > 0");
+    const int32_t N = dg->N;
+    for (int32_t i = 0; i < N; i++)
+        FB_REQUIRE_B(typeOf[i] >= 0 && typeOf[i] < nTypes, FB_ERR_ARG, "fb_batch_create: type of neuron %d out of range", i);
+    fb_batch *b = new fb_batch();
+    b->g = g;
+    fb_graph_retain(g);
+    b->p = *p;
+    BatchDev &d = b->d;
+    d.N = N; d.B = B; d.nslots = nslots_of(*p); d.nDriven = 0; d.nTypes = nTypes;
+    derive_consts(*p, N, d.k);
+    d.g = *dg;
+    FB_HIP(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking));
+    b->stream = b->ownStream;
+    const size_t NB = (size_t)N * B;
+    fb_status st;
+    float *f;
+    if ((st = balloc(b, &f, NB * 9))) return st;
+    d.v = f; d.gE = f + NB; d.gI = f + 2 * NB; d.refr = f + 3 * NB; d.trace = f + 4 * NB; d.adapt = f + 5 * NB;
+    d.res = f + 6 * NB; d.bias = f + 7 * NB; d.thr = f + 8 * NB;
+    float *drive; int32_t *driven, *drvPos, *typ, *stepsD, *active, *nActive, *cnt, *chain, *ncoll, *bgCnt, *err;
+    float *gain, *add, *bgAcc; uint32_t *rng, *sc; uint64_t *mask, *fmask, *key; CollEntry *coll;
+    unsigned long long *counters;
+    if ((st = balloc(b, &sc, NB)) || (st = balloc(b, &drive, (size_t)N)) || (st = balloc(b, &driven, (size_t)N)) ||
+        (st = balloc(b, &drvPos, (size_t)N)) || (st = balloc(b, &typ, (size_t)N)) ||
+        (st = balloc(b, &gain, (size_t)B * nTypes)) || (st = balloc(b, &rng, (size_t)B)) ||
+        (st = balloc(b, &bgAcc, 1)) || (st = balloc(b, &stepsD, (size_t)B)) ||
+        (st = balloc(b, &mask, (size_t)d.nslots * N)) || (st = balloc(b, &fmask, (size_t)d.nslots * N)) ||
+        (st = balloc(b, &active, (size_t)d.nslots * N)) || (st = balloc(b, &nActive, (size_t)d.nslots)) ||
+        (st = balloc(b, &cnt, 2 * NB)) || (st = balloc(b, &chain, 2 * NB)) || (st = balloc(b, &add, 2 * NB)) ||
+        (st = balloc(b, &key, 2 * NB)) || (st = balloc(b, &bgCnt, NB)) || (st = balloc(b, &ncoll, 1)) ||
+        (st = balloc(b, &err, 1)) || (st = balloc(b, &counters, 4)) || (st = balloc(b, &b->idxScratch, (size_t)N)))
+        return st;
+    d.collCap = std::max<int64_t>(dg->E * 4, 1);
+    if ((st = balloc(b, &coll, (size_t)d.collCap))) return st;
+    d.spikeCount = sc; d.drive = drive; d.driven = driven; d.drvPos = drvPos; d.typeOf = typ; d.gain = gain;
+    d.rng = rng; d.bgAcc = bgAcc; d.steps = stepsD; d.mask = mask; d.fmask = fmask; d.active = active;
+    d.nActive = nActive; d.cnt = cnt; d.chain = chain; d.add = add; d.key = key; d.coll = coll; d.ncoll = ncoll;
+    d.bgCnt = bgCnt; d.err = err; d.counters = counters;
+    uint32_t *jump = nullptr, *leap = nullptr;
+    if ((st = leap_tables(&jump, &leap))) return st;
+    d.leap = leap;
+    std::vector<uint32_t> r((size_t)B);
+    for (int32_t q = 0; q < B; q++) r[q] = seed_hash(seeds[q]);
+    FB_HIP(hipMemcpy(rng, r.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(typ, typeOf, sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(gain, gains, sizeof(float) * (size_t)B * nTypes, hipMemcpyHostToDevice));
+    std::vector<int32_t> neg((size_t)N, -1);
+    FB_HIP(hipMemcpy(drvPos, neg.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    FB_HIP(hipMemset(drive, 0, sizeof(float) * N));
+    FB_HIP(hipMemset(bgAcc, 0, sizeof(float)));
+    FB_HIP(hipMemset(ncoll, 0, sizeof(int32_t)));
+    FB_HIP(hipMemset(err, 0, sizeof(int32_t)));
+    FB_HIP(hipMemset(counters, 0, sizeof(unsigned long long) * 4));
+    b->steps.assign((size_t)B, INT32_MAX);
+    FB_HIP(hipMemcpy(stepsD, b->steps.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(kb_init, dim3(1024), dim3(256), 0, b->stream, d);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(b->stream));
+    b->typeOf.assign(typeOf, typeOf + N);
+    *out = b;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_destroy(fb_batch *b) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_destroy: null batch");
+    (void)hipStreamSynchronize(b->stream);
+    for (void *p : b->allocs) (void)hipFree(p);
+    void *rec[] = {b->recCnt, b->recIds, b->recMasks, b->recF};
+    for (void *p : rec)
+        if (p) (void)hipFree(p);
+    (void)hipStreamDestroy(b->ownStream);
+    fb_graph_release(b->g);
+    delete b;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_drive(fb_batch *b, const int32_t *idx, int32_t n, double hz) {
+    FB_REQUIRE_B(b && n >= 0 && (n == 0 || idx), FB_ERR_ARG, "fb_batch_set_drive: bad arguments");
+    std::vector<float> drive((size_t)b->d.N);
+    FB_HIP(hipMemcpy(drive.data(), b->d.drive, sizeof(float) * b->d.N, hipMemcpyDeviceToHost));
+    for (int32_t q = 0; q < n; q++) {
+        const int32_t i = idx[q];
+        FB_REQUIRE_B(i >= 0 && i < b->d.N, FB_ERR_ARG, "fb_batch_set_drive: index %d out of range", i);
+        drive[i] = (float)hz;
+        auto it = std::find(b->drivenList.begin(), b->drivenList.end(), i);
+        if (hz > 0) { if (it == b->drivenList.end()) b->drivenList.push_back(i); }
+        else if (it != b->drivenList.end()) b->drivenList.erase(it);
+    }
+    FB_REQUIRE_B(b->drivenList.size() <= 2047, FB_ERR_UNSUPPORTED, "fb_batch_set_drive: at most 2047 driven neurons");
+    FB_HIP(hipMemcpy(b->d.drive, drive.data(), sizeof(float) * b->d.N, hipMemcpyHostToDevice));
+    std::vector<int32_t> pos((size_t)b->d.N, -1);
+    for (size_t q = 0; q < b->drivenList.size(); q++) pos[b->drivenList[q]] = (int32_t)q;
+    FB_HIP(hipMemcpy((void *)b->d.driven, b->drivenList.data(), sizeof(int32_t) * std::max<size_t>(1, b->drivenList.size()),
+                     hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy((void *)b->d.drvPos, pos.data(), sizeof(int32_t) * b->d.N, hipMemcpyHostToDevice));
+    b->d.nDriven = (int32_t)b->drivenList.size();
+    return FB_OK;
+}
+
+// per-member thr / bias (mV) for a list of neurons
+extern "C" fb_status fb_batch_set_member(fb_batch *b, int32_t which, int32_t member, const int32_t *idx, int32_t n,
+                                         double value) {
+    FB_REQUIRE_B(b && member >= 0 && member < b->d.B && n >= 0 && (n == 0 || idx), FB_ERR_ARG,
+                 "fb_batch_set_member: bad arguments");
+    FB_REQUIRE_B(which == FB_THR || which == FB_BIAS, FB_ERR_ARG, "fb_batch_set_member: FB_THR or FB_BIAS only");
+    for (int32_t q = 0; q < n; q++)
+        FB_REQUIRE_B(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "fb_batch_set_member: index %d out of range", idx[q]);
+    if (n == 0) return FB_OK;
+    FB_HIP(hipMemcpyAsync(b->idxScratch, idx, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(kb_set_member, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                       which == FB_THR ? b->d.thr : b->d.bias, b->d.B, member, (const int32_t *)b->idxScratch, n,
+                       (float)value);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(b->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_steps(fb_batch *b, const int32_t *steps) {
+    FB_REQUIRE_B(b && steps, FB_ERR_ARG, "fb_batch_set_steps: null argument");
+    b->steps.assign(steps, steps + b->d.B);
+    // run lengths count from the current step
+    std::vector<int32_t> abs((size_t)b->d.B);
+    for (int32_t q = 0; q < b->d.B; q++) abs[q] = steps[q] < 0 ? INT32_MAX : b->t + steps[q];
+    FB_HIP(hipMemcpy(b->d.steps, abs.data(), sizeof(int32_t) * b->d.B, hipMemcpyHostToDevice));
+    return FB_OK;
+}
+
+static fb_status batch_step(fb_batch *b) {
+    BatchDev &d = b->d;
+    const int32_t head = b->head, slot = (head + d.nslots - 1) % d.nslots;
+    hipStream_t st = b->stream;
+    hipLaunchKernelGGL(kb_open, dim3(1), dim3(64), 0, st, d, slot);
+    hipLaunchKernelGGL(kb_deliver, dim3(256), dim3(256), 0, st, d, head, b->t);
+    hipLaunchKernelGGL(kb_driven, dim3(1), dim3(1024), 0, st, d, slot, b->t);
+    hipLaunchKernelGGL(kb_membrane, dim3((d.N + 3) / 4), dim3(256), 0, st, d, slot, b->t);
+    hipLaunchKernelGGL(kb_close, dim3(1), dim3(64), 0, st, d, head, slot);
+    FB_HIP(hipGetLastError());
+    b->lastSlot = slot;
+    b->head = (head + 1) % d.nslots;
+    b->t++;
+    return FB_OK;
+}
+
+// Runs nsteps; if ids/masks are given, records every step's active neurons with
+// their spike and forced masks (per step: count into nactive[step], entries
+// appended at most cap in total; the returned total may exceed cap).
+extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
+                                  uint64_t *fmasks, int64_t cap, int64_t *total) {
+    FB_REQUIRE_B(b && nsteps >= 0, FB_ERR_ARG, "fb_batch_run: bad arguments");
+    const bool rec = nactive != nullptr;
+    if (rec && !b->recCnt) {
+        FB_HIP(hipMalloc(&b->recCnt, sizeof(int32_t)));
+        FB_HIP(hipMalloc(&b->recIds, sizeof(int32_t) * b->d.N));
+        FB_HIP(hipMalloc(&b->recMasks, sizeof(uint64_t) * b->d.N));
+        FB_HIP(hipMalloc(&b->recF, sizeof(uint64_t) * b->d.N));
+    }
+    int64_t tot = 0;
+    for (int32_t s = 0; s < nsteps; s++) {
+        fb_status st = batch_step(b);
+        if (st) return st;
+        if (rec) {
+            hipLaunchKernelGGL(kb_record, dim3(16), dim3(256), 0, b->stream, b->d, b->lastSlot, b->recCnt, b->recIds,
+                               b->recMasks, b->recF, b->d.N);
+            int32_t n = 0;
+            FB_HIP(hipMemcpyAsync(&n, b->recCnt, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+            FB_HIP(hipStreamSynchronize(b->stream));
+            nactive[s] = n;
+            const int64_t take = std::max<int64_t>(0, std::min<int64_t>(n, cap - tot));
+            if (take > 0 && ids) {
+                FB_HIP(hipMemcpyAsync(ids + tot, b->recIds, sizeof(int32_t) * take, hipMemcpyDeviceToHost, b->stream));
+                FB_HIP(hipMemcpyAsync(masks + tot, b->recMasks, sizeof(uint64_t) * take, hipMemcpyDeviceToHost, b->stream));
+                FB_HIP(hipMemcpyAsync(fmasks + tot, b->recF, sizeof(uint64_t) * take, hipMemcpyDeviceToHost, b->stream));
+                FB_HIP(hipStreamSynchronize(b->stream));
+            }
+            tot += n;
+        }
+    }
+    if (total) *total = tot;
+    int32_t err = 0;
+    FB_HIP(hipMemcpyAsync(&err, b->d.err, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    FB_REQUIRE_B(!(err & FB_DEVERR_COLL_OVERFLOW), FB_ERR_OVERFLOW, "fb_batch_run: arrival chain pool overflowed");
+    FB_REQUIRE_B(!(err & 2), FB_ERR_UNSUPPORTED, "fb_batch_run: more than 2047 draws in one step");
+    return FB_OK;
+}
+
+// one member's state array (FB_V .. FB_THR, FB_SPIKECOUNT)
+extern "C" fb_status fb_batch_get_array(fb_batch *b, int32_t which, int32_t member, void *dst) {
+    FB_REQUIRE_B(b && dst && member >= 0 && member < b->d.B, FB_ERR_ARG, "fb_batch_get_array: bad arguments");
+    float *src = nullptr;
+    switch (which) {
+        case FB_V: src = b->d.v; break;
+        case FB_GE: src = b->d.gE; break;
+        case FB_GI: src = b->d.gI; break;
+        case FB_REFR: src = b->d.refr; break;
+        case FB_TRACE: src = b->d.trace; break;
+        case FB_ADAPT: src = b->d.adapt; break;
+        case FB_RES: src = b->d.res; break;
+        case FB_BIAS: src = b->d.bias; break;
+        case FB_THR: src = b->d.thr; break;
+        case FB_SPIKECOUNT: src = (float *)b->d.spikeCount; break;
+        default: set_error("fb_batch_get_array: unknown array %d", which); return FB_ERR_ARG;
+    }
+    FB_HIP(hipStreamSynchronize(b->stream));
+    // strided copy: one column of the [N][B] layout
+    FB_HIP(hipMemcpy2D(dst, sizeof(float), src + member, sizeof(float) * b->d.B, sizeof(float), b->d.N,
+                       hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_rng(fb_batch *b, uint32_t *rng) {
+    FB_REQUIRE_B(b && rng, FB_ERR_ARG, "fb_batch_rng: null argument");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    FB_HIP(hipMemcpy(rng, b->d.rng, sizeof(uint32_t) * b->d.B, hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
+                                       int64_t *steps) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_counters: null batch");
+    unsigned long long c[4];
+    FB_HIP(hipStreamSynchronize(b->stream));
+    FB_HIP(hipMemcpy(c, b->d.counters, sizeof c, hipMemcpyDeviceToHost));
+    if (edge_events) *edge_events = (int64_t)c[0];
+    if (synapse_events) *synapse_events = (int64_t)c[1];
+    if (spikes) *spikes = (int64_t)c[2];
+    if (steps) *steps = b->t;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_stream(fb_batch *b, void *stream) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_stream: null batch");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    b->stream = stream ? (hipStream_t)stream : b->ownStream;
+    return FB_OK;
+}

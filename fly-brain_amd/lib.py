@@ -65,6 +65,16 @@ _SIGS = {
     "fb_brain_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_brain_set_stream": (I32, [P, P]),
     "fb_brain_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
+    "fb_batch_create": (I32, [P, P, I32, P, P, I32, P, PP]),
+    "fb_batch_destroy": (I32, [P]),
+    "fb_batch_set_drive": (I32, [P, P, I32, D]),
+    "fb_batch_set_member": (I32, [P, I32, I32, P, I32, D]),
+    "fb_batch_set_steps": (I32, [P, P]),
+    "fb_batch_run": (I32, [P, I32, P, P, P, P, I64, PI64]),
+    "fb_batch_get_array": (I32, [P, I32, I32, P]),
+    "fb_batch_rng": (I32, [P, P]),
+    "fb_batch_counters": (I32, [P, PI64, PI64, PI64, PI64]),
+    "fb_batch_set_stream": (I32, [P, P]),
     "fb_synchronize": (I32, [P]),
 }
 

@@ -144,6 +144,37 @@ fb_status fb_brain_set_stream(fb_brain *b, void *stream);
  * engine), "nslots", "serial_deliver".  No counterpart in the reference. */
 fb_status fb_brain_stat(fb_brain *b, const char *name, double *value);
 
+/* ---------------------------------------------------------------- ensembles
+ * B <= 64 members of one brain model over one shared graph, advanced together
+ * (the one-brain-per-worker ensembles of src/brainsetup.js:8 and
+ * scripts/lif_ensemble.mjs:16-35; the lane ensemble of kernel.bend:293-378).
+ * Member b is exactly the brain fb_brain_create would make with seed seeds[b]
+ * and weights W[j] = f32(count[j] * gains[b*nTypes + typeOf[pre(j)]]) (the
+ * graph needs counts); all members share the driven set and drive rates.
+ * Members stop, state frozen and no draws, after their own step count
+ * (fb_batch_set_steps; unset = run forever). */
+typedef struct fb_batch fb_batch;
+fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B, const double *seeds, const int32_t *typeOf,
+                          int32_t nTypes, const float *gains, fb_batch **out);
+fb_status fb_batch_destroy(fb_batch *b);
+fb_status fb_batch_set_drive(fb_batch *b, const int32_t *idx, int32_t n, double hz);
+/* per-member FB_THR or FB_BIAS on a list of neurons; silencing is thr = 1e6
+ * (setThr(i, 1e6) of scripts/lif_ensemble.mjs:35) */
+fb_status fb_batch_set_member(fb_batch *b, int32_t which, int32_t member, const int32_t *idx, int32_t n, double value);
+/* steps[b] more steps for member b (negative: no limit) */
+fb_status fb_batch_set_steps(fb_batch *b, const int32_t *steps);
+/* Advances nsteps.  With nactive != NULL records, per step, the neurons that
+ * spiked in any member (nactive[s] of them) with their spike mask (bit b =
+ * member b) and forced mask (bit b = the spike was a forced one), appended to
+ * ids/masks/fmasks up to cap entries; total = all entries of the run. */
+fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
+                       uint64_t *fmasks, int64_t cap, int64_t *total);
+fb_status fb_batch_get_array(fb_batch *b, int32_t which, int32_t member, void *dst);
+fb_status fb_batch_rng(fb_batch *b, uint32_t *rng);
+fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
+                            int64_t *steps);
+fb_status fb_batch_set_stream(fb_batch *b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,87 @@
+"""GPU parity of the ensemble engine: every member of a batch is bit-identical
+to a single brain (the CPU oracle) with its own seed, per-type-gain weights,
+silenced neurons (thr = 1e6) and run length."""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle import OracleBrain
+
+pytestmark = pytest.mark.gpu
+
+lif = importlib.import_module("fly-brain_amd.lif")
+batch = importlib.import_module("fly-brain_amd.batch")
+synth = importlib.import_module("fly-brain_amd.synth")
+P = importlib.import_module("fly-brain_amd.params")
+
+
+def member_weights(g, type_of, gains_b):
+    """writeGraph with outScale[j] = gain of j's type: W = f32(count * gain)."""
+    pre = np.repeat(np.arange(g.N), np.diff(g.indptr.astype(np.int64)))
+    return (g.counts.astype(np.float64) * gains_b[type_of[pre]].astype(np.float64)).astype(np.float32)
+
+
+def run_case(g, sign, p, seeds, type_of, gains, drive_ix, rate, silenced, steps, bias=None):
+    B = len(seeds)
+    hg = lif.HipGraph(g.N, g.indptr, g.indices, g.weights, sign, g.counts)
+    eb = batch.LIFBatch(hg, p, seeds, type_of, gains)
+    eb.setDrive(drive_ix, rate)
+    for b in range(B):
+        if len(silenced[b]):
+            eb.silence(b, silenced[b])
+        if bias is not None:
+            eb.setBias(b, bias[0], bias[1])
+    eb.set_steps(steps)
+    T = int(max(steps))
+    nact, ids, masks, fm = eb.run_record(T, cap=T * g.N)
+    nf_b, ras_b = eb.member_rasters(nact, ids, masks, fm)
+    rngs = eb.rng()
+    for b in range(B):
+        W = member_weights(g, type_of, gains[b])
+        o = OracleBrain(g, p, seed=seeds[b], W=W, sign=sign)
+        o.setDrive(drive_ix.tolist(), rate)
+        for i in silenced[b]:
+            o.setThr(int(i), 1e6)
+        if bias is not None:
+            o.setBias(bias[0], bias[1])
+        nf, ras, _ = o.run(int(steps[b]), raster_cap=int(steps[b]) * g.N)
+        assert np.array_equal(nf_b[b, : steps[b]], nf), f"member {b}: per-step counts"
+        assert not nf_b[b, steps[b]:].any(), f"member {b}: spikes after its run ended"
+        assert np.array_equal(ras_b[b], ras), f"member {b}: raster"
+        for k in ("v", "gE", "gI", "refr", "trace", "adapt", "res"):
+            assert np.array_equal(eb.array(k, b).view(np.uint32), o.array(k).view(np.uint32)), f"member {b}: {k}"
+        assert np.array_equal(eb.array("spikeCount", b), o.spikeCount)
+        assert int(rngs[b]) == o.rng, f"member {b}: rng"
+    return eb
+
+
+def test_small_batch_members_match_single_brains():
+    g = synth.tiny_graph(512, 30, seed=41, p_count=0.3)
+    rng = np.random.default_rng(3)
+    B, T = 8, 10
+    type_of = rng.integers(0, T, size=g.N).astype(np.int32)
+    gains = np.exp(rng.normal(0, 0.3, size=(B, T))).astype(np.float32)
+    sign = (np.where(g.exc, 1.0, -1.0) * 0.5).astype(np.float32)
+    silenced = [np.sort(rng.choice(g.N, size=int(rng.integers(0, 25)), replace=False)) for _ in range(B)]
+    steps = rng.integers(300, 800, size=B)
+    p = P.config_params(adaptInc=0.5, depU=0.1)
+    run_case(g, sign, p, [b + 1 for b in range(B)], type_of, gains, np.arange(0, 512, 6), 350.0, silenced, steps,
+             bias=(list(range(0, 512, 3)), 6.0))
+
+
+def test_batch64_config3_members_match_single_brains():
+    """The ensemble workload: the 16,384-neuron graph, B = 64, 500 types with
+    lognormal(0, 0.2) gains, 0-5% silenced per member, uneven run lengths
+    (shortened here so 64 oracle runs finish in seconds)."""
+    g = synth.config1_graph()
+    B = 64
+    type_of = synth.type_ids(g.N, 500, 6)
+    gains = np.stack([synth.type_gains(b) for b in range(B)])
+    silenced = [synth.silenced(b, g.N) for b in range(B)]
+    steps = np.array([200 + synth.run_length(b) // 10 for b in range(B)])
+    drv = synth.driven_set(g.N, 0.02, 3)
+    eb = run_case(g, g.sign, P.config_params(), [b + 1 for b in range(B)], type_of, gains, drv, 150.0, silenced,
+                  steps)
+    c = eb.counters()
+    assert c["edge_events"] > 0 and c["spikes"] > 0
