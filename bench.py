@@ -76,6 +76,97 @@ def cpu_baseline(g, p, drv, rates, steps: int, warm: int):
     }
 
 
+def cpu_batch_baseline(g, p, drv, type_of, gains, silenced, steps: int, members: int):
+    """`members` independent instances of lif.c (or its restatement) on all
+    host cores (one per thread), `steps` steps each: the one-brain-per-worker
+    ensemble the reference runs (src/brainsetup.js:8)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import concurrent.futures as cf
+    import numpy as np
+    import oracle as orc
+    kind = "ref" if orc.available("ref") else "oracle"
+    pre = np.repeat(np.arange(g.N), np.diff(g.indptr.astype(np.int64)))
+    outdeg = g.outdeg
+    brains = []
+    for b in range(members):
+        W = (g.counts.astype(np.float64) * gains[b][type_of[pre]].astype(np.float64)).astype(np.float32)
+        o = orc.OracleBrain(g, p, seed=b + 1, kind=kind, W=W, sign=g.sign)
+        o.setDrive(drv.tolist(), 150.0)
+        for i in silenced[b]:
+            o.setThr(int(i), 1e6)
+        o._sync_driven()
+        brains.append(o)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        res = list(ex.map(lambda o: o.run(steps, raster_cap=steps * 1024), brains))
+    dt = time.perf_counter() - t0
+    D = brains[0].nslots - 1
+    events = 0
+    for nf, ras, _ in res:
+        start = np.concatenate([[0], np.cumsum(nf)])
+        events += int(outdeg[ras[: start[max(0, steps - D)]]].sum())
+    return {"value": events / dt, "unit": "synaptic events/s", "cores": cores,
+            "kind": "reference" if kind == "ref" else "port",
+            "sample": f"{members} members x {steps} steps of the 64-member ensemble workload, one member per thread"}
+
+
+def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
+    """The 64-member ensemble on the 16,384-neuron graph, and the plain single
+    brain on the same graph for the ratio."""
+    import numpy as np
+    g = synth.config1_graph()
+    p = P.config_params()
+    drv = synth.driven_set(g.N, 0.02, 3)
+    B = 64
+    type_of = synth.type_ids(g.N, 500, 6)
+    gains = np.stack([synth.type_gains(b) for b in range(B)])
+    silenced = [synth.silenced(b, g.N) for b in range(B)]
+    steps = np.array([synth.run_length(b) for b in range(B)])
+    hg = lif.HipGraph.from_synth(g)
+    eb = batch.LIFBatch(hg, p, [b + 1 for b in range(B)], type_of, gains)
+    eb.setDrive(drv, 150.0)
+    for b in range(B):
+        eb.silence(b, silenced[b])
+    eb.set_steps(steps)
+    eb.set_stream(stream.handle)
+    c0 = eb.counters()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    T = int(steps.max())
+    e0.record(stream)
+    eb.run(T)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    c1 = eb.counters()
+    ev = c1["edge_events"] - c0["edge_events"]
+    out = {"value": ev / (ms / 1000.0), "unit": "synaptic events/s (aggregate over members)", "members": B,
+           "steps": T, "member_steps": int(steps.sum()), "ms_per_step": ms / T,
+           "spikes": c1["spikes"] - c0["spikes"],
+           "workload": "16,384 neurons, ~1.07M edges, 64 members: own seeds, 500 per-type gains lognormal(0,0.2), "
+                       "0-5% silenced, run lengths U{2000..10000}"}
+    eb.destroy()
+    # the plain single brain on the same graph and drive
+    one = lif.LIFHip(hg, p, seed=1)
+    one.setDrive(drv.tolist(), 150.0)
+    one.set_stream(stream.handle)
+    one.advance(200)
+    c0 = one.counters()
+    e0.record(stream)
+    one.run_async(10000)
+    e1.record(stream)
+    e1.synchronize()
+    ms1 = e0.elapsed_ms(e1)
+    c1 = one.counters()
+    plain = (c1["edge_events"] - c0["edge_events"]) / (ms1 / 1000.0)
+    out["plain_single_value"] = plain
+    out["plain_single_ms_per_step"] = ms1 / 10000
+    out["vs_plain_single"] = out["value"] / plain
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_batch_baseline(g, p, drv, type_of, gains, silenced, min(args.cpu_steps, 2000), B)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,13 +176,14 @@ def main():
     ap.add_argument("--coba", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
               "(multi-GPU sharding is design only).", file=sys.stderr)
         sys.exit(2)
 
-    synth, P, lif, hiprt = pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt")
+    synth, P, lif, hiprt, batch = pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch")
     t_build = time.perf_counter()
     g = synth.config2_graph()
     drv = synth.driven_set(g.N, 0.01, 4)
@@ -199,6 +291,8 @@ def main():
         b2.destroy()
     out["lookahead_phases_us_total"] = phases
     out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
+    if not args.no_batch:
+        out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_cpu:
         cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)
         out["cpu_baseline"] = cb

@@ -1,0 +1,659 @@
+/*
+ * lifdiff_oracle.c -- CPU restatement of the reference's differentiable LIF:
+ * forward with a tape, checkpoint-segment replay, and the reverse sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP adjoint in
+ * fly-brain_amd/: only tests/ and bench.py's CPU baseline leg may load it.
+ *
+ * What it restates (reference paths are into Lulzx/fly-brain):
+ *   - inScale from sizeLog, and the delivered weight    src/lifdiff.js:158-164, 278-285
+ *   - forward: deliver, driven pre-pass in set order,   src/lifdiff.js:219-353
+ *     fused integrate / decay / threshold, spike lists
+ *   - checkpoints and segment replay with the arena     src/lifdiff.js:358-464
+ *   - backward: reversed decays, mode dispatch,         src/lifdiff.js:494-686
+ *     surrogate (fast sigmoid, or the logistic's own
+ *     derivative in soft mode), reversed arrivals,
+ *     adjoint clip, truncation, non-finite sanitising
+ * The reference evaluates in JavaScript doubles and stores into Float32Arrays:
+ * every local below is a double and every array store rounds to binary32, in
+ * the reference's evaluation order.
+ *
+ * Two additions the reference does not have, restated from the same
+ * derivation:
+ *   - gEdge[j] = dL/d edgeGain[j], the per-edge multiplier lifdiff.js applies
+ *     where counts enter the delivery (src/lifdiff.js:126-129, 282).  Each
+ *     arrival adds lg[q] * counts[j] * inScale[q] * eff (* inhGain); summed
+ *     over a row this is the arrival's gLogGain term.
+ *   - a filtered-rate readout loss: r_t = a r_{t-1} + (1-a) s_t / (dt/1000),
+ *     a = exp(-dt/tau), L = mean over steps and readouts of (r_t - target)^2.
+ *     Its spike adjoint is the reverse filter
+ *     lam_t = 2 (r_t - target) / (T R) + a lam_{t+1},  dL/ds_t = (1-a)/(dt/1000) lam_t,
+ *     added to the flat dLdSpike term where the reference adds dLdSpike.
+ *
+ * Pinning: tests/test_lifdiff_oracle.py checks the gradients against central
+ * finite differences of this file's own soft forward (the method and floor of
+ * scripts/grad_check.mjs:92-99), the row / column invariants of gEdge, and the
+ * hard forward's spike raster against the lif.c restatement where the two
+ * models coincide.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "lifdiff_oracle.h"
+#include "lif_oracle.h"
+
+#define FORCED_BIT 0x80000000u
+
+typedef struct {
+    float *v, *gE, *gI, *adapt, *res, *refr;
+    uint32_t rng;
+} ckpt_t;
+
+struct fdo_net {
+    int32_t N;
+    int64_t E;
+    uint32_t *indptr, *indices;
+    float *counts, *edgeGain, *sign;
+    float *in[FDO_NINPUTS];
+    uint8_t *thrMask, *biasMask;
+    int32_t *driven, nDriven;
+    fdo_params p;
+    int32_t nslots, K;
+    /* state */
+    float *st[FDO_NSTATE];
+    float *scored;
+    uint32_t rng;
+    /* loss */
+    float *dLdSpike;
+    int32_t lossFrom;
+    int32_t R;
+    int32_t *readout, *readIdx;
+    float *target;
+    double tauR;
+    /* tape */
+    int32_t steps;
+    int64_t *off;
+    int32_t *idx;
+    float *amp;
+    int64_t cap;
+    float *readS;
+    double *rate;
+    ckpt_t *ck;
+    int32_t nck;
+};
+
+static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
+
+static uint32_t xs(uint32_t *s) { return fbo_xorshift32(s); }
+/* (s >>> 8) * 2^-24, as LIFDiff._rand returns it (src/lifdiff.js:213) */
+static double rand01(uint32_t *s) { return (double)(xs(s) >> 8) * (1.0 / 16777216.0); }
+
+static double js_round(double x) {
+    double f = floor(x);
+    return (x - f >= 0.5) ? f + 1.0 : f;
+}
+
+fdo_net *fdo_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t *indices,
+                    const float *counts, const float *sign, const fdo_params *p) {
+    fdo_net *n = (fdo_net *)calloc(1, sizeof *n);
+    n->N = N;
+    n->E = E;
+    n->indptr = (uint32_t *)xcalloc(N + 1, 4);
+    memcpy(n->indptr, indptr, 4 * (size_t)(N + 1));
+    n->indices = (uint32_t *)xcalloc(E, 4);
+    memcpy(n->indices, indices, 4 * (size_t)E);
+    n->counts = (float *)xcalloc(E, 4);
+    memcpy(n->counts, counts, 4 * (size_t)E);
+    n->edgeGain = NULL;
+    n->sign = (float *)xcalloc(N, 4);
+    memcpy(n->sign, sign, 4 * (size_t)N);
+    for (int k = 0; k < FDO_NINPUTS; k++) n->in[k] = (float *)xcalloc(N, 4);
+    for (int32_t i = 0; i < N; i++) n->in[FDO_OUTSCALE][i] = 1.f;
+    n->thrMask = (uint8_t *)xcalloc(N, 1);
+    n->biasMask = (uint8_t *)xcalloc(N, 1);
+    n->driven = (int32_t *)xcalloc(N, 4);
+    n->p = *p;
+    double r = js_round(p->delay / p->dt);
+    n->nslots = (int32_t)(r < 1 ? 1 : r) + 1;
+    n->K = p->checkpoint > 0 ? p->checkpoint : 16;
+    for (int k = 0; k < FDO_NSTATE; k++) n->st[k] = (float *)xcalloc(N, 4);
+    n->scored = (float *)xcalloc(N, 4);
+    n->dLdSpike = (float *)xcalloc(N, 4);
+    n->readIdx = (int32_t *)xcalloc(N, 4);
+    for (int32_t i = 0; i < N; i++) n->readIdx[i] = -1;
+    return n;
+}
+
+static void free_tape(fdo_net *n) {
+    free(n->off); free(n->idx); free(n->amp); free(n->readS); free(n->rate);
+    n->off = NULL; n->idx = NULL; n->amp = NULL; n->readS = NULL; n->rate = NULL;
+    for (int32_t c = 0; c < n->nck; c++) {
+        free(n->ck[c].v); free(n->ck[c].gE); free(n->ck[c].gI);
+        free(n->ck[c].adapt); free(n->ck[c].res); free(n->ck[c].refr);
+    }
+    free(n->ck);
+    n->ck = NULL;
+    n->nck = 0;
+}
+
+void fdo_destroy(fdo_net *n) {
+    if (!n) return;
+    free_tape(n);
+    free(n->indptr); free(n->indices); free(n->counts); free(n->edgeGain); free(n->sign);
+    for (int k = 0; k < FDO_NINPUTS; k++) free(n->in[k]);
+    for (int k = 0; k < FDO_NSTATE; k++) free(n->st[k]);
+    free(n->thrMask); free(n->biasMask); free(n->driven); free(n->scored); free(n->dLdSpike);
+    free(n->readout); free(n->readIdx); free(n->target);
+    free(n);
+}
+
+void fdo_set_input(fdo_net *n, int32_t which, const float *values) {
+    if (which >= 0 && which < FDO_NINPUTS) memcpy(n->in[which], values, 4 * (size_t)n->N);
+}
+
+void fdo_set_edge_gain(fdo_net *n, const float *eg) {
+    if (!eg) { free(n->edgeGain); n->edgeGain = NULL; return; }
+    if (!n->edgeGain) n->edgeGain = (float *)xcalloc(n->E, 4);
+    memcpy(n->edgeGain, eg, 4 * (size_t)n->E);
+}
+
+void fdo_set_masks(fdo_net *n, const uint8_t *thrMask, const uint8_t *biasMask) {
+    if (thrMask) memcpy(n->thrMask, thrMask, (size_t)n->N);
+    if (biasMask) memcpy(n->biasMask, biasMask, (size_t)n->N);
+}
+
+/* drivenSet in insertion order (src/lifdiff.js:203-207) */
+void fdo_set_driven(fdo_net *n, const int32_t *list, int32_t count) {
+    memcpy(n->driven, list, 4 * (size_t)count);
+    n->nDriven = count;
+}
+
+void fdo_set_loss(fdo_net *n, const float *dLdSpike, int32_t lossFrom) {
+    memcpy(n->dLdSpike, dLdSpike, 4 * (size_t)n->N);
+    n->lossFrom = lossFrom;
+}
+
+void fdo_set_readout(fdo_net *n, const int32_t *idx, int32_t R, const float *targetHz, double tauMs) {
+    free(n->readout); free(n->target);
+    for (int32_t i = 0; i < n->N; i++) n->readIdx[i] = -1;
+    n->R = R;
+    n->readout = (int32_t *)xcalloc(R, 4);
+    n->target = (float *)xcalloc(R, 4);
+    memcpy(n->readout, idx, 4 * (size_t)R);
+    memcpy(n->target, targetHz, 4 * (size_t)R);
+    for (int32_t k = 0; k < R; k++) n->readIdx[idx[k]] = k;
+    n->tauR = tauMs;
+}
+
+float *fdo_state(fdo_net *n, int32_t which) { return n->st[which]; }
+uint32_t fdo_rng(const fdo_net *n) { return n->rng; }
+
+/* ------------------------------------------------------------------ constants */
+
+typedef struct {
+    double dE, dA, kRec, kM, dtS, cE, cI;
+} kconsts;
+
+static kconsts consts_of(const fdo_params *p) {
+    kconsts k;
+    k.dE = exp(-p->dt / p->tauSyn);
+    k.dA = exp(-p->dt / p->adaptTau);
+    k.kRec = p->dt / p->depTau;
+    k.kM = p->dt / p->tauM;
+    k.dtS = p->dt / 1000.0;
+    k.cE = 1.0 / (p->eExc - p->vRest);
+    k.cI = 1.0 / (p->vRest - p->eInh);
+    return k;
+}
+
+/* inScale = min(boostCap, max(1/maxSizeScale, exp(-sizeAlpha * sizeLog)))
+ * (src/lifdiff.js:161-164) */
+static void prepare(fdo_net *n) {
+    const fdo_params *p = &n->p;
+    float *in = n->st[FDO_INSCALE];
+    for (int32_t i = 0; i < n->N; i++) {
+        double s = exp(-p->sizeAlpha * (double)n->in[FDO_SIZELOG][i]);
+        double lo = 1.0 / p->maxSizeScale;
+        double m = s > lo ? s : lo;
+        in[i] = (float)(p->boostCap < m ? p->boostCap : m);
+    }
+}
+
+static void reset_state(fdo_net *n) {
+    const fdo_params *p = &n->p;
+    for (int32_t i = 0; i < n->N; i++) {
+        n->st[FDO_V][i] = (float)p->vRest;
+        n->st[FDO_GE][i] = 0.f; n->st[FDO_GI][i] = 0.f; n->st[FDO_ADAPT][i] = 0.f;
+        n->st[FDO_RES][i] = 1.f; n->st[FDO_REFR][i] = 0.f; n->st[FDO_SPIKECOUNT][i] = 0.f;
+        n->scored[i] = 0.f;
+    }
+    prepare(n);
+}
+
+/* eff = sign * wSyn * outScale * exp(logGain) * res * amp (src/lifdiff.js:278) */
+static double eff_of(const fdo_net *n, int32_t pre, double res, double amp) {
+    return (double)n->sign[pre] * n->p.wSyn * (double)n->in[FDO_OUTSCALE][pre] *
+           exp((double)n->in[FDO_LOGGAIN][pre]) * res * amp;
+}
+
+/* the arrivals of one step, in list order (src/lifdiff.js:259-286).  resAt,
+ * when given, records res[pre] as read. */
+static void deliver(fdo_net *n, int64_t a0, int64_t a1, float *resAt) {
+    float *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *res = n->st[FDO_RES];
+    const float *inS = n->st[FDO_INSCALE];
+    const float *eg = n->edgeGain;
+    for (int64_t k = a0; k < a1; k++) {
+        const int32_t pre = (int32_t)((uint32_t)n->idx[k] & ~FORCED_BIT);
+        const double amp = n->amp[k];
+        if (n->sign[pre] == 0.f) continue;
+        if (resAt) resAt[k - a0] = res[pre];
+        const double eff = eff_of(n, pre, res[pre], amp);
+        res[pre] = (float)((double)res[pre] - n->p.depU * (double)res[pre]);
+        const uint32_t a = n->indptr[pre], b = n->indptr[pre + 1];
+        if (eff > 0) {
+            for (uint32_t j = a; j < b; j++) {
+                const uint32_t q = n->indices[j];
+                gE[q] = (float)((double)gE[q] + (double)n->counts[j] * (eg ? (double)eg[j] : 1.0) * (double)inS[q] * eff);
+            }
+        } else {
+            const double ig = n->p.inhGain;
+            for (uint32_t j = a; j < b; j++) {
+                const uint32_t q = n->indices[j];
+                gI[q] = (float)((double)gI[q] + (double)n->counts[j] * (eg ? (double)eg[j] : 1.0) * (double)inS[q] * ig * eff);
+            }
+        }
+    }
+}
+
+static void push_spike(fdo_net *n, uint32_t idx, float amp) {
+    /* appends to the list of the step in progress, n->steps */
+    if (n->off[n->steps + 1] >= n->cap) {
+        n->cap = n->cap * 2 + 1024;
+        n->idx = (int32_t *)realloc(n->idx, 4 * (size_t)n->cap);
+        n->amp = (float *)realloc(n->amp, 4 * (size_t)n->cap);
+    }
+    const int64_t k = n->off[n->steps + 1]++;
+    n->idx[k] = (int32_t)idx;
+    n->amp[k] = amp;
+}
+
+static void save_ckpt(fdo_net *n, int32_t c) {
+    ckpt_t *ck = &n->ck[c];
+    const size_t sz = 4 * (size_t)n->N;
+    if (!ck->v) {
+        ck->v = (float *)malloc(sz); ck->gE = (float *)malloc(sz); ck->gI = (float *)malloc(sz);
+        ck->adapt = (float *)malloc(sz); ck->res = (float *)malloc(sz); ck->refr = (float *)malloc(sz);
+    }
+    memcpy(ck->v, n->st[FDO_V], sz); memcpy(ck->gE, n->st[FDO_GE], sz); memcpy(ck->gI, n->st[FDO_GI], sz);
+    memcpy(ck->adapt, n->st[FDO_ADAPT], sz); memcpy(ck->res, n->st[FDO_RES], sz);
+    memcpy(ck->refr, n->st[FDO_REFR], sz);
+    ck->rng = n->rng;
+}
+
+static void restore_ckpt(fdo_net *n, int32_t c) {
+    const ckpt_t *ck = &n->ck[c];
+    const size_t sz = 4 * (size_t)n->N;
+    memcpy(n->st[FDO_V], ck->v, sz); memcpy(n->st[FDO_GE], ck->gE, sz); memcpy(n->st[FDO_GI], ck->gI, sz);
+    memcpy(n->st[FDO_ADAPT], ck->adapt, sz); memcpy(n->st[FDO_RES], ck->res, sz);
+    memcpy(n->st[FDO_REFR], ck->refr, sz);
+    n->rng = ck->rng;
+}
+
+/* filtered rates of the readouts and the loss terms */
+static double readout_loss(fdo_net *n) {
+    const int32_t T = n->steps, R = n->R;
+    if (!R || !T) return 0.0;
+    const double a = exp(-n->p.dt / n->tauR), dtS = n->p.dt / 1000.0;
+    double L = 0.0;
+    for (int32_t k = 0; k < R; k++) {
+        double r = 0.0;
+        for (int32_t t = 0; t < T; t++) {
+            r = a * r + (1.0 - a) * (double)n->readS[(int64_t)t * R + k] / dtS;
+            n->rate[(int64_t)t * R + k] = r;
+            const double d = r - (double)n->target[k];
+            L += d * d;
+        }
+    }
+    return L / ((double)T * (double)R);
+}
+
+double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
+    const fdo_params *p = &n->p;
+    const int32_t N = n->N, D = n->nslots - 1, R = n->R;
+    const kconsts k = consts_of(p);
+    free_tape(n);
+    n->steps = 0;
+    n->off = (int64_t *)xcalloc((size_t)steps + 1, 8);
+    n->cap = 1024;
+    n->idx = (int32_t *)malloc(4 * (size_t)n->cap);
+    n->amp = (float *)malloc(4 * (size_t)n->cap);
+    n->readS = (float *)xcalloc((size_t)steps * (R ? R : 1), 4);
+    n->rate = (double *)xcalloc((size_t)steps * (R ? R : 1), 8);
+    if (record) {
+        n->nck = (steps + n->K - 1) / n->K + 1;
+        n->ck = (ckpt_t *)calloc((size_t)n->nck, sizeof(ckpt_t));
+    }
+    n->rng = fbo_seed_hash(seed);
+    reset_state(n);
+    float *v = n->st[FDO_V], *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *adapt = n->st[FDO_ADAPT];
+    float *res = n->st[FDO_RES], *refr = n->st[FDO_REFR], *sc = n->st[FDO_SPIKECOUNT];
+    const float *drive = n->in[FDO_DRIVE], *thrOff = n->in[FDO_THROFFSET];
+    for (int32_t t = 0; t < steps; t++) {
+        if (record && t % n->K == 0) save_ckpt(n, t / n->K);
+        n->off[t + 1] = n->off[t];
+        if (t >= D) deliver(n, n->off[t - D], n->off[t - D + 1], NULL);
+        n->steps = t;
+        const int scoredStep = t >= n->lossFrom;
+        /* driven pre-pass in set order (src/lifdiff.js:290-297) */
+        for (int32_t q = 0; q < n->nDriven; q++) {
+            const int32_t i = n->driven[q];
+            if (refr[i] > 0) continue;
+            if (rand01(&n->rng) < (double)drive[i] * k.dtS) {
+                refr[i] = (float)(p->tRef + p->dt);
+                push_spike(n, (uint32_t)i | FORCED_BIT, 1.f);
+                sc[i] = (float)((double)sc[i] + 1.0);
+                if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + 1.0);
+                adapt[i] = (float)((double)adapt[i] + p->adaptInc);
+                if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = 1.f;
+            }
+        }
+        /* integrate, decay, threshold (src/lifdiff.js:313-345) */
+        for (int32_t i = 0; i < N; i++) {
+            double vi = v[i], s = 0.0;
+            const int wasRefr = refr[i] > 0;
+            if (wasRefr) {
+                refr[i] = (float)((double)refr[i] - p->dt);
+                vi = p->vReset;
+            } else {
+                const double bias = n->biasMask[i] ? p->laminaBias : 0.0;
+                vi += (p->vRest - vi + (double)gE[i] * (p->eExc - vi) * k.cE + (double)gI[i] * (vi - p->eInh) * k.cI + bias) * k.kM;
+            }
+            gE[i] = (float)((double)gE[i] * k.dE);
+            gI[i] = (float)((double)gI[i] * k.dE);
+            adapt[i] = (float)((double)adapt[i] * k.dA);
+            res[i] = (float)((double)res[i] + (1.0 - (double)res[i]) * k.kRec);
+            if (!wasRefr) {
+                const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
+                s = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                if (s > 0) {
+                    vi = s * p->vReset + (1.0 - s) * vi;
+                    if (!p->soft) refr[i] = (float)p->tRef;
+                    adapt[i] = (float)((double)adapt[i] + s * p->adaptInc);
+                }
+                sc[i] = (float)((double)sc[i] + s);
+                if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + s);
+                if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = (float)s;
+                if (s > (p->soft ? 1e-4 : 0.5)) push_spike(n, (uint32_t)i, (float)s);
+            }
+            v[i] = (float)vi;
+        }
+    }
+    n->steps = steps;
+    double L = 0.0;
+    for (int32_t i = 0; i < N; i++) L += (double)n->dLdSpike[i] * (double)n->scored[i];
+    return L + readout_loss(n);
+}
+
+int64_t fdo_spikes(const fdo_net *n, int64_t *off, int32_t *idx, float *amp, int64_t cap) {
+    const int64_t tot = n->off ? n->off[n->steps] : 0;
+    if (off) memcpy(off, n->off, 8 * ((size_t)n->steps + 1));
+    const int64_t m = tot < cap ? tot : cap;
+    if (idx && m > 0) memcpy(idx, n->idx, 4 * (size_t)m);
+    if (amp && m > 0) memcpy(amp, n->amp, 4 * (size_t)m);
+    return tot;
+}
+
+int64_t fdo_readout(const fdo_net *n, float *s, double *rate) {
+    const int64_t m = (int64_t)n->steps * n->R;
+    if (s && m) memcpy(s, n->readS, 4 * (size_t)m);
+    if (rate && m) memcpy(rate, n->rate, 8 * (size_t)m);
+    return m;
+}
+
+/* ------------------------------------------------------------------ replay */
+
+typedef struct {
+    int32_t L;
+    float *vPre, *gE, *gI, *adapt, *sp, *resAt;
+    uint8_t *mode;
+} arena_t;
+
+/* Re-simulates [from, to) from the checkpoint at `from`, recording per step the
+ * membrane potential, conductances (after delivery, before decay), decayed
+ * adapt, spike amplitude, mode (0 integrated, 1 refractory, 2 driven spike)
+ * and res at each arrival (src/lifdiff.js:390-464). */
+static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
+    const fdo_params *p = &n->p;
+    const int32_t N = n->N, D = n->nslots - 1;
+    const kconsts k = consts_of(p);
+    restore_ckpt(n, from / n->K);
+    float *v = n->st[FDO_V], *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *adapt = n->st[FDO_ADAPT];
+    float *res = n->st[FDO_RES], *refr = n->st[FDO_REFR];
+    const float *drive = n->in[FDO_DRIVE], *thrOff = n->in[FDO_THROFFSET];
+    uint8_t *forced = (uint8_t *)xcalloc(N, 1);
+    int64_t rOff = 0;
+    for (int32_t t = from; t < to; t++) {
+        const int64_t off = (int64_t)(t - from) * N;
+        if (t >= D) {
+            deliver(n, n->off[t - D], n->off[t - D + 1], A->resAt + rOff);
+            rOff += n->off[t - D + 1] - n->off[t - D];
+        }
+        memset(forced, 0, (size_t)N);
+        for (int32_t q = 0; q < n->nDriven; q++) {
+            const int32_t i = n->driven[q];
+            if (refr[i] > 0) continue;
+            if (rand01(&n->rng) < (double)drive[i] * k.dtS) {
+                refr[i] = (float)(p->tRef + p->dt);
+                adapt[i] = (float)((double)adapt[i] + p->adaptInc);
+                forced[i] = 1;
+            }
+        }
+        for (int32_t i = 0; i < N; i++) {
+            double vi = v[i], sv = 0.0;
+            uint8_t md = 0;
+            A->vPre[off + i] = v[i]; A->gE[off + i] = gE[i]; A->gI[off + i] = gI[i];
+            const int wasRefr = refr[i] > 0;
+            if (wasRefr) {
+                refr[i] = (float)((double)refr[i] - p->dt);
+                vi = p->vReset;
+                if (forced[i]) { md = 2; sv = 1.0; } else md = 1;
+            } else {
+                const double bias = n->biasMask[i] ? p->laminaBias : 0.0;
+                vi += (p->vRest - vi + (double)gE[i] * (p->eExc - vi) * k.cE + (double)gI[i] * (vi - p->eInh) * k.cI + bias) * k.kM;
+            }
+            gE[i] = (float)((double)gE[i] * k.dE);
+            gI[i] = (float)((double)gI[i] * k.dE);
+            adapt[i] = (float)((double)adapt[i] * k.dA);
+            res[i] = (float)((double)res[i] + (1.0 - (double)res[i]) * k.kRec);
+            A->adapt[off + i] = adapt[i];
+            if (!wasRefr) {
+                const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
+                sv = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                if (sv > 0) {
+                    vi = sv * p->vReset + (1.0 - sv) * vi;
+                    if (!p->soft) refr[i] = (float)p->tRef;
+                    adapt[i] = (float)((double)adapt[i] + sv * p->adaptInc);
+                }
+            }
+            A->mode[off + i] = md;
+            A->sp[off + i] = (float)sv;
+            v[i] = (float)vi;
+        }
+    }
+    free(forced);
+}
+
+/* ------------------------------------------------------------------ backward */
+
+static float clipf(double x, double c) {
+    if (c > 0) { if (x > c) x = c; else if (x < -c) x = -c; }
+    return (float)x;
+}
+
+void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain, float *gEdge, float *gInScale,
+                  int32_t *clamped) {
+    const fdo_params *p = &n->p;
+    const int32_t N = n->N, T = n->steps, R = n->R, S = n->nslots, D = S - 1;
+    const kconsts k = consts_of(p);
+    const double aClip = p->adjClip;
+    const float *inS = n->st[FDO_INSCALE], *thrOff = n->in[FDO_THROFFSET];
+    const float *eg = n->edgeGain;
+    float *lv = (float *)xcalloc(N, 4), *lgE = (float *)xcalloc(N, 4), *lgI = (float *)xcalloc(N, 4);
+    float *lad = (float *)xcalloc(N, 4), *lres = (float *)xcalloc(N, 4);
+    float *ring = (float *)xcalloc((size_t)S * N, 4);
+    memset(gLogGain, 0, 4 * (size_t)N);
+    memset(gEdge, 0, 4 * (size_t)n->E);
+    memset(gInScale, 0, 4 * (size_t)N);
+    double g[FDO_NPARAMS] = {0};
+    double sAdaptInc = 0, sVThresh = 0, sKc = 0, sLamin = 0, sEInh = 0;
+    /* the readout loss's spike adjoint, per step and readout */
+    double *dLds = NULL;
+    if (R) {
+        const double a = exp(-p->dt / n->tauR), dtS = p->dt / 1000.0;
+        dLds = (double *)xcalloc((size_t)T * R, 8);
+        for (int32_t q = 0; q < R; q++) {
+            double lam = 0.0;
+            for (int32_t t = T - 1; t >= 0; t--) {
+                lam = 2.0 * (n->rate[(int64_t)t * R + q] - (double)n->target[q]) / ((double)T * (double)R) + a * lam;
+                dLds[(int64_t)t * R + q] = (1.0 - a) / dtS * lam;
+            }
+        }
+    }
+    arena_t A;
+    A.L = n->K;
+    A.vPre = (float *)xcalloc((size_t)A.L * N, 4); A.gE = (float *)xcalloc((size_t)A.L * N, 4);
+    A.gI = (float *)xcalloc((size_t)A.L * N, 4); A.adapt = (float *)xcalloc((size_t)A.L * N, 4);
+    A.sp = (float *)xcalloc((size_t)A.L * N, 4); A.mode = (uint8_t *)xcalloc((size_t)A.L * N, 1);
+    int64_t resCap = 1;
+    for (int32_t from = 0; from < T; from += n->K) {
+        const int32_t to = from + n->K < T ? from + n->K : T;
+        const int64_t c = (to > D ? n->off[to - D] : 0) - (from > D ? n->off[from - D] : 0);
+        if (c > resCap) resCap = c;
+    }
+    A.resAt = (float *)xcalloc((size_t)resCap, 4);
+
+    const int32_t nseg = (T + n->K - 1) / n->K;
+    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int32_t from = sgi * n->K, to = from + n->K < T ? from + n->K : T;
+        replay(n, from, to, &A);
+        for (int32_t t = to - 1; t >= from; t--) {
+            const int64_t off = (int64_t)(t - from) * N;
+            const int scoredStep = t >= n->lossFrom;
+            if (truncate > 0 && (t + 1) % truncate == 0) {
+                memset(lv, 0, 4 * (size_t)N); memset(lgE, 0, 4 * (size_t)N); memset(lgI, 0, 4 * (size_t)N);
+                memset(lad, 0, 4 * (size_t)N); memset(lres, 0, 4 * (size_t)N);
+                memset(ring, 0, 4 * (size_t)S * N);
+            }
+            float *ls = ring + (int64_t)(t % S) * N;
+            /* reversed decays and integrate-and-spike (src/lifdiff.js:555-617) */
+            for (int32_t i = 0; i < N; i++) {
+                const uint8_t md = A.mode[off + i];
+                const double lpA = lad[i];
+                const double lE = (double)lgE[i] * k.dE, lI = (double)lgI[i] * k.dE, lA = lpA * k.dA;
+                const double lR = (double)lres[i] * (1.0 - k.kRec);
+                lgE[i] = (float)lE; lgI[i] = (float)lI; lad[i] = (float)lA; lres[i] = (float)lR;
+                double dl = scoredStep ? (double)n->dLdSpike[i] : 0.0;
+                if (R && n->readIdx[i] >= 0) dl += dLds[(int64_t)t * R + n->readIdx[i]];
+                if (md == 1) { lv[i] = 0.f; continue; }
+                if (md == 2) {
+                    sAdaptInc += lA;
+                    lv[i] = 0.f; ls[i] = 0.f;
+                    continue;
+                }
+                const double vv = A.vPre[off + i], gEv = A.gE[off + i], gIv = A.gI[off + i];
+                const double bias = n->biasMask[i] ? p->laminaBias : 0.0;
+                const double u = vv + (p->vRest - vv + gEv * (p->eExc - vv) * k.cE + gIv * (vv - p->eInh) * k.cI + bias) * k.kM;
+                const double thr = p->vThresh + (double)A.adapt[off + i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
+                const double d = u - thr;
+                const double s = A.sp[off + i];
+                const double sd = 0.0, st = s;
+                const double beta = p->surrogateBeta;
+                double sg;
+                if (p->soft) sg = st * (1.0 - st) / beta;
+                else { const double q = 1.0 + fabs(d) / beta; sg = 1.0 / (beta * q * q); }
+                const double lspike = (double)ls[i] + dl;
+                const double lvNew = lv[i];
+                const double lsTot = (p->vReset - u) * lvNew + lpA * p->adaptInc + lspike;
+                const double lt = lsTot * (1.0 - sd) * sg;
+                const double lu = lvNew * (1.0 - s) + lt;
+                const double lthr = -lt;
+                sAdaptInc += lpA * s;
+                sVThresh += lthr;
+                if (n->thrMask[i]) sKc += lthr;
+                lad[i] = (float)(lA + lthr * k.dA);
+                lv[i] = (float)(lu * (1.0 + k.kM * (-1.0 - gEv * k.cE + gIv * k.cI)));
+                lgE[i] = (float)(lE + lu * k.kM * (p->eExc - vv) * k.cE);
+                lgI[i] = (float)(lI + lu * k.kM * (vv - p->eInh) * k.cI);
+                if (bias != 0.0) sLamin += lu * k.kM;
+                sEInh += lu * k.kM * gIv * k.cI * (-1.0 + (vv - p->eInh) * k.cI);
+                ls[i] = 0.f;
+            }
+            if (aClip > 0)
+                for (int32_t i = 0; i < N; i++) {
+                    lv[i] = clipf(lv[i], aClip); lgE[i] = clipf(lgE[i], aClip); lgI[i] = clipf(lgI[i], aClip);
+                    lad[i] = clipf(lad[i], aClip); lres[i] = clipf(lres[i], aClip);
+                }
+            /* reversed delivery (src/lifdiff.js:635-666), plus gEdge */
+            if (t < D) continue;
+            const int64_t a0 = n->off[t - D], a1 = n->off[t - D + 1];
+            /* this step's first entry in the segment's resAt record */
+            const int64_t ro = n->off[t - D] - n->off[(from > D ? from : D) - D];
+            const int32_t emit = t - D;
+            float *target = emit >= 0 ? ring + (int64_t)(emit % S) * N : NULL;
+            for (int64_t kk = a0; kk < a1; kk++) {
+                const int32_t pre = (int32_t)((uint32_t)n->idx[kk] & ~FORCED_BIT);
+                const double sgn = n->sign[pre], amp = n->amp[kk];
+                if (sgn == 0.0) continue;
+                const double resAtv = A.resAt[ro + (kk - a0)];
+                const double gain = exp((double)n->in[FDO_LOGGAIN][pre]) * (double)n->in[FDO_OUTSCALE][pre];
+                const double base = sgn * p->wSyn * gain * resAtv;
+                const double eff = base * amp;
+                const uint32_t a = n->indptr[pre], b = n->indptr[pre + 1];
+                double lRaw = 0.0;
+                if (eff > 0) {
+                    for (uint32_t j = a; j < b; j++) {
+                        const uint32_t q = n->indices[j];
+                        const double cw = (double)n->counts[j] * (eg ? (double)eg[j] : 1.0);
+                        lRaw += (double)lgE[q] * cw * (double)inS[q];
+                        gInScale[q] = (float)((double)gInScale[q] + (double)lgE[q] * cw * eff);
+                        gEdge[j] = (float)((double)gEdge[j] + (double)lgE[q] * (double)n->counts[j] * (double)inS[q] * eff);
+                    }
+                } else {
+                    const double ig = p->inhGain;
+                    for (uint32_t j = a; j < b; j++) {
+                        const uint32_t q = n->indices[j];
+                        const double cw = (double)n->counts[j] * (eg ? (double)eg[j] : 1.0);
+                        lRaw += (double)lgI[q] * cw * (double)inS[q] * ig;
+                        gInScale[q] = (float)((double)gInScale[q] + (double)lgI[q] * cw * ig * eff);
+                        g[FDO_G_INHGAIN] += (double)lgI[q] * cw * (double)inS[q] * eff;
+                        gEdge[j] = (float)((double)gEdge[j] + (double)lgI[q] * (double)n->counts[j] * (double)inS[q] * ig * eff);
+                    }
+                }
+                g[FDO_G_WSYN] += lRaw * sgn * gain * resAtv * amp;
+                gLogGain[pre] = (float)((double)gLogGain[pre] + lRaw * eff);
+                if (target) target[pre] = clipf((double)target[pre] + lRaw * base, aClip);
+                const double lresAfter = lres[pre];
+                g[FDO_G_DEPU] += -lresAfter * resAtv;
+                lres[pre] = clipf(lresAfter * (1.0 - p->depU) + lRaw * sgn * p->wSyn * gain * amp, aClip);
+            }
+        }
+    }
+    g[FDO_G_ADAPTINC] += sAdaptInc; g[FDO_G_VTHRESH] += sVThresh; g[FDO_G_KC] += sKc;
+    g[FDO_G_LAMINA] += sLamin; g[FDO_G_EINH] += sEInh;
+    /* sizeAlpha reaches the loss through inScale (src/lifdiff.js:670-674) */
+    for (int32_t i = 0; i < N; i++) {
+        const double raw = exp(-p->sizeAlpha * (double)n->in[FDO_SIZELOG][i]);
+        if (raw < p->boostCap && raw > 1.0 / p->maxSizeScale)
+            g[FDO_G_SIZEALPHA] += (double)gInScale[i] * (-(double)n->in[FDO_SIZELOG][i]) * raw;
+    }
+    int32_t cl[3] = {0, 0, 0};
+    for (int q = 0; q < FDO_NPARAMS; q++) if (!isfinite(g[q])) { g[q] = 0; cl[0]++; }
+    for (int32_t i = 0; i < N; i++) if (!isfinite(gLogGain[i])) { gLogGain[i] = 0.f; cl[1]++; }
+    for (int64_t j = 0; j < n->E; j++) if (!isfinite(gEdge[j])) { gEdge[j] = 0.f; cl[2]++; }
+    memcpy(params, g, sizeof g);
+    if (clamped) memcpy(clamped, cl, sizeof cl);
+    free(lv); free(lgE); free(lgI); free(lad); free(lres); free(ring); free(dLds);
+    free(A.vPre); free(A.gE); free(A.gI); free(A.adapt); free(A.sp); free(A.mode); free(A.resAt);
+}

@@ -212,3 +212,152 @@ class OracleBrain:
         else:
             ras = ras[:tot].copy()
         return nf, ras, int(tot)
+
+
+# ---------------------------------------------------------------- differentiable model
+
+DIFF_FIELDS = ("dt", "vRest", "vThresh", "vReset", "tauM", "tauSyn", "tRef", "delay",
+               "wSyn", "adaptInc", "adaptTau", "depU", "depTau",
+               "eExc", "eInh", "inhGain", "kcThreshold", "laminaBias",
+               "surrogateBeta", "adjClip", "sizeAlpha", "maxSizeScale", "boostCap")
+DIFF_INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive")
+DIFF_STATE = ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount", "inScale")
+DIFF_PARAMS = ("wSyn", "sizeAlpha", "inhGain", "eInh", "vThresh", "kcThreshold", "laminaBias",
+               "adaptInc", "depU")
+
+
+class _DiffParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in DIFF_FIELDS] + [("soft", ctypes.c_int32), ("checkpoint", ctypes.c_int32)]
+
+
+def _diff_lib():
+    if "diff" in _libs:
+        return _libs["diff"]
+    lib = ctypes.CDLL(ORACLE_SO)
+    P, I32, I64, F, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double
+    sig = {
+        "fdo_create": (P, [I32, I64, P, P, P, P, ctypes.POINTER(_DiffParams)]),
+        "fdo_destroy": (None, [P]),
+        "fdo_set_input": (None, [P, I32, P]),
+        "fdo_set_edge_gain": (None, [P, P]),
+        "fdo_set_masks": (None, [P, P, P]),
+        "fdo_set_driven": (None, [P, P, I32]),
+        "fdo_set_loss": (None, [P, P, I32]),
+        "fdo_set_readout": (None, [P, P, I32, P, D]),
+        "fdo_forward": (D, [P, I32, D, I32]),
+        "fdo_backward": (None, [P, I32, P, P, P, P, P]),
+        "fdo_state": (ctypes.POINTER(F), [P, I32]),
+        "fdo_rng": (ctypes.c_uint32, [P]),
+        "fdo_spikes": (I64, [P, P, P, P, I64]),
+        "fdo_readout": (I64, [P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _libs["diff"] = lib
+    return lib
+
+
+class OracleDiff:
+    """The differentiable model on the CPU checker: LIFDiff's forward with a
+    tape, and its backward (src/lifdiff.js:84-686), plus the per-edge gradient
+    and the filtered-rate readout loss.
+
+    ``counts`` are the delivered synapse counts (gate folded in as zeros),
+    ``sign`` is +1 / -1 / 0 per neuron; ``params`` holds LIFDiff's options."""
+
+    def __init__(self, N, indptr, indices, counts, sign, params: dict):
+        self.lib = _diff_lib()
+        self.N = int(N)
+        self.p = dict(params)
+        self._indptr = np.ascontiguousarray(indptr, dtype=np.uint32)
+        self._indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.E = int(self._indices.size)
+        self._counts = np.ascontiguousarray(counts, dtype=np.float32)
+        self._sign = np.ascontiguousarray(sign, dtype=np.float32)
+        ps = _DiffParams()
+        for n in DIFF_FIELDS:
+            setattr(ps, n, float(self.p[n]))
+        ps.soft = 1 if self.p["soft"] else 0
+        ps.checkpoint = int(self.p["checkpoint"])
+        self.h = self.lib.fdo_create(self.N, self.E, _ptr(self._indptr), _ptr(self._indices), _ptr(self._counts),
+                                     _ptr(self._sign), ctypes.byref(ps))
+        self._keep = []
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.fdo_destroy(h)
+            self.h = None
+
+    def set_input(self, name: str, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        assert a.size == self.N
+        self.lib.fdo_set_input(self.h, DIFF_INPUTS.index(name), _ptr(a))
+
+    def set_edge_gain(self, eg):
+        if eg is None:
+            self.lib.fdo_set_edge_gain(self.h, None)
+            return
+        a = np.ascontiguousarray(eg, dtype=np.float32)
+        assert a.size == self.E
+        self.lib.fdo_set_edge_gain(self.h, _ptr(a))
+
+    def set_masks(self, thr_mask=None, bias_mask=None):
+        t = None if thr_mask is None else np.ascontiguousarray(thr_mask, dtype=np.uint8)
+        b = None if bias_mask is None else np.ascontiguousarray(bias_mask, dtype=np.uint8)
+        self.lib.fdo_set_masks(self.h, None if t is None else _ptr(t), None if b is None else _ptr(b))
+
+    def set_driven(self, ix):
+        a = np.ascontiguousarray(ix, dtype=np.int32)
+        self.lib.fdo_set_driven(self.h, _ptr(a), int(a.size))
+
+    def set_loss(self, dLdSpike, loss_from: int = 0):
+        a = np.ascontiguousarray(dLdSpike, dtype=np.float32)
+        self.lib.fdo_set_loss(self.h, _ptr(a), int(loss_from))
+
+    def set_readout(self, idx, target_hz, tau_ms: float):
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        t = np.ascontiguousarray(target_hz, dtype=np.float32)
+        self.R = int(i.size)
+        self.lib.fdo_set_readout(self.h, _ptr(i), self.R, _ptr(t), float(tau_ms))
+
+    def forward(self, steps: int, seed=1, record: bool = True) -> float:
+        self.steps = int(steps)
+        return float(self.lib.fdo_forward(self.h, self.steps, float(seed), 1 if record else 0))
+
+    def backward(self, truncate: int = 0) -> dict:
+        g = np.zeros(9, dtype=np.float64)
+        lg = np.zeros(self.N, dtype=np.float32)
+        ge = np.zeros(max(1, self.E), dtype=np.float32)
+        gi = np.zeros(self.N, dtype=np.float32)
+        cl = np.zeros(3, dtype=np.int32)
+        self.lib.fdo_backward(self.h, int(truncate), _ptr(g), _ptr(lg), _ptr(ge), _ptr(gi), _ptr(cl))
+        return {"params": dict(zip(DIFF_PARAMS, g.tolist())), "logGain": lg, "edge": ge[: self.E],
+                "inScale": gi, "clamped": cl}
+
+    def state(self, name: str) -> np.ndarray:
+        ptr = self.lib.fdo_state(self.h, DIFF_STATE.index(name))
+        return np.ctypeslib.as_array(ptr, shape=(self.N,)).copy()
+
+    @property
+    def rng(self) -> int:
+        return int(self.lib.fdo_rng(self.h))
+
+    def spikes(self):
+        """(offsets[steps+1], idx, amp, forced) of the recorded spike lists."""
+        off = np.zeros(self.steps + 1, dtype=np.int64)
+        tot = int(self.lib.fdo_spikes(self.h, _ptr(off), None, None, 0))
+        idx = np.zeros(max(1, tot), dtype=np.int32)
+        amp = np.zeros(max(1, tot), dtype=np.float32)
+        self.lib.fdo_spikes(self.h, _ptr(off), _ptr(idx), _ptr(amp), tot)
+        u = idx[:tot].view(np.uint32)
+        return off, (u & 0x7FFFFFFF).astype(np.int32), amp[:tot], (u >> 31).astype(bool)
+
+    def readout(self):
+        m = self.steps * self.R
+        s = np.zeros(max(1, m), dtype=np.float32)
+        r = np.zeros(max(1, m), dtype=np.float64)
+        self.lib.fdo_readout(self.h, _ptr(s), _ptr(r))
+        return s[:m].reshape(self.steps, self.R), r[:m].reshape(self.steps, self.R)

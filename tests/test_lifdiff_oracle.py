@@ -1,0 +1,202 @@
+"""Pins the CPU restatement of the differentiable model (oracle/lifdiff_oracle.c).
+
+The reference's own gradient check (scripts/grad_check.mjs) compares the
+adjoint with central finite differences of the smooth (soft-threshold) model;
+the same comparison, with its steps and tolerance, is made here for the nine
+scalar gradients, per-neuron log-gains, per-type gains, single edges and the
+filtered-rate readout loss.  Invariants of the per-edge gradient (row and
+column sums), replay exactness across checkpoint lengths, and the drive
+stream against the lif.c restatement pin what finite differences leave open."""
+import importlib
+
+import numpy as np
+import pytest
+
+import diffcases as dc
+
+synth = importlib.import_module("fly-brain_amd.synth")
+P = importlib.import_module("fly-brain_amd.params")
+
+STEPS = 60
+
+
+@pytest.fixture(scope="module")
+def base(oracle_built):
+    pr = dc.Problem()
+    net = pr.oracle()
+    L = net.forward(STEPS, seed=1)
+    g = net.backward()
+    return pr, L, g
+
+
+def _loss(pr, **kw):
+    return pr.oracle(**kw).forward(STEPS, seed=1, record=False)
+
+
+@pytest.mark.parametrize("name", P.DIFF_PARAMS)
+def test_scalar_gradients_match_finite_differences(base, name):
+    pr, L, g = base
+    h = dc.H[name]
+    fd = (_loss(pr, params={name: pr.p[name] + h}) - _loss(pr, params={name: pr.p[name] - h})) / (2 * h)
+    ok, rel = dc.fd_ok(g["params"][name], fd, h, L)
+    assert ok, (name, g["params"][name], fd, rel)
+    assert g["params"][name] != 0.0
+
+
+def test_log_gain_gradients_match_finite_differences(base):
+    pr, L, g = base
+    h = 2e-2
+    for i in (0, 1, 2, pr.g.N // 3, pr.g.N // 2):
+        up = pr.inputs["logGain"].copy()
+        dn = up.copy()
+        up[i] += h
+        dn[i] -= h
+        fd = (_loss(pr, inputs={"logGain": up}) - _loss(pr, inputs={"logGain": dn})) / (2 * h)
+        ok, rel = dc.fd_ok(g["logGain"][i], fd, h, L)
+        assert ok, (i, g["logGain"][i], fd, rel)
+
+
+def test_type_gain_gradient_is_the_type_sum_of_log_gain_gradients(base):
+    """outScale[i] = gain[type(i)]; d L / d log gain_T = sum over the type of
+    gLogGain (src/brainmodel.js:63-100 semantics of a type gain)."""
+    pr, L, g = base
+    types = np.random.default_rng(5).integers(0, 12, size=pr.g.N)
+    h = 1e-3
+    for T in (0, 3, 7):
+        mine = types == T
+        adj = float(g["logGain"][mine].astype(np.float64).sum())
+        up = pr.inputs["outScale"].astype(np.float64).copy()
+        dn = up.copy()
+        up[mine] *= np.exp(h)
+        dn[mine] *= np.exp(-h)
+        fd = (_loss(pr, inputs={"outScale": up.astype(np.float32)}) -
+              _loss(pr, inputs={"outScale": dn.astype(np.float32)})) / (2 * h)
+        ok, rel = dc.fd_ok(adj, fd, h, L)
+        assert ok, (T, adj, fd, rel)
+
+
+def test_edge_gradients_match_finite_differences(base):
+    pr, L, g = base
+    E = pr.g.indices.size
+    top = np.argsort(-np.abs(g["edge"]))[:6]
+    h = 5e-2
+    for j in top:
+        up = np.ones(E, np.float32)
+        dn = np.ones(E, np.float32)
+        up[j] += h
+        dn[j] -= h
+        fd = (_loss(pr, edge_gain=up) - _loss(pr, edge_gain=dn)) / (2 * h)
+        assert abs(g["edge"][j] - fd) <= 2e-3 * abs(fd) + 4e-6 * abs(L) / h, (j, g["edge"][j], fd)
+
+
+def test_edge_gradient_row_and_column_invariants(base):
+    """sum over row(pre) of gEdge = gLogGain[pre]; sum over edges into q of
+    gEdge = inScale[q] * gInScale[q] (edge gains 1)."""
+    pr, L, g = base
+    pre = np.repeat(np.arange(pr.g.N), np.diff(pr.g.indptr.astype(np.int64)))
+    row = np.bincount(pre, weights=g["edge"].astype(np.float64), minlength=pr.g.N)
+    scale = np.abs(g["logGain"]).max()
+    assert np.abs(row - g["logGain"]).max() <= 1e-5 * scale
+    col = np.bincount(pr.g.indices, weights=g["edge"].astype(np.float64), minlength=pr.g.N)
+    net = pr.oracle()
+    net.forward(1, seed=1, record=False)
+    ins = net.state("inScale").astype(np.float64)
+    want = ins * g["inScale"]
+    assert np.abs(col - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_readout_rate_loss_gradients_match_finite_differences(oracle_built):
+    """The filtered-rate MSE on readouts, alone (flat spike weights zero)."""
+    pr = dc.Problem(seed=13).with_readout(R=40, tau_ms=20.0)
+    pr.dL[:] = 0.0
+    net = pr.oracle()
+    L = net.forward(STEPS, seed=2)
+    g = net.backward()
+    s, r = net.readout()
+    # the loss is the mean squared rate error
+    assert L == pytest.approx(float(((r - pr.readout[1][None, :]) ** 2).mean()), rel=1e-12)
+    for name in ("wSyn", "vThresh", "adaptInc", "depU", "inhGain"):
+        h = dc.H[name]
+        fd = (pr.oracle(params={name: pr.p[name] + h}).forward(STEPS, 2, False) -
+              pr.oracle(params={name: pr.p[name] - h}).forward(STEPS, 2, False)) / (2 * h)
+        ok, rel = dc.fd_ok(g["params"][name], fd, h, L)
+        assert ok, (name, g["params"][name], fd, rel)
+
+
+def test_rate_filter_closed_form(oracle_built):
+    """One readout driven by a forced spike train: r_t is the exponential
+    filter of s_t / (dt/1000) with a = exp(-dt/tau)."""
+    pr = dc.Problem(N=64, deg=0.0, seed=3, over={"soft": False}, drive_every=1, drive_hz=400.0)
+    pr.readout = (np.array([5], np.int32), np.array([10.0], np.float32), 20.0)
+    net = pr.oracle()
+    net.forward(200, seed=4)
+    s, r = net.readout()
+    a = np.exp(-pr.p["dt"] / 20.0)
+    want = np.zeros(200)
+    acc = 0.0
+    for t in range(200):
+        acc = a * acc + (1 - a) * float(s[t, 0]) / (pr.p["dt"] / 1000.0)
+        want[t] = acc
+    assert s[:, 0].sum() > 3
+    assert np.allclose(r[:, 0], want, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("soft", [True, False])
+def test_gradients_do_not_depend_on_checkpoint_length(oracle_built, soft):
+    """Segment replay reproduces the recorded trajectory exactly, so the
+    gradients are bit-identical for any checkpoint spacing."""
+    out = []
+    for K in (16, 7, 60, 1):
+        pr = dc.Problem(over={"soft": soft, "checkpoint": K, "tRef": 2.2 if not soft else 0.0})
+        net = pr.oracle()
+        net.forward(STEPS, seed=1)
+        out.append(net.backward())
+    for o in out[1:]:
+        assert o["params"] == out[0]["params"]
+        assert np.array_equal(o["logGain"], out[0]["logGain"])
+        assert np.array_equal(o["edge"], out[0]["edge"])
+
+
+def test_truncation_and_clip_limits(oracle_built):
+    pr = dc.Problem()
+    net = pr.oracle()
+    net.forward(STEPS, seed=1)
+    full = net.backward()
+    same = net.backward(truncate=STEPS)          # clears only before the last step
+    assert same["params"] == full["params"]
+    short = net.backward(truncate=5)
+    assert short["params"]["vThresh"] != full["params"]["vThresh"]
+    pr2 = dc.Problem(over={"adjClip": 1e30})
+    n2 = pr2.oracle()
+    n2.forward(STEPS, seed=1)
+    assert n2.backward()["params"] == full["params"]
+
+
+def test_hard_forward_drive_stream_matches_lif_c(oracle_built):
+    """With no synapses, LIFDiff's hard forward and lif.c draw the same
+    xorshift32 stream over the same driven order, so the forced spikes agree
+    step by step (src/lifdiff.js:203-213 vs src/wasm/lif.c:49-52)."""
+    from oracle import OracleBrain
+    N = 200
+    g = synth.Graph(N=N, indptr=np.zeros(N + 1, np.uint32), indices=np.zeros(0, np.uint32),
+                    counts=np.zeros(0, np.int32), exc=np.ones(N, bool))
+    drv = np.random.default_rng(2).permutation(N)[:80].astype(np.int32)
+    rates = np.random.default_rng(3).uniform(20, 400, size=80).astype(np.float32)
+    pd = P.diff_merged({"dt": 0.1, "tRef": 2.2})
+    from oracle import OracleDiff
+    od = OracleDiff(N, g.indptr, g.indices, np.zeros(0, np.float32), np.ones(N, np.float32), pd)
+    drive = np.zeros(N, np.float32)
+    drive[drv] = rates
+    od.set_input("drive", drive)
+    od.set_driven(drv)
+    od.set_loss(np.zeros(N, np.float32))
+    od.forward(3000, seed=7)
+    off, idx, amp, forced = od.spikes()
+    pl = P.config_params(tRef=2.2)
+    ob = OracleBrain(g, pl, seed=7, W=np.zeros(0, np.float32), sign=np.ones(N, np.float32))
+    for i, r in zip(drv, rates):
+        ob.setDriveOne(int(i), float(r))
+    nf, ras, _ = ob.run(3000, raster_cap=3000 * N)
+    assert forced.all() and np.array_equal(np.diff(off), nf)
+    assert np.array_equal(idx, ras)
+    assert od.rng == ob.rng
