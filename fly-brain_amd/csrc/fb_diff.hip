@@ -1,0 +1,1275 @@
+// fb_diff.hip -- the differentiable model on the GPU: forward with a tape and
+// checkpoints, segment replay into an arena, and the reverse sweep.
+//
+// The model is the reference's LIFDiff (src/lifdiff.js:84-686): COBA membrane,
+// per-neuron log-gain, type gain (outScale), threshold offset, size scaling
+// (inScale), a masked threshold term and bias, depression, adaptation, a hard
+// or logistic ("soft") threshold, and the fast-sigmoid surrogate in the
+// backward pass.  LIFDiff evaluates in doubles and stores into Float32Arrays;
+// these kernels do the same (f64 locals, one rounding per stored value, in the
+// reference's evaluation order), and deliver each step's arrivals with the
+// ordered-sum scheme of the forward engine, so a step's conductance receives
+// its arrivals in list order exactly as the serial loop adds them.
+//
+// Per forward step (t):
+//   kd_deliver   wave per arrival (the spikes of step t - D from the tape):
+//                res read and depressed, the row's addends (double) go into
+//                per-(sign class, target) accumulators keyed by arrival rank
+//                (driven position for a forced spike, N + index otherwise)
+//                and position in row.
+//   kd_driven    one workgroup: the drive draws in set order from the serial
+//                xorshift32 stream (leapfrog-parallel, as in the forward
+//                engine), forced spikes appended to the tape in set order.
+//   kd_membrane  dense pass: ordered sums, integrate, decay, threshold, spike
+//                amplitude, tape append (forward) or arena record (replay).
+// Per backward step (t, reverse):
+//   kd_bneuron   dense pass: reversed decays, mode dispatch, surrogate, the
+//                adjoint state update; scalar terms reduced per workgroup.
+//   kd_barrival  wave per arrival: the fan-out gather of lgE / lgI into lRaw,
+//                per-edge gradient, gInScale scatter, gLogGain, the spike
+//                adjoint of the emitting step, depression adjoint.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fb_internal.h"
+
+namespace fb {
+
+uint32_t seed_hash(double seed);
+fb_status leap_tables(uint32_t **jump, uint32_t **leap);
+
+constexpr uint32_t FORCED_BIT = 0x80000000u;
+constexpr int DIFF_BLOCK = 256;
+
+struct DiffColl {
+    uint64_t key;
+    double addend;
+    int32_t next;
+    int32_t pad;
+};
+
+struct DiffDev {
+    int32_t N, S, D, K, R, nDriven, soft, lossFrom;
+    int64_t E;
+    double dt, vRest, vThresh, vReset, wSyn, adaptInc, depU, eExc, eInh, inhGain, kc, lamina, beta, aClip;
+    double dE, dA, kRec, kM, dtS, cE, cI, emitThr, sizeAlpha, boostCap, minScale;
+    float forcedRefr, tRefF;
+    // graph and inputs
+    const uint32_t *indptr, *indices;
+    const float *counts, *edgeGain;   // edgeGain null = every edge at 1
+    const float *sign, *outScale, *thrOffset, *inScale, *drive, *sizeLog, *dLdSpike;
+    const double *expLogGain;
+    const uint8_t *thrMask, *biasMask;
+    const int32_t *driven, *drvPos, *readIdx;
+    // state
+    float *v, *gE, *gI, *adapt, *res, *refr, *spikeCount, *scored;
+    uint32_t *rng;
+    uint8_t *forced;
+    int32_t *dflag;
+    // ordered delivery
+    int32_t *cnt, *chain;
+    double *add;
+    uint64_t *key;
+    DiffColl *coll;
+    int64_t collHalf;
+    int32_t *ncoll;                   // [2], by step parity
+    // tape
+    int64_t *off;                     // [steps + 1]
+    unsigned long long *top;
+    int32_t *idx;
+    float *amp;
+    int64_t cap;
+    float *readS;                     // [steps][R]
+    int32_t *err;
+    unsigned long long *events;       // [0] edge events, [1] arrivals
+    const uint32_t *leap;
+    // arena, K steps
+    float *aV, *aGE, *aGI, *aAd, *aSp, *resAt;
+    uint8_t *aMode;
+    // adjoint
+    float *lv, *lgE, *lgI, *lad, *lres, *ring, *gLogGain, *gEdge, *gInScale;
+    double *accW, *accDepU, *accInh, *blockAcc;
+    double *rate, *dLds;               // [steps][R]
+    const float *target;
+};
+
+enum { DERR_COLL = 1, DERR_TAPE = 2, DERR_DRAWS = 4 };
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    return x;
+}
+
+// g + the arrivals of one accumulator, in key order, each rounded to f32 as a
+// Float32Array += does
+__device__ inline float ordered_sum_d(float g, int32_t count, uint64_t key0, double add0, int32_t head,
+                                      const DiffColl *coll) {
+    if (count <= 32) {
+        uint64_t ks[32];
+        double as[32];
+        int n = 0;
+        ks[n] = key0; as[n] = add0; n++;
+        for (int32_t p = head; p >= 0 && n < 32; p = coll[p].next) {
+            const uint64_t kk = coll[p].key;
+            const double aa = coll[p].addend;
+            int m = n++;
+            while (m > 0 && ks[m - 1] > kk) { ks[m] = ks[m - 1]; as[m] = as[m - 1]; m--; }
+            ks[m] = kk; as[m] = aa;
+        }
+        for (int m = 0; m < n; m++) g = (float)((double)g + as[m]);
+        return g;
+    }
+    uint64_t last = 0;
+    bool first = true;
+    for (int32_t c = 0; c < count; c++) {
+        uint64_t best = ~0ull;
+        double ba = 0.0;
+        if ((first || key0 > last) && key0 < best) { best = key0; ba = add0; }
+        for (int32_t p = head; p >= 0; p = coll[p].next) {
+            const uint64_t kk = coll[p].key;
+            if ((first || kk > last) && kk < best) { best = kk; ba = coll[p].addend; }
+        }
+        g = (float)((double)g + ba);
+        last = best;
+        first = false;
+    }
+    return g;
+}
+
+__device__ __forceinline__ float clip_to(double x, double c) {
+    if (c > 0) x = x > c ? c : (x < -c ? -c : x);
+    return (float)x;
+}
+
+// ------------------------------------------------------------------ forward / replay
+
+// The arrivals of step t: the tape's spikes of step t - D.  resBase is the
+// tape index of the replay segment's first arrival.
+template <bool REPLAY>
+__global__ void __launch_bounds__(256) kd_deliver(DiffDev d, int32_t t, int32_t resFromT) {
+    if (t < d.D) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    // with a one-step delay the list being read is the previous step's, whose
+    // end the tape counter still holds (its offset is written by kd_driven)
+    // (an overflowed tape, already flagged, is read only up to its capacity)
+    const int64_t a0 = min(d.off[t - d.D], d.cap);
+    const int64_t a1 = min((!REPLAY && d.D == 1) ? (int64_t)*d.top : d.off[t - d.D + 1], d.cap);
+    const int64_t resBase = REPLAY ? d.off[resFromT] : 0;
+    for (int64_t k = a0 + wave; k < a1; k += nwaves) {
+        const uint32_t e = (uint32_t)d.idx[k];
+        const int32_t pre = (int32_t)(e & ~FORCED_BIT);
+        const float sg = d.sign[pre];
+        if (sg == 0.f) continue;
+        const float resv = d.res[pre];
+        const double eff = (double)sg * d.wSyn * (double)d.outScale[pre] * d.expLogGain[pre] * (double)resv *
+                           (double)d.amp[k];
+        const uint32_t ra = d.indptr[pre], re = d.indptr[pre + 1];
+        if (lane == 0) {
+            if (REPLAY) d.resAt[k - resBase] = resv;
+            d.res[pre] = (float)((double)resv - d.depU * (double)resv);
+            if (!REPLAY) {
+                atomicAdd(&d.events[0], (unsigned long long)(re - ra));
+                atomicAdd(&d.events[1], 1ull);
+            }
+        }
+        const uint32_t rank = (e & FORCED_BIT) ? (uint32_t)d.drvPos[pre] : (uint32_t)d.N + (uint32_t)pre;
+        const bool exc = eff > 0;
+        const int32_t cls = exc ? 0 : d.N;
+        const int32_t par = t & 1;
+        for (uint32_t j = ra + lane; j < re; j += 64) {
+            const uint32_t q = d.indices[j];
+            const double egj = d.edgeGain ? (double)d.edgeGain[j] : 1.0;
+            const double addend = exc ? (double)d.counts[j] * egj * (double)d.inScale[q] * eff
+                                      : (double)d.counts[j] * egj * (double)d.inScale[q] * d.inhGain * eff;
+            const int32_t ci = cls + (int32_t)q;
+            const uint64_t kk = ((uint64_t)rank << 32) | (uint64_t)(j - ra);
+            const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+            if (slot == 0) {
+                d.add[ci] = addend;
+                d.key[ci] = kk;
+            } else {
+                const int32_t p = atomicAdd(&d.ncoll[par], 1);
+                if (p < d.collHalf) {
+                    const int32_t gp = (int32_t)(par * d.collHalf + p);
+                    d.coll[gp].key = kk;
+                    d.coll[gp].addend = addend;
+                    d.coll[gp].next = atomicExch(&d.chain[ci], gp);
+                } else {
+                    atomicOr(d.err, DERR_COLL);
+                }
+            }
+        }
+    }
+}
+
+// The drive pre-pass (src/lifdiff.js:290-297): one draw per non-refractory
+// driven neuron in set order; forced spikes go to the tape in that order.
+template <bool REPLAY>
+__global__ void __launch_bounds__(1024) kd_driven(DiffDev d, int32_t t) {
+    constexpr int T = 1024;
+    __shared__ uint32_t leapS[LEAP_LDS_WORDS];
+    __shared__ uint32_t ybase[64];
+    __shared__ uint32_t draws[LEAP_ROUND];
+    __shared__ int32_t scan[64 + T / 64];
+    __shared__ uint32_t s_rng;
+    __shared__ unsigned long long s_top;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_rng = *d.rng;
+        if (!REPLAY) {
+            s_top = *d.top;
+            d.off[t] = (int64_t)s_top;      // this step's list starts here
+        }
+    }
+    leap_load<T>(leapS, d.leap);
+    __syncthreads();
+    uint32_t base = s_rng;
+    const int32_t nD = d.nDriven;
+    const int32_t per = (nD + T - 1) / T;
+    const int32_t lo = min(nD, tid * per), hi = min(nD, lo + per);
+    int32_t mine = 0;
+    for (int32_t q = lo; q < hi; q++)
+        if (!(d.refr[d.driven[q]] > 0)) mine++;
+    int32_t totalDraws;
+    const int32_t off = wg_exclusive_scan<T>(mine, scan, &totalDraws);
+    for (int32_t q2 = lo; q2 < hi; q2++) d.dflag[q2] = 0;
+    int32_t q = lo, next = off, fired = 0;
+    for (int32_t r0 = 0; r0 < totalDraws; r0 += LEAP_ROUND) {
+        const int32_t cnt = min(LEAP_ROUND, totalDraws - r0);
+        leap_round(leapS, ybase, draws, cnt, base);
+        while (next < off + mine && next < r0 + cnt) {
+            const int32_t i = d.driven[q];
+            if (!(d.refr[i] > 0)) {
+                const double u = (double)(draws[next - r0] >> 8) * (1.0 / 16777216.0);
+                if (u < (double)d.drive[i] * d.dtS) {
+                    d.refr[i] = d.forcedRefr;
+                    d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+                    d.forced[i] = 1;
+                    if (!REPLAY) {
+                        d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
+                        if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+                        const int32_t rk = d.readIdx[i];
+                        if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
+                    }
+                    d.dflag[q] = 1;
+                    fired++;
+                }
+                next++;
+            }
+            q++;
+        }
+        base = draws[cnt - 1];
+        __syncthreads();
+    }
+    if (!REPLAY) {
+        int32_t totalFired;
+        int32_t pos = wg_exclusive_scan<T>(fired, scan, &totalFired);
+        for (int32_t q2 = lo; q2 < hi; q2++)
+            if (d.dflag[q2]) {
+                const int64_t k = (int64_t)s_top + pos++;
+                if (k < d.cap) {
+                    d.idx[k] = (int32_t)((uint32_t)d.driven[q2] | FORCED_BIT);
+                    d.amp[k] = 1.f;
+                } else {
+                    atomicOr(d.err, DERR_TAPE);
+                }
+            }
+        if (tid == 0) *d.top = s_top + (unsigned long long)totalFired;
+    }
+    if (tid == 0) *d.rng = base;
+}
+
+// Integrate, decay, threshold (src/lifdiff.js:313-345; replay 422-457).
+template <bool REPLAY>
+__global__ void __launch_bounds__(DIFF_BLOCK) kd_membrane(DiffDev d, int32_t t, int32_t r) {
+    const int32_t N = d.N;
+    const int32_t i = blockIdx.x * DIFF_BLOCK + threadIdx.x;
+    bool emit = false;
+    float ampOut = 0.f;
+    if (i < N) {
+        float gEf = d.gE[i], gIf = d.gI[i];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int32_t ci = c * N + i;
+            const int32_t n = d.cnt[ci];
+            if (n) {
+                float g = c ? gIf : gEf;
+                if (n == 1) {
+                    g = (float)((double)g + d.add[ci]);
+                } else {
+                    g = ordered_sum_d(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
+                    d.chain[ci] = -1;
+                }
+                d.cnt[ci] = 0;
+                if (c) gIf = g; else gEf = g;
+            }
+        }
+        const float vf = d.v[i];
+        double vi = vf, s = 0.0;
+        float refrf = d.refr[i], adaptf = d.adapt[i], resf = d.res[i];
+        const bool wasRefr = refrf > 0;
+        uint8_t md = 0;
+        const int64_t ar = (int64_t)r * N + i;
+        if (REPLAY) { d.aV[ar] = vf; d.aGE[ar] = gEf; d.aGI[ar] = gIf; }
+        if (wasRefr) {
+            refrf = (float)((double)refrf - d.dt);
+            vi = d.vReset;
+            if (REPLAY) {
+                if (d.forced[i]) { md = 2; s = 1.0; } else md = 1;
+            }
+        } else {
+            const double bias = d.biasMask[i] ? d.lamina : 0.0;
+            vi += (d.vRest - vi + (double)gEf * (d.eExc - vi) * d.cE + (double)gIf * (vi - d.eInh) * d.cI + bias) * d.kM;
+        }
+        gEf = (float)((double)gEf * d.dE);
+        gIf = (float)((double)gIf * d.dE);
+        adaptf = (float)((double)adaptf * d.dA);
+        resf = (float)((double)resf + (1.0 - (double)resf) * d.kRec);
+        if (REPLAY) d.aAd[ar] = adaptf;
+        if (!wasRefr) {
+            const double thr = d.vThresh + (double)adaptf + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
+            s = d.soft ? 1.0 / (1.0 + exp(-(vi - thr) / d.beta)) : (vi >= thr ? 1.0 : 0.0);
+            if (s > 0) {
+                vi = s * d.vReset + (1.0 - s) * vi;
+                if (!d.soft) refrf = d.tRefF;
+                adaptf = (float)((double)adaptf + s * d.adaptInc);
+            }
+            if (!REPLAY) {
+                d.spikeCount[i] = (float)((double)d.spikeCount[i] + s);
+                if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + s);
+                const int32_t rk = d.readIdx[i];
+                if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = (float)s;
+                emit = s > d.emitThr;
+                ampOut = (float)s;
+            }
+        }
+        if (REPLAY) { d.aMode[ar] = md; d.aSp[ar] = (float)s; }
+        d.forced[i] = 0;
+        d.v[i] = (float)vi;
+        d.gE[i] = gEf; d.gI[i] = gIf; d.adapt[i] = adaptf; d.res[i] = resf; d.refr[i] = refrf;
+    }
+    if (!REPLAY) {
+        const uint64_t m = __ballot(emit);
+        const int lane = threadIdx.x & 63;
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(d.top, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (emit) {
+            const int64_t k = (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+            if (k < d.cap) {
+                d.idx[k] = i;
+                d.amp[k] = ampOut;
+            } else {
+                atomicOr(d.err, DERR_TAPE);
+            }
+        }
+    }
+    // the other parity's collision pool was consumed by the previous step
+    if (i == 0) d.ncoll[(t + 1) & 1] = 0;
+}
+
+__global__ void kd_reset(DiffDev d, float vRestF) {
+    const int32_t N = d.N;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        d.v[i] = vRestF; d.gE[i] = 0.f; d.gI[i] = 0.f; d.adapt[i] = 0.f; d.res[i] = 1.f; d.refr[i] = 0.f;
+        d.spikeCount[i] = 0.f; d.scored[i] = 0.f; d.forced[i] = 0;
+        d.cnt[i] = 0; d.cnt[i + N] = 0; d.chain[i] = -1; d.chain[i + N] = -1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.ncoll[0] = 0; d.ncoll[1] = 0; *d.top = 0ull; d.off[0] = 0; *d.err = 0;
+    }
+}
+
+__global__ void kd_finish(DiffDev d, int32_t steps) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) d.off[steps] = (int64_t)*d.top;
+}
+
+// Restores a checkpoint's state (replay start) and clears the step buffers.
+__global__ void kd_restore(DiffDev d, const float *ck) {
+    const int32_t N = d.N;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        d.v[i] = ck[i]; d.gE[i] = ck[N + i]; d.gI[i] = ck[2 * N + i]; d.adapt[i] = ck[3 * N + i];
+        d.res[i] = ck[4 * N + i]; d.refr[i] = ck[5 * N + i];
+        d.forced[i] = 0;
+        d.cnt[i] = 0; d.cnt[i + N] = 0; d.chain[i] = -1; d.chain[i + N] = -1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.ncoll[0] = 0; d.ncoll[1] = 0;
+        *d.rng = __float_as_uint(ck[6 * N]);
+    }
+}
+
+__global__ void kd_save(DiffDev d, float *ck) {
+    const int32_t N = d.N;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        ck[i] = d.v[i]; ck[N + i] = d.gE[i]; ck[2 * N + i] = d.gI[i]; ck[3 * N + i] = d.adapt[i];
+        ck[4 * N + i] = d.res[i]; ck[5 * N + i] = d.refr[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ck[6 * N] = __uint_as_float(*d.rng);
+}
+
+// ------------------------------------------------------------------ losses
+
+// sum dLdSpike * scored counts, deterministic (one workgroup)
+__global__ void __launch_bounds__(1024) kd_loss_flat(DiffDev d, double *out) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int32_t i = threadIdx.x; i < d.N; i += 1024) acc += (double)d.dLdSpike[i] * (double)d.scored[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; w++) s += red[w];
+        out[0] = s;
+    }
+}
+
+// Filtered rates of the readouts and the loss's spike adjoint: one thread per
+// readout, forward filter then reverse filter.  Per-readout loss sums go to
+// part[k]; the caller adds them in order.
+__global__ void kd_readout(DiffDev d, int32_t T, double a, double *part) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.R) return;
+    const int32_t R = d.R;
+    const double tgt = (double)d.target[k];
+    double r = 0.0, L = 0.0;
+    for (int32_t t = 0; t < T; t++) {
+        r = a * r + (1.0 - a) * (double)d.readS[(int64_t)t * R + k] / d.dtS;
+        d.rate[(int64_t)t * R + k] = r;
+        const double e = r - tgt;
+        L += e * e;
+    }
+    part[k] = L;
+    double lam = 0.0;
+    const double TR = (double)T * (double)R;
+    for (int32_t t = T - 1; t >= 0; t--) {
+        lam = 2.0 * (d.rate[(int64_t)t * R + k] - tgt) / TR + a * lam;
+        d.dLds[(int64_t)t * R + k] = (1.0 - a) / d.dtS * lam;
+    }
+}
+
+// ------------------------------------------------------------------ backward
+
+__global__ void kd_bclear(DiffDev d, int32_t ringToo) {
+    const int32_t N = d.N;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        d.lv[i] = 0.f; d.lgE[i] = 0.f; d.lgI[i] = 0.f; d.lad[i] = 0.f; d.lres[i] = 0.f;
+        if (ringToo)
+            for (int32_t s = 0; s < d.S; s++) d.ring[(int64_t)s * N + i] = 0.f;
+    }
+}
+
+__global__ void kd_bzero(DiffDev d, int32_t nblk) {
+    const int32_t N = d.N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < d.E; q += stride) d.gEdge[q] = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += stride) {
+        d.gLogGain[i] = 0.f; d.gInScale[i] = 0.f; d.accW[i] = 0.0; d.accDepU[i] = 0.0; d.accInh[i] = 0.0;
+    }
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 5 * (int64_t)nblk; q += stride)
+        d.blockAcc[q] = 0.0;
+}
+
+// Reversed decays and integrate-and-spike of step t (src/lifdiff.js:555-617).
+__global__ void __launch_bounds__(DIFF_BLOCK) kd_bneuron(DiffDev d, int32_t t, int32_t r) {
+    __shared__ double red[5][DIFF_BLOCK / 64];
+    const int32_t N = d.N;
+    const int32_t i = blockIdx.x * DIFF_BLOCK + threadIdx.x;
+    double pA = 0, pV = 0, pK = 0, pL = 0, pEI = 0;
+    if (i < N) {
+        const int64_t ar = (int64_t)r * N + i;
+        const uint8_t md = d.aMode[ar];
+        const double lpA = d.lad[i];
+        const double lE = (double)d.lgE[i] * d.dE, lI = (double)d.lgI[i] * d.dE, lA = lpA * d.dA;
+        const double lR = (double)d.lres[i] * (1.0 - d.kRec);
+        float lgEf = (float)lE, lgIf = (float)lI, ladf = (float)lA;
+        const float lresf = (float)lR;
+        float lvf = d.lv[i];
+        float *ls = d.ring + (int64_t)(t % d.S) * N + i;
+        double dl = t >= d.lossFrom ? (double)d.dLdSpike[i] : 0.0;
+        const int32_t rk = d.readIdx[i];
+        if (rk >= 0) dl += d.dLds[(int64_t)t * d.R + rk];
+        if (md == 1) {
+            lvf = 0.f;
+        } else if (md == 2) {
+            pA += lA;
+            lvf = 0.f;
+            *ls = 0.f;
+        } else {
+            const double vv = d.aV[ar], gEv = d.aGE[ar], gIv = d.aGI[ar];
+            const double bias = d.biasMask[i] ? d.lamina : 0.0;
+            const double u = vv + (d.vRest - vv + gEv * (d.eExc - vv) * d.cE + gIv * (vv - d.eInh) * d.cI + bias) * d.kM;
+            const double thr = d.vThresh + (double)d.aAd[ar] + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
+            const double dd = u - thr;
+            const double s = d.aSp[ar];
+            const double sd = 0.0, st = s;
+            double sg;
+            if (d.soft) {
+                sg = st * (1.0 - st) / d.beta;
+            } else {
+                const double q = 1.0 + fabs(dd) / d.beta;
+                sg = 1.0 / (d.beta * q * q);
+            }
+            const double lspike = (double)*ls + dl;
+            const double lvNew = lvf;
+            const double lsTot = (d.vReset - u) * lvNew + lpA * d.adaptInc + lspike;
+            const double lt = lsTot * (1.0 - sd) * sg;
+            const double lu = lvNew * (1.0 - s) + lt;
+            const double lthr = -lt;
+            pA += lpA * s;
+            pV += lthr;
+            if (d.thrMask[i]) pK += lthr;
+            ladf = (float)(lA + lthr * d.dA);
+            lvf = (float)(lu * (1.0 + d.kM * (-1.0 - gEv * d.cE + gIv * d.cI)));
+            lgEf = (float)(lE + lu * d.kM * (d.eExc - vv) * d.cE);
+            lgIf = (float)(lI + lu * d.kM * (vv - d.eInh) * d.cI);
+            if (bias != 0.0) pL += lu * d.kM;
+            pEI += lu * d.kM * gIv * d.cI * (-1.0 + (vv - d.eInh) * d.cI);
+            *ls = 0.f;
+        }
+        if (d.aClip > 0) {
+            d.lv[i] = clip_to(lvf, d.aClip); d.lgE[i] = clip_to(lgEf, d.aClip); d.lgI[i] = clip_to(lgIf, d.aClip);
+            d.lad[i] = clip_to(ladf, d.aClip); d.lres[i] = clip_to(lresf, d.aClip);
+        } else {
+            d.lv[i] = lvf; d.lgE[i] = lgEf; d.lgI[i] = lgIf; d.lad[i] = ladf; d.lres[i] = lresf;
+        }
+    }
+    double p[5] = {pA, pV, pK, pL, pEI};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const double x = wave_sum(p[c]);
+        if (lane == 0) red[c][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double s = 0.0;
+        for (int w = 0; w < DIFF_BLOCK / 64; w++) s += red[threadIdx.x][w];
+        d.blockAcc[(int64_t)blockIdx.x * 5 + threadIdx.x] += s;
+    }
+}
+
+// Reversed delivery of step t (src/lifdiff.js:635-666) plus the per-edge gradient.
+__global__ void __launch_bounds__(256) kd_barrival(DiffDev d, int32_t t, int32_t resFromT) {
+    if (t < d.D) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t a0 = d.off[t - d.D], a1 = d.off[t - d.D + 1];
+    const int64_t resBase = d.off[resFromT];
+    const int32_t emitSlot = (t - d.D) % d.S;
+    for (int64_t k = a0 + wave; k < a1; k += nwaves) {
+        const int32_t pre = (int32_t)((uint32_t)d.idx[k] & ~FORCED_BIT);
+        const double sgn = d.sign[pre];
+        if (sgn == 0.0) continue;
+        const double amp = d.amp[k];
+        const double resAtv = d.resAt[k - resBase];
+        const double gain = d.expLogGain[pre] * (double)d.outScale[pre];
+        const double base = sgn * d.wSyn * gain * resAtv;
+        const double eff = base * amp;
+        const uint32_t ra = d.indptr[pre], re = d.indptr[pre + 1];
+        double lRaw = 0.0, dInh = 0.0;
+        if (eff > 0) {
+            for (uint32_t j = ra + lane; j < re; j += 64) {
+                const uint32_t q = d.indices[j];
+                const double c = d.counts[j];
+                const double cw = c * (d.edgeGain ? (double)d.edgeGain[j] : 1.0);
+                const double lq = d.lgE[q], inS = d.inScale[q];
+                lRaw += lq * cw * inS;
+                atomicAdd(&d.gInScale[q], (float)(lq * cw * eff));
+                d.gEdge[j] = (float)((double)d.gEdge[j] + lq * c * inS * eff);
+            }
+        } else {
+            const double ig = d.inhGain;
+            for (uint32_t j = ra + lane; j < re; j += 64) {
+                const uint32_t q = d.indices[j];
+                const double c = d.counts[j];
+                const double cw = c * (d.edgeGain ? (double)d.edgeGain[j] : 1.0);
+                const double lq = d.lgI[q], inS = d.inScale[q];
+                lRaw += lq * cw * inS * ig;
+                atomicAdd(&d.gInScale[q], (float)(lq * cw * ig * eff));
+                dInh += lq * cw * inS * eff;
+                d.gEdge[j] = (float)((double)d.gEdge[j] + lq * c * inS * ig * eff);
+            }
+        }
+        lRaw = wave_sum(lRaw);
+        dInh = wave_sum(dInh);
+        if (lane == 0) {
+            d.accW[pre] += lRaw * sgn * gain * resAtv * amp;
+            d.accInh[pre] += dInh;
+            d.gLogGain[pre] = (float)((double)d.gLogGain[pre] + lRaw * eff);
+            float *tg = d.ring + (int64_t)emitSlot * d.N + pre;
+            *tg = clip_to((double)*tg + lRaw * base, d.aClip);
+            const double lresAfter = d.lres[pre];
+            d.accDepU[pre] += -lresAfter * resAtv;
+            d.lres[pre] = clip_to(lresAfter * (1.0 - d.depU) + lRaw * sgn * d.wSyn * gain * amp, d.aClip);
+        }
+    }
+}
+
+// Final reduction into the nine scalar gradients (src/lifdiff.js:667-674), in
+// a fixed order: one workgroup.
+__global__ void __launch_bounds__(1024) kd_breduce(DiffDev d, int32_t nblk, double *out) {
+    __shared__ double red[9][16];
+    double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t b = threadIdx.x; b < nblk; b += 1024) {
+        p[7] += d.blockAcc[(int64_t)b * 5 + 0];   // adaptInc
+        p[4] += d.blockAcc[(int64_t)b * 5 + 1];   // vThresh
+        p[5] += d.blockAcc[(int64_t)b * 5 + 2];   // kcThreshold
+        p[6] += d.blockAcc[(int64_t)b * 5 + 3];   // laminaBias
+        p[3] += d.blockAcc[(int64_t)b * 5 + 4];   // eInh
+    }
+    for (int32_t i = threadIdx.x; i < d.N; i += 1024) {
+        p[0] += d.accW[i];
+        p[8] += d.accDepU[i];
+        p[2] += d.accInh[i];
+        const double sl = d.sizeLog[i];
+        const double raw = exp(-d.sizeAlpha * sl);
+        if (raw < d.boostCap && raw > d.minScale) p[1] += (double)d.gInScale[i] * (-sl) * raw;
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        const double x = wave_sum(p[c]);
+        if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        double s = 0.0;
+        for (int w = 0; w < 16; w++) s += red[threadIdx.x][w];
+        out[threadIdx.x] = s;
+    }
+}
+
+// Non-finite gradients become zero and are counted (src/lifdiff.js:676-685).
+__global__ void kd_sanitize(DiffDev d, int32_t *counts) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.N; i += stride)
+        if (!isfinite(d.gLogGain[i])) { d.gLogGain[i] = 0.f; atomicAdd(&counts[1], 1); }
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < d.E; j += stride)
+        if (!isfinite(d.gEdge[j])) { d.gEdge[j] = 0.f; atomicAdd(&counts[2], 1); }
+}
+
+}  // namespace fb
+
+// ====================================================================== host
+
+using namespace fb;
+
+struct fb_diff {
+    fb_diff_params p{};
+    DiffDev d{};
+    int32_t N = 0;
+    int64_t E = 0;
+    hipStream_t stream = nullptr, ownStream = nullptr;
+    std::vector<void *> allocs;
+    // host copies of the inputs
+    std::vector<float> in[FB_D_NINPUTS];
+    std::vector<int32_t> drivenList;
+    std::vector<float> dL;
+    std::vector<int32_t> readout;
+    std::vector<float> target;
+    double tauR = 20.0;
+    int32_t lossFrom = 0;
+    // device buffers whose size depends on the run
+    float *ckpt = nullptr;            // [ncheckpoints][6N + 1]
+    int32_t nck = 0;
+    int64_t tapeCap = 0;
+    int32_t tapeSteps = 0;
+    int32_t steps = 0;                // of the recorded forward
+    bool recorded = false;
+    int64_t perStepCap = 0;           // tape entries allowed per step on average
+    double *scal = nullptr;           // [0] flat loss, [1..9] params, [16..] readout parts
+    double *readPart = nullptr;
+    int32_t *clampDev = nullptr;
+    double lastLoss = 0.0;
+    double fwdMs = 0.0, bwdMs = 0.0, replayLaunches = 0.0;
+    int32_t nblk = 0;
+    std::vector<void *> runAllocs;    // freed and reallocated by forward()
+};
+
+#define FB_REQUIRE_D(cond, code, ...)    \
+    do {                                 \
+        if (!(cond)) {                   \
+            set_error(__VA_ARGS__);      \
+            return code;                 \
+        }                                \
+    } while (0)
+
+template <class T>
+static fb_status dalloc(std::vector<void *> &list, T **p, size_t count) {
+    void *q = nullptr;
+    FB_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    list.push_back(q);
+    *p = (T *)q;
+    return FB_OK;
+}
+
+static double js_round_d(double x) {
+    const double f = std::floor(x);
+    return (x - f >= 0.5) ? f + 1.0 : f;
+}
+
+static void derive(fb_diff *f) {
+    const fb_diff_params &p = f->p;
+    DiffDev &d = f->d;
+    const double r = js_round_d(p.delay / p.dt);
+    d.S = (int32_t)(r < 1 ? 1 : r) + 1;
+    d.D = d.S - 1;
+    d.K = p.checkpoint > 0 ? p.checkpoint : 16;
+    d.soft = p.soft ? 1 : 0;
+    d.dt = p.dt; d.vRest = p.vRest; d.vThresh = p.vThresh; d.vReset = p.vReset;
+    d.wSyn = p.wSyn; d.adaptInc = p.adaptInc; d.depU = p.depU; d.eExc = p.eExc; d.eInh = p.eInh;
+    d.inhGain = p.inhGain; d.kc = p.kcThreshold; d.lamina = p.laminaBias; d.beta = p.surrogateBeta;
+    d.aClip = p.adjClip;
+    d.dE = std::exp(-p.dt / p.tauSyn);
+    d.dA = std::exp(-p.dt / p.adaptTau);
+    d.kRec = p.dt / p.depTau;
+    d.kM = p.dt / p.tauM;
+    d.dtS = p.dt / 1000.0;
+    d.cE = 1.0 / (p.eExc - p.vRest);
+    d.cI = 1.0 / (p.vRest - p.eInh);
+    d.emitThr = p.soft ? 1e-4 : 0.5;
+    d.sizeAlpha = p.sizeAlpha;
+    d.boostCap = p.boostCap;
+    d.minScale = 1.0 / p.maxSizeScale;
+    d.forcedRefr = (float)(p.tRef + p.dt);
+    d.tRefF = (float)p.tRef;
+}
+
+// inScale and exp(logGain), formed on the host in f64 (src/lifdiff.js:158-164, 278)
+static fb_status upload_derived(fb_diff *f) {
+    const int32_t N = f->N;
+    std::vector<float> inS((size_t)N);
+    std::vector<double> eg((size_t)N);
+    const fb_diff_params &p = f->p;
+    for (int32_t i = 0; i < N; i++) {
+        const double s = std::exp(-p.sizeAlpha * (double)f->in[FB_D_SIZELOG][i]);
+        const double m = std::max(1.0 / p.maxSizeScale, s);
+        inS[i] = (float)std::min(p.boostCap, m);
+        eg[i] = std::exp((double)f->in[FB_D_LOGGAIN][i]);
+    }
+    FB_HIP(hipMemcpy((void *)f->d.inScale, inS.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy((void *)f->d.expLogGain, eg.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t *indices,
+                                    const float *counts, const float *sign, const fb_diff_params *p,
+                                    fb_diff **out) {
+    FB_REQUIRE_D(indptr && sign && p && out && (E == 0 || (indices && counts)), FB_ERR_ARG,
+                 "fb_diff_create: null argument");
+    FB_REQUIRE_D(N > 0 && E >= 0 && E < (1ll << 31), FB_ERR_ARG, "fb_diff_create: bad sizes N=%d E=%lld", N,
+                 (long long)E);
+    FB_REQUIRE_D(indptr[0] == 0 && (int64_t)indptr[N] == E, FB_ERR_ARG, "fb_diff_create: indptr must run 0..E");
+    for (int32_t i = 0; i < N; i++)
+        FB_REQUIRE_D(indptr[i] <= indptr[i + 1], FB_ERR_ARG, "fb_diff_create: indptr decreases at %d", i);
+    for (int64_t j = 0; j < E; j++)
+        FB_REQUIRE_D(indices[j] < (uint32_t)N, FB_ERR_ARG, "fb_diff_create: target %u out of range", indices[j]);
+    FB_REQUIRE_D(p->dt > 0 && p->tauM > 0 && p->tauSyn > 0 && p->adaptTau > 0 && p->depTau > 0 && p->surrogateBeta > 0,
+                 FB_ERR_ARG, "fb_diff_create: time constants and surrogateBeta must be positive");
+    fb_diff *f = new fb_diff();
+    f->p = *p;
+    f->N = N;
+    f->E = E;
+    DiffDev &d = f->d;
+    d.N = N;
+    d.E = E;
+    derive(f);
+    f->nblk = (N + DIFF_BLOCK - 1) / DIFF_BLOCK;
+    FB_HIP(hipStreamCreateWithFlags(&f->ownStream, hipStreamNonBlocking));
+    f->stream = f->ownStream;
+    for (int k = 0; k < FB_D_NINPUTS; k++) f->in[k].assign((size_t)N, k == FB_D_OUTSCALE ? 1.f : 0.f);
+    f->dL.assign((size_t)N, 0.f);
+    auto &A = f->allocs;
+    uint32_t *ip, *ix;
+    float *cn, *sg, *os, *to, *ins, *dr, *sl, *dls, *state;
+    double *elg, *accs;
+    uint8_t *masks, *forced;
+    int32_t *drv, *dpos, *rid, *dflag, *cnt, *chain, *ncoll, *err;
+    double *add;
+    uint64_t *key;
+    DiffColl *coll;
+    unsigned long long *top, *events;
+    uint32_t *rng;
+    fb_status st;
+    const size_t n = (size_t)N;
+    if ((st = dalloc(A, &ip, n + 1)) || (st = dalloc(A, &ix, (size_t)E)) || (st = dalloc(A, &cn, (size_t)E)) ||
+        (st = dalloc(A, &sg, n)) || (st = dalloc(A, &os, n)) || (st = dalloc(A, &to, n)) || (st = dalloc(A, &ins, n)) ||
+        (st = dalloc(A, &dr, n)) || (st = dalloc(A, &sl, n)) || (st = dalloc(A, &dls, n)) ||
+        (st = dalloc(A, &elg, n)) || (st = dalloc(A, &masks, 2 * n)) || (st = dalloc(A, &drv, n)) ||
+        (st = dalloc(A, &dpos, n)) || (st = dalloc(A, &rid, n)) || (st = dalloc(A, &dflag, n)) ||
+        (st = dalloc(A, &state, 8 * n)) || (st = dalloc(A, &forced, n)) || (st = dalloc(A, &rng, 1)) ||
+        (st = dalloc(A, &cnt, 2 * n)) || (st = dalloc(A, &chain, 2 * n)) || (st = dalloc(A, &add, 2 * n)) ||
+        (st = dalloc(A, &key, 2 * n)) || (st = dalloc(A, &ncoll, 2)) || (st = dalloc(A, &err, 1)) ||
+        (st = dalloc(A, &top, 1)) || (st = dalloc(A, &events, 2)) || (st = dalloc(A, &accs, 3 * n + 5 * (size_t)f->nblk)) ||
+        (st = dalloc(A, &f->scal, 16)) || (st = dalloc(A, &f->clampDev, 4)))
+        return st;
+    float *adj;
+    if ((st = dalloc(A, &adj, (size_t)(5 + d.S + 3) * n + (size_t)E))) return st;
+    d.collHalf = std::max<int64_t>(E, 1024);
+    if ((st = dalloc(A, &coll, 2 * (size_t)d.collHalf))) return st;
+    FB_HIP(hipMemcpy(ip, indptr, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (E) {
+        FB_HIP(hipMemcpy(ix, indices, sizeof(uint32_t) * E, hipMemcpyHostToDevice));
+        FB_HIP(hipMemcpy(cn, counts, sizeof(float) * E, hipMemcpyHostToDevice));
+    }
+    FB_HIP(hipMemcpy(sg, sign, sizeof(float) * n, hipMemcpyHostToDevice));
+    d.indptr = ip; d.indices = ix; d.counts = cn; d.edgeGain = nullptr; d.sign = sg;
+    d.outScale = os; d.thrOffset = to; d.inScale = ins; d.drive = dr; d.sizeLog = sl; d.dLdSpike = dls;
+    d.expLogGain = elg; d.thrMask = masks; d.biasMask = masks + n;
+    d.driven = drv; d.drvPos = dpos; d.readIdx = rid; d.dflag = dflag;
+    d.v = state; d.gE = state + n; d.gI = state + 2 * n; d.adapt = state + 3 * n; d.res = state + 4 * n;
+    d.refr = state + 5 * n; d.spikeCount = state + 6 * n; d.scored = state + 7 * n;
+    d.forced = forced; d.rng = rng; d.cnt = cnt; d.chain = chain; d.add = add; d.key = key; d.coll = coll;
+    d.ncoll = ncoll; d.err = err; d.top = top; d.events = events;
+    d.accW = accs; d.accDepU = accs + n; d.accInh = accs + 2 * n; d.blockAcc = accs + 3 * n;
+    d.lv = adj; d.lgE = adj + n; d.lgI = adj + 2 * n; d.lad = adj + 3 * n; d.lres = adj + 4 * n;
+    d.ring = adj + 5 * n; d.gLogGain = adj + (5 + d.S) * n; d.gInScale = adj + (6 + d.S) * n;
+    d.gEdge = adj + (8 + d.S) * n;
+    d.nDriven = 0; d.R = 0; d.lossFrom = 0;
+    uint32_t *jump = nullptr, *leap = nullptr;
+    if ((st = leap_tables(&jump, &leap))) return st;
+    d.leap = leap;
+    std::vector<float> ones(n, 1.f), zeros(n, 0.f);
+    std::vector<int32_t> neg(n, -1);
+    FB_HIP(hipMemcpy(os, ones.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    for (float *z : {to, dr, sl, dls}) FB_HIP(hipMemcpy(z, zeros.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    FB_HIP(hipMemset(masks, 0, 2 * n));
+    FB_HIP(hipMemcpy(dpos, neg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(rid, neg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    FB_HIP(hipMemset(events, 0, 2 * sizeof(unsigned long long)));
+    if ((st = upload_derived(f))) return st;
+    f->perStepCap = std::max<int64_t>(4096, N / 4);
+    *out = f;
+    return FB_OK;
+}
+
+static void free_run(fb_diff *f) {
+    for (void *q : f->runAllocs) (void)hipFree(q);
+    f->runAllocs.clear();
+    f->ckpt = nullptr;
+    f->d.off = nullptr; f->d.idx = nullptr; f->d.amp = nullptr; f->d.readS = nullptr;
+    f->d.aV = nullptr; f->d.rate = nullptr; f->d.dLds = nullptr;
+    f->recorded = false;
+}
+
+extern "C" fb_status fb_diff_destroy(fb_diff *f) {
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_destroy: null handle");
+    (void)hipStreamSynchronize(f->stream);
+    free_run(f);
+    for (void *q : f->allocs) (void)hipFree(q);
+    if (f->d.edgeGain) (void)hipFree((void *)f->d.edgeGain);
+    (void)hipFree((void *)f->d.target);
+    (void)hipFree(f->readPart);
+    (void)hipStreamDestroy(f->ownStream);
+    delete f;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_params(fb_diff *f, const fb_diff_params *p) {
+    FB_REQUIRE_D(f && p, FB_ERR_ARG, "fb_diff_set_params: null argument");
+    FB_REQUIRE_D(p->dt > 0 && p->tauM > 0 && p->tauSyn > 0 && p->adaptTau > 0 && p->depTau > 0 && p->surrogateBeta > 0,
+                 FB_ERR_ARG, "fb_diff_set_params: time constants and surrogateBeta must be positive");
+    const int32_t S0 = f->d.S;
+    fb_diff_params old = f->p;
+    f->p = *p;
+    derive(f);
+    if (f->d.S != S0) {
+        f->p = old;
+        derive(f);
+        set_error("fb_diff_set_params: delay / dt may not change the delay-line length");
+        return FB_ERR_ARG;
+    }
+    f->recorded = false;
+    return upload_derived(f);
+}
+
+extern "C" fb_status fb_diff_get_params(fb_diff *f, fb_diff_params *p) {
+    FB_REQUIRE_D(f && p, FB_ERR_ARG, "fb_diff_get_params: null argument");
+    *p = f->p;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_input(fb_diff *f, int32_t which, const float *values) {
+    FB_REQUIRE_D(f && values, FB_ERR_ARG, "fb_diff_set_input: null argument");
+    FB_REQUIRE_D(which >= 0 && which < FB_D_NINPUTS, FB_ERR_ARG, "fb_diff_set_input: unknown input %d", which);
+    const int32_t N = f->N;
+    f->in[which].assign(values, values + N);
+    FB_HIP(hipStreamSynchronize(f->stream));
+    float *dst = nullptr;
+    switch (which) {
+        case FB_D_OUTSCALE: dst = (float *)f->d.outScale; break;
+        case FB_D_THROFFSET: dst = (float *)f->d.thrOffset; break;
+        case FB_D_SIZELOG: dst = (float *)f->d.sizeLog; break;
+        case FB_D_DRIVE: dst = (float *)f->d.drive; break;
+        default: break;
+    }
+    if (dst) FB_HIP(hipMemcpy(dst, values, sizeof(float) * N, hipMemcpyHostToDevice));
+    f->recorded = false;
+    return upload_derived(f);
+}
+
+extern "C" fb_status fb_diff_set_edge_gain(fb_diff *f, const float *edgeGain) {
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_set_edge_gain: null handle");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (!edgeGain) {
+        if (f->d.edgeGain) FB_HIP(hipFree((void *)f->d.edgeGain));
+        f->d.edgeGain = nullptr;
+    } else {
+        if (!f->d.edgeGain) {
+            void *q = nullptr;
+            FB_HIP(hipMalloc(&q, sizeof(float) * std::max<int64_t>(1, f->E)));
+            f->d.edgeGain = (const float *)q;
+        }
+        if (f->E) FB_HIP(hipMemcpy((void *)f->d.edgeGain, edgeGain, sizeof(float) * f->E, hipMemcpyHostToDevice));
+    }
+    f->recorded = false;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_masks(fb_diff *f, const uint8_t *thrMask, const uint8_t *biasMask) {
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_set_masks: null handle");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (thrMask) FB_HIP(hipMemcpy((void *)f->d.thrMask, thrMask, (size_t)f->N, hipMemcpyHostToDevice));
+    if (biasMask) FB_HIP(hipMemcpy((void *)f->d.biasMask, biasMask, (size_t)f->N, hipMemcpyHostToDevice));
+    f->recorded = false;
+    return FB_OK;
+}
+
+// The driven set in insertion order (LIFDiff.drivenSet, src/lifdiff.js:203-207).
+extern "C" fb_status fb_diff_set_driven(fb_diff *f, const int32_t *idx, int32_t n) {
+    FB_REQUIRE_D(f && n >= 0 && (n == 0 || idx), FB_ERR_ARG, "fb_diff_set_driven: bad arguments");
+    std::vector<int32_t> pos((size_t)f->N, -1);
+    for (int32_t q = 0; q < n; q++) {
+        FB_REQUIRE_D(idx[q] >= 0 && idx[q] < f->N, FB_ERR_ARG, "fb_diff_set_driven: index %d out of range", idx[q]);
+        FB_REQUIRE_D(pos[idx[q]] < 0, FB_ERR_ARG, "fb_diff_set_driven: neuron %d listed twice", idx[q]);
+        pos[idx[q]] = q;
+    }
+    FB_HIP(hipStreamSynchronize(f->stream));
+    f->drivenList.assign(idx, idx + n);
+    if (n) FB_HIP(hipMemcpy((void *)f->d.driven, idx, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy((void *)f->d.drvPos, pos.data(), sizeof(int32_t) * f->N, hipMemcpyHostToDevice));
+    f->d.nDriven = n;
+    f->recorded = false;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_loss(fb_diff *f, const float *dLdSpike, int32_t lossFrom) {
+    FB_REQUIRE_D(f && lossFrom >= 0, FB_ERR_ARG, "fb_diff_set_loss: bad arguments");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (dLdSpike) f->dL.assign(dLdSpike, dLdSpike + f->N);
+    else f->dL.assign((size_t)f->N, 0.f);
+    FB_HIP(hipMemcpy((void *)f->d.dLdSpike, f->dL.data(), sizeof(float) * f->N, hipMemcpyHostToDevice));
+    f->d.lossFrom = lossFrom;
+    f->lossFrom = lossFrom;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_readout(fb_diff *f, const int32_t *idx, int32_t R, const float *targetHz,
+                                         double tauMs) {
+    FB_REQUIRE_D(f && R >= 0 && (R == 0 || (idx && targetHz)) && tauMs > 0, FB_ERR_ARG,
+                 "fb_diff_set_readout: bad arguments");
+    std::vector<int32_t> rid((size_t)f->N, -1);
+    for (int32_t k = 0; k < R; k++) {
+        FB_REQUIRE_D(idx[k] >= 0 && idx[k] < f->N, FB_ERR_ARG, "fb_diff_set_readout: index %d out of range", idx[k]);
+        FB_REQUIRE_D(rid[idx[k]] < 0, FB_ERR_ARG, "fb_diff_set_readout: neuron %d listed twice", idx[k]);
+        rid[idx[k]] = k;
+    }
+    FB_HIP(hipStreamSynchronize(f->stream));
+    FB_HIP(hipMemcpy((void *)f->d.readIdx, rid.data(), sizeof(int32_t) * f->N, hipMemcpyHostToDevice));
+    if (f->d.target) FB_HIP(hipFree((void *)f->d.target));
+    if (f->readPart) FB_HIP(hipFree(f->readPart));
+    void *q = nullptr;
+    FB_HIP(hipMalloc(&q, sizeof(float) * std::max(1, R)));
+    f->d.target = (const float *)q;
+    FB_HIP(hipMalloc(&f->readPart, sizeof(double) * std::max(1, R)));
+    if (R) FB_HIP(hipMemcpy(q, targetHz, sizeof(float) * R, hipMemcpyHostToDevice));
+    f->readout.assign(idx, idx + R);
+    f->target.assign(targetHz, targetHz + R);
+    f->tauR = tauMs;
+    f->d.R = R;
+    f->recorded = false;
+    return FB_OK;
+}
+
+static int grid_of(int64_t n, int block, int cap = 4096) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(cap, (n + block - 1) / block));
+}
+
+static fb_status launch_fwd_step(fb_diff *f, int32_t t, bool replay, int32_t r, int32_t resFromT) {
+    DiffDev &d = f->d;
+    hipStream_t st = f->stream;
+    if (replay) {
+        hipLaunchKernelGGL(kd_deliver<true>, dim3(256), dim3(256), 0, st, d, t, resFromT);
+        hipLaunchKernelGGL(kd_driven<true>, dim3(1), dim3(1024), 0, st, d, t);
+        hipLaunchKernelGGL(kd_membrane<true>, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, r);
+    } else {
+        hipLaunchKernelGGL(kd_deliver<false>, dim3(256), dim3(256), 0, st, d, t, 0);
+        hipLaunchKernelGGL(kd_driven<false>, dim3(1), dim3(1024), 0, st, d, t);
+        hipLaunchKernelGGL(kd_membrane<false>, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, 0);
+    }
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+static fb_status check_err(fb_diff *f, const char *who) {
+    int32_t err = 0;
+    FB_HIP(hipMemcpyAsync(&err, f->d.err, sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
+    FB_HIP(hipStreamSynchronize(f->stream));
+    FB_REQUIRE_D(!(err & DERR_COLL), FB_ERR_OVERFLOW, "%s: arrival chain pool overflowed", who);
+    FB_REQUIRE_D(!(err & DERR_TAPE), FB_ERR_OVERFLOW,
+                 "%s: the spike tape overflowed (raise the per-step capacity with fb_diff_set_tape_capacity)", who);
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_tape_capacity(fb_diff *f, int64_t perStep) {
+    FB_REQUIRE_D(f && perStep > 0, FB_ERR_ARG, "fb_diff_set_tape_capacity: bad arguments");
+    f->perStepCap = perStep;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int32_t record, double *loss) {
+    FB_REQUIRE_D(f && steps >= 0, FB_ERR_ARG, "fb_diff_forward: bad arguments");
+    FB_REQUIRE_D(f->d.nDriven <= 1 << 20, FB_ERR_UNSUPPORTED, "fb_diff_forward: driven set too large");
+    DiffDev &d = f->d;
+    const int32_t N = f->N, R = d.R;
+    hipStream_t st = f->stream;
+    FB_HIP(hipStreamSynchronize(st));
+    free_run(f);
+    fb_status s;
+    const int64_t cap = std::max<int64_t>(1024, f->perStepCap * (int64_t)std::max(1, steps));
+    if ((s = dalloc(f->runAllocs, &d.off, (size_t)steps + 1)) || (s = dalloc(f->runAllocs, &d.idx, (size_t)cap)) ||
+        (s = dalloc(f->runAllocs, &d.amp, (size_t)cap)) ||
+        (s = dalloc(f->runAllocs, &d.readS, (size_t)std::max(1, steps) * std::max(1, R))) ||
+        (s = dalloc(f->runAllocs, &d.rate, (size_t)std::max(1, steps) * std::max(1, R))) ||
+        (s = dalloc(f->runAllocs, &d.dLds, (size_t)std::max(1, steps) * std::max(1, R))))
+        return s;
+    d.cap = cap;
+    if (R) FB_HIP(hipMemsetAsync(d.readS, 0, sizeof(float) * (size_t)steps * R, st));
+    f->nck = record ? (steps + d.K - 1) / d.K + 1 : 0;
+    if (record && (s = dalloc(f->runAllocs, &f->ckpt, (size_t)f->nck * (6 * (size_t)N + 1)))) return s;
+    const uint32_t r0 = seed_hash(seed);
+    FB_HIP(hipMemcpyAsync(d.rng, &r0, sizeof r0, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(kd_reset, dim3(grid_of(N, 256)), dim3(256), 0, st, d, (float)f->p.vRest);
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    for (int32_t t = 0; t < steps; t++) {
+        if (record && t % d.K == 0)
+            hipLaunchKernelGGL(kd_save, dim3(grid_of(N, 256)), dim3(256), 0, st, d,
+                               f->ckpt + (size_t)(t / d.K) * (6 * (size_t)N + 1));
+        if ((s = launch_fwd_step(f, t, false, 0, 0))) return s;
+    }
+    hipLaunchKernelGGL(kd_finish, dim3(1), dim3(64), 0, st, d, steps);
+    hipLaunchKernelGGL(kd_loss_flat, dim3(1), dim3(1024), 0, st, d, f->scal);
+    if (R && steps) {
+        const double a = std::exp(-f->p.dt / f->tauR);
+        hipLaunchKernelGGL(kd_readout, dim3((R + 255) / 256), dim3(256), 0, st, d, steps, a, f->readPart);
+    }
+    FB_HIP(hipEventRecord(e1, st));
+    FB_HIP(hipGetLastError());
+    if ((s = check_err(f, "fb_diff_forward"))) return s;
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->fwdMs = ms;
+    double L = 0.0;
+    FB_HIP(hipMemcpy(&L, f->scal, sizeof(double), hipMemcpyDeviceToHost));
+    if (R && steps) {
+        std::vector<double> parts((size_t)R);
+        FB_HIP(hipMemcpy(parts.data(), f->readPart, sizeof(double) * R, hipMemcpyDeviceToHost));
+        double Lr = 0.0;
+        for (int32_t k = 0; k < R; k++) Lr += parts[k];
+        L += Lr / ((double)steps * (double)R);
+    }
+    f->steps = steps;
+    f->recorded = record != 0;
+    f->lastLoss = L;
+    if (loss) *loss = L;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *params, float *gLogGain, float *gEdge,
+                                      float *gInScale, int32_t *clamped) {
+    FB_REQUIRE_D(f && truncate >= 0, FB_ERR_ARG, "fb_diff_backward: bad arguments");
+    FB_REQUIRE_D(f->recorded, FB_ERR_STATE, "fb_diff_backward: no recorded forward pass (forward with record=1, "
+                                            "after the last change of inputs)");
+    DiffDev &d = f->d;
+    const int32_t N = f->N, T = f->steps, K = d.K, D = d.D;
+    hipStream_t st = f->stream;
+    fb_status s;
+    if (!d.aV) {
+        float *ar;
+        uint8_t *md;
+        int64_t resCap = 1;
+        std::vector<int64_t> off((size_t)T + 1);
+        FB_HIP(hipMemcpy(off.data(), d.off, sizeof(int64_t) * (T + 1), hipMemcpyDeviceToHost));
+        for (int32_t from = 0; from < T; from += K) {
+            const int32_t to = std::min(T, from + K);
+            if (to > D) resCap = std::max<int64_t>(resCap, off[to - D] - off[std::max(from, D) - D]);
+        }
+        if ((s = dalloc(f->runAllocs, &ar, 5 * (size_t)K * N + (size_t)resCap)) ||
+            (s = dalloc(f->runAllocs, &md, (size_t)K * N)))
+            return s;
+        d.aV = ar; d.aGE = ar + (size_t)K * N; d.aGI = ar + 2 * (size_t)K * N; d.aAd = ar + 3 * (size_t)K * N;
+        d.aSp = ar + 4 * (size_t)K * N; d.resAt = ar + 5 * (size_t)K * N; d.aMode = md;
+    }
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    const int gN = grid_of(N, 256);
+    hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
+    hipLaunchKernelGGL(kd_bzero, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d, f->nblk);
+    FB_HIP(hipMemsetAsync(f->clampDev, 0, 4 * sizeof(int32_t), st));
+    const int32_t nseg = (T + K - 1) / K;
+    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int32_t from = sgi * K, to = std::min(T, from + K);
+        const int32_t resFromT = std::max(from, D) - D;
+        hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, st, d,
+                           (const float *)(f->ckpt + (size_t)sgi * (6 * (size_t)N + 1)));
+        for (int32_t t = from; t < to; t++)
+            if ((s = launch_fwd_step(f, t, true, t - from, resFromT))) return s;
+        for (int32_t t = to - 1; t >= from; t--) {
+            if (truncate > 0 && (t + 1) % truncate == 0)
+                hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
+            hipLaunchKernelGGL(kd_bneuron, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
+            hipLaunchKernelGGL(kd_barrival, dim3(256), dim3(256), 0, st, d, t, resFromT);
+        }
+        FB_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kd_breduce, dim3(1), dim3(1024), 0, st, d, f->nblk, f->scal + 1);
+    hipLaunchKernelGGL(kd_sanitize, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d,
+                       f->clampDev);
+    FB_HIP(hipEventRecord(e1, st));
+    FB_HIP(hipGetLastError());
+    if ((s = check_err(f, "fb_diff_backward"))) return s;
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->bwdMs = ms;
+    double g[9];
+    int32_t cl[4];
+    FB_HIP(hipMemcpy(g, f->scal + 1, sizeof g, hipMemcpyDeviceToHost));
+    FB_HIP(hipMemcpy(cl, f->clampDev, sizeof cl, hipMemcpyDeviceToHost));
+    cl[0] = 0;
+    for (int q = 0; q < 9; q++)
+        if (!std::isfinite(g[q])) { g[q] = 0.0; cl[0]++; }
+    if (params) std::memcpy(params, g, sizeof g);
+    if (clamped) std::memcpy(clamped, cl, 3 * sizeof(int32_t));
+    if (gLogGain) FB_HIP(hipMemcpy(gLogGain, d.gLogGain, sizeof(float) * N, hipMemcpyDeviceToHost));
+    if (gInScale) FB_HIP(hipMemcpy(gInScale, d.gInScale, sizeof(float) * N, hipMemcpyDeviceToHost));
+    if (gEdge && f->E) FB_HIP(hipMemcpy(gEdge, d.gEdge, sizeof(float) * f->E, hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+// dL / d gain_T for type gains (outScale[i] = gain of i's type): the type sum
+// of gLogGain[i] / outScale[i], in f64 (src/brainmodel.js:63-100 semantics).
+extern "C" fb_status fb_diff_type_grads(fb_diff *f, const int32_t *typeOf, int32_t nTypes, double *gType) {
+    FB_REQUIRE_D(f && typeOf && gType && nTypes > 0, FB_ERR_ARG, "fb_diff_type_grads: bad arguments");
+    std::vector<float> lg((size_t)f->N);
+    FB_HIP(hipStreamSynchronize(f->stream));
+    FB_HIP(hipMemcpy(lg.data(), f->d.gLogGain, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    for (int32_t T = 0; T < nTypes; T++) gType[T] = 0.0;
+    for (int32_t i = 0; i < f->N; i++) {
+        FB_REQUIRE_D(typeOf[i] >= 0 && typeOf[i] < nTypes, FB_ERR_ARG, "fb_diff_type_grads: type of %d out of range", i);
+        gType[typeOf[i]] += (double)lg[i] / (double)f->in[FB_D_OUTSCALE][i];
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_get_state(fb_diff *f, int32_t which, float *dst) {
+    FB_REQUIRE_D(f && dst, FB_ERR_ARG, "fb_diff_get_state: null argument");
+    const DiffDev &d = f->d;
+    const float *src = nullptr;
+    switch (which) {
+        case FB_D_V: src = d.v; break;
+        case FB_D_GE: src = d.gE; break;
+        case FB_D_GI: src = d.gI; break;
+        case FB_D_ADAPT: src = d.adapt; break;
+        case FB_D_RES: src = d.res; break;
+        case FB_D_REFR: src = d.refr; break;
+        case FB_D_SPIKECOUNT: src = d.spikeCount; break;
+        case FB_D_INSCALE: src = d.inScale; break;
+        default: set_error("fb_diff_get_state: unknown array %d", which); return FB_ERR_ARG;
+    }
+    FB_HIP(hipStreamSynchronize(f->stream));
+    FB_HIP(hipMemcpy(dst, src, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_spikes(fb_diff *f, int64_t *off, int32_t *idx, float *amp, int64_t cap, int64_t *total) {
+    FB_REQUIRE_D(f && f->d.off, FB_ERR_STATE, "fb_diff_spikes: no forward pass");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    std::vector<int64_t> o((size_t)f->steps + 1);
+    FB_HIP(hipMemcpy(o.data(), f->d.off, sizeof(int64_t) * o.size(), hipMemcpyDeviceToHost));
+    if (off) std::memcpy(off, o.data(), sizeof(int64_t) * o.size());
+    const int64_t tot = o.back();
+    const int64_t m = std::min(tot, cap);
+    if (m > 0 && idx) FB_HIP(hipMemcpy(idx, f->d.idx, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    if (m > 0 && amp) FB_HIP(hipMemcpy(amp, f->d.amp, sizeof(float) * m, hipMemcpyDeviceToHost));
+    if (total) *total = tot;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_readout(fb_diff *f, float *s, double *rate) {
+    FB_REQUIRE_D(f && f->d.readS, FB_ERR_STATE, "fb_diff_readout: no forward pass");
+    const size_t m = (size_t)f->steps * f->d.R;
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (m && s) FB_HIP(hipMemcpy(s, f->d.readS, sizeof(float) * m, hipMemcpyDeviceToHost));
+    if (m && rate) FB_HIP(hipMemcpy(rate, f->d.rate, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_rng(fb_diff *f, uint32_t *rng) {
+    FB_REQUIRE_D(f && rng, FB_ERR_ARG, "fb_diff_rng: null argument");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    FB_HIP(hipMemcpy(rng, f->d.rng, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_stream(fb_diff *f, void *stream) {
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_set_stream: null handle");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    f->stream = stream ? (hipStream_t)stream : f->ownStream;
+    return FB_OK;
+}
+
+// forward_ms, backward_ms (HIP events around the last calls), edge_events and
+// arrivals (forward deliveries since creation), steps, tape_entries
+extern "C" fb_status fb_diff_stat(fb_diff *f, const char *name, double *value) {
+    FB_REQUIRE_D(f && name && value, FB_ERR_ARG, "fb_diff_stat: null argument");
+    if (!std::strcmp(name, "forward_ms")) { *value = f->fwdMs; return FB_OK; }
+    if (!std::strcmp(name, "backward_ms")) { *value = f->bwdMs; return FB_OK; }
+    if (!std::strcmp(name, "steps")) { *value = f->steps; return FB_OK; }
+    if (!std::strcmp(name, "edge_events") || !std::strcmp(name, "arrivals")) {
+        unsigned long long c[2];
+        FB_HIP(hipStreamSynchronize(f->stream));
+        FB_HIP(hipMemcpy(c, f->d.events, sizeof c, hipMemcpyDeviceToHost));
+        *value = (double)c[name[0] == 'e' ? 0 : 1];
+        return FB_OK;
+    }
+    if (!std::strcmp(name, "tape_entries")) {
+        int64_t tot = 0;
+        if (f->d.off) {
+            FB_HIP(hipStreamSynchronize(f->stream));
+            FB_HIP(hipMemcpy(&tot, f->d.off + f->steps, sizeof(int64_t), hipMemcpyDeviceToHost));
+        }
+        *value = (double)tot;
+        return FB_OK;
+    }
+    set_error("fb_diff_stat: unknown statistic '%s'", name);
+    return FB_ERR_ARG;
+}

@@ -75,6 +75,26 @@ _SIGS = {
     "fb_batch_rng": (I32, [P, P]),
     "fb_batch_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_batch_set_stream": (I32, [P, P]),
+    "fb_diff_create": (I32, [I32, I64, P, P, P, P, P, PP]),
+    "fb_diff_destroy": (I32, [P]),
+    "fb_diff_set_params": (I32, [P, P]),
+    "fb_diff_get_params": (I32, [P, P]),
+    "fb_diff_set_input": (I32, [P, I32, P]),
+    "fb_diff_set_edge_gain": (I32, [P, P]),
+    "fb_diff_set_masks": (I32, [P, P, P]),
+    "fb_diff_set_driven": (I32, [P, P, I32]),
+    "fb_diff_set_loss": (I32, [P, P, I32]),
+    "fb_diff_set_readout": (I32, [P, P, I32, P, D]),
+    "fb_diff_set_tape_capacity": (I32, [P, I64]),
+    "fb_diff_forward": (I32, [P, I32, D, I32, ctypes.POINTER(D)]),
+    "fb_diff_backward": (I32, [P, I32, P, P, P, P, P]),
+    "fb_diff_type_grads": (I32, [P, P, I32, P]),
+    "fb_diff_get_state": (I32, [P, I32, P]),
+    "fb_diff_spikes": (I32, [P, P, P, P, I64, PI64]),
+    "fb_diff_readout": (I32, [P, P, P]),
+    "fb_diff_rng": (I32, [P, P]),
+    "fb_diff_set_stream": (I32, [P, P]),
+    "fb_diff_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
     "fb_synchronize": (I32, [P]),
 }
 

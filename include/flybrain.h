@@ -175,6 +175,85 @@ fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_
                             int64_t *steps);
 fb_status fb_batch_set_stream(fb_batch *b, void *stream);
 
+/* ---------------------------------------------------------------- differentiable model
+ * The reference's LIFDiff (src/lifdiff.js:84-686) on the GPU: a COBA network
+ * with per-neuron log-gain, type gain (outScale), threshold offset, size
+ * scaling (inScale from sizeLog), masked threshold term (kcThreshold) and bias
+ * (laminaBias), depression and adaptation; a hard threshold with the
+ * fast-sigmoid surrogate 1/(beta (1 + |u - thr|/beta)^2) in the backward pass,
+ * or a logistic threshold ("soft") that makes the forward pass smooth.
+ * Arithmetic follows LIFDiff: doubles for every intermediate, binary32 for
+ * every stored array.  fb_diff_forward records a tape (the spike lists) and a
+ * checkpoint every `checkpoint` steps; fb_diff_backward replays one segment at
+ * a time and sweeps it in reverse (src/lifdiff.js:358-464, 494-686).
+ *
+ * Parameters are LIFDiff's options in this field order (lifdiff.js:88-112);
+ * `checkpoint` is the segment length K (the reference fixes 16). */
+typedef struct {
+    double dt, vRest, vThresh, vReset, tauM, tauSyn, tRef, delay;
+    double wSyn, adaptInc, adaptTau, depU, depTau;
+    double eExc, eInh, inhGain, kcThreshold, laminaBias;
+    double surrogateBeta, adjClip;
+    double sizeAlpha, maxSizeScale, boostCap;
+    int32_t soft, checkpoint;
+} fb_diff_params;
+
+/* per-neuron inputs (fb_diff_set_input) */
+enum { FB_D_OUTSCALE = 0, FB_D_LOGGAIN, FB_D_THROFFSET, FB_D_SIZELOG, FB_D_DRIVE, FB_D_NINPUTS };
+/* state arrays (fb_diff_get_state) */
+enum { FB_D_V = 0, FB_D_GE, FB_D_GI, FB_D_ADAPT, FB_D_RES, FB_D_REFR, FB_D_SPIKECOUNT, FB_D_INSCALE, FB_D_NSTATE };
+/* the nine scalar gradients, in DIFF_PARAMS order (src/lifdiff.js:75) */
+enum { FB_G_WSYN = 0, FB_G_SIZEALPHA, FB_G_INHGAIN, FB_G_EINH, FB_G_VTHRESH, FB_G_KC, FB_G_LAMINA,
+       FB_G_ADAPTINC, FB_G_DEPU, FB_G_NPARAMS };
+
+typedef struct fb_diff fb_diff;
+/* new LIFDiff(data, opts) (lifdiff.js:84): CSR rows in the caller's edge
+ * order (gradients per edge come back in that order), `counts` the delivered
+ * synapse count per edge with the minSyn / sensory gate folded in as zero
+ * (lifdiff.js:167-172), `sign` +1 / -1 / 0 per neuron (EXC_SIGN of its
+ * transmitter, or preSign). */
+fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t *indices, const float *counts,
+                         const float *sign, const fb_diff_params *p, fb_diff **out);
+fb_status fb_diff_destroy(fb_diff *d);
+fb_status fb_diff_set_params(fb_diff *d, const fb_diff_params *p);
+fb_status fb_diff_get_params(fb_diff *d, fb_diff_params *p);
+/* outScale, logGain, thrOffset, sizeLog or drive (Hz), N values */
+fb_status fb_diff_set_input(fb_diff *d, int32_t which, const float *values);
+/* per-edge multiplier on the delivered weight (opts.edgeGain, lifdiff.js:126-129); NULL = all 1 */
+fb_status fb_diff_set_edge_gain(fb_diff *d, const float *edgeGain);
+fb_status fb_diff_set_masks(fb_diff *d, const uint8_t *thrMask, const uint8_t *biasMask);
+/* the driven set in insertion order (setDrive / setDriveOne, lifdiff.js:196-207) */
+fb_status fb_diff_set_driven(fb_diff *d, const int32_t *idx, int32_t n);
+/* dLdSpike[N] and lossFrom of backward() (lifdiff.js:494); NULL = zero */
+fb_status fb_diff_set_loss(fb_diff *d, const float *dLdSpike, int32_t lossFrom);
+/* A filtered-rate loss on R readouts, added to the flat term: r_t = a r_{t-1}
+ * + (1-a) s_t / (dt/1000), a = exp(-dt/tau), L = mean over steps and readouts
+ * of (r_t - target)^2.  No counterpart in the reference. */
+fb_status fb_diff_set_readout(fb_diff *d, const int32_t *idx, int32_t R, const float *targetHz, double tauMs);
+/* tape entries allowed per step on average (default max(4096, N/4)) */
+fb_status fb_diff_set_tape_capacity(fb_diff *d, int64_t perStep);
+/* forward(steps, {seed, record}) (lifdiff.js:219): from reset; loss = the
+ * flat term sum dLdSpike * spike count from lossFrom on, plus the readout term */
+fb_status fb_diff_forward(fb_diff *d, int32_t steps, double seed, int32_t record, double *loss);
+/* backward(tape, dLdSpike, {truncate, lossFrom}) (lifdiff.js:494): params[9]
+ * in FB_G_* order, gLogGain[N], gEdge[E] = dL/d edgeGain (no counterpart in
+ * the reference), gInScale[N]; clamped[3] = non-finite values zeroed among
+ * params, logGain and edges.  Any output may be NULL. */
+fb_status fb_diff_backward(fb_diff *d, int32_t truncate, double *params, float *gLogGain, float *gEdge,
+                           float *gInScale, int32_t *clamped);
+/* dL/d gain_T for outScale[i] = gain of i's type: sum over the type of
+ * gLogGain[i] / outScale[i] (the type gains of src/brainmodel.js:63-100) */
+fb_status fb_diff_type_grads(fb_diff *d, const int32_t *typeOf, int32_t nTypes, double *gType);
+fb_status fb_diff_get_state(fb_diff *d, int32_t which, float *dst);
+/* the tape: off[steps+1] and entries (forced spikes have bit 31 of idx set) */
+fb_status fb_diff_spikes(fb_diff *d, int64_t *off, int32_t *idx, float *amp, int64_t cap, int64_t *total);
+/* readout spike amplitudes s[steps][R] and rates[steps][R] of the last forward */
+fb_status fb_diff_readout(fb_diff *d, float *s, double *rate);
+fb_status fb_diff_rng(fb_diff *d, uint32_t *rng);
+fb_status fb_diff_set_stream(fb_diff *d, void *stream);
+/* "forward_ms", "backward_ms", "edge_events", "arrivals", "steps", "tape_entries" */
+fb_status fb_diff_stat(fb_diff *d, const char *name, double *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,230 @@
+"""GPU parity of the differentiable model (fb_diff_*) against the CPU oracle
+restatement of LIFDiff, and of its gradients against central finite
+differences of the GPU's own soft forward pass.
+
+Hard threshold: the forward pass is bit-identical to the oracle (tape, state,
+stream, loss).  The backward pass sums the same double terms in another order
+(fan-out reductions across lanes, scalar sums across workgroups), so gradients
+agree to ~1e-6 relative.  Soft threshold: the logistic's exp runs on the GPU,
+so the forward agrees to rounding."""
+import importlib
+
+import numpy as np
+import pytest
+
+import diffcases as dc
+
+pytestmark = pytest.mark.gpu
+
+diff = importlib.import_module("fly-brain_amd.diff")
+synth = importlib.import_module("fly-brain_amd.synth")
+P = importlib.import_module("fly-brain_amd.params")
+
+STEPS = 60
+STATE = ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount")
+
+
+def gpu_of(pr, params=None, inputs=None, edge_gain=None):
+    p = {**pr.p, **(params or {})}
+    net = diff.LIFDiffHip(pr.g.N, pr.g.indptr, pr.g.indices, pr.counts, pr.sign, p)
+    return pr.apply(net, params, inputs, edge_gain)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(1e-30, np.abs(b).max()))
+
+
+def _pair(pr, steps=STEPS, seed=1):
+    o = pr.oracle()
+    g = gpu_of(pr)
+    Lo = o.forward(steps, seed=seed)
+    Lg = g.forward(steps, seed=seed)
+    return o, g, Lo, Lg
+
+
+@pytest.mark.parametrize("over", [{"soft": False, "tRef": 2.2}, {"soft": False, "tRef": 2.2, "checkpoint": 7},
+                                  {"soft": False, "tRef": 0.5, "delay": 0.5}])
+def test_hard_forward_is_bit_identical_to_oracle(over):
+    pr = dc.Problem(over=over).with_readout(R=30)
+    o, g, Lo, Lg = _pair(pr, steps=150)
+    off_o, idx_o, amp_o, f_o = o.spikes()
+    off_g, idx_g, amp_g, f_g = g.spikes()
+    assert np.array_equal(off_o, off_g)
+    assert np.array_equal(idx_o, idx_g) and np.array_equal(f_o, f_g)
+    assert np.array_equal(amp_o, amp_g)
+    for k in STATE:
+        assert np.array_equal(o.state(k).view(np.uint32), g.state(k).view(np.uint32)), k
+    assert o.rng == g.rng
+    # the loss sums the same terms in another order
+    assert Lg == pytest.approx(Lo, rel=1e-12)
+    so, ro = o.readout()
+    sg, rg = g.readout()
+    assert np.array_equal(so, sg) and np.array_equal(ro, rg)
+
+
+@pytest.mark.parametrize("over", [{"soft": False, "tRef": 2.2}, {"soft": False, "tRef": 2.2, "adjClip": 0.05}])
+def test_hard_gradients_match_oracle(over):
+    pr = dc.Problem(over=over).with_readout(R=30)
+    o, g, Lo, Lg = _pair(pr, steps=150)
+    go, gg = o.backward(), g.backward()
+    for k in P.DIFF_PARAMS:
+        a, b = gg["params"][k], go["params"][k]
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b)) + 1e-6 * abs(b), (k, a, b)
+    assert rel_err(gg["logGain"], go["logGain"]) < 1e-5
+    assert rel_err(gg["edge"], go["edge"]) < 1e-5
+    assert rel_err(gg["inScale"], go["inScale"]) < 1e-5
+    assert np.array_equal(gg["clamped"], go["clamped"])
+
+
+def test_soft_forward_and_gradients_match_oracle():
+    pr = dc.Problem()
+    o, g, Lo, Lg = _pair(pr)
+    assert Lg == pytest.approx(Lo, rel=1e-6)
+    for k in STATE:
+        assert np.allclose(g.state(k), o.state(k), rtol=1e-5, atol=1e-5), k
+    go, gg = o.backward(), g.backward()
+    for k in P.DIFF_PARAMS:
+        assert gg["params"][k] == pytest.approx(go["params"][k], rel=1e-4, abs=1e-8), k
+    assert rel_err(gg["logGain"], go["logGain"]) < 1e-4
+    assert rel_err(gg["edge"], go["edge"]) < 1e-4
+
+
+def test_gpu_gradients_match_finite_differences_of_gpu_forward():
+    """grad_check.mjs's comparison on the GPU alone: nine scalars and
+    per-neuron log-gains of the soft model."""
+    pr = dc.Problem(N=600, seed=21)
+    g = gpu_of(pr)
+    L = g.forward(STEPS, seed=1)
+    gr = g.backward()
+
+    def loss(**kw):
+        return gpu_of(pr, **kw).forward(STEPS, seed=1, record=False)
+
+    for k in P.DIFF_PARAMS:
+        h = dc.H[k]
+        fd = (loss(params={k: pr.p[k] + h}) - loss(params={k: pr.p[k] - h})) / (2 * h)
+        ok, rel = dc.fd_ok(gr["params"][k], fd, h, L)
+        assert ok, (k, gr["params"][k], fd, rel)
+    h = 2e-2
+    for i in (0, 5, 200, 400):
+        up = pr.inputs["logGain"].copy()
+        dn = up.copy()
+        up[i] += h
+        dn[i] -= h
+        fd = (loss(inputs={"logGain": up}) - loss(inputs={"logGain": dn})) / (2 * h)
+        ok, rel = dc.fd_ok(gr["logGain"][i], fd, h, L)
+        assert ok, (i, gr["logGain"][i], fd, rel)
+
+
+def test_checkpoint_length_does_not_change_gradients():
+    out = []
+    for K in (16, 5, 150):
+        pr = dc.Problem(over={"soft": False, "tRef": 2.2, "checkpoint": K}).with_readout(R=20)
+        g = gpu_of(pr)
+        g.forward(150, seed=3)
+        out.append(g.backward())
+    for o in out[1:]:
+        for k in P.DIFF_PARAMS:
+            if k != "sizeAlpha":      # reads gInScale, summed with float atomics
+                assert o["params"][k] == out[0]["params"][k], k
+        assert np.array_equal(o["logGain"], out[0]["logGain"])
+        assert np.array_equal(o["edge"], out[0]["edge"])
+
+
+def test_type_gain_gradients():
+    pr = dc.Problem(over={"soft": False, "tRef": 2.2})
+    types = np.random.default_rng(5).integers(0, 12, size=pr.g.N).astype(np.int32)
+    g = gpu_of(pr)
+    g.forward(100, seed=2)
+    gr = g.backward()
+    tg = g.type_grads(types, 12)
+    want = np.zeros(12)
+    np.add.at(want, types, gr["logGain"].astype(np.float64) / pr.inputs["outScale"].astype(np.float64))
+    assert np.allclose(tg, want, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- the full-scale workload
+
+def c4_problem(soft=False, steps=2000):
+    """165,122 neurons, ~8M edges, COBA (eInh -80), dt 0.1, 1% driven at
+    U(100, 200) Hz, 500 neuron types with lognormal(0, 0.2) gains on outgoing
+    weights, 1,000 readouts with U(0, 50) Hz targets and a 20 ms rate filter,
+    checkpoints every 100 steps, fast-sigmoid beta 0.1 mV."""
+    g = synth.config2_graph()
+    p = P.diff_merged({"dt": 0.1, "tRef": 2.2, "delay": 1.8, "wSyn": 0.275, "eInh": -80.0,
+                       "surrogateBeta": 0.1, "checkpoint": 100, "soft": soft})
+    drv = synth.driven_set(g.N, 0.01, 4)
+    rates = synth.drive_rates(drv.size, 100.0, 200.0, 5)
+    types = synth.type_ids(g.N, 500, 6)
+    gains = synth.type_gains(0)
+    rng = np.random.default_rng(7)
+    ro = np.sort(rng.choice(g.N, size=1000, replace=False)).astype(np.int32)
+    tgt = np.random.default_rng(8).uniform(0, 50, size=1000).astype(np.float32)
+    sign = np.where(g.exc, 1.0, -1.0).astype(np.float32)
+    return g, p, drv, rates, types, gains, ro, tgt, sign
+
+
+def c4_net(g, p, drv, rates, types, gains, ro, tgt, sign, cls, out_scale=None):
+    net = cls(g.N, g.indptr, g.indices, g.counts.astype(np.float32), sign, p)
+    drive = np.zeros(g.N, np.float32)
+    drive[drv] = rates
+    net.set_input("drive", drive)
+    net.set_driven(drv)
+    net.set_input("outScale", gains[types] if out_scale is None else out_scale)
+    net.set_loss(np.zeros(g.N, np.float32), 0)
+    net.set_readout(ro, tgt, 20.0)
+    return net
+
+
+def test_full_scale_forward_adjoint_matches_oracle():
+    """The full-scale workload, 600 steps (the oracle's backward at this size
+    takes seconds): hard-threshold tape bit-identical, gradients agree."""
+    from oracle import OracleDiff
+    w = c4_problem(steps=600)
+    gpu = c4_net(*w, diff.LIFDiffHip)
+    cpu = c4_net(*w, lambda N, ip, ix, c, s, p: OracleDiff(N, ip, ix, c, s, p))
+    Lg = gpu.forward(600, seed=1)
+    Lc = cpu.forward(600, seed=1)
+    assert Lg == pytest.approx(Lc, rel=1e-12)
+    og, oc = gpu.spikes(), cpu.spikes()
+    for a, b in zip(og, oc):
+        assert np.array_equal(a, b)
+    gg, gc = gpu.backward(), cpu.backward()
+    assert rel_err(gg["logGain"], gc["logGain"]) < 1e-5
+    assert rel_err(gg["edge"], gc["edge"]) < 1e-5
+    for k in ("wSyn", "vThresh", "inhGain", "eInh"):
+        assert gg["params"][k] == pytest.approx(gc["params"][k], rel=1e-6), k
+    types = w[4]
+    tg = gpu.type_grads(types, 500)
+    assert np.count_nonzero(tg) > 400
+
+
+def test_full_scale_type_gain_gradients_match_finite_differences():
+    """Soft model on the full-scale workload: 32 random type gains (seed 9),
+    central differences with eps 1e-3 on log gain, relative error <= 1e-3
+    (with grad_check's f32 noise floor)."""
+    w = c4_problem(soft=True)
+    g, p, drv, rates, types, gains, ro, tgt, sign = w
+    net = c4_net(*w, diff.LIFDiffHip)
+    L = net.forward(2000, seed=1)
+    gr = net.backward(edges=False)
+    tg = net.type_grads(types, 500) * gains.astype(np.float64)      # d L / d log gain_T
+    pick = np.random.default_rng(9).choice(500, size=32, replace=False)
+    h = 1e-3
+    fails = []
+    for T in pick:
+        up = gains.astype(np.float64).copy()
+        dn = up.copy()
+        up[T] *= np.exp(h)
+        dn[T] *= np.exp(-h)
+        net.set_input("outScale", up.astype(np.float32)[types])
+        Lp = net.forward(2000, seed=1, record=False)
+        net.set_input("outScale", dn.astype(np.float32)[types])
+        Lm = net.forward(2000, seed=1, record=False)
+        fd = (Lp - Lm) / (2 * h)
+        ok, rel = dc.fd_ok(tg[T], fd, h, L)
+        if not ok:
+            fails.append((int(T), float(tg[T]), fd, rel))
+    assert not fails, fails
