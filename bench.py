@@ -167,6 +167,84 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     return out
 
 
+def adjoint_workload(synth, P):
+    """The full-scale graph, COBA (eInh -80), dt 0.1 ms, 1% driven at
+    U(100, 200) Hz, 500 neuron types with lognormal(0, 0.2) gains on outgoing
+    weights, 1,000 readouts (targets U(0, 50) Hz, 20 ms rate filter, mean
+    squared rate error), hard threshold with the fast-sigmoid surrogate
+    (beta 0.1 mV), checkpoints every 100 steps.  The adjoint state is bounded
+    at +-10 per step (LIFDiff's adjClip): with a 0.1 mV surrogate the
+    unbounded adjoint grows past 1e35 within 2,000 steps."""
+    import numpy as np
+    g = synth.config2_graph()
+    p = P.diff_merged({"dt": 0.1, "tRef": 2.2, "delay": 1.8, "wSyn": 0.275, "eInh": -80.0,
+                       "surrogateBeta": 0.1, "checkpoint": 100, "soft": False, "adjClip": 10.0})
+    drv = synth.driven_set(g.N, 0.01, 4)
+    drive = np.zeros(g.N, np.float32)
+    drive[drv] = synth.drive_rates(drv.size, 100.0, 200.0, 5)
+    types = synth.type_ids(g.N, 500, 6)
+    gains = synth.type_gains(0)
+    ro = np.sort(np.random.default_rng(7).choice(g.N, size=1000, replace=False)).astype(np.int32)
+    tgt = np.random.default_rng(8).uniform(0, 50, size=1000).astype(np.float32)
+    sign = np.where(g.exc, 1.0, -1.0).astype(np.float32)
+
+    def setup(net):
+        net.set_input("drive", drive)
+        net.set_driven(drv)
+        net.set_input("outScale", gains[types])
+        net.set_loss(np.zeros(g.N, np.float32), 0)
+        net.set_readout(ro, tgt, 20.0)
+        return net
+    return g, p, sign, types, setup
+
+
+def bench_adjoint(args, synth, P, diff, hiprt, stream):
+    import numpy as np
+    g, p, sign, types, setup = adjoint_workload(synth, P)
+    T = args.adjoint_steps
+    net = setup(diff.LIFDiffHip(g.N, g.indptr, g.indices, g.counts.astype(np.float32), sign, p))
+    net.set_stream(stream.handle)
+    net.forward(200, seed=1)              # warm-up (allocations, first launches)
+    net.backward(edges=True)
+    ev0 = net.stat("edge_events")
+    hiprt.device_synchronize()
+    w0 = time.perf_counter()
+    L = net.forward(T, seed=1)
+    gr = net.backward(edges=True)
+    wall = time.perf_counter() - w0
+    ev = net.stat("edge_events") - ev0
+    fwd_ms, bwd_ms = net.stat("forward_ms"), net.stat("backward_ms")
+    tg = net.type_grads(types, 500)
+    secs = (fwd_ms + bwd_ms) / 1000.0
+    out = {"value": ev / secs, "unit": "synaptic events/s (forward events over forward + adjoint time)",
+           "steps": T, "forward_ms": fwd_ms, "backward_ms": bwd_ms, "ms_per_step": (fwd_ms + bwd_ms) / T,
+           "wall_s_host_clock": wall, "loss": L, "edge_events": ev,
+           "tape_spikes": net.stat("tape_entries"),
+           "grad_norms": {"type_gains": float(np.linalg.norm(tg)),
+                          "edges": float(np.linalg.norm(gr["edge"].astype(np.float64)))},
+           "non_finite_zeroed": gr["clamped"].tolist(),
+           "workload": "165,122 neurons, ~8.0M edges, COBA, 2,000 steps, 1,000 readouts (filtered-rate MSE), "
+                       "gradients w.r.t. 500 type gains and every edge weight, checkpoints every 100 steps"}
+    net.destroy()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        n = min(T, args.adjoint_cpu_steps)
+        cpu = setup(orc.OracleDiff(g.N, g.indptr, g.indices, g.counts.astype(np.float32), sign, p))
+        c0 = time.perf_counter()
+        cpu.forward(n, seed=1)
+        cpu.backward()
+        cdt = time.perf_counter() - c0
+        off, idx, amp, forced = cpu.spikes()
+        D = P.nslots(p) - 1
+        cev = int(g.outdeg[idx[: off[max(0, n - D)]]].sum())
+        out["cpu_baseline"] = {"value": cev / cdt, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+                               "sample": f"forward + adjoint over the first {n} steps of the same workload, "
+                                         "the C restatement of LIFDiff (gcc -O2), 1 thread",
+                               "ms_per_step": cdt * 1000.0 / n}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,13 +255,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
               "(multi-GPU sharding is design only).", file=sys.stderr)
         sys.exit(2)
 
-    synth, P, lif, hiprt, batch = pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch")
+    synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
+                                         pkg("diff"))
     t_build = time.perf_counter()
     g = synth.config2_graph()
     drv = synth.driven_set(g.N, 0.01, 4)
@@ -293,6 +375,8 @@ def main():
     out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
+    if not args.no_adjoint:
+        out["fwd_adjoint"] = bench_adjoint(args, synth, P, diff, hiprt, stream)
     if not args.no_cpu:
         cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)
         out["cpu_baseline"] = cb

@@ -41,6 +41,10 @@ constexpr int BMAX = 64;
 
 struct BatchDev {
     int32_t N, B, nslots, nDriven, nTypes;
+    // adapt stays exactly 0 without adaptInc, res exactly 1 without depU, and
+    // there are no background hits without bgRate: those arrays are then
+    // neither read nor written
+    int32_t liveAdapt, liveRes, liveBg;
     Consts k;
     DevGraph g;
     // [N][B]
@@ -88,6 +92,7 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
     const int32_t A = d.nActive[head];
     const int32_t *list = d.active + (int64_t)head * d.N;
     const int32_t N = d.N, B = d.B;
+    unsigned long long waveEdges = 0, waveSyn = 0;
     for (int32_t k = wave; k < A; k += nwaves) {
         const int32_t pre = list[k];
         uint64_t bits = d.mask[(int64_t)head * N + pre] & on;
@@ -96,21 +101,25 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
         const float sgn = d.g.sign[pre];
         const int32_t typ = d.typeOf[pre];
         const int32_t dp = d.drvPos[pre];
+        // lane b holds member b's gain for this row's type and its res
+        const float gLane = lane < B ? d.gain[(int64_t)lane * d.nTypes + typ] : 0.f;
+        const float rLane = (d.liveRes && lane < B) ? d.res[(int64_t)pre * B + lane] : 1.f;
         while (bits) {
             const int b = __ffsll((unsigned long long)bits) - 1;
             bits &= bits - 1;
             const int64_t ib = (int64_t)pre * B + b;
-            const float s = sgn * d.res[ib];
+            const float resb = __shfl(rLane, b, 64);
+            const float s = d.liveRes ? sgn * resb : sgn;
             if (s == 0.f) continue;
             // arrival rank in member b's lif.c list: forced spikes (driven
             // order) before threshold spikes (ascending index)
             const uint32_t rank = ((forced >> b) & 1ull) ? (uint32_t)dp : (uint32_t)N + (uint32_t)pre;
             if (lane == 0) {
-                d.res[ib] = d.res[ib] - d.k.depU * d.res[ib];
-                atomicAdd(&d.counters[0], (unsigned long long)(re - ra));
-                atomicAdd(&d.counters[1], (unsigned long long)d.g.rowSyn[pre]);
+                if (d.liveRes) d.res[ib] = resb - d.k.depU * resb;
+                waveEdges += (unsigned long long)(re - ra);
+                waveSyn += (unsigned long long)d.g.rowSyn[pre];
             }
-            const float g = d.gain[(int64_t)b * d.nTypes + typ];
+            const float g = __shfl(gLane, b, 64);
             const int32_t cls = s > 0 ? 0 : N;
             for (uint32_t j = ra + lane; j < re; j += 64) {
                 const uint32_t q = d.g.edges[j].x;
@@ -134,6 +143,10 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
                 }
             }
         }
+    }
+    if (lane == 0 && waveEdges) {
+        atomicAdd(&d.counters[0], waveEdges);
+        atomicAdd(&d.counters[1], waveSyn);
     }
 }
 
@@ -171,14 +184,24 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
         const uint32_t y = matvec(leapS + (32 + (nbg >> 5)) * LEAP_STRIDE, x0);
         x0 = (nbg & 31) ? matvec(leapS + ((nbg & 31) - 1) * LEAP_STRIDE, y) : y;
     }
-    // forced spikes: wave w owns driven positions [lo, hi) for every member
+    // forced spikes: wave w owns driven positions [lo, hi) for every member.
+    // The slice is walked CH neurons at a time with every load of a chunk in
+    // flight together; the draws themselves then run from registers.
+    constexpr int CH = 32;
     const int32_t nD = d.nDriven;
     const int32_t per = (nD + W - 1) / W;
     const int32_t lo = min(nD, wave * per), hi = min(nD, lo + per);
+    const int32_t B = d.B;
     int32_t mine = 0;
     if (on)
-        for (int32_t q = lo; q < hi; q++)
-            if (!(d.refr[(int64_t)d.driven[q] * d.B + lane] > 0)) mine++;
+        for (int32_t c0 = lo; c0 < hi; c0 += CH) {
+            float r[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++)
+                r[u] = c0 + u < hi ? d.refr[(int64_t)d.driven[c0 + u] * B + lane] : 1.f;
+#pragma unroll
+            for (int u = 0; u < CH; u++) mine += !(r[u] > 0) ? 1 : 0;
+        }
     cnt[wave][lane] = mine;
     __syncthreads();
     int32_t off = 0, total = 0;
@@ -193,25 +216,30 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
         const uint32_t y = matvec(leapS + (32 + (off >> 5)) * LEAP_STRIDE, x0);
         x = (off & 31) ? matvec(leapS + ((off & 31) - 1) * LEAP_STRIDE, y) : y;
     }
-    const float forcedRefr = d.k.tRef + d.k.dt;
-    for (int32_t q = lo; q < hi; q++) {
-        const int32_t i = d.driven[q];
-        const int64_t ib = (int64_t)i * d.B + lane;
-        bool f = false;
-        if (on && !(d.refr[ib] > 0)) {
-            x = xs32(x);
-            const float u = (float)(x >> 8) * (1.0f / 16777216.0f);
-            if (u < d.drive[i] * d.k.dtS) {
-                d.v[ib] = d.k.vReset;
-                d.refr[ib] = forcedRefr;
-                d.spikeCount[ib] += 1u;
-                d.trace[ib] = 1.f;
-                d.adapt[ib] = d.adapt[ib] + d.k.adaptInc;
-                f = true;
-            }
+    // a forced spike's effects on the neuron (lif.c:51-52) are applied by
+    // kb_membrane from the mask, so this kernel reads state and writes masks only
+    for (int32_t c0 = lo; c0 < hi; c0 += CH) {
+        float r[CH], pf[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const bool in = on && c0 + u < hi;
+            const int32_t i = c0 + u < hi ? d.driven[c0 + u] : 0;
+            r[u] = in ? d.refr[(int64_t)i * B + lane] : 1.f;
+            pf[u] = in ? d.drive[i] * d.k.dtS : 0.f;
         }
-        const uint64_t fb = __ballot(f);
-        if (lane == 0) d.fmask[(int64_t)slot * d.N + i] = fb;
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            if (c0 + u >= hi) break;
+            const int32_t i = d.driven[c0 + u];
+            bool f = false;
+            if (!(r[u] > 0)) {
+                x = xs32(x);
+                const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
+                f = uu < pf[u];
+            }
+            const uint64_t fb = __ballot(f);
+            if (lane == 0) d.fmask[(int64_t)slot * d.N + i] = fb;
+        }
     }
     if (wave == 0 && on) {
         // the stream after all of this step's draws
@@ -222,18 +250,22 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
 
 // ------------------------------------------------------------------ membrane + threshold
 
-__global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int32_t t) {
+__global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int32_t head, int32_t t) {
     const int lane = threadIdx.x & 63;
     const int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (i >= d.N) return;   // whole wave
     const int32_t B = d.B, N = d.N;
     const bool on = lane < B && t < d.steps[lane];
     bool spk = false;
+    const bool drv = d.drvPos[i] >= 0;
+    const uint64_t fm = drv ? d.fmask[(int64_t)slot * N + i] : 0ull;
     if (on) {
         const int64_t ib = (int64_t)i * B + lane;
         NeuronState s;
         s.v = d.v[ib]; s.gE = d.gE[ib]; s.gI = d.gI[ib]; s.refr = d.refr[ib];
-        s.trace = d.trace[ib]; s.adapt = d.adapt[ib]; s.res = d.res[ib];
+        s.trace = d.trace[ib];
+        s.adapt = d.liveAdapt ? d.adapt[ib] : 0.f;
+        s.res = d.liveRes ? d.res[ib] : 1.f;
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             const int64_t ci = ((int64_t)c * N + i) * B + lane;
@@ -249,10 +281,18 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
                 if (c) s.gI = g; else s.gE = g;
             }
         }
-        const int32_t nb = d.bgCnt[ib];
+        const int32_t nb = d.liveBg ? d.bgCnt[ib] : 0;
         if (nb) {
             for (int32_t q = 0; q < nb; q++) s.gE = s.gE + d.k.bgAmp;
             d.bgCnt[ib] = 0;
+        }
+        if ((fm >> lane) & 1ull) {
+            // this step's forced spike (lif.c:51-52), after every delivery
+            s.v = d.k.vReset;
+            s.refr = d.k.tRef + d.k.dt;
+            s.trace = 1.f;
+            s.adapt = s.adapt + d.k.adaptInc;
+            d.spikeCount[ib] += 1u;
         }
         membrane(d.k, s, d.bias[ib]);
         if (crosses(d.k, s, d.thr[ib])) {
@@ -261,12 +301,18 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
             spk = true;
         }
         d.v[ib] = s.v; d.gE[ib] = s.gE; d.gI[ib] = s.gI; d.refr[ib] = s.refr;
-        d.trace[ib] = s.trace; d.adapt[ib] = s.adapt; d.res[ib] = s.res;
+        d.trace[ib] = s.trace;
+        if (d.liveAdapt) d.adapt[ib] = s.adapt;
+        if (d.liveRes) d.res[ib] = s.res;
+    }
+    // this step's arrivals are consumed: their slot becomes the next step's
+    // output slot, and the chain pool is free (its entries were read above)
+    if (i == 0 && lane == 0) {
+        d.nActive[head] = 0;
+        d.ncoll[0] = 0;
     }
     const uint64_t th = __ballot(spk);
     if (lane == 0) {
-        const bool drv = d.drvPos[i] >= 0;
-        const uint64_t fm = drv ? d.fmask[(int64_t)slot * N + i] : 0ull;
         if (!drv) d.fmask[(int64_t)slot * N + i] = 0ull;
         const uint64_t m = th | fm;
         d.mask[(int64_t)slot * N + i] = m;
@@ -276,20 +322,6 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
             atomicAdd(&d.counters[2], (unsigned long long)__popcll(m));
         }
     }
-}
-
-// after a step: the slot delivered this step is empty, the pool is free
-__global__ void kb_close(BatchDev d, int32_t head, int32_t nextOut) {
-    if (threadIdx.x == 0) {
-        d.nActive[head] = 0;
-        d.ncoll[0] = 0;
-    }
-    (void)nextOut;
-}
-
-// before a step: the out slot's active count starts at zero
-__global__ void kb_open(BatchDev d, int32_t slot) {
-    if (threadIdx.x == 0) d.nActive[slot] = 0;
 }
 
 // records the active list, spike masks and forced masks of one slot
@@ -390,6 +422,9 @@ extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B,
     BatchDev &d = b->d;
     d.N = N; d.B = B; d.nslots = nslots_of(*p); d.nDriven = 0; d.nTypes = nTypes;
     derive_consts(*p, N, d.k);
+    d.liveAdapt = d.k.adaptInc != 0.f;
+    d.liveRes = d.k.depU != 0.f;
+    d.liveBg = d.k.bgEvents > 0.f;
     d.g = *dg;
     FB_HIP(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking));
     b->stream = b->ownStream;
@@ -511,11 +546,9 @@ static fb_status batch_step(fb_batch *b) {
     BatchDev &d = b->d;
     const int32_t head = b->head, slot = (head + d.nslots - 1) % d.nslots;
     hipStream_t st = b->stream;
-    hipLaunchKernelGGL(kb_open, dim3(1), dim3(64), 0, st, d, slot);
     hipLaunchKernelGGL(kb_deliver, dim3(256), dim3(256), 0, st, d, head, b->t);
     hipLaunchKernelGGL(kb_driven, dim3(1), dim3(1024), 0, st, d, slot, b->t);
-    hipLaunchKernelGGL(kb_membrane, dim3((d.N + 3) / 4), dim3(256), 0, st, d, slot, b->t);
-    hipLaunchKernelGGL(kb_close, dim3(1), dim3(64), 0, st, d, head, slot);
+    hipLaunchKernelGGL(kb_membrane, dim3((d.N + 3) / 4), dim3(256), 0, st, d, slot, head, b->t);
     FB_HIP(hipGetLastError());
     b->lastSlot = slot;
     b->head = (head + 1) % d.nslots;
