@@ -256,6 +256,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-engines", action="store_true", help="skip the per-step engine comparison")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference kernel for "
+                         "scaling HBM counters), then exit")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
     args = ap.parse_args()
@@ -264,6 +268,16 @@ def main():
               "(multi-GPU sharding is design only).", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
     t_build = time.perf_counter()
@@ -301,6 +315,7 @@ def main():
               for blk in ("drv", "reg") for ph in ("e1", "w1", "e2", "w2", "e3", "w3")}
     phases.update({f"drvstep_{ph}": brain.stat(f"phase_us_drvstep_{ph}")
                    for ph in ("inputs", "scan", "draws", "decide", "membrane", "sync")})
+    overflow_flags = int(brain.stat("persistent_overflow_flags"))
     launches = int(brain.stat("persistent_launches") - l0)
     fallback = int(brain.stat("fallback_steps") - f0)
     events = c1["edge_events"] - c0["edge_events"]
@@ -357,7 +372,7 @@ def main():
     }
     # the per-step engines on the same workload, for comparison
     cmp = {}
-    for eng in ("graph", "launch"):
+    for eng in () if args.no_engines else ("graph", "launch"):
         b2 = lif.LIFHip(hg, p, seed=1, engine=eng)
         for i, r in zip(drv.tolist(), rates.tolist()):
             b2.setDriveOne(i, r)
@@ -372,6 +387,8 @@ def main():
         cmp[eng] = {"ms_per_step": a0.elapsed_ms(a1) / n2, "steps": n2}
         b2.destroy()
     out["lookahead_phases_us_total"] = phases
+    out["lookahead_fallback_steps"] = fallback
+    out["lookahead_overflow_flags"] = overflow_flags
     out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)

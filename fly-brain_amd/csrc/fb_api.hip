@@ -33,6 +33,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
                       hipStream_t st);
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
+uint32_t persistent_overflow_flags(const DevBrain &d);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
                             int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion);
 void persistent_release(const int32_t *key);
@@ -739,6 +740,7 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
     if (n == "fallback_steps") *value = (double)b->fallbackSteps;
     else if (n == "persistent_launches") *value = (double)b->persistentLaunches;
     else if (n == "persistent_kernel_ms") *value = b->persistentMs;
+    else if (n == "persistent_overflow_flags") *value = (double)persistent_overflow_flags(b->d);
     else if (n.rfind("phase_us_", 0) == 0) {
         // phase_us_<drv|reg>_<e1|w1|e2|w2|e3|w3>: time in each phase of the
         // lookahead kernel (e = work, w = waiting at the grid barrier after it)

@@ -70,7 +70,7 @@ constexpr int DRV_CAP = PT * DPT;
 constexpr int CELLS = DPT > NPT ? DPT : NPT;
 constexpr int S_MAX = 512;          // owning blocks (driven + regular)
 constexpr int CAP_DRV = 1024;       // sub-bucket slots from the driven block to one destination
-constexpr int CAP_REG = 128;        // sub-bucket slots from a regular block to one destination
+constexpr int CAP_REG = 512;        // sub-bucket slots from a regular block to one destination
 constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
 constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
 
@@ -858,6 +858,7 @@ struct PersistBuffers {
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
     uint32_t *eown = nullptr;
+    uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (1 sub-bucket, 2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t perPar = 0, E = 0;
     int32_t eownNb = -1;
@@ -1006,6 +1007,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             for (int k = 0; k < 18; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
         }
         done += ctl[0];
+        pb.overflowSeen |= (uint32_t)ctl[1];
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {
             // a bounded buffer overflowed: the next epoch runs on the per-step engine
@@ -1023,6 +1025,11 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
     }
     return FB_OK;
+}
+
+uint32_t persistent_overflow_flags(const DevBrain &d) {
+    auto it = g_pbuf.find(d.driven);
+    return it == g_pbuf.end() ? 0u : it->second.overflowSeen;
 }
 
 }  // namespace fb

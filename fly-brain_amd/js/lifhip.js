@@ -1,0 +1,184 @@
+'use strict';
+// lifhip.js -- LIFHip, the brain interface of LIFWasm / LIFGpu over the HIP
+// engine (src/lifwasm.js:36-87, src/lifgpu.js:185-370 in Lulzx/fly-brain).
+//
+// Same constructor inputs (graph {indptr, indices, weights, sign}, N, params,
+// seed), the same fields (N, p, t, nslots, v, gE, gI, refr, trace, adapt, res,
+// bias, thr, drive, spikeCount, drivenSet) and the same methods (step,
+// setDrive, setDriveOne, setBias, setThr, addG, pulse, reset, setBackground,
+// setParams, flush, destroy, wake, nAwake).  The device holds the state; the
+// state arrays here are host copies fetched when read after a step, as
+// LIFGpu's shadows are.  drive, thr and bias are written through by the
+// setters; direct element writes into those three arrays reach the device on
+// flush().  Written in Node-12 syntax (CommonJS, no optional chaining).
+const path = require('path');
+
+const native = require(path.join(__dirname, '..', 'lib', 'flybrain_napi.node'));
+
+// DEFAULTS of src/lif.js:6-15
+const DEFAULTS = {
+  dt: 0.5, vRest: -52, vThresh: -45, vReset: -52, tauM: 20, tauSyn: 5, tRef: 2.2, delay: 1.8,
+  wSyn: 0.275, noise: 0, traceTau: 30, adaptInc: 2, adaptTau: 100, depU: 0.2, depTau: 200,
+  minSyn: 1, preInh: 0, coba: false, eExc: 0, eInh: -70, inhGain: 1,
+};
+// fb_params field order (include/flybrain.h)
+const PARAM_FIELDS = ['dt', 'vRest', 'vThresh', 'vReset', 'tauM', 'tauSyn', 'tRef', 'delay', 'traceTau',
+  'adaptInc', 'adaptTau', 'depU', 'depTau', 'coba', 'eExc', 'eInh', 'bgRate', 'bgAmp', 'preInh'];
+const ARRAYS = ['v', 'gE', 'gI', 'refr', 'trace', 'adapt', 'res', 'bias', 'thr', 'drive', 'spikeCount'];
+const HOST_OWNED = ['bias', 'thr', 'drive'];
+const ENGINES = { auto: 0, launch: 1, graph: 2, persistent: 3 };
+
+function paramsVec(p) {
+  if (native.paramsLength() !== PARAM_FIELDS.length) throw new Error('LIFHip: fb_params layout mismatch');
+  const out = new Float64Array(PARAM_FIELDS.length);
+  PARAM_FIELDS.forEach((k, i) => {
+    const v = p[k];
+    out[i] = k === 'coba' ? (v ? 1 : 0) : (v === undefined || v === null ? 0 : +v);
+  });
+  return out;
+}
+
+function nslotsOf(p) { return Math.max(1, Math.round(p.delay / p.dt)) + 1; }
+
+class LIFHip {
+  /**
+   * @param opts {graph: {indptr, indices, weights, sign, counts?}, N?, params?, seed?, engine?}
+   *   engine: 'auto' (default), 'launch', 'graph' or 'persistent'
+   */
+  constructor(opts) {
+    const g = opts.graph;
+    const N = opts.N || g.N || (g.indptr.length - 1);
+    this.N = N;
+    this.p = Object.assign({}, DEFAULTS, opts.params || {});
+    this.engine = ENGINES[opts.engine || 'auto'];
+    if (this.engine === undefined) throw new Error('LIFHip: unknown engine ' + opts.engine);
+    this._graph = native.graphCreate(N, Uint32Array.from(g.indptr), Uint32Array.from(g.indices),
+      Float32Array.from(g.weights), Float32Array.from(g.sign), g.counts ? Int32Array.from(g.counts) : null);
+    this._h = native.brainCreate(this._graph, paramsVec(this.p), opts.seed === undefined ? 1 : opts.seed);
+    this.nslots = nslotsOf(this.p);
+    this.t = 0;
+    this.drivenSet = new Set();
+    this._host = {};
+    this._stale = {};
+    for (const k of ARRAYS) {
+      this._host[k] = k === 'spikeCount' ? new Uint32Array(N) : new Float32Array(N);
+      this._stale[k] = true;
+    }
+    this._fired = new Int32Array(2 * N);
+  }
+
+  /** LIFGpu.create's shape (src/lifgpu.js:185): resolves to a ready brain. */
+  static create(opts) { return Promise.resolve(new LIFHip(opts)); }
+
+  _array(k) {
+    if (this._stale[k]) {
+      native.getArray(this._h, ARRAYS.indexOf(k), this._host[k]);
+      this._stale[k] = false;
+    }
+    return this._host[k];
+  }
+
+  _deviceMoved() { for (const k of ARRAYS) if (HOST_OWNED.indexOf(k) < 0) this._stale[k] = true; }
+
+  get v() { return this._array('v'); }
+  get gE() { return this._array('gE'); }
+  get gI() { return this._array('gI'); }
+  get refr() { return this._array('refr'); }
+  get trace() { return this._array('trace'); }
+  get adapt() { return this._array('adapt'); }
+  get res() { return this._array('res'); }
+  get bias() { return this._array('bias'); }
+  get thr() { return this._array('thr'); }
+  get drive() { return this._array('drive'); }
+  get spikeCount() { return this._array('spikeCount'); }
+  get nAwake() { return this.N; }
+  wake() {}
+
+  /** One step; returns the fired list in lif.c order (a view, valid until the next step). */
+  step() {
+    const nf = native.step(this._h, 1, this.engine);
+    this.t += this.p.dt;
+    this._deviceMoved();
+    const n = native.fired(this._h, this._fired);
+    return this._fired.subarray(0, Math.min(n, nf));
+  }
+
+  /** Many steps off the JS thread; resolves to the spike count of the last step. */
+  run(steps) {
+    return native.runAsync(this._h, steps, this.engine).then((nf) => {
+      this.t += steps * this.p.dt;
+      this._deviceMoved();
+      return nf;
+    });
+  }
+
+  setDriveOne(i, hz) {
+    native.setDriveOne(this._h, i, hz);
+    this._host.drive[i] = hz;
+    if (hz > 0) this.drivenSet.add(i); else this.drivenSet.delete(i);
+  }
+
+  setDrive(ix, hz) {
+    native.setDrive(this._h, Int32Array.from(ix), hz);
+    for (const i of ix) {
+      this._host.drive[i] = hz;
+      if (hz > 0) this.drivenSet.add(i); else this.drivenSet.delete(i);
+    }
+  }
+
+  setBias(ix, mv) {
+    native.setBias(this._h, Int32Array.from(ix), mv);
+    for (const i of ix) this._host.bias[i] = mv;
+  }
+
+  setThr(i, mv) {
+    native.setThr(this._h, i, mv);
+    this._host.thr[i] = mv;
+  }
+
+  addG(i, e, ii) {
+    native.addG(this._h, i, e, ii || 0);
+    this._stale.gE = true;
+    this._stale.gI = true;
+  }
+
+  pulse(ix, mv) {
+    native.pulse(this._h, Int32Array.from(ix), mv);
+    this._stale.gE = true;
+  }
+
+  reset() {
+    native.reset(this._h);
+    this._deviceMoved();
+  }
+
+  setBackground(hz, mv) { native.setBackground(this._h, hz, mv); }
+
+  setParams(q) {
+    Object.assign(this.p, q);
+    native.setParams(this._h, paramsVec(this.p));
+    this.nslots = nslotsOf(this.p);
+  }
+
+  /** Pushes direct element writes of drive / thr / bias to the device and
+   *  drops the cached state arrays. */
+  flush() {
+    for (const k of HOST_OWNED) if (!this._stale[k]) native.setArray(this._h, ARRAYS.indexOf(k), this._host[k]);
+    this._deviceMoved();
+  }
+
+  /** Work counters: edge events, synapse events, spikes, steps. */
+  counters() {
+    const c = native.counters(this._h);
+    return { edgeEvents: c[0], synapseEvents: c[1], spikes: c[2], steps: c[3] };
+  }
+
+  get rng() { return native.scalars(this._h)[0]; }
+
+  destroy() {
+    if (this._h) { native.brainDestroy(this._h); this._h = null; }
+    if (this._graph) { native.graphRelease(this._graph); this._graph = null; }
+  }
+}
+
+module.exports = { LIFHip, DEFAULTS, PARAM_FIELDS, native };
