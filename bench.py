@@ -167,6 +167,18 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     return out
 
 
+def traffic_per_step():
+    """traffic_bytes_per_step from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_persist_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_step:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def adjoint_workload(synth, P):
     """The full-scale graph, COBA (eInh -80), dt 0.1 ms, 1% driven at
     U(100, 200) Hz, 500 neuron types with lognormal(0, 0.2) gains on outgoing
@@ -335,6 +347,13 @@ def main():
         kern_s = secs
         kernel_name = "per-step kernel sequence (k_deliver, k_driven, k_membrane, k_compact)"
     achieved = bytes_ / kern_s / 1e9
+    traffic = None
+    if launches == 1 and not fallback:
+        # measured HBM bytes per step of this kernel (copy-calibrated counter
+        # passes), scaled to the timed launch
+        per_step = traffic_per_step()
+        if per_step:
+            traffic = per_step * args.steps
     out = {
         "metric": "synaptic events/s, full 165k LIF: plain vs batch-64 vs fwd+adjoint",
         "value": value,
@@ -360,7 +379,9 @@ def main():
         "wall_s_host_clock": wall,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": "profiles/k_persist_hbm_counters.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of k_persist, "
+                              "each scaled by the counter's reading for a 256 MiB copy" if traffic else None,
             "kernel": kernel_name,
             "bytes_per_launch": bytes_ / max(1, launches),
             "launch_ms": kms / max(1, launches) if launches else secs * 1000.0,

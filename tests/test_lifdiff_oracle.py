@@ -200,3 +200,69 @@ def test_hard_forward_drive_stream_matches_lif_c(oracle_built):
     assert forced.all() and np.array_equal(np.diff(off), nf)
     assert np.array_equal(idx, ras)
     assert od.rng == ob.rng
+
+
+# ---------------------------------------------------------------- closed forms of the forward pass
+
+def _lone(N, edges=(), counts=(), sign=None, over=None):
+    from oracle import OracleDiff
+    indptr = np.zeros(N + 1, np.uint32)
+    src = np.array([e[0] for e in edges], np.int64)
+    for s_ in src:
+        indptr[s_ + 1:] += 1
+    idx = np.array([e[1] for e in edges], np.uint32)
+    p = P.diff_merged({"dt": 0.1, **(over or {})})
+    net = OracleDiff(N, indptr, idx, np.array(counts, np.float32),
+                     np.ones(N, np.float32) if sign is None else np.array(sign, np.float32), p)
+    net.set_loss(np.zeros(N, np.float32))
+    return net, p
+
+
+def test_membrane_closed_form_under_constant_bias():
+    """No input, laminaBias b on the neuron: v_n = vRest + b (1 - (1 - kM)^n)
+    (src/lifdiff.js:320-321), below threshold."""
+    net, p = _lone(1, over={"laminaBias": 5.0})
+    net.set_masks(np.zeros(1, np.uint8), np.ones(1, np.uint8))
+    kM = p["dt"] / p["tauM"]
+    for n in (1, 10, 200, 1500):
+        net.forward(n, seed=1, record=False)
+        want = p["vRest"] + 5.0 * (1 - (1 - kM) ** n)
+        assert net.state("v")[0] == pytest.approx(want, abs=2e-4), n
+
+
+def test_delivery_lands_after_the_delay_with_the_lifdiff_weight():
+    """A forced spike of neuron 0 at step 0 reaches neuron 1 at step D with
+    counts * inScale * (sign * wSyn * outScale * exp(logGain) * res * amp)
+    (src/lifdiff.js:278-285), then decays with dE."""
+    net, p = _lone(2, edges=[(0, 1)], counts=[3.0], over={"wSyn": 0.4, "tRef": 2.2})
+    drive = np.array([1e7, 0.0], np.float32)          # a draw always fires
+    net.set_input("drive", drive)
+    net.set_input("outScale", np.array([1.5, 1.0], np.float32))
+    net.set_input("logGain", np.array([0.25, 0.0], np.float32))
+    net.set_driven(np.array([0], np.int32))
+    D = P.nslots(p) - 1
+    dE = np.exp(-p["dt"] / p["tauSyn"])
+    net.forward(D, seed=1, record=False)
+    assert net.state("gE")[1] == 0.0
+    net.forward(D + 1, seed=1, record=False)
+    eff = 1.0 * 0.4 * 1.5 * np.exp(np.float64(np.float32(0.25))) * 1.0 * 1.0
+    g = np.float32(np.float64(np.float32(0.0)) + 3.0 * 1.0 * eff)
+    assert net.state("gE")[1] == np.float32(np.float64(g) * dE)
+
+
+def test_forced_refractory_countdown_and_soft_amplitude():
+    """A forced spike sets refr = tRef + dt, then refr -= dt per step; in soft
+    mode the amplitude is the logistic 1 / (1 + exp(-(v - thr) / beta))."""
+    net, p = _lone(1, over={"tRef": 0.5})
+    net.set_input("drive", np.array([1e7], np.float32))
+    net.set_driven(np.array([0], np.int32))
+    net.forward(1, seed=1, record=False)
+    assert net.state("refr")[0] == np.float32(np.float64(np.float32(0.5 + 0.1)) - 0.1)
+    net2, p2 = _lone(1, over={"soft": True, "surrogateBeta": 1.5, "laminaBias": 30.0})
+    net2.set_masks(np.zeros(1, np.uint8), np.ones(1, np.uint8))
+    net2.forward(1, seed=1, record=True)
+    kM = p2["dt"] / p2["tauM"]
+    u = p2["vRest"] + (p2["vRest"] - p2["vRest"] + 30.0) * kM
+    s = 1.0 / (1.0 + np.exp(-(u - p2["vThresh"]) / 1.5))
+    assert net2.state("spikeCount")[0] == np.float32(s)
+    assert net2.state("v")[0] == np.float32(s * p2["vReset"] + (1 - s) * u)
