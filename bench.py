@@ -140,7 +140,10 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     ms = e0.elapsed_ms(e1)
     c1 = eb.counters()
     ev = c1["edge_events"] - c0["edge_events"]
+    la = {"launches": eb.stat("lookahead_launches"), "kernel_ms": eb.stat("lookahead_kernel_ms"),
+          "steps": eb.stat("lookahead_steps"), "fallback_steps": eb.stat("fallback_steps")}
     out = {"value": ev / (ms / 1000.0), "unit": "synaptic events/s (aggregate over members)", "members": B,
+           "engine": "lookahead" if la["steps"] else "per-step", "lookahead": la,
            "steps": T, "member_steps": int(steps.sum()), "ms_per_step": ms / T,
            "spikes": c1["spikes"] - c0["spikes"],
            "workload": "16,384 neurons, ~1.07M edges, 64 members: own seeds, 500 per-type gains lognormal(0,0.2), "

@@ -80,6 +80,15 @@ class LIFBatch:
     def set_stream(self, handle: int | None):
         _l.check(self.lib.fb_batch_set_stream(self.h, ctypes.c_void_p(handle or 0)))
 
+    def set_engine(self, engine: str):
+        """'auto', 'launch' (per-step kernels) or 'persistent' (lookahead)."""
+        _l.check(self.lib.fb_batch_set_engine(self.h, _l.ENGINES[engine]))
+
+    def stat(self, name: str) -> float:
+        v = ctypes.c_double()
+        _l.check(self.lib.fb_batch_stat(self.h, name.encode(), ctypes.byref(v)))
+        return float(v.value)
+
     def run(self, steps: int):
         _l.check(self.lib.fb_batch_run(self.h, int(steps), None, None, None, None, 0, None))
 

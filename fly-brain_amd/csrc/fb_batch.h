@@ -1,0 +1,53 @@
+// fb_batch.h -- device-side description of an ensemble (shared by the
+// per-step ensemble kernels and the ensemble lookahead kernel).
+//
+// Layout in HBM: per-member state [N][B] (member b of neuron i at i*B + b, so
+// a wave with lane = member reads one neuron of every member in one
+// transaction); the delay line as per-slot spike and forced masks (bit b =
+// member b) with a list of neurons whose mask is nonzero; accumulators
+// [2N][B] with the ordered-sum chain pool of the forward engine.
+#pragma once
+#include "fb_internal.h"
+
+namespace fb {
+
+constexpr int BMAX = 64;
+
+struct BatchDev {
+    int32_t N, B, nslots, nDriven, nTypes;
+    // adapt stays exactly 0 without adaptInc, res exactly 1 without depU, and
+    // there are no background hits without bgRate: those arrays are then
+    // neither read nor written
+    int32_t liveAdapt, liveRes, liveBg;
+    Consts k;
+    DevGraph g;
+    // [N][B]
+    float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr;
+    uint32_t *spikeCount;
+    float *drive;                  // [N], shared by every member
+    const int32_t *driven;         // [nDriven]
+    const int32_t *drvPos;         // [N]
+    const int32_t *typeOf;         // [N]
+    const float *gain;             // [B][nTypes]
+    const float *gainT;            // [nTypes][B], the same values by type
+    uint32_t *rng;                 // [B]
+    float *bgAcc;                  // [1] (the same for every member)
+    int32_t *steps;                // [B] run length of each member
+    // delay line: per slot, spike and forced masks per neuron and a list of
+    // neurons with a nonzero mask
+    uint64_t *mask, *fmask;        // [nslots][N]
+    int32_t *active, *nActive;     // [nslots][N], [nslots]
+    // accumulators [2N][B] and the chain pool
+    int32_t *cnt, *chain;
+    float *add;
+    uint64_t *key;
+    CollEntry *coll;
+    int64_t collCap;
+    int32_t *ncoll;
+    int32_t *bgCnt;                // [N][B]
+    const uint32_t *leap;
+    int32_t *err;
+    unsigned long long *counters;  // [0] edge events, [1] synapse events, [2] spikes
+};
+
+}  // namespace fb

@@ -27,53 +27,16 @@
 //                active list.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
-#include "fb_internal.h"
+#include "fb_batch.h"
 
 namespace fb {
 
 uint32_t seed_hash(double seed);
 void derive_consts(const fb_params &p, int32_t N, Consts &c);
 int32_t nslots_of(const fb_params &p);
-
-constexpr int BMAX = 64;
-
-struct BatchDev {
-    int32_t N, B, nslots, nDriven, nTypes;
-    // adapt stays exactly 0 without adaptInc, res exactly 1 without depU, and
-    // there are no background hits without bgRate: those arrays are then
-    // neither read nor written
-    int32_t liveAdapt, liveRes, liveBg;
-    Consts k;
-    DevGraph g;
-    // [N][B]
-    float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr;
-    uint32_t *spikeCount;
-    float *drive;                  // [N], shared by every member
-    const int32_t *driven;         // [nDriven]
-    const int32_t *drvPos;         // [N]
-    const int32_t *typeOf;         // [N]
-    const float *gain;             // [B][nTypes]
-    uint32_t *rng;                 // [B]
-    float *bgAcc;                  // [1] (the same for every member)
-    int32_t *steps;                // [B] run length of each member
-    // delay line: per slot, spike and forced masks per neuron and a list of
-    // neurons with a nonzero mask
-    uint64_t *mask, *fmask;        // [nslots][N]
-    int32_t *active, *nActive;     // [nslots][N], [nslots]
-    // accumulators [2N][B] and the chain pool
-    int32_t *cnt, *chain;
-    float *add;
-    uint64_t *key;
-    CollEntry *coll;
-    int64_t collCap;
-    int32_t *ncoll;
-    int32_t *bgCnt;                // [N][B]
-    const uint32_t *leap;
-    int32_t *err;
-    unsigned long long *counters;  // [0] edge events, [1] synapse events, [2] spikes
-};
 
 __device__ __forceinline__ uint64_t lanes_active(const BatchDev &d, int32_t t) {
     // members still running at launch step t
@@ -375,7 +338,18 @@ struct fb_batch {
     int32_t *idxScratch = nullptr;
     int32_t *recCnt = nullptr, *recIds = nullptr;
     uint64_t *recMasks = nullptr, *recF = nullptr;
+    int32_t engine = FB_ENGINE_AUTO;
+    uint64_t drivenVersion = 0;
+    int64_t laLaunches = 0, laFallback = 0, laSteps = 0;
+    double laMs = 0.0;
 };
+
+namespace fb {
+bool batch_la_supported(const BatchDev &d, std::string *why);
+void batch_la_release(const void *key);
+fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t epochs, uint64_t drivenVersion,
+                          uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done, double *kernelMs);
+}
 
 // graph internals needed here (defined in fb_api.hip)
 struct fb_graph;
@@ -461,6 +435,15 @@ extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B,
     FB_HIP(hipMemcpy(rng, r.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(typ, typeOf, sizeof(int32_t) * N, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(gain, gains, sizeof(float) * (size_t)B * nTypes, hipMemcpyHostToDevice));
+    {
+        float *gT = nullptr;
+        if ((st = balloc(b, &gT, (size_t)B * nTypes))) return st;
+        std::vector<float> t((size_t)B * nTypes);
+        for (int32_t q = 0; q < B; q++)
+            for (int32_t T = 0; T < nTypes; T++) t[(size_t)T * B + q] = gains[(size_t)q * nTypes + T];
+        FB_HIP(hipMemcpy(gT, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+        d.gainT = gT;
+    }
     std::vector<int32_t> neg((size_t)N, -1);
     FB_HIP(hipMemcpy(drvPos, neg.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
     FB_HIP(hipMemset(drive, 0, sizeof(float) * N));
@@ -481,6 +464,7 @@ extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B,
 extern "C" fb_status fb_batch_destroy(fb_batch *b) {
     FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_destroy: null batch");
     (void)hipStreamSynchronize(b->stream);
+    batch_la_release((const void *)b->d.v);
     for (void *p : b->allocs) (void)hipFree(p);
     void *rec[] = {b->recCnt, b->recIds, b->recMasks, b->recF};
     for (void *p : rec)
@@ -511,6 +495,7 @@ extern "C" fb_status fb_batch_set_drive(fb_batch *b, const int32_t *idx, int32_t
                      hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy((void *)b->d.drvPos, pos.data(), sizeof(int32_t) * b->d.N, hipMemcpyHostToDevice));
     b->d.nDriven = (int32_t)b->drivenList.size();
+    b->drivenVersion++;
     return FB_OK;
 }
 
@@ -559,9 +544,10 @@ static fb_status batch_step(fb_batch *b) {
 // Runs nsteps; if ids/masks are given, records every step's active neurons with
 // their spike and forced masks (per step: count into nactive[step], entries
 // appended at most cap in total; the returned total may exceed cap).
-extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
-                                  uint64_t *fmasks, int64_t cap, int64_t *total) {
-    FB_REQUIRE_B(b && nsteps >= 0, FB_ERR_ARG, "fb_batch_run: bad arguments");
+// nsteps on the per-step kernels; with recording, appends every step's
+// active neurons and masks after *tot entries
+static fb_status run_per_step(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
+                              uint64_t *fmasks, int64_t cap, int64_t *tot) {
     const bool rec = nactive != nullptr;
     if (rec && !b->recCnt) {
         FB_HIP(hipMalloc(&b->recCnt, sizeof(int32_t)));
@@ -569,7 +555,6 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
         FB_HIP(hipMalloc(&b->recMasks, sizeof(uint64_t) * b->d.N));
         FB_HIP(hipMalloc(&b->recF, sizeof(uint64_t) * b->d.N));
     }
-    int64_t tot = 0;
     for (int32_t s = 0; s < nsteps; s++) {
         fb_status st = batch_step(b);
         if (st) return st;
@@ -580,22 +565,134 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
             FB_HIP(hipMemcpyAsync(&n, b->recCnt, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
             FB_HIP(hipStreamSynchronize(b->stream));
             nactive[s] = n;
-            const int64_t take = std::max<int64_t>(0, std::min<int64_t>(n, cap - tot));
+            const int64_t take = std::max<int64_t>(0, std::min<int64_t>(n, cap - *tot));
             if (take > 0 && ids) {
-                FB_HIP(hipMemcpyAsync(ids + tot, b->recIds, sizeof(int32_t) * take, hipMemcpyDeviceToHost, b->stream));
-                FB_HIP(hipMemcpyAsync(masks + tot, b->recMasks, sizeof(uint64_t) * take, hipMemcpyDeviceToHost, b->stream));
-                FB_HIP(hipMemcpyAsync(fmasks + tot, b->recF, sizeof(uint64_t) * take, hipMemcpyDeviceToHost, b->stream));
+                FB_HIP(hipMemcpyAsync(ids + *tot, b->recIds, sizeof(int32_t) * take, hipMemcpyDeviceToHost, b->stream));
+                FB_HIP(hipMemcpyAsync(masks + *tot, b->recMasks, sizeof(uint64_t) * take, hipMemcpyDeviceToHost,
+                                      b->stream));
+                FB_HIP(hipMemcpyAsync(fmasks + *tot, b->recF, sizeof(uint64_t) * take, hipMemcpyDeviceToHost,
+                                      b->stream));
                 FB_HIP(hipStreamSynchronize(b->stream));
             }
-            tot += n;
+            *tot += n;
         }
     }
+    return FB_OK;
+}
+
+// Runs nsteps; if ids/masks are given, records every step's active neurons with
+// their spike and forced masks (per step: count into nactive[step], entries
+// appended at most cap in total; the returned total may exceed cap).
+extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
+                                  uint64_t *fmasks, int64_t cap, int64_t *total) {
+    FB_REQUIRE_B(b && nsteps >= 0, FB_ERR_ARG, "fb_batch_run: bad arguments");
+    BatchDev &d = b->d;
+    const bool rec = nactive != nullptr;
+    const int32_t D = d.nslots - 1;
+    std::string why;
+    const bool ok = batch_la_supported(d, &why);
+    bool la = b->engine == FB_ENGINE_PERSISTENT || (b->engine == FB_ENGINE_AUTO && ok && nsteps >= 2 * D);
+    if (la && !ok) {
+        set_error("fb_batch_run: the lookahead engine does not take this ensemble: %s", why.c_str());
+        return FB_ERR_UNSUPPORTED;
+    }
+    int64_t tot = 0;
+    int32_t s0 = 0;
+    fb_status st;
+    // with recording, launches are cut to whole epochs of at most ~512 steps
+    const int32_t recEpochs = std::max(1, 512 / std::max(1, D));
+    uint64_t *recM = nullptr, *recF = nullptr;
+    std::vector<uint64_t> hM, hF;
+    while (la && nsteps - s0 >= D) {
+        int32_t epochs = (nsteps - s0) / D;
+        if (rec) {
+            epochs = std::min(epochs, recEpochs);
+            const size_t n = (size_t)epochs * D * d.N;
+            if (!recM) {
+                FB_HIP(hipMalloc(&recM, sizeof(uint64_t) * (size_t)recEpochs * D * d.N));
+                FB_HIP(hipMalloc(&recF, sizeof(uint64_t) * (size_t)recEpochs * D * d.N));
+            }
+            FB_HIP(hipMemsetAsync(recM, 0, sizeof(uint64_t) * n, b->stream));
+            FB_HIP(hipMemsetAsync(recF, 0, sizeof(uint64_t) * n, b->stream));
+        }
+        int32_t done = 0;
+        if ((st = batch_la_launch(d, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done,
+                                  &b->laMs)))
+            return st;
+        b->laLaunches++;
+        const int32_t m = done * D;
+        if (rec && m > 0) {
+            hM.resize((size_t)m * d.N);
+            hF.resize((size_t)m * d.N);
+            FB_HIP(hipMemcpy(hM.data(), recM, sizeof(uint64_t) * hM.size(), hipMemcpyDeviceToHost));
+            FB_HIP(hipMemcpy(hF.data(), recF, sizeof(uint64_t) * hF.size(), hipMemcpyDeviceToHost));
+            for (int32_t q = 0; q < m; q++) {
+                int32_t cnt = 0;
+                for (int32_t i = 0; i < d.N; i++) {
+                    const uint64_t mk = hM[(size_t)q * d.N + i];
+                    if (!mk) continue;
+                    if (tot < cap && ids) {
+                        ids[tot] = i;
+                        masks[tot] = mk;
+                        fmasks[tot] = hF[(size_t)q * d.N + i];
+                    }
+                    tot++;
+                    cnt++;
+                }
+                nactive[s0 + q] = cnt;
+            }
+        }
+        b->t += m;
+        b->head = (b->head + m) % d.nslots;
+        b->lastSlot = (b->head + d.nslots - 1) % d.nslots;
+        b->laSteps += m;
+        s0 += m;
+        if (done < epochs) {
+            // a bucket overflowed: this epoch on the per-step kernels
+            const int32_t k = std::min(D, nsteps - s0);
+            if ((st = run_per_step(b, k, rec ? nactive + s0 : nullptr, ids, masks, fmasks, cap, &tot))) return st;
+            b->laFallback += k;
+            s0 += k;
+        }
+    }
+    if (recM) {
+        (void)hipFree(recM);
+        (void)hipFree(recF);
+    }
+    if ((st = run_per_step(b, nsteps - s0, rec ? nactive + s0 : nullptr, ids, masks, fmasks, cap, &tot))) return st;
     if (total) *total = tot;
     int32_t err = 0;
     FB_HIP(hipMemcpyAsync(&err, b->d.err, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
     FB_REQUIRE_B(!(err & FB_DEVERR_COLL_OVERFLOW), FB_ERR_OVERFLOW, "fb_batch_run: arrival chain pool overflowed");
     FB_REQUIRE_B(!(err & 2), FB_ERR_UNSUPPORTED, "fb_batch_run: more than 2047 draws in one step");
+    return FB_OK;
+}
+
+// FB_ENGINE_AUTO (lookahead when the ensemble allows it), FB_ENGINE_LAUNCH
+// (per-step kernels) or FB_ENGINE_PERSISTENT (lookahead, or an error)
+extern "C" fb_status fb_batch_set_engine(fb_batch *b, int32_t engine) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_engine: null batch");
+    FB_REQUIRE_B(engine == FB_ENGINE_AUTO || engine == FB_ENGINE_LAUNCH || engine == FB_ENGINE_PERSISTENT, FB_ERR_ARG,
+                 "fb_batch_set_engine: AUTO, LAUNCH or PERSISTENT");
+    b->engine = engine;
+    return FB_OK;
+}
+
+// "lookahead_launches", "lookahead_kernel_ms", "lookahead_steps",
+// "fallback_steps", "lookahead_supported"
+extern "C" fb_status fb_batch_stat(fb_batch *b, const char *name, double *value) {
+    FB_REQUIRE_B(b && name && value, FB_ERR_ARG, "fb_batch_stat: null argument");
+    const std::string n(name);
+    if (n == "lookahead_launches") *value = (double)b->laLaunches;
+    else if (n == "lookahead_kernel_ms") *value = b->laMs;
+    else if (n == "lookahead_steps") *value = (double)b->laSteps;
+    else if (n == "fallback_steps") *value = (double)b->laFallback;
+    else if (n == "lookahead_supported") *value = batch_la_supported(b->d, nullptr) ? 1.0 : 0.0;
+    else {
+        set_error("fb_batch_stat: unknown statistic '%s'", name);
+        return FB_ERR_ARG;
+    }
     return FB_OK;
 }
 

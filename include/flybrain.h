@@ -174,6 +174,16 @@ fb_status fb_batch_rng(fb_batch *b, uint32_t *rng);
 fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
                             int64_t *steps);
 fb_status fb_batch_set_stream(fb_batch *b, void *stream);
+/* FB_ENGINE_AUTO (default: the lookahead engine when the ensemble allows it
+ * and a run covers at least two delay windows), FB_ENGINE_LAUNCH (per-step
+ * kernels) or FB_ENGINE_PERSISTENT (the lookahead engine, or
+ * FB_ERR_UNSUPPORTED).  Both give identical results.  No counterpart in the
+ * reference. */
+fb_status fb_batch_set_engine(fb_batch *b, int32_t engine);
+/* "lookahead_launches", "lookahead_kernel_ms", "lookahead_steps",
+ * "fallback_steps" (steps handed to the per-step kernels after a bucket
+ * overflow), "lookahead_supported" */
+fb_status fb_batch_stat(fb_batch *b, const char *name, double *value);
 
 /* ---------------------------------------------------------------- differentiable model
  * The reference's LIFDiff (src/lifdiff.js:84-686) on the GPU: a COBA network

@@ -22,10 +22,11 @@ def member_weights(g, type_of, gains_b):
     return (g.counts.astype(np.float64) * gains_b[type_of[pre]].astype(np.float64)).astype(np.float32)
 
 
-def run_case(g, sign, p, seeds, type_of, gains, drive_ix, rate, silenced, steps, bias=None):
+def run_case(g, sign, p, seeds, type_of, gains, drive_ix, rate, silenced, steps, bias=None, engine="auto"):
     B = len(seeds)
     hg = lif.HipGraph(g.N, g.indptr, g.indices, g.weights, sign, g.counts)
     eb = batch.LIFBatch(hg, p, seeds, type_of, gains)
+    eb.set_engine(engine)
     eb.setDrive(drive_ix, rate)
     for b in range(B):
         if len(silenced[b]):
@@ -56,8 +57,8 @@ def run_case(g, sign, p, seeds, type_of, gains, drive_ix, rate, silenced, steps,
     return eb
 
 
-def test_small_batch_members_match_single_brains():
-    g = synth.tiny_graph(512, 30, seed=41, p_count=0.3)
+def _small(seed):
+    g = synth.tiny_graph(512, 30, seed=seed, p_count=0.3)
     rng = np.random.default_rng(3)
     B, T = 8, 10
     type_of = rng.integers(0, T, size=g.N).astype(np.int32)
@@ -65,12 +66,46 @@ def test_small_batch_members_match_single_brains():
     sign = (np.where(g.exc, 1.0, -1.0) * 0.5).astype(np.float32)
     silenced = [np.sort(rng.choice(g.N, size=int(rng.integers(0, 25)), replace=False)) for _ in range(B)]
     steps = rng.integers(300, 800, size=B)
+    return g, B, type_of, gains, sign, silenced, steps
+
+
+def test_small_batch_members_match_single_brains():
+    """Adaptation and depression on: the per-step kernels."""
+    g, B, type_of, gains, sign, silenced, steps = _small(41)
     p = P.config_params(adaptInc=0.5, depU=0.1)
     run_case(g, sign, p, [b + 1 for b in range(B)], type_of, gains, np.arange(0, 512, 6), 350.0, silenced, steps,
              bias=(list(range(0, 512, 3)), 6.0))
 
 
-def test_batch64_config3_members_match_single_brains():
+@pytest.mark.parametrize("engine", ["launch", "persistent"])
+def test_small_batch_both_engines_match_single_brains(engine):
+    """The lookahead engine and the per-step kernels on the same ensemble,
+    with biased neurons and uneven run lengths."""
+    g, B, type_of, gains, sign, silenced, steps = _small(43)
+    p = P.config_params()
+    eb = run_case(g, sign, p, [b + 1 for b in range(B)], type_of, gains, np.arange(0, 512, 6), 350.0, silenced,
+                  steps, bias=(list(range(0, 512, 3)), 6.0), engine=engine)
+    if engine == "persistent":
+        assert eb.stat("lookahead_steps") > 0
+
+
+def test_lookahead_ensemble_bucket_overflow_falls_back():
+    """A saturated ensemble overflows the lookahead buckets; those epochs run
+    on the per-step kernels and every member stays exact."""
+    g = synth.tiny_graph(2048, 200, seed=31, p_count=0.5)
+    B = 8
+    type_of = np.zeros(g.N, np.int32)
+    gains = np.ones((B, 1), np.float32)
+    sign = (np.where(g.exc, 1.0, -1.0) * 1.5).astype(np.float32)
+    silenced = [np.zeros(0, np.int64) for _ in range(B)]
+    steps = np.full(B, 400)
+    eb = run_case(g, sign, P.config_params(), [b + 1 for b in range(B)], type_of, gains, np.arange(0, 2048, 10),
+                  400.0, silenced, steps, bias=(list(range(2048)), 6.5), engine="persistent")
+    assert eb.stat("fallback_steps") > 0
+
+
+@pytest.mark.parametrize("engine", ["launch", "persistent"])
+def test_batch64_config3_members_match_single_brains(engine):
     """The ensemble workload: the 16,384-neuron graph, B = 64, 500 types with
     lognormal(0, 0.2) gains, 0-5% silenced per member, uneven run lengths
     (shortened here so 64 oracle runs finish in seconds)."""
@@ -82,6 +117,6 @@ def test_batch64_config3_members_match_single_brains():
     steps = np.array([200 + synth.run_length(b) // 10 for b in range(B)])
     drv = synth.driven_set(g.N, 0.02, 3)
     eb = run_case(g, g.sign, P.config_params(), [b + 1 for b in range(B)], type_of, gains, drv, 150.0, silenced,
-                  steps)
+                  steps, engine=engine)
     c = eb.counters()
     assert c["edge_events"] > 0 and c["spikes"] > 0
