@@ -691,6 +691,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     fb_status st = sync_driven(b);
     if (st) return st;
     if (nsteps == 0) return FB_OK;
+    const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
         // the lookahead engine pays a prologue per call: worth it from two epochs on
         std::string why;
@@ -707,6 +708,11 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
+            if (st == FB_ERR_UNSUPPORTED && autoChosen) {
+                // the device refused the cooperative launch before any step ran
+                st = run_graph(b, nsteps);
+                break;
+            }
             if (st) return st;
             b->lastSlot = (b->head + nsteps - 1 + b->d.nslots - 1) % b->d.nslots;
             b->head = (b->head + nsteps) % b->d.nslots;

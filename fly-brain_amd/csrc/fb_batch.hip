@@ -616,9 +616,13 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
             FB_HIP(hipMemsetAsync(recF, 0, sizeof(uint64_t) * n, b->stream));
         }
         int32_t done = 0;
-        if ((st = batch_la_launch(d, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done,
-                                  &b->laMs)))
-            return st;
+        st = batch_la_launch(d, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done, &b->laMs);
+        if (st == FB_ERR_UNSUPPORTED && b->engine == FB_ENGINE_AUTO) {
+            // the device refused the cooperative launch: the per-step kernels
+            la = false;
+            break;
+        }
+        if (st) return st;
         b->laLaunches++;
         const int32_t m = done * D;
         if (rec && m > 0) {

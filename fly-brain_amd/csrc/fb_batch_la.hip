@@ -775,7 +775,13 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
     a.eown = b.eown; a.ctl = b.ctl; a.recM = recM; a.recF = recF;
     void *args[] = {&a};
     FB_HIP(hipEventRecord(b.ev0, st));
-    FB_HIP(hipLaunchCooperativeKernel((const void *)k_batch_la, dim3(S), dim3(BT), args, 0, st));
+    const hipError_t le = hipLaunchCooperativeKernel((const void *)k_batch_la, dim3(S), dim3(BT), args, 0, st);
+    if (le != hipSuccess) {
+        // refused (the resource behind grid-wide barriers is held elsewhere): nothing ran
+        (void)hipGetLastError();
+        set_error("ensemble lookahead: cooperative launch refused: %s", hipGetErrorString(le));
+        return FB_ERR_UNSUPPORTED;
+    }
     FB_HIP(hipEventRecord(b.ev1, st));
     int32_t ctl[4];
     FB_HIP(hipMemcpyAsync(ctl, b.ctl, sizeof ctl, hipMemcpyDeviceToHost, st));

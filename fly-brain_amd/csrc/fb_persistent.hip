@@ -992,7 +992,16 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
-        FB_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st));
+        const hipError_t le = hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
+        if (le != hipSuccess) {
+            // The device refused the cooperative launch (the runtime gives the
+            // resource behind grid-wide barriers to one process at a time).
+            // Before any step ran this is reported as unsupported, so that the
+            // automatic engine choice can step with the hipGraph engine.
+            (void)hipGetLastError();
+            set_error("persistent engine: cooperative launch refused: %s", hipGetErrorString(le));
+            return done == 0 ? FB_ERR_UNSUPPORTED : FB_ERR_DEVICE;
+        }
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         int32_t ctl[4];
