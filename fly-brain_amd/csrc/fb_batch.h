@@ -36,6 +36,8 @@ struct BatchDev {
     // delay line: per slot, spike and forced masks per neuron and a list of
     // neurons with a nonzero mask
     uint64_t *mask, *fmask;        // [nslots][N]
+    int32_t *frank;                // [nslots][N] driven-list position of a forced spike when it was emitted
+                                   // (its arrival rank; the list may change while it is in flight)
     int32_t *active, *nActive;     // [nslots][N], [nslots]
     // accumulators [2N][B] and the chain pool
     int32_t *cnt, *chain;

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
         const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
         const float sgn = d.g.sign[pre];
         const int32_t typ = d.typeOf[pre];
-        const int32_t dp = d.drvPos[pre];
+        const int32_t fr = forced ? d.frank[(int64_t)head * N + pre] : 0;
         // lane b holds member b's gain for this row's type and its res
         const float gLane = lane < B ? d.gain[(int64_t)lane * d.nTypes + typ] : 0.f;
         const float rLane = (d.liveRes && lane < B) ? d.res[(int64_t)pre * B + lane] : 1.f;
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
             if (s == 0.f) continue;
             // arrival rank in member b's lif.c list: forced spikes (driven
             // order) before threshold spikes (ascending index)
-            const uint32_t rank = ((forced >> b) & 1ull) ? (uint32_t)dp : (uint32_t)N + (uint32_t)pre;
+            const uint32_t rank = ((forced >> b) & 1ull) ? (uint32_t)fr : (uint32_t)N + (uint32_t)pre;
             if (lane == 0) {
                 if (d.liveRes) d.res[ib] = resb - d.k.depU * resb;
                 waveEdges += (unsigned long long)(re - ra);
@@ -201,7 +201,10 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
                 f = uu < pf[u];
             }
             const uint64_t fb = __ballot(f);
-            if (lane == 0) d.fmask[(int64_t)slot * d.N + i] = fb;
+            if (lane == 0) {
+                d.fmask[(int64_t)slot * d.N + i] = fb;
+                d.frank[(int64_t)slot * d.N + i] = c0 + u;
+            }
         }
     }
     if (wave == 0 && on) {
@@ -416,6 +419,7 @@ extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B,
         (st = balloc(b, &gain, (size_t)B * nTypes)) || (st = balloc(b, &rng, (size_t)B)) ||
         (st = balloc(b, &bgAcc, 1)) || (st = balloc(b, &stepsD, (size_t)B)) ||
         (st = balloc(b, &mask, (size_t)d.nslots * N)) || (st = balloc(b, &fmask, (size_t)d.nslots * N)) ||
+        (st = balloc(b, &d.frank, (size_t)d.nslots * N)) ||
         (st = balloc(b, &active, (size_t)d.nslots * N)) || (st = balloc(b, &nActive, (size_t)d.nslots)) ||
         (st = balloc(b, &cnt, 2 * NB)) || (st = balloc(b, &chain, 2 * NB)) || (st = balloc(b, &add, 2 * NB)) ||
         (st = balloc(b, &key, 2 * NB)) || (st = balloc(b, &bgCnt, NB)) || (st = balloc(b, &ncoll, 1)) ||

@@ -102,6 +102,7 @@ struct LaShared {
         struct {
             int32_t pre[LSPK];
             uint32_t info[LSPK];           // step | forced << 8
+            uint32_t rank[LSPK];           // arrival rank
             uint64_t m[LSPK];
             int32_t rowOff[LSPK + 1];
             uint32_t rowA[LSPK];
@@ -252,13 +253,16 @@ __device__ __forceinline__ float apply_cell(const LaShared &s, int32_t c, float 
 // ------------------------------------------------------------------ push
 
 // Appends a spike entry (neuron, step, forced?, mask) to the block's list.
+// rank: the spike's position class in lif.c's list (driven position when it
+// was emitted for a forced spike, N + index otherwise)
 __device__ __forceinline__ void push_entry(const LaArgs &a, LaShared &s, int32_t pre, int32_t step, bool forced,
-                                           uint64_t m) {
+                                           uint32_t rank, uint64_t m) {
     if (a.d.g.sign[pre] == 0.f) return;            // delivers nothing (lif.c:37)
     const int32_t k = atomicAdd(&s.nSpk, 1);
     if (k < LSPK) {
         s.u.p.pre[k] = pre;
         s.u.p.info[k] = (uint32_t)step | (forced ? 256u : 0u);
+        s.u.p.rank[k] = rank;
         s.u.p.m[k] = m;
     }
 }
@@ -342,13 +346,12 @@ __device__ void la_push(const LaArgs &a, LaShared &s, int32_t par, int32_t tArr0
                 if (pos >= LEV) continue;              // flagged at the reservation
                 const int32_t pre = s.u.p.pre[k];
                 const int32_t step = (int32_t)(s.u.p.info[k] & 255u);
-                const bool forced = (s.u.p.info[k] & 256u) != 0;
                 const uint32_t cls = d.g.sign[pre] > 0.f ? 0u : 1u;
                 const bool drvDst = (own & OWN_DRV) != 0;
                 const uint32_t tloc = drvDst ? (own & 0xFFFFu) : (own & 0xFFu);
                 const uint32_t CW = drvDst ? (uint32_t)BDRV : (uint32_t)RNB;
                 const uint64_t cell = ((uint64_t)step * 2u + cls) * CW + tloc;
-                const uint64_t rank = forced ? (uint64_t)d.drvPos[pre] : (uint64_t)d.N + (uint64_t)pre;
+                const uint64_t rank = s.u.p.rank[k];
                 const int64_t at = ((int64_t)par * a.S + dst) * LEV + pos;
                 a.evKey[at] = (cell << KEY_CELL_SHIFT) | (rank << 23) | (uint64_t)jr;
                 a.evMask[at] = mdst;
@@ -394,8 +397,8 @@ __device__ void la_prologue(const LaArgs &a, LaShared &s, int32_t me, int32_t nb
         const int32_t slot = (a.head0 + st) % d.nslots;
         const int32_t i = d.active[(int64_t)slot * d.N + (g - off[st])];
         const uint64_t m = d.mask[(int64_t)slot * d.N + i], f = d.fmask[(int64_t)slot * d.N + i];
-        if (f) push_entry(a, s, i, st, true, f);
-        if (m & ~f) push_entry(a, s, i, st, false, m & ~f);
+        if (f) push_entry(a, s, i, st, true, (uint32_t)d.frank[(int64_t)slot * d.N + i], f);
+        if (m & ~f) push_entry(a, s, i, st, false, (uint32_t)d.N + (uint32_t)i, m & ~f);
     }
     la_push(a, s, 0, a.t0);
 }
@@ -582,7 +585,7 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
                 for (int32_t q = tid; q < D * RNB; q += BT) {
                     const int32_t st = q / RNB, tl = q % RNB;
                     const uint64_t mk = s.spk.r[st][tl];
-                    if (mk) push_entry(a, s, rbase + tl, st, false, mk);
+                    if (mk) push_entry(a, s, rbase + tl, st, false, (uint32_t)N + (uint32_t)(rbase + tl), mk);
                 }
             } else {
                 for (int32_t q = tid; q < D * a.nDrv; q += BT) {
@@ -591,8 +594,8 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
                     if (!byte) continue;
                     const int32_t i = d.driven[p];
                     const uint64_t sh = (uint64_t)MPG * me;
-                    if (byte & 15u) push_entry(a, s, i, st, true, (uint64_t)(byte & 15u) << sh);
-                    if (byte >> 4) push_entry(a, s, i, st, false, (uint64_t)(byte >> 4) << sh);
+                    if (byte & 15u) push_entry(a, s, i, st, true, (uint32_t)p, (uint64_t)(byte & 15u) << sh);
+                    if (byte >> 4) push_entry(a, s, i, st, false, (uint32_t)N + (uint32_t)i, (uint64_t)(byte >> 4) << sh);
                 }
             }
             la_push(a, s, (e + 1) & 1, tE + D);
@@ -659,7 +662,10 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
                 const uint64_t fb = (uint64_t)(byte & 15u) << sh, tb = (uint64_t)(byte >> 4) << sh;
                 const unsigned long long old = atomicOr((unsigned long long *)&d.mask[(int64_t)slot * N + i],
                                                         (unsigned long long)(fb | tb));
-                if (fb) atomicOr((unsigned long long *)&d.fmask[(int64_t)slot * N + i], (unsigned long long)fb);
+                if (fb) {
+                    atomicOr((unsigned long long *)&d.fmask[(int64_t)slot * N + i], (unsigned long long)fb);
+                    d.frank[(int64_t)slot * N + i] = p;
+                }
                 if (!old) {
                     const int32_t pos = atomicAdd(&d.nActive[slot], 1);
                     d.active[(int64_t)slot * N + pos] = i;

@@ -120,3 +120,49 @@ def test_batch64_config3_members_match_single_brains(engine):
                   steps, engine=engine)
     c = eb.counters()
     assert c["edge_events"] > 0 and c["spikes"] > 0
+
+
+def test_engine_switches_and_drive_changes_keep_members_exact():
+    """Runs alternate between the per-step kernels and the lookahead engine
+    (the delay line is handed over both ways, at lengths that are not whole
+    epochs), and the driven set changes between runs; every member stays
+    bit-identical to its single-brain run with the same calls."""
+    g = synth.tiny_graph(3000, 40, seed=53, p_count=0.3)
+    B = 21
+    rng = np.random.default_rng(9)
+    type_of = rng.integers(0, 7, size=g.N).astype(np.int32)
+    gains = np.exp(rng.normal(0, 0.2, size=(B, 7))).astype(np.float32)
+    sign = (np.where(g.exc, 1.0, -1.0) * 0.55).astype(np.float32)
+    p = P.config_params()
+    hg = lif.HipGraph(g.N, g.indptr, g.indices, g.weights, sign, g.counts)
+    eb = batch.LIFBatch(hg, p, [b + 7 for b in range(B)], type_of, gains)
+    oracles = []
+    for b in range(B):
+        o = OracleBrain(g, p, seed=b + 7, W=member_weights(g, type_of, gains[b]), sign=sign)
+        oracles.append(o)
+    drv1 = np.arange(0, 3000, 13)
+    drv2 = np.arange(5, 3000, 29)
+    plan = [("drive", drv1, 300.0), ("launch", 55), ("persistent", 200), ("launch", 17), ("persistent", 91),
+            ("drive", drv2, 260.0), ("drive", drv1[:40], 0.0), ("persistent", 150), ("launch", 40)]
+    nf_all = [[] for _ in range(B)]
+    for item in plan:
+        if item[0] == "drive":
+            eb.setDrive(item[1], item[2])
+            for o in oracles:
+                o.setDrive(item[1].tolist(), item[2])
+            continue
+        engine, n = item
+        eb.set_engine(engine)
+        nact, ids, masks, fm = eb.run_record(n, cap=n * g.N)
+        nf_b, ras_b = eb.member_rasters(nact, ids, masks, fm)
+        for b in range(B):
+            nf, ras, _ = oracles[b].run(n, raster_cap=n * g.N)
+            assert np.array_equal(nf_b[b], nf), (engine, n, b)
+            assert np.array_equal(ras_b[b], ras), (engine, n, b)
+    assert eb.stat("lookahead_steps") > 0
+    rngs = eb.rng()
+    for b in range(B):
+        for k in ("v", "gE", "gI", "refr", "trace"):
+            assert np.array_equal(eb.array(k, b).view(np.uint32), oracles[b].array(k).view(np.uint32)), (b, k)
+        assert np.array_equal(eb.array("spikeCount", b), oracles[b].spikeCount)
+        assert int(rngs[b]) == oracles[b].rng
