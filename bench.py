@@ -260,6 +260,47 @@ def bench_adjoint(args, synth, P, diff, hiprt, stream):
     return out
 
 
+def bench_adjoint_batch(args, synth, P, diff, hiprt):
+    """32 instances of the forward + adjoint workload: instance b has its own
+    type-gain perturbation (x exp(N(0, 0.05^2)) per type, seed 400+b), drive
+    stream seed 500+b and window U{500..2000} steps (seed 600+b); the
+    checkpoint interval comes from the memory / recompute trade-off; the
+    instances run concurrently and their type-gain gradients are summed."""
+    import numpy as np
+    g, p, sign, types, setup = adjoint_workload(synth, P)
+    B = 32
+    gains0 = synth.type_gains(0)
+    windows = [int(np.random.default_rng(600 + b).integers(500, 2001)) for b in range(B)]
+    K = diff.autotune_checkpoint(g.N, max(windows), B)
+    bt = diff.LIFDiffBatch(g.N, g.indptr, g.indices, g.counts.astype(np.float32), sign, {**p, "checkpoint": K}, B)
+    for b in range(B):
+        m = bt[b]
+        setup(m)
+        gb = (gains0.astype(np.float64) * np.exp(np.random.default_rng(400 + b).normal(0, 0.05, gains0.size)))
+        m.set_input("outScale", gb.astype(np.float32)[types])
+        m.set_tape_capacity(4096)
+    seeds = [500 + b for b in range(B)]
+    bt.run([100] * B, seeds)                 # warm-up: allocations and first launches
+    ev0 = sum(m.stat("edge_events") for m in bt.members)
+    hiprt.device_synchronize()
+    w0 = time.perf_counter()
+    res = bt.run(windows, seeds)
+    tg = bt.type_grads(types, 500)
+    wall = time.perf_counter() - w0
+    ev = sum(m.stat("edge_events") for m in bt.members) - ev0
+    out = {"value": ev / wall, "unit": "synaptic events/s (all instances, forward events over wall time of "
+                                      "forward + adjoint + gradient reduction)",
+           "instances": B, "checkpoint_interval": K, "member_steps": int(sum(windows)),
+           "wall_s": wall, "ms_per_member_step": wall * 1000.0 / sum(windows),
+           "losses_mean": float(np.mean([r[0] for r in res])),
+           "type_grad_norm": float(np.linalg.norm(tg)),
+           "non_finite_zeroed": int(sum(int(r[1]["clamped"].sum()) for r in res)),
+           "workload": "32 instances of the forward + adjoint workload with their own gain perturbations, "
+                       "drive seeds and windows of 500-2,000 steps, type-gain gradients reduced over instances"}
+    bt.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,6 +459,7 @@ def main():
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:
         out["fwd_adjoint"] = bench_adjoint(args, synth, P, diff, hiprt, stream)
+        out["fwd_adjoint_batch32"] = bench_adjoint_batch(args, synth, P, diff, hiprt)
     if not args.no_cpu:
         cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)
         out["cpu_baseline"] = cb

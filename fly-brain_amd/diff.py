@@ -198,3 +198,62 @@ class LIFDiffHip:
         v = ctypes.c_double()
         _l.check(self.lib.fb_diff_stat(self.h, name.encode(), ctypes.byref(v)))
         return float(v.value)
+
+
+def autotune_checkpoint(N: int, steps: int, members: int = 1, budget_bytes: float = 200e9) -> int:
+    """Checkpoint interval K for `members` concurrent instances of `steps`
+    steps: minimises members * [(steps/K + 1) * S_ckpt + K * S_arena], the
+    checkpoints (v, gE, gI, adapt, res, refr and the stream: 24 N + 4 bytes)
+    against one segment's arena (v, gE, gI, adapt, spike amplitude and a mode
+    byte: 21 N bytes per step), i.e. K = sqrt(steps * S_ckpt / S_arena).
+    Raises when even that K exceeds the budget."""
+    import math
+    s_ckpt = 24.0 * N + 4.0
+    s_arena = 21.0 * N
+    k = max(1, min(steps, int(round(math.sqrt(max(1, steps) * s_ckpt / s_arena)))))
+    need = members * ((steps / k + 1) * s_ckpt + k * s_arena)
+    if need > budget_bytes:
+        raise ValueError(f"checkpoint plan needs {need / 1e9:.1f} GB > budget {budget_bytes / 1e9:.1f} GB")
+    return k
+
+
+class LIFDiffBatch:
+    """B independent differentiable networks over one graph, each with its own
+    inputs (gains, drive, seed, window), run concurrently: one host thread and
+    one HIP stream per member, so the members' per-step kernels overlap on the
+    device.  ``type_grads`` reduces the members' type-gain gradients into one
+    shared vector."""
+
+    def __init__(self, N, indptr, indices, counts, sign, opts: dict | None, B: int, workers: int = 8):
+        self.members = [LIFDiffHip(N, indptr, indices, counts, sign, opts) for _ in range(int(B))]
+        self.workers = max(1, min(int(workers), len(self.members)))
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, b: int) -> LIFDiffHip:
+        return self.members[b]
+
+    def run(self, steps, seeds, edges: bool = False, truncate: int = 0):
+        """forward(steps[b], seeds[b]) then backward for every member,
+        concurrently; returns [(loss, grads)] in member order."""
+        import concurrent.futures as cf
+
+        def job(b):
+            m = self.members[b]
+            L = m.forward(int(steps[b]), seed=seeds[b])
+            return L, m.backward(truncate=truncate, edges=edges)
+
+        with cf.ThreadPoolExecutor(max_workers=self.workers) as ex:
+            return list(ex.map(job, range(len(self.members))))
+
+    def type_grads(self, type_of, n_types: int) -> np.ndarray:
+        """Sum over members of dL_b / d gain_T (each member's own gains)."""
+        out = np.zeros(int(n_types), np.float64)
+        for m in self.members:
+            out += m.type_grads(type_of, n_types)
+        return out
+
+    def destroy(self):
+        for m in self.members:
+            m.destroy()

@@ -228,3 +228,32 @@ def test_full_scale_type_gain_gradients_match_finite_differences():
         if not ok:
             fails.append((int(T), float(tg[T]), fd, rel))
     assert not fails, fails
+
+
+def test_concurrent_members_match_separate_runs():
+    """Members of LIFDiffBatch run concurrently (a host thread and a stream
+    each) and give exactly what each gives alone."""
+    pr = dc.Problem(over={"soft": False, "tRef": 2.2}).with_readout(R=20)
+    B = 4
+    seeds = [3, 4, 5, 6]
+    steps = [120, 150, 90, 150]
+    rng = np.random.default_rng(17)
+    scales = [(pr.inputs["outScale"] * np.exp(rng.normal(0, 0.05, pr.g.N))).astype(np.float32) for _ in range(B)]
+    types = rng.integers(0, 9, size=pr.g.N).astype(np.int32)
+    bt = diff.LIFDiffBatch(pr.g.N, pr.g.indptr, pr.g.indices, pr.counts, pr.sign, pr.p, B, workers=4)
+    for b in range(B):
+        pr.apply(bt[b], inputs={"outScale": scales[b]})
+    res = bt.run(steps, seeds, edges=True)
+    total = np.zeros(9)
+    for b in range(B):
+        one = gpu_of(pr, inputs={"outScale": scales[b]})
+        L = one.forward(steps[b], seed=seeds[b])
+        g = one.backward()
+        assert res[b][0] == L
+        assert np.array_equal(res[b][1]["logGain"], g["logGain"])
+        assert np.array_equal(res[b][1]["edge"], g["edge"])
+        for k in P.DIFF_PARAMS:
+            if k != "sizeAlpha":      # reads gInScale, summed with float atomics
+                assert res[b][1]["params"][k] == g["params"][k], (b, k)
+        total += one.type_grads(types, 9)
+    assert np.allclose(bt.type_grads(types, 9), total, rtol=1e-12, atol=0)
