@@ -265,7 +265,8 @@ def bench_adjoint_batch(args, synth, P, diff, hiprt):
     type-gain perturbation (x exp(N(0, 0.05^2)) per type, seed 400+b), drive
     stream seed 500+b and window U{500..2000} steps (seed 600+b); the
     checkpoint interval comes from the memory / recompute trade-off; the
-    instances run concurrently and their type-gain gradients are summed."""
+    instances run in lockstep (one launch per step kernel for all of them) and
+    their type-gain gradients are summed."""
     import numpy as np
     g, p, sign, types, setup = adjoint_workload(synth, P)
     B = 32
@@ -288,10 +289,12 @@ def bench_adjoint_batch(args, synth, P, diff, hiprt):
     tg = bt.type_grads(types, 500)
     wall = time.perf_counter() - w0
     ev = sum(m.stat("edge_events") for m in bt.members) - ev0
+    fwd_ms, bwd_ms = bt.stat("forward_ms"), bt.stat("backward_ms")
     out = {"value": ev / wall, "unit": "synaptic events/s (all instances, forward events over wall time of "
                                       "forward + adjoint + gradient reduction)",
            "instances": B, "checkpoint_interval": K, "member_steps": int(sum(windows)),
-           "wall_s": wall, "ms_per_member_step": wall * 1000.0 / sum(windows),
+           "wall_s": wall, "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+           "ms_per_member_step": wall * 1000.0 / sum(windows),
            "losses_mean": float(np.mean([r[0] for r in res])),
            "type_grad_norm": float(np.linalg.norm(tg)),
            "non_finite_zeroed": int(sum(int(r[1]["clamped"].sum()) for r in res)),

@@ -94,6 +94,7 @@ struct DiffDev {
     double *accW, *accDepU, *accInh, *blockAcc;
     double *rate, *dLds;               // [steps][R]
     const float *target;
+    float *ckpt;                       // checkpoints [n][6N + 1]
 };
 
 enum { DERR_COLL = 1, DERR_TAPE = 2, DERR_DRAWS = 4 };
@@ -152,7 +153,7 @@ __device__ __forceinline__ float clip_to(double x, double c) {
 // The arrivals of step t: the tape's spikes of step t - D.  resBase is the
 // tape index of the replay segment's first arrival.
 template <bool REPLAY>
-__global__ void __launch_bounds__(256) kd_deliver(DiffDev d, int32_t t, int32_t resFromT) {
+__device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t resFromT) {
     if (t < d.D) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(256) kd_deliver(DiffDev d, int32_t t, int32_t 
 // The drive pre-pass (src/lifdiff.js:290-297): one draw per non-refractory
 // driven neuron in set order; forced spikes go to the tape in that order.
 template <bool REPLAY>
-__global__ void __launch_bounds__(1024) kd_driven(DiffDev d, int32_t t) {
+__device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
     constexpr int T = 1024;
     __shared__ uint32_t leapS[LEAP_LDS_WORDS];
     __shared__ uint32_t ybase[64];
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(1024) kd_driven(DiffDev d, int32_t t) {
 
 // Integrate, decay, threshold (src/lifdiff.js:313-345; replay 422-457).
 template <bool REPLAY>
-__global__ void __launch_bounds__(DIFF_BLOCK) kd_membrane(DiffDev d, int32_t t, int32_t r) {
+__device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t r) {
     const int32_t N = d.N;
     const int32_t i = blockIdx.x * DIFF_BLOCK + threadIdx.x;
     bool emit = false;
@@ -393,8 +394,9 @@ __global__ void kd_finish(DiffDev d, int32_t steps) {
 }
 
 // Restores a checkpoint's state (replay start) and clears the step buffers.
-__global__ void kd_restore(DiffDev d, const float *ck) {
+__device__ __forceinline__ void dx_restore(const DiffDev &d, int32_t seg) {
     const int32_t N = d.N;
+    const float *ck = d.ckpt + (int64_t)seg * (6 * (int64_t)N + 1);
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         d.v[i] = ck[i]; d.gE[i] = ck[N + i]; d.gI[i] = ck[2 * N + i]; d.adapt[i] = ck[3 * N + i];
         d.res[i] = ck[4 * N + i]; d.refr[i] = ck[5 * N + i];
@@ -407,8 +409,9 @@ __global__ void kd_restore(DiffDev d, const float *ck) {
     }
 }
 
-__global__ void kd_save(DiffDev d, float *ck) {
+__device__ __forceinline__ void dx_save(const DiffDev &d, int32_t seg) {
     const int32_t N = d.N;
+    float *ck = d.ckpt + (int64_t)seg * (6 * (int64_t)N + 1);
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         ck[i] = d.v[i]; ck[N + i] = d.gE[i]; ck[2 * N + i] = d.gI[i]; ck[3 * N + i] = d.adapt[i];
         ck[4 * N + i] = d.res[i]; ck[5 * N + i] = d.refr[i];
@@ -459,7 +462,7 @@ __global__ void kd_readout(DiffDev d, int32_t T, double a, double *part) {
 
 // ------------------------------------------------------------------ backward
 
-__global__ void kd_bclear(DiffDev d, int32_t ringToo) {
+__device__ __forceinline__ void dx_bclear(const DiffDev &d, int32_t ringToo) {
     const int32_t N = d.N;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         d.lv[i] = 0.f; d.lgE[i] = 0.f; d.lgI[i] = 0.f; d.lad[i] = 0.f; d.lres[i] = 0.f;
@@ -480,7 +483,7 @@ __global__ void kd_bzero(DiffDev d, int32_t nblk) {
 }
 
 // Reversed decays and integrate-and-spike of step t (src/lifdiff.js:555-617).
-__global__ void __launch_bounds__(DIFF_BLOCK) kd_bneuron(DiffDev d, int32_t t, int32_t r) {
+__device__ __forceinline__ void dx_bneuron(const DiffDev &d, int32_t t, int32_t r) {
     __shared__ double red[5][DIFF_BLOCK / 64];
     const int32_t N = d.N;
     const int32_t i = blockIdx.x * DIFF_BLOCK + threadIdx.x;
@@ -559,7 +562,7 @@ __global__ void __launch_bounds__(DIFF_BLOCK) kd_bneuron(DiffDev d, int32_t t, i
 }
 
 // Reversed delivery of step t (src/lifdiff.js:635-666) plus the per-edge gradient.
-__global__ void __launch_bounds__(256) kd_barrival(DiffDev d, int32_t t, int32_t resFromT) {
+__device__ __forceinline__ void dx_barrival(const DiffDev &d, int32_t t, int32_t resFromT) {
     if (t < d.D) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -615,6 +618,36 @@ __global__ void __launch_bounds__(256) kd_barrival(DiffDev d, int32_t t, int32_t
         }
     }
 }
+
+// ------------------------------------------------------------------ kernels
+//
+// Each step function runs as a kernel over one network (kd_*, the network by
+// value) or over several networks in lockstep (kdb_*, blockIdx.y = network,
+// descriptions in device memory; network y takes part while t < tEnd[y]).
+
+#define FB_DIFF_KERNELS(NAME, BOUNDS, ARGS, CALL)                                                 \
+    __global__ void __launch_bounds__(BOUNDS) kd_##NAME(DiffDev d ARGS) { CALL; }                 \
+    __global__ void __launch_bounds__(BOUNDS) kdb_##NAME(const DiffDev *ds, const int32_t *tEnd,  \
+                                                         int32_t tAt ARGS) {                      \
+        if (tAt >= tEnd[blockIdx.y]) return;                                                     \
+        const DiffDev d = ds[blockIdx.y];                                                        \
+        CALL;                                                                                    \
+    }
+
+#define FB_COMMA ,
+FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf)))
+FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf)))
+FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t, (dx_driven<false>(d, t)))
+FB_DIFF_KERNELS(driven_r, 1024, FB_COMMA int32_t t, (dx_driven<true>(d, t)))
+FB_DIFF_KERNELS(membrane_f, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<false>(d, t, r)))
+FB_DIFF_KERNELS(membrane_r, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<true>(d, t, r)))
+FB_DIFF_KERNELS(restore, 256, FB_COMMA int32_t seg, (dx_restore(d, seg)))
+FB_DIFF_KERNELS(save, 256, FB_COMMA int32_t seg, (dx_save(d, seg)))
+FB_DIFF_KERNELS(bclear, 256, FB_COMMA int32_t ringToo, (dx_bclear(d, ringToo)))
+FB_DIFF_KERNELS(bneuron, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_bneuron(d, t, r)))
+FB_DIFF_KERNELS(barrival, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_barrival(d, t, rf)))
+#undef FB_COMMA
+#undef FB_DIFF_KERNELS
 
 // Final reduction into the nine scalar gradients (src/lifdiff.js:667-674), in
 // a fixed order: one workgroup.
@@ -1005,22 +1038,6 @@ static int grid_of(int64_t n, int block, int cap = 4096) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(cap, (n + block - 1) / block));
 }
 
-static fb_status launch_fwd_step(fb_diff *f, int32_t t, bool replay, int32_t r, int32_t resFromT) {
-    DiffDev &d = f->d;
-    hipStream_t st = f->stream;
-    if (replay) {
-        hipLaunchKernelGGL(kd_deliver<true>, dim3(256), dim3(256), 0, st, d, t, resFromT);
-        hipLaunchKernelGGL(kd_driven<true>, dim3(1), dim3(1024), 0, st, d, t);
-        hipLaunchKernelGGL(kd_membrane<true>, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, r);
-    } else {
-        hipLaunchKernelGGL(kd_deliver<false>, dim3(256), dim3(256), 0, st, d, t, 0);
-        hipLaunchKernelGGL(kd_driven<false>, dim3(1), dim3(1024), 0, st, d, t);
-        hipLaunchKernelGGL(kd_membrane<false>, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, 0);
-    }
-    FB_HIP(hipGetLastError());
-    return FB_OK;
-}
-
 static fb_status check_err(fb_diff *f, const char *who) {
     int32_t err = 0;
     FB_HIP(hipMemcpyAsync(&err, f->d.err, sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
@@ -1037,13 +1054,12 @@ extern "C" fb_status fb_diff_set_tape_capacity(fb_diff *f, int64_t perStep) {
     return FB_OK;
 }
 
-extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int32_t record, double *loss) {
-    FB_REQUIRE_D(f && steps >= 0, FB_ERR_ARG, "fb_diff_forward: bad arguments");
+// Buffers and initial state of a forward pass of `steps` steps, queued on st.
+static fb_status fwd_prepare(fb_diff *f, int32_t steps, double seed, int32_t record, hipStream_t st) {
     FB_REQUIRE_D(f->d.nDriven <= 1 << 20, FB_ERR_UNSUPPORTED, "fb_diff_forward: driven set too large");
     DiffDev &d = f->d;
     const int32_t N = f->N, R = d.R;
-    hipStream_t st = f->stream;
-    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipStreamSynchronize(f->stream));
     free_run(f);
     fb_status s;
     const int64_t cap = std::max<int64_t>(1024, f->perStepCap * (int64_t)std::max(1, steps));
@@ -1057,33 +1073,33 @@ extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int
     if (R) FB_HIP(hipMemsetAsync(d.readS, 0, sizeof(float) * (size_t)steps * R, st));
     f->nck = record ? (steps + d.K - 1) / d.K + 1 : 0;
     if (record && (s = dalloc(f->runAllocs, &f->ckpt, (size_t)f->nck * (6 * (size_t)N + 1)))) return s;
+    d.ckpt = f->ckpt;
     const uint32_t r0 = seed_hash(seed);
     FB_HIP(hipMemcpyAsync(d.rng, &r0, sizeof r0, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(kd_reset, dim3(grid_of(N, 256)), dim3(256), 0, st, d, (float)f->p.vRest);
-    hipEvent_t e0, e1;
-    FB_HIP(hipEventCreate(&e0));
-    FB_HIP(hipEventCreate(&e1));
-    FB_HIP(hipEventRecord(e0, st));
-    for (int32_t t = 0; t < steps; t++) {
-        if (record && t % d.K == 0)
-            hipLaunchKernelGGL(kd_save, dim3(grid_of(N, 256)), dim3(256), 0, st, d,
-                               f->ckpt + (size_t)(t / d.K) * (6 * (size_t)N + 1));
-        if ((s = launch_fwd_step(f, t, false, 0, 0))) return s;
-    }
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+// After the steps: the tape's end, the losses
+static fb_status fwd_finish_launch(fb_diff *f, int32_t steps, hipStream_t st) {
+    const DiffDev &d = f->d;
+    const int32_t R = d.R;
     hipLaunchKernelGGL(kd_finish, dim3(1), dim3(64), 0, st, d, steps);
     hipLaunchKernelGGL(kd_loss_flat, dim3(1), dim3(1024), 0, st, d, f->scal);
     if (R && steps) {
         const double a = std::exp(-f->p.dt / f->tauR);
         hipLaunchKernelGGL(kd_readout, dim3((R + 255) / 256), dim3(256), 0, st, d, steps, a, f->readPart);
     }
-    FB_HIP(hipEventRecord(e1, st));
     FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+// (after the stream is synchronised) the loss and the run's bookkeeping
+static fb_status fwd_collect(fb_diff *f, int32_t steps, int32_t record, double *loss) {
+    fb_status s;
     if ((s = check_err(f, "fb_diff_forward"))) return s;
-    float ms = 0.f;
-    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    f->fwdMs = ms;
+    const int32_t R = f->d.R;
     double L = 0.0;
     FB_HIP(hipMemcpy(&L, f->scal, sizeof(double), hipMemcpyDeviceToHost));
     if (R && steps) {
@@ -1100,14 +1116,42 @@ extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int
     return FB_OK;
 }
 
-extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *params, float *gLogGain, float *gEdge,
-                                      float *gInScale, int32_t *clamped) {
-    FB_REQUIRE_D(f && truncate >= 0, FB_ERR_ARG, "fb_diff_backward: bad arguments");
+extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int32_t record, double *loss) {
+    FB_REQUIRE_D(f && steps >= 0, FB_ERR_ARG, "fb_diff_forward: bad arguments");
+    DiffDev &d = f->d;
+    const int32_t N = f->N;
+    hipStream_t st = f->stream;
+    fb_status s;
+    if ((s = fwd_prepare(f, steps, seed, record, st))) return s;
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    const int gN = grid_of(N, 256);
+    for (int32_t t = 0; t < steps; t++) {
+        if (record && t % d.K == 0) hipLaunchKernelGGL(kd_save, dim3(gN), dim3(256), 0, st, d, t / d.K);
+        hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, d, t, 0);
+        hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, d, t);
+        hipLaunchKernelGGL(kd_membrane_f, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, 0);
+    }
+    FB_HIP(hipGetLastError());
+    if ((s = fwd_finish_launch(f, steps, st))) return s;
+    FB_HIP(hipEventRecord(e1, st));
+    FB_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->fwdMs = ms;
+    return fwd_collect(f, steps, record, loss);
+}
+
+// Arena of one checkpoint segment and the cleared adjoint state, queued on st.
+static fb_status bwd_prepare(fb_diff *f, hipStream_t st) {
     FB_REQUIRE_D(f->recorded, FB_ERR_STATE, "fb_diff_backward: no recorded forward pass (forward with record=1, "
                                             "after the last change of inputs)");
     DiffDev &d = f->d;
     const int32_t N = f->N, T = f->steps, K = d.K, D = d.D;
-    hipStream_t st = f->stream;
     fb_status s;
     if (!d.aV) {
         float *ar;
@@ -1125,41 +1169,27 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
         d.aV = ar; d.aGE = ar + (size_t)K * N; d.aGI = ar + 2 * (size_t)K * N; d.aAd = ar + 3 * (size_t)K * N;
         d.aSp = ar + 4 * (size_t)K * N; d.resAt = ar + 5 * (size_t)K * N; d.aMode = md;
     }
-    hipEvent_t e0, e1;
-    FB_HIP(hipEventCreate(&e0));
-    FB_HIP(hipEventCreate(&e1));
-    FB_HIP(hipEventRecord(e0, st));
-    const int gN = grid_of(N, 256);
-    hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
+    hipLaunchKernelGGL(kd_bclear, dim3(grid_of(N, 256)), dim3(256), 0, st, d, 1);
     hipLaunchKernelGGL(kd_bzero, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d, f->nblk);
     FB_HIP(hipMemsetAsync(f->clampDev, 0, 4 * sizeof(int32_t), st));
-    const int32_t nseg = (T + K - 1) / K;
-    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
-        const int32_t from = sgi * K, to = std::min(T, from + K);
-        const int32_t resFromT = std::max(from, D) - D;
-        hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, st, d,
-                           (const float *)(f->ckpt + (size_t)sgi * (6 * (size_t)N + 1)));
-        for (int32_t t = from; t < to; t++)
-            if ((s = launch_fwd_step(f, t, true, t - from, resFromT))) return s;
-        for (int32_t t = to - 1; t >= from; t--) {
-            if (truncate > 0 && (t + 1) % truncate == 0)
-                hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
-            hipLaunchKernelGGL(kd_bneuron, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
-            hipLaunchKernelGGL(kd_barrival, dim3(256), dim3(256), 0, st, d, t, resFromT);
-        }
-        FB_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(kd_breduce, dim3(1), dim3(1024), 0, st, d, f->nblk, f->scal + 1);
-    hipLaunchKernelGGL(kd_sanitize, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d,
-                       f->clampDev);
-    FB_HIP(hipEventRecord(e1, st));
     FB_HIP(hipGetLastError());
-    if ((s = check_err(f, "fb_diff_backward"))) return s;
-    float ms = 0.f;
-    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    f->bwdMs = ms;
+    return FB_OK;
+}
+
+static fb_status bwd_finish_launch(fb_diff *f, hipStream_t st) {
+    const DiffDev &d = f->d;
+    hipLaunchKernelGGL(kd_breduce, dim3(1), dim3(1024), 0, st, d, f->nblk, f->scal + 1);
+    hipLaunchKernelGGL(kd_sanitize, dim3(grid_of(std::max<int64_t>(f->E, f->N), 256)), dim3(256), 0, st, d,
+                       f->clampDev);
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+// The results of the last backward pass (any output may be NULL).
+extern "C" fb_status fb_diff_grads(fb_diff *f, double *params, float *gLogGain, float *gEdge, float *gInScale,
+                                   int32_t *clamped) {
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_grads: null handle");
+    FB_HIP(hipStreamSynchronize(f->stream));
     double g[9];
     int32_t cl[4];
     FB_HIP(hipMemcpy(g, f->scal + 1, sizeof g, hipMemcpyDeviceToHost));
@@ -1169,10 +1199,226 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
         if (!std::isfinite(g[q])) { g[q] = 0.0; cl[0]++; }
     if (params) std::memcpy(params, g, sizeof g);
     if (clamped) std::memcpy(clamped, cl, 3 * sizeof(int32_t));
-    if (gLogGain) FB_HIP(hipMemcpy(gLogGain, d.gLogGain, sizeof(float) * N, hipMemcpyDeviceToHost));
-    if (gInScale) FB_HIP(hipMemcpy(gInScale, d.gInScale, sizeof(float) * N, hipMemcpyDeviceToHost));
-    if (gEdge && f->E) FB_HIP(hipMemcpy(gEdge, d.gEdge, sizeof(float) * f->E, hipMemcpyDeviceToHost));
+    if (gLogGain) FB_HIP(hipMemcpy(gLogGain, f->d.gLogGain, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    if (gInScale) FB_HIP(hipMemcpy(gInScale, f->d.gInScale, sizeof(float) * f->N, hipMemcpyDeviceToHost));
+    if (gEdge && f->E) FB_HIP(hipMemcpy(gEdge, f->d.gEdge, sizeof(float) * f->E, hipMemcpyDeviceToHost));
     return FB_OK;
+}
+
+extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *params, float *gLogGain, float *gEdge,
+                                      float *gInScale, int32_t *clamped) {
+    FB_REQUIRE_D(f && truncate >= 0, FB_ERR_ARG, "fb_diff_backward: bad arguments");
+    DiffDev &d = f->d;
+    const int32_t N = f->N, T = f->steps, K = d.K, D = d.D;
+    hipStream_t st = f->stream;
+    fb_status s;
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    if ((s = bwd_prepare(f, st))) return s;
+    FB_HIP(hipEventRecord(e0, st));
+    const int gN = grid_of(N, 256);
+    const int32_t nseg = (T + K - 1) / K;
+    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int32_t from = sgi * K, to = std::min(T, from + K);
+        const int32_t resFromT = std::max(from, D) - D;
+        hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, st, d, sgi);
+        for (int32_t t = from; t < to; t++) {
+            hipLaunchKernelGGL(kd_deliver_r, dim3(256), dim3(256), 0, st, d, t, resFromT);
+            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(1024), 0, st, d, t);
+            hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
+        }
+        for (int32_t t = to - 1; t >= from; t--) {
+            if (truncate > 0 && (t + 1) % truncate == 0)
+                hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
+            hipLaunchKernelGGL(kd_bneuron, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
+            hipLaunchKernelGGL(kd_barrival, dim3(256), dim3(256), 0, st, d, t, resFromT);
+        }
+        FB_HIP(hipGetLastError());
+    }
+    if ((s = bwd_finish_launch(f, st))) return s;
+    FB_HIP(hipEventRecord(e1, st));
+    if ((s = check_err(f, "fb_diff_backward"))) return s;
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->bwdMs = ms;
+    return fb_diff_grads(f, params, gLogGain, gEdge, gInScale, clamped);
+}
+
+// ------------------------------------------------------------------ several networks in lockstep
+
+struct fb_diff_batch {
+    std::vector<fb_diff *> m;
+    DiffDev *ds = nullptr;              // device copies of the members' descriptions
+    int32_t *tEnd = nullptr;            // per member: steps below this take part
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> steps;
+    double fwdMs = 0.0, bwdMs = 0.0;
+    int32_t maxBlk = 0;
+};
+
+extern "C" fb_status fb_diff_batch_create(fb_diff *const *members, int32_t B, fb_diff_batch **out) {
+    FB_REQUIRE_D(members && out && B >= 1, FB_ERR_ARG, "fb_diff_batch_create: bad arguments");
+    for (int32_t b = 0; b < B; b++) {
+        FB_REQUIRE_D(members[b], FB_ERR_ARG, "fb_diff_batch_create: null member %d", b);
+        FB_REQUIRE_D(members[b]->d.S == members[0]->d.S && members[b]->d.K == members[0]->d.K, FB_ERR_ARG,
+                     "fb_diff_batch_create: members differ in delay length or checkpoint interval");
+        for (int32_t c = 0; c < b; c++)
+            FB_REQUIRE_D(members[c] != members[b], FB_ERR_ARG, "fb_diff_batch_create: member %d listed twice", b);
+    }
+    fb_diff_batch *bt = new fb_diff_batch();
+    bt->m.assign(members, members + B);
+    FB_HIP(hipMalloc(&bt->ds, sizeof(DiffDev) * B));
+    FB_HIP(hipMalloc(&bt->tEnd, sizeof(int32_t) * B));
+    FB_HIP(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+    for (fb_diff *f : bt->m) bt->maxBlk = std::max(bt->maxBlk, f->nblk);
+    *out = bt;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_batch_destroy(fb_diff_batch *bt) {
+    FB_REQUIRE_D(bt, FB_ERR_ARG, "fb_diff_batch_destroy: null batch");
+    (void)hipStreamSynchronize(bt->stream);
+    (void)hipFree(bt->ds);
+    (void)hipFree(bt->tEnd);
+    (void)hipStreamDestroy(bt->stream);
+    delete bt;
+    return FB_OK;
+}
+
+static fb_status batch_upload(fb_diff_batch *bt, const std::vector<int32_t> &tEnd) {
+    std::vector<DiffDev> h;
+    for (fb_diff *f : bt->m) h.push_back(f->d);
+    FB_HIP(hipMemcpyAsync(bt->ds, h.data(), sizeof(DiffDev) * h.size(), hipMemcpyHostToDevice, bt->stream));
+    return FB_OK;
+}
+
+static fb_status batch_ends(fb_diff_batch *bt, const std::vector<int32_t> &tEnd) {
+    FB_HIP(hipMemcpyAsync(bt->tEnd, tEnd.data(), sizeof(int32_t) * tEnd.size(), hipMemcpyHostToDevice, bt->stream));
+    FB_HIP(hipStreamSynchronize(bt->stream));   // tEnd is host memory owned by the caller's frame
+    return FB_OK;
+}
+
+// Forward passes of every member in lockstep: member b runs steps[b] steps
+// with seeds[b]; losses[b] receives its loss (may be NULL).
+extern "C" fb_status fb_diff_batch_forward(fb_diff_batch *bt, const int32_t *steps, const double *seeds,
+                                           int32_t record, double *losses) {
+    FB_REQUIRE_D(bt && steps && seeds, FB_ERR_ARG, "fb_diff_batch_forward: bad arguments");
+    const int32_t B = (int32_t)bt->m.size();
+    hipStream_t st = bt->stream;
+    fb_status s;
+    int32_t Tmax = 0, Nmax = 0;
+    for (int32_t b = 0; b < B; b++) {
+        FB_REQUIRE_D(steps[b] >= 0, FB_ERR_ARG, "fb_diff_batch_forward: steps[%d] < 0", b);
+        if ((s = fwd_prepare(bt->m[b], steps[b], seeds[b], record, st))) return s;
+        Tmax = std::max(Tmax, steps[b]);
+        Nmax = std::max(Nmax, bt->m[b]->N);
+    }
+    bt->steps.assign(steps, steps + B);
+    if ((s = batch_upload(bt, bt->steps)) || (s = batch_ends(bt, bt->steps))) return s;
+    const int32_t K = bt->m[0]->d.K;
+    const int gN = grid_of(Nmax, 256);
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    for (int32_t t = 0; t < Tmax; t++) {
+        if (record && t % K == 0) hipLaunchKernelGGL(kdb_save, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t / K);
+        hipLaunchKernelGGL(kdb_deliver_f, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, 0);
+        hipLaunchKernelGGL(kdb_driven_f, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
+        hipLaunchKernelGGL(kdb_membrane_f, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t, 0);
+    }
+    FB_HIP(hipGetLastError());
+    for (int32_t b = 0; b < B; b++)
+        if ((s = fwd_finish_launch(bt->m[b], steps[b], st))) return s;
+    FB_HIP(hipEventRecord(e1, st));
+    FB_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    bt->fwdMs = ms;
+    for (int32_t b = 0; b < B; b++)
+        if ((s = fwd_collect(bt->m[b], steps[b], record, losses ? losses + b : nullptr))) return s;
+    return FB_OK;
+}
+
+// Backward passes of every member in lockstep; each member's gradients are
+// then read with fb_diff_grads / fb_diff_type_grads.
+extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate) {
+    FB_REQUIRE_D(bt && truncate >= 0, FB_ERR_ARG, "fb_diff_batch_backward: bad arguments");
+    const int32_t B = (int32_t)bt->m.size();
+    hipStream_t st = bt->stream;
+    fb_status s;
+    int32_t Tmax = 0, Nmax = 0;
+    for (int32_t b = 0; b < B; b++) {
+        FB_REQUIRE_D(bt->m[b]->steps == bt->steps[b], FB_ERR_STATE,
+                     "fb_diff_batch_backward: member %d was not last run by this batch", b);
+        if ((s = bwd_prepare(bt->m[b], st))) return s;
+        Tmax = std::max(Tmax, bt->steps[b]);
+        Nmax = std::max(Nmax, bt->m[b]->N);
+    }
+    std::vector<int32_t> none((size_t)B, 0);
+    if ((s = batch_upload(bt, none))) return s;
+    const int32_t K = bt->m[0]->d.K, D = bt->m[0]->d.D;
+    const int gN = grid_of(Nmax, 256);
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    const int32_t nseg = (Tmax + K - 1) / K;
+    std::vector<int32_t> ends((size_t)B);
+    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int32_t from = sgi * K;
+        int32_t to = from;
+        // member b takes part in this segment up to its own end
+        for (int32_t b = 0; b < B; b++) {
+            ends[b] = std::min(bt->steps[b], from + K);
+            if (ends[b] < from) ends[b] = from;
+            to = std::max(to, ends[b]);
+        }
+        if ((s = batch_ends(bt, ends))) return s;
+        const int32_t resFromT = std::max(from, D) - D;
+        hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, from, sgi);
+        for (int32_t t = from; t < to; t++) {
+            hipLaunchKernelGGL(kdb_deliver_r, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, resFromT);
+            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
+            hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t,
+                               t - from);
+        }
+        for (int32_t t = to - 1; t >= from; t--) {
+            if (truncate > 0 && (t + 1) % truncate == 0)
+                hipLaunchKernelGGL(kdb_bclear, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, 1);
+            hipLaunchKernelGGL(kdb_bneuron, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t,
+                               t - from);
+            hipLaunchKernelGGL(kdb_barrival, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, resFromT);
+        }
+        FB_HIP(hipGetLastError());
+    }
+    for (int32_t b = 0; b < B; b++)
+        if ((s = bwd_finish_launch(bt->m[b], st))) return s;
+    FB_HIP(hipEventRecord(e1, st));
+    FB_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    bt->bwdMs = ms;
+    for (int32_t b = 0; b < B; b++) {
+        bt->m[b]->bwdMs = 0.0;
+        if ((s = check_err(bt->m[b], "fb_diff_batch_backward"))) return s;
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_batch_stat(fb_diff_batch *bt, const char *name, double *value) {
+    FB_REQUIRE_D(bt && name && value, FB_ERR_ARG, "fb_diff_batch_stat: null argument");
+    if (!std::strcmp(name, "forward_ms")) { *value = bt->fwdMs; return FB_OK; }
+    if (!std::strcmp(name, "backward_ms")) { *value = bt->bwdMs; return FB_OK; }
+    set_error("fb_diff_batch_stat: unknown statistic '%s'", name);
+    return FB_ERR_ARG;
 }
 
 // dL / d gain_T for type gains (outScale[i] = gain of i's type): the type sum

@@ -140,13 +140,19 @@ class LIFDiffHip:
     def backward(self, dLdSpike=None, truncate: int = 0, lossFrom: int | None = None, edges: bool = True) -> dict:
         if dLdSpike is not None or lossFrom is not None:
             self.set_loss(dLdSpike, lossFrom or 0)
+        return self._grads(lambda *o: self.lib.fb_diff_backward(self.h, int(truncate), *o), edges, truncate)
+
+    def grads(self, edges: bool = True) -> dict:
+        """The results of the last backward pass (this network's or its batch's)."""
+        return self._grads(lambda *o: self.lib.fb_diff_grads(self.h, *o), edges, 0)
+
+    def _grads(self, call, edges, truncate):
         g = np.zeros(9, np.float64)
         lg = np.zeros(self.N, np.float32)
         ge = np.zeros(max(1, self.E), np.float32) if edges else None
         gi = np.zeros(self.N, np.float32)
         cl = np.zeros(3, np.int32)
-        _l.check(self.lib.fb_diff_backward(self.h, int(truncate), _l.ptr(g), _l.ptr(lg),
-                                           _l.ptr(ge) if ge is not None else None, _l.ptr(gi), _l.ptr(cl)))
+        _l.check(call(_l.ptr(g), _l.ptr(lg), _l.ptr(ge) if ge is not None else None, _l.ptr(gi), _l.ptr(cl)))
         return {"params": dict(zip(DIFF_PARAMS, g.tolist())), "logGain": lg,
                 "edge": ge[: self.E] if ge is not None else None, "inScale": gi, "clamped": cl,
                 "truncate": int(truncate)}
@@ -218,15 +224,19 @@ def autotune_checkpoint(N: int, steps: int, members: int = 1, budget_bytes: floa
 
 
 class LIFDiffBatch:
-    """B independent differentiable networks over one graph, each with its own
-    inputs (gains, drive, seed, window), run concurrently: one host thread and
-    one HIP stream per member, so the members' per-step kernels overlap on the
-    device.  ``type_grads`` reduces the members' type-gain gradients into one
-    shared vector."""
+    """B networks over one graph, each with its own inputs (gains, drive,
+    seed, window), stepped in lockstep: every step's kernels run once for all
+    members (the member is the launch's second grid dimension).
+    ``type_grads`` reduces the members' type-gain gradients into one shared
+    vector."""
 
-    def __init__(self, N, indptr, indices, counts, sign, opts: dict | None, B: int, workers: int = 8):
+    def __init__(self, N, indptr, indices, counts, sign, opts: dict | None, B: int):
+        self.lib = _l.load()
         self.members = [LIFDiffHip(N, indptr, indices, counts, sign, opts) for _ in range(int(B))]
-        self.workers = max(1, min(int(workers), len(self.members)))
+        hs = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _l.check(self.lib.fb_diff_batch_create(hs, len(self.members), ctypes.byref(h)))
+        self.h = h
 
     def __len__(self):
         return len(self.members)
@@ -234,18 +244,23 @@ class LIFDiffBatch:
     def __getitem__(self, b: int) -> LIFDiffHip:
         return self.members[b]
 
+    def forward(self, steps, seeds, record: bool = True) -> np.ndarray:
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        sd = np.ascontiguousarray(seeds, dtype=np.float64)
+        L = np.zeros(len(self.members), np.float64)
+        _l.check(self.lib.fb_diff_batch_forward(self.h, _l.ptr(st), _l.ptr(sd), 1 if record else 0, _l.ptr(L)))
+        for m, n in zip(self.members, st.tolist()):
+            m.steps = int(n)
+        return L
+
+    def backward(self, truncate: int = 0, edges: bool = False) -> list:
+        _l.check(self.lib.fb_diff_batch_backward(self.h, int(truncate)))
+        return [m.grads(edges=edges) for m in self.members]
+
     def run(self, steps, seeds, edges: bool = False, truncate: int = 0):
-        """forward(steps[b], seeds[b]) then backward for every member,
-        concurrently; returns [(loss, grads)] in member order."""
-        import concurrent.futures as cf
-
-        def job(b):
-            m = self.members[b]
-            L = m.forward(int(steps[b]), seed=seeds[b])
-            return L, m.backward(truncate=truncate, edges=edges)
-
-        with cf.ThreadPoolExecutor(max_workers=self.workers) as ex:
-            return list(ex.map(job, range(len(self.members))))
+        """forward then backward for every member; [(loss, grads)] in member order."""
+        L = self.forward(steps, seeds)
+        return list(zip(L.tolist(), self.backward(truncate=truncate, edges=edges)))
 
     def type_grads(self, type_of, n_types: int) -> np.ndarray:
         """Sum over members of dL_b / d gain_T (each member's own gains)."""
@@ -254,6 +269,20 @@ class LIFDiffBatch:
             out += m.type_grads(type_of, n_types)
         return out
 
+    def stat(self, name: str) -> float:
+        v = ctypes.c_double()
+        _l.check(self.lib.fb_diff_batch_stat(self.h, name.encode(), ctypes.byref(v)))
+        return float(v.value)
+
     def destroy(self):
+        if getattr(self, "h", None):
+            _l.check(self.lib.fb_diff_batch_destroy(self.h))
+            self.h = None
         for m in self.members:
             m.destroy()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

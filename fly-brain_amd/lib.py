@@ -97,6 +97,12 @@ _SIGS = {
     "fb_diff_rng": (I32, [P, P]),
     "fb_diff_set_stream": (I32, [P, P]),
     "fb_diff_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
+    "fb_diff_grads": (I32, [P, P, P, P, P, P]),
+    "fb_diff_batch_create": (I32, [P, I32, PP]),
+    "fb_diff_batch_destroy": (I32, [P]),
+    "fb_diff_batch_forward": (I32, [P, P, P, I32, P]),
+    "fb_diff_batch_backward": (I32, [P, I32]),
+    "fb_diff_batch_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
     "fb_synchronize": (I32, [P]),
 }
 

@@ -251,6 +251,10 @@ fb_status fb_diff_forward(fb_diff *d, int32_t steps, double seed, int32_t record
  * params, logGain and edges.  Any output may be NULL. */
 fb_status fb_diff_backward(fb_diff *d, int32_t truncate, double *params, float *gLogGain, float *gEdge,
                            float *gInScale, int32_t *clamped);
+/* the results of the last backward pass (fb_diff_backward or a batch's),
+ * outputs as fb_diff_backward's; any may be NULL */
+fb_status fb_diff_grads(fb_diff *d, double *params, float *gLogGain, float *gEdge, float *gInScale,
+                        int32_t *clamped);
 /* dL/d gain_T for outScale[i] = gain of i's type: sum over the type of
  * gLogGain[i] / outScale[i] (the type gains of src/brainmodel.js:63-100) */
 fb_status fb_diff_type_grads(fb_diff *d, const int32_t *typeOf, int32_t nTypes, double *gType);
@@ -263,6 +267,25 @@ fb_status fb_diff_rng(fb_diff *d, uint32_t *rng);
 fb_status fb_diff_set_stream(fb_diff *d, void *stream);
 /* "forward_ms", "backward_ms", "edge_events", "arrivals", "steps", "tape_entries" */
 fb_status fb_diff_stat(fb_diff *d, const char *name, double *value);
+
+/* Several fb_diff networks (same delay length and checkpoint interval, each
+ * with its own inputs) stepped in lockstep: every step's kernels run once for
+ * all members (the member is the launch's second grid dimension), member b
+ * for its own steps[b].  The batched forward + adjoint of many instances over
+ * one connectome (one network per worker in the reference,
+ * src/brainsetup.js:8, scripts/grad_fit.mjs).  No counterpart in the
+ * reference's API. */
+typedef struct fb_diff_batch fb_diff_batch;
+fb_status fb_diff_batch_create(fb_diff *const *members, int32_t B, fb_diff_batch **out);
+fb_status fb_diff_batch_destroy(fb_diff_batch *b);
+/* forward passes: member b runs steps[b] steps from reset with seeds[b] */
+fb_status fb_diff_batch_forward(fb_diff_batch *b, const int32_t *steps, const double *seeds, int32_t record,
+                                double *losses);
+/* backward passes of the last batch forward; read each member's gradients
+ * with fb_diff_grads / fb_diff_type_grads */
+fb_status fb_diff_batch_backward(fb_diff_batch *b, int32_t truncate);
+/* "forward_ms", "backward_ms" of the last batch passes */
+fb_status fb_diff_batch_stat(fb_diff_batch *b, const char *name, double *value);
 
 #ifdef __cplusplus
 }
