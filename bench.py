@@ -371,9 +371,12 @@ def main():
     c1 = brain.counters()
     kms = brain.stat("persistent_kernel_ms") - k0
     phases = {f"{blk}_{ph}": brain.stat(f"phase_us_{blk}_{ph}")
-              for blk in ("drv", "reg") for ph in ("e1", "w1", "e2", "w2", "e3", "w3")}
+              for blk in ("drv", "reg", "spare")
+              for ph in ("gather", "sim", "push", "lists", "barrier", "prologue", "share", "barrier2")}
     phases.update({f"drvstep_{ph}": brain.stat(f"phase_us_drvstep_{ph}")
                    for ph in ("inputs", "scan", "draws", "decide", "membrane", "sync")})
+    phases.update({f"share_{ph}": brain.stat(f"phase_us_share_{ph}")
+                   for ph in ("load", "edges", "publish", "max")})
     overflow_flags = int(brain.stat("persistent_overflow_flags"))
     launches = int(brain.stat("persistent_launches") - l0)
     fallback = int(brain.stat("fallback_steps") - f0)

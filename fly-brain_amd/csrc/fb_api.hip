@@ -186,7 +186,7 @@ struct fb_brain {
     int64_t steps = 0;
     int64_t fallbackSteps = 0, persistentLaunches = 0;
     double persistentMs = 0.0;   // summed HIP-event duration of the lookahead kernel launches
-    double phaseUs[18] = {0};    // lookahead kernel phases (driven block, a regular block, driven-step parts)
+    double phaseUs[PERSIST_PROF] = {0};   // lookahead kernel clocks
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
@@ -748,16 +748,22 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
     else if (n == "persistent_kernel_ms") *value = b->persistentMs;
     else if (n == "persistent_overflow_flags") *value = (double)persistent_overflow_flags(b->d);
     else if (n.rfind("phase_us_", 0) == 0) {
-        // phase_us_<drv|reg>_<e1|w1|e2|w2|e3|w3>: time in each phase of the
-        // lookahead kernel (e = work, w = waiting at the grid barrier after it)
-        static const char *ph[6] = {"e1", "w1", "e2", "w2", "e3", "w3"};
+        // phase_us_<drv|reg|spare>_<phase>: time the driven block, the first
+        // regular block and the first spare block spent in each phase of the
+        // lookahead kernel
+        static const char *ph[NPH] = {"gather", "sim", "push", "lists", "barrier", "prologue", "share", "barrier2"};
+        static const char *blkName[3] = {"drv_", "reg_", "spare_"};
         int idx = -1;
-        for (int blk = 0; blk < 2; blk++)
-            for (int k = 0; k < 6; k++)
-                if (n == std::string("phase_us_") + (blk ? "reg_" : "drv_") + ph[k]) idx = blk * 6 + k;
+        for (int blk = 0; blk < 3; blk++)
+            for (int k = 0; k < NPH; k++)
+                if (n == std::string("phase_us_") + blkName[blk] + ph[k]) idx = blk * NPH + k;
         static const char *sub[6] = {"inputs", "scan", "draws", "decide", "membrane", "sync"};
         for (int k = 0; k < 6; k++)
-            if (n == std::string("phase_us_drvstep_") + sub[k]) idx = 12 + k;
+            if (n == std::string("phase_us_drvstep_") + sub[k]) idx = PROF_SUB + k;
+        // the shared push of the driven rows: mean per block, and the slowest block
+        static const char *sh[4] = {"load", "edges", "publish", "max"};
+        for (int k = 0; k < 4; k++)
+            if (n == std::string("phase_us_share_") + sh[k]) idx = PROF_SHARE + k;
         if (idx < 0) { set_error("fb_brain_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
         *value = b->phaseUs[idx];
     }

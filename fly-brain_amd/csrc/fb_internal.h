@@ -136,6 +136,15 @@ constexpr int LEAP_MATS = 96;
 constexpr int LEAP_WORDS = 128;           // 8 nibbles x 16 values
 constexpr int LEAP_ROUND = 2048;
 constexpr int LEAP_STRIDE = 129;
+
+// Lookahead-kernel clocks reported per launch: the NPH phases of the driven
+// block, of the first regular block and of the first spare block; the parts of
+// the driven step (6); the shared push of the driven rows (mean load / edges /
+// publish over blocks, then the slowest block's total).
+constexpr int NPH = 8;   // gather, sim, push, lists, barrier, prologue, share, barrier after the share
+constexpr int PROF_SUB = 3 * NPH;
+constexpr int PROF_SHARE = PROF_SUB + 6;
+constexpr int PERSIST_PROF = PROF_SHARE + 4;
 constexpr int LEAP_LDS_WORDS = LEAP_MATS * LEAP_STRIDE;
 
 __device__ __forceinline__ uint32_t matvec(const uint32_t *tab, uint32_t x) {

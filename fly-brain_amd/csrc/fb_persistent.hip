@@ -73,6 +73,10 @@ constexpr int CAP_DRV = 1024;       // sub-bucket slots from the driven block to
 constexpr int CAP_REG = 512;        // sub-bucket slots from a regular block to one destination
 constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
 constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
+#ifndef FB_SHARE_BLOCKS
+#define FB_SHARE_BLOCKS 64
+#endif
+constexpr int SHARE_BLOCKS = FB_SHARE_BLOCKS;   // blocks that push the driven block's rows
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -105,6 +109,7 @@ struct PersistArgs {
     int32_t *ctl;        // [0] steps done, [1] overflow flags
     uint64_t *prof;      // phase clocks (see k_persist)
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
+    int32_t *dlist;      // the driven block's row list of the epoch (publish_driven)
     int32_t flags;       // bit 0: time the driven block's step parts
 };
 
@@ -127,8 +132,9 @@ struct Shared {
     uint32_t rngNext[2];
     int32_t nev, nspk;
     unsigned long long ev, syn;
-    uint64_t ph[7];              // phase clocks of thread 0
+    uint64_t ph[NPH + 1];        // phase clocks of thread 0, then the last stamp
     uint64_t sub[7];             // driven-step clocks of thread 0
+    uint64_t share[4];           // shared-push clocks of thread 0: load, edges, publish, last stamp
 };
 
 // ------------------------------------------------------------------ helpers
@@ -228,54 +234,70 @@ __device__ __forceinline__ int64_t sb_base(const PersistArgs &a, int32_t par, in
 }
 __device__ __forceinline__ int32_t sb_cap(int32_t src) { return src == 0 ? CAP_DRV : CAP_REG; }
 
-// Writes the edge events of `ns` arriving rows into this block's sub-buckets
-// for parity `par`.  Row r is (s.spkI[r], step and key in s.spkK[r], sign x res
-// in s.spkR[r]).  Edges of all rows are spread over the block's threads (a
-// scan of the row lengths maps an edge to its row); slots come from LDS
-// counters, and four edges per thread are in flight at once.
-__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns) {
+// Row lengths of the `ns` rows of the LDS push list -> the flattened edge
+// list: s.rowOff[r] (first edge of row r, s.rowOff[ns] = total) and s.rowA[r]
+// (row start in the CSR).  Row r is (s.spkI[r], step and key in s.spkK[r],
+// sign x res in s.spkR[r]); a row whose sign x res is 0 adds nothing.  Adds
+// the rows' edge and synapse counts to s.ev / s.syn.  Returns the total.
+__device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     // row lengths -> prefix (RPT consecutive rows per thread)
     int32_t len[RPT];
     uint32_t ra[RPT];
     int32_t mySum = 0;
+    unsigned long long syn = 0;
 #pragma unroll
     for (int v = 0; v < RPT; v++) {
         const int32_t r = RPT * tid + v;
         len[v] = 0;
         ra[v] = 0;
         if (r < ns && s.spkR[r] != 0.f) {
-            ra[v] = d.g.indptr[s.spkI[r]];
-            len[v] = (int32_t)(d.g.indptr[s.spkI[r] + 1] - ra[v]);
+            const int32_t i = s.spkI[r];
+            ra[v] = d.g.indptr[i];
+            len[v] = (int32_t)(d.g.indptr[i + 1] - ra[v]);
+            syn += d.g.rowSyn[i];   // 0 for an empty row
         }
         mySum += len[v];
     }
     int32_t total;
     int32_t off = wg_exclusive_scan<PT>(mySum, s.scan, &total);
-    unsigned long long syn = 0;
 #pragma unroll
     for (int v = 0; v < RPT; v++) {
         const int32_t r = RPT * tid + v;
         if (r < ns) { s.rowOff[r] = off; s.rowA[r] = ra[v]; }
-        if (len[v]) syn += d.g.rowSyn[s.spkI[r]];
         off += len[v];
     }
     if (tid == 0) s.rowOff[ns] = total;
     if (mySum) { atomicAdd(&s.ev, (unsigned long long)mySum); atomicAdd(&s.syn, syn); }
     __syncthreads();
+    return total;
+}
+
+// Writes the edge events g0 <= g < g1 of the flattened edge list (row_scan)
+// into source `src`'s sub-buckets for parity `par`, four edges per thread in
+// flight; a scan of the row starts maps an edge to its row.  Slots come from
+// this block's LDS counters, or, with SHARED (many blocks writing the driven
+// block's sub-buckets at once), from the global counts themselves; then the
+// LDS counters only tally what this block added.
+template <bool SHARED>
+__device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns, int32_t g0,
+                                  int32_t g1) {
+    const DevBrain &d = a.d;
+    const int tid = threadIdx.x;
     const int32_t cap = sb_cap(src);
+    int32_t *gcnt = a.sbCnt + (int64_t)par * a.S * a.S + src;   // [dst * S]
     constexpr int U = 4;
-    for (int32_t g0 = tid; g0 < total; g0 += PT * U) {
+    for (int32_t gb = g0 + tid; gb < g1; gb += PT * U) {
         uint2 ed[U];
         uint32_t own[U];
         int32_t row[U];
         uint32_t j[U];
 #pragma unroll
         for (int v = 0; v < U; v++) {
-            const int32_t g = g0 + v * PT;
+            const int32_t g = gb + v * PT;
             row[v] = -1;
-            if (g < total) {
+            if (g < g1) {
                 int32_t lo = 0, hi = ns;   // last row with rowOff <= g
                 while (hi - lo > 1) {
                     const int32_t mid = (lo + hi) >> 1;
@@ -295,14 +317,24 @@ __device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, i
             const float sv = s.spkR[r];
             const int32_t dst = (int32_t)(own[v] >> 12);
             const uint32_t ql = own[v] & 4095u;
-            const int32_t pos = atomicAdd(&s.cnt[dst], 1);
+            int32_t pos = atomicAdd(&s.cnt[dst], 1);
+            if (SHARED) pos = atomicAdd(&gcnt[(int64_t)dst * a.S], 1);
             if (pos < cap) {
                 const int64_t at = sb_base(a, par, src, dst) + pos;
                 a.sbKey[at] = ev_key(kk >> 24, sv > 0 ? 0u : 1u, ql, kk & 0xFFFFFFu, j[v] - s.rowA[r]);
                 a.sbAdd[at] = __uint_as_float(ed[v].y) * sv;
+            } else if (SHARED && pos == cap) {
+                atomicOr(&a.ctl[1], 1);
             }
         }
     }
+}
+
+// Writes the events of the `ns` rows of the LDS push list into this block's
+// sub-buckets for parity `par`.
+__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns) {
+    const int32_t total = row_scan(a, s, ns);
+    push_edges<false>(a, s, par, src, ns, 0, total);
 }
 
 // After a push: publish this block's counts to every destination, check the
@@ -369,10 +401,13 @@ __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
 
 __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
     const int tid = threadIdx.x;
-    const int32_t *cnt = a.sbCnt + (int64_t)par * a.S * a.S + (int64_t)me * a.S;
+    int32_t *cnt = a.sbCnt + (int64_t)par * a.S * a.S + (int64_t)me * a.S;
     int32_t total = 0;
-    // S <= S_MAX = PT: one count per thread
-    const int32_t c = tid < a.S ? cnt[tid] : 0;
+    // S <= S_MAX = PT: one count per thread.  The driven block's count is
+    // reserved by many blocks with atomics (push_driven_share): clamp it, and
+    // clear it for the push two epochs on.
+    const int32_t c = tid < a.S ? (tid == 0 ? min(cnt[0], CAP_DRV) : cnt[tid]) : 0;
+    if (tid == 0) cnt[0] = 0;
     const int32_t off = wg_exclusive_scan<PT>(c, s.scan, &total);
     if (tid < a.S) s.pref[tid] = off;
     if (tid == 0) { s.pref[a.S] = total; a.dstTot[par * a.S + me] = 0; }
@@ -509,13 +544,16 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
         __syncthreads();
         FB_SUB(2)
         int32_t j = off;
-        uint32_t x = 0;
+        // this thread's first draw is at stream position off+1 after rng:
+        // M^(off%32+1) (M^(32 (off/32)) rng); the later ones are single steps
+        uint32_t x = nr ? matvec(s.leap + (off & 31) * LEAP_STRIDE, s.ybase[off >> 5]) : 0u;
+        bool first = true;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
             if ((nr >> u) & 1u) {
-                // stream position j+1 after rng: M^(j%32+1) (M^(32 (j/32)) rng), or one step past the previous draw
-                x = j == off ? matvec(s.leap + (j & 31) * LEAP_STRIDE, s.ybase[j >> 5]) : xs32(x);
+                if (!first) x = xs32(x);
+                first = false;
                 if (j == n - 1) s.rngNext[m & 1] = x;
                 j++;
                 const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
@@ -563,8 +601,9 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
 
 // ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
 
-__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me,
-                            bool drivenBlock) {
+// This block's spikes of the epoch -> the LDS push list (s.spkI / spkK /
+// spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
+__device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], bool drivenBlock) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t P = drivenBlock ? (d.nDriven + PT - 1) / PT : 0;
@@ -591,8 +630,81 @@ __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS
     __syncthreads();
     const int32_t ns = s.nspk;
     if (tid == 0 && ns > SPK_CAP) atomicOr(&a.ctl[1], 4);
-    push_rows(a, s, par, me, min(ns, SPK_CAP));
+    return min(ns, SPK_CAP);
+}
+
+// A regular block pushes its own spikes.
+__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me) {
+    const int32_t ns = collect_spikes(a, s, cells, false);
+    push_rows(a, s, par, me, ns);
     publish_counts(a, s, par, me);
+}
+
+// The driven block's rows are many (every forced spike), so the whole grid
+// pushes them: the driven block publishes its row list (header ns, total;
+// then rowOff[SPK_CAP + 1], rowA, spkK, spkR), and after a grid barrier
+// every block writes one contiguous share of the flattened edge list.
+__device__ void publish_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
+    const int tid = threadIdx.x;
+    const int32_t ns = collect_spikes(a, s, cells, true);
+    const int32_t total = row_scan(a, s, ns);
+    int32_t *dl = a.dlist;
+    for (int32_t r = tid; r <= ns; r += PT) dl[2 + r] = s.rowOff[r];
+    for (int32_t r = tid; r < ns; r += PT) {
+        dl[2 + (SPK_CAP + 1) + r] = (int32_t)s.rowA[r];
+        dl[2 + (SPK_CAP + 1) + SPK_CAP + r] = (int32_t)s.spkK[r];
+        dl[2 + (SPK_CAP + 1) + 2 * SPK_CAP + r] = __float_as_int(s.spkR[r]);
+    }
+    if (tid == 0) {
+        dl[0] = ns;
+        dl[1] = total;
+        if (s.ev) atomicAdd(&a.d.sc->edgeEvents, s.ev);
+        if (s.syn) atomicAdd(&a.d.sc->synEvents, s.syn);
+        s.ev = 0;
+        s.syn = 0;
+    }
+}
+
+__device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par) {
+    const int tid = threadIdx.x;
+    const bool timed = tid == 0 && (a.flags & 1);
+#define FB_SHARE(k)                                    \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.share[k] += now - s.share[3];                \
+        s.share[3] = now;                              \
+    }
+    if (timed) s.share[3] = wall_clock64();
+    const int32_t *dl = a.dlist;
+    const int32_t ns = dl[0], total = dl[1];
+    const int64_t G = min((int32_t)gridDim.x, SHARE_BLOCKS), b = blockIdx.x;
+    if (b >= G) return;
+    const int32_t g0 = (int32_t)(total * b / G), g1 = (int32_t)(total * (b + 1) / G);
+    if (g0 >= g1) return;   // the same for every thread of the block
+    __syncthreads();        // the LDS push list may still be read by this block's own push
+    for (int32_t r = tid; r <= ns; r += PT) s.rowOff[r] = dl[2 + r];
+    for (int32_t r = tid; r < ns; r += PT) {
+        s.rowA[r] = (uint32_t)dl[2 + (SPK_CAP + 1) + r];
+        s.spkK[r] = (uint32_t)dl[2 + (SPK_CAP + 1) + SPK_CAP + r];
+        s.spkR[r] = __int_as_float(dl[2 + (SPK_CAP + 1) + 2 * SPK_CAP + r]);
+    }
+    __syncthreads();
+    FB_SHARE(0)
+    push_edges<true>(a, s, par, 0, ns, g0, g1);
+    __syncthreads();
+    FB_SHARE(1)
+    // what this block added to each destination, against the event capacity
+    for (int32_t r = tid; r < a.S; r += PT) {
+        const int32_t c = s.cnt[r];
+        if (c) {
+            const int32_t tot = atomicAdd(&a.dstTot[par * a.S + r], c) + c;
+            if (tot > EV_CAP) atomicOr(&a.ctl[1], 2);
+            s.cnt[r] = 0;
+        }
+    }
+    __syncthreads();
+    FB_SHARE(2)
+#undef FB_SHARE
 }
 
 // ------------------------------------------------------------------ lists: one step's spikes in lif.c order
@@ -672,16 +784,20 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     const int32_t lo = drivenBlock || !owning ? 0 : (me - 1) * a.nb;
     const int32_t hi = drivenBlock || !owning ? 0 : min(d.N, lo + a.nb);
 
-    const bool timed = tid == 0 && me < 2;
+    // clocks of the driven block, the first regular block and the first spare block
+    const int32_t timedSlot = me == 0 ? 0 : me == 1 ? 1 : (a.nSpare > 0 && me == a.S) ? 2 : -1;
+    const bool timed = tid == 0 && timedSlot >= 0;
     if (timed) {
-        for (int k = 0; k < 6; k++) { s.ph[k] = 0; s.sub[k] = 0; }
-        s.ph[6] = wall_clock64();
+        for (int k = 0; k < NPH; k++) s.ph[k] = 0;
+        for (int k = 0; k < 6; k++) s.sub[k] = 0;
+        s.ph[NPH] = wall_clock64();
     }
+    if (tid == 0) s.share[0] = s.share[1] = s.share[2] = 0;
 #define FB_PHASE(k)                                    \
     if (timed) {                                       \
         const uint64_t now = wall_clock64();           \
-        s.ph[k] += now - s.ph[6];                      \
-        s.ph[6] = now;                                 \
+        s.ph[k] += now - s.ph[NPH];                    \
+        s.ph[NPH] = now;                               \
     }
 
     for (int r = tid; r < S_MAX; r += PT) s.cnt[r] = 0;
@@ -710,6 +826,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     __syncthreads();
 
     const int32_t L0 = min(a.D, a.nsteps);
+    // the first epoch's driven push counts into parity 1 with atomics
+    if (owning && tid == 0) a.sbCnt[(int64_t)a.S * a.S + (int64_t)me * a.S] = 0;
     if (owning) prologue_scatter(a, s, L0);
     FB_PHASE(5)
     grid.sync();
@@ -721,19 +839,31 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     }
 
     int32_t done = 0, e = 0, prevL = 0;
+    bool listsBuilt = false;   // the last epoch's lists were built before a stop
     if (!a.ctl[1]) {
         while (done < a.nsteps) {
             const int32_t L = min(a.D, a.nsteps - done);
             const int32_t par = e & 1;
+            // overflow flags of the last push: read now, acted on after the
+            // gather (which only reads), so the load overlaps the gather's
+            const int32_t stop = e > 0 ? __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             if (e > 0) build_lists(a, s, par ^ 1, done - prevL, prevL);
             FB_PHASE(3)
+            int32_t nev = 0;
+            if (owning) nev = gather(a, s, par, me);
+            FB_PHASE(0)
+            if (stop) {   // every block read the same flags after the barrier
+                listsBuilt = true;
+                break;
+            }
             if (owning) {
-                const int32_t nev = gather(a, s, par, me);
-                FB_PHASE(0)
                 if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
-                if (done + L < a.nsteps) push_spikes(a, s, cells, par ^ 1, me, drivenBlock);
+                if (done + L < a.nsteps) {
+                    if (drivenBlock) publish_driven(a, s, cells);
+                    else push_spikes(a, s, cells, par ^ 1, me);
+                }
 #pragma unroll
                 for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
@@ -744,14 +874,19 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             }
             grid.sync();
             FB_PHASE(4)
+            if (done + L < a.nsteps) {
+                push_driven_share(a, s, par ^ 1);
+                FB_PHASE(6)
+                grid.sync();
+                FB_PHASE(7)
+            }
             done += L;
             prevL = L;
             e++;
-            if (a.ctl[1]) break;   // every block reads the same flags after the barrier
         }
     }
     // the last epoch's lists, and the state back to memory
-    if (e > 0) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
+    if (e > 0 && !listsBuilt) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
     FB_PHASE(3)
     if (owning) {
 #pragma unroll
@@ -760,9 +895,16 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     }
 #undef FB_PHASE
     if (timed) {
-        for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[me * 6 + k], (unsigned long long)s.ph[k]);
+        for (int k = 0; k < NPH; k++)
+            atomicAdd((unsigned long long *)&a.prof[timedSlot * NPH + k], (unsigned long long)s.ph[k]);
         if (me == 0)
-            for (int k = 0; k < 6; k++) atomicAdd((unsigned long long *)&a.prof[12 + k], (unsigned long long)s.sub[k]);
+            for (int k = 0; k < 6; k++)
+                atomicAdd((unsigned long long *)&a.prof[PROF_SUB + k], (unsigned long long)s.sub[k]);
+    }
+    if (tid == 0 && (a.flags & 1)) {
+        for (int k = 0; k < 3; k++)
+            atomicAdd((unsigned long long *)&a.prof[PROF_SHARE + k], (unsigned long long)s.share[k]);
+        atomicMax((unsigned long long *)&a.prof[PROF_SHARE + 3], (unsigned long long)(s.share[0] + s.share[1] + s.share[2]));
     }
     if (me == 0 && tid == 0) a.ctl[0] = done;
 }
@@ -858,6 +1000,7 @@ struct PersistBuffers {
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
     uint32_t *eown = nullptr;
+    int32_t *dlist = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (1 sub-bucket, 2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t perPar = 0, E = 0;
@@ -870,12 +1013,13 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
-                  pb.ctl, pb.prof, pb.eown};
+                  pb.ctl, pb.prof, pb.eown, pb.dlist};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
+    pb.dlist = nullptr;
     pb.eownNb = -1;
     pb.eownDriven = ~0ull;
 }
@@ -939,8 +1083,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.sbCnt, sizeof(int32_t) * 2 * (size_t)lay.S * lay.S));
         FB_HIP(hipMalloc(&pb.dstTot, sizeof(int32_t) * 2 * (size_t)lay.S));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
-        FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * 18));
+        FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * PERSIST_PROF));
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
+        FB_HIP(hipMalloc(&pb.dlist, sizeof(int32_t) * (size_t)(2 + (SPK_CAP + 1) + 3 * SPK_CAP)));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
@@ -980,6 +1125,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.ctl = pb.ctl;
     a.prof = pb.prof;
     a.eown = pb.eown;
+    a.dlist = pb.dlist;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
@@ -988,7 +1134,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.head0 = cur;
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
-        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * 18, st));
+        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * PERSIST_PROF, st));
         FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
@@ -1011,9 +1157,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
         if (kernelMs) *kernelMs += ms;
         if (phaseUs) {
-            uint64_t prof[18];
+            uint64_t prof[PERSIST_PROF];
             FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
-            for (int k = 0; k < 18; k++) phaseUs[k] += (double)prof[k] / g_ps.tickHz * 1e6;
+            for (int k = 0; k < PERSIST_PROF; k++)
+                phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
         }
         done += ctl[0];
         pb.overflowSeen |= (uint32_t)ctl[1];
