@@ -137,6 +137,52 @@ constexpr int LEAP_WORDS = 128;           // 8 nibbles x 16 values
 constexpr int LEAP_ROUND = 2048;
 constexpr int LEAP_STRIDE = 129;
 
+// Grid-wide barrier for the cooperative (all workgroups resident) kernels.
+// The runtime's grid sync costs ~25 us on MI355X when ~250 workgroups arrive
+// together; this one costs ~7 us there and ~2 us when one workgroup is late.
+// Workgroups arrive on their group's counter (GBAR_GROUP a group, counters 256
+// bytes apart), the last of a group on the top counter ctr[0], and every
+// workgroup polls the top counter.  The counters only grow, so a barrier needs
+// no reset; the host zeroes them before each launch.  Memory: every wave waits
+// for its own accesses, then one agent-scope release fence and, after the
+// wait, one agent-scope acquire fence per workgroup.  The wait is bounded: after
+// GBAR_TIMEOUT wall-clock ticks it sets bit 8 in *flags and returns false, and
+// the caller leaves the kernel.
+constexpr int GBAR_GROUP = 16;
+constexpr int GBAR_MAX_BLOCKS = 1024;
+constexpr int GBAR_WORDS = 64 * (GBAR_MAX_BLOCKS / GBAR_GROUP + 1);
+constexpr uint64_t GBAR_TIMEOUT = 1000000000ull;   // 10 s of the 100 MHz wall clock
+constexpr int32_t GBAR_TIMED_OUT = 8;
+
+__device__ __forceinline__ bool grid_barrier(int32_t *ctr, int32_t *flags) {
+    __shared__ int32_t gbOk;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int32_t nblk = (int32_t)gridDim.x;
+        const int32_t g = (int32_t)blockIdx.x / GBAR_GROUP, ng = (nblk + GBAR_GROUP - 1) / GBAR_GROUP;
+        const int32_t sz = min(GBAR_GROUP, nblk - g * GBAR_GROUP);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const int32_t arr = atomicAdd(&ctr[64 * (g + 1)], 1);
+        if (arr % sz == sz - 1) atomicAdd(&ctr[0], 1);
+        const int32_t target = (arr / sz + 1) * ng;
+        const uint64_t t0 = wall_clock64();
+        int32_t ok = 1;
+        while (__hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (wall_clock64() - t0 > GBAR_TIMEOUT) {
+                atomicOr(flags, GBAR_TIMED_OUT);
+                ok = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gbOk = ok;
+    }
+    __syncthreads();
+    return gbOk != 0;
+}
+
 // Lookahead-kernel clocks reported per launch: the NPH phases of the driven
 // block, of the first regular block and of the first spare block; the parts of
 // the driven step (6); the shared push of the driven rows (mean load / edges /

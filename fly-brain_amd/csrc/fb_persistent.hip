@@ -44,7 +44,6 @@
 // steps (a neuron then arrives at most once per epoch).  A sub-bucket or LDS
 // overflow stops the launch at an epoch boundary; the host runs one epoch on
 // the per-step engine and relaunches.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -52,7 +51,6 @@
 
 #include "fb_internal.h"
 
-namespace cg = cooperative_groups;
 
 namespace fb {
 
@@ -77,6 +75,7 @@ constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold sp
 #define FB_SHARE_BLOCKS 64
 #endif
 constexpr int SHARE_BLOCKS = FB_SHARE_BLOCKS;   // blocks that push the driven block's rows
+constexpr int SHQ = 256;            // driven neurons per share block
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -109,7 +108,8 @@ struct PersistArgs {
     int32_t *ctl;        // [0] steps done, [1] overflow flags
     uint64_t *prof;      // phase clocks (see k_persist)
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
-    int32_t *dlist;      // the driven block's row list of the epoch (publish_driven)
+    float *dres;         // [2][DRV_CAP] res of each driven neuron at the epoch's end
+    int32_t *bar;        // grid barrier counters (GBAR_WORDS, zero at launch)
     int32_t flags;       // bit 0: time the driven block's step parts
 };
 
@@ -120,7 +120,6 @@ struct Shared {
     int32_t evRank[D_MAX * WPS];     // set bits of hasEv before each word
     uint16_t evStart[EV_CAP];        // first sorted event of the r-th (step, target) with arrivals
     uint32_t leap[LEAP_LDS_WORDS];
-    uint32_t ybase[64];
     int32_t pref[S_MAX + 1];
     int32_t cnt[S_MAX];          // push slots per destination
     int32_t spkI[SPK_CAP];       // push list: neuron, step << 24 | arrival key, sign x res at arrival
@@ -135,6 +134,13 @@ struct Shared {
     uint64_t ph[NPH + 1];        // phase clocks of thread 0, then the last stamp
     uint64_t sub[7];             // driven-step clocks of thread 0
     uint64_t share[4];           // shared-push clocks of thread 0: load, edges, publish, last stamp
+    // this block's share of the driven neurons (positions q0 .. q0+nq-1), fixed per launch
+    int32_t shI[SHQ];            // neuron
+    uint32_t shRA[SHQ];          // row start in the CSR
+    int32_t shLen[SHQ];          // row length
+    uint32_t shSyn[SHQ];         // synapses of the row
+    float shSign[SHQ];
+    int32_t shQ0, shNq;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -464,13 +470,18 @@ __device__ __forceinline__ void store_cell(const DevBrain &d, Cell &c) {
     c.count = 0;
 }
 
-// lif.c phase 1 for one neuron: depression if it arrives now, then this step's inputs
-__device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shared &s, int32_t nev, int32_t m,
-                                            uint32_t ql) {
+// lif.c phase 1 for one neuron: depression if it arrives now, then this
+// step's inputs (`has`: the neuron's bit of hasEv at step m)
+__device__ __forceinline__ void cell_inputs_has(const Consts &k, Cell &c, const Shared &s, int32_t nev, int32_t m,
+                                                uint32_t ql, bool has) {
     if ((c.prev >> m) & 1u) {
         if (c.sign * c.st.res != 0.f) c.st.res = c.st.res - k.depU * c.st.res;
     }
-    if ((s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u) apply_run(s, nev, (uint32_t)m, ql, c.st.gE, c.st.gI);
+    if (has) apply_run(s, nev, (uint32_t)m, ql, c.st.gE, c.st.gI);
+}
+__device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shared &s, int32_t nev, int32_t m,
+                                            uint32_t ql) {
+    cell_inputs_has(k, c, s, nev, m, ql, (s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u);
 }
 
 // ------------------------------------------------------------------ simulate
@@ -523,13 +534,20 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
     if (timed) s.sub[6] = wall_clock64();
     int32_t nPrev = 0;
     for (int32_t m = 0; m < L; m++) {
-        // lif.c phase 1: arrivals for the driven neurons
+        // lif.c phase 1: arrivals for the driven neurons.  The thread's cells
+        // are the consecutive local targets q0 .. q0+P-1 (P <= DPT): their
+        // hasEv bits come from one word, or two when they straddle one.
+        const uint32_t q0 = (uint32_t)(tid * P);
+        const int32_t w0 = m * WPS + (int32_t)(q0 >> 5);
+        uint64_t pair = s.hasEv[w0];
+        if ((q0 & 31u) + (uint32_t)P > 32u) pair |= (uint64_t)s.hasEv[w0 + 1] << 32;
+        const uint32_t evBits = (uint32_t)(pair >> (q0 & 31u));
         uint32_t nr = 0;   // bit u: cell u is not refractory and draws
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             Cell &c = cells[u];
             if (c.i >= 0) {
-                cell_inputs(d.k, c, s, nev, m, (uint32_t)(tid * P + u));
+                cell_inputs_has(d.k, c, s, nev, m, q0 + (uint32_t)u, (evBits >> u) & 1u);
                 if (!(c.st.refr > 0)) nr |= 1u << u;
             }
         }
@@ -540,13 +558,12 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
         if (nPrev > 0) rng = s.rngNext[(m - 1) & 1];   // the last draw of the previous step
         nPrev = n;
         FB_SUB(1)
-        if (tid < 64 && tid * 32 < n) s.ybase[tid] = matvec(s.leap + (32 + tid) * LEAP_STRIDE, rng);
-        __syncthreads();
         FB_SUB(2)
         int32_t j = off;
         // this thread's first draw is at stream position off+1 after rng:
-        // M^(off%32+1) (M^(32 (off/32)) rng); the later ones are single steps
-        uint32_t x = nr ? matvec(s.leap + (off & 31) * LEAP_STRIDE, s.ybase[off >> 5]) : 0u;
+        // M^(off%32+1) (M^(32 (off/32)) rng), off < 2048; the later ones are single steps
+        uint32_t x = nr ? matvec(s.leap + (off & 31) * LEAP_STRIDE, matvec(s.leap + (32 + (off >> 5)) * LEAP_STRIDE, rng))
+                        : 0u;
         bool first = true;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
@@ -593,6 +610,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
         if (c.i >= 0) {
             a.dmask[par * DRV_CAP + tid * P + u] = c.forced;
             a.tmask[par * DRV_CAP + tid * P + u] = c.mask & ~c.forced;
+            a.dres[par * DRV_CAP + tid * P + u] = c.st.res;
         }
     }
     FB_SUB(5)
@@ -601,12 +619,11 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
 
 // ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
 
-// This block's spikes of the epoch -> the LDS push list (s.spkI / spkK /
-// spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
-__device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], bool drivenBlock) {
+// A regular block's spikes of the epoch -> the LDS push list (s.spkI /
+// spkK / spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
+__device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    const int32_t P = drivenBlock ? (d.nDriven + PT - 1) / PT : 0;
     if (tid == 0) s.nspk = 0;
     __syncthreads();
 #pragma unroll
@@ -618,10 +635,8 @@ __device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)
             bits &= bits - 1;
             const int32_t e = atomicAdd(&s.nspk, 1);
             if (e < SPK_CAP) {
-                const bool forced = (c.forced >> m) & 1u;
                 s.spkI[e] = c.i;
-                s.spkK[e] = ((uint32_t)m << 24) |
-                            (forced ? (uint32_t)(tid * P + u) : KEY_THRESHOLD + (uint32_t)c.i);
+                s.spkK[e] = ((uint32_t)m << 24) | (KEY_THRESHOLD + (uint32_t)c.i);
                 // sign x res at its arrival step of the next epoch
                 s.spkR[e] = c.sign * res_after(d.k, c.st.res, m);
             }
@@ -635,37 +650,36 @@ __device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)
 
 // A regular block pushes its own spikes.
 __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me) {
-    const int32_t ns = collect_spikes(a, s, cells, false);
+    const int32_t ns = collect_spikes(a, s, cells);
     push_rows(a, s, par, me, ns);
     publish_counts(a, s, par, me);
 }
 
-// The driven block's rows are many (every forced spike), so the whole grid
-// pushes them: the driven block publishes its row list (header ns, total;
-// then rowOff[SPK_CAP + 1], rowA, spkK, spkR), and after a grid barrier
-// every block writes one contiguous share of the flattened edge list.
-__device__ void publish_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
+// The driven block's rows are many (every forced spike), so the first
+// SHARE_BLOCKS blocks push them, each for a fixed range of driven-list
+// positions whose rows it holds in LDS (share_setup).  After the grid
+// barrier that ends the epoch's simulation, a share block reads the spike
+// steps of its neurons (dmask: forced, tmask: threshold, as the driven block
+// wrote them) and their res, and writes their rows' events.
+__device__ void share_setup(const PersistArgs &a, Shared &s) {
+    const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    const int32_t ns = collect_spikes(a, s, cells, true);
-    const int32_t total = row_scan(a, s, ns);
-    int32_t *dl = a.dlist;
-    for (int32_t r = tid; r <= ns; r += PT) dl[2 + r] = s.rowOff[r];
-    for (int32_t r = tid; r < ns; r += PT) {
-        dl[2 + (SPK_CAP + 1) + r] = (int32_t)s.rowA[r];
-        dl[2 + (SPK_CAP + 1) + SPK_CAP + r] = (int32_t)s.spkK[r];
-        dl[2 + (SPK_CAP + 1) + 2 * SPK_CAP + r] = __float_as_int(s.spkR[r]);
-    }
-    if (tid == 0) {
-        dl[0] = ns;
-        dl[1] = total;
-        if (s.ev) atomicAdd(&a.d.sc->edgeEvents, s.ev);
-        if (s.syn) atomicAdd(&a.d.sc->synEvents, s.syn);
-        s.ev = 0;
-        s.syn = 0;
+    const int32_t G = min((int32_t)gridDim.x, SHARE_BLOCKS), b = blockIdx.x;
+    if (b >= G) return;
+    const int32_t q0 = (int32_t)((int64_t)d.nDriven * b / G), q1 = (int32_t)((int64_t)d.nDriven * (b + 1) / G);
+    if (tid == 0) { s.shQ0 = q0; s.shNq = q1 - q0; }
+    for (int32_t k = tid; k < q1 - q0; k += PT) {
+        const int32_t i = d.driven[q0 + k];
+        s.shI[k] = i;
+        s.shRA[k] = d.g.indptr[i];
+        s.shLen[k] = (int32_t)(d.g.indptr[i + 1] - d.g.indptr[i]);
+        s.shSyn[k] = d.g.rowSyn[i];
+        s.shSign[k] = d.g.sign[i];
     }
 }
 
 __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par) {
+    const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const bool timed = tid == 0 && (a.flags & 1);
 #define FB_SHARE(k)                                    \
@@ -675,22 +689,65 @@ __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par) 
         s.share[3] = now;                              \
     }
     if (timed) s.share[3] = wall_clock64();
-    const int32_t *dl = a.dlist;
-    const int32_t ns = dl[0], total = dl[1];
-    const int64_t G = min((int32_t)gridDim.x, SHARE_BLOCKS), b = blockIdx.x;
-    if (b >= G) return;
-    const int32_t g0 = (int32_t)(total * b / G), g1 = (int32_t)(total * (b + 1) / G);
-    if (g0 >= g1) return;   // the same for every thread of the block
+    if ((int32_t)blockIdx.x >= min((int32_t)gridDim.x, SHARE_BLOCKS)) return;   // uniform in the block
     __syncthreads();        // the LDS push list may still be read by this block's own push
-    for (int32_t r = tid; r <= ns; r += PT) s.rowOff[r] = dl[2 + r];
-    for (int32_t r = tid; r < ns; r += PT) {
-        s.rowA[r] = (uint32_t)dl[2 + (SPK_CAP + 1) + r];
-        s.spkK[r] = (uint32_t)dl[2 + (SPK_CAP + 1) + SPK_CAP + r];
-        s.spkR[r] = __int_as_float(dl[2 + (SPK_CAP + 1) + 2 * SPK_CAP + r]);
+    if (tid == 0) s.nspk = 0;
+    __syncthreads();
+    const int32_t q0 = s.shQ0, nq = s.shNq;
+    // rows: one per spike step of each neuron (a forced and a threshold spike never share a step)
+    for (int32_t k = tid; k < nq; k += PT) {
+        const uint32_t fm = a.dmask[par * DRV_CAP + q0 + k];
+        uint32_t bits = fm | a.tmask[par * DRV_CAP + q0 + k];
+        if (!bits) continue;
+        const float res = a.dres[par * DRV_CAP + q0 + k];
+        const int32_t i = s.shI[k];
+        while (bits) {
+            const int m = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int32_t e = atomicAdd(&s.nspk, 1);   // beyond SPK_CAP rows: flag 4 below
+            if (e < SPK_CAP) {
+                s.spkI[e] = k;
+                s.spkK[e] = ((uint32_t)m << 24) | (((fm >> m) & 1u) ? (uint32_t)(q0 + k) : KEY_THRESHOLD + (uint32_t)i);
+                // sign x res at its arrival step of the next epoch
+                s.spkR[e] = s.shSign[k] * res_after(d.k, res, m);
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t nsAll = s.nspk;
+    if (tid == 0 && nsAll > SPK_CAP) atomicOr(&a.ctl[1], 4);
+    const int32_t ns = min(nsAll, SPK_CAP);
+    // row lengths from the LDS table -> the flattened edge list
+    int32_t len[RPT];
+    int32_t mySum = 0;
+    unsigned long long syn = 0;
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        len[v] = 0;
+        if (r < ns && s.spkR[r] != 0.f) {
+            const int32_t k = s.spkI[r];
+            len[v] = s.shLen[k];
+            syn += s.shSyn[k];
+        }
+        mySum += len[v];
+    }
+    int32_t total;
+    int32_t off = wg_exclusive_scan<PT>(mySum, s.scan, &total);
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        if (r < ns) { s.rowOff[r] = off; s.rowA[r] = s.shRA[s.spkI[r]]; }
+        off += len[v];
+    }
+    if (tid == 0) s.rowOff[ns] = total;
+    if (mySum) {
+        atomicAdd(&d.sc->edgeEvents, (unsigned long long)mySum);
+        atomicAdd(&d.sc->synEvents, syn);
     }
     __syncthreads();
     FB_SHARE(0)
-    push_edges<true>(a, s, par, 0, ns, g0, g1);
+    push_edges<true>(a, s, par, 0, ns, 0, total);
     __syncthreads();
     FB_SHARE(1)
     // what this block added to each destination, against the event capacity
@@ -775,7 +832,6 @@ __device__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_
 
 __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     __shared__ Shared s;
-    cg::grid_group grid = cg::this_grid();
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t me = blockIdx.x;
@@ -823,6 +879,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             load_cell(d, cells[u], (u < NPT && owning && i < hi && a.drvPos[i] < 0) ? i : -1);
         }
     }
+    share_setup(a, s);
     __syncthreads();
 
     const int32_t L0 = min(a.D, a.nsteps);
@@ -830,7 +887,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     if (owning && tid == 0) a.sbCnt[(int64_t)a.S * a.S + (int64_t)me * a.S] = 0;
     if (owning) prologue_scatter(a, s, L0);
     FB_PHASE(5)
-    grid.sync();
+    if (!grid_barrier(a.bar, &a.ctl[1])) return;
     FB_PHASE(4)
     if (d.k.depU != 0.f && owning) {
 #pragma unroll
@@ -860,10 +917,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
-                if (done + L < a.nsteps) {
-                    if (drivenBlock) publish_driven(a, s, cells);
-                    else push_spikes(a, s, cells, par ^ 1, me);
-                }
+                if (done + L < a.nsteps && !drivenBlock) push_spikes(a, s, cells, par ^ 1, me);
 #pragma unroll
                 for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
@@ -872,12 +926,12 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 }
                 FB_PHASE(2)
             }
-            grid.sync();
+            if (!grid_barrier(a.bar, &a.ctl[1])) return;
             FB_PHASE(4)
             if (done + L < a.nsteps) {
                 push_driven_share(a, s, par ^ 1);
                 FB_PHASE(6)
-                grid.sync();
+                if (!grid_barrier(a.bar, &a.ctl[1])) return;
                 FB_PHASE(7)
             }
             done += L;
@@ -967,7 +1021,7 @@ static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
     if (l.S > S_MAX) return false;
     l.nSpare = std::min(D, maxB - l.S);
     l.grid = l.S + l.nSpare;
-    return true;
+    return l.grid <= GBAR_MAX_BLOCKS;
 }
 
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why) {
@@ -986,6 +1040,8 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     if (st) { *why = "cooperative launch unavailable"; return st; }
     Layout l;
     if (!layout_for(d, D, l)) { *why = "more than 1024 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
+    const int32_t shareBlocks = std::min(l.grid, SHARE_BLOCKS);
+    if ((d.nDriven + shareBlocks - 1) / shareBlocks > SHQ) { *why = "too many driven neurons for this grid"; return FB_ERR_UNSUPPORTED; }
     return FB_OK;
 }
 
@@ -1000,7 +1056,8 @@ struct PersistBuffers {
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
     uint32_t *eown = nullptr;
-    int32_t *dlist = nullptr;
+    float *dres = nullptr;
+    int32_t *bar = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (1 sub-bucket, 2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t perPar = 0, E = 0;
@@ -1013,13 +1070,14 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
-                  pb.ctl, pb.prof, pb.eown, pb.dlist};
+                  pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
-    pb.dlist = nullptr;
+    pb.dres = nullptr;
+    pb.bar = nullptr;
     pb.eownNb = -1;
     pb.eownDriven = ~0ull;
 }
@@ -1085,7 +1143,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * PERSIST_PROF));
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
-        FB_HIP(hipMalloc(&pb.dlist, sizeof(int32_t) * (size_t)(2 + (SPK_CAP + 1) + 3 * SPK_CAP)));
+        FB_HIP(hipMalloc(&pb.dres, sizeof(float) * 2 * DRV_CAP));
+        FB_HIP(hipMalloc(&pb.bar, sizeof(int32_t) * GBAR_WORDS));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
@@ -1125,7 +1184,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.ctl = pb.ctl;
     a.prof = pb.prof;
     a.eown = pb.eown;
-    a.dlist = pb.dlist;
+    a.dres = pb.dres;
+    a.bar = pb.bar;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
@@ -1134,6 +1194,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.head0 = cur;
         a.nsteps = nsteps - done;
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
+        FB_HIP(hipMemsetAsync(pb.bar, 0, sizeof(int32_t) * GBAR_WORDS, st));
         FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * PERSIST_PROF, st));
         FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
         void *args[] = {&a};
@@ -1161,6 +1222,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
             for (int k = 0; k < PERSIST_PROF; k++)
                 phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
+        }
+        if (ctl[1] & GBAR_TIMED_OUT) {
+            set_error("persistent engine: a grid barrier timed out; the brain state is undefined");
+            return FB_ERR_DEVICE;
         }
         done += ctl[0];
         pb.overflowSeen |= (uint32_t)ctl[1];
