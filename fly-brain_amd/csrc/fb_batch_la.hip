@@ -45,7 +45,6 @@
 // depression and background off, D <= 18, at most 384 driven neurons, the
 // grid fits the device.  A bucket overflow stops the launch at an epoch
 // boundary; the host then runs that epoch on the per-step engine.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <map>
@@ -53,7 +52,6 @@
 
 #include "fb_batch.h"
 
-namespace cg = cooperative_groups;
 
 namespace fb {
 
@@ -83,9 +81,11 @@ struct LaArgs {
     int32_t *dstCnt;                       // [2][S] events reserved per destination
     int32_t *addCnt;                       // [2][S] member addends reserved per destination
     const uint32_t *eown;                  // [E] owner of each edge's target
-    // [0] epochs done; [2 + p] overflow flags of the pushes into parity p
-    // (written only while pushing, read only right after a grid barrier)
+    // [0] epochs done; [1] grid barrier timed out; [2 + p] overflow flags of
+    // the pushes into parity p (written only while pushing, read only right
+    // after a grid barrier)
     int32_t *ctl;
+    int32_t *bar;                          // grid barrier counters (GBAR_WORDS, zero at launch)
     uint64_t *recM, *recF;                 // optional dense record [epochs*D][N]
 };
 
@@ -407,7 +407,6 @@ __device__ void la_prologue(const LaArgs &a, LaShared &s, int32_t me, int32_t nb
 
 __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
     __shared__ LaShared s;
-    cg::grid_group grid = cg::this_grid();
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t me = blockIdx.x;
     const BatchDev &d = a.d;
@@ -455,7 +454,7 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
     if (drv && tid < MPG) s.x0[tid] = me * MPG + tid < B ? d.rng[me * MPG + tid] : 0u;
 
     la_prologue(a, s, me, gridDim.x);
-    grid.sync();
+    if (!grid_barrier(a.bar, &a.ctl[1])) return;
 
     unsigned long long spikes = 0;
     int32_t last = -1;
@@ -601,7 +600,7 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
             la_push(a, s, (e + 1) & 1, tE + D);
         }
         last = e;
-        grid.sync();
+        if (!grid_barrier(a.bar, &a.ctl[1])) return;
         // the pushes of this epoch went to parity (e + 1) & 1; the next writes
         // to that flag come two epochs later, after every block has read it
         if (e + 1 < a.epochs && a.ctl[2 + ((e + 1) & 1)]) break;
@@ -641,7 +640,7 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
     }
     if (me == 0)
         for (int32_t q = tid; q < d.nslots; q += BT) d.nActive[q] = 0;
-    grid.sync();
+    if (!grid_barrier(a.bar, &a.ctl[1])) return;
     for (int32_t st = 0; st < D; st++) {
         const int32_t slot = (a.head0 + tL + st + d.nslots - 1) % d.nslots;
         if (!drv) {
@@ -690,6 +689,7 @@ __global__ void k_batch_eown(const uint2 *edges, int64_t E, const int32_t *drvPo
 struct LaBuffers {
     uint64_t *evKey = nullptr, *evMask = nullptr, *evAux = nullptr;
     int32_t *dstCnt = nullptr, *ctl = nullptr;   // dstCnt holds [2][S] events, then [2][S] addends
+    int32_t *bar = nullptr;
     uint32_t *eown = nullptr;
     int32_t S = 0;
     int64_t E = 0;
@@ -723,7 +723,7 @@ bool batch_la_supported(const BatchDev &d, std::string *why) {
     const int32_t R = (d.N + RNB - 1) / RNB, G = (d.B + MPG - 1) / MPG;
     if (R + G > LS_MAX) return no("too many destinations");
     const int32_t maxB = la_max_blocks();
-    if (R + G > maxB) return no("the grid does not fit the device");
+    if (R + G > maxB || R + G > GBAR_MAX_BLOCKS) return no("the grid does not fit the device");
     return true;
 }
 
@@ -732,7 +732,7 @@ void batch_la_release(const void *key) {
     if (it == g_labuf.end()) return;
     LaBuffers &b = it->second;
     for (void *p : {(void *)b.evKey, (void *)b.evMask, (void *)b.evAux, (void *)b.dstCnt, (void *)b.ctl,
-                    (void *)b.eown})
+                    (void *)b.eown, (void *)b.bar})
         if (p) (void)hipFree(p);
     if (b.ev0) (void)hipEventDestroy(b.ev0);
     if (b.ev1) (void)hipEventDestroy(b.ev1);
@@ -758,6 +758,7 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
     }
     if (!b.ctl) {
         FB_HIP(hipMalloc(&b.ctl, 4 * sizeof(int32_t)));
+        FB_HIP(hipMalloc(&b.bar, sizeof(int32_t) * GBAR_WORDS));
         FB_HIP(hipEventCreate(&b.ev0));
         FB_HIP(hipEventCreate(&b.ev1));
     }
@@ -772,13 +773,14 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
     }
     FB_HIP(hipMemsetAsync(b.dstCnt, 0, 4 * sizeof(int32_t) * S, st));
     FB_HIP(hipMemsetAsync(b.ctl, 0, 4 * sizeof(int32_t), st));
+    FB_HIP(hipMemsetAsync(b.bar, 0, sizeof(int32_t) * GBAR_WORDS, st));
     LaArgs a;
     a.d = d;
     a.t0 = t0; a.epochs = epochs; a.D = D; a.R = R; a.G = G; a.S = S; a.head0 = head0;
     a.nDrv = d.nDriven;
     a.per = std::max(1, (d.nDriven + NSL - 1) / NSL);
     a.evKey = b.evKey; a.evMask = b.evMask; a.evAux = b.evAux; a.dstCnt = b.dstCnt; a.addCnt = b.dstCnt + 2 * S;
-    a.eown = b.eown; a.ctl = b.ctl; a.recM = recM; a.recF = recF;
+    a.eown = b.eown; a.ctl = b.ctl; a.bar = b.bar; a.recM = recM; a.recF = recF;
     void *args[] = {&a};
     FB_HIP(hipEventRecord(b.ev0, st));
     const hipError_t le = hipLaunchCooperativeKernel((const void *)k_batch_la, dim3(S), dim3(BT), args, 0, st);
@@ -795,6 +797,10 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
     float ms = 0.f;
     FB_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
     if (kernelMs) *kernelMs += ms;
+    if (ctl[1] & GBAR_TIMED_OUT) {
+        set_error("ensemble lookahead: a grid barrier timed out; the ensemble state is undefined");
+        return FB_ERR_DEVICE;
+    }
     *done = ctl[0];
     return FB_OK;
 }
