@@ -678,7 +678,9 @@ __device__ void share_setup(const PersistArgs &a, Shared &s) {
     }
 }
 
-__device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par) {
+// `par`: the parity of the sub-buckets written (the next epoch's); `mpar`: the
+// parity of the driven masks read (this epoch's).
+__device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, int32_t mpar) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const bool timed = tid == 0 && (a.flags & 1);
@@ -696,10 +698,10 @@ __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par) 
     const int32_t q0 = s.shQ0, nq = s.shNq;
     // rows: one per spike step of each neuron (a forced and a threshold spike never share a step)
     for (int32_t k = tid; k < nq; k += PT) {
-        const uint32_t fm = a.dmask[par * DRV_CAP + q0 + k];
-        uint32_t bits = fm | a.tmask[par * DRV_CAP + q0 + k];
+        const uint32_t fm = a.dmask[mpar * DRV_CAP + q0 + k];
+        uint32_t bits = fm | a.tmask[mpar * DRV_CAP + q0 + k];
         if (!bits) continue;
-        const float res = a.dres[par * DRV_CAP + q0 + k];
+        const float res = a.dres[mpar * DRV_CAP + q0 + k];
         const int32_t i = s.shI[k];
         while (bits) {
             const int m = __ffs(bits) - 1;
@@ -929,7 +931,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             if (!grid_barrier(a.bar, &a.ctl[1])) return;
             FB_PHASE(4)
             if (done + L < a.nsteps) {
-                push_driven_share(a, s, par ^ 1);
+                push_driven_share(a, s, par ^ 1, par);
                 FB_PHASE(6)
                 if (!grid_barrier(a.bar, &a.ctl[1])) return;
                 FB_PHASE(7)

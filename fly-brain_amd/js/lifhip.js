@@ -40,6 +40,33 @@ function paramsVec(p) {
 
 function nslotsOf(p) { return Math.max(1, Math.round(p.delay / p.dt)) + 1; }
 
+// src/lif.js:5: unknown, ACh, GABA, Glu, DA, 5HT, OA, histamine
+const EXC_SIGN = [0, 1, -1, -1, 1, 1, 1, -1];
+const NETWORK_ARRAY_OPTIONS = ['inScale', 'outScale', 'sensoryMask', 'ntSign', 'preSign'];
+
+// LIFNetwork's constructor bake (src/lif.js:17-33): per-edge weights from the
+// synapse counts, then inhGain on inhibitory rows; the per-source sign times
+// wSyn is what lif.c reads.
+function bakeNetwork(N, indptr, indices, counts, nt, p) {
+  const SG = p.ntSign || EXC_SIGN, PS = p.preSign, inS = p.inScale, outS = p.outScale;
+  const sens = p.sensoryMask || new Uint8Array(N);
+  const signOf = (j) => (PS ? PS[j] : SG[nt[j]]);
+  const w = new Float32Array(indices.length);
+  const sign = new Float32Array(N);
+  for (let j = 0; j < N; j++) {
+    const og = outS ? outS[j] : 1, s = signOf(j), keepSensory = p.preInh > 0 && s < 0;
+    sign[j] = s * p.wSyn;
+    for (let k = indptr[j]; k < indptr[j + 1]; k++) {
+      const q = indices[k], c = counts[k];
+      w[k] = (c >= p.minSyn && (!sens[q] || keepSensory)) ? c * (inS ? inS[q] : 1) * og : 0;
+    }
+  }
+  if (p.inhGain !== 1) {
+    for (let j = 0; j < N; j++) if (signOf(j) < 0) for (let k = indptr[j]; k < indptr[j + 1]; k++) w[k] *= p.inhGain;
+  }
+  return { weights: w, sign: sign };
+}
+
 class LIFHip {
   /**
    * @param opts {graph: {indptr, indices, weights, sign, counts?}, N?, params?, seed?, engine?}
@@ -69,6 +96,24 @@ class LIFHip {
 
   /** LIFGpu.create's shape (src/lifgpu.js:185): resolves to a ready brain. */
   static create(opts) { return Promise.resolve(new LIFHip(opts)); }
+
+  /**
+   * new LIFNetwork(N, indptr, indices, weights, nt, params) (src/lif.js:17-33)
+   * on the engine: the same weight and sign bake, lif.c's step.  Gaussian
+   * noise is not part of lif.c and is refused.
+   */
+  static fromNetwork(N, indptr, indices, weights, nt, params, seed, engine) {
+    const p = Object.assign({}, DEFAULTS, params || {});
+    if (p.noise > 0) throw new Error('LIFHip.fromNetwork: noise > 0 is not part of the lif.c model');
+    const baked = bakeNetwork(N, indptr, indices, weights, nt, p);
+    const scalar = {};
+    for (const k of Object.keys(p)) if (NETWORK_ARRAY_OPTIONS.indexOf(k) < 0) scalar[k] = p[k];
+    const b = new LIFHip({ graph: { indptr: indptr, indices: indices, weights: baked.weights, sign: baked.sign },
+      N: N, params: scalar, seed: seed, engine: engine });
+    b.indptr = indptr; b.indices = indices; b.nt = nt; b.weights = baked.weights;
+    b.sensory = p.sensoryMask || new Uint8Array(N);
+    return b;
+  }
 
   _array(k) {
     if (this._stale[k]) {
@@ -181,4 +226,4 @@ class LIFHip {
   }
 }
 
-module.exports = { LIFHip, DEFAULTS, PARAM_FIELDS, native };
+module.exports = { LIFHip, DEFAULTS, PARAM_FIELDS, EXC_SIGN, bakeNetwork, native };

@@ -361,3 +361,135 @@ class OracleDiff:
         r = np.zeros(max(1, m), dtype=np.float64)
         self.lib.fdo_readout(self.h, _ptr(s), _ptr(r))
         return s[:m].reshape(self.steps, self.R), r[:m].reshape(self.steps, self.R)
+
+
+# ------------------------------------------------------------------ LIFNetwork (the JS twin)
+
+NET_FIELDS = ("dt", "vRest", "vThresh", "vReset", "tauM", "tauSyn", "tRef", "delay", "wSyn", "noise",
+              "traceTau", "adaptInc", "adaptTau", "depU", "depTau", "minSyn", "preInh", "coba", "eExc",
+              "eInh", "inhGain")
+NET_ARRAYS = ("v", "gE", "gI", "refr", "trace", "adapt", "res", "bias", "thr", "drive", "weights")
+# src/lif.js:6-15
+NET_DEFAULTS = {"dt": 0.5, "vRest": -52.0, "vThresh": -45.0, "vReset": -52.0, "tauM": 20.0, "tauSyn": 5.0,
+                "tRef": 2.2, "delay": 1.8, "wSyn": 0.275, "noise": 0.0, "traceTau": 30.0, "adaptInc": 2.0,
+                "adaptTau": 100.0, "depU": 0.2, "depTau": 200.0, "minSyn": 1.0, "preInh": 0.0, "coba": False,
+                "eExc": 0.0, "eInh": -70.0, "inhGain": 1.0}
+
+
+class _NetParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in NET_FIELDS]
+
+
+class OracleNet:
+    """LIFNetwork (src/lif.js:16-105) restated in C with Math.random = mulberry32(rseed)."""
+
+    def __init__(self, N, indptr, indices, weights, nt, params=None, inScale=None, outScale=None,
+                 sensoryMask=None, ntSign=None, preSign=None, rseed: int = 1):
+        lib = _lib_net()
+        self.lib = lib
+        self.N = int(N)
+        p = dict(NET_DEFAULTS)
+        p.update(params or {})
+        self.p = p
+        s = _NetParams()
+        for n in NET_FIELDS:
+            setattr(s, n, float(bool(p[n])) if n == "coba" else float(p[n]))
+        self._keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            x = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(x)
+            return x.ctypes.data_as(ctypes.c_void_p)
+
+        ip = np.ascontiguousarray(indptr, dtype=np.uint32)
+        self.E = int(ip[-1])
+        self.h = lib.lno_create(self.N, arr(ip, np.uint32), arr(indices, np.uint32), arr(weights, np.float32),
+                                arr(nt, np.uint8), ctypes.byref(s), arr(inScale, np.float32),
+                                arr(outScale, np.float32), arr(sensoryMask, np.uint8), arr(ntSign, np.float64),
+                                arr(preSign, np.float64), ctypes.c_uint32(rseed & 0xFFFFFFFF))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.lno_destroy(self.h)
+            self.h = None
+
+    def _ix(self, ix):
+        a = np.ascontiguousarray(ix, dtype=np.int32)
+        return a, a.ctypes.data_as(ctypes.c_void_p)
+
+    def reset(self):
+        self.lib.lno_reset(self.h)
+
+    def setBias(self, ix, mv):
+        a, p = self._ix(ix)
+        self.lib.lno_set_bias(self.h, p, int(a.size), float(mv))
+
+    def setDrive(self, ix, rate):
+        a, p = self._ix(ix)
+        self.lib.lno_set_drive(self.h, p, int(a.size), float(rate))
+
+    def setDriveOne(self, i, rate):
+        self.lib.lno_set_drive_one(self.h, int(i), float(rate))
+
+    def setThr(self, i, mv):
+        self.lib.lno_set_thr(self.h, int(i), float(mv))
+
+    def addG(self, i, e, ii=0.0):
+        self.lib.lno_add_g(self.h, int(i), float(e), float(ii or 0.0))
+
+    def pulse(self, ix, mv):
+        a, p = self._ix(ix)
+        self.lib.lno_pulse(self.h, p, int(a.size), float(mv))
+
+    def step(self) -> np.ndarray:
+        nf = int(self.lib.lno_step(self.h))
+        return np.ctypeslib.as_array(self.lib.lno_fired(self.h), shape=(max(1, self.N),))[:nf].copy()
+
+    def run(self, steps: int) -> np.ndarray:
+        nf = np.zeros(max(1, steps), dtype=np.int32)
+        self.lib.lno_run(self.h, int(steps), nf.ctypes.data_as(ctypes.c_void_p))
+        return nf[:steps]
+
+    def array(self, name: str) -> np.ndarray:
+        n = self.E if name == "weights" else self.N
+        ptr = self.lib.lno_array(self.h, NET_ARRAYS.index(name))
+        return np.ctypeslib.as_array(ptr, shape=(max(1, n),))[:n].copy()
+
+    @property
+    def spikeCount(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.lib.lno_spike_count(self.h), shape=(max(1, self.N),))[: self.N].copy()
+
+    @property
+    def nAwake(self) -> int:
+        return int(self.lib.lno_n_awake(self.h))
+
+    @property
+    def t(self) -> float:
+        return float(self.lib.lno_time(self.h))
+
+    @property
+    def nslots(self) -> int:
+        return int(self.lib.lno_nslots(self.h))
+
+
+def _lib_net():
+    lib = ctypes.CDLL(ORACLE_SO)
+    P, I32, I64, D, U32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32
+    sig = {
+        "lno_create": (P, [I32, P, P, P, P, ctypes.POINTER(_NetParams), P, P, P, P, P, U32]),
+        "lno_destroy": (None, [P]), "lno_reset": (None, [P]),
+        "lno_set_bias": (None, [P, P, I32, D]), "lno_set_drive": (None, [P, P, I32, D]),
+        "lno_set_drive_one": (None, [P, I32, D]), "lno_set_thr": (None, [P, I32, D]),
+        "lno_add_g": (None, [P, I32, D, D]), "lno_pulse": (None, [P, P, I32, D]),
+        "lno_step": (I32, [P]), "lno_fired": (ctypes.POINTER(I32), [P]),
+        "lno_array": (ctypes.POINTER(ctypes.c_float), [P, I32]),
+        "lno_spike_count": (ctypes.POINTER(U32), [P]), "lno_n_awake": (I32, [P]),
+        "lno_time": (D, [P]), "lno_nslots": (I32, [P]), "lno_run": (I64, [P, I32, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib

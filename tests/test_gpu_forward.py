@@ -98,6 +98,46 @@ def test_persistent_overflow_hands_epochs_to_the_step_engine():
     assert gpu.stat("persistent_launches") > 1
 
 
+def _multi_epoch_case(name):
+    if name == "dt05_dep_adapt":
+        # LIFNetwork's defaults (dt 0.5 ms: 4-step epochs, depression and
+        # adaptation on) with strong coupling
+        g = synth.lognormal_graph(4096, 12, 0.8, 0.3, 3)
+        sign = (np.where(g.exc, 1.0, -1.0) * 4.0).astype(np.float32)
+        p = P.merged({})
+        drv = np.arange(0, 4096, 20)
+        return g, g.counts.astype(np.float32), sign, p, drv, np.full(drv.size, 150.0)
+    g = synth.config1_graph()
+    p = P.config_params(coba=(name == "c1_coba"))
+    drv = synth.driven_set(g.N, 0.02, 3)
+    return g, g.weights, g.sign, p, drv, np.full(drv.size, 150.0)
+
+
+@pytest.mark.parametrize("name", ["c1", "c1_coba", "dt05_dep_adapt"])
+def test_persistent_multi_epoch_launches_match_oracle(name):
+    """Launches of many epochs, whole and ragged: the pushes between epochs,
+    the shared push of the driven rows and the grid barriers all run inside
+    one launch.  The state after every launch is the oracle's, bit for bit."""
+    g, W, s, p, drv, rates = _multi_epoch_case(name)
+    gpu = _hip(g, W, s, p, 7, "persistent")
+    cpu = OracleBrain(g, p, seed=7, W=W, sign=s)
+    for b in (gpu, cpu):
+        for i, r in zip(drv, rates):
+            b.setDriveOne(int(i), float(r))
+    D = gpu.nslots - 1
+    for launch in (2 * D, 5 * D + 3, 40 * D, 7):
+        print(f"multi-epoch launch: {name}, {launch} steps", flush=True)
+        gpu.advance(launch)
+        nf, ras, _ = cpu.run(launch, raster_cap=launch * g.N)
+        for k in lifcases.STATE:
+            assert np.array_equal(getattr(gpu, k).view(np.uint32), cpu.array(k).view(np.uint32)), (launch, k)
+        assert np.array_equal(gpu.spikeCount, cpu.spikeCount), launch
+        assert gpu.rng == cpu.rng, launch
+        assert np.array_equal(gpu.fired(), ras[ras.size - nf[-1]:]), launch
+    assert gpu.stat("persistent_launches") >= 4
+    assert gpu.stat("fallback_steps") == 0
+
+
 @pytest.fixture(scope="module")
 def c2():
     g = synth.config2_graph()
