@@ -213,7 +213,6 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
 
 // The drive pre-pass (src/lifdiff.js:290-297): one draw per non-refractory
 // driven neuron in set order; forced spikes go to the tape in that order.
-template <bool REPLAY>
 __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
     constexpr int T = 1024;
     __shared__ uint32_t leapS[LEAP_LDS_WORDS];
@@ -225,10 +224,8 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_rng = *d.rng;
-        if (!REPLAY) {
-            s_top = *d.top;
-            d.off[t] = (int64_t)s_top;      // this step's list starts here
-        }
+        s_top = *d.top;
+        d.off[t] = (int64_t)s_top;      // this step's list starts here
     }
     leap_load<T>(leapS, d.leap);
     __syncthreads();
@@ -254,12 +251,10 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
                     d.refr[i] = d.forcedRefr;
                     d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
                     d.forced[i] = 1;
-                    if (!REPLAY) {
-                        d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
-                        if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
-                        const int32_t rk = d.readIdx[i];
-                        if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
-                    }
+                    d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
+                    if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+                    const int32_t rk = d.readIdx[i];
+                    if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
                     d.dflag[q] = 1;
                     fired++;
                 }
@@ -270,7 +265,7 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
         base = draws[cnt - 1];
         __syncthreads();
     }
-    if (!REPLAY) {
+    {
         int32_t totalFired;
         int32_t pos = wg_exclusive_scan<T>(fired, scan, &totalFired);
         for (int32_t q2 = lo; q2 < hi; q2++)
@@ -286,6 +281,21 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
         if (tid == 0) *d.top = s_top + (unsigned long long)totalFired;
     }
     if (tid == 0) *d.rng = base;
+}
+
+// The drive pre-pass when replaying a segment: the forward put every forced
+// spike of step t on the tape (FORCED_BIT, at the head of the step's list),
+// so the replay applies their effects without drawing the stream again.
+__device__ __forceinline__ void dx_forced_replay(const DiffDev &d, int32_t t) {
+    const int64_t a0 = min(d.off[t], d.cap), a1 = min(d.off[t + 1], d.cap);
+    for (int64_t k = a0 + threadIdx.x; k < a1; k += blockDim.x) {
+        const uint32_t e = (uint32_t)d.idx[k];
+        if (!(e & FORCED_BIT)) continue;
+        const int32_t i = (int32_t)(e & ~FORCED_BIT);
+        d.refr[i] = d.forcedRefr;
+        d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+        d.forced[i] = 1;
+    }
 }
 
 // Integrate, decay, threshold (src/lifdiff.js:313-345; replay 422-457).
@@ -637,8 +647,8 @@ __device__ __forceinline__ void dx_barrival(const DiffDev &d, int32_t t, int32_t
 #define FB_COMMA ,
 FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf)))
 FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf)))
-FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t, (dx_driven<false>(d, t)))
-FB_DIFF_KERNELS(driven_r, 1024, FB_COMMA int32_t t, (dx_driven<true>(d, t)))
+FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t, (dx_driven(d, t)))
+FB_DIFF_KERNELS(driven_r, 256, FB_COMMA int32_t t, (dx_forced_replay(d, t)))
 FB_DIFF_KERNELS(membrane_f, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<false>(d, t, r)))
 FB_DIFF_KERNELS(membrane_r, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<true>(d, t, r)))
 FB_DIFF_KERNELS(restore, 256, FB_COMMA int32_t seg, (dx_restore(d, seg)))
@@ -1225,7 +1235,7 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
         hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, st, d, sgi);
         for (int32_t t = from; t < to; t++) {
             hipLaunchKernelGGL(kd_deliver_r, dim3(256), dim3(256), 0, st, d, t, resFromT);
-            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(1024), 0, st, d, t);
+            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(256), 0, st, d, t);
             hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
         }
         for (int32_t t = to - 1; t >= from; t--) {
@@ -1384,7 +1394,7 @@ extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate)
         hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, from, sgi);
         for (int32_t t = from; t < to; t++) {
             hipLaunchKernelGGL(kdb_deliver_r, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, resFromT);
-            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
+            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t);
             hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t,
                                t - from);
         }
