@@ -283,6 +283,77 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
     if (tid == 0) *d.rng = base;
 }
 
+// The drive pre-pass with at most 2048 driven neurons (one workgroup of 1024
+// threads): each thread holds its at most two consecutive driven neurons in
+// registers, finds its first stream position with a scan of non-refractory
+// flags, and jumps there with two leapfrog products read from the global
+// tables (M^(32 (off/32)), then M^(off mod 32 + 1)); the draws after it are
+// single steps.  Same draws, effects and tape order as dx_driven.
+__device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
+    constexpr int T = 1024, DP = 2;
+    __shared__ int32_t scan[2][T / 64];
+    __shared__ unsigned long long s_top;
+    __shared__ uint32_t s_rng;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_rng = *d.rng;
+        s_top = *d.top;
+        d.off[t] = (int64_t)s_top;      // this step's list starts here
+    }
+    const int32_t nD = d.nDriven, P = (nD + T - 1) / T;
+    int32_t ii[DP];
+    uint32_t nr = 0;   // bit u: driven neuron u of this thread is not refractory and draws
+#pragma unroll
+    for (int u = 0; u < DP; u++) {
+        const int32_t q = tid * P + u;
+        ii[u] = (u < P && q < nD) ? d.driven[q] : -1;
+        if (ii[u] >= 0 && !(d.refr[ii[u]] > 0)) nr |= 1u << u;
+    }
+    __syncthreads();
+    int32_t n;
+    const int32_t off = wg_scan_flags<T, DP>(nr, scan[0], &n);
+    uint32_t fm = 0;   // bit u: neuron u was forced to spike
+    if (nr) {
+        uint32_t x = matvec(d.leap + (off & 31) * LEAP_WORDS, matvec(d.leap + (32 + (off >> 5)) * LEAP_WORDS, s_rng));
+        int32_t j = off;
+        bool first = true;
+#pragma unroll
+        for (int u = 0; u < DP; u++) {
+            if (!((nr >> u) & 1u)) continue;
+            if (!first) x = xs32(x);
+            first = false;
+            const int32_t i = ii[u];
+            const double uu = (double)(x >> 8) * (1.0 / 16777216.0);
+            if (uu < (double)d.drive[i] * d.dtS) {
+                d.refr[i] = d.forcedRefr;
+                d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+                d.forced[i] = 1;
+                d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
+                if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+                const int32_t rk = d.readIdx[i];
+                if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
+                fm |= 1u << u;
+            }
+            if (j == n - 1) *d.rng = x;   // the step's last draw is the stream state after it
+            j++;
+        }
+    }
+    int32_t nf;
+    int32_t pos = wg_scan_flags<T, DP>(fm, scan[1], &nf);
+#pragma unroll
+    for (int u = 0; u < DP; u++) {
+        if (!((fm >> u) & 1u)) continue;
+        const int64_t k = (int64_t)s_top + pos++;
+        if (k < d.cap) {
+            d.idx[k] = (int32_t)((uint32_t)ii[u] | FORCED_BIT);
+            d.amp[k] = 1.f;
+        } else {
+            atomicOr(d.err, DERR_TAPE);
+        }
+    }
+    if (tid == 0) *d.top = s_top + (unsigned long long)nf;
+}
+
 // The drive pre-pass when replaying a segment: the forward put every forced
 // spike of step t on the tape (FORCED_BIT, at the head of the step's list),
 // so the replay applies their effects without drawing the stream again.
@@ -647,7 +718,8 @@ __device__ __forceinline__ void dx_barrival(const DiffDev &d, int32_t t, int32_t
 #define FB_COMMA ,
 FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf)))
 FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf)))
-FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t, (dx_driven(d, t)))
+FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t,
+                (d.nDriven <= 2048 ? dx_driven_fast(d, t) : dx_driven(d, t)))
 FB_DIFF_KERNELS(driven_r, 256, FB_COMMA int32_t t, (dx_forced_replay(d, t)))
 FB_DIFF_KERNELS(membrane_f, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<false>(d, t, r)))
 FB_DIFF_KERNELS(membrane_r, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<true>(d, t, r)))
