@@ -809,6 +809,13 @@ struct fb_diff {
     double fwdMs = 0.0, bwdMs = 0.0, replayLaunches = 0.0;
     int32_t nblk = 0;
     std::vector<void *> runAllocs;    // freed and reallocated by forward()
+    // the backward pass replays segment k-1 on its own stream while the
+    // reverse sweep of segment k runs, so it keeps two arenas (by segment parity)
+    float *arena2 = nullptr;
+    uint8_t *mode2 = nullptr;
+    hipStream_t replayStream = nullptr;
+    hipEvent_t replayDone[2] = {nullptr, nullptr}, sweepDone[2] = {nullptr, nullptr}, fork = nullptr;
+    int64_t arenaFloats = 0;          // floats of one arena (the modes are K N bytes more)
 };
 
 #define FB_REQUIRE_D(cond, code, ...)    \
@@ -974,6 +981,7 @@ static void free_run(fb_diff *f) {
     f->ckpt = nullptr;
     f->d.off = nullptr; f->d.idx = nullptr; f->d.amp = nullptr; f->d.readS = nullptr;
     f->d.aV = nullptr; f->d.rate = nullptr; f->d.dLds = nullptr;
+    f->arena2 = nullptr; f->mode2 = nullptr;
     f->recorded = false;
 }
 
@@ -986,6 +994,14 @@ extern "C" fb_status fb_diff_destroy(fb_diff *f) {
     (void)hipFree((void *)f->d.target);
     (void)hipFree(f->readPart);
     (void)hipStreamDestroy(f->ownStream);
+    if (f->replayStream) {
+        (void)hipStreamDestroy(f->replayStream);
+        for (int p = 0; p < 2; p++) {
+            (void)hipEventDestroy(f->replayDone[p]);
+            (void)hipEventDestroy(f->sweepDone[p]);
+        }
+        (void)hipEventDestroy(f->fork);
+    }
     delete f;
     return FB_OK;
 }
@@ -1250,6 +1266,7 @@ static fb_status bwd_prepare(fb_diff *f, hipStream_t st) {
             return s;
         d.aV = ar; d.aGE = ar + (size_t)K * N; d.aGI = ar + 2 * (size_t)K * N; d.aAd = ar + 3 * (size_t)K * N;
         d.aSp = ar + 4 * (size_t)K * N; d.resAt = ar + 5 * (size_t)K * N; d.aMode = md;
+        f->arenaFloats = 5 * (int64_t)K * N + resCap;
     }
     hipLaunchKernelGGL(kd_bclear, dim3(grid_of(N, 256)), dim3(256), 0, st, d, 1);
     hipLaunchKernelGGL(kd_bzero, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d, f->nblk);
@@ -1287,35 +1304,80 @@ extern "C" fb_status fb_diff_grads(fb_diff *f, double *params, float *gLogGain, 
     return FB_OK;
 }
 
+// The same description with the arena of segment parity 1.
+static DiffDev with_arena2(const fb_diff *f) {
+    DiffDev d = f->d;
+    const int64_t KN = (int64_t)d.K * f->N;
+    float *ar = f->arena2;
+    d.aV = ar; d.aGE = ar + KN; d.aGI = ar + 2 * KN; d.aAd = ar + 3 * KN; d.aSp = ar + 4 * KN; d.resAt = ar + 5 * KN;
+    d.aMode = f->mode2;
+    return d;
+}
+
 extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *params, float *gLogGain, float *gEdge,
                                       float *gInScale, int32_t *clamped) {
     FB_REQUIRE_D(f && truncate >= 0, FB_ERR_ARG, "fb_diff_backward: bad arguments");
-    DiffDev &d = f->d;
-    const int32_t N = f->N, T = f->steps, K = d.K, D = d.D;
+    const int32_t N = f->N, T = f->steps, K = f->d.K, D = f->d.D;
     hipStream_t st = f->stream;
     fb_status s;
     hipEvent_t e0, e1;
     FB_HIP(hipEventCreate(&e0));
     FB_HIP(hipEventCreate(&e1));
     if ((s = bwd_prepare(f, st))) return s;
+    if (!f->arena2) {
+        if ((s = dalloc(f->runAllocs, &f->arena2, (size_t)f->arenaFloats)) ||
+            (s = dalloc(f->runAllocs, &f->mode2, (size_t)K * N)))
+            return s;
+    }
+    if (!f->replayStream) {
+        FB_HIP(hipStreamCreateWithFlags(&f->replayStream, hipStreamNonBlocking));
+        for (int p = 0; p < 2; p++) {
+            FB_HIP(hipEventCreateWithFlags(&f->replayDone[p], hipEventDisableTiming));
+            FB_HIP(hipEventCreateWithFlags(&f->sweepDone[p], hipEventDisableTiming));
+        }
+        FB_HIP(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming));
+    }
+    const DiffDev dp[2] = {f->d, with_arena2(f)};   // by segment parity
+    hipStream_t rs = f->replayStream;
     FB_HIP(hipEventRecord(e0, st));
+    FB_HIP(hipEventRecord(f->fork, st));
+    FB_HIP(hipStreamWaitEvent(rs, f->fork, 0));
     const int gN = grid_of(N, 256);
     const int32_t nseg = (T + K - 1) / K;
+    // segment sg: restore its checkpoint and re-run it forward, filling the arena of its parity
+    auto replay = [&](int32_t sg) {
+        const DiffDev &d = dp[sg & 1];
+        const int32_t from = sg * K, to = std::min(T, from + K);
+        const int32_t resFromT = std::max(from, D) - D;
+        hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, rs, d, sg);
+        for (int32_t t = from; t < to; t++) {
+            hipLaunchKernelGGL(kd_deliver_r, dim3(256), dim3(256), 0, rs, d, t, resFromT);
+            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(256), 0, rs, d, t);
+            hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, rs, d, t, t - from);
+        }
+    };
+    if (nseg > 0) {
+        replay(nseg - 1);
+        FB_HIP(hipEventRecord(f->replayDone[(nseg - 1) & 1], rs));
+    }
     for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        // the next replay overlaps this sweep; it reuses the arena of sweep sgi + 1
+        if (sgi > 0) {
+            if (sgi + 1 < nseg) FB_HIP(hipStreamWaitEvent(rs, f->sweepDone[(sgi + 1) & 1], 0));
+            replay(sgi - 1);
+            FB_HIP(hipEventRecord(f->replayDone[(sgi - 1) & 1], rs));
+        }
+        const DiffDev &d = dp[sgi & 1];
         const int32_t from = sgi * K, to = std::min(T, from + K);
         const int32_t resFromT = std::max(from, D) - D;
-        hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, st, d, sgi);
-        for (int32_t t = from; t < to; t++) {
-            hipLaunchKernelGGL(kd_deliver_r, dim3(256), dim3(256), 0, st, d, t, resFromT);
-            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(256), 0, st, d, t);
-            hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
-        }
+        FB_HIP(hipStreamWaitEvent(st, f->replayDone[sgi & 1], 0));
         for (int32_t t = to - 1; t >= from; t--) {
             if (truncate > 0 && (t + 1) % truncate == 0)
                 hipLaunchKernelGGL(kd_bclear, dim3(gN), dim3(256), 0, st, d, 1);
             hipLaunchKernelGGL(kd_bneuron, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, t - from);
             hipLaunchKernelGGL(kd_barrival, dim3(256), dim3(256), 0, st, d, t, resFromT);
         }
+        FB_HIP(hipEventRecord(f->sweepDone[sgi & 1], st));
         FB_HIP(hipGetLastError());
     }
     if ((s = bwd_finish_launch(f, st))) return s;
