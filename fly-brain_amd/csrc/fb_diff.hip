@@ -153,11 +153,11 @@ __device__ __forceinline__ float clip_to(double x, double c) {
 // The arrivals of step t: the tape's spikes of step t - D.  resBase is the
 // tape index of the replay segment's first arrival.
 template <bool REPLAY>
-__device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t resFromT) {
+__device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t resFromT, int32_t nblk) {
     if (t < d.D) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t nwaves = ((int64_t)nblk * blockDim.x) >> 6;
     // with a one-step delay the list being read is the previous step's, whose
     // end the tape counter still holds (its offset is written by kd_driven)
     // (an overflowed tape, already flagged, is read only up to its capacity)
@@ -702,6 +702,10 @@ __device__ __forceinline__ void dx_barrival(const DiffDev &d, int32_t t, int32_t
 
 // ------------------------------------------------------------------ kernels
 //
+// Workgroups of 1024 threads that deliver in the fused forward launch (the
+// same number of waves as the 256 x 256 delivery launch).
+constexpr int DELIVER_WG = 64;
+//
 // Each step function runs as a kernel over one network (kd_*, the network by
 // value) or over several networks in lockstep (kdb_*, blockIdx.y = network,
 // descriptions in device memory; network y takes part while t < tEnd[y]).
@@ -716,8 +720,15 @@ __device__ __forceinline__ void dx_barrival(const DiffDev &d, int32_t t, int32_t
     }
 
 #define FB_COMMA ,
-FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf)))
-FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf)))
+FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf, gridDim.x)))
+FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf, gridDim.x)))
+// Delivery and the drive pre-pass of one forward step in one launch: they
+// touch disjoint state when the delay is at least two steps (delivery reads
+// lists of steps t-D and t-D+1 and the sources' res; the pre-pass reads the
+// driven neurons' refr and appends step t's list).  The last workgroup runs the
+// pre-pass (at most 2048 driven neurons), the others deliver.
+FB_DIFF_KERNELS(deliver_driven_f, 1024, FB_COMMA int32_t t,
+                (blockIdx.x == gridDim.x - 1 ? dx_driven_fast(d, t) : dx_deliver<false>(d, t, 0, gridDim.x - 1)))
 FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t,
                 (d.nDriven <= 2048 ? dx_driven_fast(d, t) : dx_driven(d, t)))
 FB_DIFF_KERNELS(driven_r, 256, FB_COMMA int32_t t, (dx_forced_replay(d, t)))
@@ -1226,10 +1237,15 @@ extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int
     FB_HIP(hipEventCreate(&e1));
     FB_HIP(hipEventRecord(e0, st));
     const int gN = grid_of(N, 256);
+    const bool fused = d.D >= 2 && d.nDriven <= 2048;
     for (int32_t t = 0; t < steps; t++) {
         if (record && t % d.K == 0) hipLaunchKernelGGL(kd_save, dim3(gN), dim3(256), 0, st, d, t / d.K);
-        hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, d, t, 0);
-        hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, d, t);
+        if (fused) {
+            hipLaunchKernelGGL(kd_deliver_driven_f, dim3(DELIVER_WG + 1), dim3(1024), 0, st, d, t);
+        } else {
+            hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, d, t, 0);
+            hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, d, t);
+        }
         hipLaunchKernelGGL(kd_membrane_f, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, 0);
     }
     FB_HIP(hipGetLastError());
@@ -1304,6 +1320,16 @@ extern "C" fb_status fb_diff_grads(fb_diff *f, double *params, float *gLogGain, 
     return FB_OK;
 }
 
+// The arena of segment parity 1 (allocated with the run's buffers).
+static fb_status alloc_arena2(fb_diff *f) {
+    fb_status s;
+    if (!f->arena2 &&
+        ((s = dalloc(f->runAllocs, &f->arena2, (size_t)f->arenaFloats)) ||
+         (s = dalloc(f->runAllocs, &f->mode2, (size_t)f->d.K * f->N))))
+        return s;
+    return FB_OK;
+}
+
 // The same description with the arena of segment parity 1.
 static DiffDev with_arena2(const fb_diff *f) {
     DiffDev d = f->d;
@@ -1323,12 +1349,7 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
     hipEvent_t e0, e1;
     FB_HIP(hipEventCreate(&e0));
     FB_HIP(hipEventCreate(&e1));
-    if ((s = bwd_prepare(f, st))) return s;
-    if (!f->arena2) {
-        if ((s = dalloc(f->runAllocs, &f->arena2, (size_t)f->arenaFloats)) ||
-            (s = dalloc(f->runAllocs, &f->mode2, (size_t)K * N)))
-            return s;
-    }
+    if ((s = bwd_prepare(f, st)) || (s = alloc_arena2(f))) return s;
     if (!f->replayStream) {
         FB_HIP(hipStreamCreateWithFlags(&f->replayStream, hipStreamNonBlocking));
         for (int p = 0; p < 2; p++) {
@@ -1401,6 +1422,8 @@ struct fb_diff_batch {
     std::vector<int32_t> steps;
     double fwdMs = 0.0, bwdMs = 0.0;
     int32_t maxBlk = 0;
+    int32_t *segEnds = nullptr;         // backward: every segment's per-member ends [nseg][B]
+    int64_t segEndsCap = 0;
 };
 
 extern "C" fb_status fb_diff_batch_create(fb_diff *const *members, int32_t B, fb_diff_batch **out) {
@@ -1428,6 +1451,7 @@ extern "C" fb_status fb_diff_batch_destroy(fb_diff_batch *bt) {
     (void)hipFree(bt->ds);
     (void)hipFree(bt->tEnd);
     (void)hipStreamDestroy(bt->stream);
+    (void)hipFree(bt->segEnds);
     delete bt;
     return FB_OK;
 }
@@ -1468,10 +1492,17 @@ extern "C" fb_status fb_diff_batch_forward(fb_diff_batch *bt, const int32_t *ste
     FB_HIP(hipEventCreate(&e0));
     FB_HIP(hipEventCreate(&e1));
     FB_HIP(hipEventRecord(e0, st));
+    bool fused = bt->m[0]->d.D >= 2;
+    for (fb_diff *f : bt->m) fused = fused && f->d.nDriven <= 2048;
     for (int32_t t = 0; t < Tmax; t++) {
         if (record && t % K == 0) hipLaunchKernelGGL(kdb_save, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t / K);
-        hipLaunchKernelGGL(kdb_deliver_f, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, 0);
-        hipLaunchKernelGGL(kdb_driven_f, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
+        if (fused) {
+            hipLaunchKernelGGL(kdb_deliver_driven_f, dim3(DELIVER_WG + 1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t,
+                               t);
+        } else {
+            hipLaunchKernelGGL(kdb_deliver_f, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, 0);
+            hipLaunchKernelGGL(kdb_driven_f, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
+        }
         hipLaunchKernelGGL(kdb_membrane_f, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t, 0);
     }
     FB_HIP(hipGetLastError());
@@ -1491,6 +1522,11 @@ extern "C" fb_status fb_diff_batch_forward(fb_diff_batch *bt, const int32_t *ste
 
 // Backward passes of every member in lockstep; each member's gradients are
 // then read with fb_diff_grads / fb_diff_type_grads.
+// The members' backward passes in lockstep.  Unlike the single network's
+// backward, the replay of a segment does not overlap the sweep of the next:
+// these kernels are bound by HBM bandwidth, and running both at once was
+// slower (379 ms sequential against 440 ms overlapped, 32 members of the
+// full-scale workload).
 extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate) {
     FB_REQUIRE_D(bt && truncate >= 0, FB_ERR_ARG, "fb_diff_batch_backward: bad arguments");
     const int32_t B = (int32_t)bt->m.size();
@@ -1508,36 +1544,50 @@ extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate)
     if ((s = batch_upload(bt, none))) return s;
     const int32_t K = bt->m[0]->d.K, D = bt->m[0]->d.D;
     const int gN = grid_of(Nmax, 256);
-    hipEvent_t e0, e1;
-    FB_HIP(hipEventCreate(&e0));
-    FB_HIP(hipEventCreate(&e1));
-    FB_HIP(hipEventRecord(e0, st));
     const int32_t nseg = (Tmax + K - 1) / K;
-    std::vector<int32_t> ends((size_t)B);
-    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+    // every segment's per-member ends, uploaded once
+    std::vector<int32_t> ends((size_t)std::max(1, nseg) * B);
+    std::vector<int32_t> segTo((size_t)std::max(1, nseg));
+    for (int32_t sgi = 0; sgi < nseg; sgi++) {
         const int32_t from = sgi * K;
         int32_t to = from;
         // member b takes part in this segment up to its own end
         for (int32_t b = 0; b < B; b++) {
-            ends[b] = std::min(bt->steps[b], from + K);
-            if (ends[b] < from) ends[b] = from;
-            to = std::max(to, ends[b]);
+            int32_t e = std::min(bt->steps[b], from + K);
+            if (e < from) e = from;
+            ends[(size_t)sgi * B + b] = e;
+            to = std::max(to, e);
         }
-        if ((s = batch_ends(bt, ends))) return s;
+        segTo[sgi] = to;
+    }
+    if (bt->segEndsCap < (int64_t)ends.size()) {
+        if (bt->segEnds) FB_HIP(hipFree(bt->segEnds));
+        FB_HIP(hipMalloc(&bt->segEnds, sizeof(int32_t) * ends.size()));
+        bt->segEndsCap = (int64_t)ends.size();
+    }
+    FB_HIP(hipMemcpyAsync(bt->segEnds, ends.data(), sizeof(int32_t) * ends.size(), hipMemcpyHostToDevice, st));
+    FB_HIP(hipStreamSynchronize(st));   // the host vector goes out of scope with this call
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    FB_HIP(hipEventRecord(e0, st));
+    for (int32_t sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int32_t *te = bt->segEnds + (size_t)sgi * B;
+        const int32_t from = sgi * K, to = segTo[sgi];
         const int32_t resFromT = std::max(from, D) - D;
-        hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, from, sgi);
+        hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, te, from, sgi);
         for (int32_t t = from; t < to; t++) {
-            hipLaunchKernelGGL(kdb_deliver_r, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, resFromT);
-            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t);
-            hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t,
+            hipLaunchKernelGGL(kdb_deliver_r, dim3(256, B), dim3(256), 0, st, bt->ds, te, t, t, resFromT);
+            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(256), 0, st, bt->ds, te, t, t);
+            hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, te, t, t,
                                t - from);
         }
         for (int32_t t = to - 1; t >= from; t--) {
             if (truncate > 0 && (t + 1) % truncate == 0)
-                hipLaunchKernelGGL(kdb_bclear, dim3(gN, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, 1);
-            hipLaunchKernelGGL(kdb_bneuron, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t,
+                hipLaunchKernelGGL(kdb_bclear, dim3(gN, B), dim3(256), 0, st, bt->ds, te, t, 1);
+            hipLaunchKernelGGL(kdb_bneuron, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, te, t, t,
                                t - from);
-            hipLaunchKernelGGL(kdb_barrival, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, resFromT);
+            hipLaunchKernelGGL(kdb_barrival, dim3(256, B), dim3(256), 0, st, bt->ds, te, t, t, resFromT);
         }
         FB_HIP(hipGetLastError());
     }

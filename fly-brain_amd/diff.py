@@ -211,13 +211,15 @@ def autotune_checkpoint(N: int, steps: int, members: int = 1, budget_bytes: floa
     steps: minimises members * [(steps/K + 1) * S_ckpt + K * S_arena], the
     checkpoints (v, gE, gI, adapt, res, refr and the stream: 24 N + 4 bytes)
     against one segment's arena (v, gE, gI, adapt, spike amplitude and a mode
-    byte: 21 N bytes per step), i.e. K = sqrt(steps * S_ckpt / S_arena).
-    Raises when even that K exceeds the budget."""
+    byte: 21 N bytes per step), i.e. K = sqrt(steps * S_ckpt / S_arena).  The
+    budget check counts two arenas, as a single network's backward keeps one
+    segment's replay beside the sweep of the next.  Raises when the plan
+    exceeds the budget."""
     import math
     s_ckpt = 24.0 * N + 4.0
     s_arena = 21.0 * N
     k = max(1, min(steps, int(round(math.sqrt(max(1, steps) * s_ckpt / s_arena)))))
-    need = members * ((steps / k + 1) * s_ckpt + k * s_arena)
+    need = members * ((steps / k + 1) * s_ckpt + 2 * k * s_arena)
     if need > budget_bytes:
         raise ValueError(f"checkpoint plan needs {need / 1e9:.1f} GB > budget {budget_bytes / 1e9:.1f} GB")
     return k
