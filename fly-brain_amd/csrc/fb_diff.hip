@@ -721,7 +721,6 @@ constexpr int DELIVER_WG = 64;
 
 #define FB_COMMA ,
 FB_DIFF_KERNELS(deliver_f, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<false>(d, t, rf, gridDim.x)))
-FB_DIFF_KERNELS(deliver_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf, (dx_deliver<true>(d, t, rf, gridDim.x)))
 // Delivery and the drive pre-pass of one forward step in one launch: they
 // touch disjoint state when the delay is at least two steps (delivery reads
 // lists of steps t-D and t-D+1 and the sources' res; the pre-pass reads the
@@ -731,7 +730,10 @@ FB_DIFF_KERNELS(deliver_driven_f, 1024, FB_COMMA int32_t t,
                 (blockIdx.x == gridDim.x - 1 ? dx_driven_fast(d, t) : dx_deliver<false>(d, t, 0, gridDim.x - 1)))
 FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t,
                 (d.nDriven <= 2048 ? dx_driven_fast(d, t) : dx_driven(d, t)))
-FB_DIFF_KERNELS(driven_r, 256, FB_COMMA int32_t t, (dx_forced_replay(d, t)))
+// Replay: delivery and the forced spikes from the tape in one launch (disjoint
+// state); the last workgroup applies the forced spikes.
+FB_DIFF_KERNELS(deliver_forced_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf,
+                (blockIdx.x == gridDim.x - 1 ? dx_forced_replay(d, t) : dx_deliver<true>(d, t, rf, gridDim.x - 1)))
 FB_DIFF_KERNELS(membrane_f, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<false>(d, t, r)))
 FB_DIFF_KERNELS(membrane_r, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<true>(d, t, r)))
 FB_DIFF_KERNELS(restore, 256, FB_COMMA int32_t seg, (dx_restore(d, seg)))
@@ -1372,8 +1374,7 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
         const int32_t resFromT = std::max(from, D) - D;
         hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, rs, d, sg);
         for (int32_t t = from; t < to; t++) {
-            hipLaunchKernelGGL(kd_deliver_r, dim3(256), dim3(256), 0, rs, d, t, resFromT);
-            hipLaunchKernelGGL(kd_driven_r, dim3(1), dim3(256), 0, rs, d, t);
+            hipLaunchKernelGGL(kd_deliver_forced_r, dim3(257), dim3(256), 0, rs, d, t, resFromT);
             hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, rs, d, t, t - from);
         }
     };
@@ -1577,8 +1578,7 @@ extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate)
         const int32_t resFromT = std::max(from, D) - D;
         hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, te, from, sgi);
         for (int32_t t = from; t < to; t++) {
-            hipLaunchKernelGGL(kdb_deliver_r, dim3(256, B), dim3(256), 0, st, bt->ds, te, t, t, resFromT);
-            hipLaunchKernelGGL(kdb_driven_r, dim3(1, B), dim3(256), 0, st, bt->ds, te, t, t);
+            hipLaunchKernelGGL(kdb_deliver_forced_r, dim3(257, B), dim3(256), 0, st, bt->ds, te, t, t, resFromT);
             hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, te, t, t,
                                t - from);
         }
