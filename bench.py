@@ -304,6 +304,48 @@ def bench_adjoint_batch(args, synth, P, diff, hiprt):
     return out
 
 
+def bench_cord(args, lif, hiprt, stream):
+    """The reference's own ventral-cord brain (21,843 cells, 1.6 M entries,
+    calibrated: COBA, dt 0.5 ms, adaptation on), built from its data files
+    (tests/golden/make_cord_fixture.py).  The first 1,000 steps must give the
+    reference kernel's published total (139,582 spikes); then the steady-state
+    rate of the lookahead engine.  Skipped when the fixture is absent."""
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", "_cord", "cord_brains.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path, allow_pickle=False)
+    p = json.loads(str(z["params"]))
+    N = int(z["indptr"].size - 1)
+    g = lif.HipGraph(N, z["indptr"], z["indices"], z["W"], z["sign"])
+    b = lif.LIFHip(g, p, seed=1, engine="persistent")
+    for i, r in zip(z["driven"].tolist(), z["rates"].tolist()):
+        b.setDriveOne(i, r)
+    b.set_stream(stream.handle)
+    b.advance(1000)
+    first = int(b.spikeCount.sum())
+    steps = max(1000, args.steps)
+    c0 = b.counters()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    e0.record(stream)
+    b.run_async(steps)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    c1 = b.counters()
+    ev = c1["edge_events"] - c0["edge_events"]
+    out = {"workload": "the reference's ventral-cord brain from its data files: 21,843 cells, 1,615,182 entries, "
+                       "calibrated parameters (COBA, dt 0.5 ms, 4-step epochs), 60 Hz to descending and 20 Hz "
+                       "to proprioceptive neurons",
+           "first_1000_steps_spikes": first, "reference_total": 139582, "matches_reference": first == 139582,
+           "steps": steps, "ms_per_step": ms / steps, "edge_events_per_s": ev / (ms / 1e3),
+           "sim_s_per_wall_s": steps * p["dt"] / 1e3 / (ms / 1e3),
+           "spikes_per_step": (c1["spikes"] - c0["spikes"]) / steps}
+    b.destroy()
+    g.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,6 +503,9 @@ def main():
     out["lookahead_fallback_steps"] = fallback
     out["lookahead_overflow_flags"] = overflow_flags
     out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
+    cord = bench_cord(args, lif, hiprt, stream)
+    if cord:
+        out["real_cord"] = cord
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:

@@ -161,6 +161,7 @@ struct fb_graph {
     int device = 0;
     DevGraph d{};
     bool rowsUnique = true;
+    bool wNonneg = true;            // no weight carries a sign bit (negative or -0)
     std::vector<void *> allocs;
     std::vector<uint32_t> indptr;   // host copy for validation
 };
@@ -179,6 +180,11 @@ struct fb_brain {
     hipStream_t stream = nullptr;      // the stream all work is queued on
     hipStream_t ownStream = nullptr;   // the brain's own stream
     std::vector<int32_t> drivenList;   // insertion order of the driven set
+    // No excitatory conductance can be -0: gE only ever received non-negative
+    // additions and decays since the last reset.  With non-negative weights a
+    // zero-weight arrival then changes nothing, and the lookahead engine may
+    // skip it (an inhibitory one, adding -0 to gI, never changes anything).
+    bool gENonneg = true;
     std::vector<uint8_t> inDriven;
     bool drivenDirty = true;
     int32_t head = 0, lastSlot = 0;
@@ -255,6 +261,8 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
     std::vector<int32_t> cnts;
     if (counts) cnts.resize((size_t)E);
     std::vector<uint32_t> perm;
+    for (int64_t j = 0; j < E; j++)
+        if (std::signbit(weights[j])) g->wNonneg = false;
     for (int32_t i = 0; i < N; i++) {
         const uint32_t a = indptr[i], b = indptr[i + 1];
         bool sorted = true;
@@ -489,6 +497,7 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
 
 extern "C" fb_status fb_reset(fb_brain *b) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_reset: null brain");
+    b->gENonneg = true;
     hipLaunchKernelGGL(k_reset_state, dim3((std::max(b->d.N, b->d.nslots) + 255) / 256), dim3(256), 0, b->stream,
                        b->d);
     FB_HIP(hipGetLastError());
@@ -596,6 +605,7 @@ extern "C" fb_status fb_set_thr(fb_brain *b, int32_t i, double mv) {
 
 extern "C" fb_status fb_add_g(fb_brain *b, int32_t i, double e, double ii) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_add_g: null brain");
+    if (std::signbit(e)) b->gENonneg = false;
     fb_status st = set_values(b, b->d.gE, &i, 1, e, 1);
     if (st) return st;
     return set_values(b, b->d.gI, &i, 1, ii, 1);
@@ -603,6 +613,7 @@ extern "C" fb_status fb_add_g(fb_brain *b, int32_t i, double e, double ii) {
 
 extern "C" fb_status fb_pulse(fb_brain *b, const int32_t *idx, int32_t n, double mv) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_pulse: null brain");
+    if (n > 0 && std::signbit(mv)) b->gENonneg = false;
     return set_values(b, b->d.gE, idx, n, mv, 1);
 }
 
@@ -691,6 +702,8 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     fb_status st = sync_driven(b);
     if (st) return st;
     if (nsteps == 0) return FB_OK;
+    // background events of negative amplitude can leave an excitatory conductance at -0
+    if (b->d.k.bgEvents > 0.f && std::signbit(b->d.k.bgAmp)) b->gENonneg = false;
     const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
         // the lookahead engine pays a prologue per call: worth it from two epochs on
@@ -706,6 +719,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             std::string why;
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
+            b->d.skipZeroW = b->g->wNonneg && b->gENonneg ? 1 : 0;
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st == FB_ERR_UNSUPPORTED && autoChosen) {
@@ -851,6 +865,12 @@ extern "C" fb_status fb_set_array(fb_brain *b, int32_t which, const void *src) {
     FB_REQUIRE(b && src, FB_ERR_ARG, "fb_set_array: null argument");
     float *dst = array_ptr(b, which);
     FB_REQUIRE(dst, FB_ERR_ARG, "fb_set_array: unknown array %d", which);
+    if (dst == b->d.gE) {
+        const float *v = (const float *)src;
+        bool nonneg = true;
+        for (int32_t i = 0; i < b->d.N; i++) nonneg = nonneg && !std::signbit(v[i]);
+        b->gENonneg = nonneg;
+    }
     FB_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)b->d.N, hipMemcpyHostToDevice, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
     return FB_OK;

@@ -77,6 +77,7 @@ struct DevGraph {
 
 struct DevBrain {
     int32_t N, nslots, nDriven, nblocks;
+    int32_t skipZeroW;         // lookahead push may skip zero-weight edges (exact: see fb_brain::gENonneg)
     int64_t ringStride;        // entries per ring slot (2N)
     Consts k;
     DevGraph g;

@@ -19,7 +19,7 @@
 //   blocks R+1..    spare blocks (when the grid has them): they turn the
 //                   previous epoch's spikes into ring lists in lif.c order.
 //
-// One grid barrier per epoch.  Within epoch e every owning block:
+// Within epoch e every owning block:
 //   gather  loads the edge events addressed to it (pushed during epoch e-1)
 //           into LDS and sorts them by (step, sign class, target, arrival
 //           key, position in row) -- lif.c's addition order per target;
@@ -30,18 +30,20 @@
 //           (a global store inside its step loop would make every workgroup
 //           barrier wait for the store to complete);
 //   push    walks the rows of its own spikes and writes their events into
-//           per-(source, destination) sub-buckets for epoch e+1, allocating
-//           slots with LDS counters (no global atomics per event).
+//           the destinations' buckets for epoch e+1 (a slot is one atomic on
+//           the destination's count).  After a grid barrier the first 64
+//           blocks push the driven block's rows the same way, then a second
+//           grid barrier ends the epoch.
 // The arrival key of a spike is its rank in lif.c's list order: a forced
 // spike's driven-list position, or 2048 + index for a threshold spike.
 //
 // A launch starts with a prologue that scatters the ring lists already in
 // the delay line (from the per-step engines or the previous launch) into the
-// sub-buckets, keyed by list position.  Preconditions (checked on the host;
+// buckets, keyed by list position.  Preconditions (checked on the host;
 // otherwise the per-step engines run): no background events, no presynaptic
 // inhibition, D <= 32, at most 2048 driven neurons and 1024 neurons per
 // block, and with synaptic depression live a refractory period of at least D
-// steps (a neuron then arrives at most once per epoch).  A sub-bucket or LDS
+// steps (a neuron then arrives at most once per epoch).  A bucket or LDS
 // overflow stops the launch at an epoch boundary; the host runs one epoch on
 // the per-step engine and relaunches.
 
@@ -60,15 +62,14 @@ constexpr int PT = 512;             // threads per block
 constexpr int NPT = 2;              // neurons per thread in a regular block
 constexpr int NB_MAX = PT * NPT;    // neurons per regular block
 constexpr int D_MAX = 32;           // epoch length bound
-constexpr int EV_CAP = 2048;        // events per destination block per epoch (LDS)
+constexpr int EV_CAP = 6144;        // events per destination block per epoch (LDS)
+constexpr int BK_STRIDE = 32;       // words between two destinations' bucket counts (their own 128-byte lines)
 constexpr int SPK_CAP = 2048;       // spikes per block per epoch (LDS push list)
 constexpr int RPT = SPK_CAP / PT;   // push rows per thread in the row scan
 constexpr int DPT = 4;              // driven neurons per thread in block 0
 constexpr int DRV_CAP = PT * DPT;
 constexpr int CELLS = DPT > NPT ? DPT : NPT;
 constexpr int S_MAX = 512;          // owning blocks (driven + regular)
-constexpr int CAP_DRV = 1024;       // sub-bucket slots from the driven block to one destination
-constexpr int CAP_REG = 512;        // sub-bucket slots from a regular block to one destination
 constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
 constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
 #ifndef FB_SHARE_BLOCKS
@@ -100,11 +101,9 @@ struct PersistArgs {
     uint32_t *tmask;     // [2][DRV_CAP] steps with a threshold spike of a driven neuron
     const int32_t *drvPos;   // [N] position in the driven list, -1 if not driven
     uint32_t *arrMask;   // [N] arrival steps of the first epoch (depression bookkeeping)
-    uint64_t *sbKey;     // [2][perPar] sub-buckets
-    float *sbAdd;
-    int64_t perPar;
-    int32_t *sbCnt;      // [2][S][S]  count from source s to destination r at [r*S + s]
-    int32_t *dstTot;     // [2][S]     events addressed to each destination
+    uint64_t *bkKey;     // [2][S][EV_CAP] each destination's bucket of the next epoch's events
+    float *bkAdd;
+    int32_t *bkCnt;      // [2][S] slots taken, one per BK_STRIDE words (zero at launch; a destination clears its own after reading)
     int32_t *ctl;        // [0] steps done, [1] overflow flags
     uint64_t *prof;      // phase clocks (see k_persist)
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
@@ -114,19 +113,26 @@ struct PersistArgs {
 };
 
 struct Shared {
-    uint64_t evKey[EV_CAP];
-    float evAdd[EV_CAP];
+    // The epoch's events live from the gather to the end of the simulation,
+    // the push list from the push to the next gather: they share the space.
+    union {
+        struct {
+            uint64_t evKey[EV_CAP];
+            float evAdd[EV_CAP];
+            uint16_t evStart[EV_CAP];    // first sorted event of the r-th (step, target) with arrivals
+        };
+        struct {
+            int32_t spkI[SPK_CAP];       // push list: neuron, step << 24 | arrival key, sign x res at arrival
+            uint32_t spkK[SPK_CAP];
+            float spkR[SPK_CAP];
+            int32_t rowOff[SPK_CAP + 1]; // first edge of each pushed row in the block's flattened edge list
+            uint32_t rowA[SPK_CAP];      // row start in the CSR
+        };
+    };
     uint32_t hasEv[D_MAX * WPS];     // bit (step, target): the target has arrivals at that step
     int32_t evRank[D_MAX * WPS];     // set bits of hasEv before each word
-    uint16_t evStart[EV_CAP];        // first sorted event of the r-th (step, target) with arrivals
     uint32_t leap[LEAP_LDS_WORDS];
-    int32_t pref[S_MAX + 1];
-    int32_t cnt[S_MAX];          // push slots per destination
-    int32_t spkI[SPK_CAP];       // push list: neuron, step << 24 | arrival key, sign x res at arrival
-    uint32_t spkK[SPK_CAP];
-    float spkR[SPK_CAP];
-    int32_t rowOff[SPK_CAP + 1]; // first edge of each pushed row in the block's flattened edge list
-    uint32_t rowA[SPK_CAP];      // row start in the CSR
+    int32_t pref[D_MAX + 1];     // prologue: first ring entry of each arrival step
     int32_t scan[64 + PT / 64];
     uint32_t rngNext[2];
     int32_t nev, nspk;
@@ -145,19 +151,31 @@ struct Shared {
 
 // ------------------------------------------------------------------ helpers
 
+// Bitonic sort of evKey/evAdd[0, n) for any n: every comparator sorts
+// ascending (the first stage of each merge compares an element with its
+// mirror in the block), so the missing elements n .. P-1 act as +infinity
+// and never need storage.
 __device__ inline void lds_sort(Shared &s, int32_t n) {
     int32_t P = 1;
     while (P < n) P <<= 1;
-    for (int32_t i = n + threadIdx.x; i < P; i += PT) { s.evKey[i] = ~0ull; s.evAdd[i] = 0.f; }
-    __syncthreads();
     for (int32_t k = 2; k <= P; k <<= 1) {
-        for (int32_t j = k >> 1; j > 0; j >>= 1) {
-            for (int32_t i = threadIdx.x; i < P; i += PT) {
+        for (int32_t i = threadIdx.x; i < n; i += PT) {
+            const int32_t l = i ^ (k - 1);
+            if (l > i && l < n) {
+                const uint64_t ki = s.evKey[i], kl = s.evKey[l];
+                if (ki > kl) {
+                    s.evKey[i] = kl; s.evKey[l] = ki;
+                    const float t = s.evAdd[i]; s.evAdd[i] = s.evAdd[l]; s.evAdd[l] = t;
+                }
+            }
+        }
+        __syncthreads();
+        for (int32_t j = k >> 2; j > 0; j >>= 1) {
+            for (int32_t i = threadIdx.x; i < n; i += PT) {
                 const int32_t l = i ^ j;
-                if (l > i) {
-                    const bool up = (i & k) == 0;
+                if (l > i && l < n) {
                     const uint64_t ki = s.evKey[i], kl = s.evKey[l];
-                    if ((ki > kl) == up) {
+                    if (ki > kl) {
                         s.evKey[i] = kl; s.evKey[l] = ki;
                         const float t = s.evAdd[i]; s.evAdd[i] = s.evAdd[l]; s.evAdd[l] = t;
                     }
@@ -169,8 +187,8 @@ __device__ inline void lds_sort(Shared &s, int32_t n) {
 }
 
 // Sorts evKey/evAdd[0, n) (keys are distinct).  Up to PT events: each thread
-// counts the smaller keys of its event and scatters it to that rank, through
-// the staging halves of the push list; beyond that, a bitonic sort.
+// counts the smaller keys of its event and writes it to that rank; beyond
+// that, a bitonic sort.
 __device__ inline void sort_events(Shared &s, int32_t n) {
     if (n > PT) { lds_sort(s, n); return; }
     const int tid = threadIdx.x;
@@ -233,12 +251,9 @@ __device__ __forceinline__ float res_after(const Consts &k, float r, int32_t m) 
 // ring head at step `t` of the launch
 __device__ __forceinline__ int32_t head_at(const PersistArgs &a, int32_t t) { return (a.head0 + t) % a.d.nslots; }
 
-__device__ __forceinline__ int64_t sb_base(const PersistArgs &a, int32_t par, int32_t src, int32_t dst) {
-    const int64_t b = src == 0 ? (int64_t)dst * CAP_DRV
-                               : (int64_t)CAP_DRV * a.S + ((int64_t)(src - 1) * a.S + dst) * CAP_REG;
-    return (int64_t)par * a.perPar + b;
+__device__ __forceinline__ int64_t bucket_at(const PersistArgs &a, int32_t par, int32_t dst) {
+    return ((int64_t)par * a.S + dst) * EV_CAP;
 }
-__device__ __forceinline__ int32_t sb_cap(int32_t src) { return src == 0 ? CAP_DRV : CAP_REG; }
 
 // Row lengths of the `ns` rows of the LDS push list -> the flattened edge
 // list: s.rowOff[r] (first edge of row r, s.rowOff[ns] = total) and s.rowA[r]
@@ -281,18 +296,14 @@ __device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns) 
 }
 
 // Writes the edge events g0 <= g < g1 of the flattened edge list (row_scan)
-// into source `src`'s sub-buckets for parity `par`, four edges per thread in
-// flight; a scan of the row starts maps an edge to its row.  Slots come from
-// this block's LDS counters, or, with SHARED (many blocks writing the driven
-// block's sub-buckets at once), from the global counts themselves; then the
-// LDS counters only tally what this block added.
-template <bool SHARED>
-__device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns, int32_t g0,
-                                  int32_t g1) {
+// into the destinations' buckets for parity `par`, four edges per thread in
+// flight; a scan of the row starts maps an edge to its row.  A slot is one
+// atomic on the destination's count; beyond EV_CAP the event is dropped and
+// flag 2 stops the launch before the epoch that would read it.
+__device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t g0, int32_t g1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    const int32_t cap = sb_cap(src);
-    int32_t *gcnt = a.sbCnt + (int64_t)par * a.S * a.S + src;   // [dst * S]
+    int32_t *cnt = a.bkCnt + (int64_t)par * a.S * BK_STRIDE;
     constexpr int U = 4;
     for (int32_t gb = g0 + tid; gb < g1; gb += PT * U) {
         uint2 ed[U];
@@ -321,45 +332,35 @@ __device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, 
             const int32_t r = row[v];
             const uint32_t kk = s.spkK[r];
             const float sv = s.spkR[r];
+            // a +0 weight adds -0 to gI, which changes nothing (x + -0 = x),
+            // and +0 to gE, which changes nothing while no excitatory
+            // conductance can be -0 (skipZeroW)
+            if (ed[v].y == 0u && (sv < 0.f || d.skipZeroW)) continue;
             const int32_t dst = (int32_t)(own[v] >> 12);
             const uint32_t ql = own[v] & 4095u;
-            int32_t pos = atomicAdd(&s.cnt[dst], 1);
-            if (SHARED) pos = atomicAdd(&gcnt[(int64_t)dst * a.S], 1);
-            if (pos < cap) {
-                const int64_t at = sb_base(a, par, src, dst) + pos;
-                a.sbKey[at] = ev_key(kk >> 24, sv > 0 ? 0u : 1u, ql, kk & 0xFFFFFFu, j[v] - s.rowA[r]);
-                a.sbAdd[at] = __uint_as_float(ed[v].y) * sv;
-            } else if (SHARED && pos == cap) {
-                atomicOr(&a.ctl[1], 1);
+            const int32_t pos = atomicAdd(&cnt[(int64_t)dst * BK_STRIDE], 1);
+            if (pos < EV_CAP) {
+                const int64_t at = bucket_at(a, par, dst) + pos;
+                a.bkKey[at] = ev_key(kk >> 24, sv > 0 ? 0u : 1u, ql, kk & 0xFFFFFFu, j[v] - s.rowA[r]);
+                a.bkAdd[at] = __uint_as_float(ed[v].y) * sv;
+            } else if (pos == EV_CAP) {
+                atomicOr(&a.ctl[1], 2);
             }
         }
     }
 }
 
-// Writes the events of the `ns` rows of the LDS push list into this block's
-// sub-buckets for parity `par`.
-__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t src, int32_t ns) {
+// Writes the events of the `ns` rows of the LDS push list into the buckets
+// for parity `par`.
+__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t ns) {
     const int32_t total = row_scan(a, s, ns);
-    push_edges<false>(a, s, par, src, ns, 0, total);
+    push_edges(a, s, par, ns, 0, total);
 }
 
-// After a push: publish this block's counts to every destination, check the
-// capacities, fold the work counters into the brain's.
-__device__ inline void publish_counts(const PersistArgs &a, Shared &s, int32_t par, int32_t src) {
-    const int tid = threadIdx.x;
+// Folds this block's work counters (row_scan) into the brain's.
+__device__ inline void publish_stats(const PersistArgs &a, Shared &s) {
     __syncthreads();
-    const int32_t cap = sb_cap(src);
-    for (int32_t r = tid; r < a.S; r += PT) {
-        const int32_t c = s.cnt[r];
-        a.sbCnt[(int64_t)par * a.S * a.S + (int64_t)r * a.S + src] = min(c, cap);
-        if (c > cap) atomicOr(&a.ctl[1], 1);
-        if (c) {
-            const int32_t tot = atomicAdd(&a.dstTot[par * a.S + r], c) + c;
-            if (tot > EV_CAP) atomicOr(&a.ctl[1], 2);
-        }
-        s.cnt[r] = 0;
-    }
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
         if (s.ev) atomicAdd(&a.d.sc->edgeEvents, s.ev);
         if (s.syn) atomicAdd(&a.d.sc->synEvents, s.syn);
         s.ev = 0;
@@ -367,7 +368,7 @@ __device__ inline void publish_counts(const PersistArgs &a, Shared &s, int32_t p
     }
 }
 
-// ------------------------------------------------------------------ prologue: ring lists -> sub-buckets
+// ------------------------------------------------------------------ prologue: ring lists -> buckets
 
 __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
     const DevBrain &d = a.d;
@@ -397,39 +398,30 @@ __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
             s.spkR[r] = sv;
         }
         __syncthreads();
-        push_rows(a, s, 0, me, ns);
+        push_rows(a, s, 0, ns);
         __syncthreads();
     }
-    publish_counts(a, s, 0, me);
+    publish_stats(a, s);
 }
 
-// ------------------------------------------------------------------ gather: sub-buckets -> sorted LDS events
+// ------------------------------------------------------------------ gather: bucket -> sorted LDS events
 
 __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
     const int tid = threadIdx.x;
-    int32_t *cnt = a.sbCnt + (int64_t)par * a.S * a.S + (int64_t)me * a.S;
-    int32_t total = 0;
-    // S <= S_MAX = PT: one count per thread.  The driven block's count is
-    // reserved by many blocks with atomics (push_driven_share): clamp it, and
-    // clear it for the push two epochs on.
-    const int32_t c = tid < a.S ? (tid == 0 ? min(cnt[0], CAP_DRV) : cnt[tid]) : 0;
-    if (tid == 0) cnt[0] = 0;
-    const int32_t off = wg_exclusive_scan<PT>(c, s.scan, &total);
-    if (tid < a.S) s.pref[tid] = off;
-    if (tid == 0) { s.pref[a.S] = total; a.dstTot[par * a.S + me] = 0; }
+    // the count is read once, then cleared for the push two epochs on
+    if (tid == 0) {
+        int32_t *cnt = a.bkCnt + ((int64_t)par * a.S + me) * BK_STRIDE;
+        s.nev = min(*cnt, EV_CAP);
+        *cnt = 0;
+    }
     for (int32_t w = tid; w < D_MAX * WPS; w += PT) s.hasEv[w] = 0u;
     __syncthreads();
-    const int32_t n = min(total, EV_CAP);
+    const int32_t n = s.nev;
+    const int64_t base = bucket_at(a, par, me);
     for (int32_t g = tid; g < n; g += PT) {
-        int32_t lo = 0, hi = a.S;   // last src with pref <= g
-        while (hi - lo > 1) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (s.pref[mid] <= g) lo = mid; else hi = mid;
-        }
-        const int64_t at = sb_base(a, par, lo, me) + (g - s.pref[lo]);
-        const uint64_t key = a.sbKey[at];
+        const uint64_t key = a.bkKey[base + g];
         s.evKey[g] = key;
-        s.evAdd[g] = a.sbAdd[at];
+        s.evAdd[g] = a.bkAdd[base + g];
         const uint32_t sl = ev_slot(key);
         const uint32_t m = sl >> 12, ql = sl & 4095u;
         atomicOr(&s.hasEv[m * WPS + (ql >> 5)], 1u << (ql & 31));
@@ -617,7 +609,7 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
 #undef FB_SUB
 }
 
-// ------------------------------------------------------------------ push: own spikes -> next epoch's sub-buckets
+// ------------------------------------------------------------------ push: own spikes -> next epoch's buckets
 
 // A regular block's spikes of the epoch -> the LDS push list (s.spkI /
 // spkK / spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
@@ -651,8 +643,8 @@ __device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)
 // A regular block pushes its own spikes.
 __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me) {
     const int32_t ns = collect_spikes(a, s, cells);
-    push_rows(a, s, par, me, ns);
-    publish_counts(a, s, par, me);
+    push_rows(a, s, par, ns);
+    publish_stats(a, s);
 }
 
 // The driven block's rows are many (every forced spike), so the first
@@ -678,7 +670,7 @@ __device__ void share_setup(const PersistArgs &a, Shared &s) {
     }
 }
 
-// `par`: the parity of the sub-buckets written (the next epoch's); `mpar`: the
+// `par`: the parity of the buckets written (the next epoch's); `mpar`: the
 // parity of the driven masks read (this epoch's).
 __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, int32_t mpar) {
     const DevBrain &d = a.d;
@@ -749,19 +741,9 @@ __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, 
     }
     __syncthreads();
     FB_SHARE(0)
-    push_edges<true>(a, s, par, 0, ns, 0, total);
+    push_edges(a, s, par, ns, 0, total);
     __syncthreads();
     FB_SHARE(1)
-    // what this block added to each destination, against the event capacity
-    for (int32_t r = tid; r < a.S; r += PT) {
-        const int32_t c = s.cnt[r];
-        if (c) {
-            const int32_t tot = atomicAdd(&a.dstTot[par * a.S + r], c) + c;
-            if (tot > EV_CAP) atomicOr(&a.ctl[1], 2);
-            s.cnt[r] = 0;
-        }
-    }
-    __syncthreads();
     FB_SHARE(2)
 #undef FB_SHARE
 }
@@ -858,7 +840,6 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         s.ph[NPH] = now;                               \
     }
 
-    for (int r = tid; r < S_MAX; r += PT) s.cnt[r] = 0;
     if (tid == 0) { s.ev = 0; s.syn = 0; s.rngNext[0] = s.rngNext[1] = 0; }
     Cell cells[CELLS];
     float pFire[DPT];
@@ -885,8 +866,6 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     __syncthreads();
 
     const int32_t L0 = min(a.D, a.nsteps);
-    // the first epoch's driven push counts into parity 1 with atomics
-    if (owning && tid == 0) a.sbCnt[(int64_t)a.S * a.S + (int64_t)me * a.S] = 0;
     if (owning) prologue_scatter(a, s, L0);
     FB_PHASE(5)
     if (!grid_barrier(a.bar, &a.ctl[1])) return;
@@ -1052,17 +1031,17 @@ struct PersistBuffers {
     uint32_t *dmask = nullptr, *tmask = nullptr;
     int32_t *drvPos = nullptr;
     uint32_t *arrMask = nullptr;
-    uint64_t *sbKey = nullptr;
-    float *sbAdd = nullptr;
-    int32_t *sbCnt = nullptr, *dstTot = nullptr;
+    uint64_t *bkKey = nullptr;
+    float *bkAdd = nullptr;
+    int32_t *bkCnt = nullptr;
     int32_t *ctl = nullptr;
     uint64_t *prof = nullptr;
     uint32_t *eown = nullptr;
     float *dres = nullptr;
     int32_t *bar = nullptr;
-    uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (1 sub-bucket, 2 events, 4 spikes)
+    uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
     int32_t N = 0, S = 0;
-    int64_t perPar = 0, E = 0;
+    int64_t E = 0;
     int32_t eownNb = -1;
     uint64_t eownDriven = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1071,12 +1050,12 @@ struct PersistBuffers {
 static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain's driven buffer
 
 static void free_buffers(PersistBuffers &pb) {
-    void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.sbKey, pb.sbAdd, pb.sbCnt, pb.dstTot,
+    void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.bkKey, pb.bkAdd, pb.bkCnt,
                   pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
-    pb.sbKey = nullptr; pb.sbAdd = nullptr; pb.sbCnt = pb.dstTot = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
+    pb.bkKey = nullptr; pb.bkAdd = nullptr; pb.bkCnt = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
     pb.dres = nullptr;
     pb.bar = nullptr;
@@ -1125,12 +1104,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     layout_for(d, D, lay);
     PersistBuffers &pb = g_pbuf[d.driven];
     const int32_t nwords = (d.N + 63) / 64;
-    const int64_t perPar = (int64_t)CAP_DRV * lay.S + (int64_t)CAP_REG * (lay.S - 1) * lay.S;
     if (pb.N != d.N || pb.S != lay.S) {
         free_buffers(pb);
         pb.N = d.N;
         pb.S = lay.S;
-        pb.perPar = perPar;
         FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords));
         FB_HIP(hipMemsetAsync(pb.spk, 0, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords, st));
         FB_HIP(hipMalloc(&pb.dmask, sizeof(uint32_t) * 2 * DRV_CAP));
@@ -1138,10 +1115,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.drvPos, sizeof(int32_t) * (size_t)d.N));
         FB_HIP(hipMalloc(&pb.arrMask, sizeof(uint32_t) * (size_t)d.N));
         FB_HIP(hipMemsetAsync(pb.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
-        FB_HIP(hipMalloc(&pb.sbKey, sizeof(uint64_t) * 2 * (size_t)perPar));
-        FB_HIP(hipMalloc(&pb.sbAdd, sizeof(float) * 2 * (size_t)perPar));
-        FB_HIP(hipMalloc(&pb.sbCnt, sizeof(int32_t) * 2 * (size_t)lay.S * lay.S));
-        FB_HIP(hipMalloc(&pb.dstTot, sizeof(int32_t) * 2 * (size_t)lay.S));
+        FB_HIP(hipMalloc(&pb.bkKey, sizeof(uint64_t) * 2 * (size_t)lay.S * EV_CAP));
+        FB_HIP(hipMalloc(&pb.bkAdd, sizeof(float) * 2 * (size_t)lay.S * EV_CAP));
+        FB_HIP(hipMalloc(&pb.bkCnt, sizeof(int32_t) * 2 * (size_t)lay.S * BK_STRIDE));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * PERSIST_PROF));
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
@@ -1178,11 +1154,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.tmask = pb.tmask;
     a.drvPos = pb.drvPos;
     a.arrMask = pb.arrMask;
-    a.sbKey = pb.sbKey;
-    a.sbAdd = pb.sbAdd;
-    a.perPar = perPar;
-    a.sbCnt = pb.sbCnt;
-    a.dstTot = pb.dstTot;
+    a.bkKey = pb.bkKey;
+    a.bkAdd = pb.bkAdd;
+    a.bkCnt = pb.bkCnt;
     a.ctl = pb.ctl;
     a.prof = pb.prof;
     a.eown = pb.eown;
@@ -1198,7 +1172,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
         FB_HIP(hipMemsetAsync(pb.bar, 0, sizeof(int32_t) * GBAR_WORDS, st));
         FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * PERSIST_PROF, st));
-        FB_HIP(hipMemsetAsync(pb.dstTot, 0, sizeof(int32_t) * 2 * (size_t)lay.S, st));
+        FB_HIP(hipMemsetAsync(pb.bkCnt, 0, sizeof(int32_t) * 2 * (size_t)lay.S * BK_STRIDE, st));
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         const hipError_t le = hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
