@@ -30,7 +30,7 @@ void set_error(const char *fmt, ...) {
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
-                      hipStream_t st);
+                      hipStream_t st, const StepFork *fk);
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
 uint32_t persistent_overflow_flags(const DevBrain &d);
@@ -179,10 +179,12 @@ struct fb_brain {
     DevBrain d{};
     hipStream_t stream = nullptr;      // the stream all work is queued on
     hipStream_t ownStream = nullptr;   // the brain's own stream
+    hipStream_t auxStream = nullptr;   // the driven draws of a step, beside its delivery
+    std::vector<hipEvent_t> forkEv, joinEv;   // one pair per step of a captured graph
     std::vector<int32_t> drivenList;   // insertion order of the driven set
     // No excitatory conductance can be -0: gE only ever received non-negative
     // additions and decays since the last reset.  With non-negative weights a
-    // zero-weight arrival then changes nothing, and the lookahead engine may
+    // zero-weight arrival then changes nothing, and the deliveries may
     // skip it (an inhibitory one, adding -0 to gI, never changes anything).
     bool gENonneg = true;
     std::vector<uint8_t> inDriven;
@@ -292,8 +294,17 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
                 if (edges[j].x == edges[j - 1].x) g->rowsUnique = false;
         }
     }
-    uint32_t *dIndptr; uint2 *dEdges; float *dSign; int32_t *dCounts = nullptr; uint32_t *dRowSyn;
+    uint32_t *dIndptr; uint2 *dEdges; float *dSign; int32_t *dCounts = nullptr; uint32_t *dRowSyn, *dInOff;
     fb_status st;
+    {
+        // in-degree prefix: the per-step delivery keeps a target's later
+        // same-step arrivals in entries inOff[q] .. inOff[q+1]-1
+        std::vector<uint32_t> inOff((size_t)N + 1, 0);
+        for (int64_t j = 0; j < E; j++) inOff[indices[j] + 1]++;
+        for (int32_t q = 0; q < N; q++) inOff[q + 1] += inOff[q];
+        if ((st = dalloc(g->allocs, &dInOff, (size_t)N + 1))) return st;
+        FB_HIP(hipMemcpy(dInOff, inOff.data(), sizeof(uint32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
+    }
     if ((st = dalloc(g->allocs, &dIndptr, (size_t)N + 1)) ||
         (st = dalloc(g->allocs, &dEdges, (size_t)std::max<int64_t>(E, 1))) ||
         (st = dalloc(g->allocs, &dSign, (size_t)N)) || (st = dalloc(g->allocs, &dRowSyn, (size_t)N)))
@@ -316,7 +327,7 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
         if (E) FB_HIP(hipMemcpy(dCounts, cnts.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
     }
     for (int32_t i = 0; i < N; i++) g->d.maxRow = std::max<int64_t>(g->d.maxRow, (int64_t)indptr[i + 1] - indptr[i]);
-    g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts; g->d.rowSyn = dRowSyn;
+    g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts; g->d.rowSyn = dRowSyn; g->d.inOff = dInOff;
     *out = g;
     return FB_OK;
 }
@@ -348,7 +359,7 @@ __global__ void k_reset_state(DevBrain d) {
 
 __global__ void k_init_scratch(DevBrain d) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 2 * d.N) { d.cnt[i] = 0; d.chain[i] = -1; }
+    if (i < 2 * d.N) d.cnt[i] = 0;
     if (i < d.N) d.bgCnt[i] = 0;
 }
 
@@ -445,6 +456,7 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     d.g = g->d;
     FB_HIP(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking));
     b->stream = b->ownStream;
+    FB_HIP(hipStreamCreateWithFlags(&b->auxStream, hipStreamNonBlocking));
     fb_status st;
     float *f;
     // ten f32 state arrays back to back, then spike counts
@@ -455,11 +467,12 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     if ((st = dalloc(b->allocs, &d.spikeCount, (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.driven, (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.cnt, 2 * (size_t)N))) return st;
-    if ((st = dalloc(b->allocs, &d.chain, 2 * (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.add, 2 * (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.key, 2 * (size_t)N))) return st;
-    d.collCap = std::max<int64_t>(E, 1);
-    if ((st = dalloc(b->allocs, &d.coll, (size_t)d.collCap))) return st;
+    // later same-step arrivals: one run per (sign class, target) sized by its in-degree
+    if ((st = dalloc(b->allocs, &d.runKey, 2 * (size_t)std::max<int64_t>(E, 1)))) return st;
+    if ((st = dalloc(b->allocs, &d.runAdd, 2 * (size_t)std::max<int64_t>(E, 1)))) return st;
+    if ((st = dalloc(b->allocs, &d.longList, 2 * (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.bgCnt, (size_t)N))) return st;
     if ((st = dalloc(b->allocs, &d.stage, (size_t)d.nblocks * 256))) return st;
     if ((st = dalloc(b->allocs, &d.blkCnt, (size_t)d.nblocks))) return st;
@@ -490,6 +503,9 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
     if (b->gatherBuf) hipFree(b->gatherBuf);
     if (b->gatherCnt) hipFree(b->gatherCnt);
     hipStreamDestroy(b->ownStream);
+    if (b->auxStream) hipStreamDestroy(b->auxStream);
+    for (hipEvent_t ev : b->forkEv) if (ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : b->joinEv) if (ev) hipEventDestroy(ev);
     fb_graph_release(b->g);
     delete b;
     return FB_OK;
@@ -651,14 +667,28 @@ static fb_status check_device_errors(fb_brain *b) {
     Scalars sc;
     FB_HIP(hipMemcpyAsync(&sc, b->d.sc, sizeof sc, hipMemcpyDeviceToHost, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
-    FB_REQUIRE(!(sc.err & FB_DEVERR_COLL_OVERFLOW), FB_ERR_OVERFLOW, "arrival chain pool overflowed");
+    FB_REQUIRE(!(sc.err & FB_DEVERR_RUN_OVERFLOW), FB_ERR_OVERFLOW, "a target received more arrivals in one step than it has in-edges");
+    return FB_OK;
+}
+
+// fork/join events for n steps in a row (a captured graph records each pair once)
+static fb_status ensure_fork_events(fb_brain *b, int32_t n) {
+    while ((int32_t)b->forkEv.size() < n) {
+        hipEvent_t f = nullptr, j = nullptr;
+        FB_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+        FB_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        b->forkEv.push_back(f);
+        b->joinEv.push_back(j);
+    }
     return FB_OK;
 }
 
 static fb_status run_launch(fb_brain *b, int32_t nsteps) {
+    if (fb_status e = ensure_fork_events(b, 1)) return e;
     for (int32_t s = 0; s < nsteps; s++) {
         const int32_t slot = (b->head + b->d.nslots - 1) % b->d.nslots;
-        fb_status st = launch_step(b->d, b->head, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
+        const StepFork fk{b->auxStream, b->forkEv[0], b->joinEv[0]};
+        fb_status st = launch_step(b->d, b->head, slot, serial_deliver(b), b->g->rowsUnique, b->stream, &fk);
         if (st) return st;
         b->lastSlot = slot;
         b->head = (b->head + 1) % b->d.nslots;
@@ -674,6 +704,7 @@ static fb_status run_graph(fb_brain *b, int32_t nsteps) {
         drop_graphs(b);
         b->graphVersion = b->version;
     }
+    if (fb_status e = ensure_fork_events(b, ns)) return e;
     while (nsteps >= ns) {
         GraphCache &gc = b->graphs[b->head];
         if (!gc.exec) {
@@ -681,7 +712,8 @@ static fb_status run_graph(fb_brain *b, int32_t nsteps) {
             int32_t h = b->head;
             for (int32_t s = 0; s < ns; s++) {
                 const int32_t slot = (h + ns - 1) % ns;
-                fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
+                const StepFork fk{b->auxStream, b->forkEv[(size_t)s], b->joinEv[(size_t)s]};
+                fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream, &fk);
                 if (st) { hipGraph_t tmp; hipStreamEndCapture(b->stream, &tmp); if (tmp) hipGraphDestroy(tmp); return st; }
                 h = (h + 1) % ns;
             }
@@ -704,6 +736,11 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     if (nsteps == 0) return FB_OK;
     // background events of negative amplitude can leave an excitatory conductance at -0
     if (b->d.k.bgEvents > 0.f && std::signbit(b->d.k.bgAmp)) b->gENonneg = false;
+    const int32_t skip = b->g->wNonneg && b->gENonneg ? 1 : 0;
+    if (skip != b->d.skipZeroW) {
+        b->d.skipZeroW = skip;
+        drop_graphs(b);   // captured launches hold the old value
+    }
     const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
         // the lookahead engine pays a prologue per call: worth it from two epochs on
@@ -719,7 +756,6 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             std::string why;
             st = persistent_supported(b->d, serial_deliver(b), &why);
             if (st) { set_error("persistent engine: %s", why.c_str()); return st; }
-            b->d.skipZeroW = b->g->wNonneg && b->gENonneg ? 1 : 0;
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st == FB_ERR_UNSUPPORTED && autoChosen) {

@@ -11,6 +11,52 @@
 
 namespace fb {
 
+// One arrival of the chain pool: the later same-step arrivals of a (sign
+// class, target, member) hang off chain[] in arbitrary order.
+struct CollEntry {
+    uint64_t key;
+    float addend;
+    int32_t next;
+};
+enum { FB_DEVERR_COLL_OVERFLOW = FB_DEVERR_RUN_OVERFLOW };
+
+// Sum of the same-step arrivals on one (sign class, target, member) in
+// arrival order: the first sits in (key, add), the others on the chain.
+// Insertion-sorts up to 32 of them; beyond that, repeated minimum search.
+__device__ inline float ordered_sum_chain(float g, int32_t count, uint64_t key0, float add0, int32_t head,
+                                          const CollEntry *coll) {
+    if (count <= 32) {
+        uint64_t ks[32];
+        float as[32];
+        int n = 0;
+        ks[n] = key0; as[n] = add0; n++;
+        for (int32_t p = head; p >= 0 && n < 32; p = coll[p].next) {
+            uint64_t kk = coll[p].key;
+            float aa = coll[p].addend;
+            int m = n++;
+            while (m > 0 && ks[m - 1] > kk) { ks[m] = ks[m - 1]; as[m] = as[m - 1]; m--; }
+            ks[m] = kk; as[m] = aa;
+        }
+        for (int m = 0; m < n; m++) g = g + as[m];
+        return g;
+    }
+    uint64_t last = 0;
+    bool first = true;
+    for (int32_t c = 0; c < count; c++) {
+        uint64_t best = ~0ull;
+        float ba = 0.f;
+        if ((first || key0 > last) && key0 < best) { best = key0; ba = add0; }
+        for (int32_t p = head; p >= 0; p = coll[p].next) {
+            uint64_t kk = coll[p].key;
+            if ((first || kk > last) && kk < best) { best = kk; ba = coll[p].addend; }
+        }
+        g = g + ba;
+        last = best;
+        first = false;
+    }
+    return g;
+}
+
 constexpr int BMAX = 64;
 
 struct BatchDev {

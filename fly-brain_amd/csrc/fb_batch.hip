@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
                 float g = c ? s.gI : s.gE;
                 if (n == 1) g = g + d.add[ci];
                 else {
-                    g = ordered_sum(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
+                    g = ordered_sum_chain(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
                     d.chain[ci] = -1;
                 }
                 d.cnt[ci] = 0;

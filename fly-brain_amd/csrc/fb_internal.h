@@ -8,12 +8,14 @@
 //   delivery   cnt i32[2N]   same-step arrivals per (sign class, target)
 //              add f32[2N]   the first arrival's addend f32(W*s)
 //              key u64[2N]   its order key (arrival position << 32 | position in row)
-//              chain i32[2N] head of the per-target list of further arrivals
-//              coll          further arrivals {key, addend, next}, one per edge event at most
+//              runKey u64[2E], runAdd f32[2E]  further arrivals: class c of target q
+//                            owns entries c*E + inOff[q] .. c*E + inOff[q+1] - 1 (each
+//                            in-edge arrives at most once per step)
 //   background bgCnt i32[N]  background hits this step (all add the same bgAmp)
 //   compaction stage i32[N], blkCnt i32[nblocks]
 // Per graph: indptr u32[N+1], edges {u32 target, f32 W}[E] interleaved so one
-// 8-byte load feeds one edge, sign f32[N], counts i32[E] (optional).
+// 8-byte load feeds one edge, sign f32[N], counts i32[E] (optional), and the
+// in-degree prefix inOff u32[N+1].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -44,25 +46,19 @@ struct Consts {
     int32_t coba;
 };
 
-struct CollEntry {
-    uint64_t key;
-    float addend;
-    int32_t next;
-};
-
 // device-side scalars of one brain
 struct Scalars {
     uint32_t rng;
     float bgAcc;
     int32_t nDrivenFired;   // forced spikes emitted this step
-    int32_t ncoll;          // entries used in coll this step
     int32_t err;            // sticky error bits (FB_DEVERR_*)
+    int32_t nLong;          // (class, target) pairs with more than LONG_RUN arrivals this step
     int32_t pad;
     unsigned long long edgeEvents;
     unsigned long long synEvents;
     unsigned long long spikes;
 };
-enum { FB_DEVERR_COLL_OVERFLOW = 1 };
+enum { FB_DEVERR_RUN_OVERFLOW = 1 };
 
 struct DevGraph {
     int32_t N;
@@ -72,28 +68,38 @@ struct DevGraph {
     const float *sign;
     const int32_t *counts;
     const uint32_t *rowSyn;  // synapse count summed over each row (0 without counts)
+    const uint32_t *inOff;   // [N+1] in-degree prefix: target q owns run entries inOff[q] .. inOff[q+1]-1
     int64_t maxRow;          // longest row
 };
 
 struct DevBrain {
     int32_t N, nslots, nDriven, nblocks;
-    int32_t skipZeroW;         // lookahead push may skip zero-weight edges (exact: see fb_brain::gENonneg)
+    int32_t skipZeroW;         // deliveries may skip zero-weight edges (exact: see fb_brain::gENonneg)
     int64_t ringStride;        // entries per ring slot (2N)
     Consts k;
     DevGraph g;
     float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr, *drive;
     uint32_t *spikeCount;
     int32_t *ring, *ringCount, *driven;
-    int32_t *cnt, *chain;
-    float *add;
+    int32_t *cnt;              // [2N] arrivals this step per (sign class, target)
+    float *add;                // [2N] the first arrival's addend and order key
     uint64_t *key;
-    CollEntry *coll;
-    int64_t collCap;
+    uint64_t *runKey;          // [2E] later arrivals: class c, target q at c*E + inOff[q] + (slot-1)
+    float *runAdd;
+    int32_t *longList;         // [2N] class*N + target of the step's long runs (k_long sums them)
     int32_t *bgCnt;
     int32_t *stage, *blkCnt;
     Scalars *sc;
     const uint32_t *jumpTab;   // 32 powers x 4 bytes x 256 entries of xorshift32 jumps
     const uint32_t *leap;      // LEAP_MATS leapfrog matrices (see leap_fill)
+};
+
+// A second stream for one step's driven draws (k_driven), which share no
+// state with the delivery (k_deliver): the step forks after the previous
+// step's kernels and joins before the membrane pass.
+struct StepFork {
+    hipStream_t aux;
+    hipEvent_t fork, join;
 };
 
 // ------------------------------------------------------------------ device helpers
@@ -318,42 +324,55 @@ __device__ __forceinline__ void spike_effects(const Consts &k, NeuronState &s, f
     s.adapt = s.adapt + k.adaptInc;
 }
 
-// Sum of the same-step arrivals on one (sign class, target) in arrival order.
-// The first arrival sits in (key, add); the others hang off chain[] in the
-// coll pool in arbitrary order.  Insertion-sorts up to 32 of them in
-// registers; beyond that falls back to repeated minimum search (exact, slow).
-__device__ inline float ordered_sum(float g, int32_t count, uint64_t key0, float add0, int32_t head,
-                                    const CollEntry *coll) {
-    if (count <= 32) {
-        uint64_t ks[32];
-        float as[32];
-        int n = 0;
-        ks[n] = key0; as[n] = add0; n++;
-        for (int32_t p = head; p >= 0 && n < 32; p = coll[p].next) {
-            uint64_t kk = coll[p].key;
-            float aa = coll[p].addend;
-            int m = n++;
-            while (m > 0 && ks[m - 1] > kk) { ks[m] = ks[m - 1]; as[m] = as[m - 1]; m--; }
-            ks[m] = kk; as[m] = aa;
-        }
-        for (int m = 0; m < n; m++) g = g + as[m];
-        return g;
+// Sum of the same-step arrivals on one (sign class, target) in arrival order:
+// the first arrival is (key0, add0), the other n-1 sit in rk/ra[0, n-1) in any
+// order.  Keys are distinct.  Up to M arrivals: a bitonic network over
+// registers (missing entries act as +infinity), then the ordered sum.
+template <int M>
+__device__ __forceinline__ float ordered_sum_net(float g, int32_t n, uint64_t key0, float add0, const uint64_t *rk,
+                                                 const float *ra) {
+    uint64_t k[M];
+    float a[M];
+    k[0] = key0;
+    a[0] = add0;
+#pragma unroll
+    for (int q = 1; q < M; q++) {
+        const bool has = q < n;
+        k[q] = has ? rk[q - 1] : ~0ull;
+        a[q] = has ? ra[q - 1] : 0.f;
     }
-    uint64_t last = 0;
-    bool first = true;
-    for (int32_t c = 0; c < count; c++) {
-        uint64_t best = ~0ull;
-        float ba = 0.f;
-        if ((first || key0 > last) && key0 < best) { best = key0; ba = add0; }
-        for (int32_t p = head; p >= 0; p = coll[p].next) {
-            uint64_t kk = coll[p].key;
-            if ((first || kk > last) && kk < best) { best = kk; ba = coll[p].addend; }
+#pragma unroll
+    for (int size = 2; size <= M; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < M; i++) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    if ((k[i] > k[j]) == up) {
+                        const uint64_t tk = k[i]; k[i] = k[j]; k[j] = tk;
+                        const float ta = a[i]; a[i] = a[j]; a[j] = ta;
+                    }
+                }
+            }
         }
-        g = g + ba;
-        last = best;
-        first = false;
     }
+#pragma unroll
+    for (int q = 0; q < M; q++)
+        if (q < n) g = g + a[q];
     return g;
+}
+
+// Runs of up to LONG_RUN arrivals are summed by the neuron's own thread;
+// longer ones by a workgroup each (k_long) before the membrane pass.
+constexpr int LONG_RUN = 16;
+
+__device__ inline float ordered_sum(float g, int32_t n, uint64_t key0, float add0, const uint64_t *rk,
+                                    const float *ra) {
+    if (n <= 4) return ordered_sum_net<4>(g, n, key0, add0, rk, ra);
+    if (n <= 8) return ordered_sum_net<8>(g, n, key0, add0, rk, ra);
+    return ordered_sum_net<LONG_RUN>(g, n, key0, add0, rk, ra);
 }
 
 }  // namespace fb

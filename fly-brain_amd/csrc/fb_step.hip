@@ -3,14 +3,18 @@
 // One step of the reference's lif_step (src/wasm/lif.c:29-80) runs as
 //   k_deliver   arrivals of ring[head] scatter into per-(sign, target)
 //               accumulators; the first arrival on a target keeps its exact
-//               addend f32(W*s) and order key, later ones go on a per-target
-//               chain.  No floating-point atomics: the sum is formed later,
-//               in arrival order, so it is bit-identical to lif.c's serial loop.
+//               addend f32(W*s) and order key, later ones go to the target's
+//               own run (one entry per in-edge).  No floating-point atomics:
+//               the sum is formed later, in arrival order, so it is
+//               bit-identical to lif.c's serial loop.  Zero-weight edges are
+//               skipped where that is exact (DevBrain::skipZeroW).
 //   k_driven    one workgroup: background draws and forced spikes.  lif.c
 //               draws from one serial xorshift32 stream; each thread counts the
 //               non-refractory driven neurons of its slice, a workgroup scan
 //               gives each slice its position in the stream, and the stream is
 //               produced 2048 states per round by GF(2) leapfrog matrices.
+//   k_long      one workgroup per (class, target) with more than 16
+//               arrivals: ranks them by order key and adds them in order.
 //   k_membrane  dense pass: apply the step's arrivals (single arrival: one
 //               add; several: the ordered sum), background hits, the membrane
 //               and decay update, the threshold test, and wave-ballot
@@ -39,23 +43,29 @@ __global__ void __launch_bounds__(256) k_deliver(DevBrain d, int32_t head) {
             atomicAdd(&d.sc->synEvents, (unsigned long long)d.g.rowSyn[pre]);
         }
         const int32_t cls = s > 0 ? 0 : d.N;
+        const int64_t runBase = s > 0 ? 0 : d.g.E;
         for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x) {
             const uint2 ed = d.g.edges[j];
+            // a +0 weight adds -0 to gI, which changes nothing, and +0 to gE,
+            // which changes nothing while no excitatory conductance can be -0
+            if (ed.y == 0u && (s < 0.f || d.skipZeroW)) continue;
             const float addend = __uint_as_float(ed.y) * s;
             const int32_t ci = cls + (int32_t)ed.x;
             const uint64_t key = ((uint64_t)(uint32_t)k << 32) | (uint64_t)(j - a);
             const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+            if (slot == LONG_RUN) d.longList[atomicAdd(&d.sc->nLong, 1)] = ci;
             if (slot == 0) {
                 d.add[ci] = addend;
                 d.key[ci] = key;
             } else {
-                const int32_t p = atomicAdd(&d.sc->ncoll, 1);
-                if (p < d.collCap) {
-                    d.coll[p].key = key;
-                    d.coll[p].addend = addend;
-                    d.coll[p].next = atomicExch(&d.chain[ci], p);
+                // each in-edge of the target arrives at most once per step
+                const uint32_t r0 = d.g.inOff[ed.x], deg = d.g.inOff[ed.x + 1] - r0;
+                if ((uint32_t)(slot - 1) < deg) {
+                    const int64_t at = runBase + r0 + (slot - 1);
+                    d.runKey[at] = key;
+                    d.runAdd[at] = addend;
                 } else {
-                    atomicOr(&d.sc->err, FB_DEVERR_COLL_OVERFLOW);
+                    atomicOr(&d.sc->err, FB_DEVERR_RUN_OVERFLOW);
                 }
             }
         }
@@ -96,6 +106,72 @@ __global__ void __launch_bounds__(1024) k_deliver_serial(DevBrain d, int32_t hea
                 const uint2 ed = d.g.edges[j];
                 acc[ed.x] = acc[ed.x] + __uint_as_float(ed.y) * s;
             }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ long runs
+
+// The step's (class, target) pairs with more than LONG_RUN arrivals: one
+// workgroup each stages the run in LDS, ranks every arrival by its order key,
+// and one thread adds the addends in that order onto the conductance, which
+// it stores; the pair's count is cleared, so k_membrane sees no arrivals
+// there.  Runs longer than LONG_CAP are ranked against global memory.
+constexpr int LONG_T = 256;
+constexpr int LONG_CAP = 4096;
+
+__global__ void __launch_bounds__(LONG_T) k_long(DevBrain d) {
+    __shared__ uint64_t ks[LONG_CAP];
+    __shared__ float as[LONG_CAP];
+    __shared__ float sorted[LONG_CAP];
+    const int tid = threadIdx.x;
+    const int32_t nLong = d.sc->nLong;
+    for (int32_t e = blockIdx.x; e < nLong; e += gridDim.x) {
+        const int32_t ci = d.longList[e];
+        const int32_t n = d.cnt[ci];
+        const int32_t c = ci >= d.N ? 1 : 0;
+        const int32_t i = ci - c * d.N;
+        const int64_t r0 = (c ? d.g.E : 0) + d.g.inOff[i];
+        const uint64_t *rk = d.runKey + r0;
+        const float *ra = d.runAdd + r0;
+        // entry 0 is the first arrival, entry q >= 1 is run entry q-1
+        const bool inLds = n <= LONG_CAP;
+        if (inLds) {
+            for (int32_t q = tid; q < n; q += LONG_T) {
+                ks[q] = q ? rk[q - 1] : d.key[ci];
+                as[q] = q ? ra[q - 1] : d.add[ci];
+            }
+            __syncthreads();
+            for (int32_t q = tid; q < n; q += LONG_T) {
+                const uint64_t mine = ks[q];
+                int32_t rank = 0;
+                for (int32_t m = 0; m < n; m++) rank += ks[m] < mine;
+                sorted[rank] = as[q];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                float g = c ? d.gI[i] : d.gE[i];
+                for (int32_t q = 0; q < n; q++) g = g + sorted[q];
+                if (c) d.gI[i] = g; else d.gE[i] = g;
+                d.cnt[ci] = 0;
+            }
+        } else if (tid == 0) {
+            // repeated minimum search (keys are distinct)
+            float g = c ? d.gI[i] : d.gE[i];
+            uint64_t last = 0;
+            for (int32_t r = 0; r < n; r++) {
+                uint64_t best = ~0ull;
+                float ba = 0.f;
+                for (int32_t q = 0; q < n; q++) {
+                    const uint64_t kk = q ? rk[q - 1] : d.key[ci];
+                    if ((r == 0 || kk > last) && kk < best) { best = kk; ba = q ? ra[q - 1] : d.add[ci]; }
+                }
+                g = g + ba;
+                last = best;
+            }
+            if (c) d.gI[i] = g; else d.gE[i] = g;
+            d.cnt[ci] = 0;
         }
         __syncthreads();
     }
@@ -204,8 +280,8 @@ __global__ void __launch_bounds__(256) k_membrane(DevBrain d) {
                 if (n == 1) {
                     g = g + d.add[ci];
                 } else {
-                    g = ordered_sum(g, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
-                    d.chain[ci] = -1;
+                    const int64_t r0 = (c ? d.g.E : 0) + d.g.inOff[i];
+                    g = ordered_sum(g, n, d.key[ci], d.add[ci], d.runKey + r0, d.runAdd + r0);
                 }
                 d.cnt[ci] = 0;
                 if (c) s.gI = g; else s.gE = g;
@@ -260,8 +336,8 @@ __global__ void __launch_bounds__(256) k_compact(DevBrain d, int32_t slot, int32
         const int32_t total = nd + before + nb;
         d.ringCount[slot] = total;
         d.ringCount[head] = 0;               // lif.c:44, after k_deliver consumed it
+        d.sc->nLong = 0;
         d.sc->spikes += (unsigned long long)total;
-        d.sc->ncoll = 0;
     }
 }
 
@@ -287,12 +363,25 @@ int deliver_grid(const DevBrain &d) {
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
-                      hipStream_t st) {
+                      hipStream_t st, const StepFork *fk) {
+    // k_deliver writes gE/gI accumulators and res; k_driven reads refr and
+    // drive and writes v, refr, trace, adapt, counts, the ring's out slot and
+    // the background hits: with a fork they run side by side
+    if (fk) {
+        FB_HIP(hipEventRecord(fk->fork, st));
+        FB_HIP(hipStreamWaitEvent(fk->aux, fk->fork, 0));
+        hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, fk->aux, d, slot);
+        FB_HIP(hipEventRecord(fk->join, fk->aux));
+    }
     if (serialDeliver)
         hipLaunchKernelGGL(k_deliver_serial, dim3(1), dim3(1024), 0, st, d, head, (int32_t)rowsUnique);
     else
         hipLaunchKernelGGL(k_deliver, dim3(deliver_grid(d)), dim3(256), 0, st, d, head);
-    hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, st, d, slot);
+    if (fk)
+        FB_HIP(hipStreamWaitEvent(st, fk->join, 0));
+    else
+        hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, st, d, slot);
+    if (!serialDeliver) hipLaunchKernelGGL(k_long, dim3(256), dim3(LONG_T), 0, st, d);
     hipLaunchKernelGGL(k_membrane, dim3(d.nblocks), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_compact, dim3(d.nblocks), dim3(256), 0, st, d, slot, head);
     FB_HIP(hipGetLastError());
