@@ -30,7 +30,7 @@ void set_error(const char *fmt, ...) {
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
-                      hipStream_t st, const StepFork *fk);
+                      hipStream_t st);
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
 uint32_t persistent_overflow_flags(const DevBrain &d);
@@ -62,6 +62,7 @@ static void build_jump_tables(std::vector<uint32_t> &tab) {
 }
 
 // Leapfrog matrices M^1..M^32, then M^(32h) for h = 0..63 (M^0 = identity),
+// then M^2048,
 // each as 8 nibble tables: entry [k][v] is the XOR of the columns 4k..4k+3
 // selected by the bits of v.
 static void put_nibbles(uint32_t *dst, const uint32_t *col) {
@@ -76,15 +77,16 @@ static void put_nibbles(uint32_t *dst, const uint32_t *col) {
 
 static void build_leap_matrices(std::vector<uint32_t> &tab) {
     auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
-    tab.assign((size_t)LEAP_MATS * LEAP_WORDS, 0);
+    tab.assign((size_t)LEAP_TABLE_MATS * LEAP_WORDS, 0);
     uint32_t col[32];
     for (int b = 0; b < 32; b++) col[b] = 1u << b;   // M^0
     put_nibbles(&tab[(size_t)32 * LEAP_WORDS], col);
-    for (int p = 1; p <= 32 * 63; p++) {
+    for (int p = 1; p <= 2048; p++) {
         for (int b = 0; b < 32; b++) col[b] = step(col[b]);   // M^p
         if (p <= 32) put_nibbles(&tab[(size_t)(p - 1) * LEAP_WORDS], col);
-        if (p % 32 == 0) put_nibbles(&tab[(size_t)(32 + p / 32) * LEAP_WORDS], col);
+        if (p % 32 == 0 && p / 32 < 64) put_nibbles(&tab[(size_t)(32 + p / 32) * LEAP_WORDS], col);
     }
+    put_nibbles(&tab[(size_t)LEAP_FAR * LEAP_WORDS], col);   // M^2048
 }
 
 static std::mutex g_jump_mu;
@@ -179,8 +181,6 @@ struct fb_brain {
     DevBrain d{};
     hipStream_t stream = nullptr;      // the stream all work is queued on
     hipStream_t ownStream = nullptr;   // the brain's own stream
-    hipStream_t auxStream = nullptr;   // the driven draws of a step, beside its delivery
-    std::vector<hipEvent_t> forkEv, joinEv;   // one pair per step of a captured graph
     std::vector<int32_t> drivenList;   // insertion order of the driven set
     // No excitatory conductance can be -0: gE only ever received non-negative
     // additions and decays since the last reset.  With non-negative weights a
@@ -456,7 +456,6 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     d.g = g->d;
     FB_HIP(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking));
     b->stream = b->ownStream;
-    FB_HIP(hipStreamCreateWithFlags(&b->auxStream, hipStreamNonBlocking));
     fb_status st;
     float *f;
     // ten f32 state arrays back to back, then spike counts
@@ -503,9 +502,6 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
     if (b->gatherBuf) hipFree(b->gatherBuf);
     if (b->gatherCnt) hipFree(b->gatherCnt);
     hipStreamDestroy(b->ownStream);
-    if (b->auxStream) hipStreamDestroy(b->auxStream);
-    for (hipEvent_t ev : b->forkEv) if (ev) hipEventDestroy(ev);
-    for (hipEvent_t ev : b->joinEv) if (ev) hipEventDestroy(ev);
     fb_graph_release(b->g);
     delete b;
     return FB_OK;
@@ -671,24 +667,10 @@ static fb_status check_device_errors(fb_brain *b) {
     return FB_OK;
 }
 
-// fork/join events for n steps in a row (a captured graph records each pair once)
-static fb_status ensure_fork_events(fb_brain *b, int32_t n) {
-    while ((int32_t)b->forkEv.size() < n) {
-        hipEvent_t f = nullptr, j = nullptr;
-        FB_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
-        FB_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-        b->forkEv.push_back(f);
-        b->joinEv.push_back(j);
-    }
-    return FB_OK;
-}
-
 static fb_status run_launch(fb_brain *b, int32_t nsteps) {
-    if (fb_status e = ensure_fork_events(b, 1)) return e;
     for (int32_t s = 0; s < nsteps; s++) {
         const int32_t slot = (b->head + b->d.nslots - 1) % b->d.nslots;
-        const StepFork fk{b->auxStream, b->forkEv[0], b->joinEv[0]};
-        fb_status st = launch_step(b->d, b->head, slot, serial_deliver(b), b->g->rowsUnique, b->stream, &fk);
+        fb_status st = launch_step(b->d, b->head, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
         if (st) return st;
         b->lastSlot = slot;
         b->head = (b->head + 1) % b->d.nslots;
@@ -704,7 +686,6 @@ static fb_status run_graph(fb_brain *b, int32_t nsteps) {
         drop_graphs(b);
         b->graphVersion = b->version;
     }
-    if (fb_status e = ensure_fork_events(b, ns)) return e;
     while (nsteps >= ns) {
         GraphCache &gc = b->graphs[b->head];
         if (!gc.exec) {
@@ -712,8 +693,7 @@ static fb_status run_graph(fb_brain *b, int32_t nsteps) {
             int32_t h = b->head;
             for (int32_t s = 0; s < ns; s++) {
                 const int32_t slot = (h + ns - 1) % ns;
-                const StepFork fk{b->auxStream, b->forkEv[(size_t)s], b->joinEv[(size_t)s]};
-                fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream, &fk);
+                fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
                 if (st) { hipGraph_t tmp; hipStreamEndCapture(b->stream, &tmp); if (tmp) hipGraphDestroy(tmp); return st; }
                 h = (h + 1) % ns;
             }

@@ -94,14 +94,6 @@ struct DevBrain {
     const uint32_t *leap;      // LEAP_MATS leapfrog matrices (see leap_fill)
 };
 
-// A second stream for one step's driven draws (k_driven), which share no
-// state with the delivery (k_deliver): the step forks after the previous
-// step's kernels and joins before the membrane pass.
-struct StepFork {
-    hipStream_t aux;
-    hipEvent_t fork, join;
-};
-
 // ------------------------------------------------------------------ device helpers
 
 __device__ __forceinline__ uint32_t xs32(uint32_t x) {
@@ -140,6 +132,10 @@ __device__ __forceinline__ uint32_t jump(const uint32_t *tab, uint32_t x, uint32
 // matrices sit LEAP_STRIDE = 129 words apart, so lanes that read the same
 // (nibble, value) entry of 32 different matrices hit 32 different banks.
 constexpr int LEAP_MATS = 96;
+// The global table holds one more matrix after them, M^2048, for jumps past
+// the first 2048 stream positions (it is not staged in LDS).
+constexpr int LEAP_FAR = 96;
+constexpr int LEAP_TABLE_MATS = 97;
 constexpr int LEAP_WORDS = 128;           // 8 nibbles x 16 values
 constexpr int LEAP_ROUND = 2048;
 constexpr int LEAP_STRIDE = 129;

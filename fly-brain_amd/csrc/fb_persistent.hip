@@ -1064,7 +1064,7 @@ static void free_buffers(PersistBuffers &pb) {
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
-                      hipStream_t st, const StepFork *fk);
+                      hipStream_t st);
 
 void persistent_release(const int32_t *key) {
     auto it = g_pbuf.find(key);
@@ -1213,7 +1213,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             const int32_t m = std::min(D, nsteps - done);
             for (int32_t s2 = 0; s2 < m; s2++) {
                 const int32_t slot = (cur + d.nslots - 1) % d.nslots;
-                fb_status r2 = launch_step(d, cur, slot, false, true, st, nullptr);
+                fb_status r2 = launch_step(d, cur, slot, false, true, st);
                 if (r2) return r2;
                 cur = (cur + 1) % d.nslots;
             }

@@ -13,6 +13,10 @@
 //               non-refractory driven neurons of its slice, a workgroup scan
 //               gives each slice its position in the stream, and the stream is
 //               produced 2048 states per round by GF(2) leapfrog matrices.
+//   k_deliver_driven  both in one launch when there is no depression, no
+//               background and at most 4096 driven neurons: a flat list of
+//               edge events over 128 workgroups, and one workgroup whose
+//               threads jump straight to their first draw.
 //   k_long      one workgroup per (class, target) with more than 16
 //               arrivals: ranks them by order key and adds them in order.
 //   k_membrane  dense pass: apply the step's arrivals (single arrival: one
@@ -28,6 +32,34 @@ namespace fb {
 
 // ------------------------------------------------------------------ deliver
 
+// One arrival's edge event into the accumulators: the first on its (class,
+// target) keeps addend and key there, later ones go to the target's run.
+__device__ __forceinline__ void deliver_edge(const DevBrain &d, uint2 ed, float s, uint64_t key) {
+    // a +0 weight adds -0 to gI, which changes nothing, and +0 to gE, which
+    // changes nothing while no excitatory conductance can be -0
+    if (ed.y == 0u && (s < 0.f || d.skipZeroW)) return;
+    const float addend = __uint_as_float(ed.y) * s;
+    const int32_t ci = (s > 0 ? 0 : d.N) + (int32_t)ed.x;
+    const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+    if (slot == LONG_RUN) d.longList[atomicAdd(&d.sc->nLong, 1)] = ci;
+    if (slot == 0) {
+        d.add[ci] = addend;
+        d.key[ci] = key;
+    } else {
+        // each in-edge of the target arrives at most once per step
+        const uint32_t r0 = d.g.inOff[ed.x], deg = d.g.inOff[ed.x + 1] - r0;
+        if ((uint32_t)(slot - 1) < deg) {
+            const int64_t at = (s > 0 ? 0 : d.g.E) + r0 + (slot - 1);
+            d.runKey[at] = key;
+            d.runAdd[at] = addend;
+        } else {
+            atomicOr(&d.sc->err, FB_DEVERR_RUN_OVERFLOW);
+        }
+    }
+}
+
+// Arrivals row by row: a workgroup per arriving neuron, which also applies
+// its synaptic depression once every thread has read res.
 __global__ void __launch_bounds__(256) k_deliver(DevBrain d, int32_t head) {
     const int32_t A = d.ringCount[head];
     const int32_t *list = d.ring + (int64_t)head * d.ringStride;
@@ -42,33 +74,86 @@ __global__ void __launch_bounds__(256) k_deliver(DevBrain d, int32_t head) {
             atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
             atomicAdd(&d.sc->synEvents, (unsigned long long)d.g.rowSyn[pre]);
         }
-        const int32_t cls = s > 0 ? 0 : d.N;
-        const int64_t runBase = s > 0 ? 0 : d.g.E;
-        for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x) {
-            const uint2 ed = d.g.edges[j];
-            // a +0 weight adds -0 to gI, which changes nothing, and +0 to gE,
-            // which changes nothing while no excitatory conductance can be -0
-            if (ed.y == 0u && (s < 0.f || d.skipZeroW)) continue;
-            const float addend = __uint_as_float(ed.y) * s;
-            const int32_t ci = cls + (int32_t)ed.x;
-            const uint64_t key = ((uint64_t)(uint32_t)k << 32) | (uint64_t)(j - a);
-            const int32_t slot = atomicAdd(&d.cnt[ci], 1);
-            if (slot == LONG_RUN) d.longList[atomicAdd(&d.sc->nLong, 1)] = ci;
-            if (slot == 0) {
-                d.add[ci] = addend;
-                d.key[ci] = key;
-            } else {
-                // each in-edge of the target arrives at most once per step
-                const uint32_t r0 = d.g.inOff[ed.x], deg = d.g.inOff[ed.x + 1] - r0;
-                if ((uint32_t)(slot - 1) < deg) {
-                    const int64_t at = runBase + r0 + (slot - 1);
-                    d.runKey[at] = key;
-                    d.runAdd[at] = addend;
-                } else {
-                    atomicOr(&d.sc->err, FB_DEVERR_RUN_OVERFLOW);
-                }
+        for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x)
+            deliver_edge(d, d.g.edges[j], s, ((uint64_t)(uint32_t)k << 32) | (uint64_t)(j - a));
+    }
+}
+
+// Arrivals as one flat list of edge events (without synaptic depression,
+// when res never changes on arrival): every workgroup builds the table of the
+// step's rows in LDS (start, length prefix, sign x res), and thread g of the
+// grid takes edge events g, g + grid, ... , so long rows spread over the
+// whole chip.  Up to DF_CAP arriving neurons; more go row by row.
+constexpr int DF_T = 1024;
+constexpr int DF_CAP = 2048;
+
+// blocks 0 .. nblk-1 of the grid share the flat list
+__device__ __forceinline__ void deliver_flat(const DevBrain &d, int32_t head, int32_t nblk) {
+    __shared__ int32_t rowOff[DF_CAP + 1];
+    __shared__ uint32_t rowA[DF_CAP];
+    __shared__ float rowS[DF_CAP];
+    __shared__ int32_t scan[64 + DF_T / 64];
+    constexpr int RPT = DF_CAP / DF_T;
+    const int tid = threadIdx.x;
+    const int32_t A = d.ringCount[head];
+    const int32_t *list = d.ring + (int64_t)head * d.ringStride;
+    if (A > DF_CAP) {
+        for (int32_t k = blockIdx.x; k < A; k += nblk) {
+            const int32_t pre = list[k];
+            const float s = d.g.sign[pre] * d.res[pre];
+            if (s == 0.f) continue;
+            const uint32_t a = d.g.indptr[pre], e = d.g.indptr[pre + 1];
+            if (threadIdx.x == 0) {
+                atomicAdd(&d.sc->edgeEvents, (unsigned long long)(e - a));
+                atomicAdd(&d.sc->synEvents, (unsigned long long)d.g.rowSyn[pre]);
             }
+            for (uint32_t j = a + threadIdx.x; j < e; j += blockDim.x)
+                deliver_edge(d, d.g.edges[j], s, ((uint64_t)(uint32_t)k << 32) | (uint64_t)(j - a));
         }
+        return;
+    }
+    int32_t len[RPT];
+    int32_t mySum = 0;
+    unsigned long long syn = 0;
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        len[v] = 0;
+        if (r < A) {
+            const int32_t pre = list[r];
+            const float s = d.g.sign[pre] * d.res[pre];
+            const uint32_t a = d.g.indptr[pre];
+            if (s != 0.f) {
+                len[v] = (int32_t)(d.g.indptr[pre + 1] - a);
+                syn += d.g.rowSyn[pre];
+            }
+            rowA[r] = a;
+            rowS[r] = s;
+        }
+        mySum += len[v];
+    }
+    int32_t total;
+    int32_t off = wg_exclusive_scan<DF_T>(mySum, scan, &total);
+#pragma unroll
+    for (int v = 0; v < RPT; v++) {
+        const int32_t r = RPT * tid + v;
+        if (r < A) rowOff[r] = off;
+        off += len[v];
+    }
+    if (tid == 0) rowOff[A] = total;
+    if (blockIdx.x == 0 && mySum) {
+        atomicAdd(&d.sc->edgeEvents, (unsigned long long)mySum);
+        atomicAdd(&d.sc->synEvents, syn);
+    }
+    __syncthreads();
+    for (int32_t g = blockIdx.x * DF_T + tid; g < total; g += nblk * DF_T) {
+        int32_t lo = 0, hi = A;   // last row with rowOff <= g (rows of length 0 share offsets)
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (rowOff[mid] <= g) lo = mid; else hi = mid;
+        }
+        const uint32_t pos = (uint32_t)(g - rowOff[lo]);
+        deliver_edge(d, d.g.edges[rowA[lo] + pos], rowS[lo], ((uint64_t)(uint32_t)lo << 32) | (uint64_t)pos);
     }
 }
 
@@ -355,6 +440,78 @@ __global__ void __launch_bounds__(256) k_gather(DevBrain d, int32_t s0, int32_t 
     }
 }
 
+// The driven draws without background events, up to 4096 driven neurons: each
+// of 1024 threads holds at most four consecutive driven-list positions, finds
+// its first stream position with a scan of non-refractory flags, and jumps
+// there with leapfrog products read from the global tables (M^2048 once past
+// 2048, then M^(32 h), then M^(f+1) for the position 2048 a + 32 h + f); its
+// later draws are single steps.  Same draws and effects as k_driven.
+constexpr int KDF_T = 1024;
+constexpr int KDF_DP = 4;
+
+__device__ __forceinline__ void driven_fast(const DevBrain &d, int32_t slot) {
+    __shared__ int32_t scan[2][KDF_T / 64];
+    const int tid = threadIdx.x;
+    const uint32_t rng = d.sc->rng;
+    const int32_t nD = d.nDriven, P = (nD + KDF_T - 1) / KDF_T;
+    int32_t ii[KDF_DP];
+    uint32_t nr = 0;   // bit u: driven neuron u of this thread is not refractory and draws
+#pragma unroll
+    for (int u = 0; u < KDF_DP; u++) {
+        const int32_t q = tid * P + u;
+        ii[u] = (u < P && q < nD) ? d.driven[q] : -1;
+        if (ii[u] >= 0 && !(d.refr[ii[u]] > 0)) nr |= 1u << u;
+    }
+    int32_t n;
+    const int32_t off = wg_scan_flags<KDF_T, KDF_DP>(nr, scan[0], &n);
+    const float forcedRefr = d.k.tRef + d.k.dt;
+    uint32_t fm = 0;   // bit u: neuron u was forced to spike
+    if (nr) {
+        const uint32_t y = off >= 2048 ? matvec(d.leap + LEAP_FAR * LEAP_WORDS, rng) : rng;
+        const int32_t o = off & 2047;
+        uint32_t x = matvec(d.leap + (o & 31) * LEAP_WORDS, matvec(d.leap + (32 + (o >> 5)) * LEAP_WORDS, y));
+        int32_t j = off;
+        bool first = true;
+#pragma unroll
+        for (int u = 0; u < KDF_DP; u++) {
+            if (!((nr >> u) & 1u)) continue;
+            if (!first) x = xs32(x);
+            first = false;
+            const int32_t i = ii[u];
+            const float pFire = d.drive[i] * d.k.dtS;
+            const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
+            if (uu < pFire) {
+                d.v[i] = d.k.vReset;
+                d.refr[i] = forcedRefr;
+                d.spikeCount[i] += 1u;
+                d.trace[i] = 1.f;
+                d.adapt[i] = d.adapt[i] + d.k.adaptInc;
+                fm |= 1u << u;
+            }
+            if (j == n - 1) d.sc->rng = x;   // the step's last draw is the stream state after it
+            j++;
+        }
+    }
+    int32_t nf;
+    int32_t pos = wg_scan_flags<KDF_T, KDF_DP>(fm, scan[1], &nf);
+    int32_t *out = d.ring + (int64_t)slot * d.ringStride;
+#pragma unroll
+    for (int u = 0; u < KDF_DP; u++)
+        if ((fm >> u) & 1u) out[pos++] = ii[u];
+    if (tid == 0) d.sc->nDrivenFired = nf;
+}
+
+// One step's delivery and driven draws in one launch: they share no state
+// (the delivery writes the arrival accumulators; the draws read refr and
+// drive and write v, refr, trace, adapt, counts and the ring's out slot), so
+// the last workgroup draws while the others deliver.
+constexpr int DF_BLOCKS = 128;
+
+__global__ void __launch_bounds__(DF_T) k_deliver_driven(DevBrain d, int32_t head, int32_t slot) {
+    if ((int32_t)blockIdx.x == DF_BLOCKS) driven_fast(d, slot);
+    else deliver_flat(d, head, DF_BLOCKS);
+}
+
 // ------------------------------------------------------------------ host side
 
 int deliver_grid(const DevBrain &d) {
@@ -363,24 +520,16 @@ int deliver_grid(const DevBrain &d) {
 }
 
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
-                      hipStream_t st, const StepFork *fk) {
-    // k_deliver writes gE/gI accumulators and res; k_driven reads refr and
-    // drive and writes v, refr, trace, adapt, counts, the ring's out slot and
-    // the background hits: with a fork they run side by side
-    if (fk) {
-        FB_HIP(hipEventRecord(fk->fork, st));
-        FB_HIP(hipStreamWaitEvent(fk->aux, fk->fork, 0));
-        hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, fk->aux, d, slot);
-        FB_HIP(hipEventRecord(fk->join, fk->aux));
-    }
-    if (serialDeliver)
-        hipLaunchKernelGGL(k_deliver_serial, dim3(1), dim3(1024), 0, st, d, head, (int32_t)rowsUnique);
-    else
-        hipLaunchKernelGGL(k_deliver, dim3(deliver_grid(d)), dim3(256), 0, st, d, head);
-    if (fk)
-        FB_HIP(hipStreamWaitEvent(st, fk->join, 0));
-    else
+                      hipStream_t st) {
+    if (!serialDeliver && d.k.depU == 0.f && d.k.bgEvents == 0.f && d.nDriven <= KDF_T * KDF_DP) {
+        hipLaunchKernelGGL(k_deliver_driven, dim3(DF_BLOCKS + 1), dim3(DF_T), 0, st, d, head, slot);
+    } else {
+        if (serialDeliver)
+            hipLaunchKernelGGL(k_deliver_serial, dim3(1), dim3(1024), 0, st, d, head, (int32_t)rowsUnique);
+        else
+            hipLaunchKernelGGL(k_deliver, dim3(deliver_grid(d)), dim3(256), 0, st, d, head);
         hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, st, d, slot);
+    }
     if (!serialDeliver) hipLaunchKernelGGL(k_long, dim3(256), dim3(LONG_T), 0, st, d);
     hipLaunchKernelGGL(k_membrane, dim3(d.nblocks), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_compact, dim3(d.nblocks), dim3(256), 0, st, d, slot, head);

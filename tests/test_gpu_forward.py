@@ -157,3 +157,25 @@ def test_full_scale_config2_matches_oracle(c2, coba, steps, engine):
     assert c["spikes"] == int(nf.sum())
     # every arriving spike delivered its whole row
     assert c["edge_events"] > 0
+
+
+@pytest.mark.parametrize("zeros", ["+0", "-0"])
+def test_step_engines_on_bursts_and_large_driven_sets(zeros):
+    """Per-step engines where a step's arrivals exceed the flat delivery's row
+    table (a pulse fires all 8,192 neurons at once), with 3,500 driven neurons
+    (stream jumps past 2,048 positions) and a third of the weights zero: +0
+    weights are skipped, and with a -0 weight in the graph only inhibitory
+    zero edges are."""
+    g = synth.tiny_graph(8192, 30, seed=41, p_count=0.4)
+    rng = np.random.default_rng(41)
+    W = g.weights.astype(np.float32).copy()
+    W[rng.random(W.size) < 0.33] = 0.0
+    if zeros == "-0":
+        W[np.nonzero(W == 0)[0][::7]] = np.float32(-0.0)
+    s = (np.where(g.exc, 1.0, -1.0) * 0.9).astype(np.float32)
+    p = P.config_params()
+    drv = rng.choice(8192, size=3500, replace=False)
+    for engine in ("launch", "graph"):
+        gpu, nf = _compare_to_oracle(g, W, s, p, 5, drv, np.full(drv.size, 300.0), 60, engine,
+                                     extra=lambda b: b.pulse(list(range(8192)), 400.0))
+        assert nf.max() > 2048
