@@ -346,6 +346,97 @@ def bench_cord(args, lif, hiprt, stream):
     return out
 
 
+def bench_brain(args, lif, hiprt, stream):
+    """The reference's whole calibrated brain (165,122 neurons, 10.5 M entries,
+    COBA, dt 0.5 ms, adaptation on), built from its data files
+    (tests/golden/make_brain_fixture.py), on the benchmark's sugar trace:
+    olfactory receptors at 6 Hz, sugar neurons at 100 Hz for 400 ms then off
+    for 300 ms, seeds 1-4.  The mean total must equal the shipped kernel's
+    published 448,344 spikes.  Timed: the four 1,400-step runs.  Skipped when
+    the fixture is absent."""
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", "_brain", "calibrated_brain.npz")
+    if not os.path.exists(path):
+        return None
+    b = pkg("connectome").load_compact(path)
+    p = b["params"]
+    g = lif.HipGraph(b["N"], b["indptr"], b["indices"], b["W"], b["sign"])
+    lam = np.nonzero(b["bias"])[0]
+    totals, ms, ev, ev1, engines = [], 0.0, 0, 0, set()
+    for seed in (1, 2, 3, 4):
+        net = lif.LIFHip(g, p, seed=seed, engine=args.engine)
+        net.set_array("thr", b["thr"])
+        net.setBias(lam, float(b["bias"][lam[0]]))
+        net.setDrive(b["orn"], 6.0)
+        for i in b["sugar"].tolist():
+            net.setDriveOne(i, 100.0)
+        net.set_stream(stream.handle)
+        c0 = net.counters()
+        e0, e1, e2, e3 = hiprt.Event(), hiprt.Event(), hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        net.run_async(800)
+        e1.record(stream)
+        e1.synchronize()
+        for i in b["sugar"].tolist():
+            net.setDriveOne(i, 0.0)
+        e2.record(stream)
+        net.run_async(600)
+        e3.record(stream)
+        e3.synchronize()
+        ms += e0.elapsed_ms(e1) + e2.elapsed_ms(e3)
+        c1 = net.counters()
+        ev += c1["edge_events"] - c0["edge_events"]
+        if seed == 1:
+            ev1 = c1["edge_events"] - c0["edge_events"]
+        totals.append(int(net.spikeCount.sum()))
+        engines.add("persistent" if net.stat("persistent_launches") > 0 else args.engine)
+        net.destroy()
+    g.release()
+    steps = 4 * 1400
+    out = {"workload": "the reference's whole calibrated brain from its data files: 165,122 neurons, 10,511,038 "
+                       "entries (4.2 M non-zero), COBA, dt 0.5 ms; sugar trace (3,044 olfactory receptors at 6 Hz, "
+                       "34 sugar neurons at 100 Hz for 800 steps, then off for 600), seeds 1-4",
+           "engine": sorted(engines), "totals": totals, "mean_total": sum(totals) / 4,
+           "reference_mean_total": 448344.0, "matches_reference": sum(totals) / 4 == 448344.0,
+           "steps": steps, "ms_per_step": ms / steps, "edge_events_per_s": ev / (ms / 1e3),
+           "sim_s_per_wall_s": steps * p["dt"] / 1e3 / (ms / 1e3)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        kind = "ref" if orc.available("ref") else "oracle"
+
+        class _G:
+            pass
+        gg = _G()
+        gg.N, gg.indptr, gg.indices = b["N"], b["indptr"], b["indices"]
+        o = orc.OracleBrain(gg, p, seed=1, kind=kind, W=b["W"], sign=b["sign"])
+        for i in np.nonzero(b["thr"])[0].tolist():
+            o.setThr(i, float(b["thr"][i]))
+        o.setBias(lam, float(b["bias"][lam[0]]))
+        o.setDrive(b["orn"].tolist(), 6.0)
+        for i in b["sugar"].tolist():
+            o.setDriveOne(i, 100.0)
+        o._sync_driven()
+        t0 = time.perf_counter()
+        o.run(800, raster_cap=1)
+        t1 = time.perf_counter()
+        for i in b["sugar"].tolist():
+            o.setDriveOne(i, 0.0)
+        o._sync_driven()
+        t2 = time.perf_counter()
+        o.run(600, raster_cap=1)
+        dt = (t1 - t0) + (time.perf_counter() - t2)
+        out["cpu_baseline"] = {"value": ev1 / dt, "unit": "synaptic events/s", "cores": 1,
+                               "kind": "reference" if kind == "ref" else "port",
+                               "sample": "seed 1 of the same trace (1,400 steps), "
+                                         + ("lif.c compiled natively (gcc -O2 -ffp-contract=off)" if kind == "ref"
+                                            else "oracle/lif_oracle.c") + ", 1 thread",
+                               "ms_per_step": dt * 1e3 / 1400, "total_spikes": int(o.spikeCount.sum()),
+                               "same_total_as_gpu": int(o.spikeCount.sum()) == totals[0]}
+        out["gpu_over_cpu"] = out["cpu_baseline"]["ms_per_step"] / out["ms_per_step"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,6 +597,9 @@ def main():
     cord = bench_cord(args, lif, hiprt, stream)
     if cord:
         out["real_cord"] = cord
+    brain = bench_brain(args, lif, hiprt, stream)
+    if brain:
+        out["real_brain"] = brain
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:
