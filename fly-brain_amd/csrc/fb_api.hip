@@ -199,7 +199,7 @@ struct fb_brain {
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
     int32_t *gatherCnt = nullptr, *gatherBuf = nullptr;
-    std::map<int32_t, GraphCache> graphs;   // keyed by start head
+    std::map<std::pair<int32_t, int32_t>, GraphCache> graphs;   // keyed by (start head, steps)
     uint64_t version = 0;                   // bumps when DevBrain arguments change
     uint64_t drivenVersion = 0;             // bumps when the device driven list is rewritten
     uint64_t graphVersion = ~0ull;
@@ -678,20 +678,25 @@ static fb_status run_launch(fb_brain *b, int32_t nsteps) {
     return FB_OK;
 }
 
-// One hipGraph holds nslots steps starting at a given head; after a replay the
-// head is back where it started, so the same graph replays again.
+// A hipGraph holds a whole number of slot cycles starting at a given head;
+// after a replay the head is back where it started, so the same graph
+// replays again.
 static fb_status run_graph(fb_brain *b, int32_t nsteps) {
     const int32_t ns = b->d.nslots;
     if (b->graphVersion != b->version) {
         drop_graphs(b);
         b->graphVersion = b->version;
     }
+    // long runs replay graphs of at least 64 steps (whole slot cycles), so the
+    // host's launch gap between replays is paid rarely
+    const int32_t big = ns * std::max(1, (64 + ns - 1) / ns);
     while (nsteps >= ns) {
-        GraphCache &gc = b->graphs[b->head];
+        const int32_t len = nsteps >= big ? big : ns;
+        GraphCache &gc = b->graphs[{b->head, len}];
         if (!gc.exec) {
             FB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int32_t h = b->head;
-            for (int32_t s = 0; s < ns; s++) {
+            for (int32_t s = 0; s < len; s++) {
                 const int32_t slot = (h + ns - 1) % ns;
                 fb_status st = launch_step(b->d, h, slot, serial_deliver(b), b->g->rowsUnique, b->stream);
                 if (st) { hipGraph_t tmp; hipStreamEndCapture(b->stream, &tmp); if (tmp) hipGraphDestroy(tmp); return st; }
@@ -700,11 +705,11 @@ static fb_status run_graph(fb_brain *b, int32_t nsteps) {
             FB_HIP(hipStreamEndCapture(b->stream, &gc.graph));
             FB_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
             gc.head = b->head;
-            gc.steps = ns;
+            gc.steps = len;
         }
         FB_HIP(hipGraphLaunch(gc.exec, b->stream));
-        b->lastSlot = (b->head + ns - 2 + ns) % ns;   // out slot of the last step in the replay
-        nsteps -= ns;
+        b->lastSlot = (b->head + len - 1 + ns - 1) % ns;   // out slot of the last step in the replay
+        nsteps -= len;
     }
     return run_launch(b, nsteps);
 }
@@ -726,7 +731,12 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
         // the lookahead engine pays a prologue per call: worth it from two epochs on
         std::string why;
         const bool longRun = nsteps >= 2 * (b->d.nslots - 1);
+        // otherwise per-step kernels: replayed as a graph for small brains,
+        // whose kernels are shorter than their launch; launched directly for
+        // large ones, whose kernels hide the launch and where a capture after
+        // every drive change would cost more than it saves
         engine = longRun && persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
+                 : b->d.N >= 65536                                                       ? FB_ENGINE_LAUNCH
                                                                                           : FB_ENGINE_GRAPH;
     }
     switch (engine) {
@@ -740,7 +750,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st == FB_ERR_UNSUPPORTED && autoChosen) {
                 // the device refused the cooperative launch before any step ran
-                st = run_graph(b, nsteps);
+                st = b->d.N >= 65536 ? run_launch(b, nsteps) : run_graph(b, nsteps);
                 break;
             }
             if (st) return st;

@@ -455,12 +455,17 @@ __device__ __forceinline__ void driven_fast(const DevBrain &d, int32_t slot) {
     const uint32_t rng = d.sc->rng;
     const int32_t nD = d.nDriven, P = (nD + KDF_T - 1) / KDF_T;
     int32_t ii[KDF_DP];
+    float pFire[KDF_DP];
     uint32_t nr = 0;   // bit u: driven neuron u of this thread is not refractory and draws
 #pragma unroll
     for (int u = 0; u < KDF_DP; u++) {
         const int32_t q = tid * P + u;
         ii[u] = (u < P && q < nD) ? d.driven[q] : -1;
-        if (ii[u] >= 0 && !(d.refr[ii[u]] > 0)) nr |= 1u << u;
+        pFire[u] = 0.f;
+        if (ii[u] >= 0) {
+            pFire[u] = d.drive[ii[u]] * d.k.dtS;
+            if (!(d.refr[ii[u]] > 0)) nr |= 1u << u;
+        }
     }
     int32_t n;
     const int32_t off = wg_scan_flags<KDF_T, KDF_DP>(nr, scan[0], &n);
@@ -478,9 +483,8 @@ __device__ __forceinline__ void driven_fast(const DevBrain &d, int32_t slot) {
             if (!first) x = xs32(x);
             first = false;
             const int32_t i = ii[u];
-            const float pFire = d.drive[i] * d.k.dtS;
             const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
-            if (uu < pFire) {
+            if (uu < pFire[u]) {
                 d.v[i] = d.k.vReset;
                 d.refr[i] = forcedRefr;
                 d.spikeCount[i] += 1u;
@@ -530,7 +534,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
             hipLaunchKernelGGL(k_deliver, dim3(deliver_grid(d)), dim3(256), 0, st, d, head);
         hipLaunchKernelGGL(k_driven, dim3(1), dim3(1024), 0, st, d, slot);
     }
-    if (!serialDeliver) hipLaunchKernelGGL(k_long, dim3(256), dim3(LONG_T), 0, st, d);
+    if (!serialDeliver) hipLaunchKernelGGL(k_long, dim3(64), dim3(LONG_T), 0, st, d);
     hipLaunchKernelGGL(k_membrane, dim3(d.nblocks), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_compact, dim3(d.nblocks), dim3(256), 0, st, d, slot, head);
     FB_HIP(hipGetLastError());
