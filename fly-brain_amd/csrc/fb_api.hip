@@ -198,6 +198,8 @@ struct fb_brain {
     std::vector<void *> allocs;
     int32_t *ringAlloc = nullptr;
     int32_t *idxScratch = nullptr;    // device indices for multi-neuron setters
+    void *deltaBuf = nullptr;         // fb_apply_deltas staging: values, indices, kinds, group offsets
+    int64_t deltaCap = 0;
     int32_t *gatherCnt = nullptr, *gatherBuf = nullptr;
     std::map<std::pair<int32_t, int32_t>, GraphCache> graphs;   // keyed by (start head, steps)
     uint64_t version = 0;                   // bumps when DevBrain arguments change
@@ -499,6 +501,7 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
     drop_graphs(b);
     persistent_release(b->d.driven);
     for (void *p : b->allocs) hipFree(p);
+    if (b->deltaBuf) hipFree(b->deltaBuf);
     if (b->gatherBuf) hipFree(b->gatherBuf);
     if (b->gatherCnt) hipFree(b->gatherCnt);
     hipStreamDestroy(b->ownStream);
@@ -627,6 +630,92 @@ extern "C" fb_status fb_pulse(fb_brain *b, const int32_t *idx, int32_t n, double
     FB_REQUIRE(b, FB_ERR_ARG, "fb_pulse: null brain");
     if (n > 0 && std::signbit(mv)) b->gENonneg = false;
     return set_values(b, b->d.gE, idx, n, mv, 1);
+}
+
+// One thread per (array, neuron) group of a delta queue: its operations in
+// queue order (sets store f32(value), adds store f32(f64(old) + value)).
+__global__ void k_apply_deltas(DevBrain d, const int32_t *gOff, const int32_t *idx, const int32_t *kind,
+                               const double *val, int32_t ngroups) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int32_t q0 = gOff[g], q1 = gOff[g + 1];
+    const int32_t i = idx[q0];
+    float *arr = nullptr;
+    switch (kind[q0]) {
+        case FB_DELTA_DRIVE: arr = d.drive; break;
+        case FB_DELTA_BIAS: arr = d.bias; break;
+        case FB_DELTA_THR: arr = d.thr; break;
+        case FB_DELTA_GE: arr = d.gE; break;
+        default: arr = d.gI; break;
+    }
+    float x = arr[i];
+    for (int32_t q = q0; q < q1; q++) {
+        const int32_t k = kind[q];
+        x = (k == FB_DELTA_GE || k == FB_DELTA_GI) ? (float)((double)x + val[q]) : (float)val[q];
+    }
+    arr[i] = x;
+}
+
+extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind,
+                                     const double *val) {
+    FB_REQUIRE(b && n >= 0 && (n == 0 || (idx && kind && val)), FB_ERR_ARG, "fb_apply_deltas: bad arguments");
+    for (int32_t q = 0; q < n; q++) {
+        FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "fb_apply_deltas: index %d out of range", idx[q]);
+        FB_REQUIRE(kind[q] >= FB_DELTA_DRIVE && kind[q] <= FB_DELTA_GI, FB_ERR_ARG, "fb_apply_deltas: bad kind %d",
+                   kind[q]);
+    }
+    if (n == 0) return FB_OK;
+    // host side of the setters, in queue order: the driven set and the sign
+    // bookkeeping of the excitatory conductances
+    for (int32_t q = 0; q < n; q++) {
+        const int32_t i = idx[q];
+        if (kind[q] == FB_DELTA_DRIVE) {
+            if (val[q] > 0) {
+                if (!b->inDriven[i]) { b->inDriven[i] = 1; b->drivenList.push_back(i); b->drivenDirty = true; }
+            } else if (b->inDriven[i]) {
+                b->inDriven[i] = 0;
+                b->drivenList.erase(std::find(b->drivenList.begin(), b->drivenList.end(), i));
+                b->drivenDirty = true;
+            }
+        } else if (kind[q] == FB_DELTA_GE && std::signbit(val[q])) {
+            b->gENonneg = false;
+        }
+    }
+    // group by (array, neuron), keeping queue order inside a group
+    auto arrayOf = [](int32_t k) { return k <= FB_DELTA_THR ? k : (k == FB_DELTA_GE ? 3 : 4); };
+    std::vector<int32_t> perm((size_t)n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) {
+        const int32_t ax = arrayOf(kind[x]), ay = arrayOf(kind[y]);
+        return ax != ay ? ax < ay : idx[x] < idx[y];
+    });
+    std::vector<int32_t> sIdx((size_t)n), sKind((size_t)n), gOff;
+    std::vector<double> sVal((size_t)n);
+    for (int32_t q = 0; q < n; q++) {
+        const int32_t p = perm[(size_t)q];
+        sIdx[(size_t)q] = idx[p]; sKind[(size_t)q] = kind[p]; sVal[(size_t)q] = val[p];
+        if (q == 0 || idx[p] != sIdx[(size_t)q - 1] || arrayOf(kind[p]) != arrayOf(sKind[(size_t)q - 1]))
+            gOff.push_back(q);
+    }
+    const int32_t ng = (int32_t)gOff.size();
+    gOff.push_back(n);
+    if ((int64_t)n > b->deltaCap) {
+        if (b->deltaBuf) FB_HIP(hipFree(b->deltaBuf));
+        b->deltaBuf = nullptr;
+        b->deltaCap = std::max<int64_t>((int64_t)n, 4096);
+        FB_HIP(hipMalloc(&b->deltaBuf, (size_t)b->deltaCap * (sizeof(double) + 3 * sizeof(int32_t)) + 4 * sizeof(int32_t)));
+    }
+    double *dVal = (double *)b->deltaBuf;
+    int32_t *dIdx = (int32_t *)(dVal + b->deltaCap), *dKind = dIdx + b->deltaCap, *dOff = dKind + b->deltaCap;
+    FB_HIP(hipMemcpyAsync(dVal, sVal.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dIdx, sIdx.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dKind, sKind.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dOff, gOff.data(), sizeof(int32_t) * ((size_t)ng + 1), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_apply_deltas, dim3((ng + 255) / 256), dim3(256), 0, b->stream, b->d, (const int32_t *)dOff,
+                       (const int32_t *)dIdx, (const int32_t *)dKind, (const double *)dVal, ng);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(b->stream));   // the host vectors and the staging buffer are reused
+    return FB_OK;
 }
 
 extern "C" fb_status fb_set_background(fb_brain *b, double hz, double mv) {

@@ -247,6 +247,24 @@ napi_value add_g(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
+// applyDeltas(b, Int32Array idx, Int32Array kind, Float64Array val, n): the first n queued setter calls
+napi_value apply_deltas(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 5)) return nullptr;
+    BRAIN_OR_NULL(0);
+    int32_t *ix, *kd;
+    double *vl;
+    size_t ni, nk, nv;
+    if (!typed(env, a.v[1], &ix, &ni) || !typed(env, a.v[2], &kd, &nk) || !typed(env, a.v[3], &vl, &nv)) return nullptr;
+    const int32_t n = i32(env, a.v[4]);
+    if (n < 0 || (size_t)n > ni || (size_t)n > nk || (size_t)n > nv) {
+        napi_throw_range_error(env, nullptr, "flybrain napi: applyDeltas count exceeds the arrays");
+        return nullptr;
+    }
+    FB_OK_OR_THROW(fb_apply_deltas(b, n, ix, kd, vl), "applyDeltas");
+    return nullptr;
+}
+
 napi_value set_background(napi_env env, napi_callback_info info) {
     Args a;
     if (!get_args(env, info, a, 3)) return nullptr;
@@ -416,6 +434,7 @@ napi_value init(napi_env env, napi_value exports) {
         {"setBias", list_setter<fb_set_bias>},
         {"setThr", set_thr},
         {"addG", add_g},
+        {"applyDeltas", apply_deltas},
         {"pulse", list_setter<fb_pulse>},
         {"setBackground", set_background},
         {"step", step},

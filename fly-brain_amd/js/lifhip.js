@@ -10,7 +10,22 @@
 // state arrays here are host copies fetched when read after a step, as
 // LIFGpu's shadows are.  drive, thr and bias are written through by the
 // setters; direct element writes into those three arrays reach the device on
-// flush().  Written in Node-12 syntax (CommonJS, no optional chaining).
+// flush().  Setter calls queue up and reach the device as one batch
+// (fb_apply_deltas) before anything that depends on them, in call order.
+//
+// Two modes:
+//   exact (default)  LIFWasm's semantics: step() runs one step and returns
+//                    its fired list; setters act before the next step.
+//   live: true       LIFGpu's protocol (src/lifgpu.js:276-346): step() queues
+//                    the step and returns the fired list of the last batch;
+//                    a batch is submitted after batchSteps steps (8) or
+//                    batchMs milliseconds (2), setter calls made during a
+//                    batch act after its steps (those made between batches,
+//                    before the next one), and the state arrays lag by at
+//                    most one batch.  flush() submits.  Batches of steps
+//                    run as one engine call, so long enough batches take the
+//                    lookahead engine.
+// Written in Node-12 syntax (CommonJS, no optional chaining).
 const path = require('path');
 
 const native = require(path.join(__dirname, '..', 'lib', 'flybrain_napi.node'));
@@ -26,6 +41,10 @@ const PARAM_FIELDS = ['dt', 'vRest', 'vThresh', 'vReset', 'tauM', 'tauSyn', 'tRe
   'adaptInc', 'adaptTau', 'depU', 'depTau', 'coba', 'eExc', 'eInh', 'bgRate', 'bgAmp', 'preInh'];
 const ARRAYS = ['v', 'gE', 'gI', 'refr', 'trace', 'adapt', 'res', 'bias', 'thr', 'drive', 'spikeCount'];
 const HOST_OWNED = ['bias', 'thr', 'drive'];
+// fb_apply_deltas kinds (include/flybrain.h)
+const DELTA = { drive: 0, bias: 1, thr: 2, gE: 3, gI: 4 };
+const DELTA_CAP = 65536;   // a full queue is applied at once
+const now = () => Number(process.hrtime.bigint()) / 1e6;
 const ENGINES = { auto: 0, launch: 1, graph: 2, persistent: 3 };
 
 function paramsVec(p) {
@@ -92,6 +111,59 @@ class LIFHip {
       this._stale[k] = true;
     }
     this._fired = new Int32Array(2 * N);
+    // setter queue
+    this._qIdx = new Int32Array(256);
+    this._qKind = new Int32Array(256);
+    this._qVal = new Float64Array(256);
+    this._qn = 0;
+    // LIFGpu's batching protocol
+    this.live = !!opts.live;
+    this.batchSteps = opts.batchSteps || 8;
+    this.batchMs = opts.batchMs === undefined ? 2 : opts.batchMs;
+    this._pending = 0;
+    this._lastSubmit = now();
+    this._lastFired = new Int32Array(0);
+  }
+
+  _push(i, kind, v) {
+    if (this._qn === this._qIdx.length) {
+      const grow = (a, C) => { const b = new C(2 * a.length); b.set(a); return b; };
+      this._qIdx = grow(this._qIdx, Int32Array);
+      this._qKind = grow(this._qKind, Int32Array);
+      this._qVal = grow(this._qVal, Float64Array);
+    }
+    this._qIdx[this._qn] = i;
+    this._qKind[this._qn] = kind;
+    this._qVal[this._qn] = v;
+    this._qn++;
+    if (this._qn >= DELTA_CAP) {
+      if (this.live) this._submit(); else this._applyQueue();
+    }
+  }
+
+  _applyQueue() {
+    if (!this._qn) return;
+    native.applyDeltas(this._h, this._qIdx, this._qKind, this._qVal, this._qn);
+    this._qn = 0;
+    for (const k of ['gE', 'gI']) this._stale[k] = true;
+  }
+
+  // exact mode: queued setters reach the device; live mode: the open batch is
+  // submitted (its steps, then the setters called during it)
+  _settle() {
+    if (this.live) this._submit(); else this._applyQueue();
+  }
+
+  _submit() {
+    if (this._pending) {
+      const nf = native.step(this._h, this._pending, this.engine);
+      this._pending = 0;
+      this._deviceMoved();
+      const n = native.fired(this._h, this._fired);
+      this._lastFired = this._fired.slice(0, Math.min(n, nf));
+    }
+    this._applyQueue();
+    this._lastSubmit = now();
   }
 
   /** LIFGpu.create's shape (src/lifgpu.js:185): resolves to a ready brain. */
@@ -117,6 +189,7 @@ class LIFHip {
 
   _array(k) {
     if (this._stale[k]) {
+      if (!this.live) this._applyQueue();
       native.getArray(this._h, ARRAYS.indexOf(k), this._host[k]);
       this._stale[k] = false;
     }
@@ -139,8 +212,19 @@ class LIFHip {
   get nAwake() { return this.N; }
   wake() {}
 
-  /** One step; returns the fired list in lif.c order (a view, valid until the next step). */
+  /** One step.  Exact mode: runs it and returns its fired list in lif.c
+   *  order (a view, valid until the next step).  Live mode: queues it and
+   *  returns the fired list of the last submitted batch. */
   step() {
+    if (this.live) {
+      // setters called while no batch was open act before the new batch
+      if (this._pending === 0) this._applyQueue();
+      this._pending++;
+      this.t += this.p.dt;
+      if (this._pending >= this.batchSteps || now() - this._lastSubmit >= this.batchMs) this._submit();
+      return this._lastFired;
+    }
+    this._applyQueue();
     const nf = native.step(this._h, 1, this.engine);
     this.t += this.p.dt;
     this._deviceMoved();
@@ -150,6 +234,7 @@ class LIFHip {
 
   /** Many steps off the JS thread; resolves to the spike count of the last step. */
   run(steps) {
+    this._settle();
     return native.runAsync(this._h, steps, this.engine).then((nf) => {
       this.t += steps * this.p.dt;
       this._deviceMoved();
@@ -158,48 +243,49 @@ class LIFHip {
   }
 
   setDriveOne(i, hz) {
-    native.setDriveOne(this._h, i, hz);
+    this._push(i, DELTA.drive, hz);
     this._host.drive[i] = hz;
     if (hz > 0) this.drivenSet.add(i); else this.drivenSet.delete(i);
   }
 
   setDrive(ix, hz) {
-    native.setDrive(this._h, Int32Array.from(ix), hz);
-    for (const i of ix) {
-      this._host.drive[i] = hz;
-      if (hz > 0) this.drivenSet.add(i); else this.drivenSet.delete(i);
-    }
+    for (const i of ix) this.setDriveOne(i, hz);
   }
 
   setBias(ix, mv) {
-    native.setBias(this._h, Int32Array.from(ix), mv);
-    for (const i of ix) this._host.bias[i] = mv;
+    for (const i of ix) {
+      this._push(i, DELTA.bias, mv);
+      this._host.bias[i] = mv;
+    }
   }
 
   setThr(i, mv) {
-    native.setThr(this._h, i, mv);
+    this._push(i, DELTA.thr, mv);
     this._host.thr[i] = mv;
   }
 
   addG(i, e, ii) {
-    native.addG(this._h, i, e, ii || 0);
-    this._stale.gE = true;
-    this._stale.gI = true;
+    this._push(i, DELTA.gE, e);
+    this._push(i, DELTA.gI, ii || 0);
   }
 
   pulse(ix, mv) {
-    native.pulse(this._h, Int32Array.from(ix), mv);
-    this._stale.gE = true;
+    for (const i of ix) this._push(i, DELTA.gE, mv);
   }
 
   reset() {
+    this._settle();
     native.reset(this._h);
     this._deviceMoved();
   }
 
-  setBackground(hz, mv) { native.setBackground(this._h, hz, mv); }
+  setBackground(hz, mv) {
+    this._settle();
+    native.setBackground(this._h, hz, mv);
+  }
 
   setParams(q) {
+    this._settle();
     Object.assign(this.p, q);
     native.setParams(this._h, paramsVec(this.p));
     this.nslots = nslotsOf(this.p);
@@ -208,12 +294,14 @@ class LIFHip {
   /** Pushes direct element writes of drive / thr / bias to the device and
    *  drops the cached state arrays. */
   flush() {
+    this._settle();
     for (const k of HOST_OWNED) if (!this._stale[k]) native.setArray(this._h, ARRAYS.indexOf(k), this._host[k]);
     this._deviceMoved();
   }
 
   /** Work counters: edge events, synapse events, spikes, steps. */
   counters() {
+    if (!this.live) this._applyQueue();
     const c = native.counters(this._h);
     return { edgeEvents: c[0], synapseEvents: c[1], spikes: c[2], steps: c[3] };
   }
@@ -221,6 +309,8 @@ class LIFHip {
   get rng() { return native.scalars(this._h)[0]; }
 
   destroy() {
+    this._qn = 0;
+    this._pending = 0;
     if (this._h) { native.brainDestroy(this._h); this._h = null; }
     if (this._graph) { native.graphRelease(this._graph); this._graph = null; }
   }

@@ -188,6 +188,31 @@ class LIFHip:
     def addG(self, i: int, e: float, ii: float = 0.0):
         _l.check(self.lib.fb_add_g(self.h, int(i), float(e), float(ii or 0.0)))
 
+    # fb_apply_deltas kinds (include/flybrain.h)
+    DELTA_DRIVE, DELTA_BIAS, DELTA_THR, DELTA_GE, DELTA_GI = range(5)
+
+    def apply_deltas(self, idx, kind, val):
+        """A queue of setter calls in one launch: element q acts as setDriveOne,
+        setBias (one index), setThr, or addG on one conductance, in order."""
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        kd = np.ascontiguousarray(kind, dtype=np.int32)
+        vl = np.ascontiguousarray(val, dtype=np.float64)
+        if not (ix.size == kd.size == vl.size):
+            raise ValueError("idx, kind and val must have one entry per call")
+        _l.check(self.lib.fb_apply_deltas(self.h, int(ix.size), _l.ptr(ix), _l.ptr(kd), _l.ptr(vl)))
+        for i, k, v in zip(ix.tolist(), kd.tolist(), vl.tolist()):
+            if k == self.DELTA_DRIVE:
+                self._drive[i] = v
+                if v > 0:
+                    if i not in self.drivenSet:
+                        self.drivenSet.append(i)
+                elif i in self.drivenSet:
+                    self.drivenSet.remove(i)
+            elif k == self.DELTA_BIAS:
+                self._bias[i] = v
+            elif k == self.DELTA_THR:
+                self._thr[i] = v
+
     def pulse(self, ix, mv: float):
         idx = np.ascontiguousarray(list(ix) if not isinstance(ix, np.ndarray) else ix, dtype=np.int32)
         _l.check(self.lib.fb_pulse(self.h, _l.ptr(idx), int(idx.size), float(mv)))

@@ -108,6 +108,15 @@ fb_status fb_set_thr(fb_brain *b, int32_t i, double mv);
 fb_status fb_add_g(fb_brain *b, int32_t i, double e, double ii);
 fb_status fb_pulse(fb_brain *b, const int32_t *idx, int32_t n, double mv);
 fb_status fb_set_background(fb_brain *b, double hz, double mv);
+/* A queue of setter calls applied in order, with one upload and one launch:
+ * element q acts as fb_set_drive_one (FB_DELTA_DRIVE), fb_set_bias with one
+ * index (FB_DELTA_BIAS), fb_set_thr (FB_DELTA_THR), or fb_add_g adding to one
+ * conductance (FB_DELTA_GE, FB_DELTA_GI) would on idx[q] with val[q].  The
+ * batched form of LIFGpu's delta queue (src/lifgpu.js:140-148, 321-346):
+ * callers that set thousands of thresholds per frame (src/sim/neuromod.js
+ * modulate()) pay one launch instead of one per call. */
+enum { FB_DELTA_DRIVE = 0, FB_DELTA_BIAS = 1, FB_DELTA_THR = 2, FB_DELTA_GE = 3, FB_DELTA_GI = 4 };
+fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind, const double *val);
 /* the driven set in insertion order (drivenSet of src/lifwasm.js:51,70-71) */
 fb_status fb_driven(const fb_brain *b, int32_t *out, int32_t cap, int32_t *n);
 

@@ -11,7 +11,8 @@ if (spec.exportsOnly) {
   process.stdout.write(JSON.stringify({ exports: Object.keys(native).sort(), paramsLength: native.paramsLength() }));
   process.exit(0);
 }
-const brain = new LIFHip({ graph: spec.graph, N: spec.N, params: spec.params, seed: spec.seed, engine: spec.engine });
+const brain = new LIFHip({ graph: spec.graph, N: spec.N, params: spec.params, seed: spec.seed, engine: spec.engine,
+  live: spec.live, batchSteps: spec.batchSteps, batchMs: spec.batchMs });
 const nfired = [];
 const raster = [];
 const run = async () => {
@@ -21,6 +22,9 @@ const run = async () => {
     else if (op[0] === 'setBias') brain.setBias(op[1], op[2]);
     else if (op[0] === 'setThr') brain.setThr(op[1], op[2]);
     else if (op[0] === 'pulse') brain.pulse(op[1], op[2]);
+    else if (op[0] === 'addG') brain.addG(op[1], op[2], op[3]);
+    else if (op[0] === 'setThrMany') { for (const [i, mv] of op[1]) brain.setThr(i, mv); }
+    else if (op[0] === 'flush') brain.flush();
     else if (op[0] === 'steps') {
       for (let s = 0; s < op[1]; s++) {
         const f = brain.step();

@@ -179,3 +179,47 @@ def test_step_engines_on_bursts_and_large_driven_sets(zeros):
         gpu, nf = _compare_to_oracle(g, W, s, p, 5, drv, np.full(drv.size, 300.0), 60, engine,
                                      extra=lambda b: b.pulse(list(range(8192)), 400.0))
         assert nf.max() > 2048
+
+
+def test_apply_deltas_matches_single_setters():
+    """fb_apply_deltas acts as the same setter calls made one by one, in order:
+    repeated sets (the last wins), repeated additions to one conductance
+    (summed in call order), drive removed and added back (to the end of the
+    driven list)."""
+    g = synth.tiny_graph(3000, 30, seed=43, p_count=0.3)
+    s = (np.where(g.exc, 1.0, -1.0) * 0.7).astype(np.float32)
+    p = P.config_params()
+    rng = np.random.default_rng(43)
+    drv = rng.choice(3000, 300, replace=False).tolist()
+    calls = [(L_DRIVE, i, 250.0) for i in drv]
+    for i, v in zip(rng.integers(0, 3000, 4000).tolist(), rng.normal(0, 2, 4000).tolist()):
+        calls.append((L_THR, i, v))
+    for i in rng.integers(0, 50, 400).tolist():
+        calls.append((L_GE if i % 3 else L_GI, i, float(rng.normal(0.5, 1.0))))
+    calls += [(L_BIAS, 7, 3.0), (L_BIAS, 7, 4.5), (L_DRIVE, drv[3], 0.0), (L_DRIVE, drv[3], 180.0)]
+    a = _hip(g, g.weights, s, p, 6, "auto")
+    b = _hip(g, g.weights, s, p, 6, "auto")
+    for k, i, v in calls:
+        if k == L_DRIVE:
+            a.setDriveOne(i, v)
+        elif k == L_BIAS:
+            a.setBias([i], v)
+        elif k == L_THR:
+            a.setThr(i, v)
+        elif k == L_GE:
+            a.addG(i, v, 0.0)
+        else:
+            a.addG(i, 0.0, v)
+    print("apply_deltas: one batch of", len(calls), "calls", flush=True)
+    b.apply_deltas([c[1] for c in calls], [c[0] for c in calls], [c[2] for c in calls])
+    for k in ("thr", "bias", "drive", "gE", "gI"):   # the device arrays
+        assert np.array_equal(a._get(k).view(np.uint32), b._get(k).view(np.uint32)), k
+    assert np.array_equal(a.thr, b.thr) and np.array_equal(a.drive, b.drive)   # host shadows
+    assert a.drivenSet == b.drivenSet
+    na, ra = a.run_record(300, cap=300 * 3000)
+    nb, rb = b.run_record(300, cap=300 * 3000)
+    assert np.array_equal(na, nb) and np.array_equal(ra, rb)
+    assert np.array_equal(a.spikeCount, b.spikeCount)
+
+
+L_DRIVE, L_BIAS, L_THR, L_GE, L_GI = range(5)

@@ -53,3 +53,87 @@ def test_lifhip_from_node_matches_oracle(tmp_path, engine):
     assert res["rng"] == o.rng
     assert res["drivenSet"] == drv
     assert res["t"] == pytest.approx(760 * p["dt"])
+
+
+def _node_case(tmp_path, spec):
+    f = tmp_path / "case.json"
+    f.write_text(json.dumps(spec))
+    out = subprocess.run(["node", os.path.join(HERE, "js", "napi_run.js"), str(f)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout)
+
+
+def _graph_spec(g, sign):
+    return {"indptr": g.indptr.tolist(), "indices": g.indices.tolist(), "weights": g.weights.tolist(),
+            "sign": sign.tolist(), "counts": g.counts.tolist()}
+
+
+def test_queued_setters_match_oracle(tmp_path):
+    """Setter calls queue up in JS and reach the device as one fb_apply_deltas
+    batch before the next step: thousands of thresholds, repeated additions to
+    one conductance (their order matters), a driven neuron removed and added
+    back (it moves to the end of the driven list)."""
+    g = synth.tiny_graph(1500, 40, seed=29, p_count=0.3)
+    sign = (np.where(g.exc, 1.0, -1.0) * 0.6).astype(np.float32)
+    p = P.config_params()
+    rng = np.random.default_rng(29)
+    drv = list(range(0, 1500, 7))
+    thr = [[int(i), float(v)] for i, v in zip(rng.integers(0, 1500, 3000), rng.normal(0, 2, 3000))]
+    adds = [["addG", 11, 0.7, 0.0], ["addG", 11, 1.3e-3, -0.2], ["addG", 12, 2.0, 0.5], ["addG", 11, 0.25, 0.0]]
+    ops = [["setDrive", drv, 220.0], ["steps", 40], ["setThrMany", thr], *adds, ["pulse", [5, 5, 6], 1.5],
+           ["setDriveOne", 14, 0.0], ["steps", 30], ["setDriveOne", 14, 300.0], ["setBias", [3, 4, 3], 2.5],
+           ["steps", 60]]
+    spec = {"N": g.N, "seed": 9, "engine": "auto", "params": p, "graph": _graph_spec(g, sign), "ops": ops}
+    print("node LIFHip: queued setters", flush=True)
+    res = _node_case(tmp_path, spec)
+    o = OracleBrain(g, p, seed=9, W=g.weights, sign=sign)
+    o.setDrive(drv, 220.0)
+    nfs, ras = [], []
+    nf, r, _ = o.run(40, raster_cap=40 * g.N); nfs += nf.tolist(); ras += r.tolist()
+    for i, v in thr:
+        o.setThr(i, v)
+    for _, i, e, ii in adds:
+        o.addG(i, e, ii)
+    o.pulse([5, 5, 6], 1.5)
+    o.setDriveOne(14, 0.0)
+    nf, r, _ = o.run(30, raster_cap=30 * g.N); nfs += nf.tolist(); ras += r.tolist()
+    o.setDriveOne(14, 300.0)
+    o.setBias([3, 4, 3], 2.5)
+    nf, r, _ = o.run(60, raster_cap=60 * g.N); nfs += nf.tolist(); ras += r.tolist()
+    assert res["nfired"] == nfs
+    assert res["raster"] == ras
+    assert res["spikeCount"] == o.spikeCount.tolist()
+    assert np.array_equal(np.array(res["v"], np.float32).view(np.uint32), o.array("v").view(np.uint32))
+    assert res["drivenSet"] == [i for i in drv if i != 14] + [14]
+
+
+def test_live_mode_batches_like_lifgpu(tmp_path):
+    """LIFGpu's protocol (src/lifgpu.js:276-346): steps run in batches of 8,
+    setter calls made during a batch act after its steps, those made between
+    batches before the next; after flush() the state equals lif.c run with the
+    setters at those batch boundaries."""
+    g = synth.tiny_graph(1500, 40, seed=31, p_count=0.3)
+    sign = (np.where(g.exc, 1.0, -1.0) * 0.6).astype(np.float32)
+    p = P.config_params()
+    drv = list(range(0, 1500, 9))
+    ops = [["setDrive", drv, 220.0], ["steps", 5], ["setThr", 7, 3.0], ["addG", 11, 2.0, 0.0], ["steps", 11],
+           ["setBias", [20, 21], 4.0], ["steps", 8], ["flush"], ["setThr", 8, -1.0], ["steps", 3], ["flush"]]
+    spec = {"N": g.N, "seed": 4, "engine": "auto", "params": p, "graph": _graph_spec(g, sign), "ops": ops,
+            "live": True, "batchSteps": 8, "batchMs": 1e9}
+    print("node LIFHip: live mode", flush=True)
+    res = _node_case(tmp_path, spec)
+    o = OracleBrain(g, p, seed=4, W=g.weights, sign=sign)
+    o.setDrive(drv, 220.0)
+    o.run(8, raster_cap=8 * g.N)            # batch 1: steps 1-8
+    o.setThr(7, 3.0)                        # called during batch 1
+    o.addG(11, 2.0, 0.0)
+    o.run(8, raster_cap=8 * g.N)            # batch 2: steps 9-16 (submitted by the 16th step)
+    o.setBias([20, 21], 4.0)                # called between batches: before batch 3
+    o.run(8, raster_cap=8 * g.N)            # batch 3: steps 17-24
+    o.setThr(8, -1.0)
+    o.run(3, raster_cap=3 * g.N)            # batch 4, submitted by flush()
+    assert res["spikeCount"] == o.spikeCount.tolist()
+    assert np.array_equal(np.array(res["v"], np.float32).view(np.uint32), o.array("v").view(np.uint32))
+    assert res["rng"] == o.rng
+    assert res["t"] == pytest.approx(27 * p["dt"])
