@@ -14,6 +14,12 @@
  *     surrogate (fast sigmoid, or the logistic's own
  *     derivative in soft mode), reversed arrivals,
  *     adjoint clip, truncation, non-finite sanitising
+ *   - the drive as an input: a schedule of drive rates  src/lifdiff.js:135-144, 228-234,
+ *     per epoch for the neurons of driveIdx, applied     338, 406-413, 453, 518-527, 565-605
+ *     by setDriveOne at each epoch start (the onEpoch
+ *     hook), driveSoft (the drive's expectation as a
+ *     graded spike), the drive gradient per epoch, and
+ *     per-epoch spike weights (dLdSpikePerEpoch)
  * The reference evaluates in JavaScript doubles and stores into Float32Arrays:
  * every local below is a double and every array store rounds to binary32, in
  * the reference's evaluation order.
@@ -49,7 +55,10 @@
 typedef struct {
     float *v, *gE, *gI, *adapt, *res, *refr;
     uint32_t rng;
+    int32_t *drv, nDrv;   /* the driven set at the checkpoint (it changes under a drive schedule) */
 } ckpt_t;
+
+#define DRIVE_MAX 0.999   /* src/lifdiff.js:72 */
 
 struct fdo_net {
     int32_t N;
@@ -82,6 +91,18 @@ struct fdo_net {
     double *rate;
     ckpt_t *ck;
     int32_t nck;
+    /* drive schedule: rates per epoch of `ep` steps for the neurons drvIdx[0..nDI) */
+    int32_t nDI, ep, nEp, driveSoft;
+    int32_t *drvIdx, *drvSlot;
+    float *drvSched;
+    float *gDrive;                /* [nEp][nDI] after backward */
+    float *dPer;                  /* [nEpPer][N] per-epoch spike weights, or NULL */
+    int32_t nEpPer;
+    double lossPer;               /* the per-epoch weighted loss of the last forward */
+    /* the drive and driven set of a run in progress (forward or replay) */
+    float *wdrive;
+    int32_t *wdrv, nW;
+    uint8_t *inW;
 };
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
@@ -123,6 +144,12 @@ fdo_net *fdo_create(int32_t N, int64_t E, const uint32_t *indptr, const uint32_t
     n->dLdSpike = (float *)xcalloc(N, 4);
     n->readIdx = (int32_t *)xcalloc(N, 4);
     for (int32_t i = 0; i < N; i++) n->readIdx[i] = -1;
+    n->drvSlot = (int32_t *)xcalloc(N, 4);
+    for (int32_t i = 0; i < N; i++) n->drvSlot[i] = -1;
+    n->ep = 1;
+    n->wdrive = (float *)xcalloc(N, 4);
+    n->wdrv = (int32_t *)xcalloc(N, 4);
+    n->inW = (uint8_t *)xcalloc(N, 1);
     return n;
 }
 
@@ -131,7 +158,7 @@ static void free_tape(fdo_net *n) {
     n->off = NULL; n->idx = NULL; n->amp = NULL; n->readS = NULL; n->rate = NULL;
     for (int32_t c = 0; c < n->nck; c++) {
         free(n->ck[c].v); free(n->ck[c].gE); free(n->ck[c].gI);
-        free(n->ck[c].adapt); free(n->ck[c].res); free(n->ck[c].refr);
+        free(n->ck[c].adapt); free(n->ck[c].res); free(n->ck[c].refr); free(n->ck[c].drv);
     }
     free(n->ck);
     n->ck = NULL;
@@ -146,7 +173,77 @@ void fdo_destroy(fdo_net *n) {
     for (int k = 0; k < FDO_NSTATE; k++) free(n->st[k]);
     free(n->thrMask); free(n->biasMask); free(n->driven); free(n->scored); free(n->dLdSpike);
     free(n->readout); free(n->readIdx); free(n->target);
+    free(n->drvIdx); free(n->drvSlot); free(n->drvSched); free(n->gDrive); free(n->dPer);
+    free(n->wdrive); free(n->wdrv); free(n->inW);
     free(n);
+}
+
+/* driveIdx / driveEpoch with the rates the onEpoch hook would set
+ * (src/lifdiff.js:135-144, 228-234): at the start of epoch e the rate of
+ * drvIdx[k] becomes sched[e*nD + k], through setDriveOne's set semantics.
+ * driveSoft replaces the Poisson drive by its expectation (src/lifdiff.js:338). */
+void fdo_set_drive_schedule(fdo_net *n, const int32_t *idx, int32_t nD, int32_t epochSteps, const float *sched,
+                            int32_t nEp, int32_t driveSoft) {
+    for (int32_t k = 0; k < n->nDI; k++) n->drvSlot[n->drvIdx[k]] = -1;
+    free(n->drvIdx); free(n->drvSched); free(n->gDrive);
+    n->drvIdx = NULL; n->drvSched = NULL; n->gDrive = NULL;
+    n->nDI = idx && sched ? nD : 0;
+    n->nEp = n->nDI ? nEp : 0;
+    n->ep = epochSteps > 0 ? epochSteps : 1;
+    n->driveSoft = driveSoft;
+    if (!n->nDI) return;
+    n->drvIdx = (int32_t *)xcalloc(nD, 4);
+    memcpy(n->drvIdx, idx, 4 * (size_t)nD);
+    for (int32_t k = 0; k < nD; k++) n->drvSlot[idx[k]] = k;
+    n->drvSched = (float *)xcalloc((size_t)nEp * nD, 4);
+    memcpy(n->drvSched, sched, 4 * (size_t)nEp * nD);
+    n->gDrive = (float *)xcalloc((size_t)nEp * nD, 4);
+}
+
+/* dLdSpikePerEpoch (src/lifdiff.js:519-522): the weight of a spike of neuron i
+ * at a step of epoch e is dPer[e*N + i] instead of dLdSpike[i]. */
+void fdo_set_loss_per_epoch(fdo_net *n, const float *dPer, int32_t nEp, int32_t epochSteps) {
+    free(n->dPer);
+    n->dPer = NULL;
+    n->nEpPer = 0;
+    if (!dPer) return;
+    n->dPer = (float *)xcalloc((size_t)nEp * n->N, 4);
+    memcpy(n->dPer, dPer, 4 * (size_t)nEp * n->N);
+    n->nEpPer = nEp;
+    n->ep = epochSteps > 0 ? epochSteps : 1;
+}
+
+void fdo_drive_grad(const fdo_net *n, float *gDrive) {
+    if (n->gDrive && gDrive) memcpy(gDrive, n->gDrive, 4 * (size_t)n->nEp * n->nDI);
+}
+
+/* setDriveOne on the run's working drive and driven set (src/lifdiff.js:205-207):
+ * an added neuron goes to the end, a removed one leaves its place */
+static void drive_set_one(fdo_net *n, int32_t i, float rate) {
+    n->wdrive[i] = rate;
+    if (rate > 0) {
+        if (!n->inW[i]) { n->inW[i] = 1; n->wdrv[n->nW++] = i; }
+    } else if (n->inW[i]) {
+        n->inW[i] = 0;
+        int32_t w = 0;
+        for (int32_t q = 0; q < n->nW; q++) if (n->wdrv[q] != i) n->wdrv[w++] = n->wdrv[q];
+        n->nW = w;
+    }
+}
+
+/* the schedule's rates for epoch e (the last epoch's rates hold beyond it) */
+static void apply_epoch(fdo_net *n, int32_t e) {
+    if (!n->nDI) return;
+    if (e >= n->nEp) e = n->nEp - 1;
+    for (int32_t k = 0; k < n->nDI; k++) drive_set_one(n, n->drvIdx[k], n->drvSched[(int64_t)e * n->nDI + k]);
+}
+
+/* the working drive and driven set at the start of a run: the inputs */
+static void start_drive(fdo_net *n) {
+    memcpy(n->wdrive, n->in[FDO_DRIVE], 4 * (size_t)n->N);
+    memset(n->inW, 0, (size_t)n->N);
+    n->nW = 0;
+    for (int32_t q = 0; q < n->nDriven; q++) { n->wdrv[n->nW++] = n->driven[q]; n->inW[n->driven[q]] = 1; }
 }
 
 void fdo_set_input(fdo_net *n, int32_t which, const float *values) {
@@ -290,6 +387,10 @@ static void save_ckpt(fdo_net *n, int32_t c) {
     memcpy(ck->adapt, n->st[FDO_ADAPT], sz); memcpy(ck->res, n->st[FDO_RES], sz);
     memcpy(ck->refr, n->st[FDO_REFR], sz);
     ck->rng = n->rng;
+    free(ck->drv);
+    ck->drv = (int32_t *)xcalloc(n->nW, 4);
+    memcpy(ck->drv, n->wdrv, 4 * (size_t)n->nW);
+    ck->nDrv = n->nW;
 }
 
 static void restore_ckpt(fdo_net *n, int32_t c) {
@@ -299,6 +400,11 @@ static void restore_ckpt(fdo_net *n, int32_t c) {
     memcpy(n->st[FDO_ADAPT], ck->adapt, sz); memcpy(n->st[FDO_RES], ck->res, sz);
     memcpy(n->st[FDO_REFR], ck->refr, sz);
     n->rng = ck->rng;
+    memcpy(n->wdrive, n->in[FDO_DRIVE], sz);
+    memset(n->inW, 0, (size_t)n->N);
+    n->nW = ck->nDrv;
+    memcpy(n->wdrv, ck->drv, 4 * (size_t)ck->nDrv);
+    for (int32_t q = 0; q < n->nW; q++) n->inW[n->wdrv[q]] = 1;
 }
 
 /* filtered rates of the readouts and the loss terms */
@@ -337,24 +443,29 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
     }
     n->rng = fbo_seed_hash(seed);
     reset_state(n);
+    start_drive(n);
+    n->lossPer = 0.0;
     float *v = n->st[FDO_V], *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *adapt = n->st[FDO_ADAPT];
     float *res = n->st[FDO_RES], *refr = n->st[FDO_REFR], *sc = n->st[FDO_SPIKECOUNT];
-    const float *drive = n->in[FDO_DRIVE], *thrOff = n->in[FDO_THROFFSET];
+    const float *drive = n->wdrive, *thrOff = n->in[FDO_THROFFSET];
     for (int32_t t = 0; t < steps; t++) {
         if (record && t % n->K == 0) save_ckpt(n, t / n->K);
+        if (n->nDI && t % n->ep == 0) apply_epoch(n, t / n->ep);   /* the onEpoch hook */
         n->off[t + 1] = n->off[t];
         if (t >= D) deliver(n, n->off[t - D], n->off[t - D + 1], NULL);
         n->steps = t;
         const int scoredStep = t >= n->lossFrom;
+        const float *wPer = n->dPer ? n->dPer + (int64_t)(t / n->ep) * N : NULL;
         /* driven pre-pass in set order (src/lifdiff.js:290-297) */
-        for (int32_t q = 0; q < n->nDriven; q++) {
-            const int32_t i = n->driven[q];
+        for (int32_t q = 0; q < n->nW && !n->driveSoft; q++) {
+            const int32_t i = n->wdrv[q];
             if (refr[i] > 0) continue;
             if (rand01(&n->rng) < (double)drive[i] * k.dtS) {
                 refr[i] = (float)(p->tRef + p->dt);
                 push_spike(n, (uint32_t)i | FORCED_BIT, 1.f);
                 sc[i] = (float)((double)sc[i] + 1.0);
                 if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + 1.0);
+                if (scoredStep && wPer) n->lossPer += (double)wPer[i];
                 adapt[i] = (float)((double)adapt[i] + p->adaptInc);
                 if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = 1.f;
             }
@@ -377,6 +488,11 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
             if (!wasRefr) {
                 const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
                 s = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                /* the drive as an expectation (src/lifdiff.js:338) */
+                if (n->driveSoft && drive[i] > 0) {
+                    const double sd = fmin(DRIVE_MAX, (double)drive[i] * k.dtS);
+                    s = sd + (1.0 - sd) * s;
+                }
                 if (s > 0) {
                     vi = s * p->vReset + (1.0 - s) * vi;
                     if (!p->soft) refr[i] = (float)p->tRef;
@@ -384,6 +500,7 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
                 }
                 sc[i] = (float)((double)sc[i] + s);
                 if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + s);
+                if (scoredStep && wPer) n->lossPer += (double)wPer[i] * s;
                 if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = (float)s;
                 if (s > (p->soft ? 1e-4 : 0.5)) push_spike(n, (uint32_t)i, (float)s);
             }
@@ -392,7 +509,8 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
     }
     n->steps = steps;
     double L = 0.0;
-    for (int32_t i = 0; i < N; i++) L += (double)n->dLdSpike[i] * (double)n->scored[i];
+    if (n->dPer) L = n->lossPer;
+    else for (int32_t i = 0; i < N; i++) L += (double)n->dLdSpike[i] * (double)n->scored[i];
     return L + readout_loss(n);
 }
 
@@ -422,8 +540,9 @@ typedef struct {
 
 /* Re-simulates [from, to) from the checkpoint at `from`, recording per step the
  * membrane potential, conductances (after delivery, before decay), decayed
- * adapt, spike amplitude, mode (0 integrated, 1 refractory, 2 driven spike)
- * and res at each arrival (src/lifdiff.js:390-464). */
+ * adapt, spike amplitude, mode (0 integrated, 1 refractory, 2 driven spike,
+ * 3 integrated with the soft drive) and res at each arrival
+ * (src/lifdiff.js:390-464). */
 static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
     const fdo_params *p = &n->p;
     const int32_t N = n->N, D = n->nslots - 1;
@@ -431,18 +550,21 @@ static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
     restore_ckpt(n, from / n->K);
     float *v = n->st[FDO_V], *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *adapt = n->st[FDO_ADAPT];
     float *res = n->st[FDO_RES], *refr = n->st[FDO_REFR];
-    const float *drive = n->in[FDO_DRIVE], *thrOff = n->in[FDO_THROFFSET];
+    const float *drive = n->wdrive, *thrOff = n->in[FDO_THROFFSET];
     uint8_t *forced = (uint8_t *)xcalloc(N, 1);
     int64_t rOff = 0;
     for (int32_t t = from; t < to; t++) {
         const int64_t off = (int64_t)(t - from) * N;
+        /* the rates the forward used: a segment need not start on an epoch
+         * (src/lifdiff.js:406-413); reapplying them is idempotent */
+        if (n->nDI && (t == from || t % n->ep == 0)) apply_epoch(n, t / n->ep);
         if (t >= D) {
             deliver(n, n->off[t - D], n->off[t - D + 1], A->resAt + rOff);
             rOff += n->off[t - D + 1] - n->off[t - D];
         }
         memset(forced, 0, (size_t)N);
-        for (int32_t q = 0; q < n->nDriven; q++) {
-            const int32_t i = n->driven[q];
+        for (int32_t q = 0; q < n->nW && !n->driveSoft; q++) {
+            const int32_t i = n->wdrv[q];
             if (refr[i] > 0) continue;
             if (rand01(&n->rng) < (double)drive[i] * k.dtS) {
                 refr[i] = (float)(p->tRef + p->dt);
@@ -471,6 +593,11 @@ static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
             if (!wasRefr) {
                 const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
                 sv = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                if (n->driveSoft && drive[i] > 0) {
+                    const double sd = fmin(DRIVE_MAX, (double)drive[i] * k.dtS);
+                    sv = sd + (1.0 - sd) * sv;
+                    md = 3;
+                }
                 if (sv > 0) {
                     vi = sv * p->vReset + (1.0 - sv) * vi;
                     if (!p->soft) refr[i] = (float)p->tRef;
@@ -506,6 +633,7 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
     memset(gLogGain, 0, 4 * (size_t)N);
     memset(gEdge, 0, 4 * (size_t)n->E);
     memset(gInScale, 0, 4 * (size_t)N);
+    if (n->gDrive) memset(n->gDrive, 0, 4 * (size_t)n->nEp * n->nDI);
     double g[FDO_NPARAMS] = {0};
     double sAdaptInc = 0, sVThresh = 0, sKc = 0, sLamin = 0, sEInh = 0;
     /* the readout loss's spike adjoint, per step and readout */
@@ -541,6 +669,9 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
         for (int32_t t = to - 1; t >= from; t--) {
             const int64_t off = (int64_t)(t - from) * N;
             const int scoredStep = t >= n->lossFrom;
+            const int32_t e = t / n->ep;
+            const float *wPer = n->dPer ? n->dPer + (int64_t)e * N : NULL;
+            float *gDr = n->gDrive ? n->gDrive + (int64_t)(e < n->nEp ? e : n->nEp - 1) * n->nDI : NULL;
             if (truncate > 0 && (t + 1) % truncate == 0) {
                 memset(lv, 0, 4 * (size_t)N); memset(lgE, 0, 4 * (size_t)N); memset(lgI, 0, 4 * (size_t)N);
                 memset(lad, 0, 4 * (size_t)N); memset(lres, 0, 4 * (size_t)N);
@@ -554,11 +685,16 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
                 const double lE = (double)lgE[i] * k.dE, lI = (double)lgI[i] * k.dE, lA = lpA * k.dA;
                 const double lR = (double)lres[i] * (1.0 - k.kRec);
                 lgE[i] = (float)lE; lgI[i] = (float)lI; lad[i] = (float)lA; lres[i] = (float)lR;
-                double dl = scoredStep ? (double)n->dLdSpike[i] : 0.0;
+                double dl = scoredStep ? (wPer ? (double)wPer[i] : (double)n->dLdSpike[i]) : 0.0;
                 if (R && n->readIdx[i] >= 0) dl += dLds[(int64_t)t * R + n->readIdx[i]];
+                const int32_t slot = n->drvSlot[i];
                 if (md == 1) { lv[i] = 0.f; continue; }
                 if (md == 2) {
+                    /* the drive fired: only the adaptation increment and the
+                     * drive's own gradient remain (src/lifdiff.js:565-574) */
                     sAdaptInc += lA;
+                    if (slot >= 0 && gDr)
+                        gDr[slot] = (float)((double)gDr[slot] + (lA * p->adaptInc + (double)ls[i] + dl) * k.dtS);
                     lv[i] = 0.f; ls[i] = 0.f;
                     continue;
                 }
@@ -568,7 +704,15 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
                 const double thr = p->vThresh + (double)A.adapt[off + i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
                 const double d = u - thr;
                 const double s = A.sp[off + i];
-                const double sd = 0.0, st = s;
+                /* with the soft drive the amplitude is sd + (1 - sd) st; the
+                 * surrogate is taken at st (src/lifdiff.js:580-585) */
+                double sd = 0.0, st = s;
+                if (md == 3) {
+                    const double rate = slot >= 0 ? (double)n->drvSched[(int64_t)(e < n->nEp ? e : n->nEp - 1) * n->nDI + slot]
+                                                  : (double)n->in[FDO_DRIVE][i];
+                    sd = fmin(DRIVE_MAX, rate * k.dtS);
+                    st = (s - sd) / (1.0 - sd);
+                }
                 const double beta = p->surrogateBeta;
                 double sg;
                 if (p->soft) sg = st * (1.0 - st) / beta;
@@ -579,6 +723,8 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
                 const double lt = lsTot * (1.0 - sd) * sg;
                 const double lu = lvNew * (1.0 - s) + lt;
                 const double lthr = -lt;
+                /* and through sd: ds/d(sd) = 1 - st, d(sd)/d(rate) = dt/1000 */
+                if (slot >= 0 && gDr) gDr[slot] = (float)((double)gDr[slot] + lsTot * (1.0 - st) * k.dtS);
                 sAdaptInc += lpA * s;
                 sVThresh += lthr;
                 if (n->thrMask[i]) sKc += lthr;
@@ -652,6 +798,8 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
     for (int q = 0; q < FDO_NPARAMS; q++) if (!isfinite(g[q])) { g[q] = 0; cl[0]++; }
     for (int32_t i = 0; i < N; i++) if (!isfinite(gLogGain[i])) { gLogGain[i] = 0.f; cl[1]++; }
     for (int64_t j = 0; j < n->E; j++) if (!isfinite(gEdge[j])) { gEdge[j] = 0.f; cl[2]++; }
+    if (n->gDrive)
+        for (int64_t q = 0; q < (int64_t)n->nEp * n->nDI; q++) if (!isfinite(n->gDrive[q])) n->gDrive[q] = 0.f;
     memcpy(params, g, sizeof g);
     if (clamped) memcpy(clamped, cl, sizeof cl);
     free(lv); free(lgE); free(lgI); free(lad); free(lres); free(ring); free(dLds);

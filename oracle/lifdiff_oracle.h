@@ -50,6 +50,17 @@ void fdo_set_masks(fdo_net *n, const uint8_t *thrMask, const uint8_t *biasMask);
 void fdo_set_driven(fdo_net *n, const int32_t *list, int32_t count);
 void fdo_set_loss(fdo_net *n, const float *dLdSpike, int32_t lossFrom);
 void fdo_set_readout(fdo_net *n, const int32_t *idx, int32_t R, const float *targetHz, double tauMs);
+/* driveIdx / driveEpoch: at the start of epoch e (every epochSteps steps) the
+ * rate of idx[k] becomes sched[e*nD + k] through setDriveOne's semantics (the
+ * last epoch's rates hold beyond nEp).  driveSoft: the drive spikes as its
+ * expectation.  NULL idx clears the schedule. */
+void fdo_set_drive_schedule(fdo_net *n, const int32_t *idx, int32_t nD, int32_t epochSteps, const float *sched,
+                            int32_t nEp, int32_t driveSoft);
+/* dLdSpikePerEpoch: the loss weight of neuron i's spikes in epoch e is
+ * dPer[e*N + i] (NULL clears) */
+void fdo_set_loss_per_epoch(fdo_net *n, const float *dPer, int32_t nEp, int32_t epochSteps);
+/* dL/d sched [nEp][nD] of the last backward */
+void fdo_drive_grad(const fdo_net *n, float *gDrive);
 /* Runs `steps` steps from reset with the given seed; records the tape when
  * `record` is nonzero.  Returns the loss. */
 double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record);

@@ -250,6 +250,9 @@ def _diff_lib():
         "fdo_rng": (ctypes.c_uint32, [P]),
         "fdo_spikes": (I64, [P, P, P, P, I64]),
         "fdo_readout": (I64, [P, P, P]),
+        "fdo_set_drive_schedule": (None, [P, P, I32, I32, P, I32, I32]),
+        "fdo_set_loss_per_epoch": (None, [P, P, I32, I32]),
+        "fdo_drive_grad": (None, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -268,6 +271,7 @@ class OracleDiff:
     ``sign`` is +1 / -1 / 0 per neuron; ``params`` holds LIFDiff's options."""
 
     def __init__(self, N, indptr, indices, counts, sign, params: dict):
+        self._dsched = None
         self.lib = _diff_lib()
         self.N = int(N)
         self.p = dict(params)
@@ -322,6 +326,33 @@ class OracleDiff:
         t = np.ascontiguousarray(target_hz, dtype=np.float32)
         self.R = int(i.size)
         self.lib.fdo_set_readout(self.h, _ptr(i), self.R, _ptr(t), float(tau_ms))
+
+    def set_drive_schedule(self, idx, epoch_steps: int, sched, drive_soft: bool = False):
+        """driveIdx / driveEpoch with the rates sched[e][k] the onEpoch hook sets
+        at the start of each epoch (src/lifdiff.js:135-144, 228-234); None clears."""
+        if idx is None:
+            self.lib.fdo_set_drive_schedule(self.h, None, 0, 1, None, 0, 1 if drive_soft else 0)
+            self._dsched = None
+            return
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        sc = np.ascontiguousarray(sched, dtype=np.float32).reshape(-1, i.size)
+        self._dsched = (i, sc)
+        self.lib.fdo_set_drive_schedule(self.h, _ptr(i), int(i.size), int(epoch_steps), _ptr(sc), int(sc.shape[0]),
+                                        1 if drive_soft else 0)
+
+    def set_loss_per_epoch(self, dper, epoch_steps: int):
+        """dLdSpikePerEpoch (src/lifdiff.js:519-522): dper[e][i]; None clears."""
+        if dper is None:
+            self.lib.fdo_set_loss_per_epoch(self.h, None, 0, int(epoch_steps))
+            return
+        a = np.ascontiguousarray(dper, dtype=np.float32).reshape(-1, self.N)
+        self.lib.fdo_set_loss_per_epoch(self.h, _ptr(a), int(a.shape[0]), int(epoch_steps))
+
+    def drive_grad(self) -> np.ndarray:
+        i, sc = self._dsched
+        out = np.zeros(sc.shape, dtype=np.float32)
+        self.lib.fdo_drive_grad(self.h, _ptr(out))
+        return out
 
     def forward(self, steps: int, seed=1, record: bool = True) -> float:
         self.steps = int(steps)

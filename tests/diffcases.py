@@ -55,6 +55,21 @@ class Problem:
         self.inputs["drive"] = drive
         self.edge_gain = None
         self.readout = None
+        self.sched = None
+        self.dper = None
+
+    def with_schedule(self, nD=20, epoch=8, steps=60, soft=True, seed=5, zeros=False):
+        """driveIdx / driveEpoch: nD neurons whose rates are set at every
+        epoch start (src/lifdiff.js:135-144); ``soft`` = driveSoft.  With
+        ``zeros`` some rates are 0, so neurons leave and re-enter the driven set."""
+        rng = np.random.default_rng(seed)
+        idx = rng.choice(self.g.N, size=nD, replace=False).astype(np.int32)
+        nEp = -(-steps // epoch)
+        vals = rng.uniform(20, 150, size=(nEp, nD)).astype(np.float32)
+        if zeros:
+            vals[rng.random(vals.shape) < 0.3] = 0.0
+        self.sched = [idx, epoch, vals, soft]
+        return self
 
     def with_readout(self, R=40, tau_ms=20.0, seed=8):
         rng = np.random.default_rng(seed)
@@ -74,6 +89,11 @@ class Problem:
         net.set_loss(self.dL, 0)
         if self.readout is not None:
             net.set_readout(*self.readout)
+        if self.sched is not None:
+            idx, ep, vals, soft = self.sched
+            net.set_drive_schedule(idx, ep, vals, drive_soft=soft)
+        if self.dper is not None:
+            net.set_loss_per_epoch(*self.dper)
         return net
 
     def oracle(self, params=None, inputs=None, edge_gain=None):

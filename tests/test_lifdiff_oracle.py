@@ -266,3 +266,110 @@ def test_forced_refractory_countdown_and_soft_amplitude():
     s = 1.0 / (1.0 + np.exp(-(u - p2["vThresh"]) / 1.5))
     assert net2.state("spikeCount")[0] == np.float32(s)
     assert net2.state("v")[0] == np.float32(s * p2["vReset"] + (1 - s) * u)
+
+
+# ---------------------------------------------------------------- the drive as an input
+
+def test_drive_gradient_matches_finite_differences(oracle_built):
+    """driveIdx with driveSoft (the smooth model scripts/vis_grad_check.mjs
+    checks the drive gradient against): dL/d rate at each epoch, against
+    central differences of the schedule entry, with grad_check's tolerance."""
+    pr = dc.Problem().with_schedule(nD=20, epoch=8, steps=STEPS, soft=True)
+    idx, ep, vals, soft = pr.sched
+    pr.dL[idx[::2]] = 1.0            # half of the driven neurons are scored themselves
+    net = pr.oracle()
+    L = net.forward(STEPS, seed=1)
+    net.backward()
+    gd = net.drive_grad()
+    h = 5.0
+    checked = 0
+    for e, k in ((0, 0), (1, 4), (2, 6), (4, 10), (6, 18), (7, 5), (3, 14)):
+        up, dn = vals.copy(), vals.copy()
+        up[e, k] += h
+        dn[e, k] -= h
+        pr.sched = [idx, ep, up, soft]
+        Lu = pr.oracle().forward(STEPS, seed=1, record=False)
+        pr.sched = [idx, ep, dn, soft]
+        Ld = pr.oracle().forward(STEPS, seed=1, record=False)
+        pr.sched = [idx, ep, vals, soft]
+        fd = (Lu - Ld) / (2 * h)
+        ok, rel = dc.fd_ok(gd[e, k], fd, h, L)
+        assert ok, (e, k, gd[e, k], fd, rel)
+        checked += abs(fd) > 4e-6 * abs(L) / h * 3
+    assert checked >= 4          # most rows are above the finite-difference noise floor
+
+
+def test_per_epoch_loss_gradients_match_finite_differences(oracle_built):
+    """dLdSpikePerEpoch (src/lifdiff.js:519-522): a loss weighting each
+    neuron's spikes per epoch; the scalar and drive gradients of that loss
+    against central differences."""
+    pr = dc.Problem().with_schedule(nD=12, epoch=8, steps=STEPS, soft=True, seed=9)
+    rng = np.random.default_rng(4)
+    nEp = -(-STEPS // 8)
+    pr.dper = (rng.normal(0, 1, size=(nEp, pr.g.N)).astype(np.float32), 8)
+    net = pr.oracle()
+    L = net.forward(STEPS, seed=1)
+    g = net.backward()
+    for name in ("wSyn", "vThresh", "adaptInc"):
+        h = dc.H[name]
+        fd = (_loss(pr, params={name: pr.p[name] + h}) - _loss(pr, params={name: pr.p[name] - h})) / (2 * h)
+        ok, rel = dc.fd_ok(g["params"][name], fd, h, L)
+        assert ok, (name, g["params"][name], fd, rel)
+    # the loss is the per-epoch weighted count, not the flat one
+    flat = pr.oracle()
+    flat_L = flat.forward(STEPS, seed=1, record=False)
+    assert flat_L == L
+    pr.dper = None
+    assert _loss(pr) != L
+
+
+def test_schedule_gradients_do_not_depend_on_checkpoint_length(oracle_built):
+    """Segments that start inside an epoch reload its rates (src/lifdiff.js:
+    406-413): the drive gradient is the same for checkpoints of 16 and 7 steps."""
+    out = []
+    for K in (16, 7):
+        pr = dc.Problem(over={"checkpoint": K}).with_schedule(nD=15, epoch=8, steps=STEPS, soft=True, zeros=True)
+        net = pr.oracle()
+        net.forward(STEPS, seed=1)
+        g = net.backward()
+        out.append((g["params"], net.drive_grad()))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_hard_schedule_matches_lif_c_with_set_drive_each_epoch(oracle_built):
+    """Hard mode, no synapses: the schedule is setDriveOne at each epoch start,
+    so neurons set to 0 leave the driven set and come back at its end; the
+    forced spikes and the stream equal lif.c driven the same way."""
+    from oracle import OracleBrain, OracleDiff
+    N, ep, T = 200, 25, 1000
+    g = synth.Graph(N=N, indptr=np.zeros(N + 1, np.uint32), indices=np.zeros(0, np.uint32),
+                    counts=np.zeros(0, np.int32), exc=np.ones(N, bool))
+    rng = np.random.default_rng(12)
+    idx = rng.permutation(N)[:60].astype(np.int32)
+    vals = rng.uniform(20, 400, size=(T // ep, 60)).astype(np.float32)
+    vals[rng.random(vals.shape) < 0.35] = 0.0
+    base_drv = np.arange(0, N, 11, dtype=np.int32)
+    drive = np.zeros(N, np.float32)
+    drive[base_drv] = 90.0
+    pd = P.diff_merged({"dt": 0.1, "tRef": 2.2})
+    od = OracleDiff(N, g.indptr, g.indices, np.zeros(0, np.float32), np.ones(N, np.float32), pd)
+    od.set_input("drive", drive)
+    od.set_driven(base_drv)
+    od.set_loss(np.zeros(N, np.float32))
+    od.set_drive_schedule(idx, ep, vals)
+    od.forward(T, seed=3)
+    off, sidx, amp, forced = od.spikes()
+    ob = OracleBrain(g, P.config_params(tRef=2.2), seed=3, W=np.zeros(0, np.float32), sign=np.ones(N, np.float32))
+    for i in base_drv:
+        ob.setDriveOne(int(i), 90.0)
+    nfs, ras = [], []
+    for e in range(T // ep):
+        for k, i in enumerate(idx):
+            ob.setDriveOne(int(i), float(vals[e, k]))
+        nf, r, _ = ob.run(ep, raster_cap=ep * N)
+        nfs += nf.tolist()
+        ras += r.tolist()
+    assert forced.all() and np.diff(off).tolist() == nfs
+    assert sidx.tolist() == ras
+    assert od.rng == ob.rng
