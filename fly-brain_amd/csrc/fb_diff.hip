@@ -95,7 +95,36 @@ struct DiffDev {
     double *rate, *dLds;               // [steps][R]
     const float *target;
     float *ckpt;                       // checkpoints [n][6N + 1]
+    // the drive as an input (driveIdx / driveEpoch): rates per epoch of `ep`
+    // steps for the neurons with drvSlot >= 0, and the drive gradient
+    int32_t nDI, ep, nEp, driveSoft;
+    const int32_t *drvSlot;            // [N] position in driveIdx, or -1; null without a schedule
+    const float *drvSched;             // [nEp][nDI]
+    float *gDrive;                     // [nEp][nDI]
+    const float *dPer;                 // [epochs][N] per-epoch spike weights, or null
+    double *lossAcc;                   // [N] the per-epoch weighted loss of a forward pass
 };
+
+constexpr double DRIVE_MAX = 0.999;   // src/lifdiff.js:72
+
+// The drive rate of neuron i at step t: the schedule's rate for its epoch
+// (held past the last one) if i is in driveIdx, else the fixed input.
+__device__ __forceinline__ float drive_at(const DiffDev &d, int32_t i, int32_t t) {
+    if (d.drvSlot) {
+        const int32_t k = d.drvSlot[i];
+        if (k >= 0) {
+            int32_t e = t / d.ep;
+            if (e >= d.nEp) e = d.nEp - 1;
+            return d.drvSched[(int64_t)e * d.nDI + k];
+        }
+    }
+    return d.drive[i];
+}
+
+// the loss weight of a spike of neuron i at step t (per epoch when dPer is set)
+__device__ __forceinline__ double spike_weight(const DiffDev &d, int32_t i, int32_t t) {
+    return d.dPer ? (double)d.dPer[(int64_t)(t / d.ep) * d.N + i] : (double)d.dLdSpike[i];
+}
 
 enum { DERR_COLL = 1, DERR_TAPE = 2, DERR_DRAWS = 4 };
 
@@ -181,7 +210,10 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
                 atomicAdd(&d.events[1], 1ull);
             }
         }
-        const uint32_t rank = (e & FORCED_BIT) ? (uint32_t)d.drvPos[pre] : (uint32_t)d.N + (uint32_t)pre;
+        // arrival order: forced spikes first, in the driven-set order of their
+        // step (the pre-pass wrote them at the head of the step's list), then
+        // threshold spikes by index (the membrane pass appends them in any order)
+        const uint32_t rank = (e & FORCED_BIT) ? (uint32_t)(k - a0) : (uint32_t)d.N + (uint32_t)pre;
         const bool exc = eff > 0;
         const int32_t cls = exc ? 0 : d.N;
         const int32_t par = t & 1;
@@ -247,12 +279,13 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
             const int32_t i = d.driven[q];
             if (!(d.refr[i] > 0)) {
                 const double u = (double)(draws[next - r0] >> 8) * (1.0 / 16777216.0);
-                if (u < (double)d.drive[i] * d.dtS) {
+                if (u < (double)drive_at(d, i, t) * d.dtS) {
                     d.refr[i] = d.forcedRefr;
                     d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
                     d.forced[i] = 1;
                     d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
                     if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+                    if (d.dPer && t >= d.lossFrom) d.lossAcc[i] += spike_weight(d, i, t);
                     const int32_t rk = d.readIdx[i];
                     if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
                     d.dflag[q] = 1;
@@ -324,12 +357,13 @@ __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
             first = false;
             const int32_t i = ii[u];
             const double uu = (double)(x >> 8) * (1.0 / 16777216.0);
-            if (uu < (double)d.drive[i] * d.dtS) {
+            if (uu < (double)drive_at(d, i, t) * d.dtS) {
                 d.refr[i] = d.forcedRefr;
                 d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
                 d.forced[i] = 1;
                 d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
                 if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+                if (d.dPer && t >= d.lossFrom) d.lossAcc[i] += spike_weight(d, i, t);
                 const int32_t rk = d.readIdx[i];
                 if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
                 fm |= 1u << u;
@@ -419,6 +453,15 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
         if (!wasRefr) {
             const double thr = d.vThresh + (double)adaptf + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
             s = d.soft ? 1.0 / (1.0 + exp(-(vi - thr) / d.beta)) : (vi >= thr ? 1.0 : 0.0);
+            // the drive as an expectation (src/lifdiff.js:338, 453)
+            if (d.driveSoft) {
+                const float dv = drive_at(d, i, t);
+                if (dv > 0) {
+                    const double sd = fmin(DRIVE_MAX, (double)dv * d.dtS);
+                    s = sd + (1.0 - sd) * s;
+                    md = 3;
+                }
+            }
             if (s > 0) {
                 vi = s * d.vReset + (1.0 - s) * vi;
                 if (!d.soft) refrf = d.tRefF;
@@ -427,6 +470,7 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
             if (!REPLAY) {
                 d.spikeCount[i] = (float)((double)d.spikeCount[i] + s);
                 if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + s);
+                if (d.dPer && t >= d.lossFrom) d.lossAcc[i] += spike_weight(d, i, t) * s;
                 const int32_t rk = d.readIdx[i];
                 if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = (float)s;
                 emit = s > d.emitThr;
@@ -506,7 +550,8 @@ __device__ __forceinline__ void dx_save(const DiffDev &d, int32_t seg) {
 __global__ void __launch_bounds__(1024) kd_loss_flat(DiffDev d, double *out) {
     __shared__ double red[16];
     double acc = 0.0;
-    for (int32_t i = threadIdx.x; i < d.N; i += 1024) acc += (double)d.dLdSpike[i] * (double)d.scored[i];
+    for (int32_t i = threadIdx.x; i < d.N; i += 1024)
+        acc += d.dPer ? d.lossAcc[i] : (double)d.dLdSpike[i] * (double)d.scored[i];
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -579,13 +624,19 @@ __device__ __forceinline__ void dx_bneuron(const DiffDev &d, int32_t t, int32_t 
         const float lresf = (float)lR;
         float lvf = d.lv[i];
         float *ls = d.ring + (int64_t)(t % d.S) * N + i;
-        double dl = t >= d.lossFrom ? (double)d.dLdSpike[i] : 0.0;
+        double dl = t >= d.lossFrom ? spike_weight(d, i, t) : 0.0;
         const int32_t rk = d.readIdx[i];
         if (rk >= 0) dl += d.dLds[(int64_t)t * d.R + rk];
+        // this neuron's entry of the drive gradient for the epoch of t
+        const int32_t slot = d.drvSlot ? d.drvSlot[i] : -1;
+        float *gDr = slot >= 0 ? d.gDrive + (int64_t)min(t / d.ep, d.nEp - 1) * d.nDI + slot : nullptr;
         if (md == 1) {
             lvf = 0.f;
         } else if (md == 2) {
+            // the drive fired: the adaptation increment and the drive's own
+            // gradient remain (src/lifdiff.js:565-574)
             pA += lA;
+            if (gDr) *gDr = (float)((double)*gDr + (lA * d.adaptInc + (double)*ls + dl) * d.dtS);
             lvf = 0.f;
             *ls = 0.f;
         } else {
@@ -595,7 +646,13 @@ __device__ __forceinline__ void dx_bneuron(const DiffDev &d, int32_t t, int32_t 
             const double thr = d.vThresh + (double)d.aAd[ar] + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
             const double dd = u - thr;
             const double s = d.aSp[ar];
-            const double sd = 0.0, st = s;
+            // with the soft drive the amplitude is sd + (1 - sd) st, and the
+            // surrogate is taken at st (src/lifdiff.js:580-585)
+            double sd = 0.0, st = s;
+            if (md == 3) {
+                sd = fmin(DRIVE_MAX, (double)drive_at(d, i, t) * d.dtS);
+                st = (s - sd) / (1.0 - sd);
+            }
             double sg;
             if (d.soft) {
                 sg = st * (1.0 - st) / d.beta;
@@ -609,6 +666,8 @@ __device__ __forceinline__ void dx_bneuron(const DiffDev &d, int32_t t, int32_t 
             const double lt = lsTot * (1.0 - sd) * sg;
             const double lu = lvNew * (1.0 - s) + lt;
             const double lthr = -lt;
+            // through sd: ds/d(sd) = 1 - st, d(sd)/d(rate) = dt/1000
+            if (gDr) *gDr = (float)((double)*gDr + lsTot * (1.0 - st) * d.dtS);
             pA += lpA * s;
             pV += lthr;
             if (d.thrMask[i]) pK += lthr;
@@ -829,6 +888,15 @@ struct fb_diff {
     hipStream_t replayStream = nullptr;
     hipEvent_t replayDone[2] = {nullptr, nullptr}, sweepDone[2] = {nullptr, nullptr}, fork = nullptr;
     int64_t arenaFloats = 0;          // floats of one arena (the modes are K N bytes more)
+    // the drive as an input: schedule (host copy), per-epoch driven lists of a run
+    std::vector<int32_t> drvIdx;
+    std::vector<float> drvSched;
+    std::vector<int32_t> epOff, epCnt;   // per epoch of the run: its driven list in epLists
+    int32_t *epLists = nullptr;          // run allocation
+    int32_t nEpPer = 0;
+    void *schedAlloc = nullptr;          // drvSlot, drvSched, gDrive
+    float *dPerAlloc = nullptr;
+    double *lossAccAlloc = nullptr;
 };
 
 #define FB_REQUIRE_D(cond, code, ...)    \
@@ -995,6 +1063,7 @@ static void free_run(fb_diff *f) {
     f->d.off = nullptr; f->d.idx = nullptr; f->d.amp = nullptr; f->d.readS = nullptr;
     f->d.aV = nullptr; f->d.rate = nullptr; f->d.dLds = nullptr;
     f->arena2 = nullptr; f->mode2 = nullptr;
+    f->epLists = nullptr;
     f->recorded = false;
 }
 
@@ -1006,6 +1075,9 @@ extern "C" fb_status fb_diff_destroy(fb_diff *f) {
     if (f->d.edgeGain) (void)hipFree((void *)f->d.edgeGain);
     (void)hipFree((void *)f->d.target);
     (void)hipFree(f->readPart);
+    if (f->schedAlloc) (void)hipFree(f->schedAlloc);
+    if (f->dPerAlloc) (void)hipFree(f->dPerAlloc);
+    if (f->lossAccAlloc) (void)hipFree(f->lossAccAlloc);
     (void)hipStreamDestroy(f->ownStream);
     if (f->replayStream) {
         (void)hipStreamDestroy(f->replayStream);
@@ -1118,6 +1190,73 @@ extern "C" fb_status fb_diff_set_loss(fb_diff *f, const float *dLdSpike, int32_t
     return FB_OK;
 }
 
+extern "C" fb_status fb_diff_set_drive_schedule(fb_diff *f, const int32_t *idx, int32_t nD, int32_t epochSteps,
+                                               const float *sched, int32_t nEpochs, int32_t driveSoft) {
+    FB_REQUIRE_D(f && nD >= 0 && (nD == 0 || (idx && sched && nEpochs > 0 && epochSteps > 0)), FB_ERR_ARG,
+                 "fb_diff_set_drive_schedule: bad arguments");
+    std::vector<int32_t> slot((size_t)f->N, -1);
+    for (int32_t k = 0; k < nD; k++) {
+        FB_REQUIRE_D(idx[k] >= 0 && idx[k] < f->N, FB_ERR_ARG, "fb_diff_set_drive_schedule: index %d out of range",
+                     idx[k]);
+        FB_REQUIRE_D(slot[idx[k]] < 0, FB_ERR_ARG, "fb_diff_set_drive_schedule: neuron %d listed twice", idx[k]);
+        slot[idx[k]] = k;
+    }
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (f->schedAlloc) FB_HIP(hipFree(f->schedAlloc));
+    f->schedAlloc = nullptr;
+    f->d.drvSlot = nullptr; f->d.drvSched = nullptr; f->d.gDrive = nullptr;
+    f->d.nDI = nD; f->d.nEp = nD ? nEpochs : 0; f->d.driveSoft = driveSoft ? 1 : 0;
+    if (nD || f->d.dPer == nullptr) f->d.ep = nD ? epochSteps : (f->d.dPer ? f->d.ep : 1);
+    f->drvIdx.assign(idx, idx + nD);
+    f->drvSched.assign(sched, sched + (size_t)(nD ? nEpochs : 0) * nD);
+    if (nD) {
+        const size_t nv = (size_t)nEpochs * nD;
+        FB_HIP(hipMalloc(&f->schedAlloc, sizeof(int32_t) * f->N + 2 * sizeof(float) * nv));
+        int32_t *ds = (int32_t *)f->schedAlloc;
+        float *sv = (float *)(ds + f->N);
+        FB_HIP(hipMemcpy(ds, slot.data(), sizeof(int32_t) * f->N, hipMemcpyHostToDevice));
+        FB_HIP(hipMemcpy(sv, sched, sizeof(float) * nv, hipMemcpyHostToDevice));
+        FB_HIP(hipMemset(sv + nv, 0, sizeof(float) * nv));
+        f->d.drvSlot = ds; f->d.drvSched = sv; f->d.gDrive = sv + nv;
+    }
+    f->recorded = false;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_loss_per_epoch(fb_diff *f, const float *dPer, int32_t nEpochs, int32_t epochSteps) {
+    FB_REQUIRE_D(f && (!dPer || (nEpochs > 0 && epochSteps > 0)), FB_ERR_ARG, "fb_diff_set_loss_per_epoch: bad arguments");
+    FB_REQUIRE_D(!dPer || !f->d.nDI || epochSteps == f->d.ep, FB_ERR_ARG,
+                 "fb_diff_set_loss_per_epoch: the epoch must be the drive schedule's (%d steps)", f->d.ep);
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (f->dPerAlloc) FB_HIP(hipFree(f->dPerAlloc));
+    f->dPerAlloc = nullptr;
+    f->d.dPer = nullptr;
+    f->nEpPer = 0;
+    if (dPer) {
+        const size_t nv = (size_t)nEpochs * f->N;
+        FB_HIP(hipMalloc(&f->dPerAlloc, sizeof(float) * nv));
+        FB_HIP(hipMemcpy(f->dPerAlloc, dPer, sizeof(float) * nv, hipMemcpyHostToDevice));
+        if (!f->lossAccAlloc) FB_HIP(hipMalloc(&f->lossAccAlloc, sizeof(double) * f->N));
+        f->d.dPer = f->dPerAlloc;
+        f->d.lossAcc = f->lossAccAlloc;
+        f->d.ep = epochSteps;
+        f->nEpPer = nEpochs;
+    }
+    f->recorded = false;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_drive_grad(fb_diff *f, float *gDrive) {
+    FB_REQUIRE_D(f && gDrive, FB_ERR_ARG, "fb_diff_drive_grad: bad arguments");
+    FB_REQUIRE_D(f->d.gDrive, FB_ERR_STATE, "fb_diff_drive_grad: no drive schedule");
+    FB_HIP(hipStreamSynchronize(f->stream));
+    const size_t nv = (size_t)f->d.nEp * f->d.nDI;
+    FB_HIP(hipMemcpy(gDrive, f->d.gDrive, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < nv; q++)
+        if (!std::isfinite(gDrive[q])) gDrive[q] = 0.f;   // as the reference sanitises (src/lifdiff.js:684)
+    return FB_OK;
+}
+
 extern "C" fb_status fb_diff_set_readout(fb_diff *f, const int32_t *idx, int32_t R, const float *targetHz,
                                          double tauMs) {
     FB_REQUIRE_D(f && R >= 0 && (R == 0 || (idx && targetHz)) && tauMs > 0, FB_ERR_ARG,
@@ -1185,6 +1324,42 @@ static fb_status fwd_prepare(fb_diff *f, int32_t steps, double seed, int32_t rec
     f->nck = record ? (steps + d.K - 1) / d.K + 1 : 0;
     if (record && (s = dalloc(f->runAllocs, &f->ckpt, (size_t)f->nck * (6 * (size_t)N + 1)))) return s;
     d.ckpt = f->ckpt;
+    FB_REQUIRE_D(!d.nDI || (int64_t)d.nEp * d.ep >= steps, FB_ERR_ARG,
+                 "fb_diff_forward: the drive schedule covers %d epochs of %d steps, fewer than %d steps", d.nEp, d.ep,
+                 steps);
+    FB_REQUIRE_D(!d.dPer || (int64_t)f->nEpPer * d.ep >= steps, FB_ERR_ARG,
+                 "fb_diff_forward: the per-epoch spike weights cover fewer than %d steps", steps);
+    if (d.dPer) FB_HIP(hipMemsetAsync(d.lossAcc, 0, sizeof(double) * N, st));
+    // the driven set at each epoch: the inputs' set, then setDriveOne with the
+    // epoch's rates (src/lifdiff.js:205-207): an added neuron goes to the end,
+    // a removed one leaves its place
+    f->epOff.clear(); f->epCnt.clear();
+    if (d.nDI && !d.driveSoft) {
+        const int32_t nE = (steps + d.ep - 1) / d.ep;
+        std::vector<int32_t> cur(f->drivenList), all;
+        std::vector<uint8_t> in((size_t)N, 0);
+        for (int32_t i : cur) in[i] = 1;
+        for (int32_t e = 0; e < nE; e++) {
+            const int32_t ee = std::min(e, d.nEp - 1);
+            for (int32_t k = 0; k < d.nDI; k++) {
+                const int32_t i = f->drvIdx[k];
+                const float rate = f->drvSched[(size_t)ee * d.nDI + k];
+                if (rate > 0) {
+                    if (!in[i]) { in[i] = 1; cur.push_back(i); }
+                } else if (in[i]) {
+                    in[i] = 0;
+                    cur.erase(std::find(cur.begin(), cur.end(), i));
+                }
+            }
+            f->epOff.push_back((int32_t)all.size());
+            f->epCnt.push_back((int32_t)cur.size());
+            all.insert(all.end(), cur.begin(), cur.end());
+        }
+        if ((s = dalloc(f->runAllocs, &f->epLists, std::max<size_t>(all.size(), 1)))) return s;
+        if (!all.empty())
+            FB_HIP(hipMemcpyAsync(f->epLists, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, st));
+        FB_HIP(hipStreamSynchronize(st));   // `all` leaves scope
+    }
     const uint32_t r0 = seed_hash(seed);
     FB_HIP(hipMemcpyAsync(d.rng, &r0, sizeof r0, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(kd_reset, dim3(grid_of(N, 256)), dim3(256), 0, st, d, (float)f->p.vRest);
@@ -1239,16 +1414,25 @@ extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int
     FB_HIP(hipEventCreate(&e1));
     FB_HIP(hipEventRecord(e0, st));
     const int gN = grid_of(N, 256);
-    const bool fused = d.D >= 2 && d.nDriven <= 2048;
     for (int32_t t = 0; t < steps; t++) {
-        if (record && t % d.K == 0) hipLaunchKernelGGL(kd_save, dim3(gN), dim3(256), 0, st, d, t / d.K);
-        if (fused) {
-            hipLaunchKernelGGL(kd_deliver_driven_f, dim3(DELIVER_WG + 1), dim3(1024), 0, st, d, t);
-        } else {
-            hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, d, t, 0);
-            hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, d, t);
+        // the driven set of the step's epoch (none with the soft drive)
+        DiffDev ds = d;
+        if (d.driveSoft) {
+            ds.nDriven = 0;
+        } else if (!f->epCnt.empty()) {
+            const int32_t e = t / d.ep;
+            ds.driven = f->epLists + f->epOff[(size_t)e];
+            ds.nDriven = f->epCnt[(size_t)e];
         }
-        hipLaunchKernelGGL(kd_membrane_f, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, d, t, 0);
+        const bool fused = ds.D >= 2 && ds.nDriven <= 2048;
+        if (record && t % d.K == 0) hipLaunchKernelGGL(kd_save, dim3(gN), dim3(256), 0, st, ds, t / d.K);
+        if (fused) {
+            hipLaunchKernelGGL(kd_deliver_driven_f, dim3(DELIVER_WG + 1), dim3(1024), 0, st, ds, t);
+        } else {
+            hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, ds, t, 0);
+            hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, ds, t);
+        }
+        hipLaunchKernelGGL(kd_membrane_f, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, ds, t, 0);
     }
     FB_HIP(hipGetLastError());
     if ((s = fwd_finish_launch(f, steps, st))) return s;
@@ -1288,6 +1472,7 @@ static fb_status bwd_prepare(fb_diff *f, hipStream_t st) {
     }
     hipLaunchKernelGGL(kd_bclear, dim3(grid_of(N, 256)), dim3(256), 0, st, d, 1);
     hipLaunchKernelGGL(kd_bzero, dim3(grid_of(std::max<int64_t>(f->E, N), 256)), dim3(256), 0, st, d, f->nblk);
+    if (d.gDrive) FB_HIP(hipMemsetAsync(d.gDrive, 0, sizeof(float) * (size_t)d.nEp * d.nDI, st));
     FB_HIP(hipMemsetAsync(f->clampDev, 0, 4 * sizeof(int32_t), st));
     FB_HIP(hipGetLastError());
     return FB_OK;
@@ -1481,6 +1666,9 @@ extern "C" fb_status fb_diff_batch_forward(fb_diff_batch *bt, const int32_t *ste
     int32_t Tmax = 0, Nmax = 0;
     for (int32_t b = 0; b < B; b++) {
         FB_REQUIRE_D(steps[b] >= 0, FB_ERR_ARG, "fb_diff_batch_forward: steps[%d] < 0", b);
+        FB_REQUIRE_D(!bt->m[b]->d.nDI && !bt->m[b]->d.dPer, FB_ERR_UNSUPPORTED,
+                     "fb_diff_batch_forward: member %d has a drive schedule or per-epoch spike weights; "
+                     "run it on its own (fb_diff_forward)", b);
         if ((s = fwd_prepare(bt->m[b], steps[b], seeds[b], record, st))) return s;
         Tmax = std::max(Tmax, steps[b]);
         Nmax = std::max(Nmax, bt->m[b]->N);

@@ -124,6 +124,34 @@ class LIFDiffHip:
         self.R = int(i.size)
         _l.check(self.lib.fb_diff_set_readout(self.h, _l.ptr(i), self.R, _l.ptr(t), float(tau_ms)))
 
+    def set_drive_schedule(self, idx, epoch_steps: int, sched, drive_soft: bool = False):
+        """driveIdx / driveEpoch (src/lifdiff.js:135-144): the rates sched[e][k]
+        that the onEpoch hook gives neuron idx[k] at the start of epoch e,
+        applied with setDriveOne's semantics; drive_soft = driveSoft.  None clears."""
+        if idx is None:
+            _l.check(self.lib.fb_diff_set_drive_schedule(self.h, None, 0, 1, None, 0, 1 if drive_soft else 0))
+            self._dsched = None
+            return
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        sc = np.ascontiguousarray(sched, dtype=np.float32).reshape(-1, i.size)
+        self._dsched = sc.shape
+        _l.check(self.lib.fb_diff_set_drive_schedule(self.h, _l.ptr(i), int(i.size), int(epoch_steps), _l.ptr(sc),
+                                                      int(sc.shape[0]), 1 if drive_soft else 0))
+
+    def set_loss_per_epoch(self, dper, epoch_steps: int):
+        """dLdSpikePerEpoch (src/lifdiff.js:519-522): dper[e][i]; None clears."""
+        if dper is None:
+            _l.check(self.lib.fb_diff_set_loss_per_epoch(self.h, None, 0, int(epoch_steps)))
+            return
+        a = np.ascontiguousarray(dper, dtype=np.float32).reshape(-1, self.N)
+        _l.check(self.lib.fb_diff_set_loss_per_epoch(self.h, _l.ptr(a), int(a.shape[0]), int(epoch_steps)))
+
+    def drive_grad(self) -> np.ndarray:
+        """dL/d sched [epochs][nD] of the last backward (LIFDiff.backward's `drive`)."""
+        out = np.zeros(self._dsched, np.float32)
+        _l.check(self.lib.fb_diff_drive_grad(self.h, _l.ptr(out)))
+        return out
+
     def set_tape_capacity(self, per_step: int):
         _l.check(self.lib.fb_diff_set_tape_capacity(self.h, int(per_step)))
 

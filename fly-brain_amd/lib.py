@@ -89,6 +89,9 @@ _SIGS = {
     "fb_diff_set_loss": (I32, [P, P, I32]),
     "fb_diff_set_readout": (I32, [P, P, I32, P, D]),
     "fb_diff_set_tape_capacity": (I32, [P, I64]),
+    "fb_diff_set_drive_schedule": (I32, [P, P, I32, I32, P, I32, I32]),
+    "fb_diff_set_loss_per_epoch": (I32, [P, P, I32, I32]),
+    "fb_diff_drive_grad": (I32, [P, P]),
     "fb_diff_forward": (I32, [P, I32, D, I32, ctypes.POINTER(D)]),
     "fb_diff_backward": (I32, [P, I32, P, P, P, P, P]),
     "fb_diff_type_grads": (I32, [P, P, I32, P]),

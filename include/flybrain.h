@@ -251,6 +251,22 @@ fb_status fb_diff_set_loss(fb_diff *d, const float *dLdSpike, int32_t lossFrom);
 fb_status fb_diff_set_readout(fb_diff *d, const int32_t *idx, int32_t R, const float *targetHz, double tauMs);
 /* tape entries allowed per step on average (default max(4096, N/4)) */
 fb_status fb_diff_set_tape_capacity(fb_diff *d, int64_t perStep);
+/* driveIdx / driveEpoch (src/lifdiff.js:135-144, 228-234): the drive of the
+ * neurons idx[0..nD) is an input.  At the start of epoch e (every epochSteps
+ * steps) their rates become sched[e*nD + k], through setDriveOne's semantics:
+ * a rate of 0 takes the neuron out of the driven set, a positive rate puts it
+ * back at the end (the onEpoch hook of LIFDiff.forward).  The schedule must
+ * cover the forward pass.  driveSoft (src/lifdiff.js:338): a driven neuron
+ * spikes with the drive's expectation sd + (1 - sd) st instead of a Poisson
+ * draw.  nD = 0 clears. */
+fb_status fb_diff_set_drive_schedule(fb_diff *d, const int32_t *idx, int32_t nD, int32_t epochSteps,
+                                     const float *sched, int32_t nEpochs, int32_t driveSoft);
+/* dLdSpikePerEpoch (src/lifdiff.js:519-522): the loss weight of a spike of
+ * neuron i in epoch e is dPer[e*N + i] (replacing dLdSpike); NULL clears. */
+fb_status fb_diff_set_loss_per_epoch(fb_diff *d, const float *dPer, int32_t nEpochs, int32_t epochSteps);
+/* dL/d sched after fb_diff_backward: [nEpochs][nD] (the `drive` result of
+ * LIFDiff.backward, src/lifdiff.js:490-492) */
+fb_status fb_diff_drive_grad(fb_diff *d, float *gDrive);
 /* forward(steps, {seed, record}) (lifdiff.js:219): from reset; loss = the
  * flat term sum dLdSpike * spike count from lossFrom on, plus the readout term */
 fb_status fb_diff_forward(fb_diff *d, int32_t steps, double seed, int32_t record, double *loss);

@@ -258,3 +258,59 @@ def test_lockstep_members_match_separate_runs():
                 assert res[b][1]["params"][k] == g["params"][k], (b, k)
         total += one.type_grads(types, 9)
     assert np.allclose(bt.type_grads(types, 9), total, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- the drive as an input
+
+def test_soft_drive_schedule_matches_oracle():
+    """driveIdx with driveSoft: the loss, the scalar gradients and the drive
+    gradient per epoch (LIFDiff.backward's `drive`) against the oracle."""
+    pr = dc.Problem().with_schedule(nD=20, epoch=8, steps=STEPS, soft=True)
+    pr.dL[pr.sched[0][::2]] = 1.0
+    print("drive schedule: soft drive", flush=True)
+    o, g, Lo, Lg = _pair(pr)
+    assert Lg == pytest.approx(Lo, rel=1e-6)
+    go, gg = o.backward(), g.backward()
+    for k in P.DIFF_PARAMS:
+        assert gg["params"][k] == pytest.approx(go["params"][k], rel=1e-4, abs=1e-8), k
+    do, dg = o.drive_grad(), g.drive_grad()
+    assert np.abs(do).max() > 0
+    assert rel_err(dg, do) < 1e-4
+
+
+@pytest.mark.parametrize("K", [16, 7])
+def test_hard_drive_schedule_matches_oracle(K):
+    """Hard threshold, rates that drop to 0 and come back (the driven set
+    changes order between epochs), checkpoints that start inside an epoch: the
+    tape is bit-identical to the oracle and the drive gradient agrees."""
+    pr = dc.Problem(over={"soft": False, "tRef": 2.2, "checkpoint": K}).with_schedule(
+        nD=30, epoch=10, steps=150, soft=False, zeros=True)
+    pr.dL[pr.sched[0]] = 1.0
+    print(f"drive schedule: hard, checkpoint {K}", flush=True)
+    o, g, Lo, Lg = _pair(pr, steps=150)
+    off_o, idx_o, amp_o, f_o = o.spikes()
+    off_g, idx_g, amp_g, f_g = g.spikes()
+    assert np.array_equal(off_o, off_g) and np.array_equal(idx_o, idx_g) and np.array_equal(f_o, f_g)
+    assert f_o.sum() > 50
+    for k in STATE:
+        assert np.array_equal(o.state(k).view(np.uint32), g.state(k).view(np.uint32)), k
+    assert o.rng == g.rng
+    go, gg = o.backward(), g.backward()
+    do, dg = o.drive_grad(), g.drive_grad()
+    assert np.abs(do).max() > 0
+    assert rel_err(dg, do) < 1e-5
+    assert rel_err(gg["logGain"], go["logGain"]) < 1e-5
+
+
+def test_per_epoch_loss_matches_oracle():
+    """dLdSpikePerEpoch: the weighted loss and its gradients against the oracle."""
+    pr = dc.Problem().with_schedule(nD=12, epoch=8, steps=STEPS, soft=True, seed=9)
+    rng = np.random.default_rng(4)
+    pr.dper = (rng.normal(0, 1, size=(-(-STEPS // 8), pr.g.N)).astype(np.float32), 8)
+    print("per-epoch spike weights", flush=True)
+    o, g, Lo, Lg = _pair(pr)
+    assert Lg == pytest.approx(Lo, rel=1e-6)
+    go, gg = o.backward(), g.backward()
+    for k in P.DIFF_PARAMS:
+        assert gg["params"][k] == pytest.approx(go["params"][k], rel=1e-4, abs=1e-8), k
+    assert rel_err(g.drive_grad(), o.drive_grad()) < 1e-4
