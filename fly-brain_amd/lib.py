@@ -92,6 +92,15 @@ _SIGS = {
     "fb_diff_set_drive_schedule": (I32, [P, P, I32, I32, P, I32, I32]),
     "fb_diff_set_loss_per_epoch": (I32, [P, P, I32, I32]),
     "fb_diff_drive_grad": (I32, [P, P]),
+    "fb_fv_model_create": (I32, [I32, I64, P, P, P, P, P, D, P]),
+    "fb_fv_model_release": (I32, [P]),
+    "fb_fv_create": (I32, [P, P]),
+    "fb_fv_destroy": (I32, [P]),
+    "fb_fv_set_input": (I32, [P, P]),
+    "fb_fv_step": (I32, [P, I32]),
+    "fb_fv_get": (I32, [P, I32, P]),
+    "fb_fv_set_v": (I32, [P, P]),
+    "fb_fv_reset": (I32, [P]),
     "fb_diff_forward": (I32, [P, I32, D, I32, ctypes.POINTER(D)]),
     "fb_diff_backward": (I32, [P, I32, P, P, P, P, P]),
     "fb_diff_type_grads": (I32, [P, P, I32, P]),

@@ -312,6 +312,32 @@ fb_status fb_diff_batch_backward(fb_diff_batch *b, int32_t truncate);
 /* "forward_ms", "backward_ms" of the last batch passes */
 fb_status fb_diff_batch_stat(fb_diff_batch *b, const char *name, double *value);
 
+/* ------------------------------------------------------------------ flyvis
+ * The optic-lobe rate network that drives the connectome from the eyes
+ * (src/flyvis.js:3-25 over fv_step, src/wasm/lif.c:82-93): passive point
+ * neurons with graded synapses,
+ *   v_i += kdt_i (-v_i + bias_i + sum_{j: v_j > 0} w_ji v_j + x_i),
+ *   kdt = dt / max(tau, dt),
+ * bit-identical to fv_step.  A model (graph by source: indptr[N+1],
+ * target[E], weight[E]; bias, tau per node) is shared by its instances, one
+ * per eye (the `shared` parts of src/flyvis.js:12). */
+typedef struct fb_fv_model fb_fv_model;
+typedef struct fb_fv fb_fv;
+enum { FB_FV_V = 0, FB_FV_ACC = 1 };
+fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *indptr, const int32_t *target, const float *weight,
+                             const float *bias, const float *tau, double dt, fb_fv_model **out);
+fb_status fb_fv_model_release(fb_fv_model *m);
+/* a new instance: v = bias, no input (new FlyVis, src/flyvis.js:4-19) */
+fb_status fb_fv_create(fb_fv_model *m, fb_fv **out);
+fb_status fb_fv_destroy(fb_fv *e);
+/* x[N]: the input of every node (setInput writes luminance into R1-R8) */
+fb_status fb_fv_set_input(fb_fv *e, const float *x);
+fb_status fb_fv_step(fb_fv *e, int32_t nsteps);
+fb_status fb_fv_get(fb_fv *e, int32_t which, float *dst);
+fb_status fb_fv_set_v(fb_fv *e, const float *v);
+/* v = bias (reset(), src/flyvis.js:24) */
+fb_status fb_fv_reset(fb_fv *e);
+
 #ifdef __cplusplus
 }
 #endif

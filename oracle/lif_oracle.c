@@ -327,3 +327,25 @@ void fbo_set_background(fbo_brain *b, double rateHz, double ampMv, double dt) {
     b->k.bgEvents = (float)((double)b->N * rateHz * dt / 1000.0);
     b->k.bgAmp = (float)ampMv;
 }
+
+/* ------------------------------------------------------------------ flyvis */
+
+/* src/wasm/lif.c:82-93: graded synapses accumulate the rectified source
+ * activity into acc in source order, then a leaky update with kdt =
+ * dt / max(tau, dt).  Every operation is binary32 in C order. */
+void fbo_fv_step(int32_t N, const float *bias, const float *kdt, const int32_t *indptr, const int32_t *target,
+                 const float *weight, float *v, float *acc, const float *x) {
+    for (int32_t i = 0; i < N; i++) acc[i] = 0.f;
+    for (int32_t j = 0; j < N; j++) {
+        const float r = v[j];
+        if (r <= 0.f) continue;
+        for (int32_t k = indptr[j]; k < indptr[j + 1]; k++) {
+            const float w = weight[k] * r;
+            acc[target[k]] = acc[target[k]] + w;
+        }
+    }
+    for (int32_t i = 0; i < N; i++) {
+        const float d = ((-v[i] + bias[i]) + acc[i]) + x[i];
+        v[i] = v[i] + kdt[i] * d;
+    }
+}

@@ -59,6 +59,12 @@ void fbo_add(fbo_brain *b, int32_t which, int32_t i, double delta);
 void fbo_store(fbo_brain *b, int32_t which, int32_t i, double value);
 void fbo_set_background(fbo_brain *b, double rateHz, double ampMv, double dt);
 
+/* The flyvis optic-lobe rate network's step (src/wasm/lif.c:82-93):
+ * acc = sum over sources j with v_j > 0 (ascending j, each row in order) of
+ * weight * v_j into acc[target]; then v += kdt * (-v + bias + acc + x). */
+void fbo_fv_step(int32_t N, const float *bias, const float *kdt, const int32_t *indptr, const int32_t *target,
+                 const float *weight, float *v, float *acc, const float *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -524,3 +524,29 @@ def _lib_net():
         f.restype = res
         f.argtypes = args
     return lib
+
+
+# ---------------------------------------------------------------- flyvis
+
+def fv_step(kind: str, bias, kdt, indptr, target, weight, v, acc, x):
+    """One fv_step on the checker: ``kind="oracle"`` runs the restatement
+    (fbo_fv_step), ``kind="ref"`` the reference's own fv_step compiled into
+    oracle/_ref (src/wasm/lif.c:86-93).  v and acc are updated in place
+    (float32 arrays)."""
+    lib = ctypes.CDLL(ORACLE_SO if kind == "oracle" else REF_SO)
+    f = lib.fbo_fv_step if kind == "oracle" else lib.fv_step
+    f.restype = None
+    P = ctypes.c_void_p
+    f.argtypes = [ctypes.c_int32, P, P, P, P, P, P, P, P]
+    arrs = [np.ascontiguousarray(a, dtype=dt) for a, dt in
+            ((bias, np.float32), (kdt, np.float32), (indptr, np.int32), (target, np.int32), (weight, np.float32),
+             (x, np.float32))]
+    assert v.dtype == np.float32 and acc.dtype == np.float32 and v.flags.c_contiguous and acc.flags.c_contiguous
+    f(int(v.size), _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3]), _ptr(arrs[4]), _ptr(v), _ptr(acc),
+      _ptr(arrs[5]))
+
+
+def fv_kdt(tau, dt: float = 0.02) -> np.ndarray:
+    """kdt = dt / max(tau, dt) in binary64, stored as binary32 (src/flyvis.js:13)."""
+    t = np.asarray(tau, dtype=np.float32).astype(np.float64)
+    return (dt / np.maximum(t, dt)).astype(np.float32)
