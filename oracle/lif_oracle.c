@@ -349,3 +349,36 @@ void fbo_fv_step(int32_t N, const float *bias, const float *kdt, const int32_t *
         v[i] = v[i] + kdt[i] * d;
     }
 }
+
+/* src/flyvisdiff.js:154-182: v_new = v + kdt (-v + bias + acc + x), so the
+ * adjoint l of v_new gives kdt (l u), bias (l kdt), the fan-in (lacc = l kdt)
+ * and v itself (l (1 - kdt)); the fan-out then passes lacc back through the
+ * rectified sources, and gives each edge lacc[target] * v_source. */
+void fbo_fv_back_step(int32_t N, const int32_t *indptr, const int32_t *target, const float *weight, const float *kdt,
+                      const float *vPre, const float *uAt, const float *dLdV, float *lv, float *lacc, float *gw,
+                      float *gb, float *gk, float *dLdLum, const int32_t *inNode, const int32_t *inCol, int32_t nIn) {
+    for (int32_t i = 0; i < N; i++) {
+        const double l = (double)lv[i] + (double)dLdV[i];
+        if (l == 0.0) { lacc[i] = 0.f; lv[i] = 0.f; continue; }
+        const double k = kdt[i];
+        gk[i] = (float)((double)gk[i] + l * (double)uAt[i]);
+        gb[i] = (float)((double)gb[i] + l * k);
+        lacc[i] = (float)(l * k);
+        lv[i] = (float)(l * (1.0 - k));
+    }
+    if (dLdLum)
+        for (int32_t q = 0; q < nIn; q++)
+            dLdLum[inCol[q]] = (float)((double)dLdLum[inCol[q]] + (double)lacc[inNode[q]]);
+    for (int32_t j = 0; j < N; j++) {
+        const double r = vPre[j];
+        if (r <= 0.0) continue;
+        double s = 0.0;
+        for (int32_t k = indptr[j]; k < indptr[j + 1]; k++) {
+            const double la = lacc[target[k]];
+            if (la == 0.0) continue;
+            s += la * (double)weight[k];
+            gw[k] = (float)((double)gw[k] + la * r);
+        }
+        lv[j] = (float)((double)lv[j] + s);
+    }
+}

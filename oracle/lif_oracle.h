@@ -64,6 +64,15 @@ void fbo_set_background(fbo_brain *b, double rateHz, double ampMv, double dt);
  * weight * v_j into acc[target]; then v += kdt * (-v + bias + acc + x). */
 void fbo_fv_step(int32_t N, const float *bias, const float *kdt, const int32_t *indptr, const int32_t *target,
                  const float *weight, float *v, float *acc, const float *x);
+/* One reverse step of FlyVisDiff.backward (src/flyvisdiff.js:154-182) at
+ * step t: vPre / uAt are the step's pre-update activity and integrator
+ * argument, dLdV its loss term; lv / lacc are the carried adjoints; gw / gb /
+ * gk accumulate; dLdLum (may be NULL) gets lacc of every input node
+ * inNode[q] added to its column inCol[q], in q order.  JavaScript semantics:
+ * binary64 arithmetic, binary32 stores. */
+void fbo_fv_back_step(int32_t N, const int32_t *indptr, const int32_t *target, const float *weight, const float *kdt,
+                      const float *vPre, const float *uAt, const float *dLdV, float *lv, float *lacc, float *gw,
+                      float *gb, float *gk, float *dLdLum, const int32_t *inNode, const int32_t *inCol, int32_t nIn);
 
 #ifdef __cplusplus
 }

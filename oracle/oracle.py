@@ -550,3 +550,76 @@ def fv_kdt(tau, dt: float = 0.02) -> np.ndarray:
     """kdt = dt / max(tau, dt) in binary64, stored as binary32 (src/flyvis.js:13)."""
     t = np.asarray(tau, dtype=np.float32).astype(np.float64)
     return (dt / np.maximum(t, dt)).astype(np.float32)
+
+
+class FlyVisDiffOracle:
+    """FlyVisDiff (src/flyvisdiff.js:36-185) on the CPU checker: a taped forward
+    (fv_step, with each step's pre-update activity and integrator argument
+    kept) and the reverse sweep.  ``model`` as parse_flyvis returns it; the
+    input map is flattened type by type as the reference does."""
+
+    def __init__(self, model: dict, kdt=None):
+        self.m = model
+        self.N, self.E = int(model["N"]), int(model["E"])
+        self.bias = np.ascontiguousarray(model["bias"], dtype=np.float32)
+        self.kdt = np.ascontiguousarray(kdt if kdt is not None else fv_kdt(model["tau"], model.get("dt") or 0.02),
+                                        dtype=np.float32)
+        self.indptr = np.ascontiguousarray(model["indptr"], dtype=np.int32)
+        self.target = np.ascontiguousarray(model["target"], dtype=np.int32)
+        self.weight = np.ascontiguousarray(model["weight"], dtype=np.float32)
+        ix, col = [], []
+        for a in (model.get("inputIdx") or {}).values():
+            ix += list(np.asarray(a).tolist())
+            col += list(range(len(a)))
+        self.inNode = np.asarray(ix, dtype=np.int32)
+        self.inCol = np.asarray(col, dtype=np.int32)
+        self.nCol = int(self.inCol.max()) + 1 if self.inCol.size else 0
+        self.v = self.bias.copy()
+        self.acc = np.zeros(self.N, np.float32)
+        self.x = np.zeros(self.N, np.float32)
+        lib = ctypes.CDLL(ORACLE_SO)
+        self._step = lib.fbo_fv_step
+        self._step.restype = None
+        self._back = lib.fbo_fv_back_step
+        self._back.restype = None
+        self.tape = None
+
+    def reset(self):
+        self.v = self.bias.copy()
+
+    def setInput(self, lum):
+        lum = np.asarray(lum, dtype=np.float32)
+        self.x[self.inNode] = lum[self.inCol]
+        self.lum = lum.copy()
+
+    def set_x(self, x):
+        self.x = np.ascontiguousarray(x, dtype=np.float32).copy()
+
+    def step(self):
+        vPre = self.v.copy()
+        self._step(ctypes.c_int32(self.N), _ptr(self.bias), _ptr(self.kdt), _ptr(self.indptr), _ptr(self.target),
+                   _ptr(self.weight), _ptr(self.v), _ptr(self.acc), _ptr(self.x))
+        if self.tape is not None:
+            u = ((-vPre + self.bias) + self.acc) + self.x       # binary32, as the step formed it
+            self.tape.append((vPre, u.astype(np.float32), getattr(self, "lum", None)))
+
+    def start_tape(self):
+        self.tape = []
+
+    def backward(self, dLdV, g=None, dLdLum=None):
+        """dLdV [steps][N]; g = {weight, bias, kdt} accumulators (f32); dLdLum
+        [steps][nCol] accumulator or None."""
+        N = self.N
+        g = g or {"weight": np.zeros(self.E, np.float32), "bias": np.zeros(N, np.float32),
+                  "kdt": np.zeros(N, np.float32)}
+        lv = np.zeros(N, np.float32)
+        lacc = np.zeros(N, np.float32)
+        dLdV = np.ascontiguousarray(dLdV, dtype=np.float32).reshape(len(self.tape), N)
+        for t in range(len(self.tape) - 1, -1, -1):
+            vPre, u, _ = self.tape[t]
+            row = None if dLdLum is None else dLdLum[t]
+            self._back(ctypes.c_int32(N), _ptr(self.indptr), _ptr(self.target), _ptr(self.weight), _ptr(self.kdt),
+                       _ptr(vPre), _ptr(u), _ptr(dLdV[t]), _ptr(lv), _ptr(lacc), _ptr(g["weight"]), _ptr(g["bias"]),
+                       _ptr(g["kdt"]), None if row is None else _ptr(row), _ptr(self.inNode), _ptr(self.inCol),
+                       ctypes.c_int32(self.inNode.size))
+        return g

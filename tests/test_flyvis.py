@@ -97,3 +97,122 @@ def test_rectified_sources_only(oracle_built):
          "tau": np.full(3, 1.0, np.float32), "dt": 0.02}
     v, acc = run_checker("oracle", m, [np.zeros(3, np.float32)], steps_per_x=1)
     assert acc[2] == np.float32(2.0) * np.float32(0.5)
+
+
+# ---------------------------------------------------------------- the adjoint (src/flyvisdiff.js)
+
+def small_model(seed=5, N=300, nCol=24):
+    m = random_model(N=N, deg=10, seed=seed)
+    m["weight"] = (m["weight"] * 2.5).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    nodes = rng.permutation(N)[: 2 * nCol]
+    m["inputIdx"] = {"R1": nodes[:nCol], "R2": nodes[nCol:]}
+    return m
+
+
+def _loss64(m, inNode, inCol, lums, dLdV, bias=None, kdt=None, weight=None):
+    """L = sum_t dLdV_t . v_t of the rate network run in f64 from v = bias of
+    the model (the start state is an input, held fixed while a parameter
+    moves)."""
+    N = int(m["N"])
+    bias = m["bias"].astype(np.float64) if bias is None else bias
+    w = m["weight"].astype(np.float64) if weight is None else weight
+    src = np.repeat(np.arange(N), np.diff(m["indptr"]))
+    v = m["bias"].astype(np.float64).copy()
+    L, vs = 0.0, []
+    for t, lum in enumerate(lums):
+        x = np.zeros(N)
+        x[inNode] = np.asarray(lum, np.float64)[inCol]
+        acc = np.zeros(N)
+        np.add.at(acc, m["target"], w * np.maximum(v, 0.0)[src])
+        v = v + kdt * (-v + bias + acc + x)
+        vs.append(v)
+        L += float(dLdV[t].astype(np.float64) @ v)
+    return L, vs
+
+
+def test_flyvis_adjoint_matches_finite_differences(oracle_built):
+    """dL/d bias, kdt, weight and luminance of L = sum_t dLdV_t . v_t against
+    central differences of the forward (src/flyvisdiff.js:4-6: the network is
+    smooth away from the rectifier's kink, so the differences are taken in f64
+    with a step far below the distance to any kink; the f64 forward is first
+    checked against the binary32 forward the reference runs)."""
+    m = small_model()
+    rng = np.random.default_rng(6)
+    T = 24
+    lums = [rng.uniform(0, 1, 24).astype(np.float32) for _ in range(T)]
+    dLdV = np.zeros((T, m["N"]), np.float32)
+    dLdV[:, rng.choice(m["N"], 40, replace=False)] = rng.normal(0, 1, (T, 40))
+    o = orc.FlyVisDiffOracle(m)
+    o.start_tape()
+    vs32 = []
+    for t in range(T):
+        o.setInput(lums[t])
+        o.step()
+        vs32.append(o.v.copy())
+    kdt = o.kdt.astype(np.float64)
+    L, vs = _loss64(m, o.inNode, o.inCol, lums, dLdV, kdt=kdt)
+    assert np.abs(np.array(vs) - np.array(vs32)).max() < 1e-4
+    dLum = np.zeros((T, o.nCol), np.float32)
+    g = o.backward(dLdV, dLdLum=dLum)
+    h = 1e-6
+
+    def fd(**kw):
+        """central difference along kw = {name: (value, direction)}"""
+        up = {k: (v[0] + h * v[1]) for k, v in kw.items()}
+        dn = {k: (v[0] - h * v[1]) for k, v in kw.items()}
+        return (_loss64(m, o.inNode, o.inCol, lums, dLdV, kdt=up.pop("kdt", kdt), **up)[0]
+                - _loss64(m, o.inNode, o.inCol, lums, dLdV, kdt=dn.pop("kdt", kdt), **dn)[0]) / (2 * h)
+
+    def close(a, f):
+        return abs(a - f) <= 1e-4 * abs(f) + 1e-5
+
+    ok = 0
+    b64 = m["bias"].astype(np.float64)
+    for i in rng.choice(m["N"], 8, replace=False):
+        e = np.zeros(m["N"])
+        e[i] = 1.0
+        f = fd(bias=(b64, e))
+        assert close(g["bias"][i], f), ("bias", i, g["bias"][i], f)
+        f = fd(kdt=(kdt, e))
+        assert close(g["kdt"][i], f), ("kdt", i, g["kdt"][i], f)
+        ok += abs(f) > 1e-3
+    w64 = m["weight"].astype(np.float64)
+    live = np.flatnonzero(g["weight"] != 0)
+    for k in rng.choice(live, 8, replace=False):
+        e = np.zeros(m["E"])
+        e[k] = 1.0
+        f = fd(weight=(w64, e))
+        assert close(g["weight"][k], f), ("weight", k, g["weight"][k], f)
+    for t, c in ((0, 3), (5, 10), (12, 0), (20, 23), (23, 7)):
+        lu = [x.astype(np.float64) for x in lums]
+        ld = [x.astype(np.float64) for x in lums]
+        lu[t][c] += h
+        ld[t][c] -= h
+        f = (_loss64(m, o.inNode, o.inCol, lu, dLdV, kdt=kdt)[0]
+             - _loss64(m, o.inNode, o.inCol, ld, dLdV, kdt=kdt)[0]) / (2 * h)
+        assert close(dLum[t, c], f), ("lum", t, c, dLum[t, c], f)
+    assert ok >= 3
+    assert np.count_nonzero(dLum[:, :o.nCol]) > T   # the input map reaches the loss
+
+
+def test_flyvis_adjoint_closed_form_lone_node(oracle_built):
+    """One node, no edges, loss v_T: dv_T/dbias = 1 - (1 - kdt)^T and
+    dv_T/dx_t = kdt (1 - kdt)^(T-1-t)."""
+    m = {"N": 1, "E": 0, "indptr": np.zeros(2, np.int32), "target": np.zeros(0, np.int32),
+         "weight": np.zeros(0, np.float32), "bias": np.array([0.3], np.float32), "tau": np.array([0.1], np.float32),
+         "dt": 0.02, "inputIdx": {"R1": np.array([0])}}
+    T = 10
+    o = orc.FlyVisDiffOracle(m)
+    o.start_tape()
+    for t in range(T):
+        o.setInput(np.array([0.5], np.float32))
+        o.step()
+    dLdV = np.zeros((T, 1), np.float32)
+    dLdV[-1, 0] = 1.0
+    dLum = np.zeros((T, 1), np.float32)
+    g = o.backward(dLdV, dLdLum=dLum)
+    k = float(o.kdt[0])
+    assert g["bias"][0] == pytest.approx(1 - (1 - k) ** T, rel=1e-6)
+    for t in range(T):
+        assert dLum[t, 0] == pytest.approx(k * (1 - k) ** (T - 1 - t), rel=1e-6)
