@@ -10,6 +10,16 @@
 // that same (source, row position) order, so every sum is bit-identical; the
 // update reads the previous activity from one buffer and writes the other.
 // A model (graph, bias, kdt) is shared by its instances (one per eye).
+//
+// The reverse sweep (FlyVisDiff.backward, src/flyvisdiff.js:128-185) runs
+// over a tape that keeps every taped step's pre-step activity and integrator
+// argument in HBM (two N-float rows a step; the reference keeps a state every
+// four steps and replays, which trades the memory a GPU has for compute).
+// One launch per step fuses the fan-out of step t+1 (per source, in row
+// order, a binary64 running sum as the reference's), the per-node pass of
+// step t and the luminance gradient of step t+1 (per hex column, in map
+// order); the two lacc rows alternate between launches.  Arithmetic follows
+// JavaScript's: binary64 operations, binary32 stores.
 #include <algorithm>
 #include <numeric>
 
@@ -22,21 +32,38 @@ struct fb_fv_model {
     int32_t *inOff = nullptr, *inSrc = nullptr;
     float *inW = nullptr, *bias = nullptr, *kdt = nullptr;
     std::vector<float> biasH;
+    // the graph by source, for the reverse sweep's fan-out
+    int32_t *indptr = nullptr, *target = nullptr;
+    float *weight = nullptr;
+    // the photoreceptor map: x[xNode[q]] = lum[xCol[q]] (the last entry of a
+    // node in map order wins), and per hex column its input nodes in map order
+    int32_t nIn = 0, nCol = 0, nX = 0;
+    int32_t *xNode = nullptr, *xCol = nullptr, *colOff = nullptr, *colNode = nullptr;
 };
 
 struct fb_fv {
     fb_fv_model *m = nullptr;
     float *v[2] = {nullptr, nullptr};
-    float *acc = nullptr, *x = nullptr;
+    float *acc = nullptr, *x = nullptr, *lum = nullptr;
     int cur = 0;
     hipStream_t stream = nullptr;
+    // tape: tapeV[t][N] the activity before taped step t, tapeU[t][N] its
+    // integrator argument ((-v + bias) + acc) + x
+    float *tapeV = nullptr, *tapeU = nullptr;
+    int32_t tapeCap = 0, tapeN = 0;
+    // reverse sweep: carried adjoints and device copies of the accumulators
+    float *lv = nullptr, *lacc[2] = {nullptr, nullptr}, *gw = nullptr, *gb = nullptr, *gk = nullptr;
+    float *dLdV = nullptr, *dLum = nullptr;
+    size_t dLdVCap = 0, dLumCap = 0;
 };
 
 namespace fb {
 
+template <bool TAPE>
 __global__ void __launch_bounds__(256) k_fv_step(int32_t N, const int32_t *inOff, const int32_t *inSrc,
                                                  const float *inW, const float *bias, const float *kdt,
-                                                 const float *vIn, float *vOut, float *acc, const float *x) {
+                                                 const float *vIn, float *vOut, float *acc, const float *x,
+                                                 float *tapeV, float *tapeU) {
     const int32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
     float a = 0.f;
@@ -50,6 +77,67 @@ __global__ void __launch_bounds__(256) k_fv_step(int32_t N, const int32_t *inOff
     const float vi = vIn[i];
     const float d = ((-vi + bias[i]) + a) + x[i];
     vOut[i] = vi + kdt[i] * d;
+    if (TAPE) {
+        tapeV[i] = vi;
+        tapeU[i] = d;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fv_lum(int32_t nX, const int32_t *xNode, const int32_t *xCol,
+                                                const float *lum, float *x) {
+    const int32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q < nX) x[xNode[q]] = lum[xCol[q]];
+}
+
+// One launch of the reverse sweep.  tNext >= 0: the fan-out and luminance
+// gradient of step tNext, from laccNext (that step's lacc, complete since the
+// previous launch).  tCur >= 0: the per-node pass of step tCur into laccCur.
+// Thread j owns lv[j], gb[j], gk[j], the row of source j in gw and, for
+// j < nCol, hex column j of dLum, so no two threads write one word.
+__global__ void __launch_bounds__(256) k_fv_back(int32_t N, int32_t nCol, int32_t tNext, int32_t tCur,
+                                                 const int32_t *indptr, const int32_t *target, const float *weight,
+                                                 const float *kdt, const float *tapeV, const float *tapeU,
+                                                 const float *dLdV, float *lv, const float *laccNext, float *laccCur,
+                                                 float *gw, float *gb, float *gk, const int32_t *colOff,
+                                                 const int32_t *colNode, float *dLum) {
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (tNext >= 0 && dLum != nullptr && j < nCol) {
+        float *row = dLum + (size_t)tNext * nCol;
+        float a = row[j];
+        for (int32_t q = colOff[j], e = colOff[j + 1]; q < e; q++)
+            a = (float)((double)a + (double)laccNext[colNode[q]]);
+        row[j] = a;
+    }
+    if (j >= N) return;
+    float lvj = lv[j];
+    if (tNext >= 0) {
+        const double r = tapeV[(size_t)tNext * N + j];
+        if (r > 0.0) {
+            double s = 0.0;
+            for (int32_t k = indptr[j], e = indptr[j + 1]; k < e; k++) {
+                const double la = laccNext[target[k]];
+                if (la == 0.0) continue;
+                s += la * (double)weight[k];
+                gw[k] = (float)((double)gw[k] + la * r);
+            }
+            lvj = (float)((double)lvj + s);
+        }
+    }
+    if (tCur >= 0) {
+        const size_t o = (size_t)tCur * N + j;
+        const double l = (double)lvj + (double)dLdV[o];
+        if (l == 0.0) {
+            laccCur[j] = 0.f;
+            lvj = 0.f;
+        } else {
+            const double k = kdt[j];
+            gk[j] = (float)((double)gk[j] + l * (double)tapeU[o]);
+            gb[j] = (float)((double)gb[j] + l * k);
+            laccCur[j] = (float)(l * k);
+            lvj = (float)(l * (1.0 - k));
+        }
+    }
+    lv[j] = lvj;
 }
 
 }  // namespace fb
@@ -67,6 +155,8 @@ using namespace fb;
 static void fv_model_free(fb_fv_model *m) {
     (void)hipFree(m->inOff); (void)hipFree(m->inSrc); (void)hipFree(m->inW);
     (void)hipFree(m->bias); (void)hipFree(m->kdt);
+    (void)hipFree(m->indptr); (void)hipFree(m->target); (void)hipFree(m->weight);
+    (void)hipFree(m->xNode); (void)hipFree(m->xCol); (void)hipFree(m->colOff); (void)hipFree(m->colNode);
     delete m;
 }
 
@@ -105,7 +195,10 @@ extern "C" fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *ind
     if (hipMalloc(&m->inOff, sizeof(int32_t) * ((size_t)N + 1)) != hipSuccess ||
         hipMalloc(&m->inSrc, sizeof(int32_t) * src.size()) != hipSuccess ||
         hipMalloc(&m->inW, sizeof(float) * w.size()) != hipSuccess ||
-        hipMalloc(&m->bias, sizeof(float) * N) != hipSuccess || hipMalloc(&m->kdt, sizeof(float) * N) != hipSuccess) {
+        hipMalloc(&m->bias, sizeof(float) * N) != hipSuccess || hipMalloc(&m->kdt, sizeof(float) * N) != hipSuccess ||
+        hipMalloc(&m->indptr, sizeof(int32_t) * ((size_t)N + 1)) != hipSuccess ||
+        hipMalloc(&m->target, sizeof(int32_t) * src.size()) != hipSuccess ||
+        hipMalloc(&m->weight, sizeof(float) * w.size()) != hipSuccess) {
         fv_model_free(m);
         set_error("fb_fv_model_create: device allocation failed");
         return FB_ERR_DEVICE;
@@ -115,7 +208,60 @@ extern "C" fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *ind
     FB_HIP(hipMemcpy(m->inW, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(m->bias, bias, sizeof(float) * N, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(m->kdt, kdt.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->indptr, indptr, sizeof(int32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
+    if (E > 0) {
+        FB_HIP(hipMemcpy(m->target, target, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
+        FB_HIP(hipMemcpy(m->weight, weight, sizeof(float) * (size_t)E, hipMemcpyHostToDevice));
+    }
     *out = m;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_model_set_kdt(fb_fv_model *m, const float *kdt) {
+    FB_REQUIRE(m && kdt, FB_ERR_ARG, "fb_fv_model_set_kdt: bad arguments");
+    FB_HIP(hipMemcpy(m->kdt, kdt, sizeof(float) * (size_t)m->N, hipMemcpyHostToDevice));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_model_set_input_map(fb_fv_model *m, int32_t nIn, const int32_t *inNode,
+                                               const int32_t *inCol) {
+    FB_REQUIRE(m && nIn > 0 && inNode && inCol, FB_ERR_ARG, "fb_fv_model_set_input_map: bad arguments");
+    FB_REQUIRE(m->nIn == 0, FB_ERR_ARG, "fb_fv_model_set_input_map: the model already has an input map");
+    int32_t nCol = 0;
+    for (int32_t q = 0; q < nIn; q++) {
+        FB_REQUIRE(inNode[q] >= 0 && inNode[q] < m->N && inCol[q] >= 0, FB_ERR_ARG,
+                   "fb_fv_model_set_input_map: entry %d out of range", q);
+        nCol = std::max(nCol, inCol[q] + 1);
+    }
+    // writes: the last entry of each node in map order
+    std::vector<int32_t> last((size_t)m->N, -1);
+    for (int32_t q = 0; q < nIn; q++) last[(size_t)inNode[q]] = q;
+    std::vector<int32_t> xNode, xCol;
+    for (int32_t q = 0; q < nIn; q++)
+        if (last[(size_t)inNode[q]] == q) {
+            xNode.push_back(inNode[q]);
+            xCol.push_back(inCol[q]);
+        }
+    // per column, its entries in map order (a stable counting sort)
+    std::vector<int32_t> off((size_t)nCol + 1, 0), node((size_t)nIn);
+    for (int32_t q = 0; q < nIn; q++) off[(size_t)inCol[q] + 1]++;
+    for (int32_t c = 0; c < nCol; c++) off[(size_t)c + 1] += off[(size_t)c];
+    std::vector<int32_t> pos(off.begin(), off.end() - 1);
+    for (int32_t q = 0; q < nIn; q++) node[(size_t)pos[(size_t)inCol[q]]++] = inNode[q];
+    if (hipMalloc(&m->xNode, sizeof(int32_t) * xNode.size()) != hipSuccess ||
+        hipMalloc(&m->xCol, sizeof(int32_t) * xCol.size()) != hipSuccess ||
+        hipMalloc(&m->colOff, sizeof(int32_t) * off.size()) != hipSuccess ||
+        hipMalloc(&m->colNode, sizeof(int32_t) * node.size()) != hipSuccess) {
+        set_error("fb_fv_model_set_input_map: device allocation failed");
+        return FB_ERR_DEVICE;
+    }
+    FB_HIP(hipMemcpy(m->xNode, xNode.data(), sizeof(int32_t) * xNode.size(), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->xCol, xCol.data(), sizeof(int32_t) * xCol.size(), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->colOff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->colNode, node.data(), sizeof(int32_t) * node.size(), hipMemcpyHostToDevice));
+    m->nIn = nIn;
+    m->nCol = nCol;
+    m->nX = (int32_t)xNode.size();
     return FB_OK;
 }
 
@@ -151,6 +297,9 @@ extern "C" fb_status fb_fv_destroy(fb_fv *e) {
     FB_REQUIRE(e, FB_ERR_ARG, "fb_fv_destroy: null instance");
     (void)hipStreamSynchronize(e->stream);
     (void)hipFree(e->v[0]); (void)hipFree(e->v[1]); (void)hipFree(e->acc); (void)hipFree(e->x);
+    (void)hipFree(e->lum); (void)hipFree(e->tapeV); (void)hipFree(e->tapeU); (void)hipFree(e->lv);
+    (void)hipFree(e->lacc[0]); (void)hipFree(e->lacc[1]); (void)hipFree(e->gw); (void)hipFree(e->gb);
+    (void)hipFree(e->gk); (void)hipFree(e->dLdV); (void)hipFree(e->dLum);
     (void)hipStreamDestroy(e->stream);
     fb_fv_model *m = e->m;
     delete e;
@@ -164,16 +313,140 @@ extern "C" fb_status fb_fv_set_input(fb_fv *e, const float *x) {
     return FB_OK;
 }
 
-extern "C" fb_status fb_fv_step(fb_fv *e, int32_t nsteps) {
-    FB_REQUIRE(e && nsteps >= 0, FB_ERR_ARG, "fb_fv_step: bad arguments");
+static fb_status fv_run(fb_fv *e, int32_t nsteps, bool tape) {
     const fb_fv_model *m = e->m;
     const int grid = (m->N + 255) / 256;
+    const size_t N = (size_t)m->N;
     for (int32_t s = 0; s < nsteps; s++) {
-        hipLaunchKernelGGL(k_fv_step, dim3(grid), dim3(256), 0, e->stream, m->N, m->inOff, m->inSrc, m->inW, m->bias,
-                           m->kdt, e->v[e->cur], e->v[e->cur ^ 1], e->acc, e->x);
+        if (tape) {
+            hipLaunchKernelGGL(k_fv_step<true>, dim3(grid), dim3(256), 0, e->stream, m->N, m->inOff, m->inSrc, m->inW,
+                               m->bias, m->kdt, e->v[e->cur], e->v[e->cur ^ 1], e->acc, e->x,
+                               e->tapeV + (size_t)e->tapeN * N, e->tapeU + (size_t)e->tapeN * N);
+            e->tapeN++;
+        } else {
+            hipLaunchKernelGGL(k_fv_step<false>, dim3(grid), dim3(256), 0, e->stream, m->N, m->inOff, m->inSrc,
+                               m->inW, m->bias, m->kdt, e->v[e->cur], e->v[e->cur ^ 1], e->acc, e->x, nullptr,
+                               nullptr);
+        }
         e->cur ^= 1;
     }
     FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_step(fb_fv *e, int32_t nsteps) {
+    FB_REQUIRE(e && nsteps >= 0, FB_ERR_ARG, "fb_fv_step: bad arguments");
+    return fv_run(e, nsteps, false);
+}
+
+extern "C" fb_status fb_fv_set_lum(fb_fv *e, const float *lum) {
+    FB_REQUIRE(e && lum, FB_ERR_ARG, "fb_fv_set_lum: bad arguments");
+    const fb_fv_model *m = e->m;
+    FB_REQUIRE(m->nIn > 0, FB_ERR_ARG, "fb_fv_set_lum: the model has no input map");
+    if (!e->lum && hipMalloc(&e->lum, sizeof(float) * (size_t)m->nCol) != hipSuccess) {
+        set_error("fb_fv_set_lum: device allocation failed");
+        return FB_ERR_DEVICE;
+    }
+    FB_HIP(hipMemcpyAsync(e->lum, lum, sizeof(float) * (size_t)m->nCol, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_fv_lum, dim3((m->nX + 255) / 256), dim3(256), 0, e->stream, m->nX, m->xNode, m->xCol, e->lum,
+                       e->x);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(e->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_tape_start(fb_fv *e, int32_t steps) {
+    FB_REQUIRE(e && steps > 0, FB_ERR_ARG, "fb_fv_tape_start: bad arguments");
+    if (steps > e->tapeCap) {
+        FB_HIP(hipStreamSynchronize(e->stream));
+        (void)hipFree(e->tapeV);
+        (void)hipFree(e->tapeU);
+        e->tapeV = e->tapeU = nullptr;
+        e->tapeCap = 0;
+        const size_t bytes = sizeof(float) * (size_t)steps * (size_t)e->m->N;
+        if (hipMalloc(&e->tapeV, bytes) != hipSuccess || hipMalloc(&e->tapeU, bytes) != hipSuccess) {
+            (void)hipFree(e->tapeV);
+            e->tapeV = nullptr;
+            set_error("fb_fv_tape_start: device allocation of %d steps failed", steps);
+            return FB_ERR_DEVICE;
+        }
+        e->tapeCap = steps;
+    }
+    e->tapeN = 0;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_step_taped(fb_fv *e, int32_t nsteps) {
+    FB_REQUIRE(e && nsteps >= 0, FB_ERR_ARG, "fb_fv_step_taped: bad arguments");
+    FB_REQUIRE(e->tapeCap > 0, FB_ERR_ARG, "fb_fv_step_taped: no tape (call fb_fv_tape_start)");
+    FB_REQUIRE((int64_t)e->tapeN + nsteps <= e->tapeCap, FB_ERR_ARG,
+               "fb_fv_step_taped: the tape holds %d steps, %d are taped", e->tapeCap, e->tapeN);
+    return fv_run(e, nsteps, true);
+}
+
+extern "C" fb_status fb_fv_tape_len(fb_fv *e, int32_t *n) {
+    FB_REQUIRE(e && n, FB_ERR_ARG, "fb_fv_tape_len: bad arguments");
+    *n = e->tapeN;
+    return FB_OK;
+}
+
+static bool fv_grow(float **p, size_t *cap, size_t n) {
+    if (n <= *cap) return true;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, sizeof(float) * n) != hipSuccess) return false;
+    *cap = n;
+    return true;
+}
+
+extern "C" fb_status fb_fv_backward(fb_fv *e, const float *dLdV, float *gW, float *gBias, float *gKdt,
+                                    float *dLdLum) {
+    FB_REQUIRE(e && dLdV && gW && gBias && gKdt, FB_ERR_ARG, "fb_fv_backward: bad arguments");
+    const fb_fv_model *m = e->m;
+    FB_REQUIRE(!dLdLum || m->nIn > 0, FB_ERR_ARG, "fb_fv_backward: a luminance gradient needs the input map");
+    const int32_t n = e->tapeN;
+    if (n == 0) return FB_OK;
+    const size_t N = (size_t)m->N, E = (size_t)m->E, nCol = (size_t)m->nCol;
+    FB_HIP(hipStreamSynchronize(e->stream));
+    if (!e->lv) {
+        if (hipMalloc(&e->lv, sizeof(float) * N) != hipSuccess || hipMalloc(&e->lacc[0], sizeof(float) * N) != hipSuccess ||
+            hipMalloc(&e->lacc[1], sizeof(float) * N) != hipSuccess ||
+            hipMalloc(&e->gw, sizeof(float) * std::max<size_t>(E, 1)) != hipSuccess ||
+            hipMalloc(&e->gb, sizeof(float) * N) != hipSuccess || hipMalloc(&e->gk, sizeof(float) * N) != hipSuccess) {
+            set_error("fb_fv_backward: device allocation failed");
+            return FB_ERR_DEVICE;
+        }
+    }
+    if (!fv_grow(&e->dLdV, &e->dLdVCap, (size_t)n * N) || (dLdLum && !fv_grow(&e->dLum, &e->dLumCap, (size_t)n * nCol))) {
+        set_error("fb_fv_backward: device allocation failed");
+        return FB_ERR_DEVICE;
+    }
+    hipStream_t st = e->stream;
+    FB_HIP(hipMemcpyAsync(e->dLdV, dLdV, sizeof(float) * (size_t)n * N, hipMemcpyHostToDevice, st));
+    if (E > 0) FB_HIP(hipMemcpyAsync(e->gw, gW, sizeof(float) * E, hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemcpyAsync(e->gb, gBias, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemcpyAsync(e->gk, gKdt, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    if (dLdLum) FB_HIP(hipMemcpyAsync(e->dLum, dLdLum, sizeof(float) * (size_t)n * nCol, hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemsetAsync(e->lv, 0, sizeof(float) * N, st));
+    float *dLum = dLdLum ? e->dLum : nullptr;
+    const int grid = (int)((std::max(N, dLdLum ? nCol : (size_t)0) + 255) / 256);
+    // launch k handles the fan-out of step n-k and the per-node pass of step
+    // n-1-k; step t's lacc lives in lacc[t & 1]
+    for (int32_t t = n - 1; t >= -1; t--) {
+        const int32_t tNext = t + 1 < n ? t + 1 : -1;
+        const float *laccNext = tNext >= 0 ? e->lacc[tNext & 1] : nullptr;
+        float *laccCur = t >= 0 ? e->lacc[t & 1] : nullptr;
+        hipLaunchKernelGGL(k_fv_back, dim3(grid), dim3(256), 0, st, m->N, (int32_t)nCol, tNext, t, m->indptr, m->target,
+                           m->weight, m->kdt, e->tapeV, e->tapeU, e->dLdV, e->lv, laccNext, laccCur, e->gw, e->gb, e->gk,
+                           m->colOff, m->colNode, dLum);
+    }
+    FB_HIP(hipGetLastError());
+    if (E > 0) FB_HIP(hipMemcpyAsync(gW, e->gw, sizeof(float) * E, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(gBias, e->gb, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(gKdt, e->gk, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+    if (dLdLum) FB_HIP(hipMemcpyAsync(dLdLum, e->dLum, sizeof(float) * (size_t)n * nCol, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
     return FB_OK;
 }
 

@@ -101,6 +101,13 @@ _SIGS = {
     "fb_fv_get": (I32, [P, I32, P]),
     "fb_fv_set_v": (I32, [P, P]),
     "fb_fv_reset": (I32, [P]),
+    "fb_fv_model_set_kdt": (I32, [P, P]),
+    "fb_fv_model_set_input_map": (I32, [P, I32, P, P]),
+    "fb_fv_set_lum": (I32, [P, P]),
+    "fb_fv_tape_start": (I32, [P, I32]),
+    "fb_fv_step_taped": (I32, [P, I32]),
+    "fb_fv_tape_len": (I32, [P, P]),
+    "fb_fv_backward": (I32, [P, P, P, P, P, P]),
     "fb_diff_forward": (I32, [P, I32, D, I32, ctypes.POINTER(D)]),
     "fb_diff_backward": (I32, [P, I32, P, P, P, P, P]),
     "fb_diff_type_grads": (I32, [P, P, I32, P]),

@@ -338,6 +338,29 @@ fb_status fb_fv_set_v(fb_fv *e, const float *v);
 /* v = bias (reset(), src/flyvis.js:24) */
 fb_status fb_fv_reset(fb_fv *e);
 
+/* The differentiable flyvis (FlyVisDiff, src/flyvisdiff.js:36-185): the same
+ * step, a tape and the reverse sweep.
+ * kdt given directly instead of from tau (model.kdt, src/flyvisdiff.js:50-53) */
+fb_status fb_fv_model_set_kdt(fb_fv_model *m, const float *kdt);
+/* the photoreceptor map, flattened type by type as src/flyvisdiff.js:60-64
+ * does: input node inNode[q] reads hex column inCol[q]; nCol = max + 1 */
+fb_status fb_fv_model_set_input_map(fb_fv_model *m, int32_t nIn, const int32_t *inNode, const int32_t *inCol);
+/* lum[nCol]: x[inNode[q]] = lum[inCol[q]] in map order (setInput,
+ * src/flyvisdiff.js:75-81) */
+fb_status fb_fv_set_lum(fb_fv *e, const float *lum);
+/* startTape(steps) (src/flyvisdiff.js:104-107): room for `steps` taped steps,
+ * none taped yet */
+fb_status fb_fv_tape_start(fb_fv *e, int32_t steps);
+/* stepTaped() nsteps times (src/flyvisdiff.js:110-114); refuses to tape past
+ * the room fb_fv_tape_start made */
+fb_status fb_fv_step_taped(fb_fv *e, int32_t nsteps);
+fb_status fb_fv_tape_len(fb_fv *e, int32_t *n);
+/* backward(dLdV, g, dLdLum) (src/flyvisdiff.js:128-185) over the n taped
+ * steps: dLdV[n][N] the loss gradient of v after each step; gW[E] (source
+ * order), gBias[N], gKdt[N] accumulators, added to (two eyes share one);
+ * dLdLum[n][nCol] accumulator or NULL.  Host pointers. */
+fb_status fb_fv_backward(fb_fv *e, const float *dLdV, float *gW, float *gBias, float *gKdt, float *dLdLum);
+
 #ifdef __cplusplus
 }
 #endif
