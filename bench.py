@@ -437,6 +437,86 @@ def bench_brain(args, lif, hiprt, stream):
     return out
 
 
+def bench_flyvis(args):
+    """The reference's flyvis optic lobe (45,669 nodes, 1,513,231 edges per
+    eye, two eyes on one model) from its data files: untaped steps, taped
+    steps with a luminance frame each, and the reverse sweep over the tape
+    into one gradient for both eyes (src/visdiff.js:170-238 drives it so).
+    Host-clock times through the C ABI (uploads and copies included; the
+    kernels' own durations are in the kernel trace).  Skipped when the
+    fixture is absent."""
+    import numpy as np
+    vdir = os.path.join(ROOT, "tests", "golden", "_vision")
+    if not os.path.exists(os.path.join(vdir, "flyvis.bin")):
+        return None
+    fv = pkg("flyvis")
+    m = fv.parse_flyvis(vdir)
+    eyes = [fv.FlyVisDiffHip(m)]
+    eyes.append(fv.FlyVisDiffHip(shared=eyes[0].sharedParts))
+    rng = np.random.default_rng(1)
+    nCol = eyes[0].nCol
+    grey = np.full(nCol, 0.5, np.float32)
+    for e in eyes:
+        e.settle(grey, 150)
+    _ = [e.v for e in eyes]
+    n_fwd = 2000
+    t0 = time.perf_counter()
+    for e in eyes:
+        e.step(n_fwd)
+    _ = [e.v for e in eyes]
+    fwd = (time.perf_counter() - t0) / (2 * n_fwd)
+    T = 200
+    lums = rng.uniform(0, 1, (T, nCol)).astype(np.float32)
+    t0 = time.perf_counter()
+    for e in eyes:
+        e.startTape(T)
+        for t in range(T):
+            e.setInput(lums[t])
+            e.stepTaped()
+    _ = [e.v for e in eyes]
+    taped = (time.perf_counter() - t0) / (2 * T)
+    dLdV = []
+    for _ in range(2):
+        d = np.zeros((T, m["N"]), np.float32)
+        d[:, rng.choice(m["N"], 2000, replace=False)] = rng.normal(0, 1, (T, 2000))
+        dLdV.append(d)
+    g = None
+    t0 = time.perf_counter()
+    for k, e in enumerate(eyes):
+        g = e.backward(dLdV[k], g, np.zeros(T * nCol, np.float32))
+    back = (time.perf_counter() - t0) / (2 * T)
+    for e in eyes[::-1]:
+        e.destroy()
+    out = {"workload": "the reference's flyvis optic lobe from its data files: 45,669 nodes, 1,513,231 edges per "
+                       "eye, two eyes on one model; 2,000 untaped steps per eye, then 200 taped steps with a "
+                       "random luminance frame each and the reverse sweep over them (loss on 2,000 nodes per step)",
+           "step_us": fwd * 1e6, "taped_step_us": taped * 1e6, "backward_us_per_step": back * 1e6,
+           "edges_per_s_forward": m["E"] / fwd}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        o = orc.FlyVisDiffOracle(m)
+        o.setInput(grey)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            o.step()
+        cf = (time.perf_counter() - t0) / 20
+        o.start_tape()
+        for t in range(20):
+            o.setInput(lums[t])
+            o.step()
+        t0 = time.perf_counter()
+        o.backward(dLdV[0][:20])
+        cb = (time.perf_counter() - t0) / 20
+        out["cpu_baseline"] = {"value": m["E"] / cf, "unit": "edges/s (forward)", "cores": 1, "kind": "port",
+                               "sample": "one eye, 20 steps forward and the reverse sweep over 20 taped steps, "
+                                         "oracle/lif_oracle.c (fbo_fv_step, fbo_fv_back_step), 1 thread",
+                               "step_us": cf * 1e6, "backward_us_per_step": cb * 1e6}
+        out["gpu_over_cpu_step"] = cf / fwd
+        out["gpu_over_cpu_backward"] = cb / back
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,6 +534,7 @@ def main():
                          "scaling HBM counters), then exit")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
+    ap.add_argument("--only", choices=["flyvis"], help="run one side leg alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -469,6 +550,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return
+    if args.only == "flyvis":
+        print(json.dumps({"flyvis": bench_flyvis(args)}), flush=True)
         return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
@@ -600,6 +684,9 @@ def main():
     brain = bench_brain(args, lif, hiprt, stream)
     if brain:
         out["real_brain"] = brain
+    vis = bench_flyvis(args)
+    if vis:
+        out["flyvis"] = vis
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:

@@ -35,6 +35,11 @@ struct fb_fv_model {
     // the graph by source, for the reverse sweep's fan-out
     int32_t *indptr = nullptr, *target = nullptr;
     float *weight = nullptr;
+    // wave groups: consecutive targets (inGrp) or sources (outGrp) with at
+    // most 64 nodes and FV_CH edges, one wave each (a node with more edges
+    // is a group of its own)
+    int32_t nInGrp = 0, nOutGrp = 0;
+    int32_t *inGrp = nullptr, *outGrp = nullptr;
     // the photoreceptor map: x[xNode[q]] = lum[xCol[q]] (the last entry of a
     // node in map order wins), and per hex column its input nodes in map order
     int32_t nIn = 0, nCol = 0, nX = 0;
@@ -59,20 +64,58 @@ struct fb_fv {
 
 namespace fb {
 
+// A wave owns 64 consecutive targets, whose in-edges are one contiguous
+// range.  It stages that range a chunk at a time: every lane loads and
+// multiplies 16 edges at once (independent loads, so the gathers overlap),
+// the products go to LDS, and each lane then adds its own target's products
+// in order.  A skipped source (v <= 0) stores +0, which leaves a sum that
+// starts at +0 unchanged, so the sums are the reference's.
+constexpr int FV_CH = 1024;            // edges staged per chunk
+constexpr int FV_PER_LANE = FV_CH / 64;
+
 template <bool TAPE>
-__global__ void __launch_bounds__(256) k_fv_step(int32_t N, const int32_t *inOff, const int32_t *inSrc,
-                                                 const float *inW, const float *bias, const float *kdt,
-                                                 const float *vIn, float *vOut, float *acc, const float *x,
-                                                 float *tapeV, float *tapeU) {
-    const int32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
+__global__ void __launch_bounds__(64) k_fv_step(const int32_t *grp, const int32_t *inOff, const int32_t *inSrc,
+                                                const float *inW, const float *bias, const float *kdt,
+                                                const float *vIn, float *vOut, float *acc, const float *x,
+                                                float *tapeV, float *tapeU) {
+    __shared__ float prod[FV_CH];
+    const int32_t lane = threadIdx.x, i0 = grp[blockIdx.x], iEnd = grp[blockIdx.x + 1], i = i0 + lane;
+    const int32_t e0 = inOff[i0], e1 = inOff[iEnd];
+    const int32_t my0 = i < iEnd ? inOff[i] : e1, my1 = i < iEnd ? inOff[i + 1] : e1;
     float a = 0.f;
-    for (int32_t q = inOff[i], e = inOff[i + 1]; q < e; q++) {
-        const float r = vIn[inSrc[q]];
-        if (r <= 0.f) continue;
-        const float w = inW[q] * r;
-        a = a + w;
+    for (int32_t c = e0; c < e1; c += FV_CH) {
+        const int32_t n = min(FV_CH, e1 - c);
+        // loads at clamped indices are unconditional, so all of them issue
+        // before the first wait; only the stores are guarded
+        int32_t src[FV_PER_LANE];
+        float w[FV_PER_LANE], r[FV_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < FV_PER_LANE; u++) {
+            const int32_t k = c + min(u * 64 + lane, n - 1);
+            src[u] = inSrc[k];
+            w[u] = inW[k];
+        }
+#pragma unroll
+        for (int u = 0; u < FV_PER_LANE; u++) r[u] = vIn[src[u]];
+#pragma unroll
+        for (int u = 0; u < FV_PER_LANE; u++) {
+            const int32_t o = u * 64 + lane;
+            if (o < n) prod[o] = r[u] <= 0.f ? 0.f : w[u] * r[u];
+        }
+        __syncthreads();
+        int32_t q = max(my0, c);
+        const int32_t e = min(my1, c + n);
+        for (; q + 8 <= e; q += 8) {        // the reads of eight terms overlap; the adds stay in order
+            float t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = prod[q - c + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) a = a + t[u];
+        }
+        for (; q < e; q++) a = a + prod[q - c];
+        __syncthreads();
     }
+    if (i >= iEnd) return;
     acc[i] = a;
     const float vi = vIn[i];
     const float d = ((-vi + bias[i]) + a) + x[i];
@@ -92,37 +135,91 @@ __global__ void __launch_bounds__(256) k_fv_lum(int32_t nX, const int32_t *xNode
 // One launch of the reverse sweep.  tNext >= 0: the fan-out and luminance
 // gradient of step tNext, from laccNext (that step's lacc, complete since the
 // previous launch).  tCur >= 0: the per-node pass of step tCur into laccCur.
-// Thread j owns lv[j], gb[j], gk[j], the row of source j in gw and, for
-// j < nCol, hex column j of dLum, so no two threads write one word.
-__global__ void __launch_bounds__(256) k_fv_back(int32_t N, int32_t nCol, int32_t tNext, int32_t tCur,
-                                                 const int32_t *indptr, const int32_t *target, const float *weight,
-                                                 const float *kdt, const float *tapeV, const float *tapeU,
-                                                 const float *dLdV, float *lv, const float *laccNext, float *laccCur,
-                                                 float *gw, float *gb, float *gk, const int32_t *colOff,
-                                                 const int32_t *colNode, float *dLum) {
-    const int32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (tNext >= 0 && dLum != nullptr && j < nCol) {
+// A wave owns 64 consecutive sources (one contiguous range of rows): lanes
+// take 16 edges of a staged chunk each, update gw of their own edges (a
+// word per edge, so in parallel) and put la * w in LDS; each lane then adds
+// its own row's terms in row order into a binary64 sum, as the reference's
+// loop does (a skipped term stores +0, which leaves a sum started at +0
+// unchanged).  Lane j also owns lv[j], gb[j], gk[j] and, in the first
+// blocks, hex column j of dLum, so no two lanes write one word.
+__global__ void __launch_bounds__(64) k_fv_back(int32_t N, int32_t nCol, int32_t nGrp, const int32_t *grp,
+                                                int32_t tNext, int32_t tCur, const int32_t *indptr, const int32_t *target, const float *weight,
+                                                const float *kdt, const float *tapeV, const float *tapeU,
+                                                const float *dLdV, float *lv, const float *laccNext, float *laccCur,
+                                                float *gw, float *gb, float *gk, const int32_t *colOff,
+                                                const int32_t *colNode, float *dLum) {
+    __shared__ double prod[FV_CH];
+    __shared__ float rS[64];
+    __shared__ int32_t rowOff[65];
+    const int32_t lane = threadIdx.x, col = blockIdx.x * 64 + lane;
+    if (tNext >= 0 && dLum != nullptr && col < nCol) {
         float *row = dLum + (size_t)tNext * nCol;
-        float a = row[j];
-        for (int32_t q = colOff[j], e = colOff[j + 1]; q < e; q++)
+        float a = row[col];
+        for (int32_t q = colOff[col], e = colOff[col + 1]; q < e; q++)
             a = (float)((double)a + (double)laccNext[colNode[q]]);
-        row[j] = a;
+        row[col] = a;
     }
-    if (j >= N) return;
-    float lvj = lv[j];
+    if ((int32_t)blockIdx.x >= nGrp) return;   // the whole block: no barrier is skipped by part of it
+    const int32_t j0 = grp[blockIdx.x], jEnd = grp[blockIdx.x + 1], j = j0 + lane;
+    float lvj = j < jEnd ? lv[j] : 0.f;
     if (tNext >= 0) {
-        const double r = tapeV[(size_t)tNext * N + j];
-        if (r > 0.0) {
-            double s = 0.0;
-            for (int32_t k = indptr[j], e = indptr[j + 1]; k < e; k++) {
-                const double la = laccNext[target[k]];
-                if (la == 0.0) continue;
-                s += la * (double)weight[k];
-                gw[k] = (float)((double)gw[k] + la * r);
+        const float r = j < jEnd ? tapeV[(size_t)tNext * N + j] : 0.f;
+        rS[lane] = r;
+        rowOff[lane] = indptr[min(j, jEnd)];
+        if (lane == 0) rowOff[64] = indptr[jEnd];
+        __syncthreads();
+        const int32_t e0 = rowOff[0], e1 = rowOff[64], my0 = rowOff[lane], my1 = rowOff[lane + 1];
+        double sum = 0.0;
+        for (int32_t c = e0; c < e1; c += FV_CH) {
+            const int32_t n = min(FV_CH, e1 - c);
+            // unconditional loads at clamped indices, then the gathers, then
+            // the guarded stores (see k_fv_step)
+            int32_t tg[FV_PER_LANE];
+            float w[FV_PER_LANE], g[FV_PER_LANE], rr[FV_PER_LANE], la[FV_PER_LANE];
+#pragma unroll
+            for (int u = 0; u < FV_PER_LANE; u++) {
+                const int32_t k = c + min(u * 64 + lane, n - 1);
+                tg[u] = target[k];
+                w[u] = weight[k];
+                g[u] = gw[k];
+                int32_t lo = 0, hi = 64;        // the source of edge k: rowOff[lo] <= k < rowOff[lo + 1]
+                while (hi - lo > 1) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (rowOff[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                rr[u] = rS[lo];
             }
-            lvj = (float)((double)lvj + s);
+#pragma unroll
+            for (int u = 0; u < FV_PER_LANE; u++) la[u] = laccNext[tg[u]];
+#pragma unroll
+            for (int u = 0; u < FV_PER_LANE; u++) {
+                const int32_t o = u * 64 + lane;
+                if (o < n) {
+                    double p = 0.0;
+                    if (!(rr[u] <= 0.f) && la[u] != 0.f) {
+                        p = (double)la[u] * (double)w[u];
+                        gw[c + o] = (float)((double)g[u] + (double)la[u] * (double)rr[u]);
+                    }
+                    prod[o] = p;
+                }
+            }
+            __syncthreads();
+            int32_t q = max(my0, c);
+            const int32_t e = min(my1, c + n);
+            for (; q + 8 <= e; q += 8) {
+                double t[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) t[u] = prod[q - c + u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) sum += t[u];
+            }
+            for (; q < e; q++) sum += prod[q - c];
+            __syncthreads();
         }
+        if (j < jEnd && !(r <= 0.f)) lvj = (float)((double)lvj + sum);
     }
+    if (j >= jEnd) return;
     if (tCur >= 0) {
         const size_t o = (size_t)tCur * N + j;
         const double l = (double)lvj + (double)dLdV[o];
@@ -152,10 +249,24 @@ using namespace fb;
         }                                \
     } while (0)
 
+// consecutive nodes in groups of at most 64 nodes and FV_CH edges (off:
+// per-node edge offsets, N + 1 of them)
+static std::vector<int32_t> fv_groups(const std::vector<int32_t> &off, int32_t N) {
+    std::vector<int32_t> g{0};
+    for (int32_t i = 0; i < N;) {
+        int32_t j = i + 1;
+        while (j < N && j - i < 64 && off[(size_t)j + 1] - off[(size_t)i] <= FV_CH) j++;
+        g.push_back(j);
+        i = j;
+    }
+    return g;
+}
+
 static void fv_model_free(fb_fv_model *m) {
     (void)hipFree(m->inOff); (void)hipFree(m->inSrc); (void)hipFree(m->inW);
     (void)hipFree(m->bias); (void)hipFree(m->kdt);
     (void)hipFree(m->indptr); (void)hipFree(m->target); (void)hipFree(m->weight);
+    (void)hipFree(m->inGrp); (void)hipFree(m->outGrp);
     (void)hipFree(m->xNode); (void)hipFree(m->xCol); (void)hipFree(m->colOff); (void)hipFree(m->colNode);
     delete m;
 }
@@ -188,9 +299,12 @@ extern "C" fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *ind
     // kdt = dt / max(tau, dt) in binary64, stored as binary32 (src/flyvis.js:13)
     std::vector<float> kdt((size_t)N);
     for (int32_t i = 0; i < N; i++) kdt[(size_t)i] = (float)(dt / std::max((double)tau[i], dt));
+    const std::vector<int32_t> inGrp = fv_groups(off, N), outGrp = fv_groups(std::vector<int32_t>(indptr, indptr + N + 1), N);
     fb_fv_model *m = new fb_fv_model();
     m->N = N;
     m->E = E;
+    m->nInGrp = (int32_t)inGrp.size() - 1;
+    m->nOutGrp = (int32_t)outGrp.size() - 1;
     m->biasH.assign(bias, bias + N);
     if (hipMalloc(&m->inOff, sizeof(int32_t) * ((size_t)N + 1)) != hipSuccess ||
         hipMalloc(&m->inSrc, sizeof(int32_t) * src.size()) != hipSuccess ||
@@ -198,7 +312,9 @@ extern "C" fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *ind
         hipMalloc(&m->bias, sizeof(float) * N) != hipSuccess || hipMalloc(&m->kdt, sizeof(float) * N) != hipSuccess ||
         hipMalloc(&m->indptr, sizeof(int32_t) * ((size_t)N + 1)) != hipSuccess ||
         hipMalloc(&m->target, sizeof(int32_t) * src.size()) != hipSuccess ||
-        hipMalloc(&m->weight, sizeof(float) * w.size()) != hipSuccess) {
+        hipMalloc(&m->weight, sizeof(float) * w.size()) != hipSuccess ||
+        hipMalloc(&m->inGrp, sizeof(int32_t) * inGrp.size()) != hipSuccess ||
+        hipMalloc(&m->outGrp, sizeof(int32_t) * outGrp.size()) != hipSuccess) {
         fv_model_free(m);
         set_error("fb_fv_model_create: device allocation failed");
         return FB_ERR_DEVICE;
@@ -209,6 +325,8 @@ extern "C" fb_status fb_fv_model_create(int32_t N, int64_t E, const int32_t *ind
     FB_HIP(hipMemcpy(m->bias, bias, sizeof(float) * N, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(m->kdt, kdt.data(), sizeof(float) * N, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(m->indptr, indptr, sizeof(int32_t) * ((size_t)N + 1), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->inGrp, inGrp.data(), sizeof(int32_t) * inGrp.size(), hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(m->outGrp, outGrp.data(), sizeof(int32_t) * outGrp.size(), hipMemcpyHostToDevice));
     if (E > 0) {
         FB_HIP(hipMemcpy(m->target, target, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
         FB_HIP(hipMemcpy(m->weight, weight, sizeof(float) * (size_t)E, hipMemcpyHostToDevice));
@@ -315,16 +433,16 @@ extern "C" fb_status fb_fv_set_input(fb_fv *e, const float *x) {
 
 static fb_status fv_run(fb_fv *e, int32_t nsteps, bool tape) {
     const fb_fv_model *m = e->m;
-    const int grid = (m->N + 255) / 256;
+    const int grid = m->nInGrp;
     const size_t N = (size_t)m->N;
     for (int32_t s = 0; s < nsteps; s++) {
         if (tape) {
-            hipLaunchKernelGGL(k_fv_step<true>, dim3(grid), dim3(256), 0, e->stream, m->N, m->inOff, m->inSrc, m->inW,
+            hipLaunchKernelGGL(k_fv_step<true>, dim3(grid), dim3(64), 0, e->stream, m->inGrp, m->inOff, m->inSrc, m->inW,
                                m->bias, m->kdt, e->v[e->cur], e->v[e->cur ^ 1], e->acc, e->x,
                                e->tapeV + (size_t)e->tapeN * N, e->tapeU + (size_t)e->tapeN * N);
             e->tapeN++;
         } else {
-            hipLaunchKernelGGL(k_fv_step<false>, dim3(grid), dim3(256), 0, e->stream, m->N, m->inOff, m->inSrc,
+            hipLaunchKernelGGL(k_fv_step<false>, dim3(grid), dim3(64), 0, e->stream, m->inGrp, m->inOff, m->inSrc,
                                m->inW, m->bias, m->kdt, e->v[e->cur], e->v[e->cur ^ 1], e->acc, e->x, nullptr,
                                nullptr);
         }
@@ -430,14 +548,15 @@ extern "C" fb_status fb_fv_backward(fb_fv *e, const float *dLdV, float *gW, floa
     if (dLdLum) FB_HIP(hipMemcpyAsync(e->dLum, dLdLum, sizeof(float) * (size_t)n * nCol, hipMemcpyHostToDevice, st));
     FB_HIP(hipMemsetAsync(e->lv, 0, sizeof(float) * N, st));
     float *dLum = dLdLum ? e->dLum : nullptr;
-    const int grid = (int)((std::max(N, dLdLum ? nCol : (size_t)0) + 255) / 256);
+    const int grid = std::max(m->nOutGrp, dLdLum ? (int)((nCol + 63) / 64) : 0);
     // launch k handles the fan-out of step n-k and the per-node pass of step
     // n-1-k; step t's lacc lives in lacc[t & 1]
     for (int32_t t = n - 1; t >= -1; t--) {
         const int32_t tNext = t + 1 < n ? t + 1 : -1;
         const float *laccNext = tNext >= 0 ? e->lacc[tNext & 1] : nullptr;
         float *laccCur = t >= 0 ? e->lacc[t & 1] : nullptr;
-        hipLaunchKernelGGL(k_fv_back, dim3(grid), dim3(256), 0, st, m->N, (int32_t)nCol, tNext, t, m->indptr, m->target,
+        hipLaunchKernelGGL(k_fv_back, dim3(grid), dim3(64), 0, st, m->N, (int32_t)nCol, m->nOutGrp, m->outGrp, tNext, t,
+                           m->indptr, m->target,
                            m->weight, m->kdt, e->tapeV, e->tapeU, e->dLdV, e->lv, laccNext, laccCur, e->gw, e->gb, e->gk,
                            m->colOff, m->colNode, dLum);
     }

@@ -84,6 +84,27 @@ def test_random_model_gradients_match_oracle(seed):
     eye1.destroy()
 
 
+def test_hub_nodes_span_several_chunks():
+    """A node with 3,000 in-edges and one with 3,000 out-edges: a wave group
+    of one node whose edges are staged in several chunks, forward and
+    backward."""
+    m = small_model(seed=12, N=1500, nCol=30)
+    rng = np.random.default_rng(12)
+    src = np.repeat(np.arange(m["N"]), np.diff(m["indptr"]))
+    src = np.concatenate([src, rng.integers(0, m["N"], 3000), np.full(3000, 11)])
+    tgt = np.concatenate([m["target"], np.full(3000, 7), rng.integers(0, m["N"], 3000)])
+    w = np.concatenate([m["weight"], rng.normal(0, 0.01, 6000).astype(np.float32)])
+    o = np.argsort(src, kind="stable")
+    m["target"], m["weight"] = tgt[o].astype(np.int32), w[o].astype(np.float32)
+    m["indptr"] = np.concatenate([[0], np.cumsum(np.bincount(src, minlength=m["N"]))]).astype(np.int32)
+    m["E"] = int(m["indptr"][-1])
+    m["bias"][11] = 0.8                       # the fan-out hub is active
+    print("flyvis adjoint: hub nodes", flush=True)
+    eye0, eye1 = _case(m, T=10, n_loss=150, seed=12)
+    eye0.destroy()
+    eye1.destroy()
+
+
 def test_kdt_given_and_tape_bounds():
     """model.kdt replaces dt / max(tau, dt) (src/flyvisdiff.js:50-53); a start
     state other than rest; taping past the tape's room is refused."""
