@@ -1344,7 +1344,9 @@ static fb_status fwd_prepare(fb_diff *f, int32_t steps, double seed, int32_t rec
             for (int32_t k = 0; k < d.nDI; k++) {
                 const int32_t i = f->drvIdx[k];
                 const float rate = f->drvSched[(size_t)ee * d.nDI + k];
-                if (rate > 0) {
+                // -0.0: a positive rate too small for binary32, which joins
+                // the set with a zero drive
+                if (rate > 0 || (rate == 0.f && std::signbit(rate))) {
                     if (!in[i]) { in[i] = 1; cur.push_back(i); }
                 } else if (in[i]) {
                     in[i] = 0;

@@ -255,7 +255,8 @@ fb_status fb_diff_set_tape_capacity(fb_diff *d, int64_t perStep);
  * neurons idx[0..nD) is an input.  At the start of epoch e (every epochSteps
  * steps) their rates become sched[e*nD + k], through setDriveOne's semantics:
  * a rate of 0 takes the neuron out of the driven set, a positive rate puts it
- * back at the end (the onEpoch hook of LIFDiff.forward).  The schedule must
+ * back at the end (the onEpoch hook of LIFDiff.forward); -0.0 stands for a
+ * positive rate too small for binary32, which joins the set with a zero drive.  The schedule must
  * cover the forward pass.  driveSoft (src/lifdiff.js:338): a driven neuron
  * spikes with the drive's expectation sd + (1 - sd) st instead of a Poisson
  * draw.  nD = 0 clears. */

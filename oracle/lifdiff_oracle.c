@@ -219,9 +219,12 @@ void fdo_drive_grad(const fdo_net *n, float *gDrive) {
 
 /* setDriveOne on the run's working drive and driven set (src/lifdiff.js:205-207):
  * an added neuron goes to the end, a removed one leaves its place */
+/* setDriveOne (src/lifdiff.js:207): the neuron joins the driven set when
+ * the rate is positive.  A rate of -0.0 stands for a positive binary64 rate
+ * that the binary32 drive array holds as zero: it joins with a zero drive. */
 static void drive_set_one(fdo_net *n, int32_t i, float rate) {
     n->wdrive[i] = rate;
-    if (rate > 0) {
+    if (rate > 0 || (rate == 0 && signbit(rate))) {
         if (!n->inW[i]) { n->inW[i] = 1; n->wdrv[n->nW++] = i; }
     } else if (n->inW[i]) {
         n->inW[i] = 0;

@@ -52,8 +52,10 @@ void fdo_set_loss(fdo_net *n, const float *dLdSpike, int32_t lossFrom);
 void fdo_set_readout(fdo_net *n, const int32_t *idx, int32_t R, const float *targetHz, double tauMs);
 /* driveIdx / driveEpoch: at the start of epoch e (every epochSteps steps) the
  * rate of idx[k] becomes sched[e*nD + k] through setDriveOne's semantics (the
- * last epoch's rates hold beyond nEp).  driveSoft: the drive spikes as its
- * expectation.  NULL idx clears the schedule. */
+ * last epoch's rates hold beyond nEp).  A rate of -0.0 stands for a positive
+ * rate that binary32 holds as zero (it joins the driven set with a zero
+ * drive).  driveSoft: the drive spikes as its expectation.  NULL idx clears
+ * the schedule. */
 void fdo_set_drive_schedule(fdo_net *n, const int32_t *idx, int32_t nD, int32_t epochSteps, const float *sched,
                             int32_t nEp, int32_t driveSoft);
 /* dLdSpikePerEpoch: the loss weight of neuron i's spikes in epoch e is

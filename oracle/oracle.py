@@ -623,3 +623,177 @@ class FlyVisDiffOracle:
                        _ptr(g["kdt"]), None if row is None else _ptr(row), _ptr(self.inNode), _ptr(self.inCol),
                        ctypes.c_int32(self.inNode.size))
         return g
+
+
+class VisualChainOracle:
+    """VisualChain (src/visdiff.js:70-258) on the CPU checker: two
+    FlyVisDiffOracle eyes, the coupling restated pair by pair in binary64 with
+    the reference's binary32 stores, and OracleDiff as the CNS.  The rates
+    every epoch start sets depend on the optic lobe alone, so they are
+    computed eye step by eye step first and given to the CNS as its drive
+    schedule -- the sequence of setDriveOne calls the onEpoch hook makes.
+
+    ``cns``: {N, indptr, indices, counts, sign} (counts before the gate);
+    ``params``: LIFDiff's options, complete; ``coupling``: gain, dead, cap and
+    "coupling" ("hard" | "soft")."""
+
+    GATE_K, CAP_KNEE = 8000.0, 0.05            # src/visdiff.js:68
+
+    def __init__(self, cns, fv_model, fv_map, params, coupling=None, fvDt=None, driveSoft=False,
+                 sensoryMask=None):
+        import math
+        self.c = {"gain": 250.0, "dead": 0.02, "cap": 200.0, "coupling": "hard", **(coupling or {})}
+        self.eyes = [FlyVisDiffOracle(fv_model, kdt=fv_model.get("kdt")),
+                     FlyVisDiffOracle(fv_model, kdt=fv_model.get("kdt"))]
+        self.fvN, self.fvE, self.nCol = self.eyes[0].N, self.eyes[0].E, self.eyes[0].nCol
+        self.dt_fv = float(fv_model.get("dt") or 0.02)
+        self.pairs = [[(int(q[0]), int(q[1])) for q in fv_map["eyes"][sd]["pairs"]] for sd in ("L", "R")]
+        idx = [n for P in self.pairs for (n, _) in P]
+        if len(set(idx)) != len(idx):
+            raise ValueError("a CNS neuron is driven by both eyes")
+        self.driveIdx = np.asarray(idx, dtype=np.int32)
+        self.nD = len(idx)
+        self.slot = {n: k for k, n in enumerate(idx)}
+        N = int(cns["N"])
+        sensory = np.zeros(N, np.uint8) if sensoryMask is None else np.asarray(sensoryMask, np.uint8).copy()
+        sensory[self.driveIdx] = 1              # src/visdiff.js:107-109
+        counts = np.asarray(cns["counts"], np.float32).copy()
+        ind = np.asarray(cns["indices"])
+        for j in range(counts.size):            # LIFDiff's gate (src/lifdiff.js:167-172)
+            if counts[j] < params["minSyn"] or sensory[ind[j]]:
+                counts[j] = 0.0
+        self.net = OracleDiff(N, cns["indptr"], cns["indices"], counts, cns["sign"], params)
+        fv_ms = fvDt if fvDt is not None else self.dt_fv * 1000.0
+        self.epoch = max(1, int(math.floor(fv_ms / params["dt"] + 0.5)))   # Math.round
+        self.driveSoft = bool(driveSoft)
+        self.vRest = None
+        self.eyeV0 = None
+
+    # the coupling (src/visdiff.js:131-159), one deviation at a time
+    def _gate(self, a):
+        """1 / (1 + exp(-K (a - dead))), which is 0 once exp overflows to
+        Infinity (past ~709.78), as in JavaScript"""
+        import math
+        try:
+            e = math.exp(-self.GATE_K * (a - self.c["dead"]))
+        except OverflowError:
+            return 0.0
+        return 1.0 / (1.0 + e)
+
+    def _rate(self, a):
+        import math
+        c = self.c
+        if c["coupling"] != "soft":
+            if a <= c["dead"]:
+                return 0.0
+            return min(c["cap"], c["gain"] * a)
+        gate = self._gate(a)
+        u = c["cap"] - c["gain"] * a
+        inner = c["cap"] - 0.5 * (u + math.sqrt(u * u + self.CAP_KNEE ** 2))
+        return max(0.0, inner * gate)
+
+    def _drate(self, a):
+        import math
+        c = self.c
+        if c["coupling"] != "soft":
+            return 0.0 if (a <= c["dead"] or c["gain"] * a >= c["cap"]) else c["gain"]
+        u = c["cap"] - c["gain"] * a
+        inner = c["cap"] - 0.5 * (u + math.sqrt(u * u + self.CAP_KNEE ** 2))
+        d_inner = 0.5 * (1 + u / math.sqrt(u * u + self.CAP_KNEE ** 2))
+        gate = self._gate(a)
+        return c["gain"] * d_inner * gate + inner * gate * (1 - gate) * self.GATE_K
+
+    def _drate_gain(self, a):
+        import math
+        c = self.c
+        if c["coupling"] != "soft":
+            return 0.0 if (a <= c["dead"] or c["gain"] * a >= c["cap"]) else a
+        u = c["cap"] - c["gain"] * a
+        d_inner = 0.5 * (1 + u / math.sqrt(u * u + self.CAP_KNEE ** 2))
+        gate = self._gate(a)
+        return a * d_inner * gate
+
+    def settle(self, level=0.5, steps=150):
+        grey = np.full(self.nCol, level, np.float32)
+        for e in self.eyes:
+            e.tape = None
+            e.reset()
+            e.setInput(grey)
+            for _ in range(steps):
+                e.step()
+        self.vRest = self.eyes[0].v.copy()
+        self.eyeV0 = [e.v.copy() for e in self.eyes]
+        return self.vRest
+
+    def forward(self, steps, seed=1, lum=None, record=True, t0=0.0):
+        if self.vRest is None:
+            self.settle()
+        if self.eyeV0 is not None:
+            for s in range(2):
+                self.eyes[s].v = self.eyeV0[s].copy()
+        epochs = -(-int(steps) // self.epoch)
+        sched = np.zeros((epochs, self.nD), np.float32)
+        aAt = np.zeros((epochs, self.nD), np.float32)
+        for e in self.eyes:
+            if record:
+                e.start_tape()
+            else:
+                e.tape = None
+        for ep in range(epochs):
+            t = t0 + ep * self.dt_fv
+            for s in range(2):
+                eye = self.eyes[s]
+                eye.setInput(lum(s, t))
+                eye.step()
+                for (n, node) in self.pairs[s]:
+                    a = float(eye.v[node]) - float(self.vRest[node])
+                    r = self._rate(a)
+                    r32 = np.float32(r)
+                    if r > 0 and r32 == 0:          # in the driven set with a zero drive
+                        r32 = np.float32(-0.0)
+                    sched[ep, self.slot[n]] = r32
+                    aAt[ep, self.slot[n]] = a
+        self.net.set_drive_schedule(self.driveIdx, self.epoch, sched, drive_soft=self.driveSoft)
+        L = self.net.forward(int(steps), seed=seed, record=record)
+        self.sched, self.aAt, self.epochs = sched, aAt, epochs
+        return L
+
+    def coupling_backward(self, gDrive):
+        """src/visdiff.js:211-233, pair by pair."""
+        c = self.c
+        gDrive = np.asarray(gDrive, np.float32).reshape(self.epochs, self.nD)
+        dLdV = [np.zeros((self.epochs, self.fvN), np.float32) for _ in range(2)]
+        gGain, live, dcount = 0.0, 0, 0
+        for e in range(self.epochs):
+            for s in range(2):
+                for (n, node) in self.pairs[s]:
+                    k = self.slot[n]
+                    a = float(self.aAt[e, k])
+                    dcount += 1
+                    dr = self._drate(a)
+                    if dr == 0:
+                        continue
+                    if a > c["dead"] and c["gain"] * a < c["cap"]:
+                        live += 1
+                    dR = float(gDrive[e, k])
+                    if dR == 0:
+                        continue
+                    gGain += dR * self._drate_gain(a)
+                    dLdV[s][e, node] = np.float32(float(dLdV[s][e, node]) + dR * dr)
+        return dLdV, gGain, (live / dcount if dcount else 0.0)
+
+    def backward(self, dLdSpike, truncate=0, lum=False):
+        self.net.set_loss(dLdSpike, 0)
+        cns = self.net.backward(truncate)
+        gDrive = self.net.drive_grad()
+        dLdV, gGain, live = self.coupling_backward(gDrive)
+        g = {"weight": np.zeros(self.fvE, np.float32), "bias": np.zeros(self.fvN, np.float32),
+             "kdt": np.zeros(self.fvN, np.float32)}
+        gLum = [np.zeros((self.epochs, self.nCol), np.float32) for _ in range(2)] if lum else [None, None]
+        for s in range(2):
+            self.eyes[s].backward(dLdV[s], g, gLum[s])
+        out = {"params": cns["params"], "logGain": cns["logGain"], "gain": gGain, "fv": g, "live": live,
+               "drive": gDrive}
+        if lum:
+            out["lum"] = gLum
+        return out
