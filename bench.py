@@ -517,6 +517,73 @@ def bench_flyvis(args):
     return out
 
 
+def bench_visual(args):
+    """The optic lobe and the whole CNS as one differentiable model
+    (src/visdiff.js, as scripts/vis_fit.mjs runs it): both eyes of the
+    reference's flyvis model, its 62,157 eye-to-CNS pairs, the calibrated
+    brain as the CNS (hard threshold, olfactory receptors at 6 Hz), 400 steps
+    of grey then a looming disc, the loss on spikes after onset; forward and
+    the adjoint through the CNS, the coupling and both eyes.  Skipped when the
+    fixtures are absent."""
+    import math
+    import numpy as np
+    vdir = os.path.join(ROOT, "tests", "golden", "_vision")
+    bpath = os.path.join(ROOT, "tests", "golden", "_brain", "calibrated_brain.npz")
+    if not (os.path.exists(os.path.join(vdir, "flyvis_map.json")) and os.path.exists(bpath)):
+        return None
+    fv, vis = pkg("flyvis"), pkg("visual")
+    m = fv.parse_flyvis(vdir)
+    with open(os.path.join(vdir, "flyvis_map.json")) as f:
+        fmap = json.load(f)
+    z = np.load(bpath, allow_pickle=False)
+    b = pkg("connectome").load_compact(bpath)
+    p = dict(b["params"])
+    cns = {"N": b["N"], "indptr": b["indptr"], "indices": b["indices"], "counts": z["counts"],
+           "sign": z["pre_sign"]}
+    steps, pre = 1200, 400
+    dirs = [np.asarray(fmap["eyes"][sd]["dirs"], np.float64) for sd in ("L", "R")]
+    c = np.array([math.cos(0.17) * math.cos(0.7), math.cos(0.17) * math.sin(0.7), math.sin(0.17)])
+
+    def loom(eye, t_epoch):
+        t = (round(t_epoch / 0.02) * 40 - pre) / 2000.0
+        if t < 0:
+            return np.full(721, 0.5, np.float32)
+        rad = (5 + 70 * max(0.0, min(1.0, t / 0.4)) ** 2) * math.pi / 180
+        ang = np.arccos(np.minimum(1.0, dirs[eye] @ c))
+        return np.where(ang < rad, 0.05, 0.5).astype(np.float32)
+    t0 = time.perf_counter()
+    ch = vis.VisualChainHip(cns, m, fmap, {**p, "soft": False, "surrogateBeta": 5.0, "sensoryMask": z["sensory"]})
+    ch.net.set_input("outScale", np.ones(b["N"], np.float32))
+    drive = np.zeros(b["N"], np.float32)
+    drive[b["orn"]] = 6.0
+    ch.net.set_input("drive", drive)
+    _, first = np.unique(b["orn"], return_index=True)         # setDrive's set: first occurrence keeps its place
+    ch.net.set_driven(b["orn"][np.sort(first)])
+    target = np.concatenate([b[k] for k in ("type:MN9", "type:GNG232", "type:DNge080")]).astype(np.int64)
+    dL = np.zeros(b["N"], np.float32)
+    dL[target] = 1.0
+    ch.net.set_loss(dL, pre)
+    ch.settle()
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L = ch.forward(steps, seed=1, lum=loom)
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = ch.backward(None, lum=True)
+    back = time.perf_counter() - t0
+    st = ch.stats()
+    out = {"workload": "the visual chain at full scale: both eyes of the flyvis model (45,669 nodes each), 62,157 "
+                       "eye-to-CNS pairs, the calibrated 165,122-neuron brain as the CNS (hard threshold), 1,200 "
+                       "steps (400 of grey, then a looming disc), loss on three cell types' spikes after onset",
+           "build_and_settle_s": build_s, "forward_ms": fwd * 1e3, "backward_ms": back * 1e3,
+           "ms_per_step_forward": fwd * 1e3 / steps, "ms_per_step_backward": back * 1e3 / steps,
+           "loss": L, "live": st["live"], "below_deadband": st["belowDeadband"], "mean_rate_hz": st["meanRate"],
+           "grad_gain": g["gain"], "nonzero_fv_weight_grads": int(np.count_nonzero(g["fv"]["weight"])),
+           "clamped": [int(x) for x in g["clamped"]]}
+    ch.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -534,7 +601,7 @@ def main():
                          "scaling HBM counters), then exit")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
-    ap.add_argument("--only", choices=["flyvis"], help="run one side leg alone and print its JSON")
+    ap.add_argument("--only", choices=["flyvis", "visual"], help="run one side leg alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -553,6 +620,9 @@ def main():
         return
     if args.only == "flyvis":
         print(json.dumps({"flyvis": bench_flyvis(args)}), flush=True)
+        return
+    if args.only == "visual":
+        print(json.dumps({"visual_chain": bench_visual(args)}), flush=True)
         return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
@@ -687,6 +757,9 @@ def main():
     vis = bench_flyvis(args)
     if vis:
         out["flyvis"] = vis
+    chain = bench_visual(args)
+    if chain:
+        out["visual_chain"] = chain
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:

@@ -235,7 +235,8 @@ class VisualChainHip:
     def backward(self, dLdSpike, truncate: int = 0, lum: bool = False) -> dict:
         """backward (src/visdiff.js:207-241): {params, logGain, gain, fv:
         {weight, bias, kdt}, lum (per eye, [epochs][nCol]) when asked, live,
-        clamped}."""
+        clamped}.  ``dLdSpike`` None keeps the loss already set on ``net``
+        (with its lossFrom)."""
         cns = self.net.backward(dLdSpike, truncate=truncate)
         gDrive = self.net.drive_grad()
         dLdV, gGain, live = self.coupling_backward(gDrive)

@@ -1,6 +1,8 @@
 """Copies the reference's flyvis model files (public/vision/flyvis.bin,
-flyvis.json, flyvis_inputs.json) into tests/golden/_vision/ for the GPU test
-of the optic-lobe step, which cannot read the reference at run time.
+flyvis.json, flyvis_inputs.json) and the eye-to-CNS pair map
+(flyvis_map.json) into tests/golden/_vision/ for the GPU tests and the bench
+of the optic lobe and the visual chain, which cannot read the reference at
+run time.
 
 Run where the reference's data exist (`__graft_entry__.build()` does it):
     python tests/golden/make_vision_fixture.py [/root/reference/public/vision]
@@ -12,7 +14,7 @@ import shutil
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FILES = ("flyvis.bin", "flyvis.json", "flyvis_inputs.json")
+FILES = ("flyvis.bin", "flyvis.json", "flyvis_inputs.json", "flyvis_map.json")
 
 
 def main(src="/root/reference/public/vision"):
