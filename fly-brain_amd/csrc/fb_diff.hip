@@ -76,6 +76,7 @@ struct DiffDev {
     DiffColl *coll;
     int64_t collHalf;
     int32_t *ncoll;                   // [2], by step parity
+    int32_t *longList, *nLong;        // accumulators past DIFF_LONG arrivals this step, and their count
     // tape
     int64_t *off;                     // [steps + 1]
     unsigned long long *top;
@@ -172,6 +173,70 @@ __device__ inline float ordered_sum_d(float g, int32_t count, uint64_t key0, dou
     return g;
 }
 
+// An accumulator with more arrivals than DIFF_LONG in one step is summed by
+// dx_long before the membrane pass: its arrivals are staged in LDS (a walk of
+// its chain), sorted by key with a bitonic network and added in that order --
+// linear in the arrivals where the selection above is quadratic.
+constexpr int DIFF_LONG = 32, DIFF_LONG_CAP = 2048, DIFF_LONG_WG = 128;
+
+__device__ __forceinline__ void dx_long(const DiffDev &d) {
+    __shared__ uint64_t ks[DIFF_LONG_CAP];
+    __shared__ double as[DIFF_LONG_CAP];
+    __shared__ int32_t s_m;
+    const int32_t N = d.N, nL = *d.nLong, tid = threadIdx.x;
+    for (int32_t p = blockIdx.x; p < nL; p += gridDim.x) {
+        const int32_t ci = d.longList[p];
+        const int32_t n = d.cnt[ci];
+        float *gp = ci < N ? d.gE + ci : d.gI + (ci - N);
+        if (n > DIFF_LONG_CAP) {                 // beyond the stage: the selection, one thread
+            if (tid == 0) *gp = ordered_sum_d(*gp, n, d.key[ci], d.add[ci], d.chain[ci], d.coll);
+        } else {
+            if (tid == 0) {
+                int32_t m = 0;
+                ks[m] = d.key[ci];
+                as[m] = d.add[ci];
+                m++;
+                for (int32_t q = d.chain[ci]; q >= 0 && m < n; q = d.coll[q].next) {
+                    ks[m] = d.coll[q].key;
+                    as[m] = d.coll[q].addend;
+                    m++;
+                }
+                s_m = m;                          // fewer than n only after a flagged pool overflow
+            }
+            __syncthreads();
+            const int32_t m = s_m;
+            int32_t P = 1;
+            while (P < m) P <<= 1;
+            for (int32_t q = m + tid; q < P; q += blockDim.x) ks[q] = ~0ull;
+            __syncthreads();
+            for (int32_t k = 2; k <= P; k <<= 1)
+                for (int32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (int32_t q = tid; q < P; q += blockDim.x) {
+                        const int32_t o = q ^ j;
+                        if (o > q) {
+                            const bool up = (q & k) == 0;
+                            if ((ks[q] > ks[o]) == up) {
+                                const uint64_t tk = ks[q]; ks[q] = ks[o]; ks[o] = tk;
+                                const double ta = as[q]; as[q] = as[o]; as[o] = ta;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            if (tid == 0) {
+                float g = *gp;
+                for (int32_t q = 0; q < m; q++) g = (float)((double)g + as[q]);
+                *gp = g;
+            }
+        }
+        if (tid == 0) {
+            d.cnt[ci] = 0;
+            d.chain[ci] = -1;
+        }
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ float clip_to(double x, double c) {
     if (c > 0) x = x > c ? c : (x < -c ? -c : x);
     return (float)x;
@@ -225,6 +290,7 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
             const int32_t ci = cls + (int32_t)q;
             const uint64_t kk = ((uint64_t)rank << 32) | (uint64_t)(j - ra);
             const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+            if (slot == DIFF_LONG) d.longList[atomicAdd(d.nLong, 1)] = ci;   // once per accumulator a step
             if (slot == 0) {
                 d.add[ci] = addend;
                 d.key[ci] = kk;
@@ -498,8 +564,12 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
             }
         }
     }
-    // the other parity's collision pool was consumed by the previous step
-    if (i == 0) d.ncoll[(t + 1) & 1] = 0;
+    // the other parity's collision pool was consumed by the previous step;
+    // dx_long has consumed this step's long list
+    if (i == 0) {
+        d.ncoll[(t + 1) & 1] = 0;
+        *d.nLong = 0;
+    }
 }
 
 __global__ void kd_reset(DiffDev d, float vRestF) {
@@ -510,7 +580,7 @@ __global__ void kd_reset(DiffDev d, float vRestF) {
         d.cnt[i] = 0; d.cnt[i + N] = 0; d.chain[i] = -1; d.chain[i + N] = -1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        d.ncoll[0] = 0; d.ncoll[1] = 0; *d.top = 0ull; d.off[0] = 0; *d.err = 0;
+        d.ncoll[0] = 0; d.ncoll[1] = 0; *d.top = 0ull; d.off[0] = 0; *d.err = 0; *d.nLong = 0;
     }
 }
 
@@ -529,7 +599,7 @@ __device__ __forceinline__ void dx_restore(const DiffDev &d, int32_t seg) {
         d.cnt[i] = 0; d.cnt[i + N] = 0; d.chain[i] = -1; d.chain[i + N] = -1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        d.ncoll[0] = 0; d.ncoll[1] = 0;
+        d.ncoll[0] = 0; d.ncoll[1] = 0; *d.nLong = 0;
         *d.rng = __float_as_uint(ck[6 * N]);
     }
 }
@@ -793,6 +863,7 @@ FB_DIFF_KERNELS(driven_f, 1024, FB_COMMA int32_t t,
 // state); the last workgroup applies the forced spikes.
 FB_DIFF_KERNELS(deliver_forced_r, 256, FB_COMMA int32_t t FB_COMMA int32_t rf,
                 (blockIdx.x == gridDim.x - 1 ? dx_forced_replay(d, t) : dx_deliver<true>(d, t, rf, gridDim.x - 1)))
+FB_DIFF_KERNELS(long_s, 256, , (dx_long(d)))
 FB_DIFF_KERNELS(membrane_f, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<false>(d, t, r)))
 FB_DIFF_KERNELS(membrane_r, DIFF_BLOCK, FB_COMMA int32_t t FB_COMMA int32_t r, (dx_membrane<true>(d, t, r)))
 FB_DIFF_KERNELS(restore, 256, FB_COMMA int32_t seg, (dx_restore(d, seg)))
@@ -997,7 +1068,7 @@ extern "C" fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr
     float *cn, *sg, *os, *to, *ins, *dr, *sl, *dls, *state;
     double *elg, *accs;
     uint8_t *masks, *forced;
-    int32_t *drv, *dpos, *rid, *dflag, *cnt, *chain, *ncoll, *err;
+    int32_t *drv, *dpos, *rid, *dflag, *cnt, *chain, *ncoll, *err, *longList, *nLong;
     double *add;
     uint64_t *key;
     DiffColl *coll;
@@ -1013,6 +1084,7 @@ extern "C" fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr
         (st = dalloc(A, &state, 8 * n)) || (st = dalloc(A, &forced, n)) || (st = dalloc(A, &rng, 1)) ||
         (st = dalloc(A, &cnt, 2 * n)) || (st = dalloc(A, &chain, 2 * n)) || (st = dalloc(A, &add, 2 * n)) ||
         (st = dalloc(A, &key, 2 * n)) || (st = dalloc(A, &ncoll, 2)) || (st = dalloc(A, &err, 1)) ||
+        (st = dalloc(A, &longList, 2 * n)) || (st = dalloc(A, &nLong, 1)) ||
         (st = dalloc(A, &top, 1)) || (st = dalloc(A, &events, 2)) || (st = dalloc(A, &accs, 3 * n + 5 * (size_t)f->nblk)) ||
         (st = dalloc(A, &f->scal, 16)) || (st = dalloc(A, &f->clampDev, 4)))
         return st;
@@ -1033,7 +1105,7 @@ extern "C" fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr
     d.v = state; d.gE = state + n; d.gI = state + 2 * n; d.adapt = state + 3 * n; d.res = state + 4 * n;
     d.refr = state + 5 * n; d.spikeCount = state + 6 * n; d.scored = state + 7 * n;
     d.forced = forced; d.rng = rng; d.cnt = cnt; d.chain = chain; d.add = add; d.key = key; d.coll = coll;
-    d.ncoll = ncoll; d.err = err; d.top = top; d.events = events;
+    d.ncoll = ncoll; d.err = err; d.top = top; d.events = events; d.longList = longList; d.nLong = nLong;
     d.accW = accs; d.accDepU = accs + n; d.accInh = accs + 2 * n; d.blockAcc = accs + 3 * n;
     d.lv = adj; d.lgE = adj + n; d.lgI = adj + 2 * n; d.lad = adj + 3 * n; d.lres = adj + 4 * n;
     d.ring = adj + 5 * n; d.gLogGain = adj + (5 + d.S) * n; d.gInScale = adj + (6 + d.S) * n;
@@ -1434,6 +1506,7 @@ extern "C" fb_status fb_diff_forward(fb_diff *f, int32_t steps, double seed, int
             hipLaunchKernelGGL(kd_deliver_f, dim3(256), dim3(256), 0, st, ds, t, 0);
             hipLaunchKernelGGL(kd_driven_f, dim3(1), dim3(1024), 0, st, ds, t);
         }
+        hipLaunchKernelGGL(kd_long_s, dim3(DIFF_LONG_WG), dim3(256), 0, st, ds);
         hipLaunchKernelGGL(kd_membrane_f, dim3(f->nblk), dim3(DIFF_BLOCK), 0, st, ds, t, 0);
     }
     FB_HIP(hipGetLastError());
@@ -1562,6 +1635,7 @@ extern "C" fb_status fb_diff_backward(fb_diff *f, int32_t truncate, double *para
         hipLaunchKernelGGL(kd_restore, dim3(gN), dim3(256), 0, rs, d, sg);
         for (int32_t t = from; t < to; t++) {
             hipLaunchKernelGGL(kd_deliver_forced_r, dim3(257), dim3(256), 0, rs, d, t, resFromT);
+            hipLaunchKernelGGL(kd_long_s, dim3(DIFF_LONG_WG), dim3(256), 0, rs, d);
             hipLaunchKernelGGL(kd_membrane_r, dim3(f->nblk), dim3(DIFF_BLOCK), 0, rs, d, t, t - from);
         }
     };
@@ -1694,6 +1768,7 @@ extern "C" fb_status fb_diff_batch_forward(fb_diff_batch *bt, const int32_t *ste
             hipLaunchKernelGGL(kdb_deliver_f, dim3(256, B), dim3(256), 0, st, bt->ds, bt->tEnd, t, t, 0);
             hipLaunchKernelGGL(kdb_driven_f, dim3(1, B), dim3(1024), 0, st, bt->ds, bt->tEnd, t, t);
         }
+        hipLaunchKernelGGL(kdb_long_s, dim3(DIFF_LONG_WG, B), dim3(256), 0, st, bt->ds, bt->tEnd, t);
         hipLaunchKernelGGL(kdb_membrane_f, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, bt->tEnd, t, t, 0);
     }
     FB_HIP(hipGetLastError());
@@ -1769,6 +1844,7 @@ extern "C" fb_status fb_diff_batch_backward(fb_diff_batch *bt, int32_t truncate)
         hipLaunchKernelGGL(kdb_restore, dim3(gN, B), dim3(256), 0, st, bt->ds, te, from, sgi);
         for (int32_t t = from; t < to; t++) {
             hipLaunchKernelGGL(kdb_deliver_forced_r, dim3(257, B), dim3(256), 0, st, bt->ds, te, t, t, resFromT);
+            hipLaunchKernelGGL(kdb_long_s, dim3(DIFF_LONG_WG, B), dim3(256), 0, st, bt->ds, te, t);
             hipLaunchKernelGGL(kdb_membrane_r, dim3(bt->maxBlk, B), dim3(DIFF_BLOCK), 0, st, bt->ds, te, t, t,
                                t - from);
         }

@@ -64,6 +64,38 @@ def test_hard_forward_is_bit_identical_to_oracle(over):
     assert np.array_equal(so, sg) and np.array_equal(ro, rg)
 
 
+def test_hub_targets_with_many_arrivals_match_oracle():
+    """Two targets that every neuron projects to (one row entry twice), under a
+    drive that fires dozens of sources a step: their accumulators take more
+    than 32 arrivals in one step, which the long pass sorts in LDS.  Tape,
+    state and gradients against the oracle, forward and replay."""
+    from types import SimpleNamespace
+    pr = dc.Problem(N=400, drive_every=2, drive_hz=900.0, over={"soft": False, "tRef": 0.5, "minSyn": 1})
+    g0 = pr.g
+    rows = []
+    for j in range(g0.N):
+        r = list(g0.indices[g0.indptr[j]:g0.indptr[j + 1]])
+        c = list(g0.counts[g0.indptr[j]:g0.indptr[j + 1]])
+        rows.append((r + [1, 2, 1], c + [3.0, 2.0, 1.0]))
+    indptr = np.concatenate([[0], np.cumsum([len(r) for r, _ in rows])]).astype(np.int64)
+    indices = np.concatenate([r for r, _ in rows]).astype(np.int32)
+    counts = np.concatenate([c for _, c in rows]).astype(np.float32)
+    pr.g = SimpleNamespace(N=g0.N, indptr=indptr, indices=indices, counts=counts, exc=g0.exc, E=int(indices.size))
+    pr.counts = P.gated_counts(counts, indices, pr.p["minSyn"])
+    print("hub targets with many arrivals a step", flush=True)
+    o, g, Lo, Lg = _pair(pr, steps=120)
+    off_o, idx_o, amp_o, f_o = o.spikes()
+    off_g, idx_g, amp_g, f_g = g.spikes()
+    assert np.array_equal(off_o, off_g) and np.array_equal(idx_o, idx_g) and np.array_equal(f_o, f_g)
+    for k in STATE:
+        assert np.array_equal(o.state(k).view(np.uint32), g.state(k).view(np.uint32)), k
+    per_step = np.diff(off_o)
+    assert per_step.max() > 40          # every spike arrives at both hubs, one of them twice
+    go, gg = o.backward(), g.backward()
+    assert rel_err(gg["logGain"], go["logGain"]) < 1e-5
+    assert rel_err(gg["edge"], go["edge"]) < 1e-5
+
+
 @pytest.mark.parametrize("over", [{"soft": False, "tRef": 2.2}, {"soft": False, "tRef": 2.2, "adjClip": 0.05}])
 def test_hard_gradients_match_oracle(over):
     pr = dc.Problem(over=over).with_readout(R=30)
