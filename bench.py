@@ -584,6 +584,39 @@ def bench_visual(args):
     return out
 
 
+def bench_cord_ensemble(args):
+    """The cord lane ensemble (cordens.bend): 64 members x 6 perturbations of
+    the reference's ventral-cord brain, 384 runs of 1,000 steps with the
+    proprioceptive loop closed on the device; every run's meanHz is checked
+    against the reference's published readings.  The reference quotes 29.5 s
+    of wall time for the 384 runs on 12 CPU cores (docs/46-cord-ir.md:169-170).
+    Skipped when the fixture is absent."""
+    import numpy as np
+    cdir = os.path.join(ROOT, "tests", "golden", "_cord")
+    path, pub = os.path.join(cdir, "cord_brains.npz"), os.path.join(cdir, "cord_search.json")
+    if not (os.path.exists(path) and os.path.exists(pub)):
+        return None
+    z = np.load(path, allow_pickle=False)
+    if "ens:poolId" not in z.files:
+        return None
+    with open(pub) as f:
+        pubd = json.load(f)
+    want = {(r["member"], ci): rd["meanHz"] for r in pubd["members"] for ci, rd in enumerate(r["reads"])}
+    ens = pkg("cordens")
+    e = ens.CordEnsemble(z, json.loads(str(z["params"])))
+    e.run(ens.run_order()[:16])                      # warm-up: graphs and first launches
+    t0 = time.perf_counter()
+    got = e.run()
+    wall = time.perf_counter() - t0
+    e.release()
+    match = sum(ens.mean_hz(x) == want[mc] for mc, x in got.items())
+    return {"workload": "the cord lane ensemble: 64 members x 6 perturbations of the reference's ventral-cord "
+                        "brain (21,843 cells), 384 runs of 1,000 steps, proprioceptive loop closed on the device",
+            "runs": len(got), "wall_s": wall, "runs_per_s": len(got) / wall,
+            "steps_per_s": len(got) * 1000 / wall, "meanHz_matches_published": match,
+            "reference_wall_s_12_cores": 29.5, "speedup_vs_reference": 29.5 / wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -601,7 +634,7 @@ def main():
                          "scaling HBM counters), then exit")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
-    ap.add_argument("--only", choices=["flyvis", "visual"], help="run one side leg alone and print its JSON")
+    ap.add_argument("--only", choices=["flyvis", "visual", "cordens"], help="run one side leg alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -623,6 +656,9 @@ def main():
         return
     if args.only == "visual":
         print(json.dumps({"visual_chain": bench_visual(args)}), flush=True)
+        return
+    if args.only == "cordens":
+        print(json.dumps({"cord_ensemble": bench_cord_ensemble(args)}), flush=True)
         return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
@@ -760,6 +796,9 @@ def main():
     chain = bench_visual(args)
     if chain:
         out["visual_chain"] = chain
+    cens = bench_cord_ensemble(args)
+    if cens:
+        out["cord_ensemble"] = cens
     if not args.no_batch:
         out["batch64"] = bench_batch64(args, synth, P, lif, batch, hiprt, stream)
     if not args.no_adjoint:

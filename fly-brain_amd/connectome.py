@@ -199,7 +199,7 @@ def load_cord(data_dir: str) -> dict:
     sets = {s["name"]: set_idx[set_ptr[i]:set_ptr[i + 1]].astype(np.int64) for i, s in enumerate(d["sets"])}
     return {"desc": d, "N": N, "E": E, "indptr": indptr.astype(np.uint32), "indices": indices.astype(np.uint32),
             "counts": at("counts"), "origIdx": at("origIdx").astype(np.int64), "role": at("role"), "nt": at("nt"),
-            "sets": sets}
+            "hl": at("hl"), "edgeClass": at("edgeClass"), "legOrder": list(d.get("legOrder", [])), "sets": sets}
 
 
 def cord_brain(data_dir: str, ablate: str | None = None) -> dict:

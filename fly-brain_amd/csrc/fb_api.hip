@@ -516,6 +516,10 @@ extern "C" fb_status fb_reset(fb_brain *b) {
     hipLaunchKernelGGL(k_reset_state, dim3((std::max(b->d.N, b->d.nslots) + 255) / 256), dim3(256), 0, b->stream,
                        b->d);
     FB_HIP(hipGetLastError());
+    if (b->d.poolOf) {                 // the pool loop starts over with the state
+        FB_HIP(hipMemsetAsync(b->d.loopState, 0, sizeof(int32_t) * 2, b->stream));
+        FB_HIP(hipMemsetAsync(b->d.loopHist, 0, sizeof(uint32_t) * b->d.loopRing, b->stream));
+    }
     b->t = 0.0;
     return FB_OK;
 }
@@ -988,6 +992,64 @@ extern "C" fb_status fb_set_array(fb_brain *b, int32_t which, const void *src) {
     }
     FB_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)b->d.N, hipMemcpyHostToDevice, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_rng(fb_brain *b, uint32_t rng) {
+    FB_REQUIRE(b && rng != 0, FB_ERR_ARG, "fb_set_rng: null brain or a zero state");
+    FB_HIP(hipMemcpyAsync(&b->d.sc->rng, &rng, sizeof rng, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_set_pool_feedback(fb_brain *b, const uint8_t *poolOf, int32_t nPools, const int32_t *targets,
+                                          int32_t nTargets, float base, float gain, int32_t delay,
+                                          int32_t recordCap) {
+    FB_REQUIRE(b, FB_ERR_ARG, "fb_set_pool_feedback: null brain");
+    FB_HIP(hipStreamSynchronize(b->stream));
+    if (!poolOf) {
+        b->d.poolOf = nullptr;
+        b->version++;
+        return FB_OK;
+    }
+    FB_REQUIRE(nPools >= 1 && nPools <= 8 && nTargets >= 0 && (nTargets == 0 || targets) && delay >= 1 &&
+                   recordCap >= 0,
+               FB_ERR_ARG, "fb_set_pool_feedback: bad arguments");
+    for (int32_t q = 0; q < nTargets; q++)
+        FB_REQUIRE(targets[q] >= 0 && targets[q] < b->d.N, FB_ERR_ARG, "fb_set_pool_feedback: target %d out of range",
+                   q);
+    const int32_t N = b->d.N, ring = delay + 1;
+    uint8_t *pool;
+    int32_t *tg, *state;
+    uint32_t *hist;
+    uint16_t *rec;
+    fb_status st;
+    if ((st = dalloc(b->allocs, &pool, (size_t)N)) || (st = dalloc(b->allocs, &tg, (size_t)std::max(nTargets, 1))) ||
+        (st = dalloc(b->allocs, &state, 2)) || (st = dalloc(b->allocs, &hist, (size_t)ring)) ||
+        (st = dalloc(b->allocs, &rec, (size_t)std::max(recordCap, 1) * nPools)))
+        return st;
+    FB_HIP(hipMemcpy(pool, poolOf, (size_t)N, hipMemcpyHostToDevice));
+    if (nTargets) FB_HIP(hipMemcpy(tg, targets, sizeof(int32_t) * nTargets, hipMemcpyHostToDevice));
+    FB_HIP(hipMemset(state, 0, sizeof(int32_t) * 2));
+    FB_HIP(hipMemset(hist, 0, sizeof(uint32_t) * ring));
+    DevBrain &d = b->d;
+    d.poolOf = pool; d.loopTargets = tg; d.loopHist = hist; d.loopRec = rec; d.loopState = state;
+    d.loopPools = nPools; d.loopNTargets = nTargets; d.loopDelay = delay; d.loopRing = ring;
+    d.loopRecCap = recordCap; d.loopBase = base; d.loopGain = gain;
+    b->version++;                      // captured launches hold the old arguments
+    return FB_OK;
+}
+
+extern "C" fb_status fb_pool_counts(fb_brain *b, uint16_t *dst, int32_t cap, int32_t *nRecords) {
+    FB_REQUIRE(b && nRecords && (cap == 0 || dst), FB_ERR_ARG, "fb_pool_counts: bad arguments");
+    FB_REQUIRE(b->d.poolOf, FB_ERR_ARG, "fb_pool_counts: no pool loop is set");
+    int32_t state[2];
+    FB_HIP(hipMemcpyAsync(state, b->d.loopState, sizeof state, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    *nRecords = state[1];
+    const int32_t n = std::min(cap, state[1]);
+    if (n > 0)
+        FB_HIP(hipMemcpy(dst, b->d.loopRec, sizeof(uint16_t) * (size_t)n * b->d.loopPools, hipMemcpyDeviceToHost));
     return FB_OK;
 }
 

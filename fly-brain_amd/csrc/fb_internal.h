@@ -92,6 +92,14 @@ struct DevBrain {
     Scalars *sc;
     const uint32_t *jumpTab;   // 32 powers x 4 bytes x 256 entries of xorshift32 jumps
     const uint32_t *leap;      // LEAP_MATS leapfrog matrices (see leap_fill)
+    // the closed pool loop (fb_set_pool_feedback); poolOf null when off
+    const uint8_t *poolOf;     // [N] pool of each neuron, or >= loopPools for none
+    const int32_t *loopTargets;
+    uint32_t *loopHist;        // [loopRing] pool totals by step
+    uint16_t *loopRec;         // [loopRecCap][loopPools] per-step pool counts
+    int32_t *loopState;        // [0] steps since the loop was set, [1] records written
+    int32_t loopPools, loopNTargets, loopDelay, loopRing, loopRecCap;
+    float loopBase, loopGain;
 };
 
 // ------------------------------------------------------------------ device helpers

@@ -1008,6 +1008,7 @@ static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why) {
     if (serialDeliver) { *why = "presynaptic inhibition or tRef <= 0 needs the serial deliver path"; return FB_ERR_UNSUPPORTED; }
     if (d.k.bgEvents != 0.f) { *why = "background events draw from the shared stream every step"; return FB_ERR_UNSUPPORTED; }
+    if (d.poolOf) { *why = "the pool loop sets the drive after every step"; return FB_ERR_UNSUPPORTED; }
     if (d.nDriven > DRV_CAP) { *why = "more than 2048 driven neurons"; return FB_ERR_UNSUPPORTED; }
     const int32_t D = d.nslots - 1;
     if (D < 1 || D > D_MAX) { *why = "delay outside 1..32 steps"; return FB_ERR_UNSUPPORTED; }

@@ -516,6 +516,46 @@ __global__ void __launch_bounds__(DF_T) k_deliver_driven(DevBrain d, int32_t hea
     else deliver_flat(d, head, DF_BLOCKS);
 }
 
+// The closed pool loop of the cord ensemble (cordens.bend:227-273), after a
+// step's list is final: tally this step's spikes by pool, record the counts,
+// and set the drive of the loop's targets for the next step from the pool
+// total `delay` steps back, base + gain * total in binary32 (the total of a
+// step before the first is 0).
+constexpr int LOOP_MAX_POOLS = 8;
+
+__global__ void __launch_bounds__(256) k_pool_loop(DevBrain d, int32_t slot) {
+    __shared__ uint32_t cnt[LOOP_MAX_POOLS];
+    __shared__ float s_drive;
+    const int32_t tid = threadIdx.x;
+    if (tid < LOOP_MAX_POOLS) cnt[tid] = 0;
+    __syncthreads();
+    const int32_t n = d.ringCount[slot];
+    const int32_t *list = d.ring + (int64_t)slot * d.ringStride;
+    for (int32_t q = tid; q < n; q += 256) {
+        const uint32_t p = d.poolOf[list[q]];
+        if ((int32_t)p < d.loopPools) atomicAdd(&cnt[p], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int32_t k = d.loopState[0] + 1;
+        uint32_t total = 0;
+        for (int32_t l = 0; l < d.loopPools; l++) total += cnt[l];
+        const int32_t r = d.loopState[1];
+        if (r < d.loopRecCap) {
+            for (int32_t l = 0; l < d.loopPools; l++) d.loopRec[(int64_t)r * d.loopPools + l] = (uint16_t)min(cnt[l], 65535u);
+            d.loopState[1] = r + 1;
+        }
+        d.loopHist[k % d.loopRing] = total;
+        const uint32_t back = k - d.loopDelay >= 1 ? d.loopHist[(k - d.loopDelay) % d.loopRing] : 0u;
+        const float g = d.loopGain * (float)back;
+        s_drive = d.loopBase + g;
+        d.loopState[0] = k;
+    }
+    __syncthreads();
+    const float v = s_drive;
+    for (int32_t q = tid; q < d.loopNTargets; q += 256) d.drive[d.loopTargets[q]] = v;
+}
+
 // ------------------------------------------------------------------ host side
 
 int deliver_grid(const DevBrain &d) {
@@ -537,6 +577,7 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
     if (!serialDeliver) hipLaunchKernelGGL(k_long, dim3(64), dim3(LONG_T), 0, st, d);
     hipLaunchKernelGGL(k_membrane, dim3(d.nblocks), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_compact, dim3(d.nblocks), dim3(256), 0, st, d, slot, head);
+    if (d.poolOf) hipLaunchKernelGGL(k_pool_loop, dim3(1), dim3(256), 0, st, d, slot);
     FB_HIP(hipGetLastError());
     return FB_OK;
 }

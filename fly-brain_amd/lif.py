@@ -114,6 +114,37 @@ class LIFHip:
             raise ValueError("array size must be N")
         _l.check(self.lib.fb_set_array(self.h, _l.ARRAYS.index(which), _l.ptr(a)))
 
+    def set_rng(self, state: int):
+        """The xorshift32 state itself (a lane of the cord ensemble starts
+        from a raw state rather than the seed hash)."""
+        _l.check(self.lib.fb_set_rng(self.h, ctypes.c_uint32(int(state) & 0xFFFFFFFF)))
+
+    def set_pool_feedback(self, pool_of, targets, base: float, gain: float, delay: int, record: int,
+                          n_pools: int | None = None):
+        """The closed pool loop (fb_set_pool_feedback): after every step the
+        step's spikes are tallied by pool (pool_of[N], >= n_pools = none),
+        recorded, and ``targets`` driven at base + gain * (the pools' total
+        ``delay`` steps back) for the next step.  pool_of None turns it off."""
+        if pool_of is None:
+            _l.check(self.lib.fb_set_pool_feedback(self.h, None, 0, None, 0, 0.0, 0.0, 1, 0))
+            self._n_pools = 0
+            return
+        pool = np.ascontiguousarray(pool_of, dtype=np.uint8)
+        if pool.size != self.N:
+            raise ValueError("pool_of must have N entries")
+        self._n_pools = int(n_pools if n_pools is not None else pool[pool < 255].max() + 1)
+        tg = np.ascontiguousarray(targets, dtype=np.int32)
+        _l.check(self.lib.fb_set_pool_feedback(self.h, _l.ptr(pool), self._n_pools, _l.ptr(tg), int(tg.size),
+                                               float(base), float(gain), int(delay), int(record)))
+
+    def pool_counts(self) -> np.ndarray:
+        """The recorded per-step pool counts, [steps][pools] uint16."""
+        n = ctypes.c_int32()
+        _l.check(self.lib.fb_pool_counts(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, self._n_pools), np.uint16)
+        _l.check(self.lib.fb_pool_counts(self.h, _l.ptr(out), n.value, ctypes.byref(n)))
+        return out
+
     v = property(lambda self: self._get("v"))
     gE = property(lambda self: self._get("gE"))
     gI = property(lambda self: self._get("gI"))

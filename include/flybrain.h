@@ -134,6 +134,22 @@ fb_status fb_get_array(fb_brain *b, int32_t which, void *dst);
 fb_status fb_set_array(fb_brain *b, int32_t which, const void *src);
 /* RNG state, ring head, background accumulator, simulated time in ms */
 fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head, float *bg_acc, double *t_ms);
+/* The xorshift32 state itself (non-zero): a lane of the cord ensemble starts
+ * from a raw state, 2654435761 + 7919 m + c (cordens.bend:296-300), not from
+ * LIFWasm's seed hash. */
+fb_status fb_set_rng(fb_brain *b, uint32_t rng);
+/* The closed proprioceptive loop of the cord ensemble (cordens.bend:227-273):
+ * after every step the step's spikes are tallied by pool (poolOf[N]: pool
+ * 0..nPools-1, any larger value none; nPools <= 8), the counts recorded (up
+ * to recordCap steps), and the drive of targets[] for the next step set to
+ * base + gain * (the pools' total `delay` steps back), in binary32, the total
+ * of a step before the first being 0.  The drive values change, the driven
+ * list does not.  Runs on the per-step engines; poolOf NULL turns it off. */
+fb_status fb_set_pool_feedback(fb_brain *b, const uint8_t *poolOf, int32_t nPools, const int32_t *targets,
+                               int32_t nTargets, float base, float gain, int32_t delay, int32_t recordCap);
+/* the recorded per-step pool counts, [n][nPools] uint16, n = *nRecords (all
+ * steps since the loop was set; at most cap copied) */
+fb_status fb_pool_counts(fb_brain *b, uint16_t *dst, int32_t cap, int32_t *nRecords);
 /* Work done since create: edge events (the sum of out-degrees of the arriving
  * spikes, the reference's "edges/ms" of scripts/profile_fly.mjs:19,25),
  * synapse events (the same sum weighted by synapse count; needs counts at

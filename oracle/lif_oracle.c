@@ -184,6 +184,7 @@ float *fbo_array(fbo_brain *b, int32_t which) {
 }
 uint32_t *fbo_spike_count(fbo_brain *b) { return b->spikeCount; }
 uint32_t fbo_rng_state(const fbo_brain *b) { return b->rng; }
+void fbo_set_rng(fbo_brain *b, uint32_t rng) { b->rng = rng; }
 int32_t fbo_head(const fbo_brain *b) { return b->head; }
 int32_t fbo_nslots_of(const fbo_brain *b) { return b->nslots; }
 float fbo_bg_acc(const fbo_brain *b) { return b->bgAcc; }

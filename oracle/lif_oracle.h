@@ -49,6 +49,8 @@ void fbo_set_driven(fbo_brain *b, const int32_t *list, int32_t n);
 float *fbo_array(fbo_brain *b, int32_t which);
 uint32_t *fbo_spike_count(fbo_brain *b);
 uint32_t fbo_rng_state(const fbo_brain *b);
+/* the xorshift32 state itself (a cord-ensemble lane starts from a raw state) */
+void fbo_set_rng(fbo_brain *b, uint32_t rng);
 int32_t fbo_head(const fbo_brain *b);
 int32_t fbo_nslots_of(const fbo_brain *b);
 float fbo_bg_acc(const fbo_brain *b);

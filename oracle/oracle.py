@@ -54,7 +54,7 @@ def _lib(kind: str):
         "set_driven": (None, [P, P, I32]),
         "array": (ctypes.POINTER(F), [P, I32]),
         "spike_count": (ctypes.POINTER(U32), [P]),
-        "rng_state": (U32, [P]), "head": (I32, [P]), "nslots_of": (I32, [P]),
+        "rng_state": (U32, [P]), "set_rng": (None, [P, U32]), "head": (I32, [P]), "nslots_of": (I32, [P]),
         "bg_acc": (F, [P]),
         "last_fired": (ctypes.POINTER(I32), [P, ctypes.POINTER(I32)]),
         "step": (I32, [P]), "run": (I64, [P, I32, P, P, I64]),
@@ -797,3 +797,72 @@ class VisualChainOracle:
         if lum:
             out["lum"] = gLum
         return out
+
+
+# ---------------------------------------------------------------- the cord lane ensemble
+
+def cord_lane(tables, params: dict, m: int, c: int, steps: int = 1000, kind: str = "oracle") -> np.ndarray:
+    """One run of the cord lane ensemble (cordens.bend) on the lif.c checker:
+    the per-step leg-pool counts [steps][6] of member m under condition c.
+
+    The configuration restates Cfg.of, Cfg.comm, Comm.factor, Gain.of and
+    Cells.go (cordens.bend:62-160): hemilineage gains on the outgoing sign,
+    the commissural factor on those edges' weights, tonic bias on
+    interneurons, the command on descending cells, 20 Hz on proprioceptors,
+    ablated hemilineages silenced by threshold, the raw stream state
+    2654435761 + 7919 m + c; the driven list is the descending cells and the
+    proprioceptors in ascending order (Ids.go, :401-416).  The loop restates
+    Run.go (:248-273): after each step the pools are tallied from its spike
+    list, and the proprioceptors' drive for the next step is
+    f32(20 + f32(0.6 * total)), total being the pools' count `pdelay` steps
+    back (0 before the first step)."""
+    bit = lambda b: (m >> b) & 1 == 1
+    f32 = np.float32
+    g13a, g13b, g19b = (f32(2.0) if bit(k) else f32(0.5) for k in (1, 2, 3))
+    tonic = f32(4.0) if bit(4) else f32(0.0)
+    pdelay = 40 if bit(5) else 10
+    abl = 1 if c == 1 else 2 if c == 2 else 4 if c == 3 else 0
+    cmd = f32(0.0) if c == 5 else f32(60.0)
+    grp = 2 if c == 4 else (1 if bit(0) else 0)
+    factor = (f32(1.0), f32(3.0), f32(0.0))[grp]
+    N = int(tables["indptr"].size - 1)
+    hl, role = tables["ens:hl"], tables["ens:role"]
+    W = np.array(tables["W"], np.float32)
+    comm = tables["ens:comm"] > 0
+    W[comm] = W[comm] * factor
+    sign = np.array(tables["sign"], np.float32)
+    for i in range(N):
+        h = int(hl[i])
+        g = (g13a if h & 1 else f32(1.0)) * (g13b if h & 2 else f32(1.0))
+        g = g * (g19b if h & 4 else f32(1.0))
+        sign[i] = sign[i] * g
+
+    class _G:
+        pass
+    gr = _G()
+    gr.N, gr.indptr, gr.indices = N, tables["indptr"], tables["indices"]
+    o = OracleBrain(gr, params, seed=1, kind=kind, W=W, sign=sign)
+    o.f["set_rng"](o.h, (2654435761 + m * 7919 + c) & 0xFFFFFFFF)
+    prop = np.zeros(N, bool)
+    prop[tables["ens:proprio"]] = True
+    o.array("bias")[:] = np.where(role == 0, tonic, f32(0.0))
+    if abl:
+        o.array("thr")[:] = np.where((hl & abl) != 0, f32(1e6), f32(0.0))
+    drive = o.array("drive")
+    drive[:] = 0.0
+    drive[role == 2] = cmd
+    drive[prop] = f32(20.0)
+    o.drivenSet = [int(i) for i in np.flatnonzero((role == 2) | prop)]
+    o._driven_dirty = True
+    pool = tables["ens:poolId"]
+    targets = tables["ens:proprio"]
+    totals = [0]                       # totals[k] = the pools' count at step k; step 0 has none
+    out = np.zeros((steps, 6), np.uint16)
+    for k in range(1, steps + 1):
+        fired = o.step()
+        cnt = np.bincount(pool[fired], minlength=7)[:6] if len(fired) else np.zeros(6, np.int64)
+        out[k - 1] = np.minimum(cnt, 65535)
+        totals.append(int(cnt.sum()))
+        back = totals[k - pdelay] if k - pdelay >= 1 else 0
+        drive[targets] = f32(20.0) + f32(0.6) * f32(back)
+    return out

@@ -104,6 +104,7 @@ void ref_set_driven(ref_brain *r, const int32_t *list, int32_t n) {
 float *ref_array(ref_brain *r, int32_t which) { return r->arrays[which]; }
 uint32_t *ref_spike_count(ref_brain *r) { return r->h.spikeCount; }
 uint32_t ref_rng_state(const ref_brain *r) { return r->h.rng; }
+void ref_set_rng(ref_brain *r, uint32_t rng) { r->h.rng = rng; }
 int32_t ref_head(const ref_brain *r) { return r->h.head; }
 int32_t ref_nslots_of(const ref_brain *r) { return r->h.nslots; }
 float ref_bg_acc(const ref_brain *r) { return r->h.bgAcc; }

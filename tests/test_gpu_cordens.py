@@ -1,0 +1,40 @@
+"""The cord lane ensemble on the GPU (CordEnsemble over fb_set_pool_feedback,
+fb_set_rng and the per-step engines) against the checker and the reference's
+published readings: per-step leg-pool counts bit-identical to the lif.c
+checker's runs (both per-step engines), and all 384 runs' meanHz equal to
+public/data/cord_search.json (copied into tests/golden/_cord/)."""
+import importlib
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from test_cordens import SAMPLE, _fixture
+
+pytestmark = pytest.mark.gpu
+ens = importlib.import_module("fly-brain_amd.cordens")
+
+
+@pytest.mark.parametrize("engine", ["graph", "launch"])
+def test_runs_match_checker(engine):
+    z, p, want = _fixture()
+    e = ens.CordEnsemble(z, p, engine=engine)
+    runs = SAMPLE if engine == "graph" else SAMPLE[:4]
+    print(f"cord ensemble: {len(runs)} runs, engine {engine}", flush=True)
+    got = e.run(runs)
+    for m, c in runs:
+        ref = orc.cord_lane(z, p, m, c, 1000)
+        assert np.array_equal(got[(m, c)], ref), (m, c)
+        assert ens.mean_hz(got[(m, c)]) == want[(m, c)]
+    e.release()
+
+
+def test_all_runs_reproduce_published_readings():
+    z, p, want = _fixture()
+    e = ens.CordEnsemble(z, p)
+    print("cord ensemble: all 384 runs", flush=True)
+    got = e.run()
+    assert len(got) == 384
+    bad = [(mc, ens.mean_hz(x), want[mc]) for mc, x in got.items() if ens.mean_hz(x) != want[mc]]
+    assert not bad, bad[:10]
+    e.release()
