@@ -24,7 +24,8 @@ this brain) configured from its member and condition index alone
   `pdelay` steps back), the loop running on the device (fb_set_pool_feedback).
 
 Each distinct (commissural factor, hemilineage gains) pair gets one device
-graph; runs go several at a time, each brain on its own stream."""
+graph, and each (graph, loop delay) pair a set of brains that later runs of
+that kind reuse; runs go several at a time, each brain on its own stream."""
 from __future__ import annotations
 
 import numpy as np
@@ -111,6 +112,7 @@ class CordEnsemble:
         self.driven = np.flatnonzero((self.role == 2) | is_prop).astype(np.int32)
         self.engine = engine
         self._graphs = {}
+        self._free = {}
 
     def graph(self, cfg: dict) -> "lif.HipGraph":
         key = (cfg["comm"], float(cfg["g13a"]), float(cfg["g13b"]), float(cfg["g19b"]))
@@ -123,43 +125,59 @@ class CordEnsemble:
             self._graphs[key] = g
         return g
 
-    def start(self, m: int, c: int, steps: int = STEPS, stream=None) -> "lif.LIFHip":
-        """A run set up and its steps queued (on ``stream`` when given)."""
-        cfg = config(m, c)
+    def _brain(self, cfg: dict, steps: int) -> "lif.LIFHip":
+        """A brain for this configuration's graph and loop delay: a finished
+        one of the same kind when there is one (its captured step graphs stay
+        valid: a run changes data, not kernel arguments), else a new one."""
+        key = (cfg["comm"], float(cfg["g13a"]), float(cfg["g13b"]), float(cfg["g19b"]), cfg["pdelay"])
+        free = self._free.setdefault(key, [])
+        while free:
+            b = free.pop()
+            if b._cap >= steps:
+                b.reset()                                   # state and the loop's history
+                return b
+            b.destroy()
         b = lif.LIFHip(self.graph(cfg), self.p, seed=1, engine=self.engine)
-        if stream is not None:
-            b.set_stream(stream.handle)
-        b.set_rng(cfg["seed"])
-        bias = np.where(self.role == 0, cfg["tonic"], np.float32(0.0)).astype(np.float32)
-        b.set_array("bias", bias)
-        if cfg["abl"]:
-            thr = np.where((self.hl & cfg["abl"]) != 0, np.float32(1e6), np.float32(0.0)).astype(np.float32)
-            b.set_array("thr", thr)
         _l.check(b.lib.fb_set_drive(b.h, _l.ptr(self.driven), int(self.driven.size), 1.0))   # the list, ascending
+        b.set_pool_feedback(self.pool, self.proprio, float(PROPRIO_HZ), float(PGAIN), cfg["pdelay"], steps,
+                            n_pools=N_POOLS)
+        b._cap, b._key = steps, key
+        return b
+
+    def start(self, m: int, c: int, steps: int = STEPS) -> "lif.LIFHip":
+        """A run set up and its steps queued on its brain's stream."""
+        cfg = config(m, c)
+        b = self._brain(cfg, steps)
+        b.set_rng(cfg["seed"])
+        b.set_array("bias", np.where(self.role == 0, cfg["tonic"], np.float32(0.0)).astype(np.float32))
+        thr = np.zeros(self.N, np.float32)
+        if cfg["abl"]:
+            thr[(self.hl & cfg["abl"]) != 0] = np.float32(1e6)
+        b.set_array("thr", thr)
         drive = np.zeros(self.N, np.float32)
         drive[self.role == 2] = cfg["cmd"]
         drive[self.is_prop] = PROPRIO_HZ
-        b.set_array("drive", drive)                        # rates; the list stays
-        b.set_pool_feedback(self.pool, self.proprio, float(PROPRIO_HZ), float(PGAIN), cfg["pdelay"], steps,
-                            n_pools=N_POOLS)
+        b.set_array("drive", drive)                        # rates; the driven list stays
         b.run_async(steps)
         return b
 
-    def run(self, runs=None, steps: int = STEPS, concurrency: int = 16, streams=None) -> dict:
+    def run(self, runs=None, steps: int = STEPS, concurrency: int = 16) -> dict:
         """(m, c) -> per-step pool counts [steps][6] for each run (all 384 by
-        default), ``concurrency`` runs in flight on their own streams."""
+        default), ``concurrency`` runs in flight, each on its brain's stream."""
         runs = run_order() if runs is None else list(runs)
         out = {}
         for a in range(0, len(runs), concurrency):
-            batch = runs[a:a + concurrency]
-            live = [(mc, self.start(*mc, steps=steps, stream=None if streams is None else streams[k]))
-                    for k, mc in enumerate(batch)]
+            live = [(mc, self.start(*mc, steps=steps)) for mc in runs[a:a + concurrency]]
             for mc, b in live:
-                out[mc] = b.pool_counts()
-                b.destroy()
+                out[mc] = b.pool_counts()[:steps]
+                self._free[b._key].append(b)
         return out
 
     def release(self):
+        for free in self._free.values():
+            for b in free:
+                b.destroy()
+        self._free = {}
         for g in self._graphs.values():
             g.release()
         self._graphs = {}
