@@ -71,6 +71,32 @@ class LIFBatch:
         """setThr(i, 1e6) on every listed neuron of one member (scripts/lif_ensemble.mjs:35)."""
         self.setThr(member, ix, 1e6)
 
+    def set_lanes(self, sign_gain: bool, lane_drive=None):
+        """Lane mode (fb_batch_set_lanes): the type gain scales the sign, and
+        lane_drive [B][N] gives each member its own rates."""
+        ld = None if lane_drive is None else np.ascontiguousarray(lane_drive, dtype=np.float32).reshape(self.B, -1)
+        _l.check(self.lib.fb_batch_set_lanes(self.h, 1 if sign_gain else 0, _l.ptr(ld)))
+
+    def set_rng(self, states):
+        st = np.ascontiguousarray(states, dtype=np.uint32)
+        _l.check(self.lib.fb_batch_set_rng(self.h, _l.ptr(st)))
+
+    def set_pool_feedback(self, pool_of, n_pools: int, targets, base: float, gain: float, delays, record: int):
+        pool = np.ascontiguousarray(pool_of, dtype=np.uint8)
+        tg = np.ascontiguousarray(targets, dtype=np.int32)
+        dl = np.ascontiguousarray(delays, dtype=np.int32)
+        self._n_pools = int(n_pools)
+        _l.check(self.lib.fb_batch_set_pool_feedback(self.h, _l.ptr(pool), int(n_pools), _l.ptr(tg), int(tg.size),
+                                                     float(base), float(gain), _l.ptr(dl), int(record)))
+
+    def pool_counts(self) -> np.ndarray:
+        """[B][steps][pools] uint16 of the pool loop's record."""
+        n = ctypes.c_int32()
+        _l.check(self.lib.fb_batch_pool_counts(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros((self.B, n.value, self._n_pools), np.uint16)
+        _l.check(self.lib.fb_batch_pool_counts(self.h, _l.ptr(out), n.value, ctypes.byref(n)))
+        return out
+
     def set_steps(self, steps):
         st = np.ascontiguousarray(steps, dtype=np.int32)
         if st.size != self.B:

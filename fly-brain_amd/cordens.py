@@ -30,6 +30,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import batch as _batch
 from . import lib as _l
 from . import lif
 
@@ -86,6 +87,10 @@ def mean_hz(counts: np.ndarray) -> float:
     x = rate[i0:].astype(np.float64)
     means = x.sum(axis=0) / x.shape[0] if x.shape[0] else np.zeros(N_POOLS)
     return float(np.sort(means)[N_POOLS >> 1])
+
+
+def _batch_max() -> int:
+    return 64
 
 
 class CordEnsemble:
@@ -171,6 +176,54 @@ class CordEnsemble:
             for mc, b in live:
                 out[mc] = b.pool_counts()[:steps]
                 self._free[b._key].append(b)
+        return out
+
+    def run_lanes(self, runs=None, steps: int = STEPS) -> dict:
+        """The same runs as lanes of the lane ensemble engine, up to 64 to a
+        launch (the lanes of kernel.bend): one ensemble per commissural group
+        and 64 runs, the hemilineage gains as per-lane type gains on the sign
+        (the cell's hemilineage bits are its type), per-lane rates, seeds,
+        biases, thresholds and loop delays; the loop runs on the device for
+        every lane at once."""
+        runs = run_order() if runs is None else list(runs)
+        groups = {}
+        for mc in runs:
+            groups.setdefault(config(*mc)["comm"], []).append(mc)
+        interneurons = np.flatnonzero(self.role == 0).astype(np.int32)
+        types = np.arange(8, dtype=np.uint8)
+        out = {}
+        for grp, lst in groups.items():
+            f = COMM_FACTOR[grp]
+            W = np.where(self.comm, self.W * f, self.W).astype(np.float32)
+            g = lif.HipGraph(self.N, self.indptr, self.indices, W, self.sign,
+                             np.ones(self.indices.size, np.float32))     # counts: unused in lane mode
+            for a in range(0, len(lst), _batch_max()):
+                chunk = lst[a:a + _batch_max()]
+                cfgs = [config(*mc) for mc in chunk]
+                B = len(chunk)
+                gains = np.stack([hemilineage_gain(types, c) for c in cfgs]).astype(np.float32)
+                bt = _batch.LIFBatch(g, self.p, seeds=np.arange(1, B + 1, dtype=np.float64),
+                                     type_of=self.hl.astype(np.int32), gains=gains)
+                _l.check(bt.lib.fb_batch_set_drive(bt.h, _l.ptr(self.driven), int(self.driven.size), 1.0))
+                lane_drive = np.zeros((B, self.N), np.float32)
+                for q, c in enumerate(cfgs):
+                    lane_drive[q, self.role == 2] = c["cmd"]
+                    lane_drive[q, self.is_prop] = PROPRIO_HZ
+                bt.set_lanes(True, lane_drive)
+                bt.set_rng(np.array([c["seed"] for c in cfgs], np.uint32))
+                for q, c in enumerate(cfgs):
+                    if c["tonic"]:
+                        bt.setBias(q, interneurons, float(c["tonic"]))
+                    if c["abl"]:
+                        bt.setThr(q, np.flatnonzero((self.hl & c["abl"]) != 0).astype(np.int32), 1e6)
+                bt.set_pool_feedback(self.pool, N_POOLS, self.proprio, float(PROPRIO_HZ), float(PGAIN),
+                                     [c["pdelay"] for c in cfgs], steps)
+                bt.run(steps)
+                rec = bt.pool_counts()
+                for q, mc in enumerate(chunk):
+                    out[mc] = rec[q, :steps]
+                bt.destroy()
+            g.release()
         return out
 
     def release(self):

@@ -71,6 +71,20 @@ struct BatchDev {
     float *v, *gE, *gI, *refr, *trace, *adapt, *res, *bias, *thr;
     uint32_t *spikeCount;
     float *drive;                  // [N], shared by every member
+    // lane mode (fb_batch_set_lanes): signGain delivers W * f32(sign * gain)
+    // (the per-lane type gain scales the presynaptic sign, kernel.bend's
+    // lanes) instead of f32(count * gain) * sign; laneDrive [N][B] gives each
+    // member its own rates (null: drive[N] for all)
+    int32_t signGain;
+    float *laneDrive;
+    // the per-lane pool loop (fb_batch_set_pool_feedback); poolOf null when off
+    const uint8_t *poolOf;
+    const int32_t *loopTargets, *loopDelay;   // [nTargets], [B]
+    uint32_t *loopHist;            // [B][loopRing]
+    uint16_t *loopRec;             // [B][loopRecCap][loopPools]
+    int32_t *loopState;            // [0] steps since set, [1] records written
+    int32_t loopPools, loopNTargets, loopRing, loopRecCap;
+    float loopBase, loopGain;
     const int32_t *driven;         // [nDriven]
     const int32_t *drvPos;         // [N]
     const int32_t *typeOf;         // [N]

@@ -84,10 +84,10 @@ __global__ void __launch_bounds__(256) kb_deliver(BatchDev d, int32_t head, int3
             }
             const float g = __shfl(gLane, b, 64);
             const int32_t cls = s > 0 ? 0 : N;
+            const float sg = s * g;                  // lane mode: the gain scales the sign
             for (uint32_t j = ra + lane; j < re; j += 64) {
                 const uint32_t q = d.g.edges[j].x;
-                const float w = (float)d.g.counts[j] * g;
-                const float addend = w * s;
+                const float addend = d.signGain ? __uint_as_float(d.g.edges[j].y) * sg : ((float)d.g.counts[j] * g) * s;
                 const int64_t ci = ((int64_t)cls + q) * B + b;
                 const uint64_t kk = ((uint64_t)rank << 32) | (uint64_t)(j - ra);
                 const int32_t slot = atomicAdd(&d.cnt[ci], 1);
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
             const bool in = on && c0 + u < hi;
             const int32_t i = c0 + u < hi ? d.driven[c0 + u] : 0;
             r[u] = in ? d.refr[(int64_t)i * B + lane] : 1.f;
-            pf[u] = in ? d.drive[i] * d.k.dtS : 0.f;
+            pf[u] = in ? (d.laneDrive ? d.laneDrive[(int64_t)i * B + lane] : d.drive[i]) * d.k.dtS : 0.f;
         }
 #pragma unroll
         for (int u = 0; u < CH; u++) {
@@ -287,6 +287,59 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
             d.active[(int64_t)slot * N + p] = i;
             atomicAdd(&d.counters[2], (unsigned long long)__popcll(m));
         }
+    }
+}
+
+// The per-lane pool loop (cordens.bend:227-273) after a step: each lane's
+// spikes of the step tallied by pool (from the slot's list and masks), the
+// counts recorded, and each lane's drive of the loop's targets for the next
+// step set to base + gain * (its pools' total loopDelay[b] steps back), in
+// binary32 (0 before the first step).
+__global__ void __launch_bounds__(256) kb_pool_loop(BatchDev d, int32_t slot) {
+    __shared__ uint32_t cnt[BMAX][8];
+    __shared__ float s_drive[BMAX];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < BMAX * 8; q += 256) cnt[q >> 3][q & 7] = 0u;
+    __syncthreads();
+    const int32_t n = d.nActive[slot];
+    const int32_t *list = d.active + (int64_t)slot * d.N;
+    for (int32_t q = tid; q < n; q += 256) {
+        const int32_t i = list[q];
+        const uint32_t p = d.poolOf[i];
+        if ((int32_t)p >= d.loopPools) continue;
+        uint64_t m = d.mask[(int64_t)slot * d.N + i];
+        while (m) {
+            const int b = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            atomicAdd(&cnt[b][p], 1u);
+        }
+    }
+    __syncthreads();
+    const int32_t k = d.loopState[0] + 1, r = d.loopState[1];
+    if (tid < d.B) {
+        const int b = tid;
+        uint32_t total = 0;
+        for (int32_t l = 0; l < d.loopPools; l++) {
+            total += cnt[b][l];
+            if (r < d.loopRecCap)
+                d.loopRec[((int64_t)b * d.loopRecCap + r) * d.loopPools + l] = (uint16_t)min(cnt[b][l], 65535u);
+        }
+        uint32_t *h = d.loopHist + (int64_t)b * d.loopRing;
+        h[k % d.loopRing] = total;
+        const int32_t dl = d.loopDelay[b];
+        const uint32_t back = k - dl >= 1 ? h[(k - dl) % d.loopRing] : 0u;
+        const float gb = d.loopGain * (float)back;
+        s_drive[b] = d.loopBase + gb;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        d.loopState[0] = k;
+        if (r < d.loopRecCap) d.loopState[1] = r + 1;
+    }
+    const int32_t B = d.B;
+    for (int32_t q = tid; q < d.loopNTargets * B; q += 256) {
+        const int32_t t = q / B, b = q - t * B;
+        d.laneDrive[(int64_t)d.loopTargets[t] * B + b] = s_drive[b];
     }
 }
 
@@ -538,6 +591,7 @@ static fb_status batch_step(fb_batch *b) {
     hipLaunchKernelGGL(kb_deliver, dim3(256), dim3(256), 0, st, d, head, b->t);
     hipLaunchKernelGGL(kb_driven, dim3(1), dim3(1024), 0, st, d, slot, b->t);
     hipLaunchKernelGGL(kb_membrane, dim3((d.N + 3) / 4), dim3(256), 0, st, d, slot, head, b->t);
+    if (d.poolOf) hipLaunchKernelGGL(kb_pool_loop, dim3(1), dim3(256), 0, st, d, slot);
     FB_HIP(hipGetLastError());
     b->lastSlot = slot;
     b->head = (head + 1) % d.nslots;
@@ -581,6 +635,89 @@ static fb_status run_per_step(fb_batch *b, int32_t nsteps, int32_t *nactive, int
             *tot += n;
         }
     }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_lanes(fb_batch *b, int32_t signGain, const float *laneDrive) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_lanes: null ensemble");
+    BatchDev &d = b->d;
+    FB_HIP(hipStreamSynchronize(b->stream));
+    d.signGain = signGain ? 1 : 0;
+    if (laneDrive) {
+        const int32_t N = d.N, B = d.B;
+        if (!d.laneDrive) {
+            fb_status st = balloc(b, &d.laneDrive, (size_t)N * B);
+            if (st) return st;
+        }
+        std::vector<float> t((size_t)N * B);                 // [B][N] -> [N][B]
+        for (int32_t q = 0; q < B; q++)
+            for (int32_t i = 0; i < N; i++) t[(size_t)i * B + q] = laneDrive[(size_t)q * N + i];
+        FB_HIP(hipMemcpy(d.laneDrive, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+    } else {
+        d.laneDrive = nullptr;          // (its buffer stays with the ensemble)
+    }
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_rng(fb_batch *b, const uint32_t *states) {
+    FB_REQUIRE_B(b && states, FB_ERR_ARG, "fb_batch_set_rng: bad arguments");
+    for (int32_t q = 0; q < b->d.B; q++)
+        FB_REQUIRE_B(states[q] != 0, FB_ERR_ARG, "fb_batch_set_rng: member %d has a zero state", q);
+    FB_HIP(hipStreamSynchronize(b->stream));
+    FB_HIP(hipMemcpy(b->d.rng, states, sizeof(uint32_t) * b->d.B, hipMemcpyHostToDevice));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *poolOf, int32_t nPools,
+                                                const int32_t *targets, int32_t nTargets, float base, float gain,
+                                                const int32_t *delay, int32_t recordCap) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_pool_feedback: null ensemble");
+    BatchDev &d = b->d;
+    FB_HIP(hipStreamSynchronize(b->stream));
+    if (!poolOf) { d.poolOf = nullptr; return FB_OK; }
+    FB_REQUIRE_B(d.laneDrive, FB_ERR_ARG, "fb_batch_set_pool_feedback: set per-lane drive first (fb_batch_set_lanes)");
+    FB_REQUIRE_B(nPools >= 1 && nPools <= 8 && nTargets >= 0 && (nTargets == 0 || targets) && delay && recordCap >= 0,
+                 FB_ERR_ARG, "fb_batch_set_pool_feedback: bad arguments");
+    int32_t maxDelay = 1;
+    for (int32_t q = 0; q < d.B; q++) {
+        FB_REQUIRE_B(delay[q] >= 1, FB_ERR_ARG, "fb_batch_set_pool_feedback: member %d has delay < 1", q);
+        maxDelay = std::max(maxDelay, delay[q]);
+    }
+    for (int32_t q = 0; q < nTargets; q++)
+        FB_REQUIRE_B(targets[q] >= 0 && targets[q] < d.N, FB_ERR_ARG, "fb_batch_set_pool_feedback: target %d out of range", q);
+    const int32_t ring = maxDelay + 1;
+    uint8_t *pool;
+    int32_t *tg, *dl, *state;
+    uint32_t *hist;
+    uint16_t *rec;
+    fb_status st;
+    if ((st = balloc(b, &pool, (size_t)d.N)) || (st = balloc(b, &tg, (size_t)std::max(nTargets, 1))) ||
+        (st = balloc(b, &dl, (size_t)d.B)) || (st = balloc(b, &state, 2)) || (st = balloc(b, &hist, (size_t)d.B * ring)) ||
+        (st = balloc(b, &rec, (size_t)d.B * std::max(recordCap, 1) * nPools)))
+        return st;
+    FB_HIP(hipMemcpy(pool, poolOf, (size_t)d.N, hipMemcpyHostToDevice));
+    if (nTargets) FB_HIP(hipMemcpy(tg, targets, sizeof(int32_t) * nTargets, hipMemcpyHostToDevice));
+    FB_HIP(hipMemcpy(dl, delay, sizeof(int32_t) * d.B, hipMemcpyHostToDevice));
+    FB_HIP(hipMemset(state, 0, sizeof(int32_t) * 2));
+    FB_HIP(hipMemset(hist, 0, sizeof(uint32_t) * (size_t)d.B * ring));
+    d.poolOf = pool; d.loopTargets = tg; d.loopDelay = dl; d.loopState = state; d.loopHist = hist; d.loopRec = rec;
+    d.loopPools = nPools; d.loopNTargets = nTargets; d.loopRing = ring; d.loopRecCap = recordCap;
+    d.loopBase = base; d.loopGain = gain;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_batch_pool_counts(fb_batch *b, uint16_t *dst, int32_t cap, int32_t *nRecords) {
+    FB_REQUIRE_B(b && nRecords && (cap == 0 || dst), FB_ERR_ARG, "fb_batch_pool_counts: bad arguments");
+    const BatchDev &d = b->d;
+    FB_REQUIRE_B(d.poolOf, FB_ERR_ARG, "fb_batch_pool_counts: no pool loop is set");
+    int32_t state[2];
+    FB_HIP(hipMemcpyAsync(state, d.loopState, sizeof state, hipMemcpyDeviceToHost, b->stream));
+    FB_HIP(hipStreamSynchronize(b->stream));
+    *nRecords = state[1];
+    const int32_t n = std::min(cap, state[1]);
+    for (int32_t q = 0; q < d.B && n > 0; q++)
+        FB_HIP(hipMemcpy(dst + (size_t)q * n * d.loopPools, d.loopRec + (size_t)q * d.loopRecCap * d.loopPools,
+                         sizeof(uint16_t) * (size_t)n * d.loopPools, hipMemcpyDeviceToHost));
     return FB_OK;
 }
 

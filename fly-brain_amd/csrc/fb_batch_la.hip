@@ -716,6 +716,7 @@ bool batch_la_supported(const BatchDev &d, std::string *why) {
     const int32_t D = d.nslots - 1;
     auto no = [&](const char *w) { if (why) *why = w; return false; };
     if (d.liveAdapt || d.liveRes || d.liveBg) return no("adaptation, depression or background is on");
+    if (d.signGain || d.laneDrive || d.poolOf) return no("lane mode or the pool loop is on");
     if (D < 1 || D > BDM) return no("delay outside 1..18 steps");
     if (d.nDriven > BDRV) return no("more than 384 driven neurons");
     if (d.g.maxRow >= (1 << 23)) return no("a row longer than 2^23");

@@ -93,6 +93,10 @@ _SIGS = {
     "fb_diff_set_loss_per_epoch": (I32, [P, P, I32, I32]),
     "fb_diff_drive_grad": (I32, [P, P]),
     "fb_set_rng": (I32, [P, ctypes.c_uint32]),
+    "fb_batch_set_lanes": (I32, [P, I32, P]),
+    "fb_batch_set_rng": (I32, [P, P]),
+    "fb_batch_set_pool_feedback": (I32, [P, P, I32, P, I32, ctypes.c_float, ctypes.c_float, P, I32]),
+    "fb_batch_pool_counts": (I32, [P, P, I32, P]),
     "fb_set_pool_feedback": (I32, [P, P, I32, P, I32, ctypes.c_float, ctypes.c_float, I32, I32]),
     "fb_pool_counts": (I32, [P, P, I32, P]),
     "fb_fv_model_create": (I32, [I32, I64, P, P, P, P, P, D, P]),

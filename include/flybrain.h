@@ -195,6 +195,20 @@ fb_status fb_batch_set_steps(fb_batch *b, const int32_t *steps);
 fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
                        uint64_t *fmasks, int64_t cap, int64_t *total);
 fb_status fb_batch_get_array(fb_batch *b, int32_t which, int32_t member, void *dst);
+/* Lane mode, for the lane ensemble of cordens.bend:140-160: signGain = 1
+ * delivers W[j] * f32(sign * gain[b][type]) (the member's type gain scales
+ * the presynaptic sign; the graph's baked W is used) instead of
+ * f32(count * gain) * sign; laneDrive [B][N] gives every member its own
+ * drive rates over the shared driven list (NULL: the shared rates). */
+fb_status fb_batch_set_lanes(fb_batch *b, int32_t signGain, const float *laneDrive);
+/* each member's xorshift32 state itself, states[B] (non-zero) */
+fb_status fb_batch_set_rng(fb_batch *b, const uint32_t *states);
+/* fb_set_pool_feedback for every member at once, with its own delay[B]; the
+ * loop writes the per-lane drive (fb_batch_set_lanes first) */
+fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *poolOf, int32_t nPools, const int32_t *targets,
+                                     int32_t nTargets, float base, float gain, const int32_t *delay, int32_t recordCap);
+/* dst [B][n][nPools] uint16, n = min(cap, *nRecords) */
+fb_status fb_batch_pool_counts(fb_batch *b, uint16_t *dst, int32_t cap, int32_t *nRecords);
 fb_status fb_batch_rng(fb_batch *b, uint32_t *rng);
 fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
                             int64_t *steps);

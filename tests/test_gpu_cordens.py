@@ -1,7 +1,9 @@
-"""The cord lane ensemble on the GPU (CordEnsemble over fb_set_pool_feedback,
-fb_set_rng and the per-step engines) against the checker and the reference's
-published readings: per-step leg-pool counts bit-identical to the lif.c
-checker's runs (both per-step engines), and all 384 runs' meanHz equal to
+"""The cord lane ensemble on the GPU against the checker and the reference's
+published readings: as one brain per run (CordEnsemble.run over
+fb_set_pool_feedback, fb_set_rng and the per-step engines) and as lanes of
+the lane ensemble engine (CordEnsemble.run_lanes over fb_batch_set_lanes,
+fb_batch_set_rng and fb_batch_set_pool_feedback): per-step leg-pool counts
+bit-identical to the lif.c checker's runs, and all 384 runs' meanHz equal to
 public/data/cord_search.json (copied into tests/golden/_cord/)."""
 import importlib
 
@@ -34,6 +36,29 @@ def test_all_runs_reproduce_published_readings():
     e = ens.CordEnsemble(z, p)
     print("cord ensemble: all 384 runs", flush=True)
     got = e.run()
+    assert len(got) == 384
+    bad = [(mc, ens.mean_hz(x), want[mc]) for mc, x in got.items() if ens.mean_hz(x) != want[mc]]
+    assert not bad, bad[:10]
+    e.release()
+
+
+def test_lanes_match_checker():
+    """The sample as lanes of one ensemble per commissural group."""
+    z, p, want = _fixture()
+    e = ens.CordEnsemble(z, p)
+    print(f"cord ensemble as lanes: {len(SAMPLE)} runs", flush=True)
+    got = e.run_lanes(SAMPLE)
+    for m, c in SAMPLE:
+        ref = orc.cord_lane(z, p, m, c, 1000)
+        assert np.array_equal(got[(m, c)], ref), (m, c)
+    e.release()
+
+
+def test_all_runs_as_lanes_reproduce_published_readings():
+    z, p, want = _fixture()
+    e = ens.CordEnsemble(z, p)
+    print("cord ensemble as lanes: all 384 runs", flush=True)
+    got = e.run_lanes()
     assert len(got) == 384
     bad = [(mc, ens.mean_hz(x), want[mc]) for mc, x in got.items() if ens.mean_hz(x) != want[mc]]
     assert not bad, bad[:10]
