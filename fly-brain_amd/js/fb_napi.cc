@@ -2,7 +2,8 @@
 // runtime).  Functions map one-to-one onto include/flybrain.h; handles are
 // napi externals, arrays are TypedArrays the caller owns (copied in or out,
 // never retained).  Errors become JS exceptions carrying fb_last_error().
-// lifhip.js builds the LIFWasm-compatible class on top of these functions.
+// lifhip.js builds the LIFWasm-compatible class on top of these functions,
+// flyviship.js the FlyVis / FlyVisDiff classes on the fv* ones.
 #include <node_api.h>
 
 #include <cstdint>
@@ -418,6 +419,140 @@ napi_value params_length(napi_env env, napi_callback_info) {
     return make_i32(env, (int32_t)(sizeof(fb_params) / sizeof(double)));
 }
 
+// ---------------------------------------------------------------- optic lobe (fb_fv_*)
+
+napi_value fv_model_create(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 7)) return nullptr;
+    const int32_t N = i32(env, a.v[0]);
+    int32_t *ip, *tg;
+    float *w, *bias, *tau;
+    size_t nip, ntg, nw, nb, nt;
+    if (!typed(env, a.v[1], &ip, &nip) || !typed(env, a.v[2], &tg, &ntg) || !typed(env, a.v[3], &w, &nw) ||
+        !typed(env, a.v[4], &bias, &nb) || !typed(env, a.v[5], &tau, &nt))
+        return nullptr;
+    if (nip != (size_t)N + 1 || nw != ntg || nb != (size_t)N || nt != (size_t)N) {
+        napi_throw_range_error(env, nullptr, "fvModelCreate: array sizes do not match N and E");
+        return nullptr;
+    }
+    fb_fv_model *m = nullptr;
+    FB_OK_OR_THROW(fb_fv_model_create(N, (int64_t)ntg, ip, tg, w, bias, tau, num(env, a.v[6]), &m), "fvModelCreate");
+    return make_ext(env, m);
+}
+
+napi_value fv_model_set_kdt(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 2)) return nullptr;
+    fb_fv_model *m = (fb_fv_model *)ext(env, a.v[0]);
+    float *k;
+    size_t n;
+    if (!m || !typed(env, a.v[1], &k, &n)) return nullptr;
+    FB_OK_OR_THROW(fb_fv_model_set_kdt(m, k), "fvModelSetKdt");
+    return nullptr;
+}
+
+napi_value fv_model_set_input_map(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 3)) return nullptr;
+    fb_fv_model *m = (fb_fv_model *)ext(env, a.v[0]);
+    int32_t *nd, *cl;
+    size_t nn, nc;
+    if (!m || !typed(env, a.v[1], &nd, &nn) || !typed(env, a.v[2], &cl, &nc)) return nullptr;
+    if (nn != nc) {
+        napi_throw_range_error(env, nullptr, "fvModelSetInputMap: node and column lists differ in length");
+        return nullptr;
+    }
+    FB_OK_OR_THROW(fb_fv_model_set_input_map(m, (int32_t)nn, nd, cl), "fvModelSetInputMap");
+    return nullptr;
+}
+
+napi_value fv_model_release(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 1)) return nullptr;
+    fb_fv_model *m = (fb_fv_model *)ext(env, a.v[0]);
+    if (!m) return nullptr;
+    FB_OK_OR_THROW(fb_fv_model_release(m), "fvModelRelease");
+    return nullptr;
+}
+
+napi_value fv_create(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 1)) return nullptr;
+    fb_fv_model *m = (fb_fv_model *)ext(env, a.v[0]);
+    if (!m) return nullptr;
+    fb_fv *e = nullptr;
+    FB_OK_OR_THROW(fb_fv_create(m, &e), "fvCreate");
+    return make_ext(env, e);
+}
+
+// fv calls of the form f(e), f(e, int), f(e, array)
+template <fb_status (*F)(fb_fv *)>
+napi_value fv_call0(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 1)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    if (!e) return nullptr;
+    FB_OK_OR_THROW(F(e), "fv");
+    return nullptr;
+}
+
+template <fb_status (*F)(fb_fv *, int32_t)>
+napi_value fv_call_int(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 2)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    if (!e) return nullptr;
+    FB_OK_OR_THROW(F(e, i32(env, a.v[1])), "fv");
+    return nullptr;
+}
+
+template <fb_status (*F)(fb_fv *, const float *)>
+napi_value fv_call_arr(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 2)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    float *x;
+    size_t n;
+    if (!e || !typed(env, a.v[1], &x, &n)) return nullptr;
+    FB_OK_OR_THROW(F(e, x), "fv");
+    return nullptr;
+}
+
+napi_value fv_get(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 3)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    float *dst;
+    size_t n;
+    if (!e || !typed(env, a.v[2], &dst, &n)) return nullptr;
+    FB_OK_OR_THROW(fb_fv_get(e, i32(env, a.v[1]), dst), "fvGet");
+    return nullptr;
+}
+
+napi_value fv_tape_len(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 1)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    if (!e) return nullptr;
+    int32_t n = 0;
+    FB_OK_OR_THROW(fb_fv_tape_len(e, &n), "fvTapeLen");
+    return make_i32(env, n);
+}
+
+// fvBackward(e, dLdV, gWeight, gBias, gKdt, dLdLum | null): accumulators added to in place
+napi_value fv_backward(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 6)) return nullptr;
+    fb_fv *e = (fb_fv *)ext(env, a.v[0]);
+    float *dv, *gw, *gb, *gk, *dl;
+    size_t n0, n1, n2, n3, n4;
+    if (!e || !typed(env, a.v[1], &dv, &n0) || !typed(env, a.v[2], &gw, &n1) || !typed(env, a.v[3], &gb, &n2) ||
+        !typed(env, a.v[4], &gk, &n3) || !typed(env, a.v[5], &dl, &n4, true))
+        return nullptr;
+    FB_OK_OR_THROW(fb_fv_backward(e, dv, gw, gb, gk, dl), "fvBackward");
+    return nullptr;
+}
+
 napi_value init(napi_env env, napi_value exports) {
     struct Fn {
         const char *name;
@@ -446,6 +581,22 @@ napi_value init(napi_env env, napi_value exports) {
         {"runAsync", run_async},
         {"deviceInfo", device_info},
         {"paramsLength", params_length},
+        {"fvModelCreate", fv_model_create},
+        {"fvModelSetKdt", fv_model_set_kdt},
+        {"fvModelSetInputMap", fv_model_set_input_map},
+        {"fvModelRelease", fv_model_release},
+        {"fvCreate", fv_create},
+        {"fvDestroy", fv_call0<fb_fv_destroy>},
+        {"fvReset", fv_call0<fb_fv_reset>},
+        {"fvSetInput", fv_call_arr<fb_fv_set_input>},
+        {"fvSetLum", fv_call_arr<fb_fv_set_lum>},
+        {"fvSetV", fv_call_arr<fb_fv_set_v>},
+        {"fvStep", fv_call_int<fb_fv_step>},
+        {"fvTapeStart", fv_call_int<fb_fv_tape_start>},
+        {"fvStepTaped", fv_call_int<fb_fv_step_taped>},
+        {"fvTapeLen", fv_tape_len},
+        {"fvGet", fv_get},
+        {"fvBackward", fv_backward},
     };
     for (const Fn &f : fns) {
         napi_value v;

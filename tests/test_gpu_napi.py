@@ -137,3 +137,47 @@ def test_live_mode_batches_like_lifgpu(tmp_path):
     assert np.array_equal(np.array(res["v"], np.float32).view(np.uint32), o.array("v").view(np.uint32))
     assert res["rng"] == o.rng
     assert res["t"] == pytest.approx(27 * p["dt"])
+
+
+def test_flyvis_diff_from_node_matches_oracle(tmp_path):
+    """FlyVisDiff from JavaScript (flyviship.js over the addon's fv* calls):
+    two eyes on one model, taped steps with a luminance frame each, the
+    reverse sweep into one gradient; activity and every gradient
+    bit-identical to the CPU restatement."""
+    from test_flyvis import small_model
+    import oracle as orc
+    m = small_model(seed=21, N=900, nCol=30)
+    rng = np.random.default_rng(21)
+    T = 12
+    lums = [[rng.uniform(0, 1, 30).astype(np.float32) for _ in range(T)] for _ in range(2)]
+    dLdV = []
+    for _ in range(2):
+        d = np.zeros((T, m["N"]), np.float32)
+        d[:, rng.choice(m["N"], 80, replace=False)] = rng.normal(0, 1, (T, 80))
+        dLdV.append(d)
+    spec = {"model": {"N": m["N"], "E": m["E"], "indptr": m["indptr"].tolist(), "target": m["target"].tolist(),
+                      "weight": m["weight"].tolist(), "bias": m["bias"].tolist(), "tau": m["tau"].tolist(),
+                      "dt": m["dt"], "inputIdx": {k: np.asarray(v).tolist() for k, v in m["inputIdx"].items()}},
+            "lums": [[x.tolist() for x in lums[e]] for e in range(2)], "dLdV": [d.reshape(-1).tolist() for d in dLdV]}
+    f = tmp_path / "fv.json"
+    f.write_text(json.dumps(spec))
+    print("node FlyVisDiffHip parity", flush=True)
+    out = subprocess.run(["node", os.path.join(HERE, "js", "fv_run.js"), str(f)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout)
+    g = {"weight": np.zeros(m["E"], np.float32), "bias": np.zeros(m["N"], np.float32),
+         "kdt": np.zeros(m["N"], np.float32)}
+    for e in range(2):
+        o = orc.FlyVisDiffOracle(m)
+        o.start_tape()
+        for t in range(T):
+            o.setInput(lums[e][t])
+            o.step()
+        assert np.array_equal(np.array(res["v"][e], np.float32).view(np.uint32), o.v.view(np.uint32)), e
+        dl = np.zeros((T, o.nCol), np.float32)
+        o.backward(dLdV[e], g, dl)
+        assert np.array_equal(np.array(res["lum"][e], np.float32).view(np.uint32), dl.reshape(-1).view(np.uint32)), e
+    for k in ("weight", "bias", "kdt"):
+        assert np.array_equal(np.array(res[k], np.float32).view(np.uint32), g[k].view(np.uint32)), k
+    assert res["tape"] == T

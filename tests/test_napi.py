@@ -24,7 +24,10 @@ def test_addon_exports_the_brain_api(tmp_path):
     res = json.loads(out.stdout)
     want = {"graphCreate", "graphRelease", "brainCreate", "brainDestroy", "reset", "setParams", "setDrive",
             "setDriveOne", "setBias", "setThr", "addG", "pulse", "setBackground", "step", "fired", "getArray",
-            "setArray", "scalars", "counters", "runAsync", "deviceInfo", "paramsLength", "applyDeltas"}
+            "setArray", "scalars", "counters", "runAsync", "deviceInfo", "paramsLength", "applyDeltas",
+            "fvModelCreate", "fvModelSetKdt", "fvModelSetInputMap", "fvModelRelease", "fvCreate", "fvDestroy",
+            "fvReset", "fvSetInput", "fvSetLum", "fvSetV", "fvStep", "fvTapeStart", "fvStepTaped", "fvTapeLen",
+            "fvGet", "fvBackward"}
     assert want <= set(res["exports"])
     assert res["paramsLength"] == 19
 
