@@ -80,10 +80,11 @@ struct BatchDev {
     // the per-lane pool loop (fb_batch_set_pool_feedback); poolOf null when off
     const uint8_t *poolOf;
     const int32_t *loopTargets, *loopDelay;   // [nTargets], [B]
+    const int32_t *loopCells;      // [loopNCells] the neurons that belong to a pool
     uint32_t *loopHist;            // [B][loopRing]
     uint16_t *loopRec;             // [B][loopRecCap][loopPools]
     int32_t *loopState;            // [0] steps since set, [1] records written
-    int32_t loopPools, loopNTargets, loopRing, loopRecCap;
+    int32_t loopPools, loopNTargets, loopRing, loopRecCap, loopNCells;
     float loopBase, loopGain;
     const int32_t *driven;         // [nDriven]
     const int32_t *drvPos;         // [N]

@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int
 }
 
 // The per-lane pool loop (cordens.bend:227-273) after a step: each lane's
-// spikes of the step tallied by pool (from the slot's list and masks), the
+// spikes of the step tallied by pool (from the pool cells' masks), the
 // counts recorded, and each lane's drive of the loop's targets for the next
 // step set to base + gain * (its pools' total loopDelay[b] steps back), in
 // binary32 (0 before the first step).
@@ -301,12 +301,11 @@ __global__ void __launch_bounds__(256) kb_pool_loop(BatchDev d, int32_t slot) {
     const int tid = threadIdx.x;
     for (int q = tid; q < BMAX * 8; q += 256) cnt[q >> 3][q & 7] = 0u;
     __syncthreads();
-    const int32_t n = d.nActive[slot];
-    const int32_t *list = d.active + (int64_t)slot * d.N;
-    for (int32_t q = tid; q < n; q += 256) {
-        const int32_t i = list[q];
+    // every neuron's mask of the step is written (zero when silent), so the
+    // pools' own cells are read rather than the step's whole list
+    for (int32_t q = tid; q < d.loopNCells; q += 256) {
+        const int32_t i = d.loopCells[q];
         const uint32_t p = d.poolOf[i];
-        if ((int32_t)p >= d.loopPools) continue;
         uint64_t m = d.mask[(int64_t)slot * d.N + i];
         while (m) {
             const int b = __ffsll((unsigned long long)m) - 1;
@@ -686,6 +685,10 @@ extern "C" fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *pool
     for (int32_t q = 0; q < nTargets; q++)
         FB_REQUIRE_B(targets[q] >= 0 && targets[q] < d.N, FB_ERR_ARG, "fb_batch_set_pool_feedback: target %d out of range", q);
     const int32_t ring = maxDelay + 1;
+    std::vector<int32_t> cells;
+    for (int32_t i = 0; i < d.N; i++)
+        if ((int32_t)poolOf[i] < nPools) cells.push_back(i);
+    int32_t *cl;
     uint8_t *pool;
     int32_t *tg, *dl, *state;
     uint32_t *hist;
@@ -693,8 +696,10 @@ extern "C" fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *pool
     fb_status st;
     if ((st = balloc(b, &pool, (size_t)d.N)) || (st = balloc(b, &tg, (size_t)std::max(nTargets, 1))) ||
         (st = balloc(b, &dl, (size_t)d.B)) || (st = balloc(b, &state, 2)) || (st = balloc(b, &hist, (size_t)d.B * ring)) ||
-        (st = balloc(b, &rec, (size_t)d.B * std::max(recordCap, 1) * nPools)))
+        (st = balloc(b, &rec, (size_t)d.B * std::max(recordCap, 1) * nPools)) ||
+        (st = balloc(b, &cl, std::max<size_t>(cells.size(), 1))))
         return st;
+    if (!cells.empty()) FB_HIP(hipMemcpy(cl, cells.data(), sizeof(int32_t) * cells.size(), hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(pool, poolOf, (size_t)d.N, hipMemcpyHostToDevice));
     if (nTargets) FB_HIP(hipMemcpy(tg, targets, sizeof(int32_t) * nTargets, hipMemcpyHostToDevice));
     FB_HIP(hipMemcpy(dl, delay, sizeof(int32_t) * d.B, hipMemcpyHostToDevice));
@@ -702,7 +707,7 @@ extern "C" fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *pool
     FB_HIP(hipMemset(hist, 0, sizeof(uint32_t) * (size_t)d.B * ring));
     d.poolOf = pool; d.loopTargets = tg; d.loopDelay = dl; d.loopState = state; d.loopHist = hist; d.loopRec = rec;
     d.loopPools = nPools; d.loopNTargets = nTargets; d.loopRing = ring; d.loopRecCap = recordCap;
-    d.loopBase = base; d.loopGain = gain;
+    d.loopBase = base; d.loopGain = gain; d.loopCells = cl; d.loopNCells = (int32_t)cells.size();
     return FB_OK;
 }
 
