@@ -1002,16 +1002,29 @@ extern "C" fb_status fb_set_rng(fb_brain *b, uint32_t rng) {
     return FB_OK;
 }
 
+// frees one of the brain's own device buffers before its end of life
+static void free_owned(fb_brain *b, const void *p) {
+    if (!p) return;
+    auto it = std::find(b->allocs.begin(), b->allocs.end(), (void *)p);
+    if (it == b->allocs.end()) return;
+    b->allocs.erase(it);
+    (void)hipFree((void *)p);
+}
+
 extern "C" fb_status fb_set_pool_feedback(fb_brain *b, const uint8_t *poolOf, int32_t nPools, const int32_t *targets,
                                           int32_t nTargets, float base, float gain, int32_t delay,
                                           int32_t recordCap) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_set_pool_feedback: null brain");
     FB_HIP(hipStreamSynchronize(b->stream));
-    if (!poolOf) {
-        b->d.poolOf = nullptr;
+    if (b->d.poolOf) {                 // the previous loop's buffers
+        DevBrain &o = b->d;
+        free_owned(b, o.poolOf); free_owned(b, o.loopTargets); free_owned(b, o.loopState);
+        free_owned(b, o.loopHist); free_owned(b, o.loopRec);
+        o.poolOf = nullptr; o.loopTargets = nullptr; o.loopState = nullptr; o.loopHist = nullptr; o.loopRec = nullptr;
+        drop_graphs(b);                // captured launches hold the freed pointers
         b->version++;
-        return FB_OK;
     }
+    if (!poolOf) return FB_OK;
     FB_REQUIRE(nPools >= 1 && nPools <= 8 && nTargets >= 0 && (nTargets == 0 || targets) && delay >= 1 &&
                    recordCap >= 0,
                FB_ERR_ARG, "fb_set_pool_feedback: bad arguments");

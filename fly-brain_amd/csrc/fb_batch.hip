@@ -673,7 +673,17 @@ extern "C" fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *pool
     FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_pool_feedback: null ensemble");
     BatchDev &d = b->d;
     FB_HIP(hipStreamSynchronize(b->stream));
-    if (!poolOf) { d.poolOf = nullptr; return FB_OK; }
+    if (d.poolOf) {                    // the previous loop's buffers
+        for (const void *p : {(const void *)d.poolOf, (const void *)d.loopTargets, (const void *)d.loopDelay,
+                              (const void *)d.loopState, (const void *)d.loopHist, (const void *)d.loopRec,
+                              (const void *)d.loopCells}) {
+            auto it = std::find(b->allocs.begin(), b->allocs.end(), (void *)p);
+            if (it != b->allocs.end()) { b->allocs.erase(it); (void)hipFree((void *)p); }
+        }
+        d.poolOf = nullptr; d.loopTargets = d.loopDelay = d.loopCells = nullptr;
+        d.loopState = nullptr; d.loopHist = nullptr; d.loopRec = nullptr;
+    }
+    if (!poolOf) return FB_OK;
     FB_REQUIRE_B(d.laneDrive, FB_ERR_ARG, "fb_batch_set_pool_feedback: set per-lane drive first (fb_batch_set_lanes)");
     FB_REQUIRE_B(nPools >= 1 && nPools <= 8 && nTargets >= 0 && (nTargets == 0 || targets) && delay && recordCap >= 0,
                  FB_ERR_ARG, "fb_batch_set_pool_feedback: bad arguments");

@@ -63,3 +63,27 @@ def test_all_runs_as_lanes_reproduce_published_readings():
     bad = [(mc, ens.mean_hz(x), want[mc]) for mc, x in got.items() if ens.mean_hz(x) != want[mc]]
     assert not bad, bad[:10]
     e.release()
+
+
+def test_loop_set_twice_runs_like_once():
+    """Setting the pool loop again frees the old loop and its captured step
+    graphs; the run then equals the checker's."""
+    z, p, want = _fixture()
+    e = ens.CordEnsemble(z, p)
+    m, c = 8, 0
+    b = e.start(m, c, steps=200)
+    b.pool_counts()
+    cfg = ens.config(m, c)
+    b.reset()
+    b.set_rng(cfg["seed"])
+    b.set_pool_feedback(e.pool, e.proprio, 20.0, float(np.float32(0.6)), cfg["pdelay"], 1000, n_pools=6)
+    drive = np.zeros(e.N, np.float32)
+    drive[e.role == 2] = cfg["cmd"]
+    drive[e.is_prop] = np.float32(20.0)
+    b.set_array("drive", drive)
+    print("cord ensemble: loop set twice", flush=True)
+    b.run_async(1000)
+    got = b.pool_counts()
+    assert np.array_equal(got, orc.cord_lane(z, p, m, c, 1000))
+    b.destroy()
+    e.release()
