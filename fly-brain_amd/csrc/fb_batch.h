@@ -58,6 +58,7 @@ __device__ inline float ordered_sum_chain(float g, int32_t count, uint64_t key0,
 }
 
 constexpr int BMAX = 64;
+constexpr int DRV_WG = 4;          // workgroups of the multi-workgroup draws
 
 struct BatchDev {
     int32_t N, B, nslots, nDriven, nTypes;
@@ -92,6 +93,12 @@ struct BatchDev {
     const float *gain;             // [B][nTypes]
     const float *gainT;            // [nTypes][B], the same values by type
     uint32_t *rng;                 // [B]
+    // the draws over several workgroups (no background events): each wave's
+    // non-refractory count per lane, and the stream state after the step,
+    // handed to kb_membrane
+    int32_t *drvPart;              // [DRV_WG * 16][64]
+    uint32_t *rngNext;             // [B]
+    int32_t multiDrv;              // this step's draws took the multi-workgroup path
     float *bgAcc;                  // [1] (the same for every member)
     int32_t *steps;                // [B] run length of each member
     // delay line: per slot, spike and forced masks per neuron and a list of

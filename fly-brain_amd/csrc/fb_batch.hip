@@ -214,10 +214,92 @@ __global__ void __launch_bounds__(1024) kb_driven(BatchDev d, int32_t slot, int3
     }
 }
 
+// The draws over DRV_WG workgroups when there are no background events:
+// kb_driven_count writes each wave's non-refractory count per lane (its
+// slice of the driven list), then kb_driven_multi starts every wave at its
+// stream position (the counts of the waves before it) with the leapfrog
+// jump, draws its slice as kb_driven does, and leaves the state after the
+// step in rngNext for kb_membrane to install (every wave reads rng first).
+__global__ void __launch_bounds__(1024) kb_driven_count(BatchDev d, int32_t t) {
+    constexpr int W = 16, CH = 32;
+    const int tid = threadIdx.x, lane = tid & 63, gw = blockIdx.x * W + (tid >> 6);
+    const bool on = lane < d.B && t < d.steps[lane];
+    const int32_t nD = d.nDriven, GW = gridDim.x * W;
+    const int32_t per = (nD + GW - 1) / GW;
+    const int32_t lo = min(nD, gw * per), hi = min(nD, lo + per);
+    int32_t mine = 0;
+    if (on)
+        for (int32_t c0 = lo; c0 < hi; c0 += CH) {
+            float r[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++) r[u] = c0 + u < hi ? d.refr[(int64_t)d.driven[c0 + u] * d.B + lane] : 1.f;
+#pragma unroll
+            for (int u = 0; u < CH; u++) mine += !(r[u] > 0) ? 1 : 0;
+        }
+    d.drvPart[gw * 64 + lane] = mine;
+}
+
+__global__ void __launch_bounds__(1024) kb_driven_multi(BatchDev d, int32_t slot, int32_t t) {
+    constexpr int T = 1024, W = T / 64, CH = 32;
+    __shared__ uint32_t leapS[LEAP_LDS_WORDS];
+    const int tid = threadIdx.x, lane = tid & 63, gw = blockIdx.x * W + (tid >> 6);
+    leap_load<T>(leapS, d.leap);
+    __syncthreads();
+    const bool on = lane < d.B && t < d.steps[lane];
+    const uint32_t x0 = on ? d.rng[lane] : 0u;
+    const int32_t nD = d.nDriven, GW = gridDim.x * W, B = d.B;
+    const int32_t per = (nD + GW - 1) / GW;
+    const int32_t lo = min(nD, gw * per), hi = min(nD, lo + per);
+    int32_t off = 0, total = 0;
+    for (int32_t w = 0; w < GW; w++) {
+        const int32_t c = d.drvPart[w * 64 + lane];
+        off += w < gw ? c : 0;
+        total += c;
+    }
+    uint32_t x = x0;
+    if (on && off > 0) {
+        const uint32_t y = matvec(leapS + (32 + (off >> 5)) * LEAP_STRIDE, x0);
+        x = (off & 31) ? matvec(leapS + ((off & 31) - 1) * LEAP_STRIDE, y) : y;
+    }
+    for (int32_t c0 = lo; c0 < hi; c0 += CH) {
+        float r[CH], pf[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const bool in = on && c0 + u < hi;
+            const int32_t i = c0 + u < hi ? d.driven[c0 + u] : 0;
+            r[u] = in ? d.refr[(int64_t)i * B + lane] : 1.f;
+            pf[u] = in ? (d.laneDrive ? d.laneDrive[(int64_t)i * B + lane] : d.drive[i]) * d.k.dtS : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            if (c0 + u >= hi) break;
+            const int32_t i = d.driven[c0 + u];
+            bool f = false;
+            if (!(r[u] > 0)) {
+                x = xs32(x);
+                const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
+                f = uu < pf[u];
+            }
+            const uint64_t fb = __ballot(f);
+            if (lane == 0) {
+                d.fmask[(int64_t)slot * d.N + i] = fb;
+                d.frank[(int64_t)slot * d.N + i] = c0 + u;
+            }
+        }
+    }
+    if (gw == 0 && on) {
+        const uint32_t y = matvec(leapS + (32 + (total >> 5)) * LEAP_STRIDE, x0);
+        d.rngNext[lane] = (total & 31) ? matvec(leapS + ((total & 31) - 1) * LEAP_STRIDE, y) : y;
+    }
+}
+
 // ------------------------------------------------------------------ membrane + threshold
 
 __global__ void __launch_bounds__(256) kb_membrane(BatchDev d, int32_t slot, int32_t head, int32_t t) {
     const int lane = threadIdx.x & 63;
+    // the multi-workgroup draws left the stream state after the step
+    if (d.multiDrv && blockIdx.x == 0 && threadIdx.x < d.B && t < d.steps[threadIdx.x])
+        d.rng[threadIdx.x] = d.rngNext[threadIdx.x];
     const int32_t i = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (i >= d.N) return;   // whole wave
     const int32_t B = d.B, N = d.N;
@@ -475,7 +557,8 @@ extern "C" fb_status fb_batch_create(fb_graph *g, const fb_params *p, int32_t B,
         (st = balloc(b, &active, (size_t)d.nslots * N)) || (st = balloc(b, &nActive, (size_t)d.nslots)) ||
         (st = balloc(b, &cnt, 2 * NB)) || (st = balloc(b, &chain, 2 * NB)) || (st = balloc(b, &add, 2 * NB)) ||
         (st = balloc(b, &key, 2 * NB)) || (st = balloc(b, &bgCnt, NB)) || (st = balloc(b, &ncoll, 1)) ||
-        (st = balloc(b, &err, 1)) || (st = balloc(b, &counters, 4)) || (st = balloc(b, &b->idxScratch, (size_t)N)))
+        (st = balloc(b, &err, 1)) || (st = balloc(b, &counters, 4)) || (st = balloc(b, &b->idxScratch, (size_t)N)) ||
+        (st = balloc(b, &d.drvPart, (size_t)DRV_WG * 16 * 64)) || (st = balloc(b, &d.rngNext, (size_t)BMAX)))
         return st;
     d.collCap = std::max<int64_t>(dg->E * 4, 1);
     if ((st = balloc(b, &coll, (size_t)d.collCap))) return st;
@@ -588,7 +671,13 @@ static fb_status batch_step(fb_batch *b) {
     const int32_t head = b->head, slot = (head + d.nslots - 1) % d.nslots;
     hipStream_t st = b->stream;
     hipLaunchKernelGGL(kb_deliver, dim3(256), dim3(256), 0, st, d, head, b->t);
-    hipLaunchKernelGGL(kb_driven, dim3(1), dim3(1024), 0, st, d, slot, b->t);
+    d.multiDrv = !d.liveBg && d.k.bgEvents == 0.f && d.nDriven > 256 && d.drvPart ? 1 : 0;
+    if (d.multiDrv) {
+        hipLaunchKernelGGL(kb_driven_count, dim3(DRV_WG), dim3(1024), 0, st, d, b->t);
+        hipLaunchKernelGGL(kb_driven_multi, dim3(DRV_WG), dim3(1024), 0, st, d, slot, b->t);
+    } else {
+        hipLaunchKernelGGL(kb_driven, dim3(1), dim3(1024), 0, st, d, slot, b->t);
+    }
     hipLaunchKernelGGL(kb_membrane, dim3((d.N + 3) / 4), dim3(256), 0, st, d, slot, head, b->t);
     if (d.poolOf) hipLaunchKernelGGL(kb_pool_loop, dim3(1), dim3(256), 0, st, d, slot);
     FB_HIP(hipGetLastError());
