@@ -43,8 +43,42 @@ def algorithmic_bytes(N: int, edge_events: int, spikes: int, steps: int, live_ad
     return per_neuron * N * steps + 8 * edge_events + 4 * spikes
 
 
-def cpu_baseline(g, p, drv, rates, steps: int, warm: int):
-    """lif.c (or its restatement) on one core over `steps` steps after `warm`."""
+def cpu_env() -> dict:
+    """The host the CPU legs run on: CPU model, logical CPUs, and the CPUs
+    this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else []
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": len(aff) or os.cpu_count()}
+
+
+class pinned:
+    """Pins the calling thread to one CPU (the first it may use) for a 1-core
+    baseline, and restores its mask afterwards."""
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0) if hasattr(os, "sched_setaffinity") else None
+        self.cpu = min(self.old) if self.old else None
+        if self.old:
+            os.sched_setaffinity(0, {self.cpu})
+        return self
+
+    def __exit__(self, *exc):
+        if self.old:
+            os.sched_setaffinity(0, self.old)
+        return False
+
+
+def cpu_baseline(g, p, drv, rates, steps: int, warm: int, repeats: int = 3):
+    """lif.c (or its restatement) on one pinned core: `warm` untimed steps to
+    the same steady state the GPU is timed in, then `repeats` timed samples of
+    `steps` steps; the median sample is reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
@@ -55,24 +89,38 @@ def cpu_baseline(g, p, drv, rates, steps: int, warm: int):
     for i, r in zip(drv.tolist(), rates.tolist()):
         b.setDriveOne(i, r)
     outdeg = g.outdeg
-    nf_w, ras_w, _ = b.run(warm, raster_cap=warm * 4096)
-    t0 = time.perf_counter()
-    nf, ras, _ = b.run(steps, raster_cap=steps * 4096)
-    dt = time.perf_counter() - t0
-    # arrivals in the timed window: spikes emitted D steps earlier
     D = b.nslots - 1
-    nf_all = np.concatenate([nf_w, nf])
-    ras_all = np.concatenate([ras_w, ras])
-    start = np.concatenate([[0], np.cumsum(nf_all)])
-    lo, hi = max(0, warm - D), max(0, warm + steps - D)
-    events = int(outdeg[ras_all[start[lo]:start[hi]]].sum())
+    nf_all, ras_all = [], []
+    nf_w, ras_w, _ = b.run(warm, raster_cap=warm * 4096)
+    nf_all.append(nf_w)
+    ras_all.append(ras_w)
+    samples = []
+    with pinned() as pin:
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            nf, ras, _ = b.run(steps, raster_cap=steps * 4096)
+            samples.append(time.perf_counter() - t0)
+            nf_all.append(nf)
+            ras_all.append(ras)
+    nf_cat = np.concatenate(nf_all)
+    ras_cat = np.concatenate(ras_all)
+    start = np.concatenate([[0], np.cumsum(nf_cat)])
+    order = np.argsort(samples)
+    k = int(order[len(order) // 2])                       # the median sample
+    lo, hi = warm + k * steps - D, warm + (k + 1) * steps - D
+    events = int(outdeg[ras_cat[start[lo]:start[hi]]].sum())
+    dt = samples[k]
     return {
         "value": events / dt, "unit": "synaptic events/s", "cores": 1,
         "kind": "reference" if kind == "ref" else "port",
-        "sample": f"{steps} steps (after {warm} warm-up) of the same 165,122-neuron workload, "
-                  f"{'lif.c compiled natively (gcc -O2 -ffp-contract=off)' if kind == 'ref' else 'oracle/lif_oracle.c'}, 1 thread",
+        "sample": f"median of {repeats} samples of {steps} steps each, after {warm} untimed steps to steady "
+                  "state, of the same 165,122-neuron workload; "
+                  f"{'lif.c compiled natively (gcc -O2 -ffp-contract=off)' if kind == 'ref' else 'oracle/lif_oracle.c'}, "
+                  f"1 thread pinned to CPU {pin.cpu}",
         "sim_s_per_wall_s": steps * p["dt"] / 1000.0 / dt,
         "ms_per_step": dt * 1000.0 / steps,
+        "samples_ms_per_step": [x * 1000.0 / steps for x in samples],
+        **cpu_env(),
     }
 
 
@@ -108,7 +156,8 @@ def cpu_batch_baseline(g, p, drv, type_of, gains, silenced, steps: int, members:
         events += int(outdeg[ras[: start[max(0, steps - D)]]].sum())
     return {"value": events / dt, "unit": "synaptic events/s", "cores": cores,
             "kind": "reference" if kind == "ref" else "port",
-            "sample": f"{members} members x {steps} steps of the 64-member ensemble workload, one member per thread"}
+            "sample": f"{members} members x {steps} steps of the 64-member ensemble workload, one member per thread",
+            **cpu_env()}
 
 
 def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
@@ -165,17 +214,24 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     out["plain_single_value"] = plain
     out["plain_single_ms_per_step"] = ms1 / 10000
     out["vs_plain_single"] = out["value"] / plain
+    one.destroy()
+    hg.release()
+    for h in (e0, e1):
+        h.destroy()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_batch_baseline(g, p, drv, type_of, gains, silenced, min(args.cpu_steps, 2000), B)
     return out
 
 
-def traffic_per_step():
-    """traffic_bytes_per_step from the committed counter summary, or None."""
+def traffic_per_launch(steps: int):
+    """HBM bytes of one k_persist launch of `steps` steps, from the committed
+    counter summary (a line `traffic_bytes_per_launch_<steps>_steps: B`), or
+    None when no launch of that length was counted."""
     path = os.path.join(ROOT, "profiles", "k_persist_hbm_counters.md")
+    key = f"traffic_bytes_per_launch_{steps}_steps:"
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_step:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -301,7 +357,49 @@ def bench_adjoint_batch(args, synth, P, diff, hiprt):
            "workload": "32 instances of the forward + adjoint workload with their own gain perturbations, "
                        "drive seeds and windows of 500-2,000 steps, type-gain gradients reduced over instances"}
     bt.destroy()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_adjoint_batch_baseline(args, g, p, sign, types, setup, gains0, K, windows, seeds)
     return out
+
+
+def cpu_adjoint_batch_baseline(args, g, p, sign, types, setup, gains0, K, windows, seeds):
+    """The batched forward + adjoint on the host: the C restatement of LIFDiff,
+    one instance per host core (the reference's one-network-per-worker), each
+    instance set up as the GPU's instance b, over a bounded window."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import concurrent.futures as cf
+    import numpy as np
+    import oracle as orc
+    P = pkg("params")
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    n_inst = min(cores, len(windows))
+    steps = [min(w, args.adjoint_batch_cpu_steps) for w in windows[:n_inst]]
+    insts = []
+    for b in range(n_inst):
+        o = setup(orc.OracleDiff(g.N, g.indptr, g.indices, g.counts.astype(np.float32), sign, {**p, "checkpoint": K}))
+        gb = (gains0.astype(np.float64) * np.exp(np.random.default_rng(400 + b).normal(0, 0.05, gains0.size)))
+        o.set_input("outScale", gb.astype(np.float32)[types])
+        insts.append(o)
+
+    def one(b):
+        insts[b].forward(steps[b], seed=seeds[b])
+        insts[b].backward()
+        return b
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=n_inst) as ex:
+        list(ex.map(one, range(n_inst)))
+    dt = time.perf_counter() - t0
+    D = P.nslots(p) - 1
+    ev = 0
+    for b, o in enumerate(insts):
+        off, idx, amp, forced = o.spikes()
+        ev += int(g.outdeg[idx[: off[max(0, steps[b] - D)]]].sum())
+    return {"value": ev / dt, "unit": "synaptic events/s (all instances)", "cores": n_inst, "kind": "port",
+            "sample": f"instances 0-{n_inst - 1} of the batched workload, each forward + adjoint over its first "
+                      f"{args.adjoint_batch_cpu_steps} steps (or its window when shorter), one instance per thread, "
+                      "the C restatement of LIFDiff (gcc -O2)",
+            "member_steps": int(sum(steps)), "wall_s": dt, **cpu_env()}
 
 
 def bench_cord(args, lif, hiprt, stream):
@@ -343,6 +441,8 @@ def bench_cord(args, lif, hiprt, stream):
            "spikes_per_step": (c1["spikes"] - c0["spikes"]) / steps}
     b.destroy()
     g.release()
+    for h in (e0, e1):
+        h.destroy()
     return out
 
 
@@ -389,8 +489,10 @@ def bench_brain(args, lif, hiprt, stream):
         if seed == 1:
             ev1 = c1["edge_events"] - c0["edge_events"]
         totals.append(int(net.spikeCount.sum()))
-        engines.add("persistent" if net.stat("persistent_launches") > 0 else args.engine)
+        engines.add("persistent" if net.stat("persistent_launches") > 0 else "per-step")
         net.destroy()
+        for h in (e0, e1, e2, e3):
+            h.destroy()
     g.release()
     steps = 4 * 1400
     out = {"workload": "the reference's whole calibrated brain from its data files: 165,122 neurons, 10,511,038 "
@@ -584,7 +686,7 @@ def bench_visual(args):
     return out
 
 
-def bench_cord_ensemble(args):
+def bench_cord_ensemble(args, brain_per_run: bool = True):
     """The cord lane ensemble (cordens.bend): 64 members x 6 perturbations of
     the reference's ventral-cord brain, 384 runs of 1,000 steps with the
     proprioceptive loop closed on the device; every run's meanHz is checked
@@ -615,12 +717,112 @@ def bench_cord_ensemble(args):
            "runs": len(got), "wall_s": wall, "runs_per_s": len(got) / wall,
            "steps_per_s": len(got) * 1000 / wall, "meanHz_matches_published": match,
            "reference_wall_s_12_cores": 29.5, "speedup_vs_reference": 29.5 / wall}
+    if not brain_per_run:
+        e.release()
+        return out
     e.run(ens.run_order()[:8])                       # warm-up of the one-brain-per-run path
     t0 = time.perf_counter()
     got1 = e.run()
     out["brain_per_run_wall_s"] = time.perf_counter() - t0
     out["brain_per_run_matches_lanes"] = all(np.array_equal(got1[mc], got[mc]) for mc in got)
     e.release()
+    return out
+
+
+def bench_config_run(args, lif, hiprt, stream, hg, p, drv, rates, g):
+    """The configuration's whole run as one call: a fresh brain from rest,
+    10,000 steps (1 s simulated), on the automatic engine."""
+    b = lif.LIFHip(hg, p, seed=1)
+    for i, r in zip(drv.tolist(), rates.tolist()):
+        b.setDriveOne(i, r)
+    b.set_stream(stream.handle)
+    b.advance(0)                          # the driven list is uploaded outside the timed call
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    n = args.config_steps
+    c0 = b.counters()
+    e0.record(stream)
+    b.run_async(n)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    c1 = b.counters()
+    ev = c1["edge_events"] - c0["edge_events"]
+    out = {"steps": n, "ms": ms, "ms_per_step": ms / n, "edge_events_per_s": ev / (ms / 1e3),
+           "sim_s_per_wall_s": n * p["dt"] / 1e3 / (ms / 1e3),
+           "spikes_per_step": (c1["spikes"] - c0["spikes"]) / n,
+           "launches": int(b.stat("persistent_launches")), "kernel_ms": b.stat("persistent_kernel_ms"),
+           "fallback_steps": int(b.stat("fallback_steps")),
+           "engine": "persistent" if b.stat("persistent_launches") > 0 else "per-step",
+           "workload": "the same brain from rest: all 10,000 steps of the configuration in one call"}
+    b.destroy()
+    for h in (e0, e1):
+        h.destroy()
+    return out
+
+
+def bench_coba(args, lif, hiprt, stream, hg, P, drv, rates, g):
+    """The conductance-based variant of the full-scale forward (E_exc 0 mV,
+    E_inh -80 mV), timed in steady state on the automatic engine."""
+    p = P.config_params(coba=True)
+    b = lif.LIFHip(hg, p, seed=1)
+    for i, r in zip(drv.tolist(), rates.tolist()):
+        b.setDriveOne(i, r)
+    b.set_stream(stream.handle)
+    b.advance(args.prerun)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    n = args.coba_steps
+    c0 = b.counters()
+    l0 = b.stat("persistent_launches")
+    e0.record(stream)
+    b.run_async(n)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    c1 = b.counters()
+    ev = c1["edge_events"] - c0["edge_events"]
+    out = {"steps": n, "ms_per_step": ms / n, "edge_events_per_s": ev / (ms / 1e3),
+           "sim_s_per_wall_s": n * p["dt"] / 1e3 / (ms / 1e3), "spikes_per_step": (c1["spikes"] - c0["spikes"]) / n,
+           "engine": "persistent" if b.stat("persistent_launches") > l0 else "per-step",
+           "workload": f"the full-scale forward, COBA (E_exc 0 mV, E_inh -80 mV), {n} steps after {args.prerun} "
+                       "untimed"}
+    b.destroy()
+    for h in (e0, e1):
+        h.destroy()
+    return out
+
+
+def bench_engines(args, lif, hiprt, stream, hg, p, drv, rates):
+    """Every engine on the same steady-state workload: a long call (whole slot
+    cycles, so the graph engine replays graphs captured beforehand) and a
+    call of the timed length.  Graph capture and instantiation happen in
+    untimed calls before each measurement."""
+    out = {}
+    long_n = 76 * 40                      # 3,040 steps: whole 76-step graph replays, same head before and after
+    for eng in ("persistent", "graph", "launch"):
+        b2 = lif.LIFHip(hg, p, seed=1, engine=eng)
+        for i, r in zip(drv.tolist(), rates.tolist()):
+            b2.setDriveOne(i, r)
+        b2.set_stream(stream.handle)
+        b2.advance(args.prerun, engine="auto")
+        a0, a1 = hiprt.Event(), hiprt.Event()
+        b2.advance(long_n)                  # captures every graph the timed call replays
+        a0.record(stream)
+        b2.run_async(long_n)
+        a1.record(stream)
+        a1.synchronize()
+        res = {"ms_per_step": a0.elapsed_ms(a1) / long_n, "steps": long_n}
+        k = args.steps
+        for _ in range(b2.nslots):          # every start head of a k-step call, captured
+            b2.advance(k)
+        a0.record(stream)
+        b2.run_async(k)
+        a1.record(stream)
+        a1.synchronize()
+        res[f"ms_per_step_{k}_step_call"] = a0.elapsed_ms(a1) / k
+        out[eng] = res
+        b2.destroy()
+        for h in (a0, a1):
+            h.destroy()
     return out
 
 
@@ -631,7 +833,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--engine", default="auto", choices=["auto", "launch", "graph", "persistent"])
     ap.add_argument("--coba", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=10000)
+    ap.add_argument("--prerun", type=int, default=2000,
+                    help="untimed steps from rest to steady state before the warm-up (also the CPU baseline's)")
+    ap.add_argument("--config-steps", type=int, default=10000, help="steps of the whole-run leg")
+    ap.add_argument("--coba-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=3000, help="steps per CPU baseline sample (median of 3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -641,7 +847,9 @@ def main():
                          "scaling HBM counters), then exit")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
-    ap.add_argument("--only", choices=["flyvis", "visual", "cordens"], help="run one side leg alone and print its JSON")
+    ap.add_argument("--adjoint-batch-cpu-steps", type=int, default=400)
+    ap.add_argument("--only", choices=["headline", "flyvis", "visual", "cordens", "lanes", "brain"],
+                    help="run one leg alone and print its JSON (headline: the steady-state forward only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -667,6 +875,15 @@ def main():
     if args.only == "cordens":
         print(json.dumps({"cord_ensemble": bench_cord_ensemble(args)}), flush=True)
         return
+    if args.only == "lanes":
+        print(json.dumps({"cord_ensemble_lanes": bench_cord_ensemble(args, brain_per_run=False)}), flush=True)
+        return
+    if args.only == "brain":
+        lif, hiprt = pkg("lif"), pkg("hiprt")
+        st = hiprt.Stream()
+        print(json.dumps({"real_brain": bench_brain(args, lif, hiprt, st)}), flush=True)
+        st.destroy()
+        return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
     t_build = time.perf_counter()
@@ -682,13 +899,21 @@ def main():
           file=sys.stderr, flush=True)
 
     stream = hiprt.Stream()
+    e0, e1 = hiprt.Event(), hiprt.Event()
     brain.set_stream(stream.handle)
+    # untimed: the network from rest to its steady state (the first arrivals
+    # land D = 18 steps in; the rates settle within a few hundred steps)
+    c_pre = brain.counters()
+    w_pre = time.perf_counter()
+    brain.advance(args.prerun, engine="auto")
+    prerun = {"steps": args.prerun, "wall_ms": (time.perf_counter() - w_pre) * 1e3,
+              "spikes_per_step": (brain.counters()["spikes"] - c_pre["spikes"]) / max(1, args.prerun),
+              "engine": "persistent" if brain.stat("persistent_launches") > 0 else "per-step"}
     brain.advance(args.warmup)
     c0 = brain.counters()
     k0 = brain.stat("persistent_kernel_ms")
     l0 = brain.stat("persistent_launches")
     f0 = brain.stat("fallback_steps")
-    e0, e1 = hiprt.Event(), hiprt.Event()
     hiprt.device_synchronize()
     w0 = time.perf_counter()
     e0.record(stream)
@@ -716,24 +941,21 @@ def main():
     secs = ms / 1000.0
     value = events / secs
     bytes_ = algorithmic_bytes(g.N, events, spikes, args.steps)
-    engine = args.engine
+    engine = "persistent" if launches else ("launch" if args.engine == "auto" else args.engine)
     if launches and not fallback:
         # the lookahead kernel is the only kernel of the timed region: its own
         # HIP-event duration, averaged over launches
         kern_s = kms / 1000.0
         kernel_name = "k_persist (lookahead engine)"
-        engine = "persistent"
     else:
         kern_s = secs
-        kernel_name = "per-step kernel sequence (k_deliver, k_driven, k_membrane, k_compact)"
+        kernel_name = "per-step kernel sequence (k_deliver_driven, k_long, k_membrane, k_compact)"
     achieved = bytes_ / kern_s / 1e9
     traffic = None
     if launches == 1 and not fallback:
-        # measured HBM bytes per step of this kernel (copy-calibrated counter
-        # passes), scaled to the timed launch
-        per_step = traffic_per_step()
-        if per_step:
-            traffic = per_step * args.steps
+        # measured HBM bytes of this kernel for a launch of the same length
+        # (copy-calibrated counter passes)
+        traffic = traffic_per_launch(args.steps)
     out = {
         "metric": "synaptic events/s, full 165k LIF: plain vs batch-64 vs fwd+adjoint",
         "value": value,
@@ -749,54 +971,54 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "full-scale forward: 165,122 neurons, ~8.0M edges, ~104M synapses, "
-                        f"{'COBA' if args.coba else 'CUBA'}, dt 0.1 ms, 1% driven at U(100,200) Hz",
+                        f"{'COBA' if args.coba else 'CUBA'}, dt 0.1 ms, 1% driven at U(100,200) Hz; timed in steady "
+                        f"state after an untimed {args.prerun}-step run from rest",
             "neurons": g.N, "edges": g.E, "synapses": int(g.counts.sum()), "engine": engine,
         },
+        "steady_state_prerun": prerun,
         "sim_s_per_wall_s": args.steps * p["dt"] / 1000.0 / secs,
         "synapse_events_per_s": syn / secs,
         "spikes_per_step": spikes / args.steps,
+        "edge_events_per_step": events / args.steps,
         "mean_rate_hz": spikes / args.steps / g.N / (p["dt"] / 1000.0),
         "wall_s_host_clock": wall,
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "traffic_source": "profiles/k_persist_hbm_counters.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of k_persist, "
-                              "each scaled by the counter's reading for a 256 MiB copy" if traffic else None,
+            "traffic_source": "profiles/k_persist_hbm_counters.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of the timed "
+                              "k_persist dispatch, each scaled by the counter's reading for a 256 MiB copy"
+                              if traffic else None,
             "kernel": kernel_name,
             "bytes_per_launch": bytes_ / max(1, launches),
             "launch_ms": kms / max(1, launches) if launches else secs * 1000.0,
             "launches": launches,
             "note": "algorithmic bytes = 32 N per step (v, gE, gI, refr read+written) + 8 per edge event + "
                     "4 per spike; the lookahead kernel keeps neuron state in registers across an epoch, "
-                    "so frac > 1 means on-chip reuse, not HBM traffic",
+                    "so its HBM traffic is far below the algorithmic bytes",
         },
     }
-    # the per-step engines on the same workload, for comparison
-    cmp = {}
-    for eng in () if args.no_engines else ("graph", "launch"):
-        b2 = lif.LIFHip(hg, p, seed=1, engine=eng)
-        for i, r in zip(drv.tolist(), rates.tolist()):
-            b2.setDriveOne(i, r)
-        b2.set_stream(stream.handle)
-        n2 = min(args.steps, 3000)
-        b2.advance(200)
-        a0, a1 = hiprt.Event(), hiprt.Event()
-        a0.record(stream)
-        b2.run_async(n2)
-        a1.record(stream)
-        a1.synchronize()
-        cmp[eng] = {"ms_per_step": a0.elapsed_ms(a1) / n2, "steps": n2}
-        b2.destroy()
+    if args.only == "headline":
+        out["lookahead_phases_us_total"] = phases
+        brain.destroy()
+        hg.release()
+        for h in (e0, e1, stream):
+            h.destroy()
+        print(json.dumps(out), flush=True)
+        return
+    out["config_run"] = bench_config_run(args, lif, hiprt, stream, hg, p, drv, rates, g)
+    out["coba"] = bench_coba(args, lif, hiprt, stream, hg, P, drv, rates, g)
     out["lookahead_phases_us_total"] = phases
     out["lookahead_fallback_steps"] = fallback
     out["lookahead_overflow_flags"] = overflow_flags
-    out["engines"] = {"persistent" if launches else engine: {"ms_per_step": ms / args.steps}, **cmp}
+    if not args.no_engines:
+        out["engines"] = bench_engines(args, lif, hiprt, stream, hg, p, drv, rates)
+    brain.destroy()
     cord = bench_cord(args, lif, hiprt, stream)
     if cord:
         out["real_cord"] = cord
-    brain = bench_brain(args, lif, hiprt, stream)
-    if brain:
-        out["real_brain"] = brain
+    rb = bench_brain(args, lif, hiprt, stream)
+    if rb:
+        out["real_brain"] = rb
     vis = bench_flyvis(args)
     if vis:
         out["flyvis"] = vis
@@ -812,10 +1034,16 @@ def main():
         out["fwd_adjoint"] = bench_adjoint(args, synth, P, diff, hiprt, stream)
         out["fwd_adjoint_batch32"] = bench_adjoint_batch(args, synth, P, diff, hiprt)
     if not args.no_cpu:
-        cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, 100)
+        cb = cpu_baseline(g, p, drv, rates, args.cpu_steps, args.prerun)
         out["cpu_baseline"] = cb
         if cb:
             out["gpu_over_cpu_sim_rate"] = out["sim_s_per_wall_s"] / cb["sim_s_per_wall_s"]
+            if out.get("config_run"):
+                out["config_run"]["gpu_over_cpu_sim_rate"] = (out["config_run"]["sim_s_per_wall_s"]
+                                                              / cb["sim_s_per_wall_s"])
+    hg.release()
+    for h in (e0, e1, stream):
+        h.destroy()
     print(json.dumps(out), flush=True)
 
 

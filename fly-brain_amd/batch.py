@@ -82,7 +82,8 @@ class LIFBatch:
         _l.check(self.lib.fb_batch_set_rng(self.h, _l.ptr(st)))
 
     def set_pool_feedback(self, pool_of, n_pools: int, targets, base: float, gain: float, delays, record: int):
-        pool = np.ascontiguousarray(pool_of, dtype=np.uint8)
+        # pool_of None clears the loop
+        pool = None if pool_of is None else np.ascontiguousarray(pool_of, dtype=np.uint8)
         tg = np.ascontiguousarray(targets, dtype=np.int32)
         dl = np.ascontiguousarray(delays, dtype=np.int32)
         self._n_pools = int(n_pools)

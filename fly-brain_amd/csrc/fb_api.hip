@@ -821,9 +821,11 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     }
     const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
-        // the lookahead engine pays a prologue per call: worth it from two epochs on
+        // the lookahead engine pays a prologue per call (buffer clears and one
+        // cooperative launch); from one epoch of steps on it is ahead of a
+        // launch or a graph replay per step
         std::string why;
-        const bool longRun = nsteps >= 2 * (b->d.nslots - 1);
+        const bool longRun = nsteps >= b->d.nslots - 1;
         // otherwise per-step kernels: replayed as a graph for small brains,
         // whose kernels are shorter than their launch; launched directly for
         // large ones, whose kernels hide the launch and where a capture after
@@ -1075,6 +1077,12 @@ extern "C" fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head,
     if (head) *head = b->head;
     if (bg_acc) *bg_acc = sc.bgAcc;
     if (t_ms) *t_ms = b->t;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_brain_size(const fb_brain *b, int32_t *N) {
+    FB_REQUIRE(b && N, FB_ERR_ARG, "fb_brain_size: bad arguments");
+    *N = b->d.N;
     return FB_OK;
 }
 

@@ -473,6 +473,7 @@ struct fb_batch {
     std::vector<int32_t> typeOf;
     int32_t head = 0, lastSlot = 0, t = 0;
     int32_t *idxScratch = nullptr;
+    float *laneDriveBuf = nullptr;   // kept when lane drive is switched off, reused when it is set again
     int32_t *recCnt = nullptr, *recIds = nullptr;
     uint64_t *recMasks = nullptr, *recF = nullptr;
     int32_t engine = FB_ENGINE_AUTO;
@@ -729,20 +730,25 @@ static fb_status run_per_step(fb_batch *b, int32_t nsteps, int32_t *nactive, int
 extern "C" fb_status fb_batch_set_lanes(fb_batch *b, int32_t signGain, const float *laneDrive) {
     FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_lanes: null ensemble");
     BatchDev &d = b->d;
+    // the pool loop writes the per-lane drive after every step: it needs the buffer
+    FB_REQUIRE_B(laneDrive || !d.poolOf, FB_ERR_ARG,
+                 "fb_batch_set_lanes: the pool loop needs per-lane drive; clear it first (fb_batch_set_pool_feedback "
+                 "with no pools)");
     FB_HIP(hipStreamSynchronize(b->stream));
     d.signGain = signGain ? 1 : 0;
     if (laneDrive) {
         const int32_t N = d.N, B = d.B;
-        if (!d.laneDrive) {
-            fb_status st = balloc(b, &d.laneDrive, (size_t)N * B);
+        if (!b->laneDriveBuf) {
+            fb_status st = balloc(b, &b->laneDriveBuf, (size_t)N * B);
             if (st) return st;
         }
+        d.laneDrive = b->laneDriveBuf;
         std::vector<float> t((size_t)N * B);                 // [B][N] -> [N][B]
         for (int32_t q = 0; q < B; q++)
             for (int32_t i = 0; i < N; i++) t[(size_t)i * B + q] = laneDrive[(size_t)q * N + i];
         FB_HIP(hipMemcpy(d.laneDrive, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
     } else {
-        d.laneDrive = nullptr;          // (its buffer stays with the ensemble)
+        d.laneDrive = nullptr;          // the buffer stays in laneDriveBuf
     }
     return FB_OK;
 }

@@ -508,6 +508,21 @@ extern "C" fb_status fb_fv_tape_len(fb_fv *e, int32_t *n) {
     return FB_OK;
 }
 
+extern "C" fb_status fb_fv_model_size(const fb_fv_model *m, int32_t *N, int64_t *E) {
+    FB_REQUIRE(m && N && E, FB_ERR_ARG, "fb_fv_model_size: bad arguments");
+    *N = m->N;
+    *E = m->E;
+    return FB_OK;
+}
+
+extern "C" fb_status fb_fv_size(const fb_fv *e, int32_t *N, int64_t *E, int32_t *nCol) {
+    FB_REQUIRE(e && N && E && nCol, FB_ERR_ARG, "fb_fv_size: bad arguments");
+    *N = e->m->N;
+    *E = e->m->E;
+    *nCol = e->m->nCol;
+    return FB_OK;
+}
+
 static bool fv_grow(float **p, size_t *cap, size_t n) {
     if (n <= *cap) return true;
     (void)hipFree(*p);

@@ -1084,6 +1084,17 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
     if (q < nD) drvPos[driven[q]] = q;
 }
 
+// the launch's control words, barrier counters, clocks and bucket counts
+// cleared in one kernel (four fills would be four queued operations)
+__global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, int32_t *bkCnt, int32_t nCnt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n = gridDim.x * blockDim.x;
+    if (i < 4) ctl[i] = 0;
+    if (i < PERSIST_PROF) prof[i] = 0;
+    for (int32_t k = i; k < GBAR_WORDS; k += n) bar[k] = 0;
+    for (int32_t k = i; k < nCnt; k += n) bkCnt[k] = 0;
+}
+
 // destination block << 12 | local index, per edge: driven targets belong to
 // block 0 (index = driven-list position), others to block 1 + q / nb
 __global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos, int32_t nb) {
@@ -1170,10 +1181,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     while (done < nsteps) {
         a.head0 = cur;
         a.nsteps = nsteps - done;
-        FB_HIP(hipMemsetAsync(pb.ctl, 0, sizeof(int32_t) * 4, st));
-        FB_HIP(hipMemsetAsync(pb.bar, 0, sizeof(int32_t) * GBAR_WORDS, st));
-        FB_HIP(hipMemsetAsync(pb.prof, 0, sizeof(uint64_t) * PERSIST_PROF, st));
-        FB_HIP(hipMemsetAsync(pb.bkCnt, 0, sizeof(int32_t) * 2 * (size_t)lay.S * BK_STRIDE, st));
+        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.bkCnt,
+                           2 * lay.S * BK_STRIDE);
+        FB_HIP(hipGetLastError());
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         const hipError_t le = hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
@@ -1189,14 +1199,14 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         int32_t ctl[4];
+        uint64_t prof[PERSIST_PROF];
         FB_HIP(hipMemcpyAsync(ctl, pb.ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+        if (phaseUs) FB_HIP(hipMemcpyAsync(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost, st));
         FB_HIP(hipStreamSynchronize(st));
         float ms = 0.f;
         FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
         if (kernelMs) *kernelMs += ms;
         if (phaseUs) {
-            uint64_t prof[PERSIST_PROF];
-            FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
             for (int k = 0; k < PERSIST_PROF; k++)
                 phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
         }

@@ -50,6 +50,13 @@ class Stream:
     def synchronize(self):
         _ok(hip().hipStreamSynchronize(self.h), "hipStreamSynchronize")
 
+    def destroy(self):
+        """Releases the stream; every handle queued on it must be gone first."""
+        if self.h.value:
+            _ok(hip().hipStreamSynchronize(self.h), "hipStreamSynchronize")
+            _ok(hip().hipStreamDestroy(self.h), "hipStreamDestroy")
+            self.h = ctypes.c_void_p()
+
 
 class Event:
     def __init__(self):
@@ -61,6 +68,11 @@ class Event:
 
     def synchronize(self):
         _ok(hip().hipEventSynchronize(self.h), "hipEventSynchronize")
+
+    def destroy(self):
+        if self.h.value:
+            _ok(hip().hipEventDestroy(self.h), "hipEventDestroy")
+            self.h = ctypes.c_void_p()
 
     def elapsed_ms(self, later: "Event") -> float:
         ms = ctypes.c_float()

@@ -107,6 +107,34 @@ bool typed(napi_env env, napi_value v, T **data, size_t *len, bool optional = fa
     return true;
 }
 
+// a caller's TypedArray must hold at least `want` elements: the C side copies
+// exactly that many into or out of it
+bool long_enough(napi_env env, size_t have, size_t want, const char *what) {
+    if (have >= want) return true;
+    std::string m = std::string(what) + ": array of " + std::to_string(have) + " elements, " +
+                    std::to_string(want) + " needed";
+    napi_throw_range_error(env, nullptr, m.c_str());
+    return false;
+}
+
+bool brain_n(napi_env env, fb_brain *b, size_t *n) {
+    int32_t N = 0;
+    fb_status st = fb_brain_size(b, &N);
+    if (st != FB_OK) { throw_fb(env, st, "size"); return false; }
+    *n = (size_t)N;
+    return true;
+}
+
+bool fv_dims(napi_env env, fb_fv *e, size_t *N, size_t *E, size_t *nCol, size_t *tape) {
+    int32_t n = 0, c = 0, t = 0;
+    int64_t ne = 0;
+    fb_status st = fb_fv_size(e, &n, &ne, &c);
+    if (st == FB_OK) st = fb_fv_tape_len(e, &t);
+    if (st != FB_OK) { throw_fb(env, st, "fvSize"); return false; }
+    *N = (size_t)n; *E = (size_t)ne; *nCol = (size_t)c; *tape = (size_t)t;
+    return true;
+}
+
 napi_value make_i32(napi_env env, int32_t x) {
     napi_value r;
     napi_create_int32(env, x, &r);
@@ -303,8 +331,8 @@ napi_value get_array(napi_env env, napi_callback_info info) {
     if (!get_args(env, info, a, 3)) return nullptr;
     BRAIN_OR_NULL(0);
     float *out;
-    size_t n;
-    if (!typed(env, a.v[2], &out, &n)) return nullptr;
+    size_t n, N;
+    if (!typed(env, a.v[2], &out, &n) || !brain_n(env, b, &N) || !long_enough(env, n, N, "getArray")) return nullptr;
     FB_OK_OR_THROW(fb_get_array(b, i32(env, a.v[1]), out), "getArray");
     return nullptr;
 }
@@ -314,8 +342,8 @@ napi_value set_array(napi_env env, napi_callback_info info) {
     if (!get_args(env, info, a, 3)) return nullptr;
     BRAIN_OR_NULL(0);
     float *in;
-    size_t n;
-    if (!typed(env, a.v[2], &in, &n)) return nullptr;
+    size_t n, N;
+    if (!typed(env, a.v[2], &in, &n) || !brain_n(env, b, &N) || !long_enough(env, n, N, "setArray")) return nullptr;
     FB_OK_OR_THROW(fb_set_array(b, i32(env, a.v[1]), in), "setArray");
     return nullptr;
 }
@@ -447,6 +475,10 @@ napi_value fv_model_set_kdt(napi_env env, napi_callback_info info) {
     float *k;
     size_t n;
     if (!m || !typed(env, a.v[1], &k, &n)) return nullptr;
+    int32_t N = 0;
+    int64_t E = 0;
+    FB_OK_OR_THROW(fb_fv_model_size(m, &N, &E), "fvModelSetKdt");
+    if (!long_enough(env, n, (size_t)N, "fvModelSetKdt")) return nullptr;
     FB_OK_OR_THROW(fb_fv_model_set_kdt(m, k), "fvModelSetKdt");
     return nullptr;
 }
@@ -506,14 +538,17 @@ napi_value fv_call_int(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
-template <fb_status (*F)(fb_fv *, const float *)>
+// PerCol: the array is one value per luminance column, else one per node
+template <fb_status (*F)(fb_fv *, const float *), bool PerCol>
 napi_value fv_call_arr(napi_env env, napi_callback_info info) {
     Args a;
     if (!get_args(env, info, a, 2)) return nullptr;
     fb_fv *e = (fb_fv *)ext(env, a.v[0]);
     float *x;
-    size_t n;
-    if (!e || !typed(env, a.v[1], &x, &n)) return nullptr;
+    size_t n, N, E, nCol, tape;
+    if (!e || !typed(env, a.v[1], &x, &n) || !fv_dims(env, e, &N, &E, &nCol, &tape) ||
+        !long_enough(env, n, PerCol ? nCol : N, PerCol ? "fvSetLum" : "fvSetInput/fvSetV"))
+        return nullptr;
     FB_OK_OR_THROW(F(e, x), "fv");
     return nullptr;
 }
@@ -523,8 +558,10 @@ napi_value fv_get(napi_env env, napi_callback_info info) {
     if (!get_args(env, info, a, 3)) return nullptr;
     fb_fv *e = (fb_fv *)ext(env, a.v[0]);
     float *dst;
-    size_t n;
-    if (!e || !typed(env, a.v[2], &dst, &n)) return nullptr;
+    size_t n, N, E, nCol, tape;
+    if (!e || !typed(env, a.v[2], &dst, &n) || !fv_dims(env, e, &N, &E, &nCol, &tape) ||
+        !long_enough(env, n, N, "fvGet"))
+        return nullptr;
     FB_OK_OR_THROW(fb_fv_get(e, i32(env, a.v[1]), dst), "fvGet");
     return nullptr;
 }
@@ -548,6 +585,11 @@ napi_value fv_backward(napi_env env, napi_callback_info info) {
     size_t n0, n1, n2, n3, n4;
     if (!e || !typed(env, a.v[1], &dv, &n0) || !typed(env, a.v[2], &gw, &n1) || !typed(env, a.v[3], &gb, &n2) ||
         !typed(env, a.v[4], &gk, &n3) || !typed(env, a.v[5], &dl, &n4, true))
+        return nullptr;
+    size_t N, E, nCol, tape;
+    if (!fv_dims(env, e, &N, &E, &nCol, &tape) || !long_enough(env, n0, tape * N, "fvBackward dLdV") ||
+        !long_enough(env, n1, E, "fvBackward gWeight") || !long_enough(env, n2, N, "fvBackward gBias") ||
+        !long_enough(env, n3, N, "fvBackward gKdt") || (dl && !long_enough(env, n4, tape * nCol, "fvBackward dLdLum")))
         return nullptr;
     FB_OK_OR_THROW(fb_fv_backward(e, dv, gw, gb, gk, dl), "fvBackward");
     return nullptr;
@@ -588,9 +630,9 @@ napi_value init(napi_env env, napi_value exports) {
         {"fvCreate", fv_create},
         {"fvDestroy", fv_call0<fb_fv_destroy>},
         {"fvReset", fv_call0<fb_fv_reset>},
-        {"fvSetInput", fv_call_arr<fb_fv_set_input>},
-        {"fvSetLum", fv_call_arr<fb_fv_set_lum>},
-        {"fvSetV", fv_call_arr<fb_fv_set_v>},
+        {"fvSetInput", fv_call_arr<fb_fv_set_input, false>},
+        {"fvSetLum", fv_call_arr<fb_fv_set_lum, true>},
+        {"fvSetV", fv_call_arr<fb_fv_set_v, false>},
         {"fvStep", fv_call_int<fb_fv_step>},
         {"fvTapeStart", fv_call_int<fb_fv_tape_start>},
         {"fvStepTaped", fv_call_int<fb_fv_step_taped>},

@@ -63,6 +63,7 @@ _SIGS = {
     "fb_get_array": (I32, [P, I32, P]),
     "fb_set_array": (I32, [P, I32, P]),
     "fb_brain_scalars": (I32, [P, ctypes.POINTER(U32), PI32, ctypes.POINTER(F), ctypes.POINTER(D)]),
+    "fb_brain_size": (I32, [P, PI32]),
     "fb_brain_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_brain_set_stream": (I32, [P, P]),
     "fb_brain_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
@@ -114,6 +115,8 @@ _SIGS = {
     "fb_fv_tape_start": (I32, [P, I32]),
     "fb_fv_step_taped": (I32, [P, I32]),
     "fb_fv_tape_len": (I32, [P, P]),
+    "fb_fv_size": (I32, [P, PI32, P, PI32]),
+    "fb_fv_model_size": (I32, [P, PI32, P]),
     "fb_fv_backward": (I32, [P, P, P, P, P, P]),
     "fb_diff_forward": (I32, [P, I32, D, I32, ctypes.POINTER(D)]),
     "fb_diff_backward": (I32, [P, I32, P, P, P, P, P]),

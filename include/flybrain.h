@@ -134,6 +134,10 @@ fb_status fb_get_array(fb_brain *b, int32_t which, void *dst);
 fb_status fb_set_array(fb_brain *b, int32_t which, const void *src);
 /* RNG state, ring head, background accumulator, simulated time in ms */
 fb_status fb_brain_scalars(fb_brain *b, uint32_t *rng, int32_t *head, float *bg_acc, double *t_ms);
+/* The brain's neuron count: the length of every fb_get_array / fb_set_array
+ * buffer.  No counterpart in the reference (its typed arrays carry their
+ * length); for bindings that check a caller's buffer before a copy. */
+fb_status fb_brain_size(const fb_brain *b, int32_t *N);
 /* The xorshift32 state itself (non-zero): a lane of the cord ensemble starts
  * from a raw state, 2654435761 + 7919 m + c (cordens.bend:296-300), not from
  * LIFWasm's seed hash. */
@@ -386,6 +390,13 @@ fb_status fb_fv_tape_start(fb_fv *e, int32_t steps);
  * the room fb_fv_tape_start made */
 fb_status fb_fv_step_taped(fb_fv *e, int32_t nsteps);
 fb_status fb_fv_tape_len(fb_fv *e, int32_t *n);
+/* Nodes, edges and luminance columns of an instance's model: the lengths
+ * fb_fv_set_input / set_v / get (N), fb_fv_backward (E, tape x N, tape x
+ * nCol) and fb_fv_set_lum (nCol) copy.  No counterpart in the reference; for
+ * bindings that check a caller's buffer before a copy. */
+fb_status fb_fv_size(const fb_fv *e, int32_t *N, int64_t *E, int32_t *nCol);
+/* Nodes and edges of a model (the length fb_fv_model_set_kdt reads is N). */
+fb_status fb_fv_model_size(const fb_fv_model *m, int32_t *N, int64_t *E);
 /* backward(dLdV, g, dLdLum) (src/flyvisdiff.js:128-185) over the n taped
  * steps: dLdV[n][N] the loss gradient of v after each step; gW[E] (source
  * order), gBias[N], gKdt[N] accumulators, added to (two eyes share one);

@@ -166,3 +166,24 @@ def test_engine_switches_and_drive_changes_keep_members_exact():
             assert np.array_equal(eb.array(k, b).view(np.uint32), oracles[b].array(k).view(np.uint32)), (b, k)
         assert np.array_equal(eb.array("spikeCount", b), oracles[b].spikeCount)
         assert int(rngs[b]) == oracles[b].rng
+
+
+def test_lane_drive_cannot_be_cleared_under_the_pool_loop():
+    """The pool loop writes every lane's drive after each step: switching
+    per-lane drive off while a loop is set is refused (no step runs), and it
+    is allowed again once the loop is cleared."""
+    print("lane drive vs pool loop: argument check only", flush=True)
+    g = synth.tiny_graph(256, 8, seed=3)
+    hg = lif.HipGraph.from_synth(g)
+    bt = batch.LIFBatch(hg, P.config_params(), [1, 2], np.zeros(g.N, np.int32), np.ones((2, 1), np.float32))
+    bt.set_lanes(True, np.zeros((2, g.N), np.float32))
+    pool = np.full(g.N, 255, np.uint8)
+    pool[:6] = np.arange(6)
+    bt.set_pool_feedback(pool, 6, np.arange(10, 20, dtype=np.int32), 20.0, 1.0, [2, 3], 10)
+    with pytest.raises(RuntimeError, match="pool loop"):
+        bt.set_lanes(True, None)
+    bt.set_pool_feedback(None, 6, np.zeros(0, np.int32), 0.0, 0.0, [1, 1], 0)
+    bt.set_lanes(True, None)
+    bt.set_lanes(True, np.ones((2, g.N), np.float32))     # the kept buffer is reused
+    bt.destroy()
+    hg.release()

@@ -181,3 +181,17 @@ def test_flyvis_diff_from_node_matches_oracle(tmp_path):
     for k in ("weight", "bias", "kdt"):
         assert np.array_equal(np.array(res[k], np.float32).view(np.uint32), g[k].view(np.uint32)), k
     assert res["tape"] == T
+
+
+def test_short_arrays_raise_range_errors():
+    """Every addon call that copies a fixed number of elements into or out of
+    a caller's TypedArray refuses an array one element short with a RangeError
+    (the C side would otherwise read or write past its end)."""
+    print("node short-array checks", flush=True)
+    out = subprocess.run(["node", os.path.join(HERE, "js", "short_arrays.js")], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout)
+    for k in ("getArrayOk", "fvBackwardOk"):
+        assert res.pop(k) == "no error", k
+    assert res and all(v == "RangeError" for v in res.values()), res
