@@ -4,9 +4,9 @@
 (scripts/grad_check.mjs:51-67): a smooth (soft-threshold) network with a type
 gain and a threshold offset on every neuron, size scaling, a masked threshold
 term and a masked bias, depression and adaptation on, scored on the spike
-counts of a random 30% of the neurons.  The graph is a seeded synthetic one
-(the reference uses an induced subgraph of the real connectome, which is not
-in this container)."""
+counts of a random 30% of the neurons, on seeded synthetic graphs of any
+size.  The reference's own problem on its 800-neuron subgraph of the real
+connectome, with its published adjoint values, is tests/gradcheck.py."""
 import importlib
 
 import numpy as np

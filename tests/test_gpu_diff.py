@@ -346,3 +346,36 @@ def test_per_epoch_loss_matches_oracle():
     for k in P.DIFF_PARAMS:
         assert gg["params"][k] == pytest.approx(go["params"][k], rel=1e-4, abs=1e-8), k
     assert rel_err(g.drive_grad(), o.drive_grad()) < 1e-4
+
+
+@pytest.mark.parametrize("published", [True, False])
+def test_gpu_reproduces_the_reference_grad_check(published):
+    """The reference's gradient check on its 800-neuron subgraph of the real
+    connectome (scripts/grad_check.mjs 800 120, soft model): with the
+    published problem the GPU adjoint reproduces the published table
+    (docs/33-differentiable-brain.md:50-62) to 1e-5 relative, and on both
+    problems it agrees with the oracle to 1e-5 relative (the logistic's exp
+    is the device's)."""
+    import gradcheck as G
+    if not G.available():
+        pytest.skip("grad-check subgraph fixture absent")
+    print(f"grad_check subgraph on the GPU, published={published}", flush=True)
+    gc = G.GradCheck(published=published)
+    o = gc.oracle()
+    Lo = o.forward(G.STEPS, seed=1)
+    go = o.backward()
+    net = gc.apply(diff.LIFDiffHip(gc.N, gc.indptr, gc.indices, gc.counts, gc.sign, gc.p))
+    Lg = net.forward(G.STEPS, seed=1)
+    gg = net.backward(edges=False)
+    assert abs(Lg - Lo) <= 1e-5 * abs(Lo)
+    for k in G.PUBLISHED:
+        a, b = gg["params"][k], go["params"][k]
+        assert abs(a - b) <= 1e-5 * abs(b) + 1e-9, (k, a, b)
+        if published:
+            pub = G.PUBLISHED[k]
+            assert abs(a - pub) <= 1e-5 * abs(pub) + 5e-7, (k, a, pub)
+    assert rel_err(gg["logGain"], go["logGain"]) <= 1e-5
+    if published:
+        for i, pub in G.PUBLISHED_LOGGAIN.items():
+            assert abs(float(gg["logGain"][i]) - pub) <= 1e-5 * abs(pub) + 5e-7, (i, gg["logGain"][i], pub)
+    net.destroy()
