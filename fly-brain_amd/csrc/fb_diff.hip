@@ -104,6 +104,14 @@ struct DiffDev {
     float *gDrive;                     // [nEp][nDI]
     const float *dPer;                 // [epochs][N] per-epoch spike weights, or null
     double *lossAcc;                   // [N] the per-epoch weighted loss of a forward pass
+    const float *betaArr;              // [N] per-neuron surrogateBeta, or null (the scalar)
+    // exact32: the forward pass in lif.c's binary32 arithmetic -- every
+    // scalar as lif.c reads it from its binary32 header, and writeGraph's
+    // weights and signs baked from the inputs before each forward pass
+    int32_t exact32;
+    float xvRest, xeExc, xeInh, xvThresh, xvReset, xtRef, xadaptInc, xdt, xdepU, xlamina;
+    float xdE, xdA, xkRec, xkM, xdtS, xcE, xcI, xforcedRefr;
+    float *W32, *signW32;              // [E], [N]
 };
 
 constexpr double DRIVE_MAX = 0.999;   // src/lifdiff.js:72
@@ -258,6 +266,54 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
     const int64_t a0 = min(d.off[t - d.D], d.cap);
     const int64_t a1 = min((!REPLAY && d.D == 1) ? (int64_t)*d.top : d.off[t - d.D + 1], d.cap);
     const int64_t resBase = REPLAY ? d.off[resFromT] : 0;
+    if (!REPLAY && d.exact32) {
+        // lif.c's delivery (src/lifdiff.js:263-274): s = f32(sign * res), a zero
+        // s skipped before the depression, which rounds as lif.c's
+        // res -= depU * res; one binary32 product per edge (zero ones skipped:
+        // adding them changes no conductance)
+        for (int64_t k = a0 + wave; k < a1; k += nwaves) {
+            const uint32_t e = (uint32_t)d.idx[k];
+            const int32_t pre = (int32_t)(e & ~FORCED_BIT);
+            const float resv = d.res[pre];
+            float s = d.signW32[pre] * resv;
+            const float ampk = d.amp[k];
+            if (ampk != 1.f) s = s * ampk;
+            if (s == 0.f) continue;
+            const uint32_t ra = d.indptr[pre], re = d.indptr[pre + 1];
+            if (lane == 0) {
+                d.res[pre] = resv - d.xdepU * resv;
+                atomicAdd(&d.events[0], (unsigned long long)(re - ra));
+                atomicAdd(&d.events[1], 1ull);
+            }
+            const uint32_t rank = (e & FORCED_BIT) ? (uint32_t)(k - a0) : (uint32_t)d.N + (uint32_t)pre;
+            const int32_t cls = s > 0 ? 0 : d.N;
+            const int32_t par = t & 1;
+            for (uint32_t j = ra + lane; j < re; j += 64) {
+                const float dlt = d.W32[j] * s;
+                if (dlt == 0.f) continue;
+                const uint32_t q = d.indices[j];
+                const int32_t ci = cls + (int32_t)q;
+                const uint64_t kk = ((uint64_t)rank << 32) | (uint64_t)(j - ra);
+                const int32_t slot = atomicAdd(&d.cnt[ci], 1);
+                if (slot == DIFF_LONG) d.longList[atomicAdd(d.nLong, 1)] = ci;
+                if (slot == 0) {
+                    d.add[ci] = (double)dlt;
+                    d.key[ci] = kk;
+                } else {
+                    const int32_t p = atomicAdd(&d.ncoll[par], 1);
+                    if (p < d.collHalf) {
+                        const int32_t gp = (int32_t)(par * d.collHalf + p);
+                        d.coll[gp].key = kk;
+                        d.coll[gp].addend = (double)dlt;
+                        d.coll[gp].next = atomicExch(&d.chain[ci], gp);
+                    } else {
+                        atomicOr(d.err, DERR_COLL);
+                    }
+                }
+            }
+        }
+        return;
+    }
     for (int64_t k = a0 + wave; k < a1; k += nwaves) {
         const uint32_t e = (uint32_t)d.idx[k];
         const int32_t pre = (int32_t)(e & ~FORCED_BIT);
@@ -309,6 +365,24 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
     }
 }
 
+// A driven neuron's draw u against its rate (src/lifdiff.js:293): binary64
+// u < drive dt/1000, or under exact32 lif.c's binary32 threshold
+__device__ __forceinline__ bool drive_fires(const DiffDev &d, int32_t i, int32_t t, double u) {
+    const float rate = drive_at(d, i, t);
+    return d.exact32 ? u < (double)(rate * d.xdtS) : u < (double)rate * d.dtS;
+}
+
+// a forced spike's refractory period and adaptation increment (src/lifdiff.js:294-295)
+__device__ __forceinline__ void forced_effects(const DiffDev &d, int32_t i) {
+    if (d.exact32) {
+        d.refr[i] = d.xforcedRefr;
+        d.adapt[i] = d.adapt[i] + d.xadaptInc;
+    } else {
+        d.refr[i] = d.forcedRefr;
+        d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+    }
+}
+
 // The drive pre-pass (src/lifdiff.js:290-297): one draw per non-refractory
 // driven neuron in set order; forced spikes go to the tape in that order.
 __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
@@ -345,9 +419,8 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
             const int32_t i = d.driven[q];
             if (!(d.refr[i] > 0)) {
                 const double u = (double)(draws[next - r0] >> 8) * (1.0 / 16777216.0);
-                if (u < (double)drive_at(d, i, t) * d.dtS) {
-                    d.refr[i] = d.forcedRefr;
-                    d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+                if (drive_fires(d, i, t, u)) {
+                    forced_effects(d, i);
                     d.forced[i] = 1;
                     d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
                     if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
@@ -423,9 +496,8 @@ __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
             first = false;
             const int32_t i = ii[u];
             const double uu = (double)(x >> 8) * (1.0 / 16777216.0);
-            if (uu < (double)drive_at(d, i, t) * d.dtS) {
-                d.refr[i] = d.forcedRefr;
-                d.adapt[i] = (float)((double)d.adapt[i] + d.adaptInc);
+            if (drive_fires(d, i, t, uu)) {
+                forced_effects(d, i);
                 d.forced[i] = 1;
                 d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
                 if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
@@ -498,27 +570,50 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
         double vi = vf, s = 0.0;
         float refrf = d.refr[i], adaptf = d.adapt[i], resf = d.res[i];
         const bool wasRefr = refrf > 0;
+        const bool x32 = !REPLAY && d.exact32;
         uint8_t md = 0;
         const int64_t ar = (int64_t)r * N + i;
         if (REPLAY) { d.aV[ar] = vf; d.aGE[ar] = gEf; d.aGI[ar] = gIf; }
         if (wasRefr) {
-            refrf = (float)((double)refrf - d.dt);
-            vi = d.vReset;
+            if (x32) {
+                refrf = refrf - d.xdt;
+                vi = d.xvReset;
+            } else {
+                refrf = (float)((double)refrf - d.dt);
+                vi = d.vReset;
+            }
             if (REPLAY) {
                 if (d.forced[i]) { md = 2; s = 1.0; } else md = 1;
             }
+        } else if (x32) {
+            // lif.c's COBA expression, one binary32 rounding per operation in
+            // its order (src/lifdiff.js:317-326)
+            const float bias = d.biasMask[i] ? d.xlamina : 0.f;
+            float acc = d.xvRest - vf;
+            acc = acc + (gEf * (d.xeExc - vf)) * d.xcE;
+            acc = acc + (gIf * (vf - d.xeInh)) * d.xcI;
+            vi = (double)(vf + (acc + bias) * d.xkM);
         } else {
             const double bias = d.biasMask[i] ? d.lamina : 0.0;
             vi += (d.vRest - vi + (double)gEf * (d.eExc - vi) * d.cE + (double)gIf * (vi - d.eInh) * d.cI + bias) * d.kM;
         }
-        gEf = (float)((double)gEf * d.dE);
-        gIf = (float)((double)gIf * d.dE);
-        adaptf = (float)((double)adaptf * d.dA);
-        resf = (float)((double)resf + (1.0 - (double)resf) * d.kRec);
+        if (x32) {
+            gEf = gEf * d.xdE;
+            gIf = gIf * d.xdE;
+            adaptf = adaptf * d.xdA;
+            resf = resf + (1.f - resf) * d.xkRec;
+        } else {
+            gEf = (float)((double)gEf * d.dE);
+            gIf = (float)((double)gIf * d.dE);
+            adaptf = (float)((double)adaptf * d.dA);
+            resf = (float)((double)resf + (1.0 - (double)resf) * d.kRec);
+        }
         if (REPLAY) d.aAd[ar] = adaptf;
         if (!wasRefr) {
-            const double thr = d.vThresh + (double)adaptf + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
-            s = d.soft ? 1.0 / (1.0 + exp(-(vi - thr) / d.beta)) : (vi >= thr ? 1.0 : 0.0);
+            const double thr = x32 ? (double)((d.xvThresh + adaptf) + d.thrOffset[i])
+                                   : d.vThresh + (double)adaptf + (d.thrMask[i] ? d.kc : 0.0) + (double)d.thrOffset[i];
+            const double beta = d.betaArr ? (double)d.betaArr[i] : d.beta;
+            s = d.soft ? 1.0 / (1.0 + exp(-(vi - thr) / beta)) : (vi >= thr ? 1.0 : 0.0);
             // the drive as an expectation (src/lifdiff.js:338, 453)
             if (d.driveSoft) {
                 const float dv = drive_at(d, i, t);
@@ -529,9 +624,9 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
                 }
             }
             if (s > 0) {
-                vi = s * d.vReset + (1.0 - s) * vi;
-                if (!d.soft) refrf = d.tRefF;
-                adaptf = (float)((double)adaptf + s * d.adaptInc);
+                vi = s * (x32 ? (double)d.xvReset : d.vReset) + (1.0 - s) * vi;
+                if (!d.soft) refrf = x32 ? d.xtRef : d.tRefF;
+                adaptf = (float)((double)adaptf + s * (x32 ? (double)d.xadaptInc : d.adaptInc));
             }
             if (!REPLAY) {
                 d.spikeCount[i] = (float)((double)d.spikeCount[i] + s);
@@ -569,6 +664,23 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
     if (i == 0) {
         d.ncoll[(t + 1) & 1] = 0;
         *d.nLong = 0;
+    }
+}
+
+// exact32's writeGraph bake (src/lifdiff.js:182-189): wave per row, W[k] =
+// f32(count * edgeGain * inScale[q] * ig) with ig = (sign < 0 ? inhGain : 1) *
+// outScale[j] and sign[j] = f32(sign * wSyn), the products in binary64
+__global__ void kd_bake_x32(DiffDev d) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t j = wave; j < d.N; j += nwaves) {
+        const double sg = d.sign[j];
+        if (lane == 0) d.signW32[j] = (float)(sg * d.wSyn);
+        const double ig = (sg < 0 ? d.inhGain : 1.0) * (double)d.outScale[j];
+        for (uint32_t k = d.indptr[j] + lane; k < d.indptr[j + 1]; k += 64)
+            d.W32[k] = (float)((double)d.counts[k] * (d.edgeGain ? (double)d.edgeGain[k] : 1.0) *
+                               (double)d.inScale[d.indices[k]] * ig);
     }
 }
 
@@ -723,12 +835,13 @@ __device__ __forceinline__ void dx_bneuron(const DiffDev &d, int32_t t, int32_t 
                 sd = fmin(DRIVE_MAX, (double)drive_at(d, i, t) * d.dtS);
                 st = (s - sd) / (1.0 - sd);
             }
+            const double beta = d.betaArr ? (double)d.betaArr[i] : d.beta;
             double sg;
             if (d.soft) {
-                sg = st * (1.0 - st) / d.beta;
+                sg = st * (1.0 - st) / beta;
             } else {
-                const double q = 1.0 + fabs(dd) / d.beta;
-                sg = 1.0 / (d.beta * q * q);
+                const double q = 1.0 + fabs(dd) / beta;
+                sg = 1.0 / (beta * q * q);
             }
             const double lspike = (double)*ls + dl;
             const double lvNew = lvf;
@@ -968,6 +1081,10 @@ struct fb_diff {
     void *schedAlloc = nullptr;          // drvSlot, drvSched, gDrive
     float *dPerAlloc = nullptr;
     double *lossAccAlloc = nullptr;
+    bool thrMaskAny = false;             // exact32 refuses a threshold mask
+    bool inScaleGiven = false;           // opts.inScale: in[FB_D_INSCALE_IN] taken verbatim
+    float *betaAlloc = nullptr;          // per-neuron surrogateBeta (d.betaArr when set)
+    float *x32Alloc = nullptr;           // exact32's W32 [E] and signW32 [N]
 };
 
 #define FB_REQUIRE_D(cond, code, ...)    \
@@ -1017,6 +1134,14 @@ static void derive(fb_diff *f) {
     d.minScale = 1.0 / p.maxSizeScale;
     d.forcedRefr = (float)(p.tRef + p.dt);
     d.tRefF = (float)p.tRef;
+    // lif.c reads every scalar from its binary32 header (src/lifdiff.js:231-240)
+    d.exact32 = p.exact32 ? 1 : 0;
+    d.xvRest = (float)p.vRest; d.xeExc = (float)p.eExc; d.xeInh = (float)p.eInh; d.xvThresh = (float)p.vThresh;
+    d.xvReset = (float)p.vReset; d.xtRef = (float)p.tRef; d.xadaptInc = (float)p.adaptInc; d.xdt = (float)p.dt;
+    d.xdepU = (float)p.depU; d.xlamina = (float)p.laminaBias;
+    d.xdE = (float)d.dE; d.xdA = (float)d.dA; d.xkRec = (float)d.kRec; d.xkM = (float)d.kM; d.xdtS = (float)d.dtS;
+    d.xcE = (float)d.cE; d.xcI = (float)d.cI;
+    d.xforcedRefr = d.xtRef + d.xdt;
 }
 
 // inScale and exp(logGain), formed on the host in f64 (src/lifdiff.js:158-164, 278)
@@ -1026,9 +1151,13 @@ static fb_status upload_derived(fb_diff *f) {
     std::vector<double> eg((size_t)N);
     const fb_diff_params &p = f->p;
     for (int32_t i = 0; i < N; i++) {
-        const double s = std::exp(-p.sizeAlpha * (double)f->in[FB_D_SIZELOG][i]);
-        const double m = std::max(1.0 / p.maxSizeScale, s);
-        inS[i] = (float)std::min(p.boostCap, m);
+        if (f->inScaleGiven) {
+            inS[i] = f->in[FB_D_INSCALE_IN][i];            // taken verbatim (src/lifdiff.js:160)
+        } else {
+            const double s = std::exp(-p.sizeAlpha * (double)f->in[FB_D_SIZELOG][i]);
+            const double m = std::max(1.0 / p.maxSizeScale, s);
+            inS[i] = (float)std::min(p.boostCap, m);
+        }
         eg[i] = std::exp((double)f->in[FB_D_LOGGAIN][i]);
     }
     FB_HIP(hipMemcpy((void *)f->d.inScale, inS.data(), sizeof(float) * N, hipMemcpyHostToDevice));
@@ -1150,6 +1279,8 @@ extern "C" fb_status fb_diff_destroy(fb_diff *f) {
     if (f->schedAlloc) (void)hipFree(f->schedAlloc);
     if (f->dPerAlloc) (void)hipFree(f->dPerAlloc);
     if (f->lossAccAlloc) (void)hipFree(f->lossAccAlloc);
+    if (f->betaAlloc) (void)hipFree(f->betaAlloc);
+    if (f->x32Alloc) (void)hipFree(f->x32Alloc);
     (void)hipStreamDestroy(f->ownStream);
     if (f->replayStream) {
         (void)hipStreamDestroy(f->replayStream);
@@ -1188,11 +1319,30 @@ extern "C" fb_status fb_diff_get_params(fb_diff *f, fb_diff_params *p) {
 }
 
 extern "C" fb_status fb_diff_set_input(fb_diff *f, int32_t which, const float *values) {
-    FB_REQUIRE_D(f && values, FB_ERR_ARG, "fb_diff_set_input: null argument");
+    FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_set_input: null handle");
     FB_REQUIRE_D(which >= 0 && which < FB_D_NINPUTS, FB_ERR_ARG, "fb_diff_set_input: unknown input %d", which);
+    FB_REQUIRE_D(values || which == FB_D_INSCALE_IN || which == FB_D_BETA, FB_ERR_ARG,
+                 "fb_diff_set_input: null values (only inScale and surrogateBeta can be cleared)");
     const int32_t N = f->N;
-    f->in[which].assign(values, values + N);
     FB_HIP(hipStreamSynchronize(f->stream));
+    if (which == FB_D_BETA) {
+        if (values) {
+            for (int32_t i = 0; i < N; i++)
+                FB_REQUIRE_D(values[i] > 0, FB_ERR_ARG, "fb_diff_set_input: surrogateBeta[%d] must be positive", i);
+            if (!f->betaAlloc) FB_HIP(hipMalloc(&f->betaAlloc, sizeof(float) * (size_t)N));
+            FB_HIP(hipMemcpy(f->betaAlloc, values, sizeof(float) * N, hipMemcpyHostToDevice));
+        }
+        f->d.betaArr = values ? f->betaAlloc : nullptr;
+        f->recorded = false;
+        return FB_OK;
+    }
+    if (which == FB_D_INSCALE_IN) {
+        f->inScaleGiven = values != nullptr;
+        if (values) f->in[which].assign(values, values + N);
+        f->recorded = false;
+        return upload_derived(f);
+    }
+    f->in[which].assign(values, values + N);
     float *dst = nullptr;
     switch (which) {
         case FB_D_OUTSCALE: dst = (float *)f->d.outScale; break;
@@ -1227,7 +1377,10 @@ extern "C" fb_status fb_diff_set_edge_gain(fb_diff *f, const float *edgeGain) {
 extern "C" fb_status fb_diff_set_masks(fb_diff *f, const uint8_t *thrMask, const uint8_t *biasMask) {
     FB_REQUIRE_D(f, FB_ERR_ARG, "fb_diff_set_masks: null handle");
     FB_HIP(hipStreamSynchronize(f->stream));
-    if (thrMask) FB_HIP(hipMemcpy((void *)f->d.thrMask, thrMask, (size_t)f->N, hipMemcpyHostToDevice));
+    if (thrMask) {
+        FB_HIP(hipMemcpy((void *)f->d.thrMask, thrMask, (size_t)f->N, hipMemcpyHostToDevice));
+        f->thrMaskAny = std::any_of(thrMask, thrMask + f->N, [](uint8_t m) { return m != 0; });
+    }
     if (biasMask) FB_HIP(hipMemcpy((void *)f->d.biasMask, biasMask, (size_t)f->N, hipMemcpyHostToDevice));
     f->recorded = false;
     return FB_OK;
@@ -1433,6 +1586,15 @@ static fb_status fwd_prepare(fb_diff *f, int32_t steps, double seed, int32_t rec
         if (!all.empty())
             FB_HIP(hipMemcpyAsync(f->epLists, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, st));
         FB_HIP(hipStreamSynchronize(st));   // `all` leaves scope
+    }
+    if (d.exact32) {
+        // lif.c sums vThresh + adapt + thr only (src/lifdiff.js:178-180)
+        FB_REQUIRE_D(!f->thrMaskAny, FB_ERR_ARG,
+                     "fb_diff_forward: exact32 needs the whole threshold field in thrOffset; thrMask must be empty");
+        if (!f->x32Alloc) FB_HIP(hipMalloc(&f->x32Alloc, sizeof(float) * ((size_t)std::max<int64_t>(f->E, 1) + N)));
+        d.W32 = f->x32Alloc;
+        d.signW32 = f->x32Alloc + std::max<int64_t>(f->E, 1);
+        hipLaunchKernelGGL(kd_bake_x32, dim3(grid_of((int64_t)N * 64, 256, 8192)), dim3(256), 0, st, d);
     }
     const uint32_t r0 = seed_hash(seed);
     FB_HIP(hipMemcpyAsync(d.rng, &r0, sizeof r0, hipMemcpyHostToDevice, st));

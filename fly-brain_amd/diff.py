@@ -21,7 +21,7 @@ import numpy as np
 from . import lib as _l
 from .params import DIFF_PARAMS, diff_merged, to_diff_struct
 
-INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive")
+INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive", "inScale", "surrogateBeta")
 STATE = ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount", "inScale")
 
 
@@ -65,6 +65,14 @@ class LIFDiffHip:
         _l.check(self.lib.fb_diff_set_params(self.h, ctypes.byref(ps)))
 
     def set_input(self, name: str, values):
+        """A per-neuron input.  ``inScale`` given is taken verbatim instead of
+        the one formed from sizeLog; ``surrogateBeta`` is a per-neuron beta;
+        None clears either (the derived inScale, the scalar beta)."""
+        if values is None:
+            if name not in ("inScale", "surrogateBeta"):
+                raise ValueError(f"{name}: only inScale and surrogateBeta can be cleared")
+            _l.check(self.lib.fb_diff_set_input(self.h, INPUTS.index(name), None))
+            return
         a = np.ascontiguousarray(values, dtype=np.float32)
         if a.size != self.N:
             raise ValueError(f"{name}: expected {self.N} values")

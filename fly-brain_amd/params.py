@@ -75,7 +75,7 @@ DIFF_DEFAULTS = {
     "tRef": 2.2, "delay": 1.8, "wSyn": 0.3, "adaptInc": 0.0, "adaptTau": 100.0, "depU": 0.0,
     "depTau": 200.0, "eExc": 0.0, "eInh": -70.0, "inhGain": 1.0, "kcThreshold": 0.0,
     "laminaBias": 0.0, "surrogateBeta": 2.0, "adjClip": 0.0, "sizeAlpha": 0.0,
-    "maxSizeScale": 20.0, "boostCap": 1.0, "soft": False, "checkpoint": 16, "minSyn": 1,
+    "maxSizeScale": 20.0, "boostCap": 1.0, "soft": False, "checkpoint": 16, "minSyn": 1, "exact32": False,
 }
 
 # fb_diff_params field order (include/flybrain.h): doubles, then two int32
@@ -83,7 +83,7 @@ DIFF_FIELDS = ("dt", "vRest", "vThresh", "vReset", "tauM", "tauSyn", "tRef", "de
                "wSyn", "adaptInc", "adaptTau", "depU", "depTau",
                "eExc", "eInh", "inhGain", "kcThreshold", "laminaBias",
                "surrogateBeta", "adjClip", "sizeAlpha", "maxSizeScale", "boostCap")
-DIFF_INT_FIELDS = ("soft", "checkpoint")
+DIFF_INT_FIELDS = ("soft", "checkpoint", "exact32")
 
 # gradient order (src/lifdiff.js:75)
 DIFF_PARAMS = ("wSyn", "sizeAlpha", "inhGain", "eInh", "vThresh", "kcThreshold", "laminaBias",
@@ -107,6 +107,7 @@ def to_diff_struct(p: dict) -> FbDiffParams:
         setattr(s, n, float(p[n]))
     s.soft = 1 if p["soft"] else 0
     s.checkpoint = int(p["checkpoint"])
+    s.exact32 = 1 if p.get("exact32") else 0
     return s
 
 

@@ -249,10 +249,21 @@ typedef struct {
     double surrogateBeta, adjClip;
     double sizeAlpha, maxSizeScale, boostCap;
     int32_t soft, checkpoint;
+    /* exact32 (lifdiff.js:105-109, 173-190, 231-240, 263-274, 317-326): the
+     * forward pass in lif.c's binary32 arithmetic on writeGraph's baked
+     * weights and signs, bit-identical to lif_step.  logGain and the spike
+     * amplitude must be inert and thrMask empty (forward refuses otherwise);
+     * the depression of res rounds as lif.c does.  The backward pass and its
+     * replay are LIFDiff's binary64 model either way, as in the reference. */
+    int32_t exact32;
 } fb_diff_params;
 
-/* per-neuron inputs (fb_diff_set_input) */
-enum { FB_D_OUTSCALE = 0, FB_D_LOGGAIN, FB_D_THROFFSET, FB_D_SIZELOG, FB_D_DRIVE, FB_D_NINPUTS };
+/* per-neuron inputs (fb_diff_set_input).  FB_D_INSCALE is a caller-given
+ * inScale taken verbatim instead of the one formed from sizeLog (opts.inScale,
+ * lifdiff.js:115-118, 160); FB_D_BETA a per-neuron surrogateBeta
+ * (lifdiff.js:89-95, 335, 590).  A NULL array clears either of the two. */
+enum { FB_D_OUTSCALE = 0, FB_D_LOGGAIN, FB_D_THROFFSET, FB_D_SIZELOG, FB_D_DRIVE, FB_D_INSCALE_IN, FB_D_BETA,
+       FB_D_NINPUTS };
 /* state arrays (fb_diff_get_state) */
 enum { FB_D_V = 0, FB_D_GE, FB_D_GI, FB_D_ADAPT, FB_D_RES, FB_D_REFR, FB_D_SPIKECOUNT, FB_D_INSCALE, FB_D_NSTATE };
 /* the nine scalar gradients, in DIFF_PARAMS order (src/lifdiff.js:75) */
@@ -270,7 +281,8 @@ fb_status fb_diff_create(int32_t N, int64_t E, const uint32_t *indptr, const uin
 fb_status fb_diff_destroy(fb_diff *d);
 fb_status fb_diff_set_params(fb_diff *d, const fb_diff_params *p);
 fb_status fb_diff_get_params(fb_diff *d, fb_diff_params *p);
-/* outScale, logGain, thrOffset, sizeLog or drive (Hz), N values */
+/* outScale, logGain, thrOffset, sizeLog, drive (Hz), inScale or
+ * surrogateBeta: N values */
 fb_status fb_diff_set_input(fb_diff *d, int32_t which, const float *values);
 /* per-edge multiplier on the delivered weight (opts.edgeGain, lifdiff.js:126-129); NULL = all 1 */
 fb_status fb_diff_set_edge_gain(fb_diff *d, const float *edgeGain);

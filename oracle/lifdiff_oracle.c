@@ -20,9 +20,19 @@
  *     hook), driveSoft (the drive's expectation as a
  *     graded spike), the drive gradient per epoch, and
  *     per-epoch spike weights (dLdSpikePerEpoch)
+ *   - exact32: the forward pass in lif.c's binary32 arithmetic,     src/lifdiff.js:105-109, 173-190,
+ *     with writeGraph's baked weights and signs                      231-240, 263-274, 317-326
+ *   - a caller-given inScale, and a per-neuron surrogateBeta         src/lifdiff.js:89-95, 115-118, 160
  * The reference evaluates in JavaScript doubles and stores into Float32Arrays:
  * every local below is a double and every array store rounds to binary32, in
- * the reference's evaluation order.
+ * the reference's evaluation order.  Under exact32 every forward intermediate
+ * is binary32 instead, one rounding per C operation in lif.c's order.  One
+ * line differs from lifdiff.js there: the depression of res.  lifdiff.js
+ * writes `res[pre] -= dU * res[pre]`, one binary64 rounding before the store,
+ * where lif.c rounds the product to binary32 first; at depU = 0.15 the two
+ * disagree on about 9% of updates.  exact32 exists to be lif.c bit for bit,
+ * so this file takes lif.c's rounding (the two agree whenever depU is 0, the
+ * setting the reference audits it in).
  *
  * Two additions the reference does not have, restated from the same
  * derivation:
@@ -67,6 +77,8 @@ struct fdo_net {
     float *counts, *edgeGain, *sign;
     float *in[FDO_NINPUTS];
     uint8_t *thrMask, *biasMask;
+    int32_t inGiven[FDO_NINPUTS];   /* FDO_INSCALE_IN / FDO_BETA set */
+    float *W32, *signW32;           /* exact32: writeGraph's W and sign (src/lifdiff.js:177-190) */
     int32_t *driven, nDriven;
     fdo_params p;
     int32_t nslots, K;
@@ -167,6 +179,7 @@ static void free_tape(fdo_net *n) {
 
 void fdo_destroy(fdo_net *n) {
     if (!n) return;
+    free(n->W32); free(n->signW32);
     free_tape(n);
     free(n->indptr); free(n->indices); free(n->counts); free(n->edgeGain); free(n->sign);
     for (int k = 0; k < FDO_NINPUTS; k++) free(n->in[k]);
@@ -250,7 +263,9 @@ static void start_drive(fdo_net *n) {
 }
 
 void fdo_set_input(fdo_net *n, int32_t which, const float *values) {
-    if (which >= 0 && which < FDO_NINPUTS) memcpy(n->in[which], values, 4 * (size_t)n->N);
+    if (which < 0 || which >= FDO_NINPUTS) return;
+    n->inGiven[which] = values != NULL;
+    if (values) memcpy(n->in[which], values, 4 * (size_t)n->N);
 }
 
 void fdo_set_edge_gain(fdo_net *n, const float *eg) {
@@ -313,12 +328,37 @@ static kconsts consts_of(const fdo_params *p) {
 static void prepare(fdo_net *n) {
     const fdo_params *p = &n->p;
     float *in = n->st[FDO_INSCALE];
-    for (int32_t i = 0; i < n->N; i++) {
-        double s = exp(-p->sizeAlpha * (double)n->in[FDO_SIZELOG][i]);
-        double lo = 1.0 / p->maxSizeScale;
-        double m = s > lo ? s : lo;
-        in[i] = (float)(p->boostCap < m ? p->boostCap : m);
+    if (n->inGiven[FDO_INSCALE_IN]) {
+        memcpy(in, n->in[FDO_INSCALE_IN], 4 * (size_t)n->N);   /* taken verbatim */
+    } else {
+        for (int32_t i = 0; i < n->N; i++) {
+            double s = exp(-p->sizeAlpha * (double)n->in[FDO_SIZELOG][i]);
+            double lo = 1.0 / p->maxSizeScale;
+            double m = s > lo ? s : lo;
+            in[i] = (float)(p->boostCap < m ? p->boostCap : m);
+        }
     }
+    /* exact32 bakes writeGraph's arrays (src/lifdiff.js:177-190): W[k] =
+     * f32(count * edgeGain * inScale[q] * ig) with ig = (sign < 0 ? inhGain :
+     * 1) * outScale[j], and sign[j] = f32(sign * wSyn), each product in f64 */
+    free(n->W32); free(n->signW32);
+    n->W32 = NULL; n->signW32 = NULL;
+    if (!p->exact32) return;
+    n->W32 = (float *)xcalloc(n->E, 4);
+    n->signW32 = (float *)xcalloc(n->N, 4);
+    for (int32_t j = 0; j < n->N; j++) {
+        const double sg = n->sign[j];
+        n->signW32[j] = (float)(sg * p->wSyn);
+        const double ig = (sg < 0 ? p->inhGain : 1.0) * (double)n->in[FDO_OUTSCALE][j];
+        for (uint32_t k = n->indptr[j]; k < n->indptr[j + 1]; k++)
+            n->W32[k] = (float)((double)n->counts[k] * (n->edgeGain ? (double)n->edgeGain[k] : 1.0) *
+                                (double)in[n->indices[k]] * ig);
+    }
+}
+
+/* a neuron's surrogateBeta: the per-neuron array when given, else the scalar */
+static double beta_of(const fdo_net *n, int32_t i) {
+    return n->inGiven[FDO_BETA] ? (double)n->in[FDO_BETA][i] : n->p.surrogateBeta;
 }
 
 static void reset_state(fdo_net *n) {
@@ -363,6 +403,29 @@ static void deliver(fdo_net *n, int64_t a0, int64_t a1, float *resAt) {
                 const uint32_t q = n->indices[j];
                 gI[q] = (float)((double)gI[q] + (double)n->counts[j] * (eg ? (double)eg[j] : 1.0) * (double)inS[q] * ig * eff);
             }
+        }
+    }
+}
+
+/* exact32's delivery (src/lifdiff.js:263-274): lif.c's factorisation -- s =
+ * f32(sign * res), one binary32 product per edge into the conductance, a zero
+ * s skipped before the depression.  The depression rounds as lif.c does (see
+ * the header). */
+static void deliver_x32(fdo_net *n, int64_t a0, int64_t a1) {
+    float *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *res = n->st[FDO_RES];
+    const float dU = (float)n->p.depU;
+    for (int64_t k = a0; k < a1; k++) {
+        const int32_t pre = (int32_t)((uint32_t)n->idx[k] & ~FORCED_BIT);
+        const float amp = n->amp[k];
+        float s = n->signW32[pre] * res[pre];
+        if (amp != 1.f) s = s * amp;
+        if (s == 0.f) continue;
+        res[pre] = res[pre] - dU * res[pre];
+        const uint32_t a = n->indptr[pre], b = n->indptr[pre + 1];
+        float *g = s > 0 ? gE : gI;
+        for (uint32_t j = a; j < b; j++) {
+            const float dlt = n->W32[j] * s;
+            if (dlt != 0.f) g[n->indices[j]] += dlt;
         }
     }
 }
@@ -448,6 +511,18 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
     reset_state(n);
     start_drive(n);
     n->lossPer = 0.0;
+    const int ex = p->exact32 != 0;
+    if (ex) {
+        /* lif.c sums vThresh + adapt + thr only: the additive field must be in
+         * thrOffset (src/lifdiff.js:178-180) */
+        for (int32_t i = 0; i < N; i++)
+            if (n->thrMask[i]) return NAN;
+    }
+    /* exact32: every scalar as lif.c reads it from its binary32 header */
+    const float xvR = (float)p->vRest, xeE = (float)p->eExc, xeI = (float)p->eInh, xvTh = (float)p->vThresh,
+                xvRs = (float)p->vReset, xtRf = (float)p->tRef, xaI = (float)p->adaptInc, xdT = (float)p->dt,
+                xlB = (float)p->laminaBias, xdE = (float)k.dE, xdA = (float)k.dA, xkR = (float)k.kRec,
+                xkM = (float)k.kM, xdtS = (float)k.dtS, xcE = (float)k.cE, xcI = (float)k.cI;
     float *v = n->st[FDO_V], *gE = n->st[FDO_GE], *gI = n->st[FDO_GI], *adapt = n->st[FDO_ADAPT];
     float *res = n->st[FDO_RES], *refr = n->st[FDO_REFR], *sc = n->st[FDO_SPIKECOUNT];
     const float *drive = n->wdrive, *thrOff = n->in[FDO_THROFFSET];
@@ -455,7 +530,10 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
         if (record && t % n->K == 0) save_ckpt(n, t / n->K);
         if (n->nDI && t % n->ep == 0) apply_epoch(n, t / n->ep);   /* the onEpoch hook */
         n->off[t + 1] = n->off[t];
-        if (t >= D) deliver(n, n->off[t - D], n->off[t - D + 1], NULL);
+        if (t >= D) {
+            if (ex) deliver_x32(n, n->off[t - D], n->off[t - D + 1]);
+            else deliver(n, n->off[t - D], n->off[t - D + 1], NULL);
+        }
         n->steps = t;
         const int scoredStep = t >= n->lossFrom;
         const float *wPer = n->dPer ? n->dPer + (int64_t)(t / n->ep) * N : NULL;
@@ -463,13 +541,13 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
         for (int32_t q = 0; q < n->nW && !n->driveSoft; q++) {
             const int32_t i = n->wdrv[q];
             if (refr[i] > 0) continue;
-            if (rand01(&n->rng) < (double)drive[i] * k.dtS) {
-                refr[i] = (float)(p->tRef + p->dt);
+            if (rand01(&n->rng) < (ex ? (double)(drive[i] * xdtS) : (double)drive[i] * k.dtS)) {
+                refr[i] = ex ? xtRf + xdT : (float)(p->tRef + p->dt);
                 push_spike(n, (uint32_t)i | FORCED_BIT, 1.f);
                 sc[i] = (float)((double)sc[i] + 1.0);
                 if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + 1.0);
                 if (scoredStep && wPer) n->lossPer += (double)wPer[i];
-                adapt[i] = (float)((double)adapt[i] + p->adaptInc);
+                adapt[i] = ex ? adapt[i] + xaI : (float)((double)adapt[i] + p->adaptInc);
                 if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = 1.f;
             }
         }
@@ -478,28 +556,43 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
             double vi = v[i], s = 0.0;
             const int wasRefr = refr[i] > 0;
             if (wasRefr) {
-                refr[i] = (float)((double)refr[i] - p->dt);
-                vi = p->vReset;
+                refr[i] = ex ? refr[i] - xdT : (float)((double)refr[i] - p->dt);
+                vi = ex ? (double)xvRs : p->vReset;
+            } else if (ex) {
+                /* lif.c's COBA expression, one binary32 rounding per operation */
+                const float vf = v[i], bias = n->biasMask[i] ? xlB : 0.f;
+                float acc = xvR - vf;
+                acc = acc + (gE[i] * (xeE - vf)) * xcE;
+                acc = acc + (gI[i] * (vf - xeI)) * xcI;
+                vi = (double)(vf + (acc + bias) * xkM);
             } else {
                 const double bias = n->biasMask[i] ? p->laminaBias : 0.0;
                 vi += (p->vRest - vi + (double)gE[i] * (p->eExc - vi) * k.cE + (double)gI[i] * (vi - p->eInh) * k.cI + bias) * k.kM;
             }
-            gE[i] = (float)((double)gE[i] * k.dE);
-            gI[i] = (float)((double)gI[i] * k.dE);
-            adapt[i] = (float)((double)adapt[i] * k.dA);
-            res[i] = (float)((double)res[i] + (1.0 - (double)res[i]) * k.kRec);
+            if (ex) {
+                gE[i] = gE[i] * xdE;
+                gI[i] = gI[i] * xdE;
+                adapt[i] = adapt[i] * xdA;
+                res[i] = res[i] + (1.f - res[i]) * xkR;
+            } else {
+                gE[i] = (float)((double)gE[i] * k.dE);
+                gI[i] = (float)((double)gI[i] * k.dE);
+                adapt[i] = (float)((double)adapt[i] * k.dA);
+                res[i] = (float)((double)res[i] + (1.0 - (double)res[i]) * k.kRec);
+            }
             if (!wasRefr) {
-                const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
-                s = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                const double thr = ex ? (double)((xvTh + adapt[i]) + thrOff[i])
+                                      : p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
+                s = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / beta_of(n, i))) : (vi >= thr ? 1.0 : 0.0);
                 /* the drive as an expectation (src/lifdiff.js:338) */
                 if (n->driveSoft && drive[i] > 0) {
                     const double sd = fmin(DRIVE_MAX, (double)drive[i] * k.dtS);
                     s = sd + (1.0 - sd) * s;
                 }
                 if (s > 0) {
-                    vi = s * p->vReset + (1.0 - s) * vi;
-                    if (!p->soft) refr[i] = (float)p->tRef;
-                    adapt[i] = (float)((double)adapt[i] + s * p->adaptInc);
+                    vi = s * (ex ? (double)xvRs : p->vReset) + (1.0 - s) * vi;
+                    if (!p->soft) refr[i] = ex ? xtRf : (float)p->tRef;
+                    adapt[i] = (float)((double)adapt[i] + s * (ex ? (double)xaI : p->adaptInc));
                 }
                 sc[i] = (float)((double)sc[i] + s);
                 if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + s);
@@ -595,7 +688,7 @@ static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
             A->adapt[off + i] = adapt[i];
             if (!wasRefr) {
                 const double thr = p->vThresh + (double)adapt[i] + (n->thrMask[i] ? p->kcThreshold : 0.0) + (double)thrOff[i];
-                sv = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / p->surrogateBeta)) : (vi >= thr ? 1.0 : 0.0);
+                sv = p->soft ? 1.0 / (1.0 + exp(-(vi - thr) / beta_of(n, i))) : (vi >= thr ? 1.0 : 0.0);
                 if (n->driveSoft && drive[i] > 0) {
                     const double sd = fmin(DRIVE_MAX, (double)drive[i] * k.dtS);
                     sv = sd + (1.0 - sd) * sv;
@@ -716,7 +809,7 @@ void fdo_backward(fdo_net *n, int32_t truncate, double *params, float *gLogGain,
                     sd = fmin(DRIVE_MAX, rate * k.dtS);
                     st = (s - sd) / (1.0 - sd);
                 }
-                const double beta = p->surrogateBeta;
+                const double beta = beta_of(n, i);
                 double sg;
                 if (p->soft) sg = st * (1.0 - st) / beta;
                 else { const double q = 1.0 + fabs(d) / beta; sg = 1.0 / (beta * q * q); }

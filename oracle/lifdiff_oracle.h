@@ -20,11 +20,15 @@ typedef struct {
     double surrogateBeta, adjClip;
     double sizeAlpha, maxSizeScale, boostCap;
     int32_t soft, checkpoint;
+    int32_t exact32;   /* lif.c's binary32 arithmetic in the forward pass (src/lifdiff.js:105-109) */
 } fdo_params;
 
-/* per-neuron inputs */
+/* per-neuron inputs.  FDO_INSCALE_IN is a caller-given inScale taken verbatim
+ * instead of the one formed from sizeLog (opts.inScale, src/lifdiff.js:115-118,
+ * 160); FDO_BETA a per-neuron surrogateBeta (src/lifdiff.js:89-95).  A NULL
+ * value array clears either of these two. */
 enum {
-    FDO_OUTSCALE = 0, FDO_LOGGAIN, FDO_THROFFSET, FDO_SIZELOG, FDO_DRIVE, FDO_NINPUTS
+    FDO_OUTSCALE = 0, FDO_LOGGAIN, FDO_THROFFSET, FDO_SIZELOG, FDO_DRIVE, FDO_INSCALE_IN, FDO_BETA, FDO_NINPUTS
 };
 /* state arrays after forward() */
 enum {
@@ -64,7 +68,8 @@ void fdo_set_loss_per_epoch(fdo_net *n, const float *dPer, int32_t nEp, int32_t 
 /* dL/d sched [nEp][nD] of the last backward */
 void fdo_drive_grad(const fdo_net *n, float *gDrive);
 /* Runs `steps` steps from reset with the given seed; records the tape when
- * `record` is nonzero.  Returns the loss. */
+ * `record` is nonzero.  Returns the loss (NaN when exact32 is set with a
+ * non-empty thrMask, which LIFDiff refuses). */
 double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record);
 /* Reverse sweep over the recorded tape.  params: 9 doubles in FDO_G_* order;
  * gLogGain[N], gEdge[E] (dL/d edgeGain), gInScale[N]; clamped[3]. */

@@ -220,14 +220,15 @@ DIFF_FIELDS = ("dt", "vRest", "vThresh", "vReset", "tauM", "tauSyn", "tRef", "de
                "wSyn", "adaptInc", "adaptTau", "depU", "depTau",
                "eExc", "eInh", "inhGain", "kcThreshold", "laminaBias",
                "surrogateBeta", "adjClip", "sizeAlpha", "maxSizeScale", "boostCap")
-DIFF_INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive")
+DIFF_INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive", "inScale", "surrogateBeta")
 DIFF_STATE = ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount", "inScale")
 DIFF_PARAMS = ("wSyn", "sizeAlpha", "inhGain", "eInh", "vThresh", "kcThreshold", "laminaBias",
                "adaptInc", "depU")
 
 
 class _DiffParams(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in DIFF_FIELDS] + [("soft", ctypes.c_int32), ("checkpoint", ctypes.c_int32)]
+    _fields_ = [(n, ctypes.c_double) for n in DIFF_FIELDS] + [("soft", ctypes.c_int32), ("checkpoint", ctypes.c_int32),
+                                                              ("exact32", ctypes.c_int32)]
 
 
 def _diff_lib():
@@ -285,6 +286,7 @@ class OracleDiff:
             setattr(ps, n, float(self.p[n]))
         ps.soft = 1 if self.p["soft"] else 0
         ps.checkpoint = int(self.p["checkpoint"])
+        ps.exact32 = 1 if self.p.get("exact32") else 0
         self.h = self.lib.fdo_create(self.N, self.E, _ptr(self._indptr), _ptr(self._indices), _ptr(self._counts),
                                      _ptr(self._sign), ctypes.byref(ps))
         self._keep = []
@@ -296,6 +298,12 @@ class OracleDiff:
             self.h = None
 
     def set_input(self, name: str, values):
+        """A per-neuron input; None clears a caller-given inScale or the
+        per-neuron surrogateBeta (back to the derived inScale, the scalar)."""
+        if values is None:
+            assert name in ("inScale", "surrogateBeta")
+            self.lib.fdo_set_input(self.h, DIFF_INPUTS.index(name), None)
+            return
         a = np.ascontiguousarray(values, dtype=np.float32)
         assert a.size == self.N
         self.lib.fdo_set_input(self.h, DIFF_INPUTS.index(name), _ptr(a))

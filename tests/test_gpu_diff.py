@@ -379,3 +379,68 @@ def test_gpu_reproduces_the_reference_grad_check(published):
         for i, pub in G.PUBLISHED_LOGGAIN.items():
             assert abs(float(gg["logGain"][i]) - pub) <= 1e-5 * abs(pub) + 5e-7, (i, gg["logGain"][i], pub)
     net.destroy()
+
+
+@pytest.mark.parametrize("over", [{}, {"depU": 0.0}, {"dt": 0.1, "tRef": 2.2}])
+def test_exact32_forward_is_lifc_bit_for_bit(over):
+    """exact32 (src/lifdiff.js:105-109) on the GPU: the tape, every state
+    array and the stream equal the oracle's exact32 run, which equals the
+    reference's lif.c bit for bit (tests/test_exact32.py)."""
+    import exact32cases as xc
+    from oracle import available
+    print(f"exact32 forward on the GPU: {over}", flush=True)
+    c = xc.Case(over=over)
+    steps = 400
+    o = c.oracle()
+    o.forward(steps, seed=3)
+    net = c.apply(diff.LIFDiffHip(c.g.N, c.g.indptr, c.g.indices, c.counts, c.sign, c.p))
+    net.forward(steps, seed=3)
+    off_o, idx_o, amp_o, f_o = o.spikes()
+    off_g, idx_g, amp_g, f_g = net.spikes()
+    assert np.array_equal(off_o, off_g) and np.array_equal(idx_o, idx_g) and np.array_equal(f_o, f_g)
+    for k in STATE:
+        assert np.array_equal(o.state(k).view(np.uint32), net.state(k).view(np.uint32)), k
+    assert o.rng == net.rng
+    b = c.lifc(seed=3, kind="ref" if available("ref") else "oracle")
+    nf, ras, _ = b.run(steps, raster_cap=steps * c.g.N)
+    assert np.array_equal(np.diff(off_g), nf) and np.array_equal(idx_g, ras)
+    assert np.array_equal(net.state("v").view(np.uint32), b.array("v").view(np.uint32))
+    net.destroy()
+
+
+def test_exact32_refuses_a_threshold_mask():
+    import exact32cases as xc
+    c = xc.Case()
+    net = c.apply(diff.LIFDiffHip(c.g.N, c.g.indptr, c.g.indices, c.counts, c.sign, c.p))
+    m = np.zeros(c.g.N, np.uint8)
+    m[3] = 1
+    net.set_masks(m, c.bias_mask)
+    with pytest.raises(RuntimeError, match="thrMask"):
+        net.forward(5, seed=1)
+    net.destroy()
+
+
+def test_per_neuron_beta_and_given_in_scale_match_oracle():
+    """A split surrogateBeta (src/lifdiff.js:89-95) and a caller-given
+    inScale (src/lifdiff.js:115-118): GPU loss and gradients against the
+    oracle, soft model, to 1e-5 relative."""
+    print("per-neuron beta and given inScale on the GPU", flush=True)
+    pr = dc.Problem()
+    beta = np.full(pr.g.N, pr.p["surrogateBeta"], np.float32)
+    beta[: pr.g.N // 4] = 3.0
+    inS = (0.3 + np.random.default_rng(3).random(pr.g.N)).astype(np.float32)
+    pr.inputs["surrogateBeta"] = beta
+    pr.inputs["inScale"] = inS
+    o, g, Lo, Lg = _pair(pr)
+    go, gg = o.backward(), g.backward(edges=True)
+    assert abs(Lg - Lo) <= 1e-5 * abs(Lo)
+    for k in P.DIFF_PARAMS:
+        assert abs(gg["params"][k] - go["params"][k]) <= 1e-5 * abs(go["params"][k]) + 1e-9, k
+    assert rel_err(gg["logGain"], go["logGain"]) <= 1e-5
+    assert rel_err(gg["inScale"], go["inScale"]) <= 1e-5
+    assert np.array_equal(g.state("inScale"), inS)
+    g.set_input("inScale", None)
+    g.set_input("surrogateBeta", None)
+    L0 = g.forward(STEPS, seed=1)
+    assert abs(L0 - dc.Problem().oracle().forward(STEPS, seed=1)) <= 1e-5 * abs(L0)
+    g.destroy()

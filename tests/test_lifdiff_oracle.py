@@ -373,3 +373,61 @@ def test_hard_schedule_matches_lif_c_with_set_drive_each_epoch(oracle_built):
     assert forced.all() and np.diff(off).tolist() == nfs
     assert sidx.tolist() == ras
     assert od.rng == ob.rng
+
+
+def test_per_neuron_beta_equal_to_the_scalar_changes_nothing(base):
+    """surrogateBeta as a Float32Array (src/lifdiff.js:89-95): the same value
+    on every neuron gives the scalar run's loss and gradients exactly."""
+    pr, L, g = base
+    net = pr.oracle()
+    net.set_input("surrogateBeta", np.full(pr.g.N, pr.p["surrogateBeta"], np.float32))
+    L2 = net.forward(STEPS, seed=1)
+    g2 = net.backward()
+    assert L2 == L
+    assert g2["params"] == g["params"]
+    assert np.array_equal(g2["logGain"], g["logGain"])
+
+
+def test_per_neuron_beta_gradients_match_finite_differences():
+    """A split surrogate (a wider one on a quarter of the neurons, as the
+    reference gives visual projection neurons, src/diffsetup.js:93-101) on the
+    soft model: the nine scalar adjoints against central differences, and the
+    array is read per neuron (a changed entry changes the run)."""
+    pr = dc.Problem()
+    beta = np.full(pr.g.N, pr.p["surrogateBeta"], np.float32)
+    beta[: pr.g.N // 4] = 3.0
+    pr.inputs["surrogateBeta"] = beta
+    net = pr.oracle()
+    L = net.forward(STEPS, seed=1)
+    g = net.backward()
+    assert L != _loss(dc.Problem())
+    for name in P.DIFF_PARAMS:
+        h = dc.H[name]
+        fd = (_loss(pr, params={name: pr.p[name] + h}) - _loss(pr, params={name: pr.p[name] - h})) / (2 * h)
+        ok, rel = dc.fd_ok(g["params"][name], fd, h, L)
+        assert ok, (name, g["params"][name], fd, rel)
+
+
+def test_given_in_scale_is_taken_verbatim(base):
+    """opts.inScale (src/lifdiff.js:115-118, 160): the derived inScale handed
+    back as an input gives the same run; gInScale is then dL/d inScale, checked
+    against central differences of the given array."""
+    pr, L, g = base
+    derived = pr.oracle()
+    derived.forward(1, seed=1, record=False)
+    inS = derived.state("inScale").copy()
+    net = pr.oracle(inputs={"inScale": inS})
+    assert net.forward(STEPS, seed=1) == L
+    g2 = net.backward()
+    assert g2["params"] == g["params"]
+    h = 1e-2
+    for q in np.argsort(-np.abs(g2["inScale"]))[:4].tolist():
+        up, dn = inS.copy(), inS.copy()
+        up[q] = np.float32(float(up[q]) + h)
+        dn[q] = np.float32(float(dn[q]) - h)
+        fd = (_loss(pr, inputs={"inScale": up}) - _loss(pr, inputs={"inScale": dn})) / (2 * h)
+        ok, rel = dc.fd_ok(float(g2["inScale"][q]), fd, h, L)
+        assert ok, (q, g2["inScale"][q], fd, rel)
+    # cleared, the derived inScale is back
+    net.set_input("inScale", None)
+    assert net.forward(STEPS, seed=1) == L
