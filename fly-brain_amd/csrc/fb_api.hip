@@ -491,6 +491,7 @@ extern "C" fb_status fb_brain_create(fb_graph *g, const fb_params *p, double see
     FB_HIP(hipGetLastError());
     FB_HIP(hipStreamSynchronize(b->stream));
     b->inDriven.assign((size_t)N, 0);
+    b->drivenList.reserve((size_t)N);   // the driven set never reallocates (fb_apply_deltas_noalloc)
     *out = b;
     return FB_OK;
 }
@@ -660,17 +661,16 @@ __global__ void k_apply_deltas(DevBrain d, const int32_t *gOff, const int32_t *i
     arr[i] = x;
 }
 
-extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind,
-                                     const double *val) {
-    FB_REQUIRE(b && n >= 0 && (n == 0 || (idx && kind && val)), FB_ERR_ARG, "fb_apply_deltas: bad arguments");
+// the queue's checks and the host side of the setters, in queue order: the
+// driven set and the sign bookkeeping of the excitatory conductances (no
+// allocation: the driven list has room for every neuron)
+static fb_status deltas_host_side(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind, const double *val,
+                                  const char *who) {
+    FB_REQUIRE(b && n >= 0 && (n == 0 || (idx && kind && val)), FB_ERR_ARG, "%s: bad arguments", who);
     for (int32_t q = 0; q < n; q++) {
-        FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "fb_apply_deltas: index %d out of range", idx[q]);
-        FB_REQUIRE(kind[q] >= FB_DELTA_DRIVE && kind[q] <= FB_DELTA_GI, FB_ERR_ARG, "fb_apply_deltas: bad kind %d",
-                   kind[q]);
+        FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "%s: index %d out of range", who, idx[q]);
+        FB_REQUIRE(kind[q] >= FB_DELTA_DRIVE && kind[q] <= FB_DELTA_GI, FB_ERR_ARG, "%s: bad kind %d", who, kind[q]);
     }
-    if (n == 0) return FB_OK;
-    // host side of the setters, in queue order: the driven set and the sign
-    // bookkeeping of the excitatory conductances
     for (int32_t q = 0; q < n; q++) {
         const int32_t i = idx[q];
         if (kind[q] == FB_DELTA_DRIVE) {
@@ -685,8 +685,17 @@ extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx,
             b->gENonneg = false;
         }
     }
+    return FB_OK;
+}
+
+static int32_t delta_array_of(int32_t k) { return k <= FB_DELTA_THR ? k : (k == FB_DELTA_GE ? 3 : 4); }
+
+extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind,
+                                     const double *val) {
+    fb_status st0 = deltas_host_side(b, n, idx, kind, val, "fb_apply_deltas");
+    if (st0 || n == 0) return st0;
     // group by (array, neuron), keeping queue order inside a group
-    auto arrayOf = [](int32_t k) { return k <= FB_DELTA_THR ? k : (k == FB_DELTA_GE ? 3 : 4); };
+    auto arrayOf = delta_array_of;
     std::vector<int32_t> perm((size_t)n);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) {
@@ -719,6 +728,78 @@ extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx,
                        (const int32_t *)dIdx, (const int32_t *)dKind, (const double *)dVal, ng);
     FB_HIP(hipGetLastError());
     FB_HIP(hipStreamSynchronize(b->stream));   // the host vectors and the staging buffer are reused
+    return FB_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free deltas
+//
+// fb_apply_deltas_noalloc: the same queue applied with no device or host
+// allocation, for callers that cannot allocate (code under HIP graph capture,
+// tight serving loops).  The queue's staging lives in `scratch`, one array
+// per device for the life of the process; the grouping sort in host arrays of
+// the same bound.  Every call on a device shares them, so calls that overlap on
+// one GPU race (a mutex orders calls from this process's threads).  No part of
+// the staging needs clearing: a call writes every word it reads.
+constexpr int32_t NOALLOC_MAX_DELTAS = 65536;   // LIFGpu's delta queue bound (src/lifgpu.js)
+constexpr size_t NOALLOC_VAL = 0, NOALLOC_IDX = NOALLOC_VAL + sizeof(double) * NOALLOC_MAX_DELTAS,
+                 NOALLOC_KIND = NOALLOC_IDX + sizeof(int32_t) * NOALLOC_MAX_DELTAS,
+                 NOALLOC_OFF = NOALLOC_KIND + sizeof(int32_t) * NOALLOC_MAX_DELTAS,
+                 NOALLOC_BYTES = NOALLOC_OFF + sizeof(int32_t) * (NOALLOC_MAX_DELTAS + 1);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noallocMu;
+static unsigned char *g_scratchPtr[NOALLOC_MAX_DEVICES];   // scratch's device address, per device
+static uint64_t g_key[NOALLOC_MAX_DELTAS];
+static int32_t g_sIdx[NOALLOC_MAX_DELTAS], g_sKind[NOALLOC_MAX_DELTAS], g_gOff[NOALLOC_MAX_DELTAS + 1];
+static double g_sVal[NOALLOC_MAX_DELTAS];
+
+extern "C" fb_status fb_apply_deltas_noalloc(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind,
+                                             const double *val) {
+    FB_REQUIRE(n <= NOALLOC_MAX_DELTAS, FB_ERR_ARG, "fb_apply_deltas_noalloc: %d deltas exceed the maximum of %d", n,
+               NOALLOC_MAX_DELTAS);
+    std::lock_guard<std::mutex> lock(g_noallocMu);
+    fb_status st0 = deltas_host_side(b, n, idx, kind, val, "fb_apply_deltas_noalloc");
+    if (st0 || n == 0) return st0;
+    const int dev = b->g->device;
+    FB_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, FB_ERR_UNSUPPORTED, "fb_apply_deltas_noalloc: device %d", dev);
+    if (!g_scratchPtr[dev]) {
+        // once per device, with that device current: the array's own address
+        // on it (the name itself is only a host-side stand-in)
+        int cur = 0;
+        FB_HIP(hipGetDevice(&cur));
+        if (cur != dev) FB_HIP(hipSetDevice(dev));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) FB_HIP(hipSetDevice(cur));
+        FB_REQUIRE(e == hipSuccess && ptr, FB_ERR_DEVICE, "fb_apply_deltas_noalloc: hipGetSymbolAddress failed: %s",
+                   hipGetErrorString(e));
+        g_scratchPtr[dev] = (unsigned char *)ptr;
+    }
+    // group by (array, neuron) keeping queue order: one in-place sort of
+    // array (3 bits) | neuron (31 bits) | queue position (17 bits)
+    for (int32_t q = 0; q < n; q++)
+        g_key[q] = ((uint64_t)delta_array_of(kind[q]) << 48) | ((uint64_t)(uint32_t)idx[q] << 17) | (uint64_t)q;
+    std::sort(g_key, g_key + n);
+    int32_t ng = 0;
+    for (int32_t q = 0; q < n; q++) {
+        const int32_t p = (int32_t)(g_key[q] & 0x1FFFF);
+        g_sIdx[q] = idx[p]; g_sKind[q] = kind[p]; g_sVal[q] = val[p];
+        if (q == 0 || (g_key[q] >> 17) != (g_key[q - 1] >> 17)) g_gOff[ng++] = q;
+    }
+    g_gOff[ng] = n;
+    unsigned char *base = g_scratchPtr[dev];
+    double *dVal = (double *)(base + NOALLOC_VAL);
+    int32_t *dIdx = (int32_t *)(base + NOALLOC_IDX), *dKind = (int32_t *)(base + NOALLOC_KIND),
+            *dOff = (int32_t *)(base + NOALLOC_OFF);
+    FB_HIP(hipMemcpyAsync(dVal, g_sVal, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dIdx, g_sIdx, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dKind, g_sKind, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    FB_HIP(hipMemcpyAsync(dOff, g_gOff, sizeof(int32_t) * ((size_t)ng + 1), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_apply_deltas, dim3((ng + 255) / 256), dim3(256), 0, b->stream, b->d, (const int32_t *)dOff,
+                       (const int32_t *)dIdx, (const int32_t *)dKind, (const double *)dVal, ng);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipStreamSynchronize(b->stream));   // the host arrays and the staging are shared by the next call
     return FB_OK;
 }
 

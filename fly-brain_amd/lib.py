@@ -54,6 +54,7 @@ _SIGS = {
     "fb_set_thr": (I32, [P, I32, D]),
     "fb_add_g": (I32, [P, I32, D, D]),
     "fb_apply_deltas": (I32, [P, I32, P, P, P]),
+    "fb_apply_deltas_noalloc": (I32, [P, I32, P, P, P]),
     "fb_pulse": (I32, [P, P, I32, D]),
     "fb_set_background": (I32, [P, D, D]),
     "fb_driven": (I32, [P, P, I32, PI32]),

@@ -117,6 +117,13 @@ fb_status fb_set_background(fb_brain *b, double hz, double mv);
  * modulate()) pay one launch instead of one per call. */
 enum { FB_DELTA_DRIVE = 0, FB_DELTA_BIAS = 1, FB_DELTA_THR = 2, FB_DELTA_GE = 3, FB_DELTA_GI = 4 };
 fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind, const double *val);
+/* fb_apply_deltas with no allocation, device or host.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  At most 65,536 deltas a call (LIFGpu's
+ * queue bound); more is an FB_ERR_ARG.  The staging is one fixed array per
+ * device, held until the process ends and shared by every call on that
+ * device: calls that overlap on one GPU race. */
+fb_status fb_apply_deltas_noalloc(fb_brain *b, int32_t n, const int32_t *idx, const int32_t *kind, const double *val);
 /* the driven set in insertion order (drivenSet of src/lifwasm.js:51,70-71) */
 fb_status fb_driven(const fb_brain *b, int32_t *out, int32_t cap, int32_t *n);
 
