@@ -753,6 +753,14 @@ extern "C" fb_status fb_batch_set_lanes(fb_batch *b, int32_t signGain, const flo
     return FB_OK;
 }
 
+extern "C" fb_status fb_batch_size(const fb_batch *b, int32_t *N, int32_t *B, int32_t *nTypes) {
+    FB_REQUIRE_B(b && N && B && nTypes, FB_ERR_ARG, "fb_batch_size: bad arguments");
+    *N = b->d.N;
+    *B = b->d.B;
+    *nTypes = b->d.nTypes;
+    return FB_OK;
+}
+
 extern "C" fb_status fb_batch_set_rng(fb_batch *b, const uint32_t *states) {
     FB_REQUIRE_B(b && states, FB_ERR_ARG, "fb_batch_set_rng: bad arguments");
     for (int32_t q = 0; q < b->d.B; q++)
@@ -944,6 +952,7 @@ extern "C" fb_status fb_batch_stat(fb_batch *b, const char *name, double *value)
     else if (n == "lookahead_steps") *value = (double)b->laSteps;
     else if (n == "fallback_steps") *value = (double)b->laFallback;
     else if (n == "lookahead_supported") *value = batch_la_supported(b->d, nullptr) ? 1.0 : 0.0;
+    else if (n == "pools") *value = b->d.poolOf ? (double)b->d.loopPools : 0.0;
     else {
         set_error("fb_batch_stat: unknown statistic '%s'", name);
         return FB_ERR_ARG;

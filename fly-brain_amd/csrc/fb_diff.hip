@@ -1467,7 +1467,9 @@ extern "C" fb_status fb_diff_set_loss_per_epoch(fb_diff *f, const float *dPer, i
         f->d.ep = epochSteps;
         f->nEpPer = nEpochs;
     }
-    f->recorded = false;
+    // the tape does not depend on the weights (backward(tape, dLdSpike,
+    // {dLdSpikePerEpoch}) takes them after the forward pass); only the loss
+    // the forward pass reported does
     return FB_OK;
 }
 
@@ -2039,6 +2041,7 @@ extern "C" fb_status fb_diff_batch_stat(fb_diff_batch *bt, const char *name, dou
     FB_REQUIRE_D(bt && name && value, FB_ERR_ARG, "fb_diff_batch_stat: null argument");
     if (!std::strcmp(name, "forward_ms")) { *value = bt->fwdMs; return FB_OK; }
     if (!std::strcmp(name, "backward_ms")) { *value = bt->bwdMs; return FB_OK; }
+    if (!std::strcmp(name, "members")) { *value = (double)bt->m.size(); return FB_OK; }
     set_error("fb_diff_batch_stat: unknown statistic '%s'", name);
     return FB_ERR_ARG;
 }
@@ -2098,6 +2101,15 @@ extern "C" fb_status fb_diff_readout(fb_diff *f, float *s, double *rate) {
     FB_HIP(hipStreamSynchronize(f->stream));
     if (m && s) FB_HIP(hipMemcpy(s, f->d.readS, sizeof(float) * m, hipMemcpyDeviceToHost));
     if (m && rate) FB_HIP(hipMemcpy(rate, f->d.rate, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_size(const fb_diff *f, int32_t *N, int64_t *E, int32_t *steps, int32_t *R) {
+    FB_REQUIRE_D(f && N && E && steps && R, FB_ERR_ARG, "fb_diff_size: bad arguments");
+    *N = f->N;
+    *E = f->E;
+    *steps = f->steps;
+    *R = f->d.R;
     return FB_OK;
 }
 

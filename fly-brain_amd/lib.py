@@ -76,6 +76,7 @@ _SIGS = {
     "fb_batch_run": (I32, [P, I32, P, P, P, P, I64, PI64]),
     "fb_batch_get_array": (I32, [P, I32, I32, P]),
     "fb_batch_rng": (I32, [P, P]),
+    "fb_batch_size": (I32, [P, PI32, PI32, PI32]),
     "fb_batch_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_batch_set_stream": (I32, [P, P]),
     "fb_batch_set_engine": (I32, [P, I32]),
@@ -126,6 +127,7 @@ _SIGS = {
     "fb_diff_spikes": (I32, [P, P, P, P, I64, PI64]),
     "fb_diff_readout": (I32, [P, P, P]),
     "fb_diff_rng": (I32, [P, P]),
+    "fb_diff_size": (I32, [P, PI32, P, PI32, PI32]),
     "fb_diff_set_stream": (I32, [P, P]),
     "fb_diff_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
     "fb_diff_grads": (I32, [P, P, P, P, P, P]),

@@ -221,6 +221,10 @@ fb_status fb_batch_set_pool_feedback(fb_batch *b, const uint8_t *poolOf, int32_t
 /* dst [B][n][nPools] uint16, n = min(cap, *nRecords) */
 fb_status fb_batch_pool_counts(fb_batch *b, uint16_t *dst, int32_t cap, int32_t *nRecords);
 fb_status fb_batch_rng(fb_batch *b, uint32_t *rng);
+/* neurons, members and types: the lengths the per-member arrays, gains and
+ * per-member lists above take.  No counterpart in the reference; for bindings
+ * that check a caller's buffer before a copy. */
+fb_status fb_batch_size(const fb_batch *b, int32_t *N, int32_t *B, int32_t *nTypes);
 fb_status fb_batch_counters(fb_batch *b, int64_t *edge_events, int64_t *synapse_events, int64_t *spikes,
                             int64_t *steps);
 fb_status fb_batch_set_stream(fb_batch *b, void *stream);
@@ -316,7 +320,9 @@ fb_status fb_diff_set_tape_capacity(fb_diff *d, int64_t perStep);
 fb_status fb_diff_set_drive_schedule(fb_diff *d, const int32_t *idx, int32_t nD, int32_t epochSteps,
                                      const float *sched, int32_t nEpochs, int32_t driveSoft);
 /* dLdSpikePerEpoch (src/lifdiff.js:519-522): the loss weight of a spike of
- * neuron i in epoch e is dPer[e*N + i] (replacing dLdSpike); NULL clears. */
+ * neuron i in epoch e is dPer[e*N + i] (replacing dLdSpike); NULL clears.
+ * Like dLdSpike it may be set between the forward and the backward pass (the
+ * loss the forward pass reported is then the old one's). */
 fb_status fb_diff_set_loss_per_epoch(fb_diff *d, const float *dPer, int32_t nEpochs, int32_t epochSteps);
 /* dL/d sched after fb_diff_backward: [nEpochs][nD] (the `drive` result of
  * LIFDiff.backward, src/lifdiff.js:490-492) */
@@ -343,6 +349,10 @@ fb_status fb_diff_spikes(fb_diff *d, int64_t *off, int32_t *idx, float *amp, int
 /* readout spike amplitudes s[steps][R] and rates[steps][R] of the last forward */
 fb_status fb_diff_readout(fb_diff *d, float *s, double *rate);
 fb_status fb_diff_rng(fb_diff *d, uint32_t *rng);
+/* neurons, edges, steps of the recorded forward pass and readouts: the
+ * lengths the inputs, gradients, tape and readout arrays take.  No
+ * counterpart in the reference; for bindings that check a caller's buffer. */
+fb_status fb_diff_size(const fb_diff *d, int32_t *N, int64_t *E, int32_t *steps, int32_t *R);
 fb_status fb_diff_set_stream(fb_diff *d, void *stream);
 /* "forward_ms", "backward_ms", "edge_events", "arrivals", "steps", "tape_entries" */
 fb_status fb_diff_stat(fb_diff *d, const char *name, double *value);

@@ -195,3 +195,136 @@ def test_short_arrays_raise_range_errors():
     for k in ("getArrayOk", "fvBackwardOk"):
         assert res.pop(k) == "no error", k
     assert res and all(v == "RangeError" for v in res.values()), res
+
+
+# ---------------------------------------------------------------- the gradient and the ensemble from Node
+
+def _node_json(tmp_path, spec, script="diff_run.js"):
+    f = tmp_path / "case.json"
+    f.write_text(json.dumps(spec))
+    out = subprocess.run(["node", os.path.join(HERE, "js", script), str(f)], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout)
+
+
+def test_lifdiffhip_from_node_reproduces_grad_check(tmp_path):
+    """The reference's gradient check (scripts/grad_check.mjs 800 120, the
+    published problem) driven from JavaScript through LIFDiffHip with
+    LIFDiff's own constructor inputs (data {N, indptr, indices, weights, nt},
+    opts with P0 and the LCG's arrays): the published adjoint table
+    (docs/33-differentiable-brain.md:50-62) to 1e-5, and the oracle's
+    gradients to 1e-5."""
+    import gradcheck as G
+    if not G.available():
+        pytest.skip("grad-check subgraph fixture absent")
+    print("LIFDiffHip from Node: grad_check", flush=True)
+    gc = G.GradCheck(published=True)
+    z = np.load(G.FIXTURE, allow_pickle=False)
+    data = {"N": gc.N, "indptr": gc.indptr.tolist(), "indices": gc.indices.tolist(),
+            "weights": gc.raw_counts.astype(int).tolist(), "nt": z["nt"].astype(int).tolist()}
+    opts = {**G.P0, "coba": True, "sizeLog": gc.sizeLog.tolist(), "thrMask": gc.thr_mask.tolist(),
+            "biasMask": gc.bias_mask.tolist(), "logGain": gc.logGain.tolist()}
+    res = _node_json(tmp_path, {"mode": "diff", "data": data, "opts": opts, "steps": G.STEPS, "seed": 1,
+                                "dL": gc.dL.tolist()})
+    o = gc.oracle()
+    Lo = o.forward(G.STEPS, seed=1)
+    go = o.backward()
+    # grad_check's loss: the targets' summed spike counts (grad_check.mjs:77-78)
+    L = float(np.dot(np.array(res["spikeCount"], np.float64), gc.dL.astype(np.float64)))
+    assert abs(L - Lo) <= 1e-5 * abs(Lo)
+    for k, pub in G.PUBLISHED.items():
+        assert abs(res["params"][k] - pub) <= 1e-5 * abs(pub) + 5e-7, (k, res["params"][k], pub)
+        assert abs(res["params"][k] - go["params"][k]) <= 1e-5 * abs(go["params"][k]) + 1e-9, k
+    lg = np.array(res["logGain"], np.float32)
+    for i, pub in G.PUBLISHED_LOGGAIN.items():
+        assert abs(float(lg[i]) - pub) <= 1e-5 * abs(pub) + 5e-7
+    assert float(np.abs(lg - go["logGain"]).max()) <= 1e-5 * float(np.abs(go["logGain"]).max())
+
+
+def test_lifdiffhip_from_node_exact32_is_lifc(tmp_path):
+    """exact32 from Node: the tape and the membrane state are the reference
+    lif.c's, bit for bit, on the exact32 case with the gate, size scaling,
+    outScale, inhGain, depression and adaptation on."""
+    import exact32cases as xc
+    from oracle import available
+    print("LIFDiffHip from Node: exact32", flush=True)
+    c = xc.Case()
+    data = {"N": c.g.N, "indptr": c.g.indptr.tolist(), "indices": c.g.indices.tolist(),
+            "weights": c.g.counts.astype(int).tolist(), "nt": np.where(c.g.exc, 1, 2).tolist()}
+    opts = {k: v for k, v in c.p.items() if k != "checkpoint"}
+    opts.update({"sizeLog": c.sizeLog.tolist(), "outScale": c.outScale.tolist(), "thrOffset": c.thrOffset.tolist(),
+                 "biasMask": c.bias_mask.tolist()})
+    drive = [[int(i), float(c.drive[i])] for i in c.driven]
+    res = _node_json(tmp_path, {"mode": "diff", "data": data, "opts": opts, "drive": drive, "steps": 400,
+                                "seed": 3, "dL": [0.0] * c.g.N})
+    b = c.lifc(seed=3, kind="ref" if available("ref") else "oracle")
+    nf, ras, _ = b.run(400, raster_cap=400 * c.g.N)
+    assert np.array_equal(np.diff(res["spikeOff"]), nf) and res["spikeIdx"] == ras.tolist()
+    assert np.array_equal(np.array(res["v"], np.float32).view(np.uint32), b.array("v").view(np.uint32))
+    assert res["rng"] == b.rng
+
+
+def test_lifdiffbatchhip_from_node_equals_separate_networks(tmp_path):
+    """Three networks with their own gains, drives, losses and windows in one
+    LIFDiffBatchHip from Node give exactly what each gives alone."""
+    import diffcases as dc
+    print("LIFDiffBatchHip from Node", flush=True)
+    pr = dc.Problem(over={"soft": False, "tRef": 2.2})
+    g = pr.g
+    data = {"N": g.N, "indptr": g.indptr.tolist(), "indices": g.indices.tolist(),
+            "weights": g.counts.astype(int).tolist(), "nt": np.where(g.exc, 1, 2).tolist()}
+    rng = np.random.default_rng(5)
+    members = []
+    for b in range(3):
+        opts = {**{k: v for k, v in pr.p.items() if k != "checkpoint"}, "minSyn": 1,
+                "outScale": (pr.inputs["outScale"] * np.exp(rng.normal(0, 0.05, g.N))).astype(np.float32).tolist(),
+                "logGain": pr.inputs["logGain"].tolist(), "thrOffset": pr.inputs["thrOffset"].tolist(),
+                "sizeLog": pr.inputs["sizeLog"].tolist(), "thrMask": pr.thr_mask.tolist(),
+                "biasMask": pr.bias_mask.tolist()}
+        drive = [[int(i), 60.0 + 10 * b] for i in pr.driven]
+        dL = np.zeros(g.N, np.float32)
+        dL[rng.choice(g.N, 40, replace=False)] = 1.0
+        members.append({"opts": opts, "drive": drive, "dL": dL.tolist()})
+    res = _node_json(tmp_path, {"mode": "diffbatch", "data": data, "members": members, "steps": [90, 120, 57],
+                                "seeds": [3, 4, 5]})
+    for b in range(3):
+        x, y = res["batched"][b], res["alone"][b]
+        assert x["loss"] == y["loss"] and x["spikeIdx"] == y["spikeIdx"] and x["spikeOff"] == y["spikeOff"], b
+        assert x["logGain"] == y["logGain"] and x["edge"] == y["edge"], b
+        for k in P.DIFF_PARAMS:
+            if k != "sizeAlpha":
+                assert x["params"][k] == y["params"][k], (b, k)
+
+
+def test_lifbatchhip_from_node_matches_the_oracle(tmp_path):
+    """A 16-member ensemble from Node (own seeds, per-type gains, silenced
+    neurons, uneven run lengths): every member equals its own lif.c oracle
+    run (spike counts, membrane state, stream)."""
+    print("LIFBatchHip from Node", flush=True)
+    g = synth.tiny_graph(1200, 30, seed=31, p_count=0.3)
+    p = P.config_params()
+    B, T = 16, 7
+    rng = np.random.default_rng(2)
+    type_of = rng.integers(0, T, g.N).astype(np.int32)
+    gains = np.exp(rng.normal(0, 0.2, (B, T))).astype(np.float32)
+    drv = list(range(0, g.N, 25))
+    silence = [[b, rng.choice(g.N, 20, replace=False).tolist()] for b in range(B)]
+    steps = rng.integers(100, 300, B).tolist()
+    spec = {"mode": "batch", "N": g.N, "params": p, "seeds": [b + 1 for b in range(B)], "typeOf": type_of.tolist(),
+            "gains": gains.tolist(), "drive": [drv, 150.0], "silence": silence, "steps": steps, "run": max(steps),
+            "graph": {"indptr": g.indptr.tolist(), "indices": g.indices.tolist(), "weights": g.weights.tolist(),
+                      "sign": g.sign.tolist(), "counts": g.counts.tolist()}}
+    res = _node_json(tmp_path, spec)
+    pre = np.repeat(np.arange(g.N), np.diff(g.indptr.astype(np.int64)))
+    for b in range(B):
+        W = (g.counts.astype(np.float64) * gains[b][type_of[pre]].astype(np.float64)).astype(np.float32)
+        o = OracleBrain(g, p, seed=b + 1, W=W, sign=g.sign)
+        o.setDrive(drv, 150.0)
+        for i in silence[b][1]:
+            o.setThr(int(i), 1e6)
+        o.run(steps[b], raster_cap=steps[b] * g.N)
+        m = res["members"][b]
+        assert m["spikeCount"] == o.spikeCount.tolist(), b
+        assert np.array_equal(np.array(m["v"], np.float32).view(np.uint32), o.array("v").view(np.uint32)), b
+        assert m["rng"] == o.rng, b
