@@ -190,7 +190,9 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     c1 = eb.counters()
     ev = c1["edge_events"] - c0["edge_events"]
     la = {"launches": eb.stat("lookahead_launches"), "kernel_ms": eb.stat("lookahead_kernel_ms"),
-          "steps": eb.stat("lookahead_steps"), "fallback_steps": eb.stat("fallback_steps")}
+          "steps": eb.stat("lookahead_steps"), "fallback_steps": eb.stat("fallback_steps"),
+          "shape_neurons_per_block": eb.stat("lookahead_shape"),
+          "autotune_us_per_epoch": {str(k): eb.stat(f"autotune_us_per_epoch_{k}") for k in (80, 96, 128)}}
     out = {"value": ev / (ms / 1000.0), "unit": "synaptic events/s (aggregate over members)", "members": B,
            "engine": "lookahead" if la["steps"] else "per-step", "lookahead": la,
            "steps": T, "member_steps": int(steps.sum()), "ms_per_step": ms / T,

@@ -77,6 +77,11 @@ class LIFBatch:
         ld = None if lane_drive is None else np.ascontiguousarray(lane_drive, dtype=np.float32).reshape(self.B, -1)
         _l.check(self.lib.fb_batch_set_lanes(self.h, 1 if sign_gain else 0, _l.ptr(ld)))
 
+    def set_shape(self, neurons_per_block: int):
+        """The lookahead engine's regular-block shape (80, 96 or 128 neurons per
+        workgroup), or 0 for the autotuner's choice (fb_batch_set_shape)."""
+        _l.check(self.lib.fb_batch_set_shape(self.h, int(neurons_per_block)))
+
     def set_rng(self, states):
         st = np.ascontiguousarray(states, dtype=np.uint32)
         _l.check(self.lib.fb_batch_set_rng(self.h, _l.ptr(st)))

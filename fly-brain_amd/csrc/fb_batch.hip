@@ -27,6 +27,7 @@
 //                active list.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -480,13 +481,26 @@ struct fb_batch {
     uint64_t drivenVersion = 0;
     int64_t laLaunches = 0, laFallback = 0, laSteps = 0;
     double laMs = 0.0;
+    // the lookahead engine's regular-block shape: chosen by the autotuner
+    // (laShape, -1 until measured) or forced by the caller (laForced)
+    int32_t laShape = -1, laForced = -1;
+    double tuneMs[LA_NSHAPES] = {0, 0, 0};
+    int32_t tuneEpochs[LA_NSHAPES] = {0, 0, 0};
+    struct Trial {
+        int32_t shape, epochs;
+        bool timed;
+    };
+    std::vector<Trial> tunePlan;      // launches still to run before the choice
 };
 
 namespace fb {
 bool batch_la_supported(const BatchDev &d, std::string *why);
+bool batch_la_shape_fits(const BatchDev &d, int k);
+int32_t batch_la_shape_neurons(int k);
 void batch_la_release(const void *key);
-fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t epochs, uint64_t drivenVersion,
-                          uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done, double *kernelMs);
+fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, int32_t epochs,
+                          uint64_t drivenVersion, uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done,
+                          double *kernelMs);
 }
 
 // graph internals needed here (defined in fb_api.hip)
@@ -862,8 +876,36 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
     const int32_t recEpochs = std::max(1, 512 / std::max(1, D));
     uint64_t *recM = nullptr, *recF = nullptr;
     std::vector<uint64_t> hM, hF;
+    // The autotuner: the first run long enough runs its first epochs on every
+    // shape that fits, twice round, keeps the fastest for the rest of the run
+    // and every later one.  All shapes give the same bits, so the trials are
+    // the run's own epochs.
+    constexpr int32_t TRIAL_EPOCHS = 2;
+    std::vector<int32_t> fits;
+    for (int k = 0; k < LA_NSHAPES; k++)
+        if (la && batch_la_shape_fits(d, k)) fits.push_back(k);
+    if (la && b->laForced < 0 && b->laShape < 0 && b->tunePlan.empty() && !fits.empty()) {
+        if (fits.size() == 1) {
+            b->laShape = fits[0];
+        } else if ((nsteps - s0) / D >= 2 + 2 * (int32_t)fits.size() * TRIAL_EPOCHS) {
+            b->tunePlan.push_back({fits[0], 1, false});        // out of the transient of a fresh start
+            for (int round = 0; round < 2; round++)
+                for (int32_t k : fits) b->tunePlan.push_back({k, TRIAL_EPOCHS, true});
+            for (int k = 0; k < LA_NSHAPES; k++) { b->tuneMs[k] = 0.0; b->tuneEpochs[k] = 0; }
+        }
+    }
     while (la && nsteps - s0 >= D) {
         int32_t epochs = (nsteps - s0) / D;
+        int32_t shape = b->laForced >= 0 ? b->laForced : b->laShape >= 0 ? b->laShape : (fits.empty() ? 0 : fits[0]);
+        bool timed = false;
+        const bool trial = b->laForced < 0 && !b->tunePlan.empty();
+        if (trial) {
+            const fb_batch::Trial tr = b->tunePlan.front();
+            b->tunePlan.erase(b->tunePlan.begin());
+            shape = tr.shape;
+            epochs = std::min(epochs, tr.epochs);
+            timed = tr.timed;
+        }
         if (rec) {
             epochs = std::min(epochs, recEpochs);
             const size_t n = (size_t)epochs * D * d.N;
@@ -875,7 +917,9 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
             FB_HIP(hipMemsetAsync(recF, 0, sizeof(uint64_t) * n, b->stream));
         }
         int32_t done = 0;
-        st = batch_la_launch(d, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done, &b->laMs);
+        const double ms0 = b->laMs;
+        st = batch_la_launch(d, shape, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done,
+                             &b->laMs);
         if (st == FB_ERR_UNSUPPORTED && b->engine == FB_ENGINE_AUTO) {
             // the device refused the cooperative launch: the per-step kernels
             la = false;
@@ -883,6 +927,18 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
         }
         if (st) return st;
         b->laLaunches++;
+        if (timed && done == epochs) {
+            b->tuneMs[shape] += b->laMs - ms0;
+            b->tuneEpochs[shape] += done;
+        }
+        if (trial && b->tunePlan.empty()) {
+            int best = -1;
+            for (int32_t k : fits)
+                if (b->tuneEpochs[k] > 0 &&
+                    (best < 0 || b->tuneMs[k] / b->tuneEpochs[k] < b->tuneMs[best] / b->tuneEpochs[best]))
+                    best = k;
+            b->laShape = best >= 0 ? best : fits[0];
+        }
         const int32_t m = done * D;
         if (rec && m > 0) {
             hM.resize((size_t)m * d.N);
@@ -932,6 +988,29 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
     return FB_OK;
 }
 
+// The lookahead engine's regular-block shape: 80, 96 or 128 neurons per
+// block, or 0 for the autotuner's choice (measured again on the next run long
+// enough).  All shapes give the same bits.
+extern "C" fb_status fb_batch_set_shape(fb_batch *b, int32_t neuronsPerBlock) {
+    FB_REQUIRE_B(b, FB_ERR_ARG, "fb_batch_set_shape: null batch");
+    b->tunePlan.clear();
+    if (neuronsPerBlock == 0) {
+        b->laForced = -1;
+        b->laShape = -1;
+        return FB_OK;
+    }
+    for (int k = 0; k < LA_NSHAPES; k++)
+        if (batch_la_shape_neurons(k) == neuronsPerBlock) {
+            FB_REQUIRE_B(batch_la_shape_fits(b->d, k), FB_ERR_UNSUPPORTED,
+                         "fb_batch_set_shape: %d neurons per block does not fit the device for this ensemble",
+                         neuronsPerBlock);
+            b->laForced = k;
+            return FB_OK;
+        }
+    set_error("fb_batch_set_shape: %d is not a shape (80, 96, 128 or 0)", neuronsPerBlock);
+    return FB_ERR_ARG;
+}
+
 // FB_ENGINE_AUTO (lookahead when the ensemble allows it), FB_ENGINE_LAUNCH
 // (per-step kernels) or FB_ENGINE_PERSISTENT (lookahead, or an error)
 extern "C" fb_status fb_batch_set_engine(fb_batch *b, int32_t engine) {
@@ -953,7 +1032,16 @@ extern "C" fb_status fb_batch_stat(fb_batch *b, const char *name, double *value)
     else if (n == "fallback_steps") *value = (double)b->laFallback;
     else if (n == "lookahead_supported") *value = batch_la_supported(b->d, nullptr) ? 1.0 : 0.0;
     else if (n == "pools") *value = b->d.poolOf ? (double)b->d.loopPools : 0.0;
-    else {
+    else if (n == "lookahead_shape") {
+        const int k = b->laForced >= 0 ? b->laForced : b->laShape;
+        *value = k >= 0 ? (double)batch_la_shape_neurons(k) : 0.0;
+    } else if (n.rfind("autotune_us_per_epoch_", 0) == 0) {
+        // the autotuner's measurement of one shape (0: not measured)
+        const int32_t nb = std::atoi(n.c_str() + 22);
+        *value = 0.0;
+        for (int k = 0; k < LA_NSHAPES; k++)
+            if (batch_la_shape_neurons(k) == nb && b->tuneEpochs[k] > 0) *value = 1e3 * b->tuneMs[k] / b->tuneEpochs[k];
+    } else {
         set_error("fb_batch_stat: unknown statistic '%s'", name);
         return FB_ERR_ARG;
     }

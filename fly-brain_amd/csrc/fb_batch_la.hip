@@ -57,8 +57,10 @@ namespace fb {
 
 constexpr int BT = 512;                    // threads per block
 constexpr int BWV = BT / 64;               // waves per block
-constexpr int NPW = 10;                    // neurons per wave, regular block
-constexpr int RNB = BWV * NPW;             // neurons per regular block
+// A regular block's shape -- NPW neurons per wave, RNB = 8 NPW neurons per
+// block -- is a template parameter: the autotuner picks it per ensemble
+// (LA_SHAPES below).  Every shape gives the same bits: a block's shape decides
+// which block simulates a neuron, never the order of any addition.
 constexpr int MPG = 4;                     // members per driven block
 constexpr int SPW = 64 / MPG;              // slices per wave
 constexpr int NSL = SPW * BWV;             // slices of the driven list per member
@@ -89,6 +91,7 @@ struct LaArgs {
     uint64_t *recM, *recF;                 // optional dense record [epochs*D][N]
 };
 
+template <int RNB>
 struct LaShared {
     uint32_t cell[CELLS_D / 2];            // two u16 counters per word: cell ends after the gather
     uint64_t mask[LEV];
@@ -123,7 +126,8 @@ struct LaShared {
     int32_t nEv, nSpk, nAdd;
 };
 
-__device__ __forceinline__ uint32_t cell_end(const LaShared &s, int32_t c) {
+template <int RNB>
+__device__ __forceinline__ uint32_t cell_end(const LaShared<RNB> &s, int32_t c) {
     return (s.cell[c >> 1] >> ((c & 1) * 16)) & 0xFFFFu;
 }
 
@@ -134,14 +138,16 @@ __device__ __forceinline__ uint32_t jump_to(const uint32_t *leapS, uint32_t x, i
 }
 
 // members running at absolute step t
-__device__ __forceinline__ uint64_t on_mask(const LaShared &s, int32_t B, int32_t t) {
+template <int RNB>
+__device__ __forceinline__ uint64_t on_mask(const LaShared<RNB> &s, int32_t B, int32_t t) {
     const int lane = threadIdx.x & 63;
     return __ballot(lane < B && t < s.steps[lane]);
 }
 
 // ------------------------------------------------------------------ gather
 
-__device__ void la_gather(const LaArgs &a, LaShared &s, int32_t par, int32_t me, bool drv) {
+template <int RNB>
+__device__ void la_gather(const LaArgs &a, LaShared<RNB> &s, int32_t par, int32_t me, bool drv) {
     const int tid = threadIdx.x;
     const int32_t CW = drv ? BDRV : RNB;
     const int32_t ncells = a.D * 2 * CW;
@@ -237,7 +243,8 @@ __device__ void la_gather(const LaArgs &a, LaShared &s, int32_t par, int32_t me,
 }
 
 // the arrivals of one cell, in order, for this lane's member
-__device__ __forceinline__ float apply_cell(const LaShared &s, int32_t c, float g, int lane) {
+template <int RNB>
+__device__ __forceinline__ float apply_cell(const LaShared<RNB> &s, int32_t c, float g, int lane) {
     const int32_t st = c ? (int32_t)cell_end(s, c - 1) : 0, en = (int32_t)cell_end(s, c);
     const uint64_t below = (1ull << lane) - 1ull;
     for (int32_t k = st; k < en; k++) {
@@ -255,7 +262,8 @@ __device__ __forceinline__ float apply_cell(const LaShared &s, int32_t c, float 
 // Appends a spike entry (neuron, step, forced?, mask) to the block's list.
 // rank: the spike's position class in lif.c's list (driven position when it
 // was emitted for a forced spike, N + index otherwise)
-__device__ __forceinline__ void push_entry(const LaArgs &a, LaShared &s, int32_t pre, int32_t step, bool forced,
+template <int RNB>
+__device__ __forceinline__ void push_entry(const LaArgs &a, LaShared<RNB> &s, int32_t pre, int32_t step, bool forced,
                                            uint32_t rank, uint64_t m) {
     if (a.d.g.sign[pre] == 0.f) return;            // delivers nothing (lif.c:37)
     const int32_t k = atomicAdd(&s.nSpk, 1);
@@ -285,7 +293,8 @@ __device__ __forceinline__ int32_t next_dst(const LaArgs &a, uint32_t own, uint6
 
 // Pushes the block's spike list into the buckets of parity `par`; the events
 // arrive at absolute steps tArr0 + step.
-__device__ void la_push(const LaArgs &a, LaShared &s, int32_t par, int32_t tArr0) {
+template <int RNB>
+__device__ void la_push(const LaArgs &a, LaShared<RNB> &s, int32_t par, int32_t tArr0) {
     const int tid = threadIdx.x;
     const BatchDev &d = a.d;
     __syncthreads();
@@ -380,7 +389,8 @@ __device__ void la_push(const LaArgs &a, LaShared &s, int32_t par, int32_t tArr0
 
 // The delay line's pending spikes (arrivals at t0 .. t0 + D - 1): block `me`
 // takes every nblk-th entry of the D slot lists.
-__device__ void la_prologue(const LaArgs &a, LaShared &s, int32_t me, int32_t nblk) {
+template <int RNB>
+__device__ void la_prologue(const LaArgs &a, LaShared<RNB> &s, int32_t me, int32_t nblk) {
     const BatchDev &d = a.d;
     if (threadIdx.x == 0) s.nSpk = 0;
     __syncthreads();
@@ -405,8 +415,11 @@ __device__ void la_prologue(const LaArgs &a, LaShared &s, int32_t me, int32_t nb
 
 // ------------------------------------------------------------------ the kernel
 
+template <int NPW>
 __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
-    __shared__ LaShared s;
+    constexpr int RNB = BWV * NPW;
+    static_assert(sizeof(LaShared<RNB>) <= 160 * 1024, "LDS of one workgroup");
+    __shared__ LaShared<RNB> s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t me = blockIdx.x;
     const BatchDev &d = a.d;
@@ -676,43 +689,60 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
 
 // owner of each edge's target: a driven neuron's list position (with the
 // driven bit), else regular block << 8 | position in the block
-__global__ void k_batch_eown(const uint2 *edges, int64_t E, const int32_t *drvPos, uint32_t *eown) {
+__global__ void k_batch_eown(const uint2 *edges, int64_t E, const int32_t *drvPos, uint32_t *eown, int32_t rnb) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < E; j += (int64_t)gridDim.x * blockDim.x) {
         const int32_t q = (int32_t)edges[j].x;
         const int32_t dp = drvPos[q];
-        eown[j] = dp >= 0 ? (OWN_DRV | (uint32_t)dp) : ((uint32_t)(q / RNB) << 8) | (uint32_t)(q % RNB);
+        eown[j] = dp >= 0 ? (OWN_DRV | (uint32_t)dp) : ((uint32_t)(q / rnb) << 8) | (uint32_t)(q % rnb);
     }
 }
 
 // ------------------------------------------------------------------ host side
+
+// The shapes the autotuner chooses among: neurons per wave of a regular block
+// (80, 96 or 128 neurons per block).  Fewer, larger regular blocks mean fewer
+// destinations to push to and gather from per epoch, and a longer serial walk
+// per wave per step; which wins depends on the graph, the activity and the
+// device, so the engine measures it.
+constexpr int LA_SHAPES[LA_NSHAPES] = {10, 12, 16};
+
+static const void *la_kernel(int k) {
+    switch (k) {
+        case 0: return (const void *)k_batch_la<10>;
+        case 1: return (const void *)k_batch_la<12>;
+        default: return (const void *)k_batch_la<16>;
+    }
+}
+
+int32_t batch_la_shape_neurons(int k) { return BWV * LA_SHAPES[k]; }
 
 struct LaBuffers {
     uint64_t *evKey = nullptr, *evMask = nullptr, *evAux = nullptr;
     int32_t *dstCnt = nullptr, *ctl = nullptr;   // dstCnt holds [2][S] events, then [2][S] addends
     int32_t *bar = nullptr;
     uint32_t *eown = nullptr;
-    int32_t S = 0;
+    int32_t S = 0;                               // destinations the buffers hold (grown, never shrunk)
     int64_t E = 0;
-    const int32_t *drvPosKey = nullptr;
+    int32_t eownRnb = 0;
     uint64_t drivenVersion = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 static std::map<const void *, LaBuffers> g_labuf;    // keyed by the ensemble's device state
 
-static int32_t la_max_blocks() {
-    static int32_t maxB = -1;
-    if (maxB < 0) {
+static int32_t la_max_blocks(int k) {
+    static int32_t maxB[LA_NSHAPES] = {-1, -1, -1};
+    if (maxB[k] < 0) {
         int dev = 0, ncu = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_batch_la, BT, 0) != hipSuccess) return 0;
-        maxB = ncu * per;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, la_kernel(k), BT, 0) != hipSuccess) return 0;
+        maxB[k] = ncu * per;
     }
-    return maxB;
+    return maxB[k];
 }
 
-bool batch_la_supported(const BatchDev &d, std::string *why) {
+static bool la_common(const BatchDev &d, std::string *why) {
     const int32_t D = d.nslots - 1;
     auto no = [&](const char *w) { if (why) *why = w; return false; };
     if (d.liveAdapt || d.liveRes || d.liveBg) return no("adaptation, depression or background is on");
@@ -721,11 +751,23 @@ bool batch_la_supported(const BatchDev &d, std::string *why) {
     if (d.nDriven > BDRV) return no("more than 384 driven neurons");
     if (d.g.maxRow >= (1 << 23)) return no("a row longer than 2^23");
     if ((int64_t)2 * d.N >= (1 << 24)) return no("N too large for the arrival rank");
-    const int32_t R = (d.N + RNB - 1) / RNB, G = (d.B + MPG - 1) / MPG;
-    if (R + G > LS_MAX) return no("too many destinations");
-    const int32_t maxB = la_max_blocks();
-    if (R + G > maxB || R + G > GBAR_MAX_BLOCKS) return no("the grid does not fit the device");
     return true;
+}
+
+bool batch_la_shape_fits(const BatchDev &d, int k) {
+    const int32_t rnb = batch_la_shape_neurons(k);
+    const int32_t R = (d.N + rnb - 1) / rnb, G = (d.B + MPG - 1) / MPG;
+    if (R + G > LS_MAX) return false;
+    const int32_t maxB = la_max_blocks(k);
+    return R + G <= maxB && R + G <= GBAR_MAX_BLOCKS;
+}
+
+bool batch_la_supported(const BatchDev &d, std::string *why) {
+    if (!la_common(d, why)) return false;
+    for (int k = 0; k < LA_NSHAPES; k++)
+        if (batch_la_shape_fits(d, k)) return true;
+    if (why) *why = "the grid does not fit the device";
+    return false;
 }
 
 void batch_la_release(const void *key) {
@@ -740,14 +782,17 @@ void batch_la_release(const void *key) {
     g_labuf.erase(it);
 }
 
-// Runs `epochs` whole epochs from absolute step t0 with ring head head0.
-// Returns the epochs actually done in *done (fewer after a bucket overflow).
-fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t epochs, uint64_t drivenVersion,
-                          uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done, double *kernelMs) {
+// Runs `epochs` whole epochs from absolute step t0 with ring head head0, with
+// regular blocks of shape k.  Returns the epochs actually done in *done (fewer
+// after a bucket overflow) and adds the kernel's HIP-event time to *kernelMs.
+fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, int32_t epochs,
+                          uint64_t drivenVersion, uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done,
+                          double *kernelMs) {
     const int32_t D = d.nslots - 1;
-    const int32_t R = (d.N + RNB - 1) / RNB, G = (d.B + MPG - 1) / MPG, S = R + G;
+    const int32_t rnb = batch_la_shape_neurons(k);
+    const int32_t R = (d.N + rnb - 1) / rnb, G = (d.B + MPG - 1) / MPG, S = R + G;
     LaBuffers &b = g_labuf[(const void *)d.v];
-    if (b.S != S) {
+    if (b.S < S) {
         for (void *p : {(void *)b.evKey, (void *)b.evMask, (void *)b.evAux, (void *)b.dstCnt})
             if (p) FB_HIP(hipFree(p));
         const size_t n = 2 * (size_t)S * LEV;
@@ -763,14 +808,17 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
         FB_HIP(hipEventCreate(&b.ev0));
         FB_HIP(hipEventCreate(&b.ev1));
     }
-    if (b.E != d.g.E || b.drivenVersion != drivenVersion) {
-        if (b.eown) FB_HIP(hipFree(b.eown));
-        FB_HIP(hipMalloc(&b.eown, sizeof(uint32_t) * std::max<int64_t>(1, d.g.E)));
+    if (b.E != d.g.E || b.drivenVersion != drivenVersion || b.eownRnb != rnb) {
+        if (b.E != d.g.E || !b.eown) {
+            if (b.eown) FB_HIP(hipFree(b.eown));
+            FB_HIP(hipMalloc(&b.eown, sizeof(uint32_t) * std::max<int64_t>(1, d.g.E)));
+        }
         const int grid = (int)std::min<int64_t>(4096, (d.g.E + 255) / 256 + 1);
-        hipLaunchKernelGGL(k_batch_eown, dim3(grid), dim3(256), 0, st, d.g.edges, d.g.E, d.drvPos, b.eown);
+        hipLaunchKernelGGL(k_batch_eown, dim3(grid), dim3(256), 0, st, d.g.edges, d.g.E, d.drvPos, b.eown, rnb);
         FB_HIP(hipGetLastError());
         b.E = d.g.E;
         b.drivenVersion = drivenVersion;
+        b.eownRnb = rnb;
     }
     FB_HIP(hipMemsetAsync(b.dstCnt, 0, 4 * sizeof(int32_t) * S, st));
     FB_HIP(hipMemsetAsync(b.ctl, 0, 4 * sizeof(int32_t), st));
@@ -784,7 +832,7 @@ fb_status batch_la_launch(const BatchDev &d, int32_t t0, int32_t head0, int32_t 
     a.eown = b.eown; a.ctl = b.ctl; a.bar = b.bar; a.recM = recM; a.recF = recF;
     void *args[] = {&a};
     FB_HIP(hipEventRecord(b.ev0, st));
-    const hipError_t le = hipLaunchCooperativeKernel((const void *)k_batch_la, dim3(S), dim3(BT), args, 0, st);
+    const hipError_t le = hipLaunchCooperativeKernel(la_kernel(k), dim3(S), dim3(BT), args, 0, st);
     if (le != hipSuccess) {
         // refused (the resource behind grid-wide barriers is held elsewhere): nothing ran
         (void)hipGetLastError();

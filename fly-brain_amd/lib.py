@@ -80,6 +80,7 @@ _SIGS = {
     "fb_batch_counters": (I32, [P, PI64, PI64, PI64, PI64]),
     "fb_batch_set_stream": (I32, [P, P]),
     "fb_batch_set_engine": (I32, [P, I32]),
+    "fb_batch_set_shape": (I32, [P, I32]),
     "fb_batch_stat": (I32, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
     "fb_diff_create": (I32, [I32, I64, P, P, P, P, P, PP]),
     "fb_diff_destroy": (I32, [P]),

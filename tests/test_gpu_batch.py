@@ -187,3 +187,52 @@ def test_lane_drive_cannot_be_cleared_under_the_pool_loop():
     bt.set_lanes(True, np.ones((2, g.N), np.float32))     # the kept buffer is reused
     bt.destroy()
     hg.release()
+
+
+def test_every_lookahead_shape_and_the_autotuner_give_the_same_bits():
+    """The ensemble lookahead engine cut into regular blocks of 80, 96 and 128
+    neurons (fb_batch_set_shape), and the autotuner's own choice: every member
+    equals its single-brain oracle run for each, and the autotuner reports
+    the shape it chose and what it measured for every candidate."""
+    g = synth.tiny_graph(4096, 40, seed=47, p_count=0.3)
+    rng = np.random.default_rng(9)
+    B, T = 64, 40
+    type_of = rng.integers(0, T, size=g.N).astype(np.int32)
+    gains = np.exp(rng.normal(0, 0.2, size=(B, T))).astype(np.float32)
+    sign = (np.where(g.exc, 1.0, -1.0) * 0.5).astype(np.float32)
+    silenced = [np.sort(rng.choice(g.N, size=int(rng.integers(0, 150)), replace=False)) for _ in range(B)]
+    steps = rng.integers(400, 700, size=B)
+    p = P.config_params()
+    for shape in (80, 96, 128, 0):
+        print(f"ensemble lookahead shape {shape or 'autotuned'}", flush=True)
+        hg = lif.HipGraph(g.N, g.indptr, g.indices, g.weights, sign, g.counts)
+        eb = batch.LIFBatch(hg, p, [b + 1 for b in range(B)], type_of, gains)
+        eb.set_engine("persistent")
+        eb.set_shape(shape)
+        eb.setDrive(np.arange(0, g.N, 23), 150.0)
+        for b in range(B):
+            if len(silenced[b]):
+                eb.silence(b, silenced[b])
+        eb.set_steps(steps)
+        eb.run(int(steps.max()))
+        if shape:
+            assert eb.stat("lookahead_shape") == shape
+        else:
+            # a shape whose trial epochs overflowed its buckets has no measurement
+            chosen = int(eb.stat("lookahead_shape"))
+            assert chosen in (80, 96, 128)
+            assert eb.stat(f"autotune_us_per_epoch_{chosen}") > 0
+        for b in range(0, B, 9):
+            W = member_weights(g, type_of, gains[b])
+            o = OracleBrain(g, p, seed=b + 1, W=W, sign=sign)
+            o.setDrive(np.arange(0, g.N, 23).tolist(), 150.0)
+            for i in silenced[b]:
+                o.setThr(int(i), 1e6)
+            o.run(int(steps[b]), raster_cap=int(steps[b]) * g.N)
+            for k in ("v", "gE", "gI", "refr"):
+                assert np.array_equal(eb.array(k, b).view(np.uint32), o.array(k).view(np.uint32)), (shape, b, k)
+            assert np.array_equal(eb.array("spikeCount", b), o.spikeCount), (shape, b)
+        if shape == 80:
+            assert eb.stat("fallback_steps") == 0       # larger blocks may overflow their buckets: exact anyway
+        eb.destroy()
+        hg.release()
