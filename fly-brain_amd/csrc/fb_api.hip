@@ -431,6 +431,10 @@ static fb_status alloc_ring(fb_brain *b) {
 }
 
 static void drop_graphs(fb_brain *b) {
+    // an executable graph is destroyed only after its launches have finished:
+    // the brain's last call may have left replays queued (a call that changes
+    // the zero-weight skip flag drops the graphs before it queues anything)
+    if (!b->graphs.empty()) (void)hipStreamSynchronize(b->stream);
     for (auto &kv : b->graphs) {
         if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) hipGraphDestroy(kv.second.graph);

@@ -236,3 +236,43 @@ def test_every_lookahead_shape_and_the_autotuner_give_the_same_bits():
             assert eb.stat("fallback_steps") == 0       # larger blocks may overflow their buckets: exact anyway
         eb.destroy()
         hg.release()
+
+
+def test_batch64_config3_full_length_members():
+    """The ensemble workload at its own run lengths (2,000-10,000 steps,
+    U{2000..10000} per member) on the automatic engine (the lookahead kernel
+    with its autotuned shape): the members with the shortest and the longest
+    runs, and two more, equal their single-brain oracle runs -- state, spike
+    counts and stream -- after the whole run."""
+    g = synth.config1_graph()
+    B = 64
+    type_of = synth.type_ids(g.N, 500, 6)
+    gains = np.stack([synth.type_gains(b) for b in range(B)])
+    silenced = [synth.silenced(b, g.N) for b in range(B)]
+    steps = np.array([synth.run_length(b) for b in range(B)])
+    drv = synth.driven_set(g.N, 0.02, 3)
+    p = P.config_params()
+    print(f"C3 at full length: runs {steps.min()}..{steps.max()} steps", flush=True)
+    hg = lif.HipGraph.from_synth(g)
+    eb = batch.LIFBatch(hg, p, [b + 1 for b in range(B)], type_of, gains)
+    eb.setDrive(drv, 150.0)
+    for b in range(B):
+        if len(silenced[b]):
+            eb.silence(b, silenced[b])
+    eb.set_steps(steps)
+    eb.run(int(steps.max()))
+    assert eb.stat("lookahead_steps") > 0
+    rngs = eb.rng()
+    pick = sorted({int(np.argmin(steps)), int(np.argmax(steps)), 1, 33})
+    for b in pick:
+        o = OracleBrain(g, p, seed=b + 1, W=member_weights(g, type_of, gains[b]), sign=g.sign)
+        o.setDrive(drv.tolist(), 150.0)
+        for i in silenced[b]:
+            o.setThr(int(i), 1e6)
+        o.run(int(steps[b]), raster_cap=1)
+        for k in ("v", "gE", "gI", "refr", "trace"):
+            assert np.array_equal(eb.array(k, b).view(np.uint32), o.array(k).view(np.uint32)), (b, k)
+        assert np.array_equal(eb.array("spikeCount", b), o.spikeCount), b
+        assert int(rngs[b]) == o.rng, b
+    eb.destroy()
+    hg.release()
