@@ -26,6 +26,8 @@
 //                ballot; neurons with any spike are appended to the slot's
 //                active list.
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstdlib>
 #include <string>
@@ -856,6 +858,16 @@ extern "C" fb_status fb_batch_pool_counts(fb_batch *b, uint16_t *dst, int32_t ca
 // Runs nsteps; if ids/masks are given, records every step's active neurons with
 // their spike and forced masks (per step: count into nactive[step], entries
 // appended at most cap in total; the returned total may exceed cap).
+// the autotuner's choices, by (graph, neurons, members, driven neurons)
+struct TuneKey {
+    const void *graph;
+    int32_t N, B, nDriven;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(graph, N, B, nDriven) < std::tie(o.graph, o.N, o.B, o.nDriven);
+    }
+};
+static std::map<TuneKey, int32_t> g_tuned;
+
 extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive, int32_t *ids, uint64_t *masks,
                                   uint64_t *fmasks, int64_t cap, int64_t *total) {
     FB_REQUIRE_B(b && nsteps >= 0, FB_ERR_ARG, "fb_batch_run: bad arguments");
@@ -878,14 +890,19 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
     std::vector<uint64_t> hM, hF;
     // The autotuner: the first run long enough runs its first epochs on every
     // shape that fits, twice round, keeps the fastest for the rest of the run
-    // and every later one.  All shapes give the same bits, so the trials are
-    // the run's own epochs.
+    // and every later one, and leaves the choice for later ensembles of the
+    // same graph, size and driven set.  All shapes give the same bits, so the
+    // trials are the run's own epochs.
     constexpr int32_t TRIAL_EPOCHS = 2;
     std::vector<int32_t> fits;
     for (int k = 0; k < LA_NSHAPES; k++)
         if (la && batch_la_shape_fits(d, k)) fits.push_back(k);
+    const TuneKey tkey{(const void *)d.g.edges, d.N, d.B, d.nDriven};
     if (la && b->laForced < 0 && b->laShape < 0 && b->tunePlan.empty() && !fits.empty()) {
-        if (fits.size() == 1) {
+        auto hit = g_tuned.find(tkey);
+        if (hit != g_tuned.end()) {
+            b->laShape = hit->second;
+        } else if (fits.size() == 1) {
             b->laShape = fits[0];
         } else if ((nsteps - s0) / D >= 2 + 2 * (int32_t)fits.size() * TRIAL_EPOCHS) {
             b->tunePlan.push_back({fits[0], 1, false});        // out of the transient of a fresh start
@@ -938,6 +955,7 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
                     (best < 0 || b->tuneMs[k] / b->tuneEpochs[k] < b->tuneMs[best] / b->tuneEpochs[best]))
                     best = k;
             b->laShape = best >= 0 ? best : fits[0];
+            g_tuned[tkey] = b->laShape;
         }
         const int32_t m = done * D;
         if (rec && m > 0) {
@@ -997,6 +1015,7 @@ extern "C" fb_status fb_batch_set_shape(fb_batch *b, int32_t neuronsPerBlock) {
     if (neuronsPerBlock == 0) {
         b->laForced = -1;
         b->laShape = -1;
+        g_tuned.erase(TuneKey{(const void *)b->d.g.edges, b->d.N, b->d.B, b->d.nDriven});
         return FB_OK;
     }
     for (int k = 0; k < LA_NSHAPES; k++)

@@ -355,6 +355,7 @@ __global__ void __launch_bounds__(256) k_membrane(DevBrain d) {
         NeuronState s;
         s.v = d.v[i]; s.gE = d.gE[i]; s.gI = d.gI[i]; s.refr = d.refr[i];
         s.trace = d.trace[i]; s.adapt = d.adapt[i]; s.res = d.res[i];
+        const float refr0 = s.refr, trace0 = s.trace, adapt0 = s.adapt, res0 = s.res;
         // this step's arrivals, excitatory then inhibitory accumulator
 #pragma unroll
         for (int c = 0; c < 2; c++) {
@@ -384,8 +385,14 @@ __global__ void __launch_bounds__(256) k_membrane(DevBrain d) {
             d.spikeCount[i] += 1u;
             spk = true;
         }
-        d.v[i] = s.v; d.gE[i] = s.gE; d.gI[i] = s.gI; d.refr[i] = s.refr;
-        d.trace[i] = s.trace; d.adapt[i] = s.adapt; d.res[i] = s.res;
+        d.v[i] = s.v; d.gE[i] = s.gE; d.gI[i] = s.gI;
+        // arrays that often keep their bits (a neuron that is not refractory,
+        // never spiked, or without adaptation or depression) are written only
+        // when they change
+        if (__float_as_uint(s.refr) != __float_as_uint(refr0)) d.refr[i] = s.refr;
+        if (__float_as_uint(s.trace) != __float_as_uint(trace0)) d.trace[i] = s.trace;
+        if (__float_as_uint(s.adapt) != __float_as_uint(adapt0)) d.adapt[i] = s.adapt;
+        if (__float_as_uint(s.res) != __float_as_uint(res0)) d.res[i] = s.res;
     }
     // ascending compaction inside the block: wave ballot, then wave prefix
     const uint64_t mask = __ballot(spk);
