@@ -211,10 +211,19 @@ __device__ __forceinline__ uint32_t matvec(const uint32_t *tab, uint32_t x) {
     return r;
 }
 
-// global layout (stride 128) -> LDS layout (stride 129); every thread of the block calls it
-template <int T>
+// global layout (stride 128) -> LDS layout (stride 129), the first MATS
+// matrices; every thread of the block calls it
+template <int T, int MATS = LEAP_MATS>
 __device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap) {
-    for (int q = threadIdx.x; q < LEAP_MATS * LEAP_WORDS; q += T) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = leap[q];
+    for (int q = threadIdx.x; q < MATS * LEAP_WORDS; q += T) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = leap[q];
+}
+
+// M^J x for 1 <= J from LDS tables holding all LEAP_TABLE_MATS matrices:
+// with J - 1 = 2048 a + 32 h + f, M^J = M^(f+1) M^(32 h) (M^2048)^a.
+__device__ __forceinline__ uint32_t leap_jump(const uint32_t *leapS, uint32_t J, uint32_t x) {
+    const uint32_t o = J - 1u;
+    for (uint32_t a = o >> 11; a; a--) x = matvec(leapS + LEAP_FAR * LEAP_STRIDE, x);
+    return matvec(leapS + (o & 31u) * LEAP_STRIDE, matvec(leapS + (32u + ((o >> 5) & 63u)) * LEAP_STRIDE, x));
 }
 
 // Exclusive scan over a T-thread workgroup of per-thread counts given as K

@@ -7,14 +7,21 @@
 // neuron needs anything from another workgroup -- except the forced spikes,
 // which draw from one serial xorshift32 stream in driven-list order.  Roles:
 //
-//   block 0         the driven block: owns every driven neuron (state in
-//                   registers, list order across threads).  Per step: a ballot
-//                   scan gives each thread its position in the stream; 64
-//                   threads compute M^(32h) x for the step's coarse offsets
-//                   and every thread finishes its own draws with one
-//                   nibble-table product and xorshift steps (M = one
-//                   xorshift32 step as a matrix over GF(2)).
-//   blocks 1..R     regular blocks: block b owns a contiguous, 64-aligned
+//   blocks 0..G-1   the driven blocks: block k owns the driven-list positions
+//                   [kQ, (k+1)Q) (Q <= 2048; state in registers, list order
+//                   across threads).  Per step: a ballot scan gives each
+//                   thread its position in the stream, and every thread jumps
+//                   to its first draw with nibble-table products and takes
+//                   the later ones by xorshift steps (M = one xorshift32 step
+//                   as a matrix over GF(2)).  With G > 1 the blocks swap their
+//                   non-refractory counts through tagged words in global
+//                   memory (xc): block k needs the counts of blocks < k for
+//                   its stream offset and every count of the previous step
+//                   for the stream state.  A neuron's refractoriness at step
+//                   t is fixed by the end of step t-1 (arrivals do not touch
+//                   it), so each count is published one step ahead, and the
+//                   swap stays off the serial chain.
+//   blocks G..R     regular blocks: block b owns a contiguous, 64-aligned
 //                   range of up to 1024 neurons (2 per thread, in registers).
 //   blocks R+1..    spare blocks (when the grid has them): they turn the
 //                   previous epoch's spikes into ring lists in lif.c order.
@@ -35,13 +42,13 @@
 //           blocks push the driven block's rows the same way, then a second
 //           grid barrier ends the epoch.
 // The arrival key of a spike is its rank in lif.c's list order: a forced
-// spike's driven-list position, or 2048 + index for a threshold spike.
+// spike's driven-list position, or 8192 + index for a threshold spike.
 //
 // A launch starts with a prologue that scatters the ring lists already in
 // the delay line (from the per-step engines or the previous launch) into the
 // buckets, keyed by list position.  Preconditions (checked on the host;
 // otherwise the per-step engines run): no background events, no presynaptic
-// inhibition, D <= 32, at most 2048 driven neurons and 1024 neurons per
+// inhibition, D <= 32, at most 8192 driven neurons and 1024 neurons per
 // block, and with synaptic depression live a refractory period of at least D
 // steps (a neuron then arrives at most once per epoch).  A bucket or LDS
 // overflow stops the launch at an epoch boundary; the host runs one epoch on
@@ -66,12 +73,16 @@ constexpr int EV_CAP = 6144;        // events per destination block per epoch (L
 constexpr int BK_STRIDE = 32;       // words between two destinations' bucket counts (their own 128-byte lines)
 constexpr int SPK_CAP = 2048;       // spikes per block per epoch (LDS push list)
 constexpr int RPT = SPK_CAP / PT;   // push rows per thread in the row scan
-constexpr int DPT = 4;              // driven neurons per thread in block 0
-constexpr int DRV_CAP = PT * DPT;
+constexpr int DPT = 4;              // driven neurons per thread in a driven block
+constexpr int DRV_BLK = PT * DPT;   // driven neurons per driven block
+constexpr int DRV_BLOCKS = 4;       // driven blocks at most
+constexpr int DRV_CAP = DRV_BLK * DRV_BLOCKS;
+constexpr int XC_RING = 8;          // exchange words per driven block (a ring over steps)
 constexpr int CELLS = DPT > NPT ? DPT : NPT;
 constexpr int S_MAX = 512;          // owning blocks (driven + regular)
-constexpr int WPS = (NB_MAX > DRV_CAP ? NB_MAX : DRV_CAP) / 32;   // words of one step in the hasEv bitmap
-constexpr uint32_t KEY_THRESHOLD = 2048;   // arrival key offset of threshold spikes
+constexpr int WPS = (NB_MAX > DRV_BLK ? NB_MAX : DRV_BLK) / 32;   // words of one step in the hasEv bitmap
+constexpr uint32_t KEY_THRESHOLD = DRV_CAP;   // arrival key offset of threshold spikes
+constexpr int LEAP_PERSIST_MATS = LEAP_TABLE_MATS;   // M^1..M^32, M^(32h), and M^2048, all in LDS
 #ifndef FB_SHARE_BLOCKS
 #define FB_SHARE_BLOCKS 64
 #endif
@@ -93,7 +104,9 @@ struct PersistArgs {
     int32_t nsteps;
     int32_t D;           // epoch length (the delay in steps)
     int32_t nb;          // neurons per regular block (multiple of 64)
-    int32_t S;           // owning blocks: the driven block + regular blocks
+    int32_t G;           // driven blocks (blocks 0 .. G-1)
+    int32_t Q;           // driven-list positions per driven block
+    int32_t S;           // owning blocks: the driven blocks + regular blocks
     int32_t nSpare;      // spare blocks after them
     int32_t nwords;      // 64-bit words of one step's spike bitmask
     uint64_t *spk;       // [2][D_MAX][nwords] threshold spikes of non-driven neurons
@@ -109,6 +122,7 @@ struct PersistArgs {
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
     float *dres;         // [2][DRV_CAP] res of each driven neuron at the epoch's end
     int32_t *bar;        // grid barrier counters (GBAR_WORDS, zero at launch)
+    uint64_t *xc;        // [DRV_BLOCKS][XC_RING] (step + 1) << 32 | non-refractory count, zero at launch
     int32_t flags;       // bit 0: time the driven block's step parts
 };
 
@@ -131,7 +145,7 @@ struct Shared {
     };
     uint32_t hasEv[D_MAX * WPS];     // bit (step, target): the target has arrivals at that step
     int32_t evRank[D_MAX * WPS];     // set bits of hasEv before each word
-    uint32_t leap[LEAP_LDS_WORDS];
+    uint32_t leap[LEAP_PERSIST_MATS * LEAP_STRIDE];
     int32_t pref[D_MAX + 1];     // prologue: first ring entry of each arrival step
     int32_t scan[64 + PT / 64];
     uint32_t rngNext[2];
@@ -609,6 +623,158 @@ __device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]
 #undef FB_SUB
 }
 
+// ------------------------------------------------------------------ several driven blocks
+
+// Driven block `me` of G: its count of non-refractory neurons for launch step
+// t goes to xc[me][t % XC_RING] as (t + 1) << 32 | count.
+__device__ __forceinline__ void xc_publish(const PersistArgs &a, int32_t me, int32_t t, int32_t n) {
+    uint64_t *w = a.xc + (int64_t)me * XC_RING + (t % XC_RING);
+    (void)__hip_atomic_exchange(w, ((uint64_t)(uint32_t)(t + 1) << 32) | (uint32_t)n, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits for driven block k's count of step t (every lane of the wave loads
+// the same word).  A block publishes step t's count before it can need any
+// count of step t, and no block runs more than two steps ahead of another,
+// so the ring slot holds step t's word or an older one.  The wait is bounded
+// like the grid barrier's: on a timeout it sets the flag, gives 0 and stops
+// waiting for the rest of the launch (`dead`); the next epoch boundary stops
+// every block.
+__device__ __forceinline__ uint32_t xc_wait(const PersistArgs &a, int32_t k, int32_t t, bool &dead) {
+    const uint64_t *w = a.xc + (int64_t)k * XC_RING + (t % XC_RING);
+    const uint32_t tag = (uint32_t)(t + 1);
+    uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(v >> 32) == tag || dead) return dead ? 0u : (uint32_t)v;
+    const uint64_t t0 = wall_clock64();
+    while ((uint32_t)(v >> 32) != tag) {
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > GBAR_TIMEOUT) {
+            atomicOr(&a.ctl[1], GBAR_TIMED_OUT);
+            dead = true;
+            return 0u;
+        }
+        v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return (uint32_t)v;
+}
+
+// This block's non-refractory flags (bit u: cell u draws) and its ballot
+// scan; every thread of the block calls it.
+__device__ __forceinline__ uint32_t driven_flags(const Cell (&cells)[CELLS]) {
+    uint32_t nr = 0;
+#pragma unroll
+    for (int u = 0; u < DPT; u++)
+        if (cells[u].i >= 0 && !(cells[u].st.refr > 0)) nr |= 1u << u;
+    return nr;
+}
+
+// The driven block of several: as sim_driven, with the stream position of a
+// draw = (counts of the blocks before this one) + (position in this block),
+// and the stream state of step t = M^(total draws of step t-1) applied to
+// that of step t-1.  `t0`: the launch step of the epoch's first step; `nr`,
+// `off`: this block's flags and offsets for step t0, computed and published
+// at the end of the previous step (or before the first epoch), and left for
+// the step after the epoch.
+__device__ void sim_driven_x(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
+                             int32_t nev, int32_t par, int32_t L, int32_t t0, uint32_t &rng, uint32_t &nr,
+                             int32_t &off, bool &dead) {
+    const DevBrain &d = a.d;
+    const int tid = threadIdx.x;
+    const int32_t me = blockIdx.x;
+    const int32_t qb = me * a.Q;
+    const int32_t nDk = min(a.Q, d.nDriven - qb);
+    const int32_t P = (nDk + PT - 1) / PT;
+    const float forcedRefr = d.k.tRef + d.k.dt;
+    const bool timed = tid == 0 && (a.flags & 1);
+#define FB_SUB(k)                                      \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.sub[k] += now - s.sub[6];                    \
+        s.sub[6] = now;                                \
+    }
+    if (timed) s.sub[6] = wall_clock64();
+    for (int32_t m = 0; m < L; m++) {
+        const int32_t t = t0 + m;
+        // lif.c phase 1: arrivals for the driven neurons (local targets
+        // q0 .. q0+P-1 of this block)
+        const uint32_t q0 = (uint32_t)(tid * P);
+        const int32_t w0 = m * WPS + (int32_t)(q0 >> 5);
+        uint64_t pair = s.hasEv[w0];
+        if ((q0 & 31u) + (uint32_t)P > 32u) pair |= (uint64_t)s.hasEv[w0 + 1] << 32;
+        const uint32_t evBits = (uint32_t)(pair >> (q0 & 31u));
+#pragma unroll
+        for (int u = 0; u < DPT; u++) {
+            Cell &c = cells[u];
+            if (c.i >= 0) cell_inputs_has(d.k, c, s, nev, m, q0 + (uint32_t)u, (evBits >> u) & 1u);
+        }
+        FB_SUB(0)
+        // the other blocks' counts: this step's before this block, and the
+        // previous step's total for the stream state
+        uint32_t before = 0, prevTotal = 0;
+        for (int32_t k = 0; k < a.G; k++) {
+            if (k < me) before += xc_wait(a, k, t, dead);
+            if (t > 0) prevTotal += xc_wait(a, k, t - 1, dead);
+        }
+        if (prevTotal) rng = leap_jump(s.leap, prevTotal, rng);
+        FB_SUB(1)
+        FB_SUB(2)
+        // lif.c phase 2: one draw per non-refractory driven neuron, in list order
+        uint32_t x = nr ? leap_jump(s.leap, before + (uint32_t)off + 1u, rng) : 0u;
+        bool first = true;
+#pragma unroll
+        for (int u = 0; u < DPT; u++) {
+            Cell &c = cells[u];
+            if ((nr >> u) & 1u) {
+                if (!first) x = xs32(x);
+                first = false;
+                const float uu = (float)(x >> 8) * (1.0f / 16777216.0f);
+                if (uu < pFire[u]) {
+                    c.st.v = d.k.vReset;
+                    c.st.refr = forcedRefr;
+                    c.st.trace = 1.f;
+                    c.st.adapt = c.st.adapt + d.k.adaptInc;
+                    c.count++;
+                    c.mask |= 1u << m;
+                    c.forced |= 1u << m;
+                }
+            }
+        }
+        FB_SUB(3)
+        // lif.c phases 3-4 for the driven neurons
+#pragma unroll
+        for (int u = 0; u < DPT; u++) {
+            Cell &c = cells[u];
+            if (c.i >= 0) {
+                membrane(d.k, c.st, c.bias);
+                if (crosses(d.k, c.st, c.thr)) {
+                    spike_effects(d.k, c.st, d.k.tRef);
+                    c.count++;
+                    c.mask |= 1u << m;
+                }
+            }
+        }
+        // the next step's flags, fixed now: publish this block's count
+        nr = driven_flags(cells);
+        int32_t n;
+        off = wg_scan_flags<PT, DPT>(nr, s.scan + (m & 1) * 16, &n);
+        if (tid == 0) xc_publish(a, me, t + 1, n);
+        FB_SUB(4)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < DPT; u++) {
+        const Cell &c = cells[u];
+        if (c.i >= 0) {
+            const int32_t q = qb + tid * P + u;
+            a.dmask[par * DRV_CAP + q] = c.forced;
+            a.tmask[par * DRV_CAP + q] = c.mask & ~c.forced;
+            a.dres[par * DRV_CAP + q] = c.st.res;
+        }
+    }
+    FB_SUB(5)
+#undef FB_SUB
+}
+
 // ------------------------------------------------------------------ push: own spikes -> next epoch's buckets
 
 // A regular block's spikes of the epoch -> the LDS push list (s.spkI /
@@ -801,8 +967,8 @@ __device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t
 // lists of epoch steps [t0, t0+L), spread over the spare blocks (or the
 // regular blocks when the grid has no spare ones)
 __device__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_t t0, int32_t L) {
-    const int32_t first = a.nSpare > 0 ? a.S : 1;
-    const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - 1);
+    const int32_t first = a.nSpare > 0 ? a.S : a.G;
+    const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - a.G);
     const int32_t b = (int32_t)blockIdx.x - first;
     if (b < 0 || b >= count) return;
     for (int32_t m = b; m < L; m += count) build_list(a, s, par, t0 + m, m);
@@ -819,13 +985,13 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t me = blockIdx.x;
-    const bool drivenBlock = me == 0;
+    const bool drivenBlock = me < a.G;
     const bool owning = me < a.S;
-    const int32_t lo = drivenBlock || !owning ? 0 : (me - 1) * a.nb;
+    const int32_t lo = drivenBlock || !owning ? 0 : (me - a.G) * a.nb;
     const int32_t hi = drivenBlock || !owning ? 0 : min(d.N, lo + a.nb);
 
-    // clocks of the driven block, the first regular block and the first spare block
-    const int32_t timedSlot = me == 0 ? 0 : me == 1 ? 1 : (a.nSpare > 0 && me == a.S) ? 2 : -1;
+    // clocks of the first driven block, the first regular block and the first spare block
+    const int32_t timedSlot = me == 0 ? 0 : me == a.G ? 1 : (a.nSpare > 0 && me == a.S) ? 2 : -1;
     const bool timed = tid == 0 && timedSlot >= 0;
     if (timed) {
         for (int k = 0; k < NPH; k++) s.ph[k] = 0;
@@ -844,17 +1010,29 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     Cell cells[CELLS];
     float pFire[DPT];
     uint32_t rng = 0;
+    uint32_t xNr = 0;      // several driven blocks: the next step's flags and offset (sim_driven_x)
+    int32_t xOff = 0;
+    bool xDead = false;
     if (drivenBlock) {
-        const int32_t P = (d.nDriven + PT - 1) / PT;
+        const int32_t qb = me * a.Q;
+        const int32_t nDk = min(a.Q, d.nDriven - qb);
+        const int32_t P = (nDk + PT - 1) / PT;
 #pragma unroll
         for (int u = 0; u < DPT; u++) {
             const int32_t q = tid * P + u;
-            const bool has = u < P && q < d.nDriven;
-            load_cell(d, cells[u], has ? d.driven[q] : -1);
+            const bool has = u < P && q < nDk;
+            load_cell(d, cells[u], has ? d.driven[qb + q] : -1);
             pFire[u] = has ? d.drive[cells[u].i] * d.k.dtS : 0.f;
         }
-        leap_load<PT>(s.leap, d.leap);
+        leap_load<PT, LEAP_PERSIST_MATS>(s.leap, d.leap);
         rng = d.sc->rng;
+        if (a.G > 1) {
+            // the first step's count, published before anything can wait for it
+            xNr = driven_flags(cells);
+            int32_t n;
+            xOff = wg_scan_flags<PT, DPT>(xNr, s.scan + 32, &n);
+            if (tid == 0) xc_publish(a, me, 0, n);
+        }
     } else {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) {
@@ -895,7 +1073,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 break;
             }
             if (owning) {
-                if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
+                if (drivenBlock && a.G > 1) sim_driven_x(a, s, cells, pFire, nev, par, L, done, rng, xNr, xOff, xDead);
+                else if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
                 if (done + L < a.nsteps && !drivenBlock) push_spikes(a, s, cells, par ^ 1, me);
@@ -926,7 +1105,15 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     if (owning) {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) store_cell(d, cells[u]);
-        if (drivenBlock && tid == 0) d.sc->rng = rng;
+        if (me == 0) {
+            if (a.G > 1 && done > 0) {
+                // the stream after the last step: its draws over every driven block
+                uint32_t total = 0;
+                for (int32_t k = 0; k < a.G; k++) total += xc_wait(a, k, done - 1, xDead);
+                if (total) rng = leap_jump(s.leap, total, rng);
+            }
+            if (tid == 0) d.sc->rng = rng;
+        }
     }
 #undef FB_PHASE
     if (timed) {
@@ -982,23 +1169,26 @@ static int32_t refractory_steps(const Consts &k) {
 }
 
 struct Layout {
-    int32_t nb, S, nSpare, grid;
+    int32_t nb, G, Q, S, nSpare, grid;
 };
 
+// The driven list splits evenly over the fewest driven blocks that hold it.
 // Regular blocks own 64-aligned ranges; a few blocks of the grid are left
 // spare for the list building when the network leaves room.
 static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
     const int32_t maxB = g_ps.maxBlocks;
-    if (maxB < 2) return false;
-    const int32_t want = std::max(1, maxB - 1 - D);   // regular blocks if D spare blocks fit
+    l.G = std::max(1, (d.nDriven + DRV_BLK - 1) / DRV_BLK);
+    l.Q = (d.nDriven + l.G - 1) / l.G;
+    if (maxB < l.G + 1) return false;
+    const int32_t want = std::max(1, maxB - l.G - D);   // regular blocks if D spare blocks fit
     int64_t nb = ((int64_t)(d.N + want - 1) / want + 63) / 64 * 64;
     if (nb > NB_MAX) {
-        nb = ((int64_t)(d.N + (maxB - 1) - 1) / (maxB - 1) + 63) / 64 * 64;   // no spares
+        nb = ((int64_t)(d.N + (maxB - l.G) - 1) / (maxB - l.G) + 63) / 64 * 64;   // no spares
         if (nb > NB_MAX) return false;
     }
     l.nb = (int32_t)nb;
     const int32_t need = (int32_t)((d.N + nb - 1) / nb);
-    l.S = need + 1;
+    l.S = need + l.G;
     if (l.S > S_MAX) return false;
     l.nSpare = std::min(D, maxB - l.S);
     l.grid = l.S + l.nSpare;
@@ -1009,14 +1199,14 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     if (serialDeliver) { *why = "presynaptic inhibition or tRef <= 0 needs the serial deliver path"; return FB_ERR_UNSUPPORTED; }
     if (d.k.bgEvents != 0.f) { *why = "background events draw from the shared stream every step"; return FB_ERR_UNSUPPORTED; }
     if (d.poolOf) { *why = "the pool loop sets the drive after every step"; return FB_ERR_UNSUPPORTED; }
-    if (d.nDriven > DRV_CAP) { *why = "more than 2048 driven neurons"; return FB_ERR_UNSUPPORTED; }
+    if (d.nDriven > DRV_CAP) { *why = "more than 8192 driven neurons"; return FB_ERR_UNSUPPORTED; }
     const int32_t D = d.nslots - 1;
     if (D < 1 || D > D_MAX) { *why = "delay outside 1..32 steps"; return FB_ERR_UNSUPPORTED; }
     if (d.k.depU != 0.f && refractory_steps(d.k) < D) {
         *why = "synaptic depression with a refractory period shorter than the delay";
         return FB_ERR_UNSUPPORTED;
     }
-    if (KEY_THRESHOLD + (int64_t)d.N >= (1LL << 20)) { *why = "more than 1,046,527 neurons (arrival key field)"; return FB_ERR_UNSUPPORTED; }
+    if (KEY_THRESHOLD + (int64_t)d.N >= (1LL << 20)) { *why = "more than 1,040,383 neurons (arrival key field)"; return FB_ERR_UNSUPPORTED; }
     if (d.g.maxRow >= (1LL << 26)) { *why = "a row longer than 2^26 edges (row position field)"; return FB_ERR_UNSUPPORTED; }
     fb_status st = persist_init();
     if (st) { *why = "cooperative launch unavailable"; return st; }
@@ -1026,6 +1216,13 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     if ((d.nDriven + shareBlocks - 1) / shareBlocks > SHQ) { *why = "too many driven neurons for this grid"; return FB_ERR_UNSUPPORTED; }
     return FB_OK;
 }
+
+// Whether the automatic engine should take the lookahead engine for a run of
+// at least one epoch (it is supported): with one driven block it is ahead of
+// the per-step engines; with several, the per-step exchange of draw counts
+// and the skew of a connectome's hub blocks make the per-step engines the
+// faster choice today.
+bool persistent_preferred(const DevBrain &d) { return d.nDriven <= DRV_BLK; }
 
 struct PersistBuffers {
     uint64_t *spk = nullptr;
@@ -1040,10 +1237,11 @@ struct PersistBuffers {
     uint32_t *eown = nullptr;
     float *dres = nullptr;
     int32_t *bar = nullptr;
+    uint64_t *xc = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t E = 0;
-    int32_t eownNb = -1;
+    int32_t eownNb = -1, eownG = -1;
     uint64_t eownDriven = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1052,7 +1250,7 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.bkKey, pb.bkAdd, pb.bkCnt,
-                  pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar};
+                  pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar, pb.xc};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
@@ -1060,7 +1258,9 @@ static void free_buffers(PersistBuffers &pb) {
     pb.eown = nullptr;
     pb.dres = nullptr;
     pb.bar = nullptr;
+    pb.xc = nullptr;
     pb.eownNb = -1;
+    pb.eownG = -1;
     pb.eownDriven = ~0ull;
 }
 
@@ -1084,25 +1284,31 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
     if (q < nD) drvPos[driven[q]] = q;
 }
 
-// the launch's control words, barrier counters, clocks and bucket counts
-// cleared in one kernel (four fills would be four queued operations)
-__global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, int32_t *bkCnt, int32_t nCnt) {
+// the launch's control words, barrier counters, clocks, exchange words and
+// bucket counts cleared in one kernel (five fills would be five queued
+// operations)
+__global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, uint64_t *xc, int32_t *bkCnt, int32_t nCnt) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = gridDim.x * blockDim.x;
     if (i < 4) ctl[i] = 0;
     if (i < PERSIST_PROF) prof[i] = 0;
+    if (i < DRV_BLOCKS * XC_RING) xc[i] = 0;
     for (int32_t k = i; k < GBAR_WORDS; k += n) bar[k] = 0;
     for (int32_t k = i; k < nCnt; k += n) bkCnt[k] = 0;
 }
 
-// destination block << 12 | local index, per edge: driven targets belong to
-// block 0 (index = driven-list position), others to block 1 + q / nb
-__global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos, int32_t nb) {
+// destination block << 12 | local index, per edge: a driven target at list
+// position p belongs to driven block p / Q (index p mod Q), others to block
+// G + q / nb
+__global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos, int32_t nb,
+                             int32_t G, int32_t Q) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < E; j += (int64_t)gridDim.x * blockDim.x) {
         const int32_t q = (int32_t)edges[j].x;
         const int32_t dp = drvPos[q];
         const int32_t blk = q / nb;
-        eown[j] = dp >= 0 ? (uint32_t)dp : ((uint32_t)(1 + blk) << 12) | (uint32_t)(q - blk * nb);
+        const int32_t db = dp >= 0 ? dp / Q : 0;
+        eown[j] = dp >= 0 ? ((uint32_t)db << 12) | (uint32_t)(dp - db * Q)
+                          : ((uint32_t)(G + blk) << 12) | (uint32_t)(q - blk * nb);
     }
 }
 
@@ -1135,29 +1341,33 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
         FB_HIP(hipMalloc(&pb.dres, sizeof(float) * 2 * DRV_CAP));
         FB_HIP(hipMalloc(&pb.bar, sizeof(int32_t) * GBAR_WORDS));
+        FB_HIP(hipMalloc(&pb.xc, sizeof(uint64_t) * DRV_BLOCKS * XC_RING));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
         }
     }
     // driven positions and edge owners (again when the driven set or layout changed)
-    if (pb.eownDriven != drivenVersion || pb.eownNb != lay.nb) {
+    if (pb.eownDriven != drivenVersion || pb.eownNb != lay.nb || pb.eownG != lay.G) {
         hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
         if (d.nDriven)
             hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
                                d.nDriven);
         if (d.g.E)
             hipLaunchKernelGGL(k_edge_owner, dim3(1024), dim3(256), 0, st, pb.eown, d.g.edges, d.g.E, pb.drvPos,
-                               lay.nb);
+                               lay.nb, lay.G, std::max(lay.Q, 1));
         FB_HIP(hipGetLastError());
         pb.eownDriven = drivenVersion;
         pb.eownNb = lay.nb;
+        pb.eownG = lay.G;
     }
 
     PersistArgs a{};
     a.d = d;
     a.D = D;
     a.nb = lay.nb;
+    a.G = lay.G;
+    a.Q = lay.Q;
     a.S = lay.S;
     a.nSpare = lay.nSpare;
     a.nwords = nwords;
@@ -1174,6 +1384,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.eown = pb.eown;
     a.dres = pb.dres;
     a.bar = pb.bar;
+    a.xc = pb.xc;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
@@ -1181,7 +1392,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     while (done < nsteps) {
         a.head0 = cur;
         a.nsteps = nsteps - done;
-        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.bkCnt,
+        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.bkCnt,
                            2 * lay.S * BK_STRIDE);
         FB_HIP(hipGetLastError());
         void *args[] = {&a};

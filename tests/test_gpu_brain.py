@@ -55,7 +55,7 @@ def run_hip(g, b, seed, engine):
     return net
 
 
-@pytest.mark.parametrize("engine", ["graph", "launch", "auto"])
+@pytest.mark.parametrize("engine", ["graph", "launch", "persistent", "auto"])
 def test_engine_reproduces_the_sugar_trace(brain, engine):
     print(f"calibrated brain: sugar trace, engine={engine}", flush=True)
     g = lif.HipGraph(brain["N"], brain["indptr"], brain["indices"], brain["W"], brain["sign"])
@@ -71,10 +71,11 @@ def test_engine_reproduces_the_sugar_trace(brain, engine):
         assert round(sum(int(c[ix].sum()) for c in counts) / 4 / ix.size, 3) == want
 
 
-def test_seed_one_matches_the_oracle(brain):
-    print("calibrated brain: seed 1 against the oracle", flush=True)
+@pytest.mark.parametrize("engine", ["launch", "persistent"])
+def test_seed_one_matches_the_oracle(brain, engine):
+    print(f"calibrated brain: seed 1 against the oracle, engine={engine}", flush=True)
     g = lif.HipGraph(brain["N"], brain["indptr"], brain["indices"], brain["W"], brain["sign"])
-    net = run_hip(g, brain, 1, "auto")
+    net = run_hip(g, brain, 1, engine)
     gg = _G()
     gg.N, gg.indptr, gg.indices = brain["N"], brain["indptr"], brain["indices"]
     o = OracleBrain(gg, brain["params"], seed=1, W=brain["W"], sign=brain["sign"])

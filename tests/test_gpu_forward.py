@@ -138,6 +138,41 @@ def test_persistent_multi_epoch_launches_match_oracle(name):
     assert gpu.stat("fallback_steps") == 0
 
 
+@pytest.mark.parametrize("ndrv,dt05", [(2049, False), (3500, False), (6000, True), (8192, False)])
+def test_persistent_several_driven_blocks_match_oracle(ndrv, dt05):
+    """More than 2,048 driven neurons: the lookahead engine splits the driven
+    list over up to four driven workgroups, which swap their per-step draw
+    counts.  Whole and ragged launches of many epochs on a 65,536-neuron
+    graph (2,049 driven: the list split 1,025 + 1,024; 8,192: four full
+    blocks; at most ~1,100 events per driven block and epoch, well inside
+    its bucket), and 4-step epochs with depression and adaptation on; the
+    state, counts, RNG and last fired list equal the oracle's after every
+    launch."""
+    g = synth.tiny_graph(65536, 16, seed=43, p_count=0.4)
+    rng = np.random.default_rng(43)
+    s = (np.where(g.exc, 1.0, -1.0) * (0.9 if not dt05 else 2.5)).astype(np.float32)
+    p = P.merged({}) if dt05 else P.config_params()
+    drv = rng.choice(g.N, size=ndrv, replace=False)
+    rates = rng.uniform(30.0, 300.0, size=ndrv)
+    gpu = _hip(g, g.weights, s, p, 11, "persistent")
+    cpu = OracleBrain(g, p, seed=11, W=g.weights, sign=s)
+    for b in (gpu, cpu):
+        for i, r in zip(drv, rates):
+            b.setDriveOne(int(i), float(r))
+    D = gpu.nslots - 1
+    for launch in (2 * D, 5 * D + 3, 30 * D, 7):
+        print(f"several driven blocks: {ndrv} driven, dt05={dt05}, {launch} steps", flush=True)
+        gpu.advance(launch)
+        nf, ras, _ = cpu.run(launch, raster_cap=launch * g.N)
+        for k in lifcases.STATE:
+            assert np.array_equal(getattr(gpu, k).view(np.uint32), cpu.array(k).view(np.uint32)), (launch, k)
+        assert np.array_equal(gpu.spikeCount, cpu.spikeCount), launch
+        assert gpu.rng == cpu.rng, launch
+        assert np.array_equal(gpu.fired(), ras[ras.size - nf[-1]:]), launch
+    assert gpu.stat("persistent_launches") >= 4
+    assert gpu.stat("fallback_steps") == 0
+
+
 @pytest.fixture(scope="module")
 def c2():
     g = synth.config2_graph()
