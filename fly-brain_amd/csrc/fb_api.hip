@@ -167,6 +167,7 @@ struct fb_graph {
     bool wNonneg = true;            // no weight carries a sign bit (negative or -0)
     std::vector<void *> allocs;
     std::vector<uint32_t> indptr;   // host copy for validation
+    std::vector<uint64_t> chunkCost;   // DevGraph::chunkCost
 };
 
 struct GraphCache {
@@ -330,6 +331,16 @@ extern "C" fb_status fb_graph_create(int32_t N, int64_t E, const uint32_t *indpt
         if (E) FB_HIP(hipMemcpy(dCounts, cnts.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice));
     }
     for (int32_t i = 0; i < N; i++) g->d.maxRow = std::max<int64_t>(g->d.maxRow, (int64_t)indptr[i + 1] - indptr[i]);
+    g->chunkCost.assign(((size_t)N + 63) / 64, 0);
+    for (int32_t i = 0; i < N; i++) {
+        g->chunkCost[(size_t)i >> 6] += 16;
+        for (uint32_t j = indptr[i]; j < indptr[i + 1]; j++)
+            if (weights[j] != 0.f) {
+                g->chunkCost[(size_t)i >> 6]++;
+                g->chunkCost[(size_t)indices[j] >> 6]++;
+            }
+    }
+    g->d.chunkCost = g->chunkCost.data();
     g->d.N = N; g->d.E = E; g->d.indptr = dIndptr; g->d.edges = dEdges; g->d.sign = dSign; g->d.counts = dCounts; g->d.rowSyn = dRowSyn; g->d.inOff = dInOff;
     *out = g;
     return FB_OK;

@@ -70,6 +70,10 @@ struct DevGraph {
     const uint32_t *rowSyn;  // synapse count summed over each row (0 without counts)
     const uint32_t *inOff;   // [N+1] in-degree prefix: target q owns run entries inOff[q] .. inOff[q+1]-1
     int64_t maxRow;          // longest row
+    // host memory, never read on the device: per 64-neuron chunk, the work it
+    // brings the lookahead engine (non-zero in-edges + non-zero out-edges +
+    // 16 per neuron), which its regular blocks balance
+    const uint64_t *chunkCost;
 };
 
 struct DevBrain {

@@ -57,6 +57,8 @@
 #include <algorithm>
 #include <cstdlib>
 #include <map>
+#include <tuple>
+#include <vector>
 
 #include "fb_internal.h"
 
@@ -88,6 +90,8 @@ constexpr int LEAP_PERSIST_MATS = LEAP_TABLE_MATS;   // M^1..M^32, M^(32h), and 
 #endif
 constexpr int SHARE_BLOCKS = FB_SHARE_BLOCKS;   // blocks that push the driven block's rows
 constexpr int SHQ = 256;            // driven neurons per share block
+constexpr int LONG_ROW = 1024;      // a regular block's rows longer than this are pushed by the whole grid
+constexpr int LR_CAP = SPK_CAP;     // long rows per epoch in the shared list (more stay with their block)
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -103,7 +107,8 @@ struct PersistArgs {
     int32_t head0;       // ring head at launch
     int32_t nsteps;
     int32_t D;           // epoch length (the delay in steps)
-    int32_t nb;          // neurons per regular block (multiple of 64)
+    int32_t nb;          // largest regular block, in neurons (a multiple of 64, at most NB_MAX)
+    const int32_t *bstart;   // [S-G+1] regular block r owns neurons bstart[r] .. bstart[r+1]-1
     int32_t G;           // driven blocks (blocks 0 .. G-1)
     int32_t Q;           // driven-list positions per driven block
     int32_t S;           // owning blocks: the driven blocks + regular blocks
@@ -123,6 +128,10 @@ struct PersistArgs {
     float *dres;         // [2][DRV_CAP] res of each driven neuron at the epoch's end
     int32_t *bar;        // grid barrier counters (GBAR_WORDS, zero at launch)
     uint64_t *xc;        // [DRV_BLOCKS][XC_RING] (step + 1) << 32 | non-refractory count, zero at launch
+    int32_t *lrCnt;      // [2] long rows queued by the epochs of each parity (zero at launch)
+    int32_t *lrI;        // [2][LR_CAP] a queued long row: neuron, step << 24 | arrival key, sign x res at arrival
+    uint32_t *lrK;
+    float *lrR;
     int32_t flags;       // bit 0: time the driven block's step parts
 };
 
@@ -133,7 +142,7 @@ struct Shared {
         struct {
             uint64_t evKey[EV_CAP];
             float evAdd[EV_CAP];
-            uint16_t evStart[EV_CAP];    // first sorted event of the r-th (step, target) with arrivals
+            uint16_t evStart[EV_CAP];    // the r-th (step, target) with arrivals: counts, then starts, then ends of its run
         };
         struct {
             int32_t spkI[SPK_CAP];       // push list: neuron, step << 24 | arrival key, sign x res at arrival
@@ -165,94 +174,90 @@ struct Shared {
 
 // ------------------------------------------------------------------ helpers
 
-// Bitonic sort of evKey/evAdd[0, n) for any n: every comparator sorts
-// ascending (the first stage of each merge compares an element with its
-// mirror in the block), so the missing elements n .. P-1 act as +infinity
-// and never need storage.
-__device__ inline void lds_sort(Shared &s, int32_t n) {
-    int32_t P = 1;
-    while (P < n) P <<= 1;
-    for (int32_t k = 2; k <= P; k <<= 1) {
-        for (int32_t i = threadIdx.x; i < n; i += PT) {
-            const int32_t l = i ^ (k - 1);
-            if (l > i && l < n) {
-                const uint64_t ki = s.evKey[i], kl = s.evKey[l];
-                if (ki > kl) {
-                    s.evKey[i] = kl; s.evKey[l] = ki;
-                    const float t = s.evAdd[i]; s.evAdd[i] = s.evAdd[l]; s.evAdd[l] = t;
-                }
-            }
-        }
-        __syncthreads();
-        for (int32_t j = k >> 2; j > 0; j >>= 1) {
-            for (int32_t i = threadIdx.x; i < n; i += PT) {
-                const int32_t l = i ^ j;
-                if (l > i && l < n) {
-                    const uint64_t ki = s.evKey[i], kl = s.evKey[l];
-                    if (ki > kl) {
-                        s.evKey[i] = kl; s.evKey[l] = ki;
-                        const float t = s.evAdd[i]; s.evAdd[i] = s.evAdd[l]; s.evAdd[l] = t;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Sorts evKey/evAdd[0, n) (keys are distinct).  Up to PT events: each thread
-// counts the smaller keys of its event and writes it to that rank; beyond
-// that, a bitonic sort.
-__device__ inline void sort_events(Shared &s, int32_t n) {
-    if (n > PT) { lds_sort(s, n); return; }
-    const int tid = threadIdx.x;
-    uint64_t mine = 0;
-    float add = 0.f;
-    int32_t rank = 0;
-    if (tid < n) {
-        mine = s.evKey[tid];
-        add = s.evAdd[tid];
-        for (int32_t q = 0; q < n; q++) rank += s.evKey[q] < mine;
-    }
-    __syncthreads();
-    if (tid < n) { s.evKey[rank] = mine; s.evAdd[rank] = add; }
-    __syncthreads();
-}
-
-// After the sort: evStart[r] = first event of the r-th (step, target) with
-// arrivals, r counted in hasEv bit order, so a neuron finds its run in O(1).
-__device__ inline void index_events(Shared &s, int32_t n) {
-    const int tid = threadIdx.x;
-    constexpr int W = D_MAX * WPS, PER = W / PT;
-    int32_t cnt = 0;
-#pragma unroll
-    for (int v = 0; v < PER; v++) cnt += __popc(s.hasEv[tid * PER + v]);
-    int32_t total;
-    int32_t off = wg_exclusive_scan<PT>(cnt, s.scan, &total);
-#pragma unroll
-    for (int v = 0; v < PER; v++) { s.evRank[tid * PER + v] = off; off += __popc(s.hasEv[tid * PER + v]); }
-    __syncthreads();
-    for (int32_t g = tid; g < n; g += PT) {
-        const uint32_t sl = ev_slot(s.evKey[g]);
-        if (g == 0 || ev_slot(s.evKey[g - 1]) != sl) {
-            const uint32_t m = sl >> 12, ql = sl & 4095u;
-            const int32_t w = (int32_t)(m * WPS + (ql >> 5));
-            const int32_t r = s.evRank[w] + __popc(s.hasEv[w] & ((1u << (ql & 31)) - 1u));
-            s.evStart[r] = (uint16_t)g;
-        }
-    }
-    __syncthreads();
-}
-
-// This step's arrivals on one neuron: the contiguous run of its (step, target)
-__device__ __forceinline__ void apply_run(const Shared &s, int32_t n, uint32_t m, uint32_t ql, float &gE, float &gI) {
+// The run of a (step, target) pair: its rank among the pairs with events of
+// the epoch, counted in hasEv bit order (step-major, then target), which is
+// the order of the sorted keys.
+__device__ __forceinline__ int32_t run_of(const Shared &s, uint32_t m, uint32_t ql) {
     const int32_t w = (int32_t)(m * WPS + (ql >> 5));
-    const int32_t r = s.evRank[w] + __popc(s.hasEv[w] & ((1u << (ql & 31)) - 1u));
-    const uint32_t sl = (m << 12) | ql;
-    for (int32_t q = s.evStart[r]; q < n; q++) {
-        const uint64_t key = s.evKey[q];
-        if (ev_slot(key) != sl) break;
-        if (ev_class(key)) gI = gI + s.evAdd[q]; else gE = gE + s.evAdd[q];
+    return s.evRank[w] + __popc(s.hasEv[w] & ((1u << (ql & 31)) - 1u));
+}
+__device__ __forceinline__ int32_t run_of_key(const Shared &s, uint64_t key) {
+    const uint32_t sl = ev_slot(key);
+    return run_of(s, sl >> 12, sl & 4095u);
+}
+
+// evStart as 32-bit words for LDS atomics: run r is the 16-bit half r & 1 of
+// word r >> 1 (every value stays below EV_CAP < 65,536, so no carry crosses
+// halves).  Adds 1 to run r's half and returns its old value.
+__device__ __forceinline__ uint32_t run_inc(Shared &s, int32_t r) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(s.evStart);
+    const uint32_t sh = (uint32_t)(r & 1) * 16u;
+    return (atomicAdd(&w[r >> 1], 1u << sh) >> sh) & 0xFFFFu;
+}
+
+// Runs longer than this are ordered by ranks computed by a whole wave.
+constexpr int RUN_SMALL = 16;
+
+// After the counting sort every run r = [end(r-1), end(r)) (end = evStart)
+// holds its events in arbitrary order: order each by key (class, arrival
+// key, position in row), which is lif.c's addition order.  A run of at most
+// RUN_SMALL events is insertion-sorted in place by one thread; a longer one
+// by one wave: each lane ranks its events against the run and writes them to
+// their slot in this block's bucket (free once read), and they are copied
+// back.  Every thread of the block calls it.
+__device__ void sort_runs(const PersistArgs &a, Shared &s, int32_t nRuns, int64_t base) {
+    const int tid = threadIdx.x;
+    int anyLong = 0;
+    for (int32_t r = tid; r < nRuns; r += PT) {
+        const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
+        if (b1 - b0 < 2) continue;
+        if (b1 - b0 > RUN_SMALL) { anyLong = 1; continue; }
+        for (int32_t i = b0 + 1; i < b1; i++) {
+            const uint64_t k = s.evKey[i];
+            const float v = s.evAdd[i];
+            int32_t j = i - 1;
+            while (j >= b0 && s.evKey[j] > k) {
+                s.evKey[j + 1] = s.evKey[j];
+                s.evAdd[j + 1] = s.evAdd[j];
+                j--;
+            }
+            s.evKey[j + 1] = k;
+            s.evAdd[j + 1] = v;
+        }
+    }
+    if (!__syncthreads_or(anyLong)) return;
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int32_t r = wave; r < nRuns; r += PT / 64) {
+        const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
+        if (b1 - b0 <= RUN_SMALL) continue;
+        for (int32_t q = b0 + lane; q < b1; q += 64) {
+            const uint64_t k = s.evKey[q];
+            int32_t rank = 0;
+            for (int32_t h = b0; h < b1; h++) rank += s.evKey[h] < k;   // keys are distinct
+            a.bkKey[base + b0 + rank] = k;
+            a.bkAdd[base + b0 + rank] = s.evAdd[q];
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int32_t r = wave; r < nRuns; r += PT / 64) {
+        const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
+        if (b1 - b0 <= RUN_SMALL) continue;
+        for (int32_t q = b0 + lane; q < b1; q += 64) {
+            s.evKey[q] = a.bkKey[base + q];
+            s.evAdd[q] = a.bkAdd[base + q];
+        }
+    }
+    __syncthreads();
+}
+
+// This step's arrivals on one neuron: its (step, target) run, excitatory
+// events first, each class in lif.c's order
+__device__ __forceinline__ void apply_run(const Shared &s, uint32_t m, uint32_t ql, float &gE, float &gI) {
+    const int32_t r = run_of(s, m, ql);
+    const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
+    for (int32_t q = b0; q < b1; q++) {
+        if (ev_class(s.evKey[q])) gI = gI + s.evAdd[q]; else gE = gE + s.evAdd[q];
     }
 }
 
@@ -272,15 +277,20 @@ __device__ __forceinline__ int64_t bucket_at(const PersistArgs &a, int32_t par, 
 // Row lengths of the `ns` rows of the LDS push list -> the flattened edge
 // list: s.rowOff[r] (first edge of row r, s.rowOff[ns] = total) and s.rowA[r]
 // (row start in the CSR).  Row r is (s.spkI[r], step and key in s.spkK[r],
-// sign x res in s.spkR[r]); a row whose sign x res is 0 adds nothing.  Adds
-// the rows' edge and synapse counts to s.ev / s.syn.  Returns the total.
-__device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns) {
+// sign x res in s.spkR[r]); a row whose sign x res is 0 adds nothing.  With
+// `count`, adds the rows' edge and synapse counts to s.ev / s.syn.  With
+// lpar >= 0, a row longer than LONG_ROW goes to the shared list of parity
+// lpar instead (while it has room) and adds nothing here: after the grid
+// barrier every block pushes a slice of those rows (push_long_share).
+// Returns the total.
+__device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns, bool count = true,
+                                   int32_t lpar = -1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     // row lengths -> prefix (RPT consecutive rows per thread)
     int32_t len[RPT];
     uint32_t ra[RPT];
-    int32_t mySum = 0;
+    int32_t mySum = 0, evs = 0;
     unsigned long long syn = 0;
 #pragma unroll
     for (int v = 0; v < RPT; v++) {
@@ -293,6 +303,17 @@ __device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns) 
             len[v] = (int32_t)(d.g.indptr[i + 1] - ra[v]);
             syn += d.g.rowSyn[i];   // 0 for an empty row
         }
+        evs += len[v];
+        if (lpar >= 0 && len[v] > LONG_ROW) {
+            const int32_t q = atomicAdd(&a.lrCnt[lpar], 1);
+            if (q < LR_CAP) {
+                const int64_t at = (int64_t)lpar * LR_CAP + q;
+                a.lrI[at] = s.spkI[r];
+                a.lrK[at] = s.spkK[r];
+                a.lrR[at] = s.spkR[r];
+                len[v] = 0;
+            }
+        }
         mySum += len[v];
     }
     int32_t total;
@@ -304,7 +325,7 @@ __device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns) 
         off += len[v];
     }
     if (tid == 0) s.rowOff[ns] = total;
-    if (mySum) { atomicAdd(&s.ev, (unsigned long long)mySum); atomicAdd(&s.syn, syn); }
+    if (count && evs) { atomicAdd(&s.ev, (unsigned long long)evs); atomicAdd(&s.syn, syn); }
     __syncthreads();
     return total;
 }
@@ -366,9 +387,31 @@ __device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, 
 
 // Writes the events of the `ns` rows of the LDS push list into the buckets
 // for parity `par`.
-__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t ns) {
-    const int32_t total = row_scan(a, s, ns);
+__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t lpar = -1) {
+    const int32_t total = row_scan(a, s, ns, true, lpar);
     push_edges(a, s, par, ns, 0, total);
+}
+
+// The long rows the regular blocks queued this epoch (list parity lpar), as
+// one flattened edge list split evenly over the grid: block b writes edges
+// [b T / G, (b+1) T / G) into the buckets of parity par.  The owners counted
+// their events.  Every thread of every block calls it.
+__device__ void push_long_share(const PersistArgs &a, Shared &s, int32_t par, int32_t lpar) {
+    const int tid = threadIdx.x;
+    const int32_t n = min(__hip_atomic_load(&a.lrCnt[lpar], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), LR_CAP);
+    if (n == 0) return;   // the same value in every block after the grid barrier
+    __syncthreads();      // the LDS push list may still be read by this block's driven share push
+    for (int32_t r = tid; r < n; r += PT) {
+        const int64_t at = (int64_t)lpar * LR_CAP + r;
+        s.spkI[r] = a.lrI[at];
+        s.spkK[r] = a.lrK[at];
+        s.spkR[r] = a.lrR[at];
+    }
+    __syncthreads();
+    const int32_t total = row_scan(a, s, n, false);
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    push_edges(a, s, par, n, (int32_t)(total * b / G), (int32_t)(total * (b + 1) / G));
+    __syncthreads();
 }
 
 // Folds this block's work counters (row_scan) into the brain's.
@@ -420,6 +463,11 @@ __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
 
 // ------------------------------------------------------------------ gather: bucket -> sorted LDS events
 
+// The epoch's events of this block, from its bucket into LDS, ordered by
+// (step, target, class, arrival key, position in row): a counting sort over
+// the (step, target) pairs with events -- the bucket is read three times
+// (which pairs, how many each, where each goes) -- then each pair's run in
+// key order (sort_runs).  Afterwards evStart[r] is the end of run r.
 __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
     const int tid = threadIdx.x;
     // the count is read once, then cleared for the push two epochs on
@@ -429,20 +477,54 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
         *cnt = 0;
     }
     for (int32_t w = tid; w < D_MAX * WPS; w += PT) s.hasEv[w] = 0u;
+    uint32_t *rw = reinterpret_cast<uint32_t *>(s.evStart);
+    for (int32_t w = tid; w < EV_CAP / 2; w += PT) rw[w] = 0u;
     __syncthreads();
     const int32_t n = s.nev;
     const int64_t base = bucket_at(a, par, me);
+    // which (step, target) pairs have events
     for (int32_t g = tid; g < n; g += PT) {
-        const uint64_t key = a.bkKey[base + g];
-        s.evKey[g] = key;
-        s.evAdd[g] = a.bkAdd[base + g];
-        const uint32_t sl = ev_slot(key);
+        const uint32_t sl = ev_slot(a.bkKey[base + g]);
         const uint32_t m = sl >> 12, ql = sl & 4095u;
         atomicOr(&s.hasEv[m * WPS + (ql >> 5)], 1u << (ql & 31));
     }
     __syncthreads();
-    if (n > 1) sort_events(s, n);
-    index_events(s, n);
+    // each pair's run index: set bits of hasEv before it
+    constexpr int W = D_MAX * WPS, PER = W / PT;
+    int32_t c = 0;
+#pragma unroll
+    for (int v = 0; v < PER; v++) c += __popc(s.hasEv[tid * PER + v]);
+    int32_t nRuns;
+    int32_t off = wg_exclusive_scan<PT>(c, s.scan, &nRuns);
+#pragma unroll
+    for (int v = 0; v < PER; v++) { s.evRank[tid * PER + v] = off; off += __popc(s.hasEv[tid * PER + v]); }
+    __syncthreads();
+    // events per run
+    for (int32_t g = tid; g < n; g += PT) run_inc(s, run_of_key(s, a.bkKey[base + g]));
+    __syncthreads();
+    // counts -> first slot of each run (16-bit stores: two runs share a word)
+    const int32_t per = (nRuns + PT - 1) / PT;
+    const int32_t r0 = min(nRuns, tid * per), r1 = min(nRuns, r0 + per);
+    int32_t sum = 0;
+    for (int32_t r = r0; r < r1; r++) sum += s.evStart[r];
+    int32_t tot;
+    int32_t at = wg_exclusive_scan<PT>(sum, s.scan, &tot);
+    for (int32_t r = r0; r < r1; r++) {
+        const int32_t k = s.evStart[r];
+        s.evStart[r] = (uint16_t)at;
+        at += k;
+    }
+    __syncthreads();
+    // every event to the next free slot of its run
+    for (int32_t g = tid; g < n; g += PT) {
+        const uint64_t key = a.bkKey[base + g];
+        const float add = a.bkAdd[base + g];
+        const uint32_t q = run_inc(s, run_of_key(s, key));
+        s.evKey[q] = key;
+        s.evAdd[q] = add;
+    }
+    __syncthreads();
+    sort_runs(a, s, nRuns, base);
     return n;
 }
 
@@ -483,7 +565,7 @@ __device__ __forceinline__ void cell_inputs_has(const Consts &k, Cell &c, const 
     if ((c.prev >> m) & 1u) {
         if (c.sign * c.st.res != 0.f) c.st.res = c.st.res - k.depU * c.st.res;
     }
-    if (has) apply_run(s, nev, (uint32_t)m, ql, c.st.gE, c.st.gI);
+    if (has) apply_run(s, (uint32_t)m, ql, c.st.gE, c.st.gI);
 }
 __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shared &s, int32_t nev, int32_t m,
                                             uint32_t ql) {
@@ -807,9 +889,11 @@ __device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)
 }
 
 // A regular block pushes its own spikes.
-__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t me) {
+// A regular block pushes its own spikes (buckets of parity par); rows longer
+// than LONG_ROW go to the shared list of parity lpar.
+__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t lpar) {
     const int32_t ns = collect_spikes(a, s, cells);
-    push_rows(a, s, par, ns);
+    push_rows(a, s, par, ns, lpar);
     publish_stats(a, s);
 }
 
@@ -987,8 +1071,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     const int32_t me = blockIdx.x;
     const bool drivenBlock = me < a.G;
     const bool owning = me < a.S;
-    const int32_t lo = drivenBlock || !owning ? 0 : (me - a.G) * a.nb;
-    const int32_t hi = drivenBlock || !owning ? 0 : min(d.N, lo + a.nb);
+    const int32_t lo = drivenBlock || !owning ? 0 : a.bstart[me - a.G];
+    const int32_t hi = drivenBlock || !owning ? 0 : a.bstart[me - a.G + 1];
 
     // clocks of the first driven block, the first regular block and the first spare block
     const int32_t timedSlot = me == 0 ? 0 : me == a.G ? 1 : (a.nSpare > 0 && me == a.S) ? 2 : -1;
@@ -1063,6 +1147,10 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             // overflow flags of the last push: read now, acted on after the
             // gather (which only reads), so the load overlaps the gather's
             const int32_t stop = e > 0 ? __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            // the long-row list of the other parity was last read before the
+            // previous epoch's second barrier and is next written after this
+            // epoch's: clear it now
+            if (me == 0 && tid == 0) (void)atomicExch(&a.lrCnt[par ^ 1], 0);
             if (e > 0) build_lists(a, s, par ^ 1, done - prevL, prevL);
             FB_PHASE(3)
             int32_t nev = 0;
@@ -1077,7 +1165,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 else if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
-                if (done + L < a.nsteps && !drivenBlock) push_spikes(a, s, cells, par ^ 1, me);
+                if (done + L < a.nsteps && !drivenBlock) push_spikes(a, s, cells, par ^ 1, par);
 #pragma unroll
                 for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
@@ -1090,6 +1178,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             FB_PHASE(4)
             if (done + L < a.nsteps) {
                 push_driven_share(a, s, par ^ 1, par);
+                push_long_share(a, s, par ^ 1, par);
                 FB_PHASE(6)
                 if (!grid_barrier(a.bar, &a.ctl[1])) return;
                 FB_PHASE(7)
@@ -1170,11 +1259,44 @@ static int32_t refractory_steps(const Consts &k) {
 
 struct Layout {
     int32_t nb, G, Q, S, nSpare, grid;
+    std::vector<int32_t> starts;   // regular block r owns neurons starts[r] .. starts[r+1]-1 (64-aligned; last = N)
 };
 
+// Greedy packing of the 64-neuron chunks into consecutive ranges of at most
+// CPB chunks whose cost stays within T (a single heavier chunk takes a range
+// of its own); the chunk index of each range's start.
+static int32_t pack_chunks(const uint64_t *cost, int32_t nC, int32_t CPB, uint64_t T, std::vector<int32_t> *starts) {
+    int32_t blocks = 0, inBlk = 0;
+    uint64_t cur = 0;
+    for (int32_t c = 0; c < nC; c++) {
+        if (blocks == 0 || inBlk == CPB || (inBlk > 0 && cur + cost[c] > T)) {
+            if (starts) starts->push_back(c);
+            blocks++;
+            inBlk = 0;
+            cur = 0;
+        }
+        cur += cost[c];
+        inBlk++;
+    }
+    return blocks;
+}
+
 // The driven list splits evenly over the fewest driven blocks that hold it.
-// Regular blocks own 64-aligned ranges; a few blocks of the grid are left
-// spare for the list building when the network leaves room.
+// Regular blocks own 64-aligned ranges of at most 1024 neurons, as many as
+// equal ranges of the largest size that leaves D blocks of the grid spare
+// for the list building (or none spare, if it must), with boundaries that
+// balance the chunks' cost (DevGraph::chunkCost): a connectome's hubs sit
+// together, and equal ranges would hand one block many times the average
+// number of events to sort and rows to push.  Cached per graph and grid.
+struct LayoutKey {
+    const uint64_t *cost;
+    int32_t N, G, D, maxB;
+    bool operator<(const LayoutKey &o) const {
+        return std::tie(cost, N, G, D, maxB) < std::tie(o.cost, o.N, o.G, o.D, o.maxB);
+    }
+};
+static std::map<LayoutKey, std::vector<int32_t>> g_layouts;
+
 static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
     const int32_t maxB = g_ps.maxBlocks;
     l.G = std::max(1, (d.nDriven + DRV_BLK - 1) / DRV_BLK);
@@ -1186,9 +1308,32 @@ static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
         nb = ((int64_t)(d.N + (maxB - l.G) - 1) / (maxB - l.G) + 63) / 64 * 64;   // no spares
         if (nb > NB_MAX) return false;
     }
-    l.nb = (int32_t)nb;
     const int32_t need = (int32_t)((d.N + nb - 1) / nb);
-    l.S = need + l.G;
+    const LayoutKey key{d.g.chunkCost, d.N, l.G, D, maxB};
+    auto it = g_layouts.find(key);
+    if (it == g_layouts.end()) {
+        const int32_t nC = (d.N + 63) / 64, CPB = NB_MAX / 64;
+        std::vector<int32_t> st;
+        if (d.g.chunkCost) {
+            uint64_t lo = 0, hi = 0;
+            for (int32_t c = 0; c < nC; c++) { lo = std::max(lo, d.g.chunkCost[c]); hi += d.g.chunkCost[c]; }
+            while (lo < hi) {   // the smallest bound that packs into `need` ranges
+                const uint64_t mid = lo + (hi - lo) / 2;
+                if (pack_chunks(d.g.chunkCost, nC, CPB, mid, nullptr) <= need) hi = mid; else lo = mid + 1;
+            }
+            pack_chunks(d.g.chunkCost, nC, CPB, lo, &st);
+            if ((int32_t)st.size() > need) st.clear();
+        }
+        if (st.empty())   // equal ranges
+            for (int32_t r = 0; r < need; r++) st.push_back((int32_t)(r * nb / 64));
+        for (int32_t &c : st) c *= 64;
+        st.push_back(d.N);
+        it = g_layouts.emplace(key, std::move(st)).first;
+    }
+    l.starts = it->second;
+    l.nb = 0;
+    for (size_t r = 0; r + 1 < l.starts.size(); r++) l.nb = std::max(l.nb, l.starts[r + 1] - l.starts[r]);
+    l.S = (int32_t)l.starts.size() - 1 + l.G;
     if (l.S > S_MAX) return false;
     l.nSpare = std::min(D, maxB - l.S);
     l.grid = l.S + l.nSpare;
@@ -1238,10 +1383,16 @@ struct PersistBuffers {
     float *dres = nullptr;
     int32_t *bar = nullptr;
     uint64_t *xc = nullptr;
+    int32_t *lrCnt = nullptr, *lrI = nullptr;
+    uint32_t *lrK = nullptr;
+    float *lrR = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t E = 0;
-    int32_t eownNb = -1, eownG = -1;
+    int32_t eownG = -1;
+    std::vector<int32_t> starts;     // the regular ranges bstart / chunkBlk hold
+    int32_t *bstart = nullptr;       // [S_MAX + 1]
+    int32_t *chunkBlk = nullptr;     // [N/64] regular block of each 64-neuron chunk
     uint64_t eownDriven = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1250,7 +1401,8 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.bkKey, pb.bkAdd, pb.bkCnt,
-                  pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar, pb.xc};
+                  pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar, pb.xc, pb.bstart, pb.chunkBlk,
+                  pb.lrCnt, pb.lrI, pb.lrK, pb.lrR};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
@@ -1259,7 +1411,12 @@ static void free_buffers(PersistBuffers &pb) {
     pb.dres = nullptr;
     pb.bar = nullptr;
     pb.xc = nullptr;
-    pb.eownNb = -1;
+    pb.lrCnt = pb.lrI = nullptr;
+    pb.lrK = nullptr;
+    pb.lrR = nullptr;
+    pb.bstart = nullptr;
+    pb.chunkBlk = nullptr;
+    pb.starts.clear();
     pb.eownG = -1;
     pb.eownDriven = ~0ull;
 }
@@ -1287,28 +1444,30 @@ __global__ void k_drvpos_set(int32_t *drvPos, const int32_t *driven, int32_t nD)
 // the launch's control words, barrier counters, clocks, exchange words and
 // bucket counts cleared in one kernel (five fills would be five queued
 // operations)
-__global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, uint64_t *xc, int32_t *bkCnt, int32_t nCnt) {
+__global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, uint64_t *xc, int32_t *lrCnt, int32_t *bkCnt,
+                                int32_t nCnt) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t n = gridDim.x * blockDim.x;
     if (i < 4) ctl[i] = 0;
     if (i < PERSIST_PROF) prof[i] = 0;
     if (i < DRV_BLOCKS * XC_RING) xc[i] = 0;
+    if (i < 2) lrCnt[i] = 0;
     for (int32_t k = i; k < GBAR_WORDS; k += n) bar[k] = 0;
     for (int32_t k = i; k < nCnt; k += n) bkCnt[k] = 0;
 }
 
 // destination block << 12 | local index, per edge: a driven target at list
 // position p belongs to driven block p / Q (index p mod Q), others to block
-// G + q / nb
-__global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos, int32_t nb,
-                             int32_t G, int32_t Q) {
+// G + r for the regular range r that holds them
+__global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, const int32_t *drvPos,
+                             const int32_t *chunkBlk, const int32_t *bstart, int32_t G, int32_t Q) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < E; j += (int64_t)gridDim.x * blockDim.x) {
         const int32_t q = (int32_t)edges[j].x;
         const int32_t dp = drvPos[q];
-        const int32_t blk = q / nb;
+        const int32_t blk = chunkBlk[q >> 6];
         const int32_t db = dp >= 0 ? dp / Q : 0;
         eown[j] = dp >= 0 ? ((uint32_t)db << 12) | (uint32_t)(dp - db * Q)
-                          : ((uint32_t)(G + blk) << 12) | (uint32_t)(q - blk * nb);
+                          : ((uint32_t)(G + blk) << 12) | (uint32_t)(q - bstart[blk]);
     }
 }
 
@@ -1342,23 +1501,40 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.dres, sizeof(float) * 2 * DRV_CAP));
         FB_HIP(hipMalloc(&pb.bar, sizeof(int32_t) * GBAR_WORDS));
         FB_HIP(hipMalloc(&pb.xc, sizeof(uint64_t) * DRV_BLOCKS * XC_RING));
+        FB_HIP(hipMalloc(&pb.bstart, sizeof(int32_t) * (S_MAX + 1)));
+        FB_HIP(hipMalloc(&pb.lrCnt, sizeof(int32_t) * 2));
+        FB_HIP(hipMalloc(&pb.lrI, sizeof(int32_t) * 2 * LR_CAP));
+        FB_HIP(hipMalloc(&pb.lrK, sizeof(uint32_t) * 2 * LR_CAP));
+        FB_HIP(hipMalloc(&pb.lrR, sizeof(float) * 2 * LR_CAP));
+        FB_HIP(hipMalloc(&pb.chunkBlk, sizeof(int32_t) * (size_t)((d.N + 63) / 64)));
         if (!pb.ev0) {
             FB_HIP(hipEventCreate(&pb.ev0));
             FB_HIP(hipEventCreate(&pb.ev1));
         }
     }
     // driven positions and edge owners (again when the driven set or layout changed)
-    if (pb.eownDriven != drivenVersion || pb.eownNb != lay.nb || pb.eownG != lay.G) {
+    if (pb.starts != lay.starts) {
+        // the regular ranges: block starts, and each chunk's block for the edge owners
+        const int32_t R = (int32_t)lay.starts.size() - 1, nC = (d.N + 63) / 64;
+        std::vector<int32_t> cb((size_t)nC);
+        for (int32_t r = 0; r < R; r++)
+            for (int32_t c = lay.starts[r] / 64; c < (lay.starts[r + 1] + 63) / 64; c++) cb[(size_t)c] = r;
+        FB_HIP(hipMemcpyAsync(pb.bstart, lay.starts.data(), sizeof(int32_t) * (size_t)(R + 1), hipMemcpyHostToDevice, st));
+        FB_HIP(hipMemcpyAsync(pb.chunkBlk, cb.data(), sizeof(int32_t) * (size_t)nC, hipMemcpyHostToDevice, st));
+        FB_HIP(hipStreamSynchronize(st));   // the host vector goes out of scope
+        pb.starts = lay.starts;
+        pb.eownG = -1;
+    }
+    if (pb.eownDriven != drivenVersion || pb.eownG != lay.G) {
         hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
         if (d.nDriven)
             hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
                                d.nDriven);
         if (d.g.E)
             hipLaunchKernelGGL(k_edge_owner, dim3(1024), dim3(256), 0, st, pb.eown, d.g.edges, d.g.E, pb.drvPos,
-                               lay.nb, lay.G, std::max(lay.Q, 1));
+                               pb.chunkBlk, pb.bstart, lay.G, std::max(lay.Q, 1));
         FB_HIP(hipGetLastError());
         pb.eownDriven = drivenVersion;
-        pb.eownNb = lay.nb;
         pb.eownG = lay.G;
     }
 
@@ -1366,6 +1542,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.d = d;
     a.D = D;
     a.nb = lay.nb;
+    a.bstart = pb.bstart;
     a.G = lay.G;
     a.Q = lay.Q;
     a.S = lay.S;
@@ -1385,6 +1562,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.dres = pb.dres;
     a.bar = pb.bar;
     a.xc = pb.xc;
+    a.lrCnt = pb.lrCnt;
+    a.lrI = pb.lrI;
+    a.lrK = pb.lrK;
+    a.lrR = pb.lrR;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
@@ -1392,8 +1573,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     while (done < nsteps) {
         a.head0 = cur;
         a.nsteps = nsteps - done;
-        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.bkCnt,
-                           2 * lay.S * BK_STRIDE);
+        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.lrCnt,
+                           pb.bkCnt, 2 * lay.S * BK_STRIDE);
         FB_HIP(hipGetLastError());
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
