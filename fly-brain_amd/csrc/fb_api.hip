@@ -33,7 +33,6 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
                       hipStream_t st);
 fb_status launch_gather(const DevBrain &d, int32_t s0, int32_t count, int32_t *cnt, int32_t *dst, hipStream_t st);
 fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why);
-bool persistent_preferred(const DevBrain &d);
 uint32_t persistent_overflow_flags(const DevBrain &d);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
                             int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion);
@@ -927,8 +926,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
         // whose kernels are shorter than their launch; launched directly for
         // large ones, whose kernels hide the launch and where a capture after
         // every drive change would cost more than it saves
-        engine = longRun && persistent_supported(b->d, serial_deliver(b), &why) == FB_OK && persistent_preferred(b->d)
-                     ? FB_ENGINE_PERSISTENT
+        engine = longRun && persistent_supported(b->d, serial_deliver(b), &why) == FB_OK ? FB_ENGINE_PERSISTENT
                  : b->d.N >= 65536                                                       ? FB_ENGINE_LAUNCH
                                                                                           : FB_ENGINE_GRAPH;
     }

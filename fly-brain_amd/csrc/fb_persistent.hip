@@ -92,6 +92,7 @@ constexpr int SHARE_BLOCKS = FB_SHARE_BLOCKS;   // blocks that push the driven b
 constexpr int SHQ = 256;            // driven neurons per share block
 constexpr int LONG_ROW = 1024;      // a regular block's rows longer than this are pushed by the whole grid
 constexpr int LR_CAP = SPK_CAP;     // long rows per epoch in the shared list (more stay with their block)
+constexpr int RUN_LIST = 1024;      // (step, target) runs a gather can list for ordering by a wave
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -159,6 +160,8 @@ struct Shared {
     int32_t scan[64 + PT / 64];
     uint32_t rngNext[2];
     int32_t nev, nspk;
+    int32_t nRunList;                // sort_runs: runs ordered by a wave
+    uint16_t runList[RUN_LIST];
     unsigned long long ev, syn;
     uint64_t ph[NPH + 1];        // phase clocks of thread 0, then the last stamp
     uint64_t sub[7];             // driven-step clocks of thread 0
@@ -195,23 +198,35 @@ __device__ __forceinline__ uint32_t run_inc(Shared &s, int32_t r) {
     return (atomicAdd(&w[r >> 1], 1u << sh) >> sh) & 0xFFFFu;
 }
 
-// Runs longer than this are ordered by ranks computed by a whole wave.
-constexpr int RUN_SMALL = 16;
+// Runs of at most RUN_SMALL events are insertion-sorted by one thread; longer
+// ones are listed (up to RUN_LIST of them) and ordered by a wave.
+constexpr int RUN_SMALL = 4;
+constexpr int GREG = 4;             // events per thread a gather ranks in registers
 
 // After the counting sort every run r = [end(r-1), end(r)) (end = evStart)
 // holds its events in arbitrary order: order each by key (class, arrival
-// key, position in row), which is lif.c's addition order.  A run of at most
-// RUN_SMALL events is insertion-sorted in place by one thread; a longer one
-// by one wave: each lane ranks its events against the run and writes them to
-// their slot in this block's bucket (free once read), and they are copied
-// back.  Every thread of the block calls it.
+// key, position in row), which is lif.c's addition order.  A short run is
+// insertion-sorted in place by one thread.  A longer run goes to a list, and
+// a wave ranks it: each lane takes one event (a run of up to 64) and
+// counts the smaller keys, and the lanes write their events to their ranks
+// once every lane has read the run -- or, for a run longer than 64, the
+// lanes write to this block's bucket (free once read) and the run is copied
+// back after a block barrier.  More than RUN_LIST long runs: the waves walk
+// every run instead of the list.  Every thread of the block calls it.
 __device__ void sort_runs(const PersistArgs &a, Shared &s, int32_t nRuns, int64_t base) {
     const int tid = threadIdx.x;
-    int anyLong = 0;
+    if (tid == 0) s.nRunList = 0;
+    __syncthreads();
+    int big = 0;
     for (int32_t r = tid; r < nRuns; r += PT) {
         const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
         if (b1 - b0 < 2) continue;
-        if (b1 - b0 > RUN_SMALL) { anyLong = 1; continue; }
+        if (b1 - b0 > RUN_SMALL) {
+            const int32_t at = atomicAdd(&s.nRunList, 1);
+            if (at < RUN_LIST) s.runList[at] = (uint16_t)r;
+            big |= b1 - b0 > 64;
+            continue;
+        }
         for (int32_t i = b0 + 1; i < b1; i++) {
             const uint64_t k = s.evKey[i];
             const float v = s.evAdd[i];
@@ -225,24 +240,42 @@ __device__ void sort_runs(const PersistArgs &a, Shared &s, int32_t nRuns, int64_
             s.evAdd[j + 1] = v;
         }
     }
-    if (!__syncthreads_or(anyLong)) return;
+    big = __syncthreads_or(big);
+    const int32_t nList = s.nRunList;
+    if (nList == 0) return;
+    const bool listed = nList <= RUN_LIST;
+    const int32_t count = listed ? nList : nRuns;
     const int lane = tid & 63, wave = tid >> 6;
-    for (int32_t r = wave; r < nRuns; r += PT / 64) {
+    for (int32_t x = wave; x < count; x += PT / 64) {
+        const int32_t r = listed ? (int32_t)s.runList[x] : x;
         const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
-        if (b1 - b0 <= RUN_SMALL) continue;
-        for (int32_t q = b0 + lane; q < b1; q += 64) {
-            const uint64_t k = s.evKey[q];
+        if (b1 - b0 <= RUN_SMALL) continue;                 // uniform in the wave
+        if (b1 - b0 <= 64) {
+            const int32_t q = b0 + lane;
+            uint64_t k = 0;
+            float v = 0.f;
             int32_t rank = 0;
+            if (q < b1) { k = s.evKey[q]; v = s.evAdd[q]; }
             for (int32_t h = b0; h < b1; h++) rank += s.evKey[h] < k;   // keys are distinct
-            a.bkKey[base + b0 + rank] = k;
-            a.bkAdd[base + b0 + rank] = s.evAdd[q];
+            __builtin_amdgcn_wave_barrier();
+            if (q < b1) { s.evKey[b0 + rank] = k; s.evAdd[b0 + rank] = v; }
+        } else {
+            for (int32_t q = b0 + lane; q < b1; q += 64) {
+                const uint64_t k = s.evKey[q];
+                int32_t rank = 0;
+                for (int32_t h = b0; h < b1; h++) rank += s.evKey[h] < k;
+                a.bkKey[base + b0 + rank] = k;
+                a.bkAdd[base + b0 + rank] = s.evAdd[q];
+            }
         }
     }
+    if (!big) { __syncthreads(); return; }
     __threadfence_block();
     __syncthreads();
-    for (int32_t r = wave; r < nRuns; r += PT / 64) {
+    for (int32_t x = wave; x < count; x += PT / 64) {
+        const int32_t r = listed ? (int32_t)s.runList[x] : x;
         const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
-        if (b1 - b0 <= RUN_SMALL) continue;
+        if (b1 - b0 <= 64) continue;
         for (int32_t q = b0 + lane; q < b1; q += 64) {
             s.evKey[q] = a.bkKey[base + q];
             s.evAdd[q] = a.bkAdd[base + q];
@@ -465,9 +498,10 @@ __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
 
 // The epoch's events of this block, from its bucket into LDS, ordered by
 // (step, target, class, arrival key, position in row): a counting sort over
-// the (step, target) pairs with events -- the bucket is read three times
-// (which pairs, how many each, where each goes) -- then each pair's run in
-// key order (sort_runs).  Afterwards evStart[r] is the end of run r.
+// the (step, target) pairs with events -- which pairs, how many events each,
+// where each goes -- then each pair's run in key order (ranks within the
+// run, or sort_runs past GREG * PT events).  Afterwards evStart[r] is the
+// end of run r.
 __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
     const int tid = threadIdx.x;
     // the count is read once, then cleared for the push two epochs on
@@ -482,11 +516,12 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
     __syncthreads();
     const int32_t n = s.nev;
     const int64_t base = bucket_at(a, par, me);
-    // which (step, target) pairs have events
+    // which (step, target) pairs have events (the keys copied to LDS for the count)
     for (int32_t g = tid; g < n; g += PT) {
-        const uint32_t sl = ev_slot(a.bkKey[base + g]);
-        const uint32_t m = sl >> 12, ql = sl & 4095u;
-        atomicOr(&s.hasEv[m * WPS + (ql >> 5)], 1u << (ql & 31));
+        const uint64_t key = a.bkKey[base + g];
+        s.evKey[g] = key;
+        const uint32_t sl = ev_slot(key);
+        atomicOr(&s.hasEv[(sl >> 12) * WPS + ((sl & 4095u) >> 5)], 1u << (sl & 31u));
     }
     __syncthreads();
     // each pair's run index: set bits of hasEv before it
@@ -500,7 +535,7 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
     for (int v = 0; v < PER; v++) { s.evRank[tid * PER + v] = off; off += __popc(s.hasEv[tid * PER + v]); }
     __syncthreads();
     // events per run
-    for (int32_t g = tid; g < n; g += PT) run_inc(s, run_of_key(s, a.bkKey[base + g]));
+    for (int32_t g = tid; g < n; g += PT) run_inc(s, run_of_key(s, s.evKey[g]));
     __syncthreads();
     // counts -> first slot of each run (16-bit stores: two runs share a word)
     const int32_t per = (nRuns + PT - 1) / PT;
@@ -515,7 +550,7 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
         at += k;
     }
     __syncthreads();
-    // every event to the next free slot of its run
+    // every event to the next free slot of its run, from the bucket
     for (int32_t g = tid; g < n; g += PT) {
         const uint64_t key = a.bkKey[base + g];
         const float add = a.bkAdd[base + g];
@@ -524,7 +559,39 @@ __device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t 
         s.evAdd[q] = add;
     }
     __syncthreads();
-    sort_runs(a, s, nRuns, base);
+    if (n > GREG * PT) {
+        sort_runs(a, s, nRuns, base);
+        return n;
+    }
+    // each run in key order: thread t ranks the events at positions t, t+PT,
+    // ... within their runs (the lanes of a wave mostly share a run, so the
+    // keys they compare are LDS broadcasts), then, once every rank is known,
+    // moves them
+    uint64_t pk[GREG];
+    float pa[GREG];
+    int32_t pd[GREG];
+#pragma unroll
+    for (int v = 0; v < GREG; v++) {
+        const int32_t q = tid + v * PT;
+        pd[v] = -1;
+        pk[v] = 0;
+        pa[v] = 0.f;
+        if (q < n) {
+            const uint64_t k = s.evKey[q];
+            const int32_t r = run_of_key(s, k);
+            const int32_t b0 = r ? s.evStart[r - 1] : 0, b1 = s.evStart[r];
+            if (b1 - b0 > 1) {
+                int32_t rank = 0;
+                for (int32_t h = b0; h < b1; h++) rank += s.evKey[h] < k;   // keys are distinct
+                if (b0 + rank != q) { pd[v] = b0 + rank; pk[v] = k; pa[v] = s.evAdd[q]; }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < GREG; v++)
+        if (pd[v] >= 0) { s.evKey[pd[v]] = pk[v]; s.evAdd[pd[v]] = pa[v]; }
+    __syncthreads();
     return n;
 }
 
@@ -1362,13 +1429,6 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     return FB_OK;
 }
 
-// Whether the automatic engine should take the lookahead engine for a run of
-// at least one epoch (it is supported): with one driven block it is ahead of
-// the per-step engines; with several, the per-step exchange of draw counts
-// and the skew of a connectome's hub blocks make the per-step engines the
-// faster choice today.
-bool persistent_preferred(const DevBrain &d) { return d.nDriven <= DRV_BLK; }
-
 struct PersistBuffers {
     uint64_t *spk = nullptr;
     uint32_t *dmask = nullptr, *tmask = nullptr;
@@ -1389,6 +1449,7 @@ struct PersistBuffers {
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
     int32_t N = 0, S = 0;
     int64_t E = 0;
+    int device = -1;
     int32_t eownG = -1;
     std::vector<int32_t> starts;     // the regular ranges bstart / chunkBlk hold
     int32_t *bstart = nullptr;       // [S_MAX + 1]
@@ -1424,12 +1485,25 @@ static void free_buffers(PersistBuffers &pb) {
 fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serialDeliver, bool rowsUnique,
                       hipStream_t st);
 
+// Buffers of destroyed brains, kept for the next brain of the same shape on
+// the same device (a brain per seed or per run would otherwise allocate and
+// free ~100 MB each time).  The brain's stream is idle when it is released.
+static std::vector<PersistBuffers> g_pool;
+constexpr size_t POOL_MAX = 2;
+
 void persistent_release(const int32_t *key) {
     auto it = g_pbuf.find(key);
     if (it == g_pbuf.end()) return;
-    free_buffers(it->second);
-    if (it->second.ev0) { (void)hipEventDestroy(it->second.ev0); (void)hipEventDestroy(it->second.ev1); }
+    PersistBuffers pb = std::move(it->second);
     g_pbuf.erase(it);
+    if (pb.spk && g_pool.size() < POOL_MAX) {
+        pb.eownDriven = ~0ull;   // the next brain's driven set is its own
+        pb.overflowSeen = 0;
+        g_pool.push_back(std::move(pb));
+        return;
+    }
+    free_buffers(pb);
+    if (pb.ev0) { (void)hipEventDestroy(pb.ev0); (void)hipEventDestroy(pb.ev1); }
 }
 
 __global__ void k_drvpos(int32_t *drvPos, int32_t N) {
@@ -1479,12 +1553,29 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     const int32_t D = d.nslots - 1;
     Layout lay;
     layout_for(d, D, lay);
-    PersistBuffers &pb = g_pbuf[d.driven];
     const int32_t nwords = (d.N + 63) / 64;
-    if (pb.N != d.N || pb.S != lay.S) {
+    int dev = 0;
+    FB_HIP(hipGetDevice(&dev));
+    if (g_pbuf.find(d.driven) == g_pbuf.end()) {
+        for (size_t k = 0; k < g_pool.size(); k++) {
+            PersistBuffers &q = g_pool[k];
+            if (q.N == d.N && q.S == lay.S && q.E == d.g.E && q.device == dev) {
+                // a released brain's buffers: the bitmaps and arrival masks start clear
+                FB_HIP(hipMemsetAsync(q.spk, 0, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords, st));
+                FB_HIP(hipMemsetAsync(q.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
+                g_pbuf[d.driven] = std::move(q);
+                g_pool.erase(g_pool.begin() + (std::ptrdiff_t)k);
+                break;
+            }
+        }
+    }
+    PersistBuffers &pb = g_pbuf[d.driven];
+    if (pb.N != d.N || pb.S != lay.S || pb.E != d.g.E || pb.device != dev) {
         free_buffers(pb);
         pb.N = d.N;
         pb.S = lay.S;
+        pb.E = d.g.E;
+        pb.device = dev;
         FB_HIP(hipMalloc(&pb.spk, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords));
         FB_HIP(hipMemsetAsync(pb.spk, 0, sizeof(uint64_t) * 2 * D_MAX * (size_t)nwords, st));
         FB_HIP(hipMalloc(&pb.dmask, sizeof(uint32_t) * 2 * DRV_CAP));
