@@ -219,7 +219,20 @@ __device__ __forceinline__ uint32_t matvec(const uint32_t *tab, uint32_t x) {
 // matrices; every thread of the block calls it
 template <int T, int MATS = LEAP_MATS>
 __device__ __forceinline__ void leap_load(uint32_t *leapS, const uint32_t *leap) {
-    for (int q = threadIdx.x; q < MATS * LEAP_WORDS; q += T) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = leap[q];
+    // every load issued before the first store: one memory round trip, not
+    // one per word a thread copies
+    constexpr int WORDS = MATS * LEAP_WORDS, PER = (WORDS + T - 1) / T;
+    uint32_t v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int q = (int)threadIdx.x + k * T;
+        v[k] = q < WORDS ? leap[q] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int q = (int)threadIdx.x + k * T;
+        if (q < WORDS) leapS[(q >> 7) * LEAP_STRIDE + (q & 127)] = v[k];
+    }
 }
 
 // M^J x for 1 <= J from LDS tables holding all LEAP_TABLE_MATS matrices:

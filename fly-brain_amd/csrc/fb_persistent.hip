@@ -56,6 +56,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -123,7 +124,11 @@ struct PersistArgs {
     uint64_t *bkKey;     // [2][S][EV_CAP] each destination's bucket of the next epoch's events
     float *bkAdd;
     int32_t *bkCnt;      // [2][S] slots taken, one per BK_STRIDE words (zero at launch; a destination clears its own after reading)
-    int32_t *ctl;        // [0] steps done, [1] overflow flags
+    int32_t *ctl;        // [0] steps done, [1] overflow flags, [2] blocks that have left (zero at launch)
+    int32_t *hostCtl;    // pinned host memory: [0] steps done, [1] flags, [2] tag, from the last block out
+    uint64_t *hostProf;  // pinned host memory: the phase clocks, from the last block out
+    int32_t tag;         // this launch's number
+    float *dstage;       // [DSTAGE_F][DRV_CAP] the driven neurons' state at launch, in driven-list order
     uint64_t *prof;      // phase clocks (see k_persist)
     const uint32_t *eown;    // [E] destination block << 12 | local index of each edge's target
     float *dres;         // [2][DRV_CAP] res of each driven neuron at the epoch's end
@@ -159,7 +164,7 @@ struct Shared {
     int32_t pref[D_MAX + 1];     // prologue: first ring entry of each arrival step
     int32_t scan[64 + PT / 64];
     uint32_t rngNext[2];
-    int32_t nev, nspk;
+    int32_t nev, nspk, last;
     int32_t nRunList;                // sort_runs: runs ordered by a wave
     uint16_t runList[RUN_LIST];
     unsigned long long ev, syn;
@@ -213,7 +218,7 @@ constexpr int GREG = 4;             // events per thread a gather ranks in regis
 // lanes write to this block's bucket (free once read) and the run is copied
 // back after a block barrier.  More than RUN_LIST long runs: the waves walk
 // every run instead of the list.  Every thread of the block calls it.
-__device__ void sort_runs(const PersistArgs &a, Shared &s, int32_t nRuns, int64_t base) {
+__device__ __forceinline__ void sort_runs(const PersistArgs &a, Shared &s, int32_t nRuns, int64_t base) {
     const int tid = threadIdx.x;
     if (tid == 0) s.nRunList = 0;
     __syncthreads();
@@ -316,7 +321,7 @@ __device__ __forceinline__ int64_t bucket_at(const PersistArgs &a, int32_t par, 
 // lpar instead (while it has room) and adds nothing here: after the grid
 // barrier every block pushes a slice of those rows (push_long_share).
 // Returns the total.
-__device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns, bool count = true,
+__device__ __forceinline__ int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns, bool count = true,
                                    int32_t lpar = -1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
@@ -368,7 +373,7 @@ __device__ inline int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns, 
 // flight; a scan of the row starts maps an edge to its row.  A slot is one
 // atomic on the destination's count; beyond EV_CAP the event is dropped and
 // flag 2 stops the launch before the epoch that would read it.
-__device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t g0, int32_t g1) {
+__device__ __forceinline__ void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t g0, int32_t g1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     int32_t *cnt = a.bkCnt + (int64_t)par * a.S * BK_STRIDE;
@@ -420,7 +425,7 @@ __device__ inline void push_edges(const PersistArgs &a, Shared &s, int32_t par, 
 
 // Writes the events of the `ns` rows of the LDS push list into the buckets
 // for parity `par`.
-__device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t lpar = -1) {
+__device__ __forceinline__ void push_rows(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t lpar = -1) {
     const int32_t total = row_scan(a, s, ns, true, lpar);
     push_edges(a, s, par, ns, 0, total);
 }
@@ -429,7 +434,7 @@ __device__ inline void push_rows(const PersistArgs &a, Shared &s, int32_t par, i
 // one flattened edge list split evenly over the grid: block b writes edges
 // [b T / G, (b+1) T / G) into the buckets of parity par.  The owners counted
 // their events.  Every thread of every block calls it.
-__device__ void push_long_share(const PersistArgs &a, Shared &s, int32_t par, int32_t lpar) {
+__device__ __forceinline__ void push_long_share(const PersistArgs &a, Shared &s, int32_t par, int32_t lpar) {
     const int tid = threadIdx.x;
     const int32_t n = min(__hip_atomic_load(&a.lrCnt[lpar], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), LR_CAP);
     if (n == 0) return;   // the same value in every block after the grid barrier
@@ -448,7 +453,7 @@ __device__ void push_long_share(const PersistArgs &a, Shared &s, int32_t par, in
 }
 
 // Folds this block's work counters (row_scan) into the brain's.
-__device__ inline void publish_stats(const PersistArgs &a, Shared &s) {
+__device__ __forceinline__ void publish_stats(const PersistArgs &a, Shared &s) {
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s.ev) atomicAdd(&a.d.sc->edgeEvents, s.ev);
@@ -460,13 +465,16 @@ __device__ inline void publish_stats(const PersistArgs &a, Shared &s) {
 
 // ------------------------------------------------------------------ prologue: ring lists -> buckets
 
-__device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
+__device__ __forceinline__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t me = blockIdx.x;
-    int32_t acc = 0;
+    // the arrival steps' list lengths, loaded in parallel, then their prefix
+    if (tid < L0) s.pref[tid + 1] = d.ringCount[head_at(a, tid)];
+    __syncthreads();
     if (tid == 0) {
-        for (int32_t m = 0; m < L0; m++) { s.pref[m] = acc; acc += d.ringCount[head_at(a, m)]; }
+        int32_t acc = 0;
+        for (int32_t m = 0; m < L0; m++) { const int32_t c = s.pref[m + 1]; s.pref[m] = acc; acc += c; }
         s.pref[L0] = acc;
     }
     __syncthreads();
@@ -502,7 +510,7 @@ __device__ void prologue_scatter(const PersistArgs &a, Shared &s, int32_t L0) {
 // where each goes -- then each pair's run in key order (ranks within the
 // run, or sort_runs past GREG * PT events).  Afterwards evStart[r] is the
 // end of run r.
-__device__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
+__device__ __forceinline__ int32_t gather(const PersistArgs &a, Shared &s, int32_t par, int32_t me) {
     const int tid = threadIdx.x;
     // the count is read once, then cleared for the push two epochs on
     if (tid == 0) {
@@ -607,13 +615,15 @@ struct Cell {
     int32_t i;        // neuron index, -1 if the slot is empty
 };
 
+// No branch: an empty slot (i = -1) reads neuron 0 and is never used or
+// stored, so every cell's loads can be in flight at once.
 __device__ __forceinline__ void load_cell(const DevBrain &d, Cell &c, int32_t i) {
     c.i = i;
     c.count = 0; c.mask = 0; c.prev = 0; c.forced = 0;
-    if (i < 0) return;
-    c.st.v = d.v[i]; c.st.gE = d.gE[i]; c.st.gI = d.gI[i]; c.st.refr = d.refr[i];
-    c.st.trace = d.trace[i]; c.st.adapt = d.adapt[i]; c.st.res = d.res[i];
-    c.bias = d.bias[i]; c.thr = d.thr[i]; c.sign = d.g.sign[i];
+    const int32_t j = i < 0 ? 0 : i;
+    c.st.v = d.v[j]; c.st.gE = d.gE[j]; c.st.gI = d.gI[j]; c.st.refr = d.refr[j];
+    c.st.trace = d.trace[j]; c.st.adapt = d.adapt[j]; c.st.res = d.res[j];
+    c.bias = d.bias[j]; c.thr = d.thr[j]; c.sign = d.g.sign[j];
 }
 
 __device__ __forceinline__ void store_cell(const DevBrain &d, Cell &c) {
@@ -641,7 +651,7 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
 
 // ------------------------------------------------------------------ simulate
 
-__device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t nev, int32_t par,
+__device__ __forceinline__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t nev, int32_t par,
                             int32_t L, int32_t lo) {
     const DevBrain &d = a.d;
     const int lane = threadIdx.x & 63;
@@ -672,7 +682,7 @@ __device__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS
     }
 }
 
-__device__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
+__device__ __forceinline__ void sim_driven(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
                            int32_t nev, int32_t par, int32_t L, uint32_t &rng) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
@@ -824,7 +834,7 @@ __device__ __forceinline__ uint32_t driven_flags(const Cell (&cells)[CELLS]) {
 // `off`: this block's flags and offsets for step t0, computed and published
 // at the end of the previous step (or before the first epoch), and left for
 // the step after the epoch.
-__device__ void sim_driven_x(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
+__device__ __forceinline__ void sim_driven_x(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], const float (&pFire)[DPT],
                              int32_t nev, int32_t par, int32_t L, int32_t t0, uint32_t &rng, uint32_t &nr,
                              int32_t &off, bool &dead) {
     const DevBrain &d = a.d;
@@ -928,7 +938,7 @@ __device__ void sim_driven_x(const PersistArgs &a, Shared &s, Cell (&cells)[CELL
 
 // A regular block's spikes of the epoch -> the LDS push list (s.spkI /
 // spkK / spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
-__device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
+__device__ __forceinline__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     if (tid == 0) s.nspk = 0;
@@ -958,7 +968,7 @@ __device__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)
 // A regular block pushes its own spikes.
 // A regular block pushes its own spikes (buckets of parity par); rows longer
 // than LONG_ROW go to the shared list of parity lpar.
-__device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t lpar) {
+__device__ __forceinline__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t lpar) {
     const int32_t ns = collect_spikes(a, s, cells);
     push_rows(a, s, par, ns, lpar);
     publish_stats(a, s);
@@ -970,7 +980,7 @@ __device__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS
 // barrier that ends the epoch's simulation, a share block reads the spike
 // steps of its neurons (dmask: forced, tmask: threshold, as the driven block
 // wrote them) and their res, and writes their rows' events.
-__device__ void share_setup(const PersistArgs &a, Shared &s) {
+__device__ __forceinline__ void share_setup(const PersistArgs &a, Shared &s) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t G = min((int32_t)gridDim.x, SHARE_BLOCKS), b = blockIdx.x;
@@ -989,7 +999,7 @@ __device__ void share_setup(const PersistArgs &a, Shared &s) {
 
 // `par`: the parity of the buckets written (the next epoch's); `mpar`: the
 // parity of the driven masks read (this epoch's).
-__device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, int32_t mpar) {
+__device__ __forceinline__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, int32_t mpar) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const bool timed = tid == 0 && (a.flags & 1);
@@ -1067,7 +1077,7 @@ __device__ void push_driven_share(const PersistArgs &a, Shared &s, int32_t par, 
 
 // ------------------------------------------------------------------ lists: one step's spikes in lif.c order
 
-__device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t t, int32_t m) {
+__device__ __forceinline__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t t, int32_t m) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     const int32_t nD = d.nDriven;
@@ -1117,7 +1127,7 @@ __device__ void build_list(const PersistArgs &a, Shared &s, int32_t par, int32_t
 
 // lists of epoch steps [t0, t0+L), spread over the spare blocks (or the
 // regular blocks when the grid has no spare ones)
-__device__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_t t0, int32_t L) {
+__device__ __forceinline__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_t t0, int32_t L) {
     const int32_t first = a.nSpare > 0 ? a.S : a.G;
     const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - a.G);
     const int32_t b = (int32_t)blockIdx.x - first;
@@ -1127,6 +1137,51 @@ __device__ void build_lists(const PersistArgs &a, Shared &s, int32_t par, int32_
         // the slot delivered at the epoch's last step is empty until a later list fills it
         a.d.ringCount[head_at(a, t0 + L - 1)] = 0;
     }
+}
+
+// ------------------------------------------------------------------ the driven neurons' state
+
+// Fields of the driven stage: the cell state, then bias, threshold, sign and
+// drive, then the neuron index (as bits).
+constexpr int DSTAGE_F = 12;
+
+// Every block but the driven ones copies a share of the driven neurons' state
+// into the stage in driven-list order (block b of nb: positions b*PT + tid,
+// b*PT + tid + nb*PT, ...).  Read by one workgroup, those scattered lines
+// take ~20 us at a launch's start; spread over the grid they are one round
+// trip, and the driven blocks read the stage coalesced after the barrier.
+__device__ __forceinline__ void stage_driven(const PersistArgs &a, int32_t b, int32_t nb) {
+    const DevBrain &d = a.d;
+    float *st = a.dstage;
+    for (int32_t q = b * PT + (int32_t)threadIdx.x; q < d.nDriven; q += nb * PT) {
+        const int32_t i = d.driven[q];
+        st[0 * DRV_CAP + q] = d.v[i];
+        st[1 * DRV_CAP + q] = d.gE[i];
+        st[2 * DRV_CAP + q] = d.gI[i];
+        st[3 * DRV_CAP + q] = d.refr[i];
+        st[4 * DRV_CAP + q] = d.trace[i];
+        st[5 * DRV_CAP + q] = d.adapt[i];
+        st[6 * DRV_CAP + q] = d.res[i];
+        st[7 * DRV_CAP + q] = d.bias[i];
+        st[8 * DRV_CAP + q] = d.thr[i];
+        st[9 * DRV_CAP + q] = d.g.sign[i];
+        st[10 * DRV_CAP + q] = d.drive[i];
+        st[11 * DRV_CAP + q] = __int_as_float(i);
+    }
+}
+
+// A driven cell from the stage (q = driven-list position, -1 for an empty slot).
+__device__ __forceinline__ void load_staged_cell(const PersistArgs &a, Cell &c, float &pFire, int32_t q) {
+    const float *st = a.dstage;
+    const int32_t j = q < 0 ? 0 : q;
+    c.count = 0; c.mask = 0; c.prev = 0; c.forced = 0;
+    c.st.v = st[0 * DRV_CAP + j]; c.st.gE = st[1 * DRV_CAP + j]; c.st.gI = st[2 * DRV_CAP + j];
+    c.st.refr = st[3 * DRV_CAP + j]; c.st.trace = st[4 * DRV_CAP + j]; c.st.adapt = st[5 * DRV_CAP + j];
+    c.st.res = st[6 * DRV_CAP + j];
+    c.bias = st[7 * DRV_CAP + j]; c.thr = st[8 * DRV_CAP + j]; c.sign = st[9 * DRV_CAP + j];
+    const float dr = st[10 * DRV_CAP + j];
+    c.i = q < 0 ? -1 : __float_as_int(st[11 * DRV_CAP + j]);
+    pFire = q < 0 ? 0.f : dr * a.d.k.dtS;
 }
 
 // ------------------------------------------------------------------ the kernel
@@ -1165,31 +1220,16 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     int32_t xOff = 0;
     bool xDead = false;
     if (drivenBlock) {
-        const int32_t qb = me * a.Q;
-        const int32_t nDk = min(a.Q, d.nDriven - qb);
-        const int32_t P = (nDk + PT - 1) / PT;
-#pragma unroll
-        for (int u = 0; u < DPT; u++) {
-            const int32_t q = tid * P + u;
-            const bool has = u < P && q < nDk;
-            load_cell(d, cells[u], has ? d.driven[qb + q] : -1);
-            pFire[u] = has ? d.drive[cells[u].i] * d.k.dtS : 0.f;
-        }
+        // the driven cells come from the stage the other blocks fill (after the first barrier)
         leap_load<PT, LEAP_PERSIST_MATS>(s.leap, d.leap);
         rng = d.sc->rng;
-        if (a.G > 1) {
-            // the first step's count, published before anything can wait for it
-            xNr = driven_flags(cells);
-            int32_t n;
-            xOff = wg_scan_flags<PT, DPT>(xNr, s.scan + 32, &n);
-            if (tid == 0) xc_publish(a, me, 0, n);
-        }
     } else {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) {
             const int32_t i = lo + tid + u * PT;
             load_cell(d, cells[u], (u < NPT && owning && i < hi && a.drvPos[i] < 0) ? i : -1);
         }
+        stage_driven(a, me - a.G, (int32_t)gridDim.x - a.G);
     }
     share_setup(a, s);
     __syncthreads();
@@ -1199,6 +1239,23 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     FB_PHASE(5)
     if (!grid_barrier(a.bar, &a.ctl[1])) return;
     FB_PHASE(4)
+    if (drivenBlock) {
+        const int32_t qb = me * a.Q;
+        const int32_t nDk = min(a.Q, d.nDriven - qb);
+        const int32_t P = (nDk + PT - 1) / PT;
+#pragma unroll
+        for (int u = 0; u < DPT; u++) {
+            const int32_t q = tid * P + u;
+            load_staged_cell(a, cells[u], pFire[u], u < P && q < nDk ? qb + q : -1);
+        }
+        if (a.G > 1) {
+            // the first step's count
+            xNr = driven_flags(cells);
+            int32_t n;
+            xOff = wg_scan_flags<PT, DPT>(xNr, s.scan + 32, &n);
+            if (tid == 0) xc_publish(a, me, 0, n);
+        }
+    }
     if (d.k.depU != 0.f && owning) {
 #pragma unroll
         for (int u = 0; u < CELLS; u++)
@@ -1285,6 +1342,41 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         atomicMax((unsigned long long *)&a.prof[PROF_SHARE + 3], (unsigned long long)(s.share[0] + s.share[1] + s.share[2]));
     }
     if (me == 0 && tid == 0) a.ctl[0] = done;
+    // The last block out hands the results to the host (pinned memory, so
+    // the host needs no copy after the launch) and leaves the counters a
+    // launch needs at zero -- barrier counters, exchange words, long-row
+    // lists, flags, clocks -- so that the next launch needs no clearing
+    // kernel.  (After a launch without overflow the bucket counts are zero
+    // already: every gather clears the counts it reads, and the last epoch
+    // pushes nothing; after an overflow the host clears them.)  A block that
+    // left early after a barrier timeout never arrives here, and the host
+    // then reads the device words and clears them itself.
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        s.last = atomicAdd(&a.ctl[2], 1) == (int32_t)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s.last) {
+        __threadfence();
+        for (int32_t k = tid; k < GBAR_WORDS; k += PT) a.bar[k] = 0;
+        if (tid < DRV_BLOCKS * XC_RING) a.xc[tid] = 0;
+        if (tid < 2) a.lrCnt[tid] = 0;
+        if (tid < PERSIST_PROF) {
+            unsigned long long *pk = (unsigned long long *)&a.prof[tid];
+            a.hostProf[tid] = __hip_atomic_load(pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a.prof[tid] = 0;
+        }
+        if (tid == 0) {
+            a.hostCtl[0] = __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a.hostCtl[1] = __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a.ctl[1] = 0;
+            a.ctl[2] = 0;
+        }
+        __syncthreads();
+        __threadfence_system();
+        if (tid == 0) __hip_atomic_store(&a.hostCtl[2], a.tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -1447,6 +1539,11 @@ struct PersistBuffers {
     uint32_t *lrK = nullptr;
     float *lrR = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
+    float *dstage = nullptr;
+    int32_t *hostCtl = nullptr;      // pinned: the last block's results (PersistArgs::hostCtl)
+    uint64_t *hostProf = nullptr;
+    int32_t tag = 0;                 // launches so far
+    bool needClear = true;           // the counters a launch needs zero are not known to be
     int32_t N = 0, S = 0;
     int64_t E = 0;
     int device = -1;
@@ -1463,9 +1560,14 @@ static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain
 static void free_buffers(PersistBuffers &pb) {
     void *ps[] = {pb.spk, pb.dmask, pb.tmask, pb.drvPos, pb.arrMask, pb.bkKey, pb.bkAdd, pb.bkCnt,
                   pb.ctl, pb.prof, pb.eown, pb.dres, pb.bar, pb.xc, pb.bstart, pb.chunkBlk,
-                  pb.lrCnt, pb.lrI, pb.lrK, pb.lrR};
+                  pb.lrCnt, pb.lrI, pb.lrK, pb.lrR, pb.dstage};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    if (pb.hostCtl) (void)hipHostFree(pb.hostCtl);
+    if (pb.hostProf) (void)hipHostFree(pb.hostProf);
+    pb.hostCtl = nullptr;
+    pb.hostProf = nullptr;
+    pb.needClear = true;
     pb.spk = nullptr; pb.dmask = pb.tmask = nullptr; pb.drvPos = nullptr; pb.arrMask = nullptr;
     pb.bkKey = nullptr; pb.bkAdd = nullptr; pb.bkCnt = nullptr; pb.ctl = nullptr; pb.prof = nullptr;
     pb.eown = nullptr;
@@ -1475,6 +1577,7 @@ static void free_buffers(PersistBuffers &pb) {
     pb.lrCnt = pb.lrI = nullptr;
     pb.lrK = nullptr;
     pb.lrR = nullptr;
+    pb.dstage = nullptr;
     pb.bstart = nullptr;
     pb.chunkBlk = nullptr;
     pb.starts.clear();
@@ -1587,6 +1690,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.bkAdd, sizeof(float) * 2 * (size_t)lay.S * EV_CAP));
         FB_HIP(hipMalloc(&pb.bkCnt, sizeof(int32_t) * 2 * (size_t)lay.S * BK_STRIDE));
         FB_HIP(hipMalloc(&pb.ctl, sizeof(int32_t) * 4));
+        FB_HIP(hipHostMalloc((void **)&pb.hostCtl, sizeof(int32_t) * 4, hipHostMallocDefault));
+        FB_HIP(hipHostMalloc((void **)&pb.hostProf, sizeof(uint64_t) * PERSIST_PROF, hipHostMallocDefault));
+        pb.hostCtl[2] = 0;
+        pb.needClear = true;
         FB_HIP(hipMalloc(&pb.prof, sizeof(uint64_t) * PERSIST_PROF));
         FB_HIP(hipMalloc(&pb.eown, sizeof(uint32_t) * (size_t)std::max<int64_t>(d.g.E, 1)));
         FB_HIP(hipMalloc(&pb.dres, sizeof(float) * 2 * DRV_CAP));
@@ -1594,6 +1701,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.xc, sizeof(uint64_t) * DRV_BLOCKS * XC_RING));
         FB_HIP(hipMalloc(&pb.bstart, sizeof(int32_t) * (S_MAX + 1)));
         FB_HIP(hipMalloc(&pb.lrCnt, sizeof(int32_t) * 2));
+        FB_HIP(hipMalloc(&pb.dstage, sizeof(float) * DSTAGE_F * DRV_CAP));
         FB_HIP(hipMalloc(&pb.lrI, sizeof(int32_t) * 2 * LR_CAP));
         FB_HIP(hipMalloc(&pb.lrK, sizeof(uint32_t) * 2 * LR_CAP));
         FB_HIP(hipMalloc(&pb.lrR, sizeof(float) * 2 * LR_CAP));
@@ -1660,13 +1768,21 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
     a.flags = profileSteps ? 1 : 0;
 
+    a.hostCtl = pb.hostCtl;
+    a.dstage = pb.dstage;
+    a.hostProf = pb.hostProf;
+
     int32_t done = 0, cur = head;
     while (done < nsteps) {
         a.head0 = cur;
         a.nsteps = nsteps - done;
-        hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.lrCnt,
-                           pb.bkCnt, 2 * lay.S * BK_STRIDE);
-        FB_HIP(hipGetLastError());
+        a.tag = ++pb.tag;
+        if (pb.needClear) {
+            hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.lrCnt,
+                               pb.bkCnt, 2 * lay.S * BK_STRIDE);
+            FB_HIP(hipGetLastError());
+            pb.needClear = false;
+        }
         void *args[] = {&a};
         FB_HIP(hipEventRecord(pb.ev0, st));
         const hipError_t le = hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
@@ -1681,11 +1797,20 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
+        FB_HIP(hipStreamSynchronize(st));
         int32_t ctl[4];
         uint64_t prof[PERSIST_PROF];
-        FB_HIP(hipMemcpyAsync(ctl, pb.ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
-        if (phaseUs) FB_HIP(hipMemcpyAsync(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost, st));
-        FB_HIP(hipStreamSynchronize(st));
+        if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) {
+            // the last block out wrote the results and cleared the counters
+            ctl[0] = pb.hostCtl[0];
+            ctl[1] = pb.hostCtl[1];
+            memcpy(prof, pb.hostProf, sizeof prof);
+        } else {
+            // a block left early: read the device words, and clear before the next launch
+            FB_HIP(hipMemcpy(ctl, pb.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+            FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
+            pb.needClear = true;
+        }
         float ms = 0.f;
         FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
         if (kernelMs) *kernelMs += ms;
@@ -1699,6 +1824,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
         done += ctl[0];
         pb.overflowSeen |= (uint32_t)ctl[1];
+        // a stop can leave bucket counts behind (an overflow in the prologue's
+        // push skips every gather): clear before the next launch
+        if (ctl[1]) pb.needClear = true;
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {
             // a bounded buffer overflowed: the next epoch runs on the per-step engine
