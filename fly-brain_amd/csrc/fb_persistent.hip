@@ -1797,7 +1797,23 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
-        FB_HIP(hipStreamSynchronize(st));
+        // Wait by polling: the last block out writes the tag to pinned memory
+        // as its final act, and the end event follows; a blocking stream
+        // synchronisation takes ~16 us longer to return.  A kernel that ends
+        // without the tag (a block left early) is caught by the stream query.
+        for (uint32_t spin = 1;; spin++) {
+            if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) break;
+            if ((spin & 1023u) == 0) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) FB_HIP(q);
+            }
+        }
+        for (;;) {
+            const hipError_t q = hipEventQuery(pb.ev1);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) FB_HIP(q);
+        }
         int32_t ctl[4];
         uint64_t prof[PERSIST_PROF];
         if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) {
