@@ -861,8 +861,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
     ap.add_argument("--adjoint-batch-cpu-steps", type=int, default=400)
-    ap.add_argument("--only", choices=["headline", "flyvis", "visual", "cordens", "lanes", "brain"],
-                    help="run one leg alone and print its JSON (headline: the steady-state forward only)")
+    ap.add_argument("--only", choices=["headline", "flyvis", "visual", "cordens", "lanes", "brain", "adjoint"],
+                    help="run one leg alone and print its JSON (headline: the steady-state forward only; "
+                         "adjoint: the full-scale forward + adjoint)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -899,6 +900,11 @@ def main():
         return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
+    if args.only == "adjoint":
+        st = hiprt.Stream()
+        print(json.dumps({"fwd_adjoint": bench_adjoint(args, synth, P, diff, hiprt, st)}), flush=True)
+        st.destroy()
+        return
     t_build = time.perf_counter()
     g = synth.config2_graph()
     drv = synth.driven_set(g.N, 0.01, 4)

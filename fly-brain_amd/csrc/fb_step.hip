@@ -23,9 +23,10 @@
 //               add; several: the ordered sum), background hits, the membrane
 //               and decay update, the threshold test, and wave-ballot
 //               compaction of the spikes into a per-block staging list.
-//   k_compact   block prefix over the staged counts: the ring slot gets the
-//               forced spikes first, then threshold spikes in ascending index,
-//               which is lif.c's output order.
+//   k_compact   block prefix over the staged counts (only blocks that staged
+//               spikes, and the last block, compute one): the ring slot gets
+//               the forced spikes first, then threshold spikes in ascending
+//               index, which is lif.c's output order.
 #include "fb_internal.h"
 
 namespace fb {
@@ -411,6 +412,11 @@ __global__ void __launch_bounds__(256) k_membrane(DevBrain d) {
 __global__ void __launch_bounds__(256) k_compact(DevBrain d, int32_t slot, int32_t head) {
     __shared__ int32_t red[256];
     const int32_t b = blockIdx.x;
+    const int32_t nb = d.blkCnt[b];
+    // a block without spikes has nothing to place; only the last one also
+    // totals the step.  So the prefix sums cost (spiking blocks + 1) x
+    // blocks reads, not blocks x blocks.
+    if (nb == 0 && b != (int32_t)gridDim.x - 1) return;
     int32_t sum = 0;
     for (int32_t j = threadIdx.x; j < b; j += 256) sum += d.blkCnt[j];
     red[threadIdx.x] = sum;
@@ -421,7 +427,6 @@ __global__ void __launch_bounds__(256) k_compact(DevBrain d, int32_t slot, int32
     }
     const int32_t before = red[0];
     const int32_t nd = d.sc->nDrivenFired;
-    const int32_t nb = d.blkCnt[b];
     int32_t *out = d.ring + (int64_t)slot * d.ringStride;
     for (int32_t t = threadIdx.x; t < nb; t += 256) out[nd + before + t] = d.stage[b * 256 + t];
     if (b == (int32_t)gridDim.x - 1 && threadIdx.x == 0) {
