@@ -37,6 +37,7 @@ uint32_t persistent_overflow_flags(const DevBrain &d);
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
                             int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion);
 void persistent_release(const int32_t *key);
+void persistent_take_time(const int32_t *key, double *kernelMs);
 
 // ------------------------------------------------------------------ jump tables
 
@@ -976,7 +977,10 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
     std::string why;
     if (n == "fallback_steps") *value = (double)b->fallbackSteps;
     else if (n == "persistent_launches") *value = (double)b->persistentLaunches;
-    else if (n == "persistent_kernel_ms") *value = b->persistentMs;
+    else if (n == "persistent_kernel_ms") {
+        persistent_take_time(b->d.driven, &b->persistentMs);
+        *value = b->persistentMs;
+    }
     else if (n == "persistent_overflow_flags") *value = (double)persistent_overflow_flags(b->d);
     else if (n.rfind("phase_us_", 0) == 0) {
         // phase_us_<drv|reg|spare>_<phase>: time the driven block, the first
@@ -995,6 +999,12 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
         static const char *sh[4] = {"load", "edges", "publish", "max"};
         for (int k = 0; k < 4; k++)
             if (n == std::string("phase_us_share_") + sh[k]) idx = PROF_SHARE + k;
+        // the launch's span: spread of the block starts, the first start to the
+        // last block's end, and the last block out's hand-over to the host
+        // (and the first start to the last end of a driven, regular, spare block)
+        static const char *sp[7] = {"start_spread", "body", "handover", "", "end_drv", "end_reg", "end_spare"};
+        for (int k = 0; k < 7; k++)
+            if (k != 3 && n == std::string("phase_us_span_") + sp[k]) idx = PROF_SPAN + k;
         if (idx < 0) { set_error("fb_brain_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
         *value = b->phaseUs[idx];
     }

@@ -771,6 +771,7 @@ bool batch_la_supported(const BatchDev &d, std::string *why) {
 }
 
 void batch_la_release(const void *key) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
     auto it = g_labuf.find(key);
     if (it == g_labuf.end()) return;
     LaBuffers &b = it->second;
@@ -788,6 +789,7 @@ void batch_la_release(const void *key) {
 fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, int32_t epochs,
                           uint64_t drivenVersion, uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done,
                           double *kernelMs) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
     const int32_t D = d.nslots - 1;
     const int32_t rnb = batch_la_shape_neurons(k);
     const int32_t R = (d.N + rnb - 1) / rnb, G = (d.B + MPG - 1) / MPG, S = R + G;

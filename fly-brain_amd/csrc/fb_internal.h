@@ -18,6 +18,7 @@
 // in-degree prefix inOff u32[N+1].
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <stdint.h>
 #include <string>
 #include <vector>
@@ -163,6 +164,15 @@ constexpr int LEAP_STRIDE = 129;
 // wait, one agent-scope acquire fence per workgroup.  The wait is bounded: after
 // GBAR_TIMEOUT wall-clock ticks it sets bit 8 in *flags and returns false, and
 // the caller leaves the kernel.
+// Kernels with grid barriers need every workgroup resident at once.  Their
+// grids hold at most one workgroup per CU at the kernel's occupancy, and the
+// library runs one of them at a time per process: this lock is held from a
+// launch until the host has seen the kernel end (the lookahead engine and the
+// ensemble lookahead engine).  Kernels of other streams are finite, so their
+// CUs free up; and a barrier wait is bounded, so a grid that cannot become
+// resident (another process's grid-barrier kernel) ends in an error.
+std::mutex &grid_launch_mutex();
+
 constexpr int GBAR_GROUP = 16;
 constexpr int GBAR_MAX_BLOCKS = 1024;
 constexpr int GBAR_WORDS = 64 * (GBAR_MAX_BLOCKS / GBAR_GROUP + 1);
@@ -201,11 +211,16 @@ __device__ __forceinline__ bool grid_barrier(int32_t *ctr, int32_t *flags) {
 // Lookahead-kernel clocks reported per launch: the NPH phases of the driven
 // block, of the first regular block and of the first spare block; the parts of
 // the driven step (6); the shared push of the driven rows (mean load / edges /
-// publish over blocks, then the slowest block's total).
+// publish over blocks, then the slowest block's total); the launch's span
+// (7 clocks: ~first block start, last block start, last block end, the last
+// block out's final write, the last end of a driven, a regular and a spare
+// block; the first is stored complemented so that every slot is an atomic
+// maximum over a zeroed word).
 constexpr int NPH = 8;   // gather, sim, push, lists, barrier, prologue, share, barrier after the share
 constexpr int PROF_SUB = 3 * NPH;
 constexpr int PROF_SHARE = PROF_SUB + 6;
-constexpr int PERSIST_PROF = PROF_SHARE + 4;
+constexpr int PROF_SPAN = PROF_SHARE + 4;
+constexpr int PERSIST_PROF = PROF_SPAN + 7;
 constexpr int LEAP_LDS_WORDS = LEAP_MATS * LEAP_STRIDE;
 
 __device__ __forceinline__ uint32_t matvec(const uint32_t *tab, uint32_t x) {

@@ -124,7 +124,8 @@ struct PersistArgs {
     uint64_t *bkKey;     // [2][S][EV_CAP] each destination's bucket of the next epoch's events
     float *bkAdd;
     int32_t *bkCnt;      // [2][S] slots taken, one per BK_STRIDE words (zero at launch; a destination clears its own after reading)
-    int32_t *ctl;        // [0] steps done, [1] overflow flags, [2] blocks that have left (zero at launch)
+    int32_t *ctl;        // [0] steps done, [1] overflow flags, [2] blocks that have left, [3] driven blocks
+                         // whose final state is in the stage (zero at launch)
     int32_t *hostCtl;    // pinned host memory: [0] steps done, [1] flags, [2] tag, from the last block out
     uint64_t *hostProf;  // pinned host memory: the phase clocks, from the last block out
     int32_t tag;         // this launch's number
@@ -649,6 +650,25 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
     cell_inputs_has(k, c, s, nev, m, ql, (s.hasEv[m * WPS + (ql >> 5)] >> (ql & 31)) & 1u);
 }
 
+// A driven block's final cell state -> the stage (coalesced) at the launch's
+// end; the list-building blocks then write a share of it each to the brain's
+// arrays (unstage_driven).  One workgroup storing ~1,650 scattered neurons'
+// eight fields took ~25 us at a launch's end.
+__device__ __forceinline__ void stage_out(const PersistArgs &a, const Cell (&cells)[CELLS], int32_t qb, int32_t P) {
+    float *st = a.dstage;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < DPT; u++) {
+        const Cell &c = cells[u];
+        if (c.i < 0) continue;
+        const int32_t q = qb + tid * P + u;
+        st[0 * DRV_CAP + q] = c.st.v; st[1 * DRV_CAP + q] = c.st.gE; st[2 * DRV_CAP + q] = c.st.gI;
+        st[3 * DRV_CAP + q] = c.st.refr; st[4 * DRV_CAP + q] = c.st.trace; st[5 * DRV_CAP + q] = c.st.adapt;
+        st[6 * DRV_CAP + q] = c.st.res;
+        st[12 * DRV_CAP + q] = __uint_as_float(c.count);
+    }
+}
+
 // ------------------------------------------------------------------ simulate
 
 __device__ __forceinline__ void sim_regular(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t nev, int32_t par,
@@ -1142,8 +1162,10 @@ __device__ __forceinline__ void build_lists(const PersistArgs &a, Shared &s, int
 // ------------------------------------------------------------------ the driven neurons' state
 
 // Fields of the driven stage: the cell state, then bias, threshold, sign and
-// drive, then the neuron index (as bits).
-constexpr int DSTAGE_F = 12;
+// drive, then the neuron index (as bits); at the launch's end the driven
+// blocks put their cells' final state in fields 0-6 and the spike-count
+// increment (as bits) in field 12.
+constexpr int DSTAGE_F = 13;
 
 // Every block but the driven ones copies a share of the driven neurons' state
 // into the stage in driven-list order (block b of nb: positions b*PT + tid,
@@ -1184,6 +1206,20 @@ __device__ __forceinline__ void load_staged_cell(const PersistArgs &a, Cell &c, 
     pFire = q < 0 ? 0.f : dr * a.d.k.dtS;
 }
 
+// A share of the stage (stage_out) -> the brain's arrays: block b of nb
+// writes positions b*PT + tid, b*PT + tid + nb*PT, ... (as stage_driven).
+__device__ __forceinline__ void unstage_driven(const PersistArgs &a, int32_t b, int32_t nb) {
+    const DevBrain &d = a.d;
+    const float *st = a.dstage;
+    for (int32_t q = b * PT + (int32_t)threadIdx.x; q < d.nDriven; q += nb * PT) {
+        const int32_t i = __float_as_int(st[11 * DRV_CAP + q]);
+        d.v[i] = st[0 * DRV_CAP + q]; d.gE[i] = st[1 * DRV_CAP + q]; d.gI[i] = st[2 * DRV_CAP + q];
+        d.refr[i] = st[3 * DRV_CAP + q]; d.trace[i] = st[4 * DRV_CAP + q]; d.adapt[i] = st[5 * DRV_CAP + q];
+        d.res[i] = st[6 * DRV_CAP + q];
+        d.spikeCount[i] += __float_as_uint(st[12 * DRV_CAP + q]);
+    }
+}
+
 // ------------------------------------------------------------------ the kernel
 
 __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
@@ -1199,6 +1235,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     // clocks of the first driven block, the first regular block and the first spare block
     const int32_t timedSlot = me == 0 ? 0 : me == a.G ? 1 : (a.nSpare > 0 && me == a.S) ? 2 : -1;
     const bool timed = tid == 0 && timedSlot >= 0;
+    const uint64_t tStart = wall_clock64();
     if (timed) {
         for (int k = 0; k < NPH; k++) s.ph[k] = 0;
         for (int k = 0; k < 6; k++) s.sub[k] = 0;
@@ -1312,12 +1349,45 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             e++;
         }
     }
-    // the last epoch's lists, and the state back to memory
-    if (e > 0 && !listsBuilt) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
+    // the last epoch's lists, and the state back to memory.  After a complete
+    // run the driven blocks hand their cells over through the stage: each
+    // writes it and counts itself in ctl[3]; the list-building blocks wait
+    // for all G and write a share each.  After a stop they store their cells.
+    const bool staged = e > 0 && !listsBuilt;
+    if (staged) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
     FB_PHASE(3)
-    if (owning) {
+    if (drivenBlock && staged) {
+        stage_out(a, cells, me * a.Q, (min(a.Q, d.nDriven - me * a.Q) + PT - 1) / PT);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            atomicAdd(&a.ctl[3], 1);
+        }
+    } else if (owning) {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) store_cell(d, cells[u]);
+    }
+    if (staged) {
+        const int32_t first = a.nSpare > 0 ? a.S : a.G;
+        const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - a.G);
+        if (me >= first && me < first + count) {
+            if (tid == 0) {
+                const uint64_t t0 = wall_clock64();
+                while (__hip_atomic_load(&a.ctl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.G) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > GBAR_TIMEOUT) {
+                        atomicOr(&a.ctl[1], GBAR_TIMED_OUT);
+                        break;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();
+            unstage_driven(a, me - first, count);
+        }
+    }
+    if (owning) {
         if (me == 0) {
             if (a.G > 1 && done > 0) {
                 // the stream after the last step: its draws over every driven block
@@ -1353,6 +1423,12 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     // then reads the device words and clears them itself.
     __syncthreads();
     if (tid == 0) {
+        unsigned long long *span = (unsigned long long *)&a.prof[PROF_SPAN];
+        atomicMax(&span[0], (unsigned long long)~tStart);
+        atomicMax(&span[1], (unsigned long long)tStart);
+        const unsigned long long tEnd = wall_clock64();
+        atomicMax(&span[2], tEnd);
+        atomicMax(&span[drivenBlock ? 4 : owning ? 5 : 6], tEnd);
         __threadfence();
         s.last = atomicAdd(&a.ctl[2], 1) == (int32_t)gridDim.x - 1;
     }
@@ -1364,7 +1440,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         if (tid < 2) a.lrCnt[tid] = 0;
         if (tid < PERSIST_PROF) {
             unsigned long long *pk = (unsigned long long *)&a.prof[tid];
-            a.hostProf[tid] = __hip_atomic_load(pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a.hostProf[tid] = tid == PROF_SPAN + 3 ? (uint64_t)wall_clock64()
+                                                   : __hip_atomic_load(pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a.prof[tid] = 0;
         }
         if (tid == 0) {
@@ -1372,6 +1449,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             a.hostCtl[1] = __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a.ctl[1] = 0;
             a.ctl[2] = 0;
+            a.ctl[3] = 0;
         }
         __syncthreads();
         __threadfence_system();
@@ -1396,10 +1474,6 @@ static fb_status persist_init() {
     FB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_persist, PT, 0));
     hipDeviceProp_t prop;
     FB_HIP(hipGetDeviceProperties(&prop, dev));
-    if (!prop.cooperativeLaunch) {
-        set_error("device does not support cooperative launch");
-        return FB_ERR_UNSUPPORTED;
-    }
     g_ps.maxBlocks = perCU * prop.multiProcessorCount;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
@@ -1499,7 +1573,7 @@ static bool layout_for(const DevBrain &d, int32_t D, Layout &l) {
     return l.grid <= GBAR_MAX_BLOCKS;
 }
 
-fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why) {
+static fb_status supported_locked(const DevBrain &d, bool serialDeliver, std::string *why) {
     if (serialDeliver) { *why = "presynaptic inhibition or tRef <= 0 needs the serial deliver path"; return FB_ERR_UNSUPPORTED; }
     if (d.k.bgEvents != 0.f) { *why = "background events draw from the shared stream every step"; return FB_ERR_UNSUPPORTED; }
     if (d.poolOf) { *why = "the pool loop sets the drive after every step"; return FB_ERR_UNSUPPORTED; }
@@ -1513,12 +1587,18 @@ fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::strin
     if (KEY_THRESHOLD + (int64_t)d.N >= (1LL << 20)) { *why = "more than 1,040,383 neurons (arrival key field)"; return FB_ERR_UNSUPPORTED; }
     if (d.g.maxRow >= (1LL << 26)) { *why = "a row longer than 2^26 edges (row position field)"; return FB_ERR_UNSUPPORTED; }
     fb_status st = persist_init();
-    if (st) { *why = "cooperative launch unavailable"; return st; }
+    if (st) { *why = "the device's occupancy query failed"; return st; }
     Layout l;
     if (!layout_for(d, D, l)) { *why = "more than 1024 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
     const int32_t shareBlocks = std::min(l.grid, SHARE_BLOCKS);
     if ((d.nDriven + shareBlocks - 1) / shareBlocks > SHQ) { *why = "too many driven neurons for this grid"; return FB_ERR_UNSUPPORTED; }
     return FB_OK;
+}
+
+// (the layouts are cached in a map the launches also use)
+fb_status persistent_supported(const DevBrain &d, bool serialDeliver, std::string *why) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
+    return supported_locked(d, serialDeliver, why);
 }
 
 struct PersistBuffers {
@@ -1553,7 +1633,21 @@ struct PersistBuffers {
     int32_t *chunkBlk = nullptr;     // [N/64] regular block of each 64-neuron chunk
     uint64_t eownDriven = ~0ull;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timerPending = false;       // the last launch's ev0 .. ev1 time is not taken yet
 };
+
+// Adds the last launch's kernel time (ev0 .. ev1) to *kernelMs once its end
+// event is done (waiting for it).
+static hipError_t take_kernel_time(PersistBuffers &pb, double *kernelMs) {
+    if (!pb.timerPending) return hipSuccess;
+    pb.timerPending = false;
+    hipError_t e = hipEventSynchronize(pb.ev1);
+    if (e != hipSuccess) return e;
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, pb.ev0, pb.ev1);
+    if (e == hipSuccess && kernelMs) *kernelMs += ms;
+    return e;
+}
 
 static std::map<const int32_t *, PersistBuffers> g_pbuf;   // keyed by the brain's driven buffer
 
@@ -1594,11 +1688,19 @@ fb_status launch_step(const DevBrain &d, int32_t head, int32_t slot, bool serial
 static std::vector<PersistBuffers> g_pool;
 constexpr size_t POOL_MAX = 2;
 
+void persistent_take_time(const int32_t *key, double *kernelMs) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
+    auto it = g_pbuf.find(key);
+    if (it != g_pbuf.end()) (void)take_kernel_time(it->second, kernelMs);
+}
+
 void persistent_release(const int32_t *key) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
     auto it = g_pbuf.find(key);
     if (it == g_pbuf.end()) return;
     PersistBuffers pb = std::move(it->second);
     g_pbuf.erase(it);
+    pb.timerPending = false;
     if (pb.spk && g_pool.size() < POOL_MAX) {
         pb.eownDriven = ~0ull;   // the next brain's driven set is its own
         pb.overflowSeen = 0;
@@ -1648,10 +1750,16 @@ __global__ void k_edge_owner(uint32_t *eown, const uint2 *edges, int64_t E, cons
     }
 }
 
+std::mutex &grid_launch_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hipStream_t st, int64_t *fallbackSteps,
                             int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
     std::string why;
-    fb_status rc = persistent_supported(d, false, &why);
+    fb_status rc = supported_locked(d, false, &why);
     if (rc) { set_error("persistent engine: %s", why.c_str()); return rc; }
     const int32_t D = d.nslots - 1;
     Layout lay;
@@ -1784,23 +1892,29 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             pb.needClear = false;
         }
         void *args[] = {&a};
+        // the previous launch's kernel time, now that its end event is done
+        FB_HIP(take_kernel_time(pb, kernelMs));
         FB_HIP(hipEventRecord(pb.ev0, st));
-        const hipError_t le = hipLaunchCooperativeKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
+        // A plain launch: the grid is one workgroup per CU at the kernel's
+        // occupancy and holds grid_launch_mutex (see there), which is what a
+        // cooperative launch would guarantee; the cooperative launch path
+        // cost ~13 us more per launch.
+        const hipError_t le = hipLaunchKernel((const void *)k_persist, dim3(lay.grid), dim3(PT), args, 0, st);
         if (le != hipSuccess) {
-            // The device refused the cooperative launch (the runtime gives the
-            // resource behind grid-wide barriers to one process at a time).
             // Before any step ran this is reported as unsupported, so that the
-            // automatic engine choice can step with the hipGraph engine.
+            // automatic engine choice can step with the per-step engines.
             (void)hipGetLastError();
-            set_error("persistent engine: cooperative launch refused: %s", hipGetErrorString(le));
+            set_error("persistent engine: launch refused: %s", hipGetErrorString(le));
             return done == 0 ? FB_ERR_UNSUPPORTED : FB_ERR_DEVICE;
         }
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         // Wait by polling: the last block out writes the tag to pinned memory
-        // as its final act, and the end event follows; a blocking stream
-        // synchronisation takes ~16 us longer to return.  A kernel that ends
-        // without the tag (a block left early) is caught by the stream query.
+        // as its final act; a blocking stream synchronisation takes ~16 us
+        // longer to return.  A kernel that ends without the tag (a block left
+        // early) is caught by the stream query.  After the tag nothing of the
+        // launch is left to wait for: the end event's time is taken at the
+        // next launch or when the statistic is read (persistent_kernel_ms).
         for (uint32_t spin = 1;; spin++) {
             if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) break;
             if ((spin & 1023u) == 0) {
@@ -1809,11 +1923,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
                 if (q != hipErrorNotReady) FB_HIP(q);
             }
         }
-        for (;;) {
-            const hipError_t q = hipEventQuery(pb.ev1);
-            if (q == hipSuccess) break;
-            if (q != hipErrorNotReady) FB_HIP(q);
-        }
+        pb.timerPending = true;
         int32_t ctl[4];
         uint64_t prof[PERSIST_PROF];
         if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) {
@@ -1827,12 +1937,20 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             FB_HIP(hipMemcpy(prof, pb.prof, sizeof prof, hipMemcpyDeviceToHost));
             pb.needClear = true;
         }
-        float ms = 0.f;
-        FB_HIP(hipEventElapsedTime(&ms, pb.ev0, pb.ev1));
-        if (kernelMs) *kernelMs += ms;
         if (phaseUs) {
-            for (int k = 0; k < PERSIST_PROF; k++)
+            for (int k = 0; k < PROF_SPAN; k++)
                 phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
+            // the span: spread of the block starts, first start to last block
+            // end, last block end to the final write (a slot stays zero when a
+            // block left early)
+            const uint64_t t0 = ~prof[PROF_SPAN];
+            if (prof[PROF_SPAN] && prof[PROF_SPAN + 3]) {
+                phaseUs[PROF_SPAN] += (double)(prof[PROF_SPAN + 1] - t0) / g_ps.tickHz * 1e6;
+                phaseUs[PROF_SPAN + 1] += (double)(prof[PROF_SPAN + 2] - t0) / g_ps.tickHz * 1e6;
+                phaseUs[PROF_SPAN + 2] += (double)(prof[PROF_SPAN + 3] - prof[PROF_SPAN + 2]) / g_ps.tickHz * 1e6;
+                for (int k = 4; k < 7; k++)
+                    if (prof[PROF_SPAN + k]) phaseUs[PROF_SPAN + k] += (double)(prof[PROF_SPAN + k] - t0) / g_ps.tickHz * 1e6;
+            }
         }
         if (ctl[1] & GBAR_TIMED_OUT) {
             set_error("persistent engine: a grid barrier timed out; the brain state is undefined");
@@ -1863,6 +1981,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
 }
 
 uint32_t persistent_overflow_flags(const DevBrain &d) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
     auto it = g_pbuf.find(d.driven);
     return it == g_pbuf.end() ? 0u : it->second.overflowSeen;
 }
