@@ -105,6 +105,11 @@ struct DiffDev {
     const float *dPer;                 // [epochs][N] per-epoch spike weights, or null
     double *lossAcc;                   // [N] the per-epoch weighted loss of a forward pass
     const float *betaArr;              // [N] per-neuron surrogateBeta, or null (the scalar)
+    // the spike train (forward's spikeTrain): step t < trainSteps forces the
+    // neurons trainIdx[trainPtr[t] .. trainPtr[t + 1]) in that order
+    int32_t trainSteps;
+    const int64_t *trainPtr;
+    const int32_t *trainIdx;
     // exact32: the forward pass in lif.c's binary32 arithmetic -- every
     // scalar as lif.c reads it from its binary32 header, and writeGraph's
     // weights and signs baked from the inputs before each forward pass
@@ -383,6 +388,38 @@ __device__ __forceinline__ void forced_effects(const DiffDev &d, int32_t i) {
     }
 }
 
+// The spike train's step t (src/lifdiff.js:298-307), after the drive
+// pre-pass and by the same workgroup: each listed neuron, in list order, is
+// forced like a driven spike without a draw unless it is refractory by then
+// (drawn this step, listed before, or still refractory), and goes to the tape
+// after the step's driven spikes.  The train is a few neurons a step: thread
+// 0 walks it.
+__device__ __forceinline__ void dx_train(const DiffDev &d, int32_t t) {
+    if (t >= d.trainSteps) return;
+    __syncthreads();   // the pre-pass's refractory writes and the tape top
+    if (threadIdx.x != 0) return;
+    unsigned long long top = *d.top;
+    for (int64_t k = d.trainPtr[t]; k < d.trainPtr[t + 1]; k++) {
+        const int32_t i = d.trainIdx[k];
+        if (d.refr[i] > 0) continue;
+        forced_effects(d, i);
+        d.forced[i] = 1;
+        d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
+        if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + 1.0);
+        if (d.dPer && t >= d.lossFrom) d.lossAcc[i] += spike_weight(d, i, t);
+        const int32_t rk = d.readIdx[i];
+        if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = 1.f;
+        if ((int64_t)top < d.cap) {
+            d.idx[top] = (int32_t)((uint32_t)i | FORCED_BIT);
+            d.amp[top] = 1.f;
+        } else {
+            atomicOr(d.err, DERR_TAPE);
+        }
+        top++;
+    }
+    *d.top = top;
+}
+
 // The drive pre-pass (src/lifdiff.js:290-297): one draw per non-refractory
 // driven neuron in set order; forced spikes go to the tape in that order.
 __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
@@ -453,6 +490,7 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
         if (tid == 0) *d.top = s_top + (unsigned long long)totalFired;
     }
     if (tid == 0) *d.rng = base;
+    dx_train(d, t);
 }
 
 // The drive pre-pass with at most 2048 driven neurons (one workgroup of 1024
@@ -524,6 +562,7 @@ __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
         }
     }
     if (tid == 0) *d.top = s_top + (unsigned long long)nf;
+    dx_train(d, t);
 }
 
 // The drive pre-pass when replaying a segment: the forward put every forced
@@ -1085,6 +1124,7 @@ struct fb_diff {
     bool inScaleGiven = false;           // opts.inScale: in[FB_D_INSCALE_IN] taken verbatim
     float *betaAlloc = nullptr;          // per-neuron surrogateBeta (d.betaArr when set)
     float *x32Alloc = nullptr;           // exact32's W32 [E] and signW32 [N]
+    void *trainAlloc = nullptr;          // the spike train: trainPtr [steps + 1], trainIdx
 };
 
 #define FB_REQUIRE_D(cond, code, ...)    \
@@ -1281,6 +1321,7 @@ extern "C" fb_status fb_diff_destroy(fb_diff *f) {
     if (f->lossAccAlloc) (void)hipFree(f->lossAccAlloc);
     if (f->betaAlloc) (void)hipFree(f->betaAlloc);
     if (f->x32Alloc) (void)hipFree(f->x32Alloc);
+    if (f->trainAlloc) (void)hipFree(f->trainAlloc);
     (void)hipStreamDestroy(f->ownStream);
     if (f->replayStream) {
         (void)hipStreamDestroy(f->replayStream);
@@ -1470,6 +1511,39 @@ extern "C" fb_status fb_diff_set_loss_per_epoch(fb_diff *f, const float *dPer, i
     // the tape does not depend on the weights (backward(tape, dLdSpike,
     // {dLdSpikePerEpoch}) takes them after the forward pass); only the loss
     // the forward pass reported does
+    return FB_OK;
+}
+
+extern "C" fb_status fb_diff_set_spike_train(fb_diff *f, int32_t steps, const int64_t *ptr, const int32_t *idx) {
+    FB_REQUIRE_D(f && steps >= 0 && (steps == 0 || ptr), FB_ERR_ARG, "fb_diff_set_spike_train: bad arguments");
+    if (steps > 0) {
+        FB_REQUIRE_D(ptr[0] == 0, FB_ERR_ARG, "fb_diff_set_spike_train: ptr[0] must be 0");
+        for (int32_t t = 0; t < steps; t++)
+            FB_REQUIRE_D(ptr[t + 1] >= ptr[t], FB_ERR_ARG, "fb_diff_set_spike_train: ptr must not decrease (step %d)", t);
+        FB_REQUIRE_D(ptr[steps] == 0 || idx, FB_ERR_ARG, "fb_diff_set_spike_train: null idx");
+        for (int64_t k = 0; k < ptr[steps]; k++)
+            FB_REQUIRE_D(idx[k] >= 0 && idx[k] < f->N, FB_ERR_ARG, "fb_diff_set_spike_train: neuron %d out of range",
+                         idx[k]);
+    }
+    FB_HIP(hipStreamSynchronize(f->stream));
+    if (f->trainAlloc) FB_HIP(hipFree(f->trainAlloc));
+    f->trainAlloc = nullptr;
+    f->d.trainSteps = 0;
+    f->d.trainPtr = nullptr;
+    f->d.trainIdx = nullptr;
+    if (steps > 0) {
+        const int64_t nIdx = ptr[steps];
+        const size_t ptrBytes = sizeof(int64_t) * ((size_t)steps + 1);
+        FB_HIP(hipMalloc(&f->trainAlloc, ptrBytes + sizeof(int32_t) * (size_t)std::max<int64_t>(1, nIdx)));
+        char *base = (char *)f->trainAlloc;
+        FB_HIP(hipMemcpy(base, ptr, ptrBytes, hipMemcpyHostToDevice));
+        if (nIdx) FB_HIP(hipMemcpy(base + ptrBytes, idx, sizeof(int32_t) * (size_t)nIdx, hipMemcpyHostToDevice));
+        f->d.trainSteps = steps;
+        f->d.trainPtr = (const int64_t *)base;
+        f->d.trainIdx = (const int32_t *)(base + ptrBytes);
+    }
+    // a recorded tape keeps the train it ran with (its forced spikes are on
+    // the tape, and the replay applies them from there)
     return FB_OK;
 }
 

@@ -25,6 +25,18 @@ INPUTS = ("outScale", "logGain", "thrOffset", "sizeLog", "drive", "inScale", "su
 STATE = ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount", "inScale")
 
 
+def train_csr(train):
+    """A spike train (a sequence over steps of index lists, or None) as
+    (ptr int64 [steps + 1], idx int32); None gives an empty train."""
+    if train is None:
+        return np.zeros(1, np.int64), np.zeros(0, np.int32)
+    lists = [np.asarray(x if x is not None else [], dtype=np.int64).ravel() for x in train]
+    ptr = np.zeros(len(lists) + 1, np.int64)
+    if lists:
+        ptr[1:] = np.cumsum([x.size for x in lists])
+    idx = np.concatenate(lists).astype(np.int32) if lists and ptr[-1] else np.zeros(0, np.int32)
+    return ptr, idx
+
 class LIFDiffHip:
     def __init__(self, N, indptr, indices, counts, sign, opts: dict | None = None):
         self.lib = _l.load()
@@ -167,7 +179,17 @@ class LIFDiffHip:
         _l.check(self.lib.fb_diff_set_stream(self.h, ctypes.c_void_p(handle) if handle else None))
 
     # ---------------------------------------------------------------- passes
-    def forward(self, steps: int, seed=1, record: bool = True) -> float:
+    def set_spike_train(self, train):
+        """spikeTrain (src/lifdiff.js:298-307): train[t] lists the neurons forced
+        to spike at step t (None or empty for none); None clears."""
+        ptr, idx = train_csr(train)
+        _l.check(self.lib.fb_diff_set_spike_train(self.h, int(ptr.size - 1), _l.ptr(ptr),
+                                                   _l.ptr(idx) if idx.size else None))
+
+    def forward(self, steps: int, seed=1, record: bool = True, spikeTrain=None) -> float:
+        """forward(steps, {seed, record, spikeTrain}) (src/lifdiff.js:219): the
+        train applies to this pass only, as in the reference."""
+        self.set_spike_train(spikeTrain)
         L = ctypes.c_double()
         _l.check(self.lib.fb_diff_forward(self.h, int(steps), float(seed), 1 if record else 0, ctypes.byref(L)))
         self.steps = int(steps)

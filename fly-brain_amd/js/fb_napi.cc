@@ -835,6 +835,31 @@ napi_value diff_drive_grad(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
+// diffSetSpikeTrain(h, Float64Array ptr | null, Int32Array idx | null): step t
+// forces idx[ptr[t] .. ptr[t+1]) (ptr holds steps + 1 integral offsets from
+// 0); null clears
+napi_value diff_set_spike_train(napi_env env, napi_callback_info info) {
+    Args a;
+    if (!get_args(env, info, a, 3)) return nullptr;
+    DIFF_OR_NULL(0);
+    double *pd;
+    int32_t *ix;
+    size_t np_ = 0, ni = 0;
+    if (!typed(env, a.v[1], &pd, &np_, true) || !typed(env, a.v[2], &ix, &ni, true)) return nullptr;
+    if (!pd || np_ < 2) {
+        FB_OK_OR_THROW(fb_diff_set_spike_train(f, 0, nullptr, nullptr), "diffSetSpikeTrain");
+        return nullptr;
+    }
+    std::vector<int64_t> ptr(np_);
+    for (size_t k = 0; k < np_; k++) ptr[k] = (int64_t)pd[k];
+    if (ptr[np_ - 1] < 0 || !long_enough(env, ni, (size_t)ptr[np_ - 1], "diffSetSpikeTrain")) {
+        if (ptr[np_ - 1] < 0) napi_throw_range_error(env, nullptr, "diffSetSpikeTrain: negative offset");
+        return nullptr;
+    }
+    FB_OK_OR_THROW(fb_diff_set_spike_train(f, (int32_t)(np_ - 1), ptr.data(), ix), "diffSetSpikeTrain");
+    return nullptr;
+}
+
 // diffForward(h, steps, seed, record) -> loss
 napi_value diff_forward(napi_env env, napi_callback_info info) {
     Args a;
@@ -1308,6 +1333,7 @@ napi_value init(napi_env env, napi_value exports) {
         {"diffSetDriveSchedule", diff_set_drive_schedule},
         {"diffSetLossPerEpoch", diff_set_loss_per_epoch},
         {"diffDriveGrad", diff_drive_grad},
+        {"diffSetSpikeTrain", diff_set_spike_train},
         {"diffForward", diff_forward},
         {"diffBackward", diff_backward},
         {"diffTypeGrads", diff_type_grads},

@@ -8,7 +8,7 @@
 // inScale, logGain, outScale, thrOffset, edgeGain, preSign, driveIdx,
 // driveEpoch, surrogateBeta as a number or a Float32Array, soft, driveSoft,
 // exact32, adjClip), the same methods (setDrive, setDriveOne, reset,
-// forward(steps, {seed, record, onEpoch}), backward(tape, dLdSpike, {truncate,
+// forward(steps, {seed, record, onEpoch, spikeTrain}), backward(tape, dLdSpike, {truncate,
 // lossFrom, dLdSpikePerEpoch})) and results ({params, logGain, drive,
 // truncate, clamped}).  The device holds the state and the tape; the tape
 // object returned by forward() names the run it belongs to, and only the last
@@ -19,7 +19,8 @@
 // it may call setDrive / setDriveOne.  Here it is called for every epoch before
 // the pass, and the rates of driveIdx it leaves are the pass's drive schedule:
 // the same run whenever the callback does not read the network's own state
-// (the visual chain's coupling does not).  spikeTrain is not supported.
+// (the visual chain's coupling does not).  spikeTrain[t] (an index list or
+// null per step) forces those neurons at step t of that pass.
 // Written in Node-12 syntax (CommonJS, no optional chaining).
 const { native, EXC_SIGN } = require('./lifhip.js');
 
@@ -136,16 +137,28 @@ class LIFDiffHip {
     }
   }
 
-  /** forward(steps, {seed, record, onEpoch}) -> tape (src/lifdiff.js:219) */
+  /** forward(steps, {seed, record, onEpoch, spikeTrain}) -> tape (src/lifdiff.js:219) */
   forward(steps, o) {
     o = o || {};
-    if (o.spikeTrain) throw new Error('LIFDiffHip: spikeTrain is not supported');
     const seed = o.seed === undefined ? 1 : o.seed, record = o.record === undefined ? true : o.record;
     this._uploadDrive(steps, o.onEpoch || null);
+    this._uploadTrain(o.spikeTrain || null);
     const loss = native.diffForward(this._h, steps, seed, record ? 1 : 0);
     const tape = { steps, seed, loss, net: this, recorded: !!record };
+    if (record && o.spikeTrain) tape.spikeTrain = o.spikeTrain;   // src/lifdiff.js:243
     this._tape = tape;
     return tape;
+  }
+
+  // spikeTrain (src/lifdiff.js:298-307) as offsets and indices; null clears
+  _uploadTrain(train) {
+    if (!train) { native.diffSetSpikeTrain(this._h, null, null); return; }
+    const ptr = new Float64Array(train.length + 1);
+    for (let t = 0; t < train.length; t++) ptr[t + 1] = ptr[t] + (train[t] ? train[t].length : 0);
+    const idx = new Int32Array(Math.max(1, ptr[train.length]));
+    let k = 0;
+    for (let t = 0; t < train.length; t++) if (train[t]) for (const i of train[t]) idx[k++] = i;
+    native.diffSetSpikeTrain(this._h, ptr, idx);
   }
 
   /** the tape's spike lists: [{idx, amp, forced}] per step, lif.c's order */

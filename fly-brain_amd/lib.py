@@ -96,6 +96,7 @@ _SIGS = {
     "fb_diff_set_drive_schedule": (I32, [P, P, I32, I32, P, I32, I32]),
     "fb_diff_set_loss_per_epoch": (I32, [P, P, I32, I32]),
     "fb_diff_drive_grad": (I32, [P, P]),
+    "fb_diff_set_spike_train": (I32, [P, I32, P, P]),
     "fb_set_rng": (I32, [P, ctypes.c_uint32]),
     "fb_batch_set_lanes": (I32, [P, I32, P]),
     "fb_batch_set_rng": (I32, [P, P]),

@@ -337,6 +337,14 @@ fb_status fb_diff_set_loss_per_epoch(fb_diff *d, const float *dPer, int32_t nEpo
 /* dL/d sched after fb_diff_backward: [nEpochs][nD] (the `drive` result of
  * LIFDiff.backward, src/lifdiff.js:490-492) */
 fb_status fb_diff_drive_grad(fb_diff *d, float *gDrive);
+/* forward(steps, {spikeTrain}) (src/lifdiff.js:219, 243, 298-307; the replay
+ * at 432-438): at step t < steps the neurons idx[ptr[t] .. ptr[t+1]) are
+ * forced to spike, in that order, after the step's drive draws and without a
+ * draw -- a neuron refractory by then is skipped -- and join the step's list
+ * after its driven spikes.  ptr has steps + 1 entries from 0; steps = 0
+ * clears.  It applies to the forward passes that follow; a recorded tape
+ * keeps the forced spikes it ran with. */
+fb_status fb_diff_set_spike_train(fb_diff *d, int32_t steps, const int64_t *ptr, const int32_t *idx);
 /* forward(steps, {seed, record}) (lifdiff.js:219): from reset; loss = the
  * flat term sum dLdSpike * spike count from lossFrom on, plus the readout term */
 fb_status fb_diff_forward(fb_diff *d, int32_t steps, double seed, int32_t record, double *loss);

@@ -23,6 +23,8 @@
  *   - exact32: the forward pass in lif.c's binary32 arithmetic,     src/lifdiff.js:105-109, 173-190,
  *     with writeGraph's baked weights and signs                      231-240, 263-274, 317-326
  *   - a caller-given inScale, and a per-neuron surrogateBeta         src/lifdiff.js:89-95, 115-118, 160
+ *   - forward's spikeTrain: listed neurons forced like driven        src/lifdiff.js:219, 243, 298-307,
+ *     spikes after the draws, kept with the tape for the replay      432-438
  * The reference evaluates in JavaScript doubles and stores into Float32Arrays:
  * every local below is a double and every array store rounds to binary32, in
  * the reference's evaluation order.  Under exact32 every forward intermediate
@@ -115,6 +117,10 @@ struct fdo_net {
     float *wdrive;
     int32_t *wdrv, nW;
     uint8_t *inW;
+    /* the spike train for the next forward pass, and the one the tape ran with */
+    int32_t trainSteps, tapeTrainSteps;
+    int64_t *trainPtr, *tapeTrainPtr;
+    int32_t *trainIdx, *tapeTrainIdx;
 };
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
@@ -175,6 +181,9 @@ static void free_tape(fdo_net *n) {
     free(n->ck);
     n->ck = NULL;
     n->nck = 0;
+    free(n->tapeTrainPtr); free(n->tapeTrainIdx);
+    n->tapeTrainPtr = NULL; n->tapeTrainIdx = NULL;
+    n->tapeTrainSteps = 0;
 }
 
 void fdo_destroy(fdo_net *n) {
@@ -188,7 +197,24 @@ void fdo_destroy(fdo_net *n) {
     free(n->readout); free(n->readIdx); free(n->target);
     free(n->drvIdx); free(n->drvSlot); free(n->drvSched); free(n->gDrive); free(n->dPer);
     free(n->wdrive); free(n->wdrv); free(n->inW);
+    free(n->trainPtr); free(n->trainIdx);
     free(n);
+}
+
+/* forward's spikeTrain (src/lifdiff.js:219, 298-307): step t < steps forces
+ * idx[ptr[t] .. ptr[t+1]) in that order; steps = 0 clears.  The next forward
+ * pass copies it into its tape, whose replay applies it (src/lifdiff.js:243,
+ * 432-438). */
+void fdo_set_spike_train(fdo_net *n, int32_t steps, const int64_t *ptr, const int32_t *idx) {
+    free(n->trainPtr); free(n->trainIdx);
+    n->trainPtr = NULL; n->trainIdx = NULL;
+    n->trainSteps = 0;
+    if (steps <= 0) return;
+    n->trainPtr = (int64_t *)malloc(8 * ((size_t)steps + 1));
+    memcpy(n->trainPtr, ptr, 8 * ((size_t)steps + 1));
+    n->trainIdx = (int32_t *)xcalloc((size_t)ptr[steps], 4);
+    if (ptr[steps]) memcpy(n->trainIdx, idx, 4 * (size_t)ptr[steps]);
+    n->trainSteps = steps;
 }
 
 /* driveIdx / driveEpoch with the rates the onEpoch hook would set
@@ -511,6 +537,14 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
     reset_state(n);
     start_drive(n);
     n->lossPer = 0.0;
+    if (n->trainSteps) {   /* the tape keeps the train it ran with */
+        const int64_t nI = n->trainPtr[n->trainSteps];
+        n->tapeTrainPtr = (int64_t *)malloc(8 * ((size_t)n->trainSteps + 1));
+        memcpy(n->tapeTrainPtr, n->trainPtr, 8 * ((size_t)n->trainSteps + 1));
+        n->tapeTrainIdx = (int32_t *)xcalloc((size_t)nI, 4);
+        if (nI) memcpy(n->tapeTrainIdx, n->trainIdx, 4 * (size_t)nI);
+        n->tapeTrainSteps = n->trainSteps;
+    }
     const int ex = p->exact32 != 0;
     if (ex) {
         /* lif.c sums vThresh + adapt + thr only: the additive field must be in
@@ -537,20 +571,31 @@ double fdo_forward(fdo_net *n, int32_t steps, double seed, int32_t record) {
         n->steps = t;
         const int scoredStep = t >= n->lossFrom;
         const float *wPer = n->dPer ? n->dPer + (int64_t)(t / n->ep) * N : NULL;
+        /* a forced spike (src/lifdiff.js:293-296, 303-305) */
+#define FORCE_SPIKE(i)                                                                   \
+    do {                                                                                 \
+        refr[i] = ex ? xtRf + xdT : (float)(p->tRef + p->dt);                            \
+        push_spike(n, (uint32_t)(i) | FORCED_BIT, 1.f);                                  \
+        sc[i] = (float)((double)sc[i] + 1.0);                                            \
+        if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + 1.0);              \
+        if (scoredStep && wPer) n->lossPer += (double)wPer[i];                           \
+        adapt[i] = ex ? adapt[i] + xaI : (float)((double)adapt[i] + p->adaptInc);        \
+        if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = 1.f;     \
+    } while (0)
         /* driven pre-pass in set order (src/lifdiff.js:290-297) */
         for (int32_t q = 0; q < n->nW && !n->driveSoft; q++) {
             const int32_t i = n->wdrv[q];
             if (refr[i] > 0) continue;
-            if (rand01(&n->rng) < (ex ? (double)(drive[i] * xdtS) : (double)drive[i] * k.dtS)) {
-                refr[i] = ex ? xtRf + xdT : (float)(p->tRef + p->dt);
-                push_spike(n, (uint32_t)i | FORCED_BIT, 1.f);
-                sc[i] = (float)((double)sc[i] + 1.0);
-                if (scoredStep) n->scored[i] = (float)((double)n->scored[i] + 1.0);
-                if (scoredStep && wPer) n->lossPer += (double)wPer[i];
-                adapt[i] = ex ? adapt[i] + xaI : (float)((double)adapt[i] + p->adaptInc);
-                if (R && n->readIdx[i] >= 0) n->readS[(int64_t)t * R + n->readIdx[i]] = 1.f;
-            }
+            if (rand01(&n->rng) < (ex ? (double)(drive[i] * xdtS) : (double)drive[i] * k.dtS)) FORCE_SPIKE(i);
         }
+        /* the spike train: listed neurons in list order, no draw (src/lifdiff.js:298-307) */
+        if (t < n->trainSteps)
+            for (int64_t kk = n->trainPtr[t]; kk < n->trainPtr[t + 1]; kk++) {
+                const int32_t i = n->trainIdx[kk];
+                if (refr[i] > 0) continue;
+                FORCE_SPIKE(i);
+            }
+#undef FORCE_SPIKE
         /* integrate, decay, threshold (src/lifdiff.js:313-345) */
         for (int32_t i = 0; i < N; i++) {
             double vi = v[i], s = 0.0;
@@ -668,6 +713,15 @@ static void replay(fdo_net *n, int32_t from, int32_t to, arena_t *A) {
                 forced[i] = 1;
             }
         }
+        /* the tape's spike train, as forward applied it (src/lifdiff.js:432-438) */
+        if (t < n->tapeTrainSteps)
+            for (int64_t kk = n->tapeTrainPtr[t]; kk < n->tapeTrainPtr[t + 1]; kk++) {
+                const int32_t i = n->tapeTrainIdx[kk];
+                if (refr[i] > 0) continue;
+                refr[i] = (float)(p->tRef + p->dt);
+                adapt[i] = (float)((double)adapt[i] + p->adaptInc);
+                forced[i] = 1;
+            }
         for (int32_t i = 0; i < N; i++) {
             double vi = v[i], sv = 0.0;
             uint8_t md = 0;

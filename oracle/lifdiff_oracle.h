@@ -67,6 +67,7 @@ void fdo_set_drive_schedule(fdo_net *n, const int32_t *idx, int32_t nD, int32_t 
 void fdo_set_loss_per_epoch(fdo_net *n, const float *dPer, int32_t nEp, int32_t epochSteps);
 /* dL/d sched [nEp][nD] of the last backward */
 void fdo_drive_grad(const fdo_net *n, float *gDrive);
+void fdo_set_spike_train(fdo_net *n, int32_t steps, const int64_t *ptr, const int32_t *idx);
 /* Runs `steps` steps from reset with the given seed; records the tape when
  * `record` is nonzero.  Returns the loss (NaN when exact32 is set with a
  * non-empty thrMask, which LIFDiff refuses). */

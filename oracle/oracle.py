@@ -254,6 +254,7 @@ def _diff_lib():
         "fdo_set_drive_schedule": (None, [P, P, I32, I32, P, I32, I32]),
         "fdo_set_loss_per_epoch": (None, [P, P, I32, I32]),
         "fdo_drive_grad": (None, [P, P]),
+        "fdo_set_spike_train": (None, [P, I32, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -362,7 +363,22 @@ class OracleDiff:
         self.lib.fdo_drive_grad(self.h, _ptr(out))
         return out
 
-    def forward(self, steps: int, seed=1, record: bool = True) -> float:
+    def set_spike_train(self, train):
+        """train[t]: the neurons forced at step t (None or empty for none); None clears."""
+        if train is None:
+            self.lib.fdo_set_spike_train(self.h, 0, None, None)
+            return
+        lists = [np.asarray(x if x is not None else [], dtype=np.int32).ravel() for x in train]
+        ptr = np.zeros(len(lists) + 1, dtype=np.int64)
+        ptr[1:] = np.cumsum([x.size for x in lists]) if lists else []
+        idx = np.concatenate(lists).astype(np.int32) if lists and ptr[-1] else np.zeros(1, np.int32)
+        self._train = (ptr, idx)
+        self.lib.fdo_set_spike_train(self.h, len(lists), _ptr(ptr), _ptr(idx))
+
+    def forward(self, steps: int, seed=1, record: bool = True, spikeTrain=None) -> float:
+        """forward(steps, {seed, record, spikeTrain}) (src/lifdiff.js:219); the
+        train applies to this pass, as in the reference."""
+        self.set_spike_train(spikeTrain)
         self.steps = int(steps)
         return float(self.lib.fdo_forward(self.h, self.steps, float(seed), 1 if record else 0))
 

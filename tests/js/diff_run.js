@@ -2,8 +2,8 @@
 // Runs the differentiable model or an ensemble through the addon's
 // LIFDiffHip / LIFDiffBatchHip / LIFBatchHip classes for the parity tests:
 // reads a JSON case and writes the results as JSON.
-//   mode 'diff':      {data, opts, drive: [[i, hz]...], steps, seed, dL, edges}
-//                     -> {loss, params, logGain, clamped, spikes, v, rng}
+//   mode 'diff':      {data, opts, drive: [[i, hz]...], steps, seed, dL, edges, spikeTrain}
+//                     -> {loss, params, logGain, clamped, spikes, v, rng, trainKept}
 //   mode 'diffbatch': {data, members: [{opts, drive, dL}], steps[], seeds[]}
 //                     -> per member {loss, params, logGain} batched and alone
 //   mode 'batch':     {graph, N, params, seeds, typeOf, gains, drive: [ix, hz],
@@ -44,10 +44,12 @@ function result(fwd, tape, g) {
 let out;
 if (spec.mode === 'diff') {
   const net = build(spec.data, spec.opts, spec.drive);
-  const tape = net.forward(spec.steps, { seed: spec.seed });
+  const train = spec.spikeTrain || null;
+  const tape = net.forward(spec.steps, { seed: spec.seed, spikeTrain: train });
   const fwd = afterForward(net, tape);
   const g = net.backward(tape, Float32Array.from(spec.dL), { edges: !!spec.edges });
   out = result(fwd, tape, g);
+  if (train) out.trainKept = tape.spikeTrain === train;
   net.destroy();
 } else if (spec.mode === 'diffbatch') {
   const ms = spec.members.map((m) => build(spec.data, m.opts, m.drive));

@@ -444,3 +444,39 @@ def test_per_neuron_beta_and_given_in_scale_match_oracle():
     L0 = g.forward(STEPS, seed=1)
     assert abs(L0 - dc.Problem().oracle().forward(STEPS, seed=1)) <= 1e-5 * abs(L0)
     g.destroy()
+
+
+@pytest.mark.parametrize("drive_every", [5, 0])
+def test_spike_train_forward_and_gradients_match_oracle(drive_every):
+    """forward(steps, {spikeTrain}) (src/lifdiff.js:298-307): a random train
+    -- neurons repeated within a step, listed while refractory, listed next to
+    driven neurons -- on top of the drive (or alone); the tape, state, stream
+    and loss bit-identical to the oracle, the gradients (replay from the tape's
+    forced spikes) to 1e-5, and the train applies to its own pass only."""
+    pr = dc.Problem(N=300, seed=29, over={"soft": False, "tRef": 1.0}, drive_every=drive_every)
+    rng = np.random.default_rng(4)
+    T = 150
+    train = [rng.integers(0, pr.g.N, size=rng.integers(0, 12)).tolist() if rng.random() < 0.7 else None
+             for _ in range(T - 10)]
+    train[3] = [7, 7, 8, 7]
+    o, g = pr.oracle(), gpu_of(pr)
+    Lo = o.forward(T, seed=5, spikeTrain=train)
+    Lg = g.forward(T, seed=5, spikeTrain=train)
+    off_o, idx_o, amp_o, f_o = o.spikes()
+    off_g, idx_g, amp_g, f_g = g.spikes()
+    assert np.array_equal(off_o, off_g) and np.array_equal(idx_o, idx_g)
+    assert np.array_equal(f_o, f_g) and np.array_equal(amp_o, amp_g)
+    assert int(f_g.sum()) > 300
+    for k in STATE:
+        assert np.array_equal(o.state(k).view(np.uint32), g.state(k).view(np.uint32)), k
+    assert o.rng == g.rng
+    assert Lg == pytest.approx(Lo, rel=1e-12)
+    go, gg = o.backward(), g.backward()
+    for k in P.DIFF_PARAMS:
+        a, b = gg["params"][k], go["params"][k]
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b)) + 1e-6 * abs(b), (k, a, b)
+    assert rel_err(gg["logGain"], go["logGain"]) < 1e-5
+    assert rel_err(gg["edge"], go["edge"]) < 1e-5
+    # the next pass without a train is the plain forward
+    Lo2, Lg2 = o.forward(T, seed=5), g.forward(T, seed=5)
+    assert np.array_equal(o.spikes()[1], g.spikes()[1]) and Lg2 == pytest.approx(Lo2, rel=1e-12)

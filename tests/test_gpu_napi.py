@@ -328,3 +328,53 @@ def test_lifbatchhip_from_node_matches_the_oracle(tmp_path):
         assert m["spikeCount"] == o.spikeCount.tolist(), b
         assert np.array_equal(np.array(m["v"], np.float32).view(np.uint32), o.array("v").view(np.uint32)), b
         assert m["rng"] == o.rng, b
+
+
+def test_lifdiffhip_from_node_spike_train(tmp_path):
+    """forward(steps, {spikeTrain}) from Node (src/lifdiff.js:219, 243,
+    298-307): scripts/vnc_regression.mjs's burst -- a 20 Hz train on neuron 0
+    of a 0 -> 1 edge (count 10, wSyn 5) fires neuron 1, and the tape keeps the
+    train -- and on a random network with drive the tape, state, stream and
+    gradients of the oracle (bit-identical forward, gradients to 1e-5)."""
+    import diffcases as dc
+    print("LIFDiffHip from Node: spikeTrain", flush=True)
+    data = {"N": 2, "indptr": [0, 1, 1], "indices": [1], "weights": [10], "nt": [1, 1]}
+    train = [[0] if (t % 5 == 0 and t < 50) else None for t in range(60)]
+    res = _node_json(tmp_path, {"mode": "diff", "data": data, "opts": {"wSyn": 5, "minSyn": 1}, "steps": 60,
+                                "seed": 1, "dL": [0.0, 1.0], "spikeTrain": train})
+    assert res["trainKept"]
+    # dt 0.5 ms, tRef 2.2 ms: a forced spike leaves neuron 0 refractory for
+    # 2.7 ms, so every second listing (5 steps = 2.5 ms on) is skipped
+    assert res["spikeIdx"].count(1) > 0 and res["spikeIdx"].count(0) == 5
+    pr = dc.Problem(over={"soft": False, "tRef": 2.2}, seed=31)
+    g = pr.g
+    data = {"N": g.N, "indptr": g.indptr.tolist(), "indices": g.indices.tolist(),
+            "weights": g.counts.astype(int).tolist(), "nt": np.where(g.exc, 1, 2).tolist()}
+    opts = {**{k: v for k, v in pr.p.items() if k != "checkpoint"},
+            "outScale": pr.inputs["outScale"].tolist(), "logGain": pr.inputs["logGain"].tolist(),
+            "thrOffset": pr.inputs["thrOffset"].tolist(), "sizeLog": pr.inputs["sizeLog"].tolist(),
+            "thrMask": pr.thr_mask.tolist(), "biasMask": pr.bias_mask.tolist()}
+    rng = np.random.default_rng(9)
+    train = [rng.integers(0, g.N, size=rng.integers(0, 6)).tolist() if rng.random() < 0.6 else None
+             for _ in range(100)]
+    drive = [[int(i), 60.0] for i in pr.driven]
+    res = _node_json(tmp_path, {"mode": "diff", "data": data, "opts": opts, "drive": drive, "steps": 120, "seed": 3,
+                                "dL": pr.dL.tolist(), "spikeTrain": train})
+    o = pr.oracle()
+    Lo = o.forward(120, seed=3, spikeTrain=train)
+    off, idx, _, forced = o.spikes()
+    ras = []
+    for t in range(120):   # lif.c's order: forced as listed, then threshold spikes by index
+        seg, fs = idx[off[t]:off[t + 1]], forced[off[t]:off[t + 1]]
+        ras += seg[fs].tolist() + sorted(seg[~fs].tolist())
+    assert res["spikeOff"] == off.tolist() and res["spikeIdx"] == ras
+    assert np.array_equal(np.array(res["v"], np.float32).view(np.uint32), o.state("v").view(np.uint32))
+    # the loss: the targets' summed spike counts (LIFDiff's forward reports
+    # none; the caller's dLdSpike goes to backward)
+    L = float(np.dot(np.array(res["spikeCount"], np.float64), pr.dL.astype(np.float64)))
+    assert res["rng"] == o.rng and abs(L - Lo) <= 1e-12 * abs(Lo)
+    go = o.backward()
+    for k in P.DIFF_PARAMS:
+        assert abs(res["params"][k] - go["params"][k]) <= 1e-5 * abs(go["params"][k]) + 1e-9, k
+    lg = np.array(res["logGain"], np.float32)
+    assert float(np.abs(lg - go["logGain"]).max()) <= 1e-5 * float(np.abs(go["logGain"]).max())

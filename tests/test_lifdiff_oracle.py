@@ -431,3 +431,78 @@ def test_given_in_scale_is_taken_verbatim(base):
     # cleared, the derived inScale is back
     net.set_input("inScale", None)
     assert net.forward(STEPS, seed=1) == L
+
+
+# ---------------------------------------------------------------- forward's spikeTrain
+
+def _forced_lists(net):
+    """The forced spikes of each recorded step, in list order."""
+    off, idx, _, forced = net.spikes()
+    return [idx[off[t]:off[t + 1]][forced[off[t]:off[t + 1]]].tolist() for t in range(off.size - 1)]
+
+
+@pytest.mark.parametrize("soft", [False, True])
+def test_spike_train_is_the_driven_spikes_it_replays(oracle_built, soft):
+    """A train that lists, step by step, the spikes a driven run forced gives a
+    run without drive the same forward pass bit for bit (state, spike lists)
+    and the same gradients: forced like a driven spike, without a draw, at the
+    head of the step's list (src/lifdiff.js:290-307)."""
+    pr = dc.Problem(N=300, seed=17, over={"soft": soft, "tRef": 1.0}, drive_every=4, drive_hz=150.0)
+    a = pr.oracle()
+    T = 120
+    La = a.forward(T, seed=3)
+    train = _forced_lists(a)
+    assert sum(len(x) for x in train) > 100
+    b = pr.oracle(inputs={"drive": np.zeros(pr.g.N, np.float32)})
+    b.set_driven(np.zeros(0, np.int32))
+    Lb = b.forward(T, seed=99, spikeTrain=train)
+    assert Lb == La
+    for k in ("v", "gE", "gI", "adapt", "res", "refr", "spikeCount"):
+        assert np.array_equal(a.state(k).view(np.uint32), b.state(k).view(np.uint32)), k
+    oa, ia, aa, fa = a.spikes()
+    ob, ib, ab, fb = b.spikes()
+    assert np.array_equal(oa, ob) and np.array_equal(ia, ib) and np.array_equal(aa, ab) and np.array_equal(fa, fb)
+    ga = a.backward()
+    gb = b.backward()
+    assert ga["params"] == gb["params"]
+    assert np.array_equal(ga["logGain"], gb["logGain"]) and np.array_equal(ga["edge"], gb["edge"])
+
+
+def test_spike_train_skips_refractory_and_repeated_neurons(oracle_built):
+    """A listed neuron that is refractory -- listed twice in a step, listed on
+    the next step inside tRef, or drawn by its drive in the same step -- is
+    skipped (src/lifdiff.js:303)."""
+    net, p = _lone(3, over={"tRef": 0.25})
+    net.set_input("drive", np.array([1e7, 0.0, 0.0], np.float32))   # neuron 0's draw always fires
+    net.set_driven(np.array([0], np.int32))
+    # forced: refr = tRef + dt = 0.35 ms, down 0.1 ms a step: refractory at
+    # steps 1-3, free again at step 4
+    train = [[1, 1, 0, 2], [1], [], [1], [1]]
+    net.forward(5, seed=1, spikeTrain=train)
+    assert _forced_lists(net) == [[0, 1, 2], [], [], [], [0, 1]]
+    assert net.state("spikeCount").tolist() == [2.0, 2.0, 1.0]
+
+
+def test_spike_train_burst_reaches_its_target_scaled_by_log_gain(oracle_built):
+    """scripts/vnc_regression.mjs:40-79: a 20 Hz train on neuron 0 of a
+    0 -> 1 edge (count 10, wSyn 5) delivers conductance and fires neuron 1; a
+    log-gain of log 4 on the source makes the delivered conductance 4x."""
+    net, p = _lone(2, edges=[(0, 1)], counts=[10.0], over={"wSyn": 5.0, "dt": 0.5})
+    train = [[0] if (t % 5 == 0 and t < 50) else None for t in range(60)]
+    gmax = 0.0
+    for n in range(1, 61):
+        net.forward(n, seed=1, record=False, spikeTrain=train)
+        gmax = max(gmax, float(net.state("gE")[1]))
+    net.forward(60, seed=1, spikeTrain=train)
+    off, idx, _, forced = net.spikes()
+    assert gmax > 0 and int(((idx == 1) & ~forced).sum()) > 0
+    one = [[0]] + [None] * 29
+
+    def peak(lg):
+        n2, _ = _lone(2, edges=[(0, 1)], counts=[10.0], over={"wSyn": 1.0, "dt": 0.5})
+        n2.set_input("logGain", np.array([lg, 0.0], np.float32))
+        return max(float((n2.forward(n, seed=1, record=False, spikeTrain=one), n2.state("gE")[1])[1])
+                   for n in range(1, 31))
+
+    g0, g4 = peak(0.0), peak(float(np.log(4.0)))
+    assert g0 > 0 and g4 == pytest.approx(4.0 * g0, rel=1e-6)
