@@ -1116,18 +1116,20 @@ __device__ __forceinline__ void build_list(const PersistArgs &a, Shared &s, int3
             atomicOr((unsigned long long *)&bits[i >> 6], 1ull << (i & 63));
         }
     }
-    int32_t ndTot;
-    int32_t pos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
-    for (int32_t q = dlo; q < dhi; q++)
-        if ((fl[q] >> m) & 1u) out[pos++] = d.driven[q];
+    // the bitmask's atomics are complete before any thread reads it (a fence
+    // with only them in flight: the list's own stores come after it)
     __threadfence_block();
-    __syncthreads();
+    int32_t ndTot;
+    int32_t fpos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
     const int32_t W = (a.nwords + PT - 1) / PT;
     const int32_t wlo = min(a.nwords, tid * W), whi = min(a.nwords, wlo + W);
     int32_t cnt = 0;
     for (int32_t w = wlo; w < whi; w++) cnt += __popcll(__hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    // the forced spikes first in the list, in driven-list order
+    for (int32_t q = dlo; q < dhi; q++)
+        if ((fl[q] >> m) & 1u) out[fpos++] = d.driven[q];
     int32_t thTot;
-    pos = ndTot + wg_exclusive_scan<PT>(cnt, s.scan, &thTot);
+    int32_t pos = ndTot + wg_exclusive_scan<PT>(cnt, s.scan, &thTot);
     for (int32_t w = wlo; w < whi; w++) {
         uint64_t b = __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!b) continue;
