@@ -372,9 +372,11 @@ __device__ __forceinline__ void dx_deliver(const DiffDev &d, int32_t t, int32_t 
 
 // A driven neuron's draw u against its rate (src/lifdiff.js:293): binary64
 // u < drive dt/1000, or under exact32 lif.c's binary32 threshold
-__device__ __forceinline__ bool drive_fires(const DiffDev &d, int32_t i, int32_t t, double u) {
-    const float rate = drive_at(d, i, t);
+__device__ __forceinline__ bool drive_fires_at(const DiffDev &d, float rate, double u) {
     return d.exact32 ? u < (double)(rate * d.xdtS) : u < (double)rate * d.dtS;
+}
+__device__ __forceinline__ bool drive_fires(const DiffDev &d, int32_t i, int32_t t, double u) {
+    return drive_fires_at(d, drive_at(d, i, t), u);
 }
 
 // a forced spike's refractory period and adaptation increment (src/lifdiff.js:294-295)
@@ -496,15 +498,23 @@ __device__ __forceinline__ void dx_driven(const DiffDev &d, int32_t t) {
 // The drive pre-pass with at most 2048 driven neurons (one workgroup of 1024
 // threads): each thread holds its at most two consecutive driven neurons in
 // registers, finds its first stream position with a scan of non-refractory
-// flags, and jumps there with two leapfrog products read from the global
-// tables (M^(32 (off/32)), then M^(off mod 32 + 1)); the draws after it are
-// single steps.  Same draws, effects and tape order as dx_driven.
+// flags, and jumps there in two leapfrog products, M^(32 (off/32)) then
+// M^(off mod 32 + 1); the draws after it are single steps.  The fine tables
+// (M^1..M^32) go to LDS and the 64 coarse products M^(32 h) x of the step's
+// stream state are formed while the driven neurons' refractory state and
+// rates load, so that after the scan a jump is one LDS product.  (Loading the
+// state a forced spike updates for every driven neuron up front costs more
+// than it saves: those are scattered loads from one workgroup.)  Same draws,
+// effects and tape order as dx_driven.
 __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
     constexpr int T = 1024, DP = 2;
     __shared__ int32_t scan[2][T / 64];
     __shared__ unsigned long long s_top;
     __shared__ uint32_t s_rng;
+    __shared__ uint32_t fineS[32 * LEAP_STRIDE];
+    __shared__ uint32_t ybase[64];
     const int tid = threadIdx.x;
+    leap_load<T, 32>(fineS, d.leap);
     if (tid == 0) {
         s_rng = *d.rng;
         s_top = *d.top;
@@ -512,19 +522,22 @@ __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
     }
     const int32_t nD = d.nDriven, P = (nD + T - 1) / T;
     int32_t ii[DP];
+    float rate[DP];
     uint32_t nr = 0;   // bit u: driven neuron u of this thread is not refractory and draws
 #pragma unroll
     for (int u = 0; u < DP; u++) {
         const int32_t q = tid * P + u;
         ii[u] = (u < P && q < nD) ? d.driven[q] : -1;
+        rate[u] = ii[u] >= 0 ? drive_at(d, ii[u], t) : 0.f;
         if (ii[u] >= 0 && !(d.refr[ii[u]] > 0)) nr |= 1u << u;
     }
-    __syncthreads();
+    __syncthreads();   // s_rng, s_top and the fine tables
+    if (tid < 64) ybase[tid] = matvec(d.leap + (32 + tid) * LEAP_WORDS, s_rng);
     int32_t n;
-    const int32_t off = wg_scan_flags<T, DP>(nr, scan[0], &n);
+    const int32_t off = wg_scan_flags<T, DP>(nr, scan[0], &n);   // its barrier publishes ybase
     uint32_t fm = 0;   // bit u: neuron u was forced to spike
     if (nr) {
-        uint32_t x = matvec(d.leap + (off & 31) * LEAP_WORDS, matvec(d.leap + (32 + (off >> 5)) * LEAP_WORDS, s_rng));
+        uint32_t x = matvec(fineS + (off & 31) * LEAP_STRIDE, ybase[off >> 5]);
         int32_t j = off;
         bool first = true;
 #pragma unroll
@@ -534,7 +547,7 @@ __device__ __forceinline__ void dx_driven_fast(const DiffDev &d, int32_t t) {
             first = false;
             const int32_t i = ii[u];
             const double uu = (double)(x >> 8) * (1.0 / 16777216.0);
-            if (drive_fires(d, i, t, uu)) {
+            if (drive_fires_at(d, rate[u], uu)) {
                 forced_effects(d, i);
                 d.forced[i] = 1;
                 d.spikeCount[i] = (float)((double)d.spikeCount[i] + 1.0);
