@@ -651,8 +651,8 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
 }
 
 // A driven block's final cell state -> the stage (coalesced) at the launch's
-// end; the list-building blocks then write a share of it each to the brain's
-// arrays (unstage_driven).  One workgroup storing ~1,650 scattered neurons'
+// end; the regular blocks then write a share of it each to the brain's arrays
+// (unstage_driven).  One workgroup storing ~1,650 scattered neurons'
 // eight fields took ~25 us at a launch's end.
 __device__ __forceinline__ void stage_out(const PersistArgs &a, const Cell (&cells)[CELLS], int32_t qb, int32_t P) {
     float *st = a.dstage;
@@ -1353,8 +1353,9 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     }
     // the last epoch's lists, and the state back to memory.  After a complete
     // run the driven blocks hand their cells over through the stage: each
-    // writes it and counts itself in ctl[3]; the list-building blocks wait
-    // for all G and write a share each.  After a stop they store their cells.
+    // writes it and counts itself in ctl[3]; the regular blocks (which are
+    // done first) wait for all G and write a share each.  After a stop the
+    // driven blocks store their cells.
     const bool staged = e > 0 && !listsBuilt;
     if (staged) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
     FB_PHASE(3)
@@ -1371,8 +1372,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         for (int u = 0; u < CELLS; u++) store_cell(d, cells[u]);
     }
     if (staged) {
-        const int32_t first = a.nSpare > 0 ? a.S : a.G;
-        const int32_t count = a.nSpare > 0 ? a.nSpare : max(1, a.S - a.G);
+        const int32_t first = a.S > a.G ? a.G : a.S;
+        const int32_t count = a.S > a.G ? a.S - a.G : max(1, a.nSpare);
         if (me >= first && me < first + count) {
             if (tid == 0) {
                 const uint64_t t0 = wall_clock64();
@@ -1431,7 +1432,11 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         const unsigned long long tEnd = wall_clock64();
         atomicMax(&span[2], tEnd);
         atomicMax(&span[drivenBlock ? 4 : owning ? 5 : 6], tEnd);
-        __threadfence();
+        // the last block out reads ctl[0], which block 0 stored; everything
+        // else it reads is atomics, and the state stores need only the
+        // kernel's end (an agent-scope release in every block would write
+        // back the L2 once per block)
+        if (me == 0) __threadfence();
         s.last = atomicAdd(&a.ctl[2], 1) == (int32_t)gridDim.x - 1;
     }
     __syncthreads();
