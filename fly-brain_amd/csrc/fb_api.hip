@@ -918,9 +918,9 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     }
     const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
-        // the lookahead engine pays a prologue per call (buffer clears and one
-        // cooperative launch); from one epoch of steps on it is ahead of a
-        // launch or a graph replay per step
+        // the lookahead engine pays a prologue per call (one launch that
+        // loads the state and scatters the delay line); from one epoch of
+        // steps on it is ahead of a launch or a graph replay per step
         std::string why;
         const bool longRun = nsteps >= b->d.nslots - 1;
         // otherwise per-step kernels: replayed as a graph for small brains,
@@ -941,7 +941,7 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
             st = launch_persistent(b->d, b->head, nsteps, b->stream, &b->fallbackSteps, &b->persistentLaunches,
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st == FB_ERR_UNSUPPORTED && autoChosen) {
-                // the device refused the cooperative launch before any step ran
+                // the device refused the launch before any step ran
                 st = b->d.N >= 65536 ? run_launch(b, nsteps) : run_graph(b, nsteps);
                 break;
             }

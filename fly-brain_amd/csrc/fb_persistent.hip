@@ -1,4 +1,4 @@
-// fb_persistent.hip -- the lookahead engine: one cooperative launch advances
+// fb_persistent.hip -- the lookahead engine: one grid-barrier launch advances
 // many LIF steps, with the delay line as a lookahead window.
 //
 // A spike emitted at step t is delivered at step t + D (D = nslots - 1, 18 at

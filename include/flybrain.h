@@ -63,8 +63,8 @@ enum {
     FB_ENGINE_AUTO = 0,        /* the fastest engine valid for the brain's parameters */
     FB_ENGINE_LAUNCH = 1,      /* one kernel sequence per step */
     FB_ENGINE_GRAPH = 2,       /* the same sequence captured in a hipGraph, replayed */
-    FB_ENGINE_PERSISTENT = 3   /* one cooperative launch advances many steps; the
-                                  synaptic delay is used as a lookahead window */
+    FB_ENGINE_PERSISTENT = 3   /* one launch of a grid-barrier kernel advances many
+                                  steps; the synaptic delay is used as a lookahead window */
 };
 
 typedef struct fb_graph fb_graph;
