@@ -621,6 +621,7 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
         const float vf = d.v[i];
         double vi = vf, s = 0.0;
         float refrf = d.refr[i], adaptf = d.adapt[i], resf = d.res[i];
+        const float refr0 = refrf, adapt0 = adaptf, res0 = resf;
         const bool wasRefr = refrf > 0;
         const bool x32 = !REPLAY && d.exact32;
         uint8_t md = 0;
@@ -681,8 +682,13 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
                 adaptf = (float)((double)adaptf + s * (x32 ? (double)d.xadaptInc : d.adaptInc));
             }
             if (!REPLAY) {
-                d.spikeCount[i] = (float)((double)d.spikeCount[i] + s);
-                if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + s);
+                // adding s = 0 changes neither count (they start at +0 and
+                // only grow), so a neuron without a spike reads and writes
+                // neither
+                if (s != 0.0) {
+                    d.spikeCount[i] = (float)((double)d.spikeCount[i] + s);
+                    if (t >= d.lossFrom) d.scored[i] = (float)((double)d.scored[i] + s);
+                }
                 if (d.dPer && t >= d.lossFrom) d.lossAcc[i] += spike_weight(d, i, t) * s;
                 const int32_t rk = d.readIdx[i];
                 if (rk >= 0) d.readS[(int64_t)t * d.R + rk] = (float)s;
@@ -691,9 +697,14 @@ __device__ __forceinline__ void dx_membrane(const DiffDev &d, int32_t t, int32_t
             }
         }
         if (REPLAY) { d.aMode[ar] = md; d.aSp[ar] = (float)s; }
-        d.forced[i] = 0;
+        // a forced neuron is refractory here (its pre-pass set refr > 0); the
+        // slowly changing arrays are written only where a value changed
+        if (wasRefr) d.forced[i] = 0;
         d.v[i] = (float)vi;
-        d.gE[i] = gEf; d.gI[i] = gIf; d.adapt[i] = adaptf; d.res[i] = resf; d.refr[i] = refrf;
+        d.gE[i] = gEf; d.gI[i] = gIf;
+        if (__float_as_uint(adaptf) != __float_as_uint(adapt0)) d.adapt[i] = adaptf;
+        if (__float_as_uint(resf) != __float_as_uint(res0)) d.res[i] = resf;
+        if (__float_as_uint(refrf) != __float_as_uint(refr0)) d.refr[i] = refrf;
     }
     if (!REPLAY) {
         const uint64_t m = __ballot(emit);
