@@ -506,3 +506,14 @@ def test_spike_train_burst_reaches_its_target_scaled_by_log_gain(oracle_built):
 
     g0, g4 = peak(0.0), peak(float(np.log(4.0)))
     assert g0 > 0 and g4 == pytest.approx(4.0 * g0, rel=1e-6)
+
+
+def test_spike_train_offsets_for_the_boundary():
+    """train_csr, the host side of fb_diff_set_spike_train: offsets from 0 over
+    the steps, empty steps (None or []) kept, list order kept."""
+    diff = importlib.import_module("fly-brain_amd.diff")
+    ptr, idx = diff.train_csr([[3, 1, 3], None, [], [7]])
+    assert ptr.dtype == np.int64 and idx.dtype == np.int32
+    assert ptr.tolist() == [0, 3, 3, 3, 4] and idx.tolist() == [3, 1, 3, 7]
+    ptr, idx = diff.train_csr(None)
+    assert ptr.tolist() == [0] and idx.size == 0

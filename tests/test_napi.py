@@ -30,7 +30,7 @@ def test_addon_exports_the_brain_api(tmp_path):
             "fvGet", "fvBackward",
             "diffCreate", "diffDestroy", "diffSetParams", "diffSetInput", "diffSetEdgeGain", "diffSetMasks",
             "diffSetDriven", "diffSetLoss", "diffSetReadout", "diffSetTapeCapacity", "diffSetDriveSchedule",
-            "diffSetLossPerEpoch", "diffDriveGrad", "diffForward", "diffBackward", "diffTypeGrads", "diffGetState",
+            "diffSetLossPerEpoch", "diffDriveGrad", "diffSetSpikeTrain", "diffForward", "diffBackward", "diffTypeGrads", "diffGetState",
             "diffSpikes", "diffRng", "diffStat", "diffSize", "diffParamsLength",
             "diffBatchCreate", "diffBatchDestroy", "diffBatchForward", "diffBatchBackward", "diffBatchStat",
             "batchCreate", "batchDestroy", "batchSetDrive", "batchSetMember", "batchSetSteps", "batchRun",
