@@ -861,9 +861,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-cpu-steps", type=int, default=1000)
     ap.add_argument("--adjoint-batch-cpu-steps", type=int, default=400)
-    ap.add_argument("--only", choices=["headline", "flyvis", "visual", "cordens", "lanes", "brain", "adjoint"],
+    ap.add_argument("--only", choices=["headline", "flyvis", "visual", "cordens", "lanes", "brain", "cord",
+                                       "adjoint", "adjoint32", "batch"],
                     help="run one leg alone and print its JSON (headline: the steady-state forward only; "
-                         "adjoint: the full-scale forward + adjoint)")
+                         "adjoint: the full-scale forward + adjoint; adjoint32: its 32-instance batch; "
+                         "batch: the 64-member ensemble; cord: the ventral-cord brain)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not measured "
@@ -892,14 +894,25 @@ def main():
     if args.only == "lanes":
         print(json.dumps({"cord_ensemble_lanes": bench_cord_ensemble(args, brain_per_run=False)}), flush=True)
         return
-    if args.only == "brain":
+    if args.only in ("brain", "cord"):
         lif, hiprt = pkg("lif"), pkg("hiprt")
         st = hiprt.Stream()
-        print(json.dumps({"real_brain": bench_brain(args, lif, hiprt, st)}), flush=True)
+        if args.only == "brain":
+            print(json.dumps({"real_brain": bench_brain(args, lif, hiprt, st)}), flush=True)
+        else:
+            print(json.dumps({"real_cord": bench_cord(args, lif, hiprt, st)}), flush=True)
         st.destroy()
         return
     synth, P, lif, hiprt, batch, diff = (pkg("synth"), pkg("params"), pkg("lif"), pkg("hiprt"), pkg("batch"),
                                          pkg("diff"))
+    if args.only == "batch":
+        st = hiprt.Stream()
+        print(json.dumps({"batch64": bench_batch64(args, synth, P, lif, batch, hiprt, st)}), flush=True)
+        st.destroy()
+        return
+    if args.only == "adjoint32":
+        print(json.dumps({"fwd_adjoint_batch32": bench_adjoint_batch(args, synth, P, diff, hiprt)}), flush=True)
+        return
     if args.only == "adjoint":
         st = hiprt.Stream()
         print(json.dumps({"fwd_adjoint": bench_adjoint(args, synth, P, diff, hiprt, st)}), flush=True)
