@@ -483,6 +483,7 @@ struct fb_batch {
     uint64_t drivenVersion = 0;
     int64_t laLaunches = 0, laFallback = 0, laSteps = 0;
     double laMs = 0.0;
+    double laPhaseUs[10] = {0};      // ensemble lookahead clocks: driven then regular workgroup, 5 phases each
     // the lookahead engine's regular-block shape: chosen by the autotuner
     // (laShape, -1 until measured) or forced by the caller (laForced)
     int32_t laShape = -1, laForced = -1;
@@ -502,7 +503,7 @@ int32_t batch_la_shape_neurons(int k);
 void batch_la_release(const void *key);
 fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, int32_t epochs,
                           uint64_t drivenVersion, uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done,
-                          double *kernelMs);
+                          double *kernelMs, double *phaseUs);
 }
 
 // graph internals needed here (defined in fb_api.hip)
@@ -936,7 +937,7 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
         int32_t done = 0;
         const double ms0 = b->laMs;
         st = batch_la_launch(d, shape, b->t, b->head, epochs, b->drivenVersion, recM, recF, b->stream, &done,
-                             &b->laMs);
+                             &b->laMs, b->laPhaseUs);
         if (st == FB_ERR_UNSUPPORTED && b->engine == FB_ENGINE_AUTO) {
             // the device refused the cooperative launch: the per-step kernels
             la = false;
@@ -1054,6 +1055,17 @@ extern "C" fb_status fb_batch_stat(fb_batch *b, const char *name, double *value)
     else if (n == "lookahead_shape") {
         const int k = b->laForced >= 0 ? b->laForced : b->laShape;
         *value = k >= 0 ? (double)batch_la_shape_neurons(k) : 0.0;
+    } else if (n.rfind("la_phase_us_", 0) == 0) {
+        // la_phase_us_<drv|reg>_<prologue|gather|sim|push|barrier>: the first
+        // driven and the first regular workgroup of the ensemble lookahead kernel
+        static const char *blk[2] = {"drv_", "reg_"};
+        static const char *ph[5] = {"prologue", "gather", "sim", "push", "barrier"};
+        int idx = -1;
+        for (int q = 0; q < 2; q++)
+            for (int k = 0; k < 5; k++)
+                if (n == std::string("la_phase_us_") + blk[q] + ph[k]) idx = q * 5 + k;
+        if (idx < 0) { set_error("fb_batch_stat: unknown statistic '%s'", name); return FB_ERR_ARG; }
+        *value = b->laPhaseUs[idx];
     } else if (n.rfind("autotune_us_per_epoch_", 0) == 0) {
         // the autotuner's measurement of one shape (0: not measured)
         const int32_t nb = std::atoi(n.c_str() + 22);

@@ -73,6 +73,7 @@ constexpr int LADD = 3072;                 // member addends per destination per
 constexpr int LSPK = 1024;                 // spike entries one block pushes per epoch
 constexpr int LS_MAX = 512;                // destinations
 constexpr int LMATS = 48;                  // M^1..M^32 and M^(32h), h < 16: offsets below 512
+constexpr int LA_NPH = 5;                  // clocked phases: prologue, gather, simulate, push, barrier
 constexpr uint32_t OWN_DRV = 0x80000000u;
 constexpr int KEY_CELL_SHIFT = 47;
 
@@ -89,6 +90,9 @@ struct LaArgs {
     int32_t *ctl;
     int32_t *bar;                          // grid barrier counters (GBAR_WORDS, zero at launch)
     uint64_t *recM, *recF;                 // optional dense record [epochs*D][N]
+    uint64_t *prof;                        // [LA_PROF] phase clocks (see k_batch_la), added per launch
+    uint32_t *dspk;                        // [G][BDM][BDRV/4] each driven group's spike bytes of the epoch
+    int32_t *dflag;                        // [G] 1 + the epoch whose bytes dspk holds (zero at launch)
 };
 
 template <int RNB>
@@ -116,6 +120,7 @@ struct LaShared {
         uint32_t d[BDM][BDRV / 4];         // driven: a byte per (step, neuron): forced | threshold << 4
     } spk;
     uint32_t leapS[LMATS * LEAP_STRIDE];
+    uint64_t ph[LA_NPH + 1];               // phase clocks of thread 0, then the last stamp
     int32_t dcnt[LS_MAX];
     int32_t dbase[LS_MAX];
     int32_t dadd[LS_MAX];
@@ -466,8 +471,24 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
     }
     if (drv && tid < MPG) s.x0[tid] = me * MPG + tid < B ? d.rng[me * MPG + tid] : 0u;
 
+    // clocks of the first driven and the first regular workgroup, per phase:
+    // prologue, gather, simulate, push, grid barrier
+    const int timedSlot = me == 0 ? 0 : me == a.G ? 1 : -1;
+    const bool timed = tid == 0 && timedSlot >= 0;
+    if (timed) {
+        for (int k = 0; k < LA_NPH; k++) s.ph[k] = 0;
+        s.ph[LA_NPH] = wall_clock64();
+    }
+#define LA_PHASE(k)                                    \
+    if (timed) {                                       \
+        const uint64_t now = wall_clock64();           \
+        s.ph[k] += now - s.ph[LA_NPH];                 \
+        s.ph[LA_NPH] = now;                            \
+    }
     la_prologue(a, s, me, gridDim.x);
+    LA_PHASE(0)
     if (!grid_barrier(a.bar, &a.ctl[1])) return;
+    LA_PHASE(4)
 
     unsigned long long spikes = 0;
     int32_t last = -1;
@@ -477,6 +498,7 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
     for (int32_t e = 0; e < a.epochs && !(e == 0 && a.ctl[2]); e++) {
         const int32_t par = e & 1;
         la_gather(a, s, par, dstId, drv);
+        LA_PHASE(1)
         const int32_t tE = a.t0 + e * D;
         if (!drv) {
             for (int32_t st = 0; st < D; st++) {
@@ -590,7 +612,22 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
                 __syncthreads();
             }
         }
-        // push this epoch's spikes for the next one
+        // push this epoch's spikes for the next one.  A driven group's rows
+        // are most of an epoch's events and its workgroup finishes its steps
+        // first, so it publishes its spike bytes and every workgroup pushes a
+        // share of one group's rows beside its own
+        if (e + 1 < a.epochs && drv) {
+            __syncthreads();
+            uint32_t *out = a.dspk + (int64_t)me * BDM * (BDRV / 4);
+            for (int32_t q = tid; q < D * (BDRV / 4); q += BT) out[q] = s.spk.d[q / (BDRV / 4)][q % (BDRV / 4)];
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&a.dflag[me], e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        LA_PHASE(2)
         if (e + 1 < a.epochs) {
             __syncthreads();
             if (!drv) {
@@ -599,26 +636,47 @@ __global__ void __launch_bounds__(BT) k_batch_la(LaArgs a) {
                     const uint64_t mk = s.spk.r[st][tl];
                     if (mk) push_entry(a, s, rbase + tl, st, false, (uint32_t)N + (uint32_t)(rbase + tl), mk);
                 }
-            } else {
-                for (int32_t q = tid; q < D * a.nDrv; q += BT) {
-                    const int32_t st = q / a.nDrv, p = q % a.nDrv;
-                    const uint32_t byte = (s.spk.d[st][p >> 2] >> (8 * (p & 3))) & 255u;
-                    if (!byte) continue;
-                    const int32_t i = d.driven[p];
-                    const uint64_t sh = (uint64_t)MPG * me;
-                    if (byte & 15u) push_entry(a, s, i, st, true, (uint32_t)p, (uint64_t)(byte & 15u) << sh);
-                    if (byte >> 4) push_entry(a, s, i, st, false, (uint32_t)N + (uint32_t)i, (uint64_t)(byte >> 4) << sh);
+            }
+            // workgroup k takes driven group g = k mod G and, of its (step,
+            // position) pairs q, those with q = k div G (mod H), H the
+            // workgroups of that group
+            const int32_t g = me % a.G, jg = me / a.G, H = (a.S - g + a.G - 1) / a.G;
+            if (tid == 0) {
+                const uint64_t t0 = wall_clock64();
+                while (__hip_atomic_load(&a.dflag[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e + 1) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > GBAR_TIMEOUT) {
+                        atomicOr(&a.ctl[1], GBAR_TIMED_OUT);
+                        break;
+                    }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();
+            const uint32_t *src = a.dspk + (int64_t)g * BDM * (BDRV / 4);
+            const uint64_t sh = (uint64_t)MPG * g;
+            for (int32_t q = jg + tid * H; q < D * a.nDrv; q += BT * H) {
+                const int32_t st = q / a.nDrv, p = q % a.nDrv;
+                const uint32_t byte = (src[st * (BDRV / 4) + (p >> 2)] >> (8 * (p & 3))) & 255u;
+                if (!byte) continue;
+                const int32_t i = d.driven[p];
+                if (byte & 15u) push_entry(a, s, i, st, true, (uint32_t)p, (uint64_t)(byte & 15u) << sh);
+                if (byte >> 4) push_entry(a, s, i, st, false, (uint32_t)N + (uint32_t)i, (uint64_t)(byte >> 4) << sh);
             }
             la_push(a, s, (e + 1) & 1, tE + D);
         }
+        LA_PHASE(3)
         last = e;
         if (!grid_barrier(a.bar, &a.ctl[1])) return;
+        LA_PHASE(4)
         // the pushes of this epoch went to parity (e + 1) & 1; the next writes
         // to that flag come two epochs later, after every block has read it
         if (e + 1 < a.epochs && a.ctl[2 + ((e + 1) & 1)]) break;
     }
     if (spikes) atomicAdd(&d.counters[2], spikes);
+    if (timed && a.prof)
+        for (int k = 0; k < LA_NPH; k++) atomicAdd((unsigned long long *)&a.prof[timedSlot * LA_NPH + k], (unsigned long long)s.ph[k]);
+#undef LA_PHASE
 
     // ---- write back: state, streams, and the last epoch's spikes into the delay line
 #pragma unroll
@@ -717,6 +775,9 @@ static const void *la_kernel(int k) {
 int32_t batch_la_shape_neurons(int k) { return BWV * LA_SHAPES[k]; }
 
 struct LaBuffers {
+    uint64_t *prof = nullptr;                    // [2 * LA_NPH] phase clocks of the last launch
+    uint32_t *dspk = nullptr;                    // [LS_MAX / MPG][BDM][BDRV / 4]
+    int32_t *dflag = nullptr;                    // [LS_MAX / MPG]
     uint64_t *evKey = nullptr, *evMask = nullptr, *evAux = nullptr;
     int32_t *dstCnt = nullptr, *ctl = nullptr;   // dstCnt holds [2][S] events, then [2][S] addends
     int32_t *bar = nullptr;
@@ -776,7 +837,7 @@ void batch_la_release(const void *key) {
     if (it == g_labuf.end()) return;
     LaBuffers &b = it->second;
     for (void *p : {(void *)b.evKey, (void *)b.evMask, (void *)b.evAux, (void *)b.dstCnt, (void *)b.ctl,
-                    (void *)b.eown, (void *)b.bar})
+                    (void *)b.eown, (void *)b.bar, (void *)b.prof, (void *)b.dspk, (void *)b.dflag})
         if (p) (void)hipFree(p);
     if (b.ev0) (void)hipEventDestroy(b.ev0);
     if (b.ev1) (void)hipEventDestroy(b.ev1);
@@ -788,7 +849,7 @@ void batch_la_release(const void *key) {
 // after a bucket overflow) and adds the kernel's HIP-event time to *kernelMs.
 fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, int32_t epochs,
                           uint64_t drivenVersion, uint64_t *recM, uint64_t *recF, hipStream_t st, int32_t *done,
-                          double *kernelMs) {
+                          double *kernelMs, double *phaseUs) {
     std::lock_guard<std::mutex> lock(grid_launch_mutex());
     const int32_t D = d.nslots - 1;
     const int32_t rnb = batch_la_shape_neurons(k);
@@ -807,6 +868,9 @@ fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, i
     if (!b.ctl) {
         FB_HIP(hipMalloc(&b.ctl, 4 * sizeof(int32_t)));
         FB_HIP(hipMalloc(&b.bar, sizeof(int32_t) * GBAR_WORDS));
+        FB_HIP(hipMalloc(&b.prof, sizeof(uint64_t) * 2 * LA_NPH));
+        FB_HIP(hipMalloc(&b.dspk, sizeof(uint32_t) * (LS_MAX / MPG) * BDM * (BDRV / 4)));
+        FB_HIP(hipMalloc(&b.dflag, sizeof(int32_t) * (LS_MAX / MPG)));
         FB_HIP(hipEventCreate(&b.ev0));
         FB_HIP(hipEventCreate(&b.ev1));
     }
@@ -825,13 +889,16 @@ fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, i
     FB_HIP(hipMemsetAsync(b.dstCnt, 0, 4 * sizeof(int32_t) * S, st));
     FB_HIP(hipMemsetAsync(b.ctl, 0, 4 * sizeof(int32_t), st));
     FB_HIP(hipMemsetAsync(b.bar, 0, sizeof(int32_t) * GBAR_WORDS, st));
+    FB_HIP(hipMemsetAsync(b.prof, 0, sizeof(uint64_t) * 2 * LA_NPH, st));
+    FB_HIP(hipMemsetAsync(b.dflag, 0, sizeof(int32_t) * (LS_MAX / MPG), st));
     LaArgs a;
     a.d = d;
     a.t0 = t0; a.epochs = epochs; a.D = D; a.R = R; a.G = G; a.S = S; a.head0 = head0;
     a.nDrv = d.nDriven;
     a.per = std::max(1, (d.nDriven + NSL - 1) / NSL);
     a.evKey = b.evKey; a.evMask = b.evMask; a.evAux = b.evAux; a.dstCnt = b.dstCnt; a.addCnt = b.dstCnt + 2 * S;
-    a.eown = b.eown; a.ctl = b.ctl; a.bar = b.bar; a.recM = recM; a.recF = recF;
+    a.eown = b.eown; a.ctl = b.ctl; a.bar = b.bar; a.recM = recM; a.recF = recF; a.prof = b.prof;
+    a.dspk = b.dspk; a.dflag = b.dflag;
     void *args[] = {&a};
     FB_HIP(hipEventRecord(b.ev0, st));
     const hipError_t le = hipLaunchCooperativeKernel(la_kernel(k), dim3(S), dim3(BT), args, 0, st);
@@ -843,11 +910,24 @@ fb_status batch_la_launch(const BatchDev &d, int k, int32_t t0, int32_t head0, i
     }
     FB_HIP(hipEventRecord(b.ev1, st));
     int32_t ctl[4];
+    uint64_t prof[2 * LA_NPH];
     FB_HIP(hipMemcpyAsync(ctl, b.ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(prof, b.prof, sizeof prof, hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     FB_HIP(hipEventElapsedTime(&ms, b.ev0, b.ev1));
     if (kernelMs) *kernelMs += ms;
+    if (phaseUs) {
+        static double tickHz = 0.0;
+        if (tickHz == 0.0) {
+            int dev = 0, khz = 0;
+            tickHz = hipGetDevice(&dev) == hipSuccess &&
+                             hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0
+                         ? khz * 1000.0
+                         : 1e8;
+        }
+        for (int k = 0; k < 2 * LA_NPH; k++) phaseUs[k] += (double)prof[k] / tickHz * 1e6;
+    }
     if (ctl[1] & GBAR_TIMED_OUT) {
         set_error("ensemble lookahead: a grid barrier timed out; the ensemble state is undefined");
         return FB_ERR_DEVICE;
