@@ -959,7 +959,7 @@ def main():
     kms = brain.stat("persistent_kernel_ms") - k0
     phases = {f"{blk}_{ph}": brain.stat(f"phase_us_{blk}_{ph}")
               for blk in ("drv", "reg", "spare")
-              for ph in ("gather", "sim", "push", "lists", "barrier", "prologue", "share", "barrier2")}
+              for ph in ("gather", "sim", "push", "lists", "barrier", "prologue", "share", "pushwait")}
     phases.update({f"drvstep_{ph}": brain.stat(f"phase_us_drvstep_{ph}")
                    for ph in ("inputs", "scan", "draws", "decide", "membrane", "sync")})
     phases.update({f"share_{ph}": brain.stat(f"phase_us_share_{ph}")

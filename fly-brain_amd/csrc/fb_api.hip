@@ -986,7 +986,7 @@ extern "C" fb_status fb_brain_stat(fb_brain *b, const char *name, double *value)
         // phase_us_<drv|reg|spare>_<phase>: time the driven block, the first
         // regular block and the first spare block spent in each phase of the
         // lookahead kernel
-        static const char *ph[NPH] = {"gather", "sim", "push", "lists", "barrier", "prologue", "share", "barrier2"};
+        static const char *ph[NPH] = {"gather", "sim", "push", "lists", "barrier", "prologue", "share", "pushwait"};
         static const char *blkName[3] = {"drv_", "reg_", "spare_"};
         int idx = -1;
         for (int blk = 0; blk < 3; blk++)

@@ -216,7 +216,7 @@ __device__ __forceinline__ bool grid_barrier(int32_t *ctr, int32_t *flags) {
 // block out's final write, the last end of a driven, a regular and a spare
 // block; the first is stored complemented so that every slot is an atomic
 // maximum over a zeroed word).
-constexpr int NPH = 8;   // gather, sim, push, lists, barrier, prologue, share, barrier after the share
+constexpr int NPH = 8;   // gather, sim, push, lists, barrier, prologue, share, the wait for the pushes before the share
 constexpr int PROF_SUB = 3 * NPH;
 constexpr int PROF_SHARE = PROF_SUB + 6;
 constexpr int PROF_SPAN = PROF_SHARE + 4;

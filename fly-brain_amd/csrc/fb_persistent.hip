@@ -94,6 +94,7 @@ constexpr int SHQ = 256;            // driven neurons per share block
 constexpr int LONG_ROW = 1024;      // a regular block's rows longer than this are pushed by the whole grid
 constexpr int LR_CAP = SPK_CAP;     // long rows per epoch in the shared list (more stay with their block)
 constexpr int RUN_LIST = 1024;      // (step, target) runs a gather can list for ordering by a wave
+constexpr int LR_WORDS = 64 + GBAR_WORDS + DRV_BLOCKS;   // lrCnt[2], then from word 64 PersistArgs::epc
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -136,6 +137,11 @@ struct PersistArgs {
     int32_t *bar;        // grid barrier counters (GBAR_WORDS, zero at launch)
     uint64_t *xc;        // [DRV_BLOCKS][XC_RING] (step + 1) << 32 | non-refractory count, zero at launch
     int32_t *lrCnt;      // [2] long rows queued by the epochs of each parity (zero at launch)
+    int32_t *epc;        // regular workgroups whose pushes are done, summed over the launch's epochs and
+                         // counted as grid_barrier counts arrivals: [64 (g + 1)] the regular workgroups of
+                         // group g (GBAR_GROUP a group), [0] the groups all of whose regular workgroups
+                         // have; [GBAR_WORDS + k] 1 + the last epoch whose masks driven workgroup k wrote
+                         // (lrCnt + 64; zero at launch)
     int32_t *lrI;        // [2][LR_CAP] a queued long row: neuron, step << 24 | arrival key, sign x res at arrival
     uint32_t *lrK;
     float *lrR;
@@ -319,8 +325,8 @@ __device__ __forceinline__ int64_t bucket_at(const PersistArgs &a, int32_t par, 
 // sign x res in s.spkR[r]); a row whose sign x res is 0 adds nothing.  With
 // `count`, adds the rows' edge and synapse counts to s.ev / s.syn.  With
 // lpar >= 0, a row longer than LONG_ROW goes to the shared list of parity
-// lpar instead (while it has room) and adds nothing here: after the grid
-// barrier every block pushes a slice of those rows (push_long_share).
+// lpar instead (while it has room) and adds nothing here: once every regular
+// block has pushed, every block pushes a slice of those rows (push_long_share).
 // Returns the total.
 __device__ __forceinline__ int32_t row_scan(const PersistArgs &a, Shared &s, int32_t ns, bool count = true,
                                    int32_t lpar = -1) {
@@ -438,7 +444,7 @@ __device__ __forceinline__ void push_rows(const PersistArgs &a, Shared &s, int32
 __device__ __forceinline__ void push_long_share(const PersistArgs &a, Shared &s, int32_t par, int32_t lpar) {
     const int tid = threadIdx.x;
     const int32_t n = min(__hip_atomic_load(&a.lrCnt[lpar], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), LR_CAP);
-    if (n == 0) return;   // the same value in every block after the grid barrier
+    if (n == 0) return;   // the same value in every block once every regular block has pushed
     __syncthreads();      // the LDS push list may still be read by this block's driven share push
     for (int32_t r = tid; r < n; r += PT) {
         const int64_t at = (int64_t)lpar * LR_CAP + r;
@@ -996,8 +1002,8 @@ __device__ __forceinline__ void push_spikes(const PersistArgs &a, Shared &s, Cel
 
 // The driven block's rows are many (every forced spike), so the first
 // SHARE_BLOCKS blocks push them, each for a fixed range of driven-list
-// positions whose rows it holds in LDS (share_setup).  After the grid
-// barrier that ends the epoch's simulation, a share block reads the spike
+// positions whose rows it holds in LDS (share_setup).  Once the driven blocks
+// have published the epoch (PersistArgs::epc), a share block reads the spike
 // steps of its neurons (dmask: forced, tmask: threshold, as the driven block
 // wrote them) and their res, and writes their rows' events.
 __device__ __forceinline__ void share_setup(const PersistArgs &a, Shared &s) {
@@ -1311,8 +1317,8 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             // gather (which only reads), so the load overlaps the gather's
             const int32_t stop = e > 0 ? __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             // the long-row list of the other parity was last read before the
-            // previous epoch's second barrier and is next written after this
-            // epoch's: clear it now
+            // previous epoch's barrier and is next written in the next epoch:
+            // clear it now
             if (me == 0 && tid == 0) (void)atomicExch(&a.lrCnt[par ^ 1], 0);
             if (e > 0) build_lists(a, s, par ^ 1, done - prevL, prevL);
             FB_PHASE(3)
@@ -1328,24 +1334,67 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 else if (drivenBlock) sim_driven(a, s, cells, pFire, nev, par, L, rng);
                 else sim_regular(a, s, cells, nev, par, L, lo);
                 FB_PHASE(1)
-                if (done + L < a.nsteps && !drivenBlock) push_spikes(a, s, cells, par ^ 1, par);
+                const bool more = done + L < a.nsteps;
+                if (more && !drivenBlock) push_spikes(a, s, cells, par ^ 1, par);
 #pragma unroll
                 for (int u = 0; u < CELLS; u++) {
                     cells[u].prev = cells[u].mask;
                     cells[u].mask = 0;
                     cells[u].forced = 0;
                 }
+                if (more) {
+                    // a regular workgroup's pushes (and long rows queued) are
+                    // done; a driven workgroup's masks of the epoch are written
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                    if (tid == 0) {
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                        if (drivenBlock) {
+                            __hip_atomic_store(&a.epc[GBAR_WORDS + me], e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else {
+                            // the last of its group's regular workgroups to
+                            // arrive counts the group
+                            const int32_t g = me / GBAR_GROUP;
+                            const int32_t nr = min((g + 1) * GBAR_GROUP, a.S) - max(g * GBAR_GROUP, a.G);
+                            const int32_t arr = atomicAdd(&a.epc[64 * (g + 1)], 1);
+                            if (arr % nr == nr - 1) atomicAdd(&a.epc[0], 1);
+                        }
+                    }
+                }
                 FB_PHASE(2)
             }
-            if (!grid_barrier(a.bar, &a.ctl[1])) return;
-            FB_PHASE(4)
             if (done + L < a.nsteps) {
+                // the driven rows and the long rows go out before the epoch's
+                // one grid barrier: every workgroup waits for all regular
+                // workgroups' pushes (the long rows are queued by then), a
+                // share workgroup also for the driven workgroups' masks
+                if (tid == 0) {
+                    const uint64_t t0 = wall_clock64();
+                    const bool share = me < min((int32_t)gridDim.x, SHARE_BLOCKS);
+                    // groups that hold regular workgroups (those are G .. S-1)
+                    const int32_t ng = a.S > a.G ? (a.S - 1) / GBAR_GROUP - a.G / GBAR_GROUP + 1 : 0;
+                    bool late = false;
+                    while (!late) {
+                        bool ready = __hip_atomic_load(&a.epc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (e + 1) * ng;
+                        for (int32_t k = 0; share && ready && k < a.G; k++)
+                            ready = __hip_atomic_load(&a.epc[GBAR_WORDS + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= e + 1;
+                        if (ready) break;
+                        __builtin_amdgcn_s_sleep(1);
+                        if (wall_clock64() - t0 > GBAR_TIMEOUT) {
+                            atomicOr(&a.ctl[1], GBAR_TIMED_OUT);
+                            late = true;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                }
+                __syncthreads();
+                FB_PHASE(7)
                 push_driven_share(a, s, par ^ 1, par);
                 push_long_share(a, s, par ^ 1, par);
                 FB_PHASE(6)
-                if (!grid_barrier(a.bar, &a.ctl[1])) return;
-                FB_PHASE(7)
             }
+            if (!grid_barrier(a.bar, &a.ctl[1])) return;
+            FB_PHASE(4)
             done += L;
             prevL = L;
             e++;
@@ -1444,7 +1493,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
         __threadfence();
         for (int32_t k = tid; k < GBAR_WORDS; k += PT) a.bar[k] = 0;
         if (tid < DRV_BLOCKS * XC_RING) a.xc[tid] = 0;
-        if (tid < 2) a.lrCnt[tid] = 0;
+        for (int32_t k = tid; k < LR_WORDS; k += PT) a.lrCnt[k] = 0;
         if (tid < PERSIST_PROF) {
             unsigned long long *pk = (unsigned long long *)&a.prof[tid];
             a.hostProf[tid] = tid == PROF_SPAN + 3 ? (uint64_t)wall_clock64()
@@ -1737,7 +1786,7 @@ __global__ void k_persist_clear(int32_t *ctl, int32_t *bar, uint64_t *prof, uint
     if (i < 4) ctl[i] = 0;
     if (i < PERSIST_PROF) prof[i] = 0;
     if (i < DRV_BLOCKS * XC_RING) xc[i] = 0;
-    if (i < 2) lrCnt[i] = 0;
+    for (int32_t k = i; k < LR_WORDS; k += n) lrCnt[k] = 0;
     for (int32_t k = i; k < GBAR_WORDS; k += n) bar[k] = 0;
     for (int32_t k = i; k < nCnt; k += n) bkCnt[k] = 0;
 }
@@ -1815,7 +1864,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.bar, sizeof(int32_t) * GBAR_WORDS));
         FB_HIP(hipMalloc(&pb.xc, sizeof(uint64_t) * DRV_BLOCKS * XC_RING));
         FB_HIP(hipMalloc(&pb.bstart, sizeof(int32_t) * (S_MAX + 1)));
-        FB_HIP(hipMalloc(&pb.lrCnt, sizeof(int32_t) * 2));
+        FB_HIP(hipMalloc(&pb.lrCnt, sizeof(int32_t) * LR_WORDS));
         FB_HIP(hipMalloc(&pb.dstage, sizeof(float) * DSTAGE_F * DRV_CAP));
         FB_HIP(hipMalloc(&pb.lrI, sizeof(int32_t) * 2 * LR_CAP));
         FB_HIP(hipMalloc(&pb.lrK, sizeof(uint32_t) * 2 * LR_CAP));
@@ -1877,6 +1926,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.bar = pb.bar;
     a.xc = pb.xc;
     a.lrCnt = pb.lrCnt;
+    a.epc = pb.lrCnt + 64;
     a.lrI = pb.lrI;
     a.lrK = pb.lrK;
     a.lrR = pb.lrR;
