@@ -38,6 +38,11 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
                             int64_t *launches, double *kernelMs, double *phaseUs, uint64_t drivenVersion);
 void persistent_release(const int32_t *key);
 void persistent_take_time(const int32_t *key, double *kernelMs);
+// The lookahead engine leaves the driven neurons' state in its stage between
+// launches: brings the brain's arrays up to date, and drops the stage's copy
+// of what the caller changes next (state: the neurons' state arrays; params:
+// bias, thr, drive, the driven list, the constants).
+fb_status persistent_sync(const DevBrain &d, hipStream_t st, bool state, bool params);
 
 // ------------------------------------------------------------------ jump tables
 
@@ -407,6 +412,12 @@ static fb_status sync_driven(fb_brain *b) {
     return FB_OK;
 }
 
+// Every entry point that reads or writes a brain's arrays, or changes what
+// the lookahead engine stages, calls this first.
+static fb_status sync_stage(fb_brain *b, bool state, bool params) {
+    return persistent_sync(b->d, b->stream, state, params);
+}
+
 static float *array_ptr(fb_brain *b, int32_t which) {
     switch (which) {
         case FB_V: return b->d.v;
@@ -529,6 +540,8 @@ extern "C" fb_status fb_brain_destroy(fb_brain *b) {
 
 extern "C" fb_status fb_reset(fb_brain *b) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_reset: null brain");
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     b->gENonneg = true;
     hipLaunchKernelGGL(k_reset_state, dim3((std::max(b->d.N, b->d.nslots) + 255) / 256), dim3(256), 0, b->stream,
                        b->d);
@@ -544,6 +557,8 @@ extern "C" fb_status fb_reset(fb_brain *b) {
 extern "C" fb_status fb_set_params(fb_brain *b, const fb_params *p) {
     FB_REQUIRE(b && p, FB_ERR_ARG, "fb_set_params: null argument");
     FB_REQUIRE(p->dt > 0 && p->tauM > 0 && p->tauSyn > 0, FB_ERR_ARG, "fb_set_params: dt, tauM, tauSyn must be > 0");
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     FB_HIP(hipStreamSynchronize(b->stream));
     const int32_t ns = nslots_of(*p);
     b->p = *p;
@@ -571,6 +586,8 @@ static fb_status set_values(fb_brain *b, float *arr, const int32_t *idx, int32_t
     for (int32_t q = 0; q < n; q++)
         FB_REQUIRE(idx[q] >= 0 && idx[q] < b->d.N, FB_ERR_ARG, "setter: index %d out of range", idx[q]);
     if (n == 0) return FB_OK;
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     if (n == 1) {
         hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, b->stream, arr, (const int32_t *)nullptr, 1, idx[0], value,
                            mode, 0);
@@ -710,6 +727,7 @@ extern "C" fb_status fb_apply_deltas(fb_brain *b, int32_t n, const int32_t *idx,
                                      const double *val) {
     fb_status st0 = deltas_host_side(b, n, idx, kind, val, "fb_apply_deltas");
     if (st0 || n == 0) return st0;
+    if ((st0 = sync_stage(b, true, true))) return st0;
     // group by (array, neuron), keeping queue order inside a group
     auto arrayOf = delta_array_of;
     std::vector<int32_t> perm((size_t)n);
@@ -777,6 +795,7 @@ extern "C" fb_status fb_apply_deltas_noalloc(fb_brain *b, int32_t n, const int32
     std::lock_guard<std::mutex> lock(g_noallocMu);
     fb_status st0 = deltas_host_side(b, n, idx, kind, val, "fb_apply_deltas_noalloc");
     if (st0 || n == 0) return st0;
+    if ((st0 = sync_stage(b, true, true))) return st0;   // a kernel launch at most, no allocation
     const int dev = b->g->device;
     FB_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, FB_ERR_UNSUPPORTED, "fb_apply_deltas_noalloc: device %d", dev);
     if (!g_scratchPtr[dev]) {
@@ -821,6 +840,8 @@ extern "C" fb_status fb_apply_deltas_noalloc(fb_brain *b, int32_t n, const int32
 
 extern "C" fb_status fb_set_background(fb_brain *b, double hz, double mv) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_set_background: null brain");
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     b->p.bgRate = hz;
     b->p.bgAmp = mv;
     b->d.k.bgEvents = (float)((double)b->d.N * hz * b->p.dt / 1000.0);
@@ -931,6 +952,11 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
                  : b->d.N >= 65536                                                       ? FB_ENGINE_LAUNCH
                                                                                           : FB_ENGINE_GRAPH;
     }
+    if (engine != FB_ENGINE_PERSISTENT) {
+        // the per-step engines step the brain's arrays
+        st = sync_stage(b, true, false);
+        if (st) return st;
+    }
     switch (engine) {
         case FB_ENGINE_LAUNCH: st = run_launch(b, nsteps); break;
         case FB_ENGINE_GRAPH: st = run_graph(b, nsteps); break;
@@ -942,6 +968,8 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
                                    &b->persistentMs, b->phaseUs, b->drivenVersion);
             if (st == FB_ERR_UNSUPPORTED && autoChosen) {
                 // the device refused the launch before any step ran
+                st = sync_stage(b, true, false);
+                if (st) return st;
                 st = b->d.N >= 65536 ? run_launch(b, nsteps) : run_graph(b, nsteps);
                 break;
             }
@@ -1083,6 +1111,8 @@ extern "C" fb_status fb_get_array(fb_brain *b, int32_t which, void *dst) {
     FB_REQUIRE(b && dst, FB_ERR_ARG, "fb_get_array: null argument");
     float *src = array_ptr(b, which);
     FB_REQUIRE(src, FB_ERR_ARG, "fb_get_array: unknown array %d", which);
+    fb_status s0 = sync_stage(b, false, false);
+    if (s0) return s0;
     FB_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)b->d.N, hipMemcpyDeviceToHost, b->stream));
     FB_HIP(hipStreamSynchronize(b->stream));
     return FB_OK;
@@ -1092,6 +1122,8 @@ extern "C" fb_status fb_set_array(fb_brain *b, int32_t which, const void *src) {
     FB_REQUIRE(b && src, FB_ERR_ARG, "fb_set_array: null argument");
     float *dst = array_ptr(b, which);
     FB_REQUIRE(dst, FB_ERR_ARG, "fb_set_array: unknown array %d", which);
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     if (dst == b->d.gE) {
         const float *v = (const float *)src;
         bool nonneg = true;
@@ -1123,6 +1155,8 @@ extern "C" fb_status fb_set_pool_feedback(fb_brain *b, const uint8_t *poolOf, in
                                           int32_t nTargets, float base, float gain, int32_t delay,
                                           int32_t recordCap) {
     FB_REQUIRE(b, FB_ERR_ARG, "fb_set_pool_feedback: null brain");
+    fb_status s0 = sync_stage(b, true, true);
+    if (s0) return s0;
     FB_HIP(hipStreamSynchronize(b->stream));
     if (b->d.poolOf) {                 // the previous loop's buffers
         DevBrain &o = b->d;

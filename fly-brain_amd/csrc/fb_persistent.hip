@@ -95,6 +95,9 @@ constexpr int LONG_ROW = 1024;      // a regular block's rows longer than this a
 constexpr int LR_CAP = SPK_CAP;     // long rows per epoch in the shared list (more stay with their block)
 constexpr int RUN_LIST = 1024;      // (step, target) runs a gather can list for ordering by a wave
 constexpr int LR_WORDS = 64 + GBAR_WORDS + DRV_BLOCKS;   // lrCnt[2], then from word 64 PersistArgs::epc
+// PersistArgs::flags: the stage already holds the driven neurons' state (the
+// last launch left it there) / their bias, thr, sign, drive and index
+constexpr int32_t STAGE_STATE = 2, STAGE_PARAMS = 4;
 
 // event key: step(5) | local target(12) | class(1) | arrival key(20) | position in row(26).
 // Sorted, the events of one (step, target) are contiguous, excitatory first,
@@ -125,8 +128,7 @@ struct PersistArgs {
     uint64_t *bkKey;     // [2][S][EV_CAP] each destination's bucket of the next epoch's events
     float *bkAdd;
     int32_t *bkCnt;      // [2][S] slots taken, one per BK_STRIDE words (zero at launch; a destination clears its own after reading)
-    int32_t *ctl;        // [0] steps done, [1] overflow flags, [2] blocks that have left, [3] driven blocks
-                         // whose final state is in the stage (zero at launch)
+    int32_t *ctl;        // [0] steps done, [1] overflow flags, [2] blocks that have left (zero at launch)
     int32_t *hostCtl;    // pinned host memory: [0] steps done, [1] flags, [2] tag, from the last block out
     uint64_t *hostProf;  // pinned host memory: the phase clocks, from the last block out
     int32_t tag;         // this launch's number
@@ -145,7 +147,7 @@ struct PersistArgs {
     int32_t *lrI;        // [2][LR_CAP] a queued long row: neuron, step << 24 | arrival key, sign x res at arrival
     uint32_t *lrK;
     float *lrR;
-    int32_t flags;       // bit 0: time the driven block's step parts
+    int32_t flags;       // bit 0: time the driven block's step parts; STAGE_STATE, STAGE_PARAMS
 };
 
 struct Shared {
@@ -496,7 +498,13 @@ __device__ __forceinline__ void prologue_scatter(const PersistArgs &a, Shared &s
             while (s.pref[m + 1] <= g) m++;
             const int32_t k = g - s.pref[m];
             const int32_t pre = d.ring[(int64_t)head_at(a, m) * d.ringStride + k];
-            const float sv = d.g.sign[pre] * res_after(d.k, d.res[pre], m);
+            // a driven neuron's res is the stage's while the stage holds its state
+            float res = d.res[pre];
+            if (a.flags & STAGE_STATE) {
+                const int32_t q = a.drvPos[pre];
+                if (q >= 0) res = a.dstage[6 * DRV_CAP + q];
+            }
+            const float sv = d.g.sign[pre] * res_after(d.k, res, m);
             if (sv != 0.f && d.k.depU != 0.f) atomicOr(&a.arrMask[pre], 1u << m);
             s.spkI[r] = pre;
             s.spkK[r] = ((uint32_t)m << 24) | (uint32_t)k;
@@ -657,9 +665,8 @@ __device__ __forceinline__ void cell_inputs(const Consts &k, Cell &c, const Shar
 }
 
 // A driven block's final cell state -> the stage (coalesced) at the launch's
-// end; the regular blocks then write a share of it each to the brain's arrays
-// (unstage_driven).  One workgroup storing ~1,650 scattered neurons'
-// eight fields took ~25 us at a launch's end.
+// end, where the next lookahead launch finds it (k_unstage writes it to the
+// brain's arrays when anything else needs them).
 __device__ __forceinline__ void stage_out(const PersistArgs &a, const Cell (&cells)[CELLS], int32_t qb, int32_t P) {
     float *st = a.dstage;
     const int tid = threadIdx.x;
@@ -1169,34 +1176,49 @@ __device__ __forceinline__ void build_lists(const PersistArgs &a, Shared &s, int
 
 // ------------------------------------------------------------------ the driven neurons' state
 
-// Fields of the driven stage: the cell state, then bias, threshold, sign and
-// drive, then the neuron index (as bits); at the launch's end the driven
-// blocks put their cells' final state in fields 0-6 and the spike-count
-// increment (as bits) in field 12.
+// Fields of the driven stage, in driven-list order: the cell state (0-6),
+// then bias, threshold, sign and drive, then the neuron index (as bits), then
+// the spike-count increments the brain's spikeCount does not hold yet (as
+// bits).  At a launch's end the driven blocks leave their cells' state and
+// increments there, and the stage stays the driven neurons' state between
+// lookahead launches: the brain's arrays are brought up to date (k_unstage)
+// only when something else reads or writes them.  So a run of lookahead
+// calls pays no hand-over at either end: ~1,650 scattered neurons' eight
+// fields cost a launch's end 10-15 us even spread over the regular blocks,
+// each dependent global load there taking 3-5 us.
 constexpr int DSTAGE_F = 13;
 
 // Every block but the driven ones copies a share of the driven neurons' state
-// into the stage in driven-list order (block b of nb: positions b*PT + tid,
-// b*PT + tid + nb*PT, ...).  Read by one workgroup, those scattered lines
-// take ~20 us at a launch's start; spread over the grid they are one round
-// trip, and the driven blocks read the stage coalesced after the barrier.
+// and parameters into the stage, the parts the flags do not mark as there
+// already (block b of nb: positions b*PT + tid, b*PT + tid + nb*PT, ...).
+// Read by one workgroup, those scattered lines take ~20 us at a launch's
+// start; spread over the grid they are one round trip, and the driven blocks
+// read the stage coalesced after the barrier (before it, when the stage is
+// whole already).
 __device__ __forceinline__ void stage_driven(const PersistArgs &a, int32_t b, int32_t nb) {
     const DevBrain &d = a.d;
     float *st = a.dstage;
+    const bool state = !(a.flags & STAGE_STATE), params = !(a.flags & STAGE_PARAMS);
+    if (!state && !params) return;
     for (int32_t q = b * PT + (int32_t)threadIdx.x; q < d.nDriven; q += nb * PT) {
         const int32_t i = d.driven[q];
-        st[0 * DRV_CAP + q] = d.v[i];
-        st[1 * DRV_CAP + q] = d.gE[i];
-        st[2 * DRV_CAP + q] = d.gI[i];
-        st[3 * DRV_CAP + q] = d.refr[i];
-        st[4 * DRV_CAP + q] = d.trace[i];
-        st[5 * DRV_CAP + q] = d.adapt[i];
-        st[6 * DRV_CAP + q] = d.res[i];
-        st[7 * DRV_CAP + q] = d.bias[i];
-        st[8 * DRV_CAP + q] = d.thr[i];
-        st[9 * DRV_CAP + q] = d.g.sign[i];
-        st[10 * DRV_CAP + q] = d.drive[i];
-        st[11 * DRV_CAP + q] = __int_as_float(i);
+        if (state) {
+            st[0 * DRV_CAP + q] = d.v[i];
+            st[1 * DRV_CAP + q] = d.gE[i];
+            st[2 * DRV_CAP + q] = d.gI[i];
+            st[3 * DRV_CAP + q] = d.refr[i];
+            st[4 * DRV_CAP + q] = d.trace[i];
+            st[5 * DRV_CAP + q] = d.adapt[i];
+            st[6 * DRV_CAP + q] = d.res[i];
+            st[12 * DRV_CAP + q] = 0.f;
+        }
+        if (params) {
+            st[7 * DRV_CAP + q] = d.bias[i];
+            st[8 * DRV_CAP + q] = d.thr[i];
+            st[9 * DRV_CAP + q] = d.g.sign[i];
+            st[10 * DRV_CAP + q] = d.drive[i];
+            st[11 * DRV_CAP + q] = __int_as_float(i);
+        }
     }
 }
 
@@ -1204,7 +1226,8 @@ __device__ __forceinline__ void stage_driven(const PersistArgs &a, int32_t b, in
 __device__ __forceinline__ void load_staged_cell(const PersistArgs &a, Cell &c, float &pFire, int32_t q) {
     const float *st = a.dstage;
     const int32_t j = q < 0 ? 0 : q;
-    c.count = 0; c.mask = 0; c.prev = 0; c.forced = 0;
+    c.mask = 0; c.prev = 0; c.forced = 0;
+    c.count = __float_as_uint(st[12 * DRV_CAP + j]);   // increments the brain's spikeCount lacks
     c.st.v = st[0 * DRV_CAP + j]; c.st.gE = st[1 * DRV_CAP + j]; c.st.gI = st[2 * DRV_CAP + j];
     c.st.refr = st[3 * DRV_CAP + j]; c.st.trace = st[4 * DRV_CAP + j]; c.st.adapt = st[5 * DRV_CAP + j];
     c.st.res = st[6 * DRV_CAP + j];
@@ -1212,20 +1235,6 @@ __device__ __forceinline__ void load_staged_cell(const PersistArgs &a, Cell &c, 
     const float dr = st[10 * DRV_CAP + j];
     c.i = q < 0 ? -1 : __float_as_int(st[11 * DRV_CAP + j]);
     pFire = q < 0 ? 0.f : dr * a.d.k.dtS;
-}
-
-// A share of the stage (stage_out) -> the brain's arrays: block b of nb
-// writes positions b*PT + tid, b*PT + tid + nb*PT, ... (as stage_driven).
-__device__ __forceinline__ void unstage_driven(const PersistArgs &a, int32_t b, int32_t nb) {
-    const DevBrain &d = a.d;
-    const float *st = a.dstage;
-    for (int32_t q = b * PT + (int32_t)threadIdx.x; q < d.nDriven; q += nb * PT) {
-        const int32_t i = __float_as_int(st[11 * DRV_CAP + q]);
-        d.v[i] = st[0 * DRV_CAP + q]; d.gE[i] = st[1 * DRV_CAP + q]; d.gI[i] = st[2 * DRV_CAP + q];
-        d.refr[i] = st[3 * DRV_CAP + q]; d.trace[i] = st[4 * DRV_CAP + q]; d.adapt[i] = st[5 * DRV_CAP + q];
-        d.res[i] = st[6 * DRV_CAP + q];
-        d.spikeCount[i] += __float_as_uint(st[12 * DRV_CAP + q]);
-    }
 }
 
 // ------------------------------------------------------------------ the kernel
@@ -1264,10 +1273,23 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     uint32_t xNr = 0;      // several driven blocks: the next step's flags and offset (sim_driven_x)
     int32_t xOff = 0;
     bool xDead = false;
+    // the driven cells come from the stage: at once when the last launch left
+    // it whole, else after the barrier that ends the other blocks' staging
+    const bool stagedEarly = (a.flags & (STAGE_STATE | STAGE_PARAMS)) == (STAGE_STATE | STAGE_PARAMS);
+    auto loadDriven = [&]() {
+        const int32_t qb = me * a.Q;
+        const int32_t nDk = min(a.Q, d.nDriven - qb);
+        const int32_t P = (nDk + PT - 1) / PT;
+#pragma unroll
+        for (int u = 0; u < DPT; u++) {
+            const int32_t q = tid * P + u;
+            load_staged_cell(a, cells[u], pFire[u], u < P && q < nDk ? qb + q : -1);
+        }
+    };
     if (drivenBlock) {
-        // the driven cells come from the stage the other blocks fill (after the first barrier)
         leap_load<PT, LEAP_PERSIST_MATS>(s.leap, d.leap);
         rng = d.sc->rng;
+        if (stagedEarly) loadDriven();
     } else {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) {
@@ -1285,14 +1307,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     if (!grid_barrier(a.bar, &a.ctl[1])) return;
     FB_PHASE(4)
     if (drivenBlock) {
-        const int32_t qb = me * a.Q;
-        const int32_t nDk = min(a.Q, d.nDriven - qb);
-        const int32_t P = (nDk + PT - 1) / PT;
-#pragma unroll
-        for (int u = 0; u < DPT; u++) {
-            const int32_t q = tid * P + u;
-            load_staged_cell(a, cells[u], pFire[u], u < P && q < nDk ? qb + q : -1);
-        }
+        if (!stagedEarly) loadDriven();
         if (a.G > 1) {
             // the first step's count
             xNr = driven_flags(cells);
@@ -1400,44 +1415,15 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             e++;
         }
     }
-    // the last epoch's lists, and the state back to memory.  After a complete
-    // run the driven blocks hand their cells over through the stage: each
-    // writes it and counts itself in ctl[3]; the regular blocks (which are
-    // done first) wait for all G and write a share each.  After a stop the
-    // driven blocks store their cells.
-    const bool staged = e > 0 && !listsBuilt;
-    if (staged) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
+    // the last epoch's lists, and the state back: the regular cells to the
+    // brain's arrays, the driven cells to the stage (also after a stop)
+    if (e > 0 && !listsBuilt) build_lists(a, s, (e - 1) & 1, done - prevL, prevL);
     FB_PHASE(3)
-    if (drivenBlock && staged) {
+    if (drivenBlock) {
         stage_out(a, cells, me * a.Q, (min(a.Q, d.nDriven - me * a.Q) + PT - 1) / PT);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            atomicAdd(&a.ctl[3], 1);
-        }
     } else if (owning) {
 #pragma unroll
         for (int u = 0; u < CELLS; u++) store_cell(d, cells[u]);
-    }
-    if (staged) {
-        const int32_t first = a.S > a.G ? a.G : a.S;
-        const int32_t count = a.S > a.G ? a.S - a.G : max(1, a.nSpare);
-        if (me >= first && me < first + count) {
-            if (tid == 0) {
-                const uint64_t t0 = wall_clock64();
-                while (__hip_atomic_load(&a.ctl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.G) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (wall_clock64() - t0 > GBAR_TIMEOUT) {
-                        atomicOr(&a.ctl[1], GBAR_TIMED_OUT);
-                        break;
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            __syncthreads();
-            unstage_driven(a, me - first, count);
-        }
     }
     if (owning) {
         if (me == 0) {
@@ -1505,7 +1491,6 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
             a.hostCtl[1] = __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a.ctl[1] = 0;
             a.ctl[2] = 0;
-            a.ctl[3] = 0;
         }
         __syncthreads();
         __threadfence_system();
@@ -1676,6 +1661,11 @@ struct PersistBuffers {
     float *lrR = nullptr;
     uint32_t overflowSeen = 0;       // OR of the overflow flags of every launch (2 events, 4 spikes)
     float *dstage = nullptr;
+    // The stage between launches (DSTAGE_F): its state fields hold the driven
+    // neurons' state, its parameter fields the driven list's parameters; the
+    // brain's arrays lag the stage until k_unstage runs; stageN neurons.
+    bool stageState = false, stageParams = false, arraysBehind = false;
+    int32_t stageN = 0;
     int32_t *hostCtl = nullptr;      // pinned: the last block's results (PersistArgs::hostCtl)
     uint64_t *hostProf = nullptr;
     int32_t tag = 0;                 // launches so far
@@ -1757,6 +1747,7 @@ void persistent_release(const int32_t *key) {
     PersistBuffers pb = std::move(it->second);
     g_pbuf.erase(it);
     pb.timerPending = false;
+    pb.stageState = pb.stageParams = pb.arraysBehind = false;   // the brain's state goes with it
     if (pb.spk && g_pool.size() < POOL_MAX) {
         pb.eownDriven = ~0ull;   // the next brain's driven set is its own
         pb.overflowSeen = 0;
@@ -1765,6 +1756,40 @@ void persistent_release(const int32_t *key) {
     }
     free_buffers(pb);
     if (pb.ev0) { (void)hipEventDestroy(pb.ev0); (void)hipEventDestroy(pb.ev1); }
+}
+
+// The stage's driven neurons -> the brain's arrays (state and spike counts);
+// the stage keeps the state, and its increments are cleared once added.
+__global__ void k_unstage(DevBrain d, float *st, int32_t n) {
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int32_t i = __float_as_int(st[11 * DRV_CAP + q]);
+    d.v[i] = st[0 * DRV_CAP + q]; d.gE[i] = st[1 * DRV_CAP + q]; d.gI[i] = st[2 * DRV_CAP + q];
+    d.refr[i] = st[3 * DRV_CAP + q]; d.trace[i] = st[4 * DRV_CAP + q]; d.adapt[i] = st[5 * DRV_CAP + q];
+    d.res[i] = st[6 * DRV_CAP + q];
+    const uint32_t inc = __float_as_uint(st[12 * DRV_CAP + q]);
+    if (inc) {
+        d.spikeCount[i] += inc;
+        st[12 * DRV_CAP + q] = 0.f;
+    }
+}
+
+static hipError_t unstage_locked(const DevBrain &d, PersistBuffers &pb, hipStream_t st) {
+    if (!pb.arraysBehind) return hipSuccess;
+    pb.arraysBehind = false;
+    if (pb.stageN > 0) hipLaunchKernelGGL(k_unstage, dim3((pb.stageN + 255) / 256), dim3(256), 0, st, d, pb.dstage, pb.stageN);
+    return hipGetLastError();
+}
+
+fb_status persistent_sync(const DevBrain &d, hipStream_t st, bool state, bool params) {
+    std::lock_guard<std::mutex> lock(grid_launch_mutex());
+    auto it = g_pbuf.find(d.driven);
+    if (it == g_pbuf.end()) return FB_OK;
+    PersistBuffers &pb = it->second;
+    FB_HIP(unstage_locked(d, pb, st));
+    if (state) pb.stageState = false;
+    if (params) pb.stageParams = false;
+    return FB_OK;
 }
 
 __global__ void k_drvpos(int32_t *drvPos, int32_t N) {
@@ -1838,6 +1863,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     }
     PersistBuffers &pb = g_pbuf[d.driven];
     if (pb.N != d.N || pb.S != lay.S || pb.E != d.g.E || pb.device != dev) {
+        if (pb.device == dev) FB_HIP(unstage_locked(d, pb, st));   // the stage goes with the buffers
+        pb.stageState = pb.stageParams = pb.arraysBehind = false;
         free_buffers(pb);
         pb.N = d.N;
         pb.S = lay.S;
@@ -1889,6 +1916,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         pb.eownG = -1;
     }
     if (pb.eownDriven != drivenVersion || pb.eownG != lay.G) {
+        // a new driven list or split: the stage's positions change
+        FB_HIP(unstage_locked(d, pb, st));
+        pb.stageState = pb.stageParams = false;
         hipLaunchKernelGGL(k_drvpos, dim3((d.N + 255) / 256), dim3(256), 0, st, pb.drvPos, d.N);
         if (d.nDriven)
             hipLaunchKernelGGL(k_drvpos_set, dim3((d.nDriven + 255) / 256), dim3(256), 0, st, pb.drvPos, d.driven,
@@ -1931,7 +1961,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.lrK = pb.lrK;
     a.lrR = pb.lrR;
     static const bool profileSteps = getenv("FB_PROFILE_DRIVEN_STEPS") != nullptr;
-    a.flags = profileSteps ? 1 : 0;
 
     a.hostCtl = pb.hostCtl;
     a.dstage = pb.dstage;
@@ -1942,6 +1971,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         a.head0 = cur;
         a.nsteps = nsteps - done;
         a.tag = ++pb.tag;
+        a.flags = (profileSteps ? 1 : 0) | (pb.stageState ? STAGE_STATE : 0) | (pb.stageParams ? STAGE_PARAMS : 0);
         if (pb.needClear) {
             hipLaunchKernelGGL(k_persist_clear, dim3(16), dim3(256), 0, st, pb.ctl, pb.bar, pb.prof, pb.xc, pb.lrCnt,
                                pb.bkCnt, 2 * lay.S * BK_STRIDE);
@@ -1964,6 +1994,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             set_error("persistent engine: launch refused: %s", hipGetErrorString(le));
             return done == 0 ? FB_ERR_UNSUPPORTED : FB_ERR_DEVICE;
         }
+        // the driven neurons' state is in the stage from now on
+        pb.stageState = pb.stageParams = pb.arraysBehind = true;
+        pb.stageN = d.nDriven;
         FB_HIP(hipEventRecord(pb.ev1, st));
         if (launches) (*launches)++;
         // Wait by polling: the last block out writes the tag to pinned memory
@@ -2020,7 +2053,10 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         if (ctl[1]) pb.needClear = true;
         cur = (cur + ctl[0]) % d.nslots;
         if (done < nsteps) {
-            // a bounded buffer overflowed: the next epoch runs on the per-step engine
+            // a bounded buffer overflowed: the next epoch runs on the per-step
+            // engine, on the brain's arrays
+            FB_HIP(unstage_locked(d, pb, st));
+            pb.stageState = false;
             if (d.k.depU != 0.f)
                 FB_HIP(hipMemsetAsync(pb.arrMask, 0, sizeof(uint32_t) * (size_t)d.N, st));
             const int32_t m = std::min(D, nsteps - done);
