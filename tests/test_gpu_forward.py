@@ -147,6 +147,52 @@ def test_persistent_multi_epoch_launches_match_oracle(name):
     assert gpu.stat("fallback_steps") == 0
 
 
+@pytest.mark.parametrize("name", ["c1", "dt05_dep_adapt"])
+def test_persistent_stage_between_calls_matches_oracle(name):
+    """Between lookahead calls the driven neurons' state lives in the
+    engine's stage.  Setters on driven neurons (pulse, addG), changes of the
+    driven set and a per-step engine between lookahead calls must find that
+    state and leave theirs for the next call.  Nothing is read back until the
+    end, so every write-back comes from the entry points themselves; then
+    the state, counts and RNG equal the oracle's."""
+    g, W, s, p, drv, rates = _multi_epoch_case(name)
+    gpu = _hip(g, W, s, p, 5, "persistent")
+    cpu = OracleBrain(g, p, seed=5, W=W, sign=s)
+    for b in (gpu, cpu):
+        for i, r in zip(drv, rates):
+            b.setDriveOne(int(i), float(r))
+    D = gpu.nslots - 1
+    d0 = [int(i) for i in drv[:8]]
+    fresh = [int(i) for i in np.setdiff1d(np.arange(g.N), drv)[:4]]
+
+    def both(f):
+        f(gpu)
+        f(cpu)
+
+    def run(n, engine=None):
+        print(f"stage case {name}: {n} steps on {engine or 'persistent'}", flush=True)
+        gpu.advance(n, engine)
+        cpu.run(n, raster_cap=n * g.N)
+
+    run(2 * D + 3)
+    run(D + 2)   # back to back: the second call starts from the stage
+    both(lambda b: b.pulse(d0, 3.0))
+    run(D)
+    both(lambda b: b.addG(d0[0], 0.5, 0.25))
+    for i in fresh:
+        both(lambda b: b.setDriveOne(i, 200.0))
+    both(lambda b: b.setDriveOne(d0[1], 0.0))
+    run(3 * D + 1)
+    run(5, "launch")
+    run(2 * D)
+    run(7)
+    for k in lifcases.STATE:
+        assert np.array_equal(getattr(gpu, k).view(np.uint32), cpu.array(k).view(np.uint32)), k
+    assert np.array_equal(gpu.spikeCount, cpu.spikeCount)
+    assert gpu.rng == cpu.rng
+    assert gpu.stat("fallback_steps") == 0
+
+
 @pytest.mark.parametrize("ndrv,dt05", [(2049, False), (3500, False), (6000, True), (8192, False)])
 def test_persistent_several_driven_blocks_match_oracle(ndrv, dt05):
     """More than 2,048 driven neurons: the lookahead engine splits the driven
