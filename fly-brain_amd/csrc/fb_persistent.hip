@@ -38,15 +38,17 @@
 //           barrier wait for the store to complete);
 //   push    walks the rows of its own spikes and writes their events into
 //           the destinations' buckets for epoch e+1 (a slot is one atomic on
-//           the destination's count).  After a grid barrier the first 64
-//           blocks push the driven block's rows the same way, then a second
-//           grid barrier ends the epoch.
+//           the destination's count).  Once every regular block has pushed
+//           and the driven blocks have written their spike masks, the 64
+//           blocks after the driven ones push the driven rows the same way,
+//           and one grid barrier ends the epoch.
 // The arrival key of a spike is its rank in lif.c's list order: a forced
 // spike's driven-list position, or 8192 + index for a threshold spike.
 //
-// A launch starts with a prologue that scatters the ring lists already in
-// the delay line (from the per-step engines or the previous launch) into the
-// buckets, keyed by list position.  Preconditions (checked on the host;
+// A launch starts with a prologue in which the regular blocks scatter the
+// ring lists already in the delay line (from the per-step engines or the
+// previous launch) into the buckets, keyed by list position, while the
+// driven blocks load their cells from the stage (DSTAGE_F).  Preconditions (checked on the host;
 // otherwise the per-step engines run): no background events, no presynaptic
 // inhibition, D <= 32, at most 8192 driven neurons and 1024 neurons per
 // block, and with synaptic depression live a refractory period of at least D
@@ -94,7 +96,7 @@ constexpr int SHQ = 256;            // driven neurons per share block
 constexpr int LONG_ROW = 1024;      // a regular block's rows longer than this are pushed by the whole grid
 constexpr int LR_CAP = SPK_CAP;     // long rows per epoch in the shared list (more stay with their block)
 constexpr int RUN_LIST = 1024;      // (step, target) runs a gather can list for ordering by a wave
-constexpr int LR_WORDS = 64 + GBAR_WORDS + DRV_BLOCKS;   // lrCnt[2], then from word 64 PersistArgs::epc
+constexpr int LR_WORDS = 64 + GBAR_WORDS + DRV_BLOCKS;   // lrCnt[2], pflag[2], then from word 64 epc
 // PersistArgs::flags: the stage already holds the driven neurons' state (the
 // last launch left it there) / their bias, thr, sign, drive and index
 constexpr int32_t STAGE_STATE = 2, STAGE_PARAMS = 4;
@@ -139,6 +141,7 @@ struct PersistArgs {
     int32_t *bar;        // grid barrier counters (GBAR_WORDS, zero at launch)
     uint64_t *xc;        // [DRV_BLOCKS][XC_RING] (step + 1) << 32 | non-refractory count, zero at launch
     int32_t *lrCnt;      // [2] long rows queued by the epochs of each parity (zero at launch)
+    int32_t *pflag;      // [2] overflow flags of the pushes into the buckets of each parity (lrCnt + 2)
     int32_t *epc;        // regular workgroups whose pushes are done, summed over the launch's epochs and
                          // counted as grid_barrier counts arrivals: [64 (g + 1)] the regular workgroups of
                          // group g (GBAR_GROUP a group), [0] the groups all of whose regular workgroups
@@ -382,6 +385,15 @@ __device__ __forceinline__ int32_t row_scan(const PersistArgs &a, Shared &s, int
 // flight; a scan of the row starts maps an edge to its row.  A slot is one
 // atomic on the destination's count; beyond EV_CAP the event is dropped and
 // flag 2 stops the launch before the epoch that would read it.
+// An overflow of a push into the buckets of parity par.  The host reads the
+// flags in ctl[1]; the blocks read pflag[par] at the top of the epoch that
+// gathers those buckets, a word no block writes during that epoch (its pushes
+// go to the other parity), so every block takes the same exit.
+__device__ __forceinline__ void flag_overflow(const PersistArgs &a, int32_t par, int32_t bit) {
+    atomicOr(&a.ctl[1], bit);
+    atomicOr(&a.pflag[par], bit);
+}
+
 __device__ __forceinline__ void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t g0, int32_t g1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
@@ -426,7 +438,7 @@ __device__ __forceinline__ void push_edges(const PersistArgs &a, Shared &s, int3
                 a.bkKey[at] = ev_key(kk >> 24, sv > 0 ? 0u : 1u, ql, kk & 0xFFFFFFu, j[v] - s.rowA[r]);
                 a.bkAdd[at] = __uint_as_float(ed[v].y) * sv;
             } else if (pos == EV_CAP) {
-                atomicOr(&a.ctl[1], 2);
+                flag_overflow(a, par, 2);
             }
         }
     }
@@ -488,12 +500,15 @@ __device__ __forceinline__ void prologue_scatter(const PersistArgs &a, Shared &s
     }
     __syncthreads();
     const int32_t Atot = s.pref[L0];
-    // this block scatters arrivals g = me, me + S, me + 2S, ... in chunks of SPK_CAP rows
-    const int32_t mine = Atot > me ? (Atot - me + a.S - 1) / a.S : 0;
+    // regular block r of R scatters arrivals g = r, r + R, r + 2R, ... in
+    // chunks of SPK_CAP rows (not the driven blocks: their prologue is the
+    // launch's critical path)
+    const int32_t rb = me - a.G, R = a.S - a.G;
+    const int32_t mine = Atot > rb ? (Atot - rb + R - 1) / R : 0;
     for (int32_t c0 = 0; c0 < mine; c0 += SPK_CAP) {
         const int32_t ns = min(SPK_CAP, mine - c0);
         for (int32_t r = tid; r < ns; r += PT) {
-            const int32_t g = me + (c0 + r) * a.S;
+            const int32_t g = rb + (c0 + r) * R;
             int32_t m = 0;
             while (s.pref[m + 1] <= g) m++;
             const int32_t k = g - s.pref[m];
@@ -971,7 +986,7 @@ __device__ __forceinline__ void sim_driven_x(const PersistArgs &a, Shared &s, Ce
 
 // A regular block's spikes of the epoch -> the LDS push list (s.spkI /
 // spkK / spkR); returns the row count (at most SPK_CAP, flag 4 beyond).
-__device__ __forceinline__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS]) {
+__device__ __forceinline__ int32_t collect_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
     if (tid == 0) s.nspk = 0;
@@ -994,7 +1009,7 @@ __device__ __forceinline__ int32_t collect_spikes(const PersistArgs &a, Shared &
     }
     __syncthreads();
     const int32_t ns = s.nspk;
-    if (tid == 0 && ns > SPK_CAP) atomicOr(&a.ctl[1], 4);
+    if (tid == 0 && ns > SPK_CAP) flag_overflow(a, par, 4);
     return min(ns, SPK_CAP);
 }
 
@@ -1002,23 +1017,31 @@ __device__ __forceinline__ int32_t collect_spikes(const PersistArgs &a, Shared &
 // A regular block pushes its own spikes (buckets of parity par); rows longer
 // than LONG_ROW go to the shared list of parity lpar.
 __device__ __forceinline__ void push_spikes(const PersistArgs &a, Shared &s, Cell (&cells)[CELLS], int32_t par, int32_t lpar) {
-    const int32_t ns = collect_spikes(a, s, cells);
+    const int32_t ns = collect_spikes(a, s, cells, par);
     push_rows(a, s, par, ns, lpar);
     publish_stats(a, s);
 }
 
-// The driven block's rows are many (every forced spike), so the first
-// SHARE_BLOCKS blocks push them, each for a fixed range of driven-list
-// positions whose rows it holds in LDS (share_setup).  Once the driven blocks
+// The driven block's rows are many (every forced spike), so the SHARE_BLOCKS
+// blocks after the driven ones push them (share_index), each for a fixed
+// range of driven-list positions whose rows it holds in LDS (share_setup).
+// Not the driven blocks themselves: their serial chain is the launch's
+// critical path, from the prologue on.  Once the driven blocks
 // have published the epoch (PersistArgs::epc), a share block reads the spike
 // steps of its neurons (dmask: forced, tmask: threshold, as the driven block
 // wrote them) and their res, and writes their rows' events.
+// This block's index among the share blocks, or -1.
+__device__ __forceinline__ int32_t share_index(const PersistArgs &a) {
+    const int32_t b = (int32_t)blockIdx.x - a.G;
+    return b >= 0 && b < min((int32_t)gridDim.x - a.G, SHARE_BLOCKS) ? b : -1;
+}
+
 __device__ __forceinline__ void share_setup(const PersistArgs &a, Shared &s) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
-    const int32_t G = min((int32_t)gridDim.x, SHARE_BLOCKS), b = blockIdx.x;
-    if (b >= G) return;
-    const int32_t q0 = (int32_t)((int64_t)d.nDriven * b / G), q1 = (int32_t)((int64_t)d.nDriven * (b + 1) / G);
+    const int32_t nsh = min((int32_t)gridDim.x - a.G, SHARE_BLOCKS), b = share_index(a);
+    if (b < 0) return;
+    const int32_t q0 = (int32_t)((int64_t)d.nDriven * b / nsh), q1 = (int32_t)((int64_t)d.nDriven * (b + 1) / nsh);
     if (tid == 0) { s.shQ0 = q0; s.shNq = q1 - q0; }
     for (int32_t k = tid; k < q1 - q0; k += PT) {
         const int32_t i = d.driven[q0 + k];
@@ -1043,7 +1066,7 @@ __device__ __forceinline__ void push_driven_share(const PersistArgs &a, Shared &
         s.share[3] = now;                              \
     }
     if (timed) s.share[3] = wall_clock64();
-    if ((int32_t)blockIdx.x >= min((int32_t)gridDim.x, SHARE_BLOCKS)) return;   // uniform in the block
+    if (share_index(a) < 0) return;   // uniform in the block
     __syncthreads();        // the LDS push list may still be read by this block's own push
     if (tid == 0) s.nspk = 0;
     __syncthreads();
@@ -1069,7 +1092,7 @@ __device__ __forceinline__ void push_driven_share(const PersistArgs &a, Shared &
     }
     __syncthreads();
     const int32_t nsAll = s.nspk;
-    if (tid == 0 && nsAll > SPK_CAP) atomicOr(&a.ctl[1], 4);
+    if (tid == 0 && nsAll > SPK_CAP) flag_overflow(a, par, 4);
     const int32_t ns = min(nsAll, SPK_CAP);
     // row lengths from the LDS table -> the flattened edge list
     int32_t len[RPT];
@@ -1121,12 +1144,24 @@ __device__ __forceinline__ void build_list(const PersistArgs &a, Shared &s, int3
     uint64_t *bits = a.spk + ((int64_t)par * D_MAX + m) * a.nwords;
     const int32_t P = (nD + PT - 1) / PT;
     const int32_t dlo = min(nD, tid * P), dhi = min(nD, dlo + P);
+    // Each loop issues a batch of loads before it uses any: a loop that used
+    // each load before issuing the next paid ~1 us per word at a launch's end.
+    constexpr int BQ = 4, BW = 2;
     int32_t nf = 0;
-    for (int32_t q = dlo; q < dhi; q++) {
-        nf += (fl[q] >> m) & 1u;
-        if ((tf[q] >> m) & 1u) {   // threshold spikes of driven neurons join the bitmask
-            const int32_t i = d.driven[q];
-            atomicOr((unsigned long long *)&bits[i >> 6], 1ull << (i & 63));
+    for (int32_t q0 = dlo; q0 < dhi; q0 += BQ) {
+        uint32_t f[BQ], th[BQ];
+        int32_t ii[BQ];
+#pragma unroll
+        for (int j = 0; j < BQ; j++) {
+            const int32_t q = min(q0 + j, dhi - 1);
+            f[j] = fl[q]; th[j] = tf[q]; ii[j] = d.driven[q];
+        }
+#pragma unroll
+        for (int j = 0; j < BQ; j++) {
+            if (q0 + j >= dhi) break;
+            nf += (f[j] >> m) & 1u;
+            if ((th[j] >> m) & 1u)   // threshold spikes of driven neurons join the bitmask
+                atomicOr((unsigned long long *)&bits[ii[j] >> 6], 1ull << (ii[j] & 63));
         }
     }
     // the bitmask's atomics are complete before any thread reads it (a fence
@@ -1136,21 +1171,54 @@ __device__ __forceinline__ void build_list(const PersistArgs &a, Shared &s, int3
     int32_t fpos = wg_exclusive_scan<PT>(nf, s.scan, &ndTot);
     const int32_t W = (a.nwords + PT - 1) / PT;
     const int32_t wlo = min(a.nwords, tid * W), whi = min(a.nwords, wlo + W);
+    // the words are kept in LDS for the second pass when they fit (the
+    // events' space is free between an epoch's push and the next gather)
+    uint64_t *keep = a.nwords <= EV_CAP ? s.evKey : nullptr;
     int32_t cnt = 0;
-    for (int32_t w = wlo; w < whi; w++) cnt += __popcll(__hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int32_t w0 = wlo; w0 < whi; w0 += BW) {
+        uint64_t v[BW];
+#pragma unroll
+        for (int j = 0; j < BW; j++)
+            v[j] = w0 + j < whi ? __hip_atomic_load(&bits[w0 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+#pragma unroll
+        for (int j = 0; j < BW; j++) {
+            cnt += __popcll(v[j]);
+            if (keep && w0 + j < whi) keep[w0 + j] = v[j];
+        }
+    }
     // the forced spikes first in the list, in driven-list order
-    for (int32_t q = dlo; q < dhi; q++)
-        if ((fl[q] >> m) & 1u) out[fpos++] = d.driven[q];
+    for (int32_t q0 = dlo; q0 < dhi; q0 += BQ) {
+        uint32_t f[BQ];
+        int32_t ii[BQ];
+#pragma unroll
+        for (int j = 0; j < BQ; j++) {
+            const int32_t q = min(q0 + j, dhi - 1);
+            f[j] = fl[q]; ii[j] = d.driven[q];
+        }
+#pragma unroll
+        for (int j = 0; j < BQ; j++)
+            if (q0 + j < dhi && ((f[j] >> m) & 1u)) out[fpos++] = ii[j];
+    }
     int32_t thTot;
     int32_t pos = ndTot + wg_exclusive_scan<PT>(cnt, s.scan, &thTot);
-    for (int32_t w = wlo; w < whi; w++) {
-        uint64_t b = __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!b) continue;
-        bits[w] = 0ull;
-        while (b) {
-            const int bit = __ffsll((unsigned long long)b) - 1;
-            out[pos++] = (w << 6) + bit;
-            b &= b - 1;
+    for (int32_t w0 = wlo; w0 < whi; w0 += BW) {
+        uint64_t v[BW];
+#pragma unroll
+        for (int j = 0; j < BW; j++)
+            v[j] = w0 + j >= whi ? 0ull
+                   : keep        ? keep[w0 + j]
+                                 : __hip_atomic_load(&bits[w0 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int j = 0; j < BW; j++) {
+            uint64_t b = v[j];
+            if (!b) continue;
+            const int32_t w = w0 + j;
+            bits[w] = 0ull;
+            while (b) {
+                const int bit = __ffsll((unsigned long long)b) - 1;
+                out[pos++] = (w << 6) + bit;
+                b &= b - 1;
+            }
         }
     }
     if (tid == 0) {
@@ -1302,7 +1370,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
     __syncthreads();
 
     const int32_t L0 = min(a.D, a.nsteps);
-    if (owning) prologue_scatter(a, s, L0);
+    if (owning && !drivenBlock) prologue_scatter(a, s, L0);
     FB_PHASE(5)
     if (!grid_barrier(a.bar, &a.ctl[1])) return;
     FB_PHASE(4)
@@ -1324,13 +1392,15 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
 
     int32_t done = 0, e = 0, prevL = 0;
     bool listsBuilt = false;   // the last epoch's lists were built before a stop
-    if (!a.ctl[1]) {
+    {
         while (done < a.nsteps) {
             const int32_t L = min(a.D, a.nsteps - done);
             const int32_t par = e & 1;
-            // overflow flags of the last push: read now, acted on after the
-            // gather (which only reads), so the load overlaps the gather's
-            const int32_t stop = e > 0 ? __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            // overflow flags of the pushes into this epoch's buckets (the
+            // previous epoch's, the prologue's at e = 0): read now, acted on
+            // after the gather (which only reads), so the load overlaps the
+            // gather's
+            const int32_t stop = __hip_atomic_load(&a.pflag[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // the long-row list of the other parity was last read before the
             // previous epoch's barrier and is next written in the next epoch:
             // clear it now
@@ -1385,7 +1455,7 @@ __global__ void __launch_bounds__(PT) k_persist(PersistArgs a) {
                 // share workgroup also for the driven workgroups' masks
                 if (tid == 0) {
                     const uint64_t t0 = wall_clock64();
-                    const bool share = me < min((int32_t)gridDim.x, SHARE_BLOCKS);
+                    const bool share = share_index(a) >= 0;
                     // groups that hold regular workgroups (those are G .. S-1)
                     const int32_t ng = a.S > a.G ? (a.S - 1) / GBAR_GROUP - a.G / GBAR_GROUP + 1 : 0;
                     bool late = false;
@@ -1631,8 +1701,11 @@ static fb_status supported_locked(const DevBrain &d, bool serialDeliver, std::st
     if (st) { *why = "the device's occupancy query failed"; return st; }
     Layout l;
     if (!layout_for(d, D, l)) { *why = "more than 1024 neurons per workgroup at this grid size"; return FB_ERR_UNSUPPORTED; }
-    const int32_t shareBlocks = std::min(l.grid, SHARE_BLOCKS);
-    if ((d.nDriven + shareBlocks - 1) / shareBlocks > SHQ) { *why = "too many driven neurons for this grid"; return FB_ERR_UNSUPPORTED; }
+    const int32_t shareBlocks = std::min(l.grid - l.G, SHARE_BLOCKS);
+    if (shareBlocks < 1 || (d.nDriven + shareBlocks - 1) / shareBlocks > SHQ) {
+        *why = "too many driven neurons for this grid";
+        return FB_ERR_UNSUPPORTED;
+    }
     return FB_OK;
 }
 
@@ -1956,6 +2029,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
     a.bar = pb.bar;
     a.xc = pb.xc;
     a.lrCnt = pb.lrCnt;
+    a.pflag = pb.lrCnt + 2;
     a.epc = pb.lrCnt + 64;
     a.lrI = pb.lrI;
     a.lrK = pb.lrK;
@@ -2029,7 +2103,7 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         }
         if (phaseUs) {
             for (int k = 0; k < PROF_SPAN; k++)
-                phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
+                phaseUs[k] += (double)prof[k] / (k >= PROF_SHARE && k < PROF_SHARE + 3 ? std::min(lay.grid - lay.G, SHARE_BLOCKS) : 1) / g_ps.tickHz * 1e6;
             // the span: spread of the block starts, first start to last block
             // end, last block end to the final write (a slot stays zero when a
             // block left early)
