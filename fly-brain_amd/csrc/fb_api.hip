@@ -940,10 +940,14 @@ static fb_status run_steps(fb_brain *b, int32_t nsteps, int32_t engine) {
     const bool autoChosen = engine == FB_ENGINE_AUTO;
     if (engine == FB_ENGINE_AUTO) {
         // the lookahead engine pays a prologue per call (one launch that
-        // loads the state and scatters the delay line); from one epoch of
-        // steps on it is ahead of a launch or a graph replay per step
+        // loads the state and scatters the delay line).  Measured on MI355X
+        // (events around a call, in a loop of such calls): 57 us + 2.7 us a
+        // step at 165,122 neurons against 19 us a step per-step, 44 us + 1.5
+        // us a step at 16,384 against 12 us; so from 4 steps on for brains of
+        // that size, from one epoch of steps on for smaller ones (whose
+        // per-step kernels are shorter)
         std::string why;
-        const bool longRun = nsteps >= b->d.nslots - 1;
+        const bool longRun = nsteps >= b->d.nslots - 1 || (nsteps >= 4 && b->d.N >= 16384);
         // otherwise per-step kernels: replayed as a graph for small brains,
         // whose kernels are shorter than their launch; launched directly for
         // large ones, whose kernels hide the launch and where a capture after
