@@ -380,11 +380,6 @@ __device__ __forceinline__ int32_t row_scan(const PersistArgs &a, Shared &s, int
     return total;
 }
 
-// Writes the edge events g0 <= g < g1 of the flattened edge list (row_scan)
-// into the destinations' buckets for parity `par`, four edges per thread in
-// flight; a scan of the row starts maps an edge to its row.  A slot is one
-// atomic on the destination's count; beyond EV_CAP the event is dropped and
-// flag 2 stops the launch before the epoch that would read it.
 // An overflow of a push into the buckets of parity par.  The host reads the
 // flags in ctl[1]; the blocks read pflag[par] at the top of the epoch that
 // gathers those buckets, a word no block writes during that epoch (its pushes
@@ -394,6 +389,11 @@ __device__ __forceinline__ void flag_overflow(const PersistArgs &a, int32_t par,
     atomicOr(&a.pflag[par], bit);
 }
 
+// Writes the edge events g0 <= g < g1 of the flattened edge list (row_scan)
+// into the destinations' buckets for parity `par`, four edges per thread in
+// flight; a scan of the row starts maps an edge to its row.  A slot is one
+// atomic on the destination's count; beyond EV_CAP the event is dropped and
+// flag 2 stops the launch before the epoch that would read it.
 __device__ __forceinline__ void push_edges(const PersistArgs &a, Shared &s, int32_t par, int32_t ns, int32_t g0, int32_t g1) {
     const DevBrain &d = a.d;
     const int tid = threadIdx.x;
