@@ -249,6 +249,37 @@ def test_full_scale_config2_matches_oracle(c2, coba, steps, engine):
     assert c["edge_events"] > 0
 
 
+def test_full_scale_calls_back_to_back_match_oracle(c2):
+    """The headline's call pattern at full scale on the automatic engine: a
+    run to steady state, short and ragged calls back to back (5 and 20 steps
+    go to the lookahead kernel too, each starting from the stage the last one
+    left), a setter on driven neurons between two of them, and nothing read
+    back until the end; then the state, counts and RNG equal the oracle's."""
+    g, drv, rates = c2
+    p = P.config_params()
+    gpu = _hip(g, g.weights, g.sign, p, 1, "auto")
+    cpu = OracleBrain(g, p, seed=1, W=g.weights, sign=g.sign)
+    for b in (gpu, cpu):
+        for i, r in zip(drv.tolist(), rates.tolist()):
+            b.setDriveOne(int(i), float(r))
+    l0 = 0
+    for n in (300, 5, 20, 20, 1, 37):
+        print(f"full-scale back-to-back call: {n} steps", flush=True)
+        gpu.advance(n)
+        cpu.run(n, raster_cap=n * 4096)
+        if n == 20 and l0 == 0:
+            l0 = gpu.stat("persistent_launches")
+            for b in (gpu, cpu):
+                b.pulse(drv[:16].tolist(), 2.5)
+    for k in lifcases.STATE:
+        assert np.array_equal(getattr(gpu, k).view(np.uint32), cpu.array(k).view(np.uint32)), k
+    assert np.array_equal(gpu.spikeCount, cpu.spikeCount)
+    assert gpu.rng == cpu.rng
+    # every call but the 1-step one ran on the lookahead kernel
+    assert gpu.stat("persistent_launches") >= 5
+    assert gpu.stat("fallback_steps") == 0
+
+
 @pytest.mark.parametrize("zeros", ["+0", "-0"])
 def test_step_engines_on_bursts_and_large_driven_sets(zeros):
     """Per-step engines where a step's arrivals exceed the flat delivery's row
