@@ -1751,19 +1751,23 @@ struct PersistBuffers {
     int32_t *bstart = nullptr;       // [S_MAX + 1]
     int32_t *chunkBlk = nullptr;     // [N/64] regular block of each 64-neuron chunk
     uint64_t eownDriven = ~0ull;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timerPending = false;       // the last launch's ev0 .. ev1 time is not taken yet
+    // start and end events of the launches, two pairs used in turn: a
+    // launch's time is taken after the next launch is queued, so that no
+    // host work on the last launch's events comes before the next dispatch
+    hipEvent_t ev[2][2] = {};
+    int evSlot = 0;                  // the pair the next launch records
+    bool timerPending[2] = {};       // the pair's time is not taken yet
 };
 
-// Adds the last launch's kernel time (ev0 .. ev1) to *kernelMs once its end
-// event is done (waiting for it).
-static hipError_t take_kernel_time(PersistBuffers &pb, double *kernelMs) {
-    if (!pb.timerPending) return hipSuccess;
-    pb.timerPending = false;
-    hipError_t e = hipEventSynchronize(pb.ev1);
+// Adds the kernel time of event pair k (start .. end) to *kernelMs once its
+// end event is done (waiting for it).
+static hipError_t take_kernel_time(PersistBuffers &pb, int k, double *kernelMs) {
+    if (!pb.timerPending[k]) return hipSuccess;
+    pb.timerPending[k] = false;
+    hipError_t e = hipEventSynchronize(pb.ev[k][1]);
     if (e != hipSuccess) return e;
     float ms = 0.f;
-    e = hipEventElapsedTime(&ms, pb.ev0, pb.ev1);
+    e = hipEventElapsedTime(&ms, pb.ev[k][0], pb.ev[k][1]);
     if (e == hipSuccess && kernelMs) *kernelMs += ms;
     return e;
 }
@@ -1810,7 +1814,8 @@ constexpr size_t POOL_MAX = 2;
 void persistent_take_time(const int32_t *key, double *kernelMs) {
     std::lock_guard<std::mutex> lock(grid_launch_mutex());
     auto it = g_pbuf.find(key);
-    if (it != g_pbuf.end()) (void)take_kernel_time(it->second, kernelMs);
+    if (it != g_pbuf.end())
+        for (int k = 0; k < 2; k++) (void)take_kernel_time(it->second, k, kernelMs);
 }
 
 void persistent_release(const int32_t *key) {
@@ -1819,7 +1824,7 @@ void persistent_release(const int32_t *key) {
     if (it == g_pbuf.end()) return;
     PersistBuffers pb = std::move(it->second);
     g_pbuf.erase(it);
-    pb.timerPending = false;
+    pb.timerPending[0] = pb.timerPending[1] = false;
     pb.stageState = pb.stageParams = pb.arraysBehind = false;   // the brain's state goes with it
     if (pb.spk && g_pool.size() < POOL_MAX) {
         pb.eownDriven = ~0ull;   // the next brain's driven set is its own
@@ -1828,7 +1833,9 @@ void persistent_release(const int32_t *key) {
         return;
     }
     free_buffers(pb);
-    if (pb.ev0) { (void)hipEventDestroy(pb.ev0); (void)hipEventDestroy(pb.ev1); }
+    for (auto &pair : pb.ev)
+        for (hipEvent_t &e : pair)
+            if (e) (void)hipEventDestroy(e);
 }
 
 // The stage's driven neurons -> the brain's arrays (state and spike counts);
@@ -1970,10 +1977,9 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         FB_HIP(hipMalloc(&pb.lrK, sizeof(uint32_t) * 2 * LR_CAP));
         FB_HIP(hipMalloc(&pb.lrR, sizeof(float) * 2 * LR_CAP));
         FB_HIP(hipMalloc(&pb.chunkBlk, sizeof(int32_t) * (size_t)((d.N + 63) / 64)));
-        if (!pb.ev0) {
-            FB_HIP(hipEventCreate(&pb.ev0));
-            FB_HIP(hipEventCreate(&pb.ev1));
-        }
+        for (auto &pair : pb.ev)
+            for (hipEvent_t &e : pair)
+                if (!e) FB_HIP(hipEventCreate(&e));
     }
     // driven positions and edge owners (again when the driven set or layout changed)
     if (pb.starts != lay.starts) {
@@ -2053,9 +2059,8 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
             pb.needClear = false;
         }
         void *args[] = {&a};
-        // the previous launch's kernel time, now that its end event is done
-        FB_HIP(take_kernel_time(pb, kernelMs));
-        FB_HIP(hipEventRecord(pb.ev0, st));
+        const int ek = pb.evSlot;
+        FB_HIP(hipEventRecord(pb.ev[ek][0], st));
         // A plain launch: the grid is one workgroup per CU at the kernel's
         // occupancy and holds grid_launch_mutex (see there), which is what a
         // cooperative launch would guarantee; the cooperative launch path
@@ -2071,7 +2076,11 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
         // the driven neurons' state is in the stage from now on
         pb.stageState = pb.stageParams = pb.arraysBehind = true;
         pb.stageN = d.nDriven;
-        FB_HIP(hipEventRecord(pb.ev1, st));
+        FB_HIP(hipEventRecord(pb.ev[ek][1], st));
+        pb.timerPending[ek] = true;
+        pb.evSlot = ek ^ 1;
+        // the previous launch's kernel time, now that this one is queued
+        FB_HIP(take_kernel_time(pb, ek ^ 1, kernelMs));
         if (launches) (*launches)++;
         // Wait by polling: the last block out writes the tag to pinned memory
         // as its final act; a blocking stream synchronisation takes ~16 us
@@ -2087,7 +2096,6 @@ fb_status launch_persistent(const DevBrain &d, int32_t head, int32_t nsteps, hip
                 if (q != hipErrorNotReady) FB_HIP(q);
             }
         }
-        pb.timerPending = true;
         int32_t ctl[4];
         uint64_t prof[PERSIST_PROF];
         if (__atomic_load_n(&pb.hostCtl[2], __ATOMIC_ACQUIRE) == a.tag) {
