@@ -108,7 +108,7 @@ struct LaShared {
         } g;
         struct {
             int32_t pre[LSPK];
-            uint32_t info[LSPK];           // step | forced << 8
+            uint32_t info[LSPK];           // step | forced << 8 | inhibitory << 9
             uint32_t rank[LSPK];           // arrival rank
             uint64_t m[LSPK];
             int32_t rowOff[LSPK + 1];
@@ -128,6 +128,7 @@ struct LaShared {
     int32_t wtot[BWV][MPG];
     uint32_t x0[MPG];
     int32_t steps[64];
+    uint64_t onm[BDM];                     // la_push: members running at each arrival step
     int32_t nEv, nSpk, nAdd;
 };
 
@@ -270,11 +271,12 @@ __device__ __forceinline__ float apply_cell(const LaShared<RNB> &s, int32_t c, f
 template <int RNB>
 __device__ __forceinline__ void push_entry(const LaArgs &a, LaShared<RNB> &s, int32_t pre, int32_t step, bool forced,
                                            uint32_t rank, uint64_t m) {
-    if (a.d.g.sign[pre] == 0.f) return;            // delivers nothing (lif.c:37)
+    const float sg = a.d.g.sign[pre];
+    if (sg == 0.f) return;                         // delivers nothing (lif.c:37)
     const int32_t k = atomicAdd(&s.nSpk, 1);
     if (k < LSPK) {
         s.u.p.pre[k] = pre;
-        s.u.p.info[k] = (uint32_t)step | (forced ? 256u : 0u);
+        s.u.p.info[k] = (uint32_t)step | (forced ? 256u : 0u) | (sg > 0.f ? 0u : 512u);
         s.u.p.rank[k] = rank;
         s.u.p.m[k] = m;
     }
@@ -309,67 +311,102 @@ __device__ void la_push(const LaArgs &a, LaShared<RNB> &s, int32_t par, int32_t 
         s.dcnt[q] = 0;
         s.dadd[q] = 0;
     }
-    // row offsets of the flattened edge list
+    // members running at each arrival step (one ballot per step)
+    if (tid < 64)
+        for (int32_t st = 0; st < a.D; st++) {
+            const uint64_t on = on_mask(s, d.B, tArr0 + st);
+            if (tid == 0) s.onm[st] = on;
+        }
+    __syncthreads();
+    // Every loop below issues a batch of loads before it uses any: a loop
+    // that used each global load before issuing the next paid a memory round
+    // trip per iteration.
+    constexpr int UB = 4;
+    // row lengths (kept in rowOff until the scan) and the event counts
     const int32_t pe = (ns + BT - 1) / BT;
     const int32_t k0 = min(ns, tid * pe), k1 = min(ns, k0 + pe);
     int32_t len = 0;
     unsigned long long ev = 0, syn = 0;
-    for (int32_t k = k0; k < k1; k++) {
-        const int32_t pre = s.u.p.pre[k];
-        const uint32_t ra = d.g.indptr[pre], re = d.g.indptr[pre + 1];
-        s.u.p.rowA[k] = ra;
-        len += (int32_t)(re - ra);
-        const int32_t step = (int32_t)(s.u.p.info[k] & 255u);
-        const int32_t ta = tArr0 + step;
-        uint64_t on = 0;
-        for (int b = 0; b < d.B; b++) on |= (ta < s.steps[b] ? 1ull : 0ull) << b;
-        const int32_t c = __popcll(s.u.p.m[k] & on);
-        ev += (unsigned long long)(re - ra) * (unsigned long long)c;
-        syn += (unsigned long long)d.g.rowSyn[pre] * (unsigned long long)c;
+    for (int32_t kb = k0; kb < k1; kb += UB) {
+        uint32_t ra[UB], re[UB], sy[UB];
+#pragma unroll
+        for (int v = 0; v < UB; v++) {
+            const int32_t pre = s.u.p.pre[min(kb + v, k1 - 1)];
+            ra[v] = d.g.indptr[pre];
+            re[v] = d.g.indptr[pre + 1];
+            sy[v] = d.g.rowSyn[pre];
+        }
+#pragma unroll
+        for (int v = 0; v < UB; v++) {
+            const int32_t k = kb + v;
+            if (k >= k1) break;
+            s.u.p.rowA[k] = ra[v];
+            s.u.p.rowOff[k] = (int32_t)(re[v] - ra[v]);
+            len += (int32_t)(re[v] - ra[v]);
+            const int32_t c = __popcll(s.u.p.m[k] & s.onm[s.u.p.info[k] & 255u]);
+            ev += (unsigned long long)(re[v] - ra[v]) * (unsigned long long)c;
+            syn += (unsigned long long)sy[v] * (unsigned long long)c;
+        }
     }
     int32_t totalEdges;
     int32_t run = wg_exclusive_scan<BT>(len, s.scan, &totalEdges);
     for (int32_t k = k0; k < k1; k++) {
+        const int32_t l = s.u.p.rowOff[k];
         s.u.p.rowOff[k] = run;
-        run += (int32_t)(d.g.indptr[s.u.p.pre[k] + 1] - s.u.p.rowA[k]);
+        run += l;
     }
     if (tid == 0) s.u.p.rowOff[ns] = totalEdges;
     if (ev) atomicAdd(&d.counters[0], ev);
     if (syn) atomicAdd(&d.counters[1], syn);
     __syncthreads();
     for (int pass = 0; pass < 2; pass++) {
-        for (int32_t e = tid; e < totalEdges; e += BT) {
-            int32_t lo = 0, hi = ns - 1;           // last k with rowOff[k] <= e
-            while (lo < hi) {
-                const int32_t mid = (lo + hi + 1) >> 1;
-                if (s.u.p.rowOff[mid] <= e) lo = mid; else hi = mid - 1;
-            }
-            const int32_t k = lo;
-            const uint32_t jr = (uint32_t)(e - s.u.p.rowOff[k]);
-            const uint32_t j = s.u.p.rowA[k] + jr;
-            const uint32_t own = a.eown[j];
-            uint64_t mleft = s.u.p.m[k], mdst = 0;
-            int32_t dst;
-            while ((dst = next_dst(a, own, mleft, mdst)) >= 0) {
-                if (pass == 0) {
-                    atomicAdd(&s.dcnt[dst], 1);
-                    atomicAdd(&s.dadd[dst], __popcll(mdst));
-                    continue;
+        // thread tid takes edges tid, tid + BT, ... as before, UB at a time
+        for (int32_t eb = tid; eb < totalEdges; eb += BT * UB) {
+            int32_t kk[UB];
+            uint32_t own[UB], cn[UB];
+#pragma unroll
+            for (int v = 0; v < UB; v++) {
+                const int32_t e = min(eb + v * BT, totalEdges - 1);
+                int32_t lo = 0, hi = ns - 1;           // last k with rowOff[k] <= e
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (s.u.p.rowOff[mid] <= e) lo = mid; else hi = mid - 1;
                 }
-                const int32_t pos = s.dbase[dst] + atomicAdd(&s.dcnt[dst], 1);
-                if (pos >= LEV) continue;              // flagged at the reservation
-                const int32_t pre = s.u.p.pre[k];
-                const int32_t step = (int32_t)(s.u.p.info[k] & 255u);
-                const uint32_t cls = d.g.sign[pre] > 0.f ? 0u : 1u;
-                const bool drvDst = (own & OWN_DRV) != 0;
-                const uint32_t tloc = drvDst ? (own & 0xFFFFu) : (own & 0xFFu);
-                const uint32_t CW = drvDst ? (uint32_t)BDRV : (uint32_t)RNB;
-                const uint64_t cell = ((uint64_t)step * 2u + cls) * CW + tloc;
-                const uint64_t rank = s.u.p.rank[k];
-                const int64_t at = ((int64_t)par * a.S + dst) * LEV + pos;
-                a.evKey[at] = (cell << KEY_CELL_SHIFT) | (rank << 23) | (uint64_t)jr;
-                a.evMask[at] = mdst;
-                a.evAux[at] = ((uint64_t)(uint32_t)pre << 32) | (uint64_t)(uint32_t)d.g.counts[j];
+                kk[v] = lo;
+                const uint32_t j = s.u.p.rowA[lo] + (uint32_t)(e - s.u.p.rowOff[lo]);
+                own[v] = a.eown[j];
+                cn[v] = pass ? (uint32_t)d.g.counts[j] : 0u;
+            }
+#pragma unroll
+            for (int v = 0; v < UB; v++) {
+                const int32_t e = eb + v * BT;
+                if (e >= totalEdges) break;
+                const int32_t k = kk[v];
+                const uint32_t jr = (uint32_t)(e - s.u.p.rowOff[k]);
+                uint64_t mleft = s.u.p.m[k], mdst = 0;
+                int32_t dst;
+                while ((dst = next_dst(a, own[v], mleft, mdst)) >= 0) {
+                    if (pass == 0) {
+                        atomicAdd(&s.dcnt[dst], 1);
+                        atomicAdd(&s.dadd[dst], __popcll(mdst));
+                        continue;
+                    }
+                    const int32_t pos = s.dbase[dst] + atomicAdd(&s.dcnt[dst], 1);
+                    if (pos >= LEV) continue;              // flagged at the reservation
+                    const int32_t pre = s.u.p.pre[k];
+                    const uint32_t info = s.u.p.info[k];
+                    const int32_t step = (int32_t)(info & 255u);
+                    const uint32_t cls = (info >> 9) & 1u;
+                    const bool drvDst = (own[v] & OWN_DRV) != 0;
+                    const uint32_t tloc = drvDst ? (own[v] & 0xFFFFu) : (own[v] & 0xFFu);
+                    const uint32_t CW = drvDst ? (uint32_t)BDRV : (uint32_t)RNB;
+                    const uint64_t cell = ((uint64_t)step * 2u + cls) * CW + tloc;
+                    const uint64_t rank = s.u.p.rank[k];
+                    const int64_t at = ((int64_t)par * a.S + dst) * LEV + pos;
+                    a.evKey[at] = (cell << KEY_CELL_SHIFT) | (rank << 23) | (uint64_t)jr;
+                    a.evMask[at] = mdst;
+                    a.evAux[at] = ((uint64_t)(uint32_t)pre << 32) | (uint64_t)cn[v];
+                }
             }
         }
         __syncthreads();
