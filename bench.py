@@ -186,7 +186,7 @@ def bench_batch64(args, synth, P, lif, batch, hiprt, stream):
     tuner = ensemble()
     tuner.run(400)
     tuning = {"shape_neurons_per_block": tuner.stat("lookahead_shape"),
-              "us_per_epoch": {str(k): tuner.stat(f"autotune_us_per_epoch_{k}") for k in (80, 96, 128)},
+              "us_per_epoch": {str(k): tuner.stat(f"autotune_us_per_epoch_{k}") for k in (72, 80, 96, 128)},
               "steps": 400}
     tuner.destroy()
     eb = ensemble()

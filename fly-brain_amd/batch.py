@@ -78,7 +78,7 @@ class LIFBatch:
         _l.check(self.lib.fb_batch_set_lanes(self.h, 1 if sign_gain else 0, _l.ptr(ld)))
 
     def set_shape(self, neurons_per_block: int):
-        """The lookahead engine's regular-block shape (80, 96 or 128 neurons per
+        """The lookahead engine's regular-block shape (72, 80, 96 or 128 neurons per
         workgroup), or 0 for the autotuner's choice (fb_batch_set_shape)."""
         _l.check(self.lib.fb_batch_set_shape(self.h, int(neurons_per_block)))
 

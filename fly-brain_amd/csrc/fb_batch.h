@@ -59,7 +59,7 @@ __device__ inline float ordered_sum_chain(float g, int32_t count, uint64_t key0,
 
 constexpr int BMAX = 64;
 constexpr int DRV_WG = 4;          // workgroups of the multi-workgroup draws
-constexpr int LA_NSHAPES = 3;      // regular-block shapes of the ensemble lookahead engine
+constexpr int LA_NSHAPES = 4;      // regular-block shapes of the ensemble lookahead engine
 
 struct BatchDev {
     int32_t N, B, nslots, nDriven, nTypes;

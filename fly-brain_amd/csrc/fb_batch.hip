@@ -487,8 +487,8 @@ struct fb_batch {
     // the lookahead engine's regular-block shape: chosen by the autotuner
     // (laShape, -1 until measured) or forced by the caller (laForced)
     int32_t laShape = -1, laForced = -1;
-    double tuneMs[LA_NSHAPES] = {0, 0, 0};
-    int32_t tuneEpochs[LA_NSHAPES] = {0, 0, 0};
+    double tuneMs[LA_NSHAPES] = {};
+    int32_t tuneEpochs[LA_NSHAPES] = {};
     struct Trial {
         int32_t shape, epochs;
         bool timed;
@@ -1007,7 +1007,7 @@ extern "C" fb_status fb_batch_run(fb_batch *b, int32_t nsteps, int32_t *nactive,
     return FB_OK;
 }
 
-// The lookahead engine's regular-block shape: 80, 96 or 128 neurons per
+// The lookahead engine's regular-block shape: 72, 80, 96 or 128 neurons per
 // block, or 0 for the autotuner's choice (measured again on the next run long
 // enough).  All shapes give the same bits.
 extern "C" fb_status fb_batch_set_shape(fb_batch *b, int32_t neuronsPerBlock) {
@@ -1027,7 +1027,7 @@ extern "C" fb_status fb_batch_set_shape(fb_batch *b, int32_t neuronsPerBlock) {
             b->laForced = k;
             return FB_OK;
         }
-    set_error("fb_batch_set_shape: %d is not a shape (80, 96, 128 or 0)", neuronsPerBlock);
+    set_error("fb_batch_set_shape: %d is not a shape (72, 80, 96, 128 or 0)", neuronsPerBlock);
     return FB_ERR_ARG;
 }
 

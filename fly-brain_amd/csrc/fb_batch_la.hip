@@ -795,16 +795,17 @@ __global__ void k_batch_eown(const uint2 *edges, int64_t E, const int32_t *drvPo
 // ------------------------------------------------------------------ host side
 
 // The shapes the autotuner chooses among: neurons per wave of a regular block
-// (80, 96 or 128 neurons per block).  Fewer, larger regular blocks mean fewer
+// (72, 80, 96 or 128 neurons per block).  Fewer, larger regular blocks mean fewer
 // destinations to push to and gather from per epoch, and a longer serial walk
 // per wave per step; which wins depends on the graph, the activity and the
 // device, so the engine measures it.
-constexpr int LA_SHAPES[LA_NSHAPES] = {10, 12, 16};
+constexpr int LA_SHAPES[LA_NSHAPES] = {9, 10, 12, 16};
 
 static const void *la_kernel(int k) {
     switch (k) {
-        case 0: return (const void *)k_batch_la<10>;
-        case 1: return (const void *)k_batch_la<12>;
+        case 0: return (const void *)k_batch_la<9>;
+        case 1: return (const void *)k_batch_la<10>;
+        case 2: return (const void *)k_batch_la<12>;
         default: return (const void *)k_batch_la<16>;
     }
 }
@@ -829,7 +830,7 @@ struct LaBuffers {
 static std::map<const void *, LaBuffers> g_labuf;    // keyed by the ensemble's device state
 
 static int32_t la_max_blocks(int k) {
-    static int32_t maxB[LA_NSHAPES] = {-1, -1, -1};
+    static int32_t maxB[LA_NSHAPES] = {-1, -1, -1, -1};
     if (maxB[k] < 0) {
         int dev = 0, ncu = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 0;

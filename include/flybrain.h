@@ -235,13 +235,13 @@ fb_status fb_batch_set_stream(fb_batch *b, void *stream);
  * reference. */
 fb_status fb_batch_set_engine(fb_batch *b, int32_t engine);
 /* The lookahead engine's regular-block shape, the tile the ensemble is cut
- * into: 80, 96 or 128 neurons (x all members) per workgroup, or 0 (default)
+ * into: 72, 80, 96 or 128 neurons (x all members) per workgroup, or 0 (default)
  * for the autotuner, which runs the first epochs of the first run long
  * enough on every shape that fits the device, twice round, and keeps the
  * fastest; later ensembles of the same graph, size and driven set take that
  * choice without measuring (0 measures again).  Every shape gives the same
  * bits.  "lookahead_shape" and
- * "autotune_us_per_epoch_<80|96|128>" (fb_batch_stat) report the choice and
+ * "autotune_us_per_epoch_<72|80|96|128>" (fb_batch_stat) report the choice and
  * its measurements.  No counterpart in the reference. */
 fb_status fb_batch_set_shape(fb_batch *b, int32_t neuronsPerBlock);
 /* "lookahead_launches", "lookahead_kernel_ms", "lookahead_steps",

@@ -190,8 +190,8 @@ def test_lane_drive_cannot_be_cleared_under_the_pool_loop():
 
 
 def test_every_lookahead_shape_and_the_autotuner_give_the_same_bits():
-    """The ensemble lookahead engine cut into regular blocks of 80, 96 and 128
-    neurons (fb_batch_set_shape), and the autotuner's own choice: every member
+    """The ensemble lookahead engine cut into regular blocks of 72, 80, 96 and
+    128 neurons (fb_batch_set_shape), and the autotuner's own choice: every member
     equals its single-brain oracle run for each, and the autotuner reports
     the shape it chose and what it measured for every candidate."""
     g = synth.tiny_graph(4096, 40, seed=47, p_count=0.3)
@@ -203,7 +203,7 @@ def test_every_lookahead_shape_and_the_autotuner_give_the_same_bits():
     silenced = [np.sort(rng.choice(g.N, size=int(rng.integers(0, 150)), replace=False)) for _ in range(B)]
     steps = rng.integers(400, 700, size=B)
     p = P.config_params()
-    for shape in (80, 96, 128, 0):
+    for shape in (72, 80, 96, 128, 0):
         print(f"ensemble lookahead shape {shape or 'autotuned'}", flush=True)
         hg = lif.HipGraph(g.N, g.indptr, g.indices, g.weights, sign, g.counts)
         eb = batch.LIFBatch(hg, p, [b + 1 for b in range(B)], type_of, gains)
@@ -220,7 +220,7 @@ def test_every_lookahead_shape_and_the_autotuner_give_the_same_bits():
         else:
             # a shape whose trial epochs overflowed its buckets has no measurement
             chosen = int(eb.stat("lookahead_shape"))
-            assert chosen in (80, 96, 128)
+            assert chosen in (72, 80, 96, 128)
             assert eb.stat(f"autotune_us_per_epoch_{chosen}") > 0
         for b in range(0, B, 9):
             W = member_weights(g, type_of, gains[b])
