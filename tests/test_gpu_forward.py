@@ -280,6 +280,36 @@ def test_full_scale_calls_back_to_back_match_oracle(c2):
     assert gpu.stat("fallback_steps") == 0
 
 
+def test_automatic_engine_takes_the_lookahead_kernel_from_four_steps():
+    """The automatic engine's rule: on a brain of 16,384 neurons or more, calls
+    of 4 steps or more run on the lookahead kernel and shorter ones on the
+    per-step kernels; on a smaller brain, calls shorter than an epoch stay
+    per-step.  The state after the mixed calls equals the oracle's."""
+    g = synth.config1_graph()
+    p = P.config_params()
+    drv = synth.driven_set(g.N, 0.02, 3)
+    gpu = _hip(g, g.weights, g.sign, p, 2, "auto")
+    cpu = OracleBrain(g, p, seed=2, W=g.weights, sign=g.sign)
+    for b in (gpu, cpu):
+        for i in drv.tolist():
+            b.setDriveOne(int(i), 150.0)
+    for n, lookahead in ((3, False), (4, True), (3, False), (5, True), (1, False), (4, True)):
+        l0 = gpu.stat("persistent_launches")
+        gpu.advance(n)
+        cpu.run(n, raster_cap=n * g.N)
+        assert (gpu.stat("persistent_launches") > l0) == lookahead, n
+    for k in lifcases.STATE:
+        assert np.array_equal(getattr(gpu, k).view(np.uint32), cpu.array(k).view(np.uint32)), k
+    assert np.array_equal(gpu.spikeCount, cpu.spikeCount)
+    assert gpu.rng == cpu.rng
+    small = synth.tiny_graph(4096, 20, seed=3, p_count=0.3)
+    s = (np.where(small.exc, 1.0, -1.0) * 0.5).astype(np.float32)
+    b = _hip(small, small.weights, s, p, 2, "auto")
+    b.setDriveOne(5, 150.0)
+    b.advance(4)
+    assert b.stat("persistent_launches") == 0
+
+
 @pytest.mark.parametrize("zeros", ["+0", "-0"])
 def test_step_engines_on_bursts_and_large_driven_sets(zeros):
     """Per-step engines where a step's arrivals exceed the flat delivery's row
